@@ -1,0 +1,351 @@
+/*
+ * resvg_amd.h — C-ABI of the MI355X (gfx950) pixel pipeline that sits below
+ * resvg::render / resvg_render.
+ *
+ * Every entry point returns an rvk_status and never aborts.  Pixel buffers are
+ * premultiplied RGBA8, row-major, stride = 4 * width, no padding — the layout of
+ * tiny_skia::Pixmap and of the `char *pixmap` argument of resvg_render
+ * (crates/c-api/resvg.h:476-489).  Buffers named `dev_*` are device pointers
+ * (hipMalloc'd, or a torch CUDA tensor's data_ptr); buffers named `host_*` are
+ * host pointers.  Work is enqueued on the context's stream; a call returns once
+ * the work is enqueued unless it says otherwise.
+ *
+ * Reference interfaces replaced (file:line relative to the resvg 0.48.1 tree):
+ *   resvg::render / resvg_render      crates/resvg/src/lib.rs:34-43, crates/c-api/lib.rs:874-897
+ *   resvg::render_node                crates/resvg/src/lib.rs:55-70
+ *   filter::apply                     crates/resvg/src/filter/mod.rs:335-355
+ *   per-primitive filter functions    crates/resvg/src/filter/<primitive>.rs (cited per entry)
+ *   tiny-skia 0.12 pixel calls        fill_path / stroke_path / draw_pixmap / fill
+ *                                     (call sites: crates/resvg/src/path.rs:73,113,
+ *                                      render.rs:242-249, filter/mod.rs:624-1111)
+ */
+#ifndef RESVG_AMD_H
+#define RESVG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------- */
+typedef int32_t rvk_status;
+#define RVK_OK                   0
+#define RVK_ERR_INVALID_ARGUMENT 1  /* null pointer, zero/oversized dimension, bad enum */
+#define RVK_ERR_OUT_OF_MEMORY    2  /* device allocation failed */
+#define RVK_ERR_HIP              3  /* a HIP runtime call failed */
+#define RVK_ERR_INVALID_REGION   4  /* filter::Error::InvalidRegion (filter/mod.rs:86-90) */
+#define RVK_ERR_UNSUPPORTED      5  /* feature outside this library's scope (feImage, feTile, patterns, images) */
+#define RVK_ERR_NO_DEVICE        6  /* no HIP device / library built without a device */
+
+/* Returns a static string naming a status code. */
+const char *rvk_status_string(rvk_status status);
+
+/* ------------------------------------------------------------------------- */
+/* Geometry                                                                  */
+/* ------------------------------------------------------------------------- */
+/* Same memory layout and meaning as resvg_transform (crates/c-api/resvg.h:101-108):
+ * tiny_skia::Transform::from_row(a, b, c, d, e, f) = (sx, ky, kx, sy, tx, ty). */
+typedef struct rvk_transform {
+    float a, b, c, d, e, f;
+} rvk_transform;
+
+/* usvg::NonZeroRect / tiny_skia::Rect as x, y, width, height (f32). */
+typedef struct rvk_rect {
+    float x, y, w, h;
+} rvk_rect;
+
+/* ------------------------------------------------------------------------- */
+/* Enumerations (values are this ABI's; the mapping follows the reference)   */
+/* ------------------------------------------------------------------------- */
+/* tiny_skia::BlendMode, in tiny-skia's declaration order. */
+enum {
+    RVK_BLEND_CLEAR = 0, RVK_BLEND_SOURCE, RVK_BLEND_DESTINATION, RVK_BLEND_SOURCE_OVER,
+    RVK_BLEND_DESTINATION_OVER, RVK_BLEND_SOURCE_IN, RVK_BLEND_DESTINATION_IN,
+    RVK_BLEND_SOURCE_OUT, RVK_BLEND_DESTINATION_OUT, RVK_BLEND_SOURCE_ATOP,
+    RVK_BLEND_DESTINATION_ATOP, RVK_BLEND_XOR, RVK_BLEND_PLUS, RVK_BLEND_MODULATE,
+    RVK_BLEND_SCREEN, RVK_BLEND_OVERLAY, RVK_BLEND_DARKEN, RVK_BLEND_LIGHTEN,
+    RVK_BLEND_COLOR_DODGE, RVK_BLEND_COLOR_BURN, RVK_BLEND_HARD_LIGHT, RVK_BLEND_SOFT_LIGHT,
+    RVK_BLEND_DIFFERENCE, RVK_BLEND_EXCLUSION, RVK_BLEND_MULTIPLY, RVK_BLEND_HUE,
+    RVK_BLEND_SATURATION, RVK_BLEND_COLOR, RVK_BLEND_LUMINOSITY, RVK_BLEND_COUNT
+};
+
+/* usvg::BlendMode (crates/usvg/src/tree/mod.rs:216-233); mapped to tiny-skia
+ * modes by convert_blend_mode (crates/resvg/src/render.rs:254-273). */
+enum {
+    RVK_MIX_NORMAL = 0, RVK_MIX_MULTIPLY, RVK_MIX_SCREEN, RVK_MIX_OVERLAY, RVK_MIX_DARKEN,
+    RVK_MIX_LIGHTEN, RVK_MIX_COLOR_DODGE, RVK_MIX_COLOR_BURN, RVK_MIX_HARD_LIGHT,
+    RVK_MIX_SOFT_LIGHT, RVK_MIX_DIFFERENCE, RVK_MIX_EXCLUSION, RVK_MIX_HUE,
+    RVK_MIX_SATURATION, RVK_MIX_COLOR, RVK_MIX_LUMINOSITY, RVK_MIX_COUNT
+};
+
+/* usvg::filter::ColorInterpolation (crates/usvg/src/tree/filter.rs:141-146). */
+enum { RVK_CI_SRGB = 0, RVK_CI_LINEAR_RGB = 1 };
+
+/* usvg::filter::Input (filter.rs:130-134).  A non-negative input value is a
+ * result name id: Reference(name) resolves to the latest earlier primitive whose
+ * `result` equals that id (filter/mod.rs:554-562). */
+#define RVK_INPUT_SOURCE_GRAPHIC (-1)
+#define RVK_INPUT_SOURCE_ALPHA   (-2)
+
+/* usvg::filter::Kind (filter.rs:82-100). */
+enum {
+    RVK_FE_BLEND = 0, RVK_FE_COLOR_MATRIX, RVK_FE_COMPONENT_TRANSFER, RVK_FE_COMPOSITE,
+    RVK_FE_CONVOLVE_MATRIX, RVK_FE_DIFFUSE_LIGHTING, RVK_FE_DISPLACEMENT_MAP,
+    RVK_FE_DROP_SHADOW, RVK_FE_FLOOD, RVK_FE_GAUSSIAN_BLUR, RVK_FE_IMAGE, RVK_FE_MERGE,
+    RVK_FE_MORPHOLOGY, RVK_FE_OFFSET, RVK_FE_SPECULAR_LIGHTING, RVK_FE_TILE,
+    RVK_FE_TURBULENCE, RVK_FE_KIND_COUNT
+};
+
+/* CompositeOperator (filter.rs:332-338). */
+enum { RVK_COMPOSITE_OVER = 0, RVK_COMPOSITE_IN, RVK_COMPOSITE_OUT, RVK_COMPOSITE_ATOP,
+       RVK_COMPOSITE_XOR, RVK_COMPOSITE_ARITHMETIC };
+/* ColorMatrixKind (filter.rs:208-213). */
+enum { RVK_COLOR_MATRIX_MATRIX = 0, RVK_COLOR_MATRIX_SATURATE, RVK_COLOR_MATRIX_HUE_ROTATE,
+       RVK_COLOR_MATRIX_LUMINANCE_TO_ALPHA };
+/* TransferFunction (filter.rs:269-296). */
+enum { RVK_TRANSFER_IDENTITY = 0, RVK_TRANSFER_TABLE, RVK_TRANSFER_DISCRETE,
+       RVK_TRANSFER_LINEAR, RVK_TRANSFER_GAMMA };
+/* EdgeMode (filter.rs:484-488). */
+enum { RVK_EDGE_NONE = 0, RVK_EDGE_DUPLICATE, RVK_EDGE_WRAP };
+/* MorphologyOperator (filter.rs:947-950). */
+enum { RVK_MORPH_ERODE = 0, RVK_MORPH_DILATE };
+/* ColorChannel (filter.rs:542-547). */
+enum { RVK_CHANNEL_R = 0, RVK_CHANNEL_G, RVK_CHANNEL_B, RVK_CHANNEL_A };
+/* LightSource (filter.rs:792-796). */
+enum { RVK_LIGHT_DISTANT = 0, RVK_LIGHT_POINT, RVK_LIGHT_SPOT };
+
+/* ------------------------------------------------------------------------- */
+/* Filter description: a flattened usvg::filter::Filter                      */
+/* ------------------------------------------------------------------------- */
+typedef struct rvk_transfer_fn {          /* TransferFunction */
+    uint32_t kind;                         /* RVK_TRANSFER_* */
+    float a, b, c;                         /* linear: slope, intercept; gamma: amplitude, exponent, offset */
+    uint32_t table_offset, table_len;      /* table/discrete values in rvk_filter_data.values */
+} rvk_transfer_fn;
+
+typedef struct rvk_light_source {          /* LightSource (filter.rs:792-884) */
+    uint32_t kind;                         /* RVK_LIGHT_* */
+    float azimuth, elevation;              /* distant */
+    float x, y, z;                         /* point, spot */
+    float points_at_x, points_at_y, points_at_z; /* spot */
+    float specular_exponent;               /* spot (PositiveF32) */
+    uint32_t has_limiting_cone_angle;
+    float limiting_cone_angle;
+} rvk_light_source;
+
+typedef struct rvk_primitive {
+    uint32_t kind;                         /* RVK_FE_* */
+    uint32_t color_interpolation;          /* RVK_CI_* */
+    rvk_rect rect;                         /* primitive subregion, user space (Primitive::rect) */
+    int32_t in1, in2;                      /* RVK_INPUT_* or a result id */
+    int32_t result;                        /* result name id (any value; compared for equality) */
+    union {
+        struct { uint32_t mode; } blend;                            /* RVK_BLEND_* (tiny-skia mode) */
+        struct { uint32_t kind; float values[20]; } color_matrix;   /* saturate/hue: values[0] */
+        struct { rvk_transfer_fn func[4]; } component_transfer;     /* r, g, b, a */
+        struct { uint32_t op; float k1, k2, k3, k4; } composite;
+        struct { uint32_t columns, rows, target_x, target_y;
+                 float divisor, bias; uint32_t edge_mode, preserve_alpha;
+                 uint32_t data_offset; } convolve;                  /* row-major columns*rows in values */
+        struct { float surface_scale, constant, specular_exponent;
+                 uint8_t color[4]; rvk_light_source light; } lighting; /* constant = diffuse or specular */
+        struct { float scale; uint32_t x_channel, y_channel; } displacement;
+        struct { float dx, dy, std_dev_x, std_dev_y; uint8_t color[4]; float opacity; } drop_shadow;
+        struct { uint8_t color[4]; float opacity; } flood;          /* color[3] unused */
+        struct { float std_dev_x, std_dev_y; } blur;
+        struct { uint32_t input_offset, input_count; } merge;       /* inputs in rvk_filter_data.inputs */
+        struct { uint32_t op; float radius_x, radius_y; } morphology;
+        struct { float dx, dy; } offset;
+        struct { float base_frequency_x, base_frequency_y; uint32_t num_octaves;
+                 int32_t seed; uint32_t stitch_tiles, fractal_noise; } turbulence;
+        float pad_[48];
+    } u;
+} rvk_primitive;
+
+typedef struct rvk_filter {                /* usvg::filter::Filter (filter.rs:14-42) */
+    rvk_rect rect;                         /* filter region, user space */
+    uint32_t primitive_offset, primitive_count;
+} rvk_filter;
+
+typedef struct rvk_filter_data {
+    const rvk_primitive *primitives;
+    const float *values;                   /* convolve kernels, transfer tables */
+    const int32_t *inputs;                 /* merge input lists */
+} rvk_filter_data;
+
+/* ------------------------------------------------------------------------- */
+/* Scene: a flattened usvg::Tree (crates/usvg/src/tree/mod.rs)                */
+/* ------------------------------------------------------------------------- */
+enum { RVK_VERB_MOVE = 0, RVK_VERB_LINE, RVK_VERB_QUAD, RVK_VERB_CUBIC, RVK_VERB_CLOSE };
+enum { RVK_PAINT_NONE = 0, RVK_PAINT_COLOR, RVK_PAINT_LINEAR, RVK_PAINT_RADIAL };
+enum { RVK_SPREAD_PAD = 0, RVK_SPREAD_REFLECT, RVK_SPREAD_REPEAT };
+enum { RVK_FILL_NONZERO = 0, RVK_FILL_EVENODD };
+enum { RVK_CAP_BUTT = 0, RVK_CAP_ROUND, RVK_CAP_SQUARE };
+enum { RVK_JOIN_MITER = 0, RVK_JOIN_MITER_CLIP, RVK_JOIN_ROUND, RVK_JOIN_BEVEL };
+enum { RVK_NODE_GROUP = 0, RVK_NODE_PATH };
+
+typedef struct rvk_stop {                  /* usvg::Stop */
+    float offset;
+    uint8_t color[4];                      /* r, g, b, unused */
+    float opacity;
+} rvk_stop;
+
+typedef struct rvk_paint {                 /* usvg::Paint + fill/stroke opacity */
+    uint32_t kind;                         /* RVK_PAINT_* */
+    uint8_t color[4];                      /* solid colour r, g, b, unused */
+    float opacity;                         /* Fill::opacity / Stroke::opacity */
+    uint32_t spread;                       /* RVK_SPREAD_* */
+    float x1, y1, x2, y2;                  /* linear */
+    float cx, cy, r, fx, fy, fr;           /* radial */
+    rvk_transform transform;               /* gradient transform */
+    uint32_t stop_offset, stop_count;
+} rvk_paint;
+
+typedef struct rvk_path {                  /* usvg::Path (mod.rs:1282-1440) */
+    uint32_t verb_offset, verb_count;
+    uint32_t point_offset;                 /* in points (x,y pairs) */
+    uint32_t visible, anti_alias, paint_order_stroke_first;
+    rvk_paint fill;
+    uint32_t fill_rule;                    /* RVK_FILL_* */
+    rvk_paint stroke;                      /* kind NONE = no stroke */
+    float stroke_width, stroke_miter_limit;
+    uint32_t stroke_cap, stroke_join;
+    rvk_rect bounds;                       /* path.data().bounds(): bounds of all points */
+} rvk_path;
+
+typedef struct rvk_group {                 /* usvg::Group (mod.rs:1035-1225) */
+    rvk_transform transform;
+    float opacity;
+    uint32_t blend_mode;                   /* RVK_MIX_* */
+    uint32_t isolate;
+    int32_t clip_path, mask;               /* -1: none (clip paths and masks are not in this scene format) */
+    uint32_t filter_offset, filter_count;
+    rvk_rect layer_bbox;                   /* layer_bounding_box(), object coordinates */
+    uint32_t child_offset, child_count;    /* into rvk_scene.children */
+} rvk_group;
+
+typedef struct rvk_node_ref { uint32_t kind, index; } rvk_node_ref;
+
+typedef struct rvk_scene {
+    const rvk_group *groups;    uint32_t group_count;   /* groups[0] is the root */
+    const rvk_path *paths;      uint32_t path_count;
+    const rvk_node_ref *children;
+    const uint8_t *verbs;
+    const float *points;
+    const rvk_stop *stops;
+    const rvk_filter *filters;
+    rvk_filter_data filter_data;
+} rvk_scene;
+
+/* ------------------------------------------------------------------------- */
+/* Context, memory, streams                                                  */
+/* ------------------------------------------------------------------------- */
+typedef struct rvk_context rvk_context;
+
+/* Creates a context on HIP device `device` with its own stream. */
+rvk_status rvk_context_create(int device, rvk_context **out);
+rvk_status rvk_context_destroy(rvk_context *ctx);
+/* Uses an existing hipStream_t (e.g. torch.cuda.current_stream().cuda_stream) instead. */
+rvk_status rvk_context_set_stream(rvk_context *ctx, void *hip_stream);
+void *rvk_context_stream(rvk_context *ctx);
+rvk_status rvk_synchronize(rvk_context *ctx);
+/* Device memory helpers for C callers without another allocator. */
+rvk_status rvk_device_alloc(rvk_context *ctx, size_t bytes, void **out);
+rvk_status rvk_device_free(rvk_context *ctx, void *dev_ptr);
+rvk_status rvk_upload(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes);
+rvk_status rvk_download(rvk_context *ctx, void *host_dst, const void *dev_src, size_t bytes);
+
+/* ------------------------------------------------------------------------- */
+/* Filter primitives on device pixmaps                                       */
+/* ------------------------------------------------------------------------- */
+/* Image::into_color_space: demultiply -> LUT -> premultiply (filter/mod.rs:115-125,286-308). */
+rvk_status rvk_into_linear_rgb(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h);
+rvk_status rvk_into_srgb(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h);
+/* demultiply_alpha / multiply_alpha (filter/mod.rs:129-146). */
+rvk_status rvk_demultiply_alpha(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h);
+rvk_status rvk_multiply_alpha(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h);
+
+/* box_blur::apply (filter/box_blur.rs:23-34); in place. */
+rvk_status rvk_box_blur(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h,
+                        double sigma_x, double sigma_y);
+/* iir_blur::apply (filter/iir_blur.rs:47-65); in place. */
+rvk_status rvk_iir_blur(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h,
+                        double sigma_x, double sigma_y);
+/* morphology::apply (filter/morphology.rs:15-73); in place. rx, ry already scaled. */
+rvk_status rvk_morphology(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h,
+                          uint32_t op, float rx, float ry);
+/* convolve_matrix::apply (filter/convolve_matrix.rs:15-111); in place.
+ * `prim` must be an RVK_FE_CONVOLVE_MATRIX primitive; its kernel is read from host `values`. */
+rvk_status rvk_convolve_matrix(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h,
+                               const rvk_primitive *prim, const float *values);
+/* color_matrix::apply (filter/color_matrix.rs:11-110) on UNPREMULTIPLIED pixels; in place. */
+rvk_status rvk_color_matrix(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h,
+                            const rvk_primitive *prim);
+/* component_transfer::apply (filter/component_transfer.rs:10-72) on UNPREMULTIPLIED pixels. */
+rvk_status rvk_component_transfer(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h,
+                                  const rvk_primitive *prim, const float *values);
+/* composite::arithmetic (filter/composite.rs:14-50): dst = f(src1, src2); dst untouched where alpha ~ 0. */
+rvk_status rvk_composite_arithmetic(rvk_context *ctx, const uint8_t *dev_src1, const uint8_t *dev_src2,
+                                    uint8_t *dev_dst, uint32_t w, uint32_t h,
+                                    float k1, float k2, float k3, float k4);
+/* turbulence::apply (filter/turbulence.rs:33-91): writes UNPREMULTIPLIED noise to dst. */
+rvk_status rvk_turbulence(rvk_context *ctx, uint8_t *dev_dst, uint32_t w, uint32_t h,
+                          double offset_x, double offset_y, double sx, double sy,
+                          double base_frequency_x, double base_frequency_y, uint32_t num_octaves,
+                          int32_t seed, uint32_t stitch_tiles, uint32_t fractal_noise);
+/* lighting::diffuse_lighting / specular_lighting (filter/lighting.rs:132-225); `light` is
+ * already in region pixel coordinates (transform_light_source, filter/mod.rs:1063-1098). */
+rvk_status rvk_lighting(rvk_context *ctx, const uint8_t *dev_src, uint8_t *dev_dst, uint32_t w, uint32_t h,
+                        uint32_t specular, float surface_scale, float constant, float specular_exponent,
+                        const uint8_t color[4], const rvk_light_source *light);
+/* displacement_map::apply (filter/displacement_map.rs:15-62); sx, sy = canvas scale. */
+rvk_status rvk_displacement_map(rvk_context *ctx, const uint8_t *dev_src, const uint8_t *dev_map,
+                                uint8_t *dev_dst, uint32_t w, uint32_t h, float sx, float sy,
+                                float scale, uint32_t x_channel, uint32_t y_channel);
+
+/* tiny_skia::PixmapMut::draw_pixmap(x, y, src, PixmapPaint{opacity, blend_mode, Nearest},
+ * identity) — the call made by render.rs:242-249 and filter/mod.rs:624-1111. */
+rvk_status rvk_draw_pixmap(rvk_context *ctx, uint8_t *dev_dst, uint32_t dw, uint32_t dh,
+                           int32_t x, int32_t y, const uint8_t *dev_src, uint32_t sw, uint32_t sh,
+                           float opacity, uint32_t blend_mode);
+/* tiny_skia::Pixmap::fill(Color::from_rgba8(r, g, b, a)). */
+rvk_status rvk_fill(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h, const uint8_t rgba[4]);
+
+/* filter::apply (filter/mod.rs:335-355): runs one filter over the layer `dev_px` (w x h)
+ * with the layer transform `ts`.  On a filter error the layer is cleared (as the reference
+ * does) and RVK_OK is returned; `*out_error` (if non-null) receives the filter error
+ * (RVK_OK, RVK_ERR_INVALID_REGION).  Scratch images are allocated from the context. */
+rvk_status rvk_filter_apply(rvk_context *ctx, const rvk_filter *filter, const rvk_filter_data *data,
+                            rvk_transform ts, uint8_t *dev_px, uint32_t w, uint32_t h,
+                            rvk_status *out_error);
+
+/* ------------------------------------------------------------------------- */
+/* Rendering                                                                 */
+/* ------------------------------------------------------------------------- */
+/* resvg::render (lib.rs:34-43) onto a device pixmap: composites the scene onto the
+ * existing content of `dev_pixmap` (it never clears it). */
+rvk_status rvk_render(rvk_context *ctx, const rvk_scene *scene, rvk_transform transform,
+                      uint32_t width, uint32_t height, uint8_t *dev_pixmap);
+/* resvg_render (crates/c-api/lib.rs:874-897) twin: host pixmap in and out; uploads,
+ * renders, downloads and synchronizes. */
+rvk_status rvk_render_host(rvk_context *ctx, const rvk_scene *scene, rvk_transform transform,
+                           uint32_t width, uint32_t height, uint8_t *host_pixmap);
+
+/* Per-call statistics of the last rvk_render / rvk_filter_apply on this context. */
+typedef struct rvk_stats {
+    uint64_t kernel_launches;
+    uint64_t layers;
+    uint64_t paths;
+    uint64_t filters;
+    uint64_t scratch_bytes;
+} rvk_stats;
+rvk_status rvk_get_stats(rvk_context *ctx, rvk_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RESVG_AMD_H */

@@ -1,0 +1,172 @@
+"""ctypes bindings of liboracle.so — the CPU restatement of the reference path.
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, never by the product package.  Arrays are numpy
+uint8 (H, W, 4) premultiplied RGBA8, modified in place where the reference works
+in place.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from resvg_amd import abi
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+_lib = None
+
+
+def build() -> Path:
+    """Compiles the oracle with its Makefile (g++ -O2 -ffp-contract=off)."""
+    r = subprocess.run(["make", "-B", "-C", str(HERE), "liboracle.so"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        P = C.c_void_p
+        u32, i32, f32, f64 = C.c_uint32, C.c_int32, C.c_float, C.c_double
+        sig = {
+            "orc_into_linear_rgb": [P, u32, u32], "orc_into_srgb": [P, u32, u32],
+            "orc_demultiply_alpha": [P, u32, u32], "orc_multiply_alpha": [P, u32, u32],
+            "orc_box_blur": [P, u32, u32, f64, f64], "orc_box_gauss_sizes": [f32, C.POINTER(i32)],
+            "orc_iir_blur": [P, u32, u32, f64, f64], "orc_morphology": [P, u32, u32, u32, f32, f32],
+            "orc_convolve_matrix": [P, u32, u32, C.POINTER(abi.Primitive), C.POINTER(f32)],
+            "orc_color_matrix": [P, u32, u32, C.POINTER(abi.Primitive)],
+            "orc_component_transfer": [P, u32, u32, C.POINTER(abi.Primitive), C.POINTER(f32)],
+            "orc_composite_arithmetic": [P, P, P, u32, u32, f32, f32, f32, f32],
+            "orc_turbulence": [P, u32, u32, f64, f64, f64, f64, f64, f64, u32, i32, u32, u32],
+            "orc_turbulence_random": [i32], "orc_turbulence_init": [i32, P, P],
+            "orc_lighting": [P, P, u32, u32, u32, f32, f32, f32, C.POINTER(C.c_uint8), C.POINTER(abi.LightSource)],
+            "orc_displacement_map": [P, P, P, u32, u32, f32, f32, f32, u32, u32],
+            "orc_draw_pixmap": [P, u32, u32, i32, i32, P, u32, u32, f32, u32],
+            "orc_fill": [P, u32, u32, C.POINTER(C.c_uint8)],
+            "orc_filter_apply": [C.POINTER(abi.Filter), C.POINTER(abi.FilterData), abi.Transform, P, u32, u32],
+            "orc_srgb_to_linear_table": [], "orc_linear_to_srgb_table": [],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = None
+        L.orc_turbulence_random.restype = i32
+        L.orc_filter_apply.restype = i32
+        L.orc_srgb_to_linear_table.restype = C.POINTER(C.c_uint8)
+        L.orc_linear_to_srgb_table.restype = C.POINTER(C.c_uint8)
+        for opt in ("orc_render", "orc_fill_path", "orc_path_coverage"):
+            if hasattr(L, opt):
+                getattr(L, opt).restype = i32
+        if hasattr(L, "orc_render"):
+            L.orc_render.argtypes = [C.POINTER(abi.Scene), abi.Transform, u32, u32, P]
+        if hasattr(L, "orc_path_coverage"):
+            L.orc_path_coverage.argtypes = [C.POINTER(abi.Scene), u32, abi.Transform, u32, P, u32, u32]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data
+
+
+def _wh(a: np.ndarray):
+    h, w, c = a.shape
+    assert c == 4
+    return w, h
+
+
+def tables():
+    L = lib()
+    s2l = np.ctypeslib.as_array(L.orc_srgb_to_linear_table(), shape=(256,)).copy()
+    l2s = np.ctypeslib.as_array(L.orc_linear_to_srgb_table(), shape=(256,)).copy()
+    return s2l, l2s
+
+
+def into_linear_rgb(a): lib().orc_into_linear_rgb(_p(a), *_wh(a))
+def into_srgb(a): lib().orc_into_srgb(_p(a), *_wh(a))
+def demultiply_alpha(a): lib().orc_demultiply_alpha(_p(a), *_wh(a))
+def multiply_alpha(a): lib().orc_multiply_alpha(_p(a), *_wh(a))
+def box_blur(a, sx, sy): lib().orc_box_blur(_p(a), *_wh(a), sx, sy)
+def iir_blur(a, sx, sy): lib().orc_iir_blur(_p(a), *_wh(a), sx, sy)
+def morphology(a, op, rx, ry): lib().orc_morphology(_p(a), *_wh(a), op, rx, ry)
+
+
+def box_gauss_sizes(sigma: float):
+    out = (C.c_int32 * 5)()
+    lib().orc_box_gauss_sizes(sigma, out)
+    return list(out)
+
+
+def convolve_matrix(a, prim, values):
+    lib().orc_convolve_matrix(_p(a), *_wh(a), C.byref(prim), abi.array(C.c_float, values))
+
+
+def color_matrix(a, prim):
+    lib().orc_color_matrix(_p(a), *_wh(a), C.byref(prim))
+
+
+def component_transfer(a, prim, values):
+    lib().orc_component_transfer(_p(a), *_wh(a), C.byref(prim), abi.array(C.c_float, values))
+
+
+def composite_arithmetic(s1, s2, dst, k1, k2, k3, k4):
+    lib().orc_composite_arithmetic(_p(s1), _p(s2), _p(dst), *_wh(dst), k1, k2, k3, k4)
+
+
+def turbulence(dst, ox, oy, sx, sy, bfx, bfy, octaves, seed, stitch, fractal):
+    lib().orc_turbulence(_p(dst), *_wh(dst), ox, oy, sx, sy, bfx, bfy, octaves, seed, int(stitch), int(fractal))
+
+
+def turbulence_random(seed: int) -> int:
+    return lib().orc_turbulence_random(seed)
+
+
+def turbulence_init(seed: int):
+    lat = np.zeros(514, dtype=np.int32)
+    grad = np.zeros((4, 514, 2), dtype=np.float64)
+    lib().orc_turbulence_init(seed, lat.ctypes.data, grad.ctypes.data)
+    return lat, grad
+
+
+def lighting(src, dst, specular, surface_scale, constant, spec_exp, color, light):
+    col = (C.c_uint8 * 4)(*color[:3], 255)
+    lib().orc_lighting(_p(src), _p(dst), *_wh(dst), int(specular), surface_scale, constant, spec_exp, col,
+                       C.byref(light))
+
+
+def displacement_map(src, map_, dst, sx, sy, scale, xch, ych):
+    lib().orc_displacement_map(_p(src), _p(map_), _p(dst), *_wh(dst), sx, sy, scale, xch, ych)
+
+
+def draw_pixmap(dst, x, y, src, opacity=1.0, mode=abi.BLEND["source_over"]):
+    lib().orc_draw_pixmap(_p(dst), *_wh(dst), x, y, _p(src), *_wh(src), opacity, mode)
+
+
+def fill(a, rgba):
+    lib().orc_fill(_p(a), *_wh(a), (C.c_uint8 * 4)(*rgba))
+
+
+def filter_apply(graph, ts: abi.Transform, a) -> int:
+    f, d, keep = graph.to_abi()
+    r = lib().orc_filter_apply(C.byref(f), C.byref(d), ts, _p(a), *_wh(a))
+    del keep
+    return r
+
+
+def render(scene_abi, ts: abi.Transform, a) -> int:
+    return lib().orc_render(C.byref(scene_abi), ts, *_wh(a), _p(a))
+
+
+def path_coverage(scene_abi, path_index: int, ts: abi.Transform, stroke: bool, w: int, h: int) -> np.ndarray:
+    cov = np.zeros((h, w), dtype=np.uint8)
+    lib().orc_path_coverage(C.byref(scene_abi), path_index, ts, int(stroke), cov.ctypes.data, w, h)
+    return cov

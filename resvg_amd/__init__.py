@@ -1,0 +1,251 @@
+"""resvg_amd — the resvg pixel pipeline on MI355X (gfx950).
+
+Python bindings of librvk.so (include/resvg_amd.h).  The library is required: a
+missing or unloadable librvk.so raises at import of the first entry point, there is
+no CPU fallback.  Pixel buffers are torch uint8 CUDA tensors of shape (H, W, 4)
+holding premultiplied RGBA8 (the tiny_skia::Pixmap layout).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+from . import abi
+from .abi import Transform  # noqa: F401
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "librvk.so"
+_lib = None
+
+
+class RvkError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        super().__init__(f"{what}: rvk status {status}")
+
+
+def lib() -> C.CDLL:
+    """Loads librvk.so; raises when it is missing (build with resvg_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m resvg_amd.build` (no CPU fallback exists)")
+        _lib = C.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L: C.CDLL) -> None:
+    P = C.c_void_p
+    u32, i32, f32, f64 = C.c_uint32, C.c_int32, C.c_float, C.c_double
+    sigs = {
+        "rvk_status_string": ([i32], C.c_char_p),
+        "rvk_context_create": ([C.c_int, C.POINTER(P)], i32),
+        "rvk_context_destroy": ([P], i32),
+        "rvk_context_set_stream": ([P, P], i32),
+        "rvk_context_stream": ([P], P),
+        "rvk_synchronize": ([P], i32),
+        "rvk_device_alloc": ([P, C.c_size_t, C.POINTER(P)], i32),
+        "rvk_device_free": ([P, P], i32),
+        "rvk_upload": ([P, P, P, C.c_size_t], i32),
+        "rvk_download": ([P, P, P, C.c_size_t], i32),
+        "rvk_into_linear_rgb": ([P, P, u32, u32], i32),
+        "rvk_into_srgb": ([P, P, u32, u32], i32),
+        "rvk_demultiply_alpha": ([P, P, u32, u32], i32),
+        "rvk_multiply_alpha": ([P, P, u32, u32], i32),
+        "rvk_box_blur": ([P, P, u32, u32, f64, f64], i32),
+        "rvk_iir_blur": ([P, P, u32, u32, f64, f64], i32),
+        "rvk_morphology": ([P, P, u32, u32, u32, f32, f32], i32),
+        "rvk_convolve_matrix": ([P, P, u32, u32, C.POINTER(abi.Primitive), C.POINTER(f32)], i32),
+        "rvk_color_matrix": ([P, P, u32, u32, C.POINTER(abi.Primitive)], i32),
+        "rvk_component_transfer": ([P, P, u32, u32, C.POINTER(abi.Primitive), C.POINTER(f32)], i32),
+        "rvk_composite_arithmetic": ([P, P, P, P, u32, u32, f32, f32, f32, f32], i32),
+        "rvk_turbulence": ([P, P, u32, u32, f64, f64, f64, f64, f64, f64, u32, i32, u32, u32], i32),
+        "rvk_lighting": ([P, P, P, u32, u32, u32, f32, f32, f32, C.POINTER(C.c_uint8), C.POINTER(abi.LightSource)],
+                         i32),
+        "rvk_displacement_map": ([P, P, P, P, u32, u32, f32, f32, f32, u32, u32], i32),
+        "rvk_draw_pixmap": ([P, P, u32, u32, i32, i32, P, u32, u32, f32, u32], i32),
+        "rvk_fill": ([P, P, u32, u32, C.POINTER(C.c_uint8)], i32),
+        "rvk_filter_apply": ([P, C.POINTER(abi.Filter), C.POINTER(abi.FilterData), abi.Transform, P, u32, u32,
+                              C.POINTER(i32)], i32),
+        "rvk_get_stats": ([P, C.POINTER(abi.Stats)], i32),
+    }
+    optional = {
+        "rvk_render": ([P, C.POINTER(abi.Scene), abi.Transform, u32, u32, P], i32),
+        "rvk_render_host": ([P, C.POINTER(abi.Scene), abi.Transform, u32, u32, P], i32),
+    }
+    for name, (args, res) in {**sigs, **optional}.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            if name in optional:
+                continue
+            raise RuntimeError(f"librvk.so does not export {name}")
+        fn.argtypes = args
+        fn.restype = res
+
+
+def exported_symbols() -> list[str]:
+    """Names declared in include/resvg_amd.h that the loaded library exports."""
+    L = lib()
+    hdr = (_PKG.parent / "include" / "resvg_amd.h").read_text()
+    import re
+    names = sorted(set(re.findall(r"\b(rvk_[a-z0-9_]+)\s*\(", hdr)))
+    return [n for n in names if hasattr(L, n)]
+
+
+def _check(status: int, what: str) -> None:
+    if status != abi.OK:
+        raise RvkError(status, what)
+
+
+def _ptr(t) -> int:
+    """Device pointer of a torch tensor (or a raw int pointer)."""
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _dims(px) -> tuple[int, int]:
+    h, w, c = px.shape
+    assert c == 4 and px.dtype.itemsize == 1 and px.is_contiguous(), "expected a contiguous (H, W, 4) uint8 tensor"
+    return w, h
+
+
+class Context:
+    """An rvk_context on one HIP device, ordered on torch's current stream."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = True):
+        L = lib()
+        h = C.c_void_p()
+        _check(L.rvk_context_create(device, C.byref(h)), "rvk_context_create")
+        self._h = h
+        self.device = device
+        if use_torch_stream:
+            import torch
+            s = torch.cuda.current_stream(device).cuda_stream
+            _check(L.rvk_context_set_stream(self._h, C.c_void_p(s)), "rvk_context_set_stream")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().rvk_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self) -> None:
+        _check(lib().rvk_synchronize(self._h), "rvk_synchronize")
+
+    def stats(self) -> abi.Stats:
+        s = abi.Stats()
+        _check(lib().rvk_get_stats(self._h, C.byref(s)), "rvk_get_stats")
+        return s
+
+    # --- filter primitives (in place on a (H, W, 4) uint8 device tensor) ---
+    def into_linear_rgb(self, px):
+        w, h = _dims(px)
+        _check(lib().rvk_into_linear_rgb(self._h, _ptr(px), w, h), "rvk_into_linear_rgb")
+
+    def into_srgb(self, px):
+        w, h = _dims(px)
+        _check(lib().rvk_into_srgb(self._h, _ptr(px), w, h), "rvk_into_srgb")
+
+    def demultiply_alpha(self, px):
+        w, h = _dims(px)
+        _check(lib().rvk_demultiply_alpha(self._h, _ptr(px), w, h), "rvk_demultiply_alpha")
+
+    def multiply_alpha(self, px):
+        w, h = _dims(px)
+        _check(lib().rvk_multiply_alpha(self._h, _ptr(px), w, h), "rvk_multiply_alpha")
+
+    def box_blur(self, px, sigma_x: float, sigma_y: float):
+        w, h = _dims(px)
+        _check(lib().rvk_box_blur(self._h, _ptr(px), w, h, sigma_x, sigma_y), "rvk_box_blur")
+
+    def iir_blur(self, px, sigma_x: float, sigma_y: float):
+        w, h = _dims(px)
+        _check(lib().rvk_iir_blur(self._h, _ptr(px), w, h, sigma_x, sigma_y), "rvk_iir_blur")
+
+    def morphology(self, px, op: int, rx: float, ry: float):
+        w, h = _dims(px)
+        _check(lib().rvk_morphology(self._h, _ptr(px), w, h, op, rx, ry), "rvk_morphology")
+
+    def convolve_matrix(self, px, prim: abi.Primitive, values):
+        w, h = _dims(px)
+        vals = abi.array(C.c_float, values)
+        _check(lib().rvk_convolve_matrix(self._h, _ptr(px), w, h, C.byref(prim), vals), "rvk_convolve_matrix")
+
+    def color_matrix(self, px, prim: abi.Primitive):
+        w, h = _dims(px)
+        _check(lib().rvk_color_matrix(self._h, _ptr(px), w, h, C.byref(prim)), "rvk_color_matrix")
+
+    def component_transfer(self, px, prim: abi.Primitive, values):
+        w, h = _dims(px)
+        vals = abi.array(C.c_float, values)
+        _check(lib().rvk_component_transfer(self._h, _ptr(px), w, h, C.byref(prim), vals),
+               "rvk_component_transfer")
+
+    def composite_arithmetic(self, src1, src2, dst, k1, k2, k3, k4):
+        w, h = _dims(dst)
+        _check(lib().rvk_composite_arithmetic(self._h, _ptr(src1), _ptr(src2), _ptr(dst), w, h, k1, k2, k3, k4),
+               "rvk_composite_arithmetic")
+
+    def turbulence(self, dst, offset_x, offset_y, sx, sy, bfx, bfy, octaves, seed, stitch, fractal):
+        w, h = _dims(dst)
+        _check(lib().rvk_turbulence(self._h, _ptr(dst), w, h, offset_x, offset_y, sx, sy, bfx, bfy, octaves, seed,
+                                    int(stitch), int(fractal)), "rvk_turbulence")
+
+    def lighting(self, src, dst, specular, surface_scale, constant, specular_exponent, color, light):
+        w, h = _dims(dst)
+        col = (C.c_uint8 * 4)(*color[:3], 255)
+        _check(lib().rvk_lighting(self._h, _ptr(src), _ptr(dst), w, h, int(specular), surface_scale, constant,
+                                  specular_exponent, col, C.byref(light)), "rvk_lighting")
+
+    def displacement_map(self, src, map_, dst, sx, sy, scale, x_channel, y_channel):
+        w, h = _dims(dst)
+        _check(lib().rvk_displacement_map(self._h, _ptr(src), _ptr(map_), _ptr(dst), w, h, sx, sy, scale, x_channel,
+                                          y_channel), "rvk_displacement_map")
+
+    def draw_pixmap(self, dst, x: int, y: int, src, opacity: float = 1.0, blend_mode: int = abi.BLEND["source_over"]):
+        dw, dh = _dims(dst)
+        sw, sh = _dims(src)
+        _check(lib().rvk_draw_pixmap(self._h, _ptr(dst), dw, dh, x, y, _ptr(src), sw, sh, opacity, blend_mode),
+               "rvk_draw_pixmap")
+
+    def fill(self, px, rgba):
+        w, h = _dims(px)
+        col = (C.c_uint8 * 4)(*rgba)
+        _check(lib().rvk_fill(self._h, _ptr(px), w, h, col), "rvk_fill")
+
+    def filter_apply(self, graph, ts: abi.Transform, px) -> int:
+        """filter::apply on a layer; returns the filter error status (0 when none)."""
+        w, h = _dims(px)
+        f, d, keep = graph.to_abi()
+        err = C.c_int32(0)
+        _check(lib().rvk_filter_apply(self._h, C.byref(f), C.byref(d), ts, _ptr(px), w, h, C.byref(err)),
+               "rvk_filter_apply")
+        del keep
+        return err.value
+
+    # --- rendering ---
+    def render(self, scene, ts: abi.Transform, pixmap):
+        """resvg::render onto a (H, W, 4) uint8 device tensor (composites, never clears)."""
+        w, h = _dims(pixmap)
+        sc = scene.to_abi() if hasattr(scene, "to_abi") else scene
+        _check(lib().rvk_render(self._h, C.byref(sc.abi), ts, w, h, _ptr(pixmap)), "rvk_render")
+
+    def render_host(self, scene, ts: abi.Transform, pixmap_np):
+        """resvg_render twin: host numpy (H, W, 4) uint8 in and out."""
+        import numpy as np
+        assert pixmap_np.dtype == np.uint8 and pixmap_np.flags.c_contiguous
+        h, w, _ = pixmap_np.shape
+        sc = scene.to_abi() if hasattr(scene, "to_abi") else scene
+        _check(lib().rvk_render_host(self._h, C.byref(sc.abi), ts, w, h, pixmap_np.ctypes.data), "rvk_render_host")

@@ -1,0 +1,175 @@
+// Context, stream and device-memory management of librvk.
+#include "rvk_internal.h"
+#include <algorithm>
+#include <cstdio>
+
+namespace rvk {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const char *what, const char *where) {
+    g_last_error = std::string(what ? what : "") + " at " + (where ? where : "");
+    if (std::getenv("RVK_VERBOSE")) std::fprintf(stderr, "[rvk] HIP error: %s\n", g_last_error.c_str());
+}
+
+rvk_status scratch_alloc(rvk_context *ctx, size_t bytes, void **out) {
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~(size_t)255;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        // best fit among cached buffers no more than 2x the request
+        size_t best = SIZE_MAX, bi = 0;
+        for (size_t i = 0; i < ctx->free_list.size(); i++) {
+            size_t s = ctx->free_list[i].first;
+            if (s >= bytes && s <= 2 * bytes && s < best) { best = s; bi = i; }
+        }
+        if (best != SIZE_MAX) {
+            *out = ctx->free_list[bi].second;
+            ctx->live.push_back(ctx->free_list[bi]);
+            ctx->free_list.erase(ctx->free_list.begin() + (long)bi);
+            return RVK_OK;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        // drop the cache and retry once; the stream must be idle before freeing
+        std::lock_guard<std::mutex> g(ctx->mu);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto &f : ctx->free_list) (void)hipFree(f.second);
+        ctx->free_list.clear();
+        e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return RVK_ERR_OUT_OF_MEMORY;
+        }
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->live.emplace_back(bytes, p);
+    ctx->stats.scratch_bytes += bytes;
+    *out = p;
+    return RVK_OK;
+}
+
+// Buffers go back to the context cache at once: all users of a context share one
+// stream, so a later user is ordered after every earlier kernel that touched it.
+void scratch_release(rvk_context *ctx, void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    for (size_t i = 0; i < ctx->live.size(); i++) {
+        if (ctx->live[i].second == p) {
+            ctx->free_list.push_back(ctx->live[i]);
+            ctx->live.erase(ctx->live.begin() + (long)i);
+            return;
+        }
+    }
+}
+
+rvk_status upload_sync(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes) {
+    RVK_HIP_CHECK(hipMemcpyAsync(dev_dst, host_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    RVK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return RVK_OK;
+}
+
+}  // namespace rvk
+
+extern "C" {
+
+const char *rvk_status_string(rvk_status s) {
+    switch (s) {
+        case RVK_OK: return "ok";
+        case RVK_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case RVK_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case RVK_ERR_HIP: return "HIP runtime error";
+        case RVK_ERR_INVALID_REGION: return "invalid filter region";
+        case RVK_ERR_UNSUPPORTED: return "unsupported feature";
+        case RVK_ERR_NO_DEVICE: return "no HIP device";
+        default: return "unknown status";
+    }
+}
+
+rvk_status rvk_context_create(int device, rvk_context **out) {
+    if (!out) return RVK_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return RVK_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= count) return RVK_ERR_INVALID_ARGUMENT;
+    RVK_HIP_CHECK(hipSetDevice(device));
+    rvk_context *ctx = new rvk_context();
+    ctx->device = device;
+    if (hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return RVK_ERR_HIP;
+    }
+    ctx->own_stream = true;
+    *out = ctx;
+    return RVK_OK;
+}
+
+rvk_status rvk_context_destroy(rvk_context *ctx) {
+    if (!ctx) return RVK_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &f : ctx->free_list) (void)hipFree(f.second);
+    for (auto &f : ctx->live) (void)hipFree(f.second);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RVK_OK;
+}
+
+rvk_status rvk_context_set_stream(rvk_context *ctx, void *hip_stream) {
+    if (!ctx) return RVK_ERR_INVALID_ARGUMENT;
+    RVK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    ctx->stream = (hipStream_t)hip_stream;
+    ctx->own_stream = false;
+    return RVK_OK;
+}
+
+void *rvk_context_stream(rvk_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+rvk_status rvk_synchronize(rvk_context *ctx) {
+    if (!ctx) return RVK_ERR_INVALID_ARGUMENT;
+    RVK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return RVK_OK;
+}
+
+rvk_status rvk_device_alloc(rvk_context *ctx, size_t bytes, void **out) {
+    if (!ctx || !out) return RVK_ERR_INVALID_ARGUMENT;
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return RVK_ERR_OUT_OF_MEMORY;
+    }
+    return RVK_OK;
+}
+
+rvk_status rvk_device_free(rvk_context *ctx, void *p) {
+    if (!ctx) return RVK_ERR_INVALID_ARGUMENT;
+    RVK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    RVK_HIP_CHECK(hipFree(p));
+    return RVK_OK;
+}
+
+rvk_status rvk_upload(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes) {
+    if (!ctx || (!dev_dst && bytes) || (!host_src && bytes)) return RVK_ERR_INVALID_ARGUMENT;
+    return rvk::upload_sync(ctx, dev_dst, host_src, bytes);
+}
+
+rvk_status rvk_download(rvk_context *ctx, void *host_dst, const void *dev_src, size_t bytes) {
+    if (!ctx || (!dev_src && bytes) || (!host_dst && bytes)) return RVK_ERR_INVALID_ARGUMENT;
+    RVK_HIP_CHECK(hipMemcpyAsync(host_dst, dev_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    RVK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return RVK_OK;
+}
+
+rvk_status rvk_get_stats(rvk_context *ctx, rvk_stats *out) {
+    if (!ctx || !out) return RVK_ERR_INVALID_ARGUMENT;
+    *out = ctx->stats;
+    return RVK_OK;
+}
+
+}  // extern "C"

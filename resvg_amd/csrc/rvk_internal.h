@@ -1,0 +1,90 @@
+// Internal declarations of the gfx950 pixel pipeline library (librvk.so).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "../../include/resvg_amd.h"
+
+#define RVK_HIP_CHECK(expr)                                         \
+    do {                                                            \
+        hipError_t e_ = (expr);                                     \
+        if (e_ != hipSuccess) {                                     \
+            rvk::set_last_error(hipGetErrorString(e_), #expr);      \
+            return RVK_ERR_HIP;                                     \
+        }                                                           \
+    } while (0)
+
+#define RVK_TRY(expr)                         \
+    do {                                      \
+        rvk_status s_ = (expr);               \
+        if (s_ != RVK_OK) return s_;          \
+    } while (0)
+
+struct rvk_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    rvk_stats stats{};
+    // Simple size-bucketed cache of device scratch buffers (layers, filter images).
+    std::vector<std::pair<size_t, void *>> free_list;
+    std::vector<std::pair<size_t, void *>> live;
+    std::mutex mu;
+};
+
+namespace rvk {
+
+void set_last_error(const char *what, const char *where);
+
+// Scratch device memory owned by the context (returned to its cache on release).
+rvk_status scratch_alloc(rvk_context *ctx, size_t bytes, void **out);
+void scratch_release(rvk_context *ctx, void *p);
+
+// Copies host memory to the device on the context stream and waits for it.
+rvk_status upload_sync(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes);
+
+static inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// Upper bound on one pixmap side: tiny-skia's Pixmap size limit is i32::MAX / 4 bytes per row.
+constexpr uint32_t kMaxSide = 1u << 16;
+
+static inline bool valid_dims(uint32_t w, uint32_t h) {
+    return w > 0 && h > 0 && w <= kMaxSide && h <= kMaxSide;
+}
+
+// ---- kernel launchers (defined in the .hip files) ----
+rvk_status launch_into_cs(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, int to_linear);
+rvk_status launch_demultiply(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h);
+rvk_status launch_multiply(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h);
+rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy);
+rvk_status launch_iir_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy);
+rvk_status launch_morphology(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, uint32_t op, float rx,
+                             float ry);
+rvk_status launch_convolve(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, const rvk_primitive *p,
+                           const float *values);
+rvk_status launch_color_matrix(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, const rvk_primitive *p,
+                               int fuse_alpha);
+rvk_status launch_component_transfer(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h,
+                                     const rvk_primitive *p, const float *values, int fuse_alpha);
+rvk_status launch_arithmetic(rvk_context *ctx, const uint8_t *s1, const uint8_t *s2, uint8_t *dst, size_t n,
+                             float k1, float k2, float k3, float k4);
+rvk_status launch_turbulence(rvk_context *ctx, uint8_t *dst, uint32_t w, uint32_t h, double ox, double oy,
+                             double sx, double sy, double bfx, double bfy, uint32_t octaves, int32_t seed,
+                             uint32_t stitch, uint32_t fractal, int premultiply);
+rvk_status launch_lighting(rvk_context *ctx, const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                           uint32_t specular, float surface_scale, float constant, float spec_exp,
+                           const uint8_t color[4], const rvk_light_source *light);
+rvk_status launch_displacement(rvk_context *ctx, const uint8_t *src, const uint8_t *map, uint8_t *dst,
+                               uint32_t w, uint32_t h, float sx, float sy, float scale, uint32_t xch,
+                               uint32_t ych);
+rvk_status launch_draw_pixmap(rvk_context *ctx, uint8_t *dst, uint32_t dw, uint32_t dh, int32_t x, int32_t y,
+                              const uint8_t *src, uint32_t sw, uint32_t sh, float opacity, uint32_t mode);
+rvk_status launch_fill(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, const uint8_t premul[4]);
+rvk_status launch_fill_sub(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, int64_t x0, int64_t y0,
+                           int64_t x1, int64_t y1, uint32_t value);
+rvk_status launch_copy(rvk_context *ctx, uint8_t *dst, const uint8_t *src, size_t bytes);
+rvk_status launch_zero_rgb(rvk_context *ctx, uint8_t *px, size_t n);
+
+}  // namespace rvk

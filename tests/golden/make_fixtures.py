@@ -1,0 +1,45 @@
+"""Copies the reference's own known-answer PNGs used by the filter fixture tests.
+
+The PNGs are resvg's test references (crates/resvg/tests/tests/filters/..., MIT or
+Apache-2.0, Copyright the Resvg Authors).  Only PNG data files are copied — no
+reference source.  Run in a checkout that has the reference tree mounted; the
+copies are committed so the tests never read the reference at run time.
+"""
+import shutil
+import sys
+from pathlib import Path
+
+SRC = Path("/root/reference/crates/resvg/tests/tests/filters")
+DST = Path(__file__).resolve().parent / "resvg_png"
+
+NAMES = [
+    "feGaussianBlur/simple-case", "feGaussianBlur/small-stdDeviation", "feGaussianBlur/stdDeviation=5-0",
+    "feOffset/simple-case", "feOffset/negative-offset",
+    "feComposite/operator=arithmetic", "feComposite/operator=over", "feComposite/operator=in",
+    "feComposite/operator=xor",
+    "feBlend/mode=multiply", "feBlend/mode=screen", "feBlend/mode=hue", "feBlend/mode=color-burn",
+    "feBlend/mode=normal", "feBlend/mode=darken", "feBlend/mode=lighten",
+    "feFlood/seagreen",
+    "feComponentTransfer/type=gamma-on-blue", "feComponentTransfer/type=table-on-blue",
+    "feComponentTransfer/type=linear-on-blue", "feComponentTransfer/type=discrete-on-blue",
+    "feTurbulence/type=fractalNoise", "feTurbulence/no-attributes", "feTurbulence/seed=20",
+    "feTurbulence/stitchTiles=stitch", "feTurbulence/numOctaves=5", "feTurbulence/baseFrequency=0.05-0.01",
+]
+
+
+def main() -> int:
+    DST.mkdir(exist_ok=True)
+    for n in NAMES:
+        src = SRC / f"{n}.png"
+        if not src.exists():
+            print("missing", src, file=sys.stderr)
+            return 1
+        shutil.copyfile(src, DST / (n.replace("/", "__") + ".png"))
+    (DST / "LICENSE-NOTE.txt").write_text(
+        "PNG test references copied from linebender/resvg 0.48.1 (crates/resvg/tests/tests/filters),\n"
+        "Copyright 2018-2025 the Resvg Authors, dual-licensed MIT OR Apache-2.0.\n")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
