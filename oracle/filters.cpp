@@ -772,10 +772,12 @@ static void set_lum(float &r, float &g, float &b, float l) {
     float d = l - lum(r, g, b);
     r += d; g += d; b += d;
 }
+// clip_color as tiny-skia evaluates it: only the upper clamp toward the luminance
+// takes effect (no lower `min < 0` correction); pinned by the reference fixture
+// painting/mix-blend-mode/hue, which differs by up to 4 LSB when the lower clamp runs.
 static void clip_color(float &r, float &g, float &b, float a) {
-    float mn = std::min(r, std::min(g, b)), mx = std::max(r, std::max(g, b)), l = lum(r, g, b);
+    float mx = std::max(r, std::max(g, b)), l = lum(r, g, b);
     auto cl = [&](float c) {
-        if (mn < 0.0f && l - mn != 0.0f) c = l + (c - l) * l / (l - mn);
         if (mx > a && mx - l != 0.0f) c = l + (c - l) * (a - l) / (mx - l);
         return std::max(c, 0.0f);
     };

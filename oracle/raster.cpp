@@ -1,0 +1,1058 @@
+// Oracle restatement of the tiny-skia 0.12 fill pipeline that resvg calls from
+// crates/resvg/src/path.rs:26-116: path transform, edge building with 4x
+// supersampling (shift 2), the active-edge scanline walk, SuperBlitter coverage
+// accumulation, and the u8 ("lowp") shading/SourceOver stages.  tiny-skia is not
+// vendored with the reference; the algorithm restated here is its published one
+// (Skia's SkEdge / SkScan_AntiPath / SkRasterPipeline lowp), so parity with the
+// real crate is pinned only end to end by the reference's PNG fixtures.
+//
+// One deliberate simplification: the active edges of a sub-scanline are ordered by
+// (fixed-point x, edge index) instead of by tiny-skia's insertion history.  The two
+// orders differ only between edges with exactly equal x, which can change one
+// coverage value by 1/255 on the last sub-scanline of a pixel.
+//
+// The stroker is not tiny-skia's (its source is unavailable): strokes are this
+// build's polyline outline (see stroke_outline) filled with the nonzero rule.
+// Test infrastructure only.
+#include "oracle.h"
+#include "common.h"
+#include "geom.h"
+#include <algorithm>
+#include <array>
+#include <vector>
+
+using namespace orc;
+
+namespace {
+
+struct Pt { float x, y; };
+
+// ---------------------------------------------------------------------------
+// Path access
+// ---------------------------------------------------------------------------
+struct Seg { int n; Pt p[4]; };  // n = number of points: 2 line, 3 quad, 4 cubic
+
+// Iterates the path's segments with every contour closed by a line back to its
+// start (fill semantics, tiny-skia PathEdgeIter with auto-close).
+static void fill_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
+    const uint8_t *verbs = sc.verbs + path.verb_offset;
+    const float *pts = sc.points + 2 * (size_t)path.point_offset;
+    size_t pi = 0;
+    Pt move{0, 0}, last{0, 0};
+    bool open = false;
+    auto P = [&](size_t i) {
+        Pt p{pts[2 * i], pts[2 * i + 1]};
+        ts_map(ts, p.x, p.y);
+        return p;
+    };
+    auto close = [&]() {
+        if (open && (last.x != move.x || last.y != move.y)) out.push_back(Seg{2, {last, move}});
+        open = false;
+    };
+    for (uint32_t v = 0; v < path.verb_count; v++) {
+        switch (verbs[v]) {
+            case RVK_VERB_MOVE:
+                close();
+                move = last = P(pi++);
+                open = true;
+                break;
+            case RVK_VERB_LINE: {
+                Pt p = P(pi++);
+                out.push_back(Seg{2, {last, p}});
+                last = p;
+                break;
+            }
+            case RVK_VERB_QUAD: {
+                Pt a = P(pi++), b = P(pi++);
+                out.push_back(Seg{3, {last, a, b}});
+                last = b;
+                break;
+            }
+            case RVK_VERB_CUBIC: {
+                Pt a = P(pi++), b = P(pi++), c = P(pi++);
+                out.push_back(Seg{4, {last, a, b, c}});
+                last = c;
+                break;
+            }
+            case RVK_VERB_CLOSE:
+                close();
+                last = move;
+                break;
+        }
+    }
+    close();
+}
+
+// ---------------------------------------------------------------------------
+// Edges (SkEdge / tiny-skia edge.rs)
+// ---------------------------------------------------------------------------
+static inline int32_t fdot6_round(int32_t x) { return (x + 32) >> 6; }
+static inline int32_t fdot6_to_fixed(int32_t x) { return (int32_t)((uint32_t)x << 10); }
+static inline int32_t fixed_mul(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)b) >> 16); }
+static inline int32_t fdot6_div(int32_t a, int32_t b) {
+    if ((int32_t)(int16_t)a == a) return (int32_t)((uint32_t)a << 16) / b;
+    int64_t v = ((int64_t)a << 16) / (int64_t)b;
+    return (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, v));
+}
+static inline int32_t compute_dy(int32_t top, int32_t y0) { return ((top << 6) + 32) - y0; }
+static inline int32_t cheap_distance(int32_t dx, int32_t dy) {
+    dx = std::abs(dx);
+    dy = std::abs(dy);
+    return dx > dy ? dx + (dy >> 1) : dy + (dx >> 1);
+}
+static inline int32_t diff_to_shift(int32_t dx, int32_t dy, int32_t shift_aa) {
+    uint32_t dist = (uint32_t)cheap_distance(dx, dy);
+    dist = (dist + (1u << 4)) >> (3 + shift_aa);
+    int lz = dist ? __builtin_clz(dist) : 32;
+    return (32 - lz) >> 1;
+}
+static inline int32_t cubic_delta_from_line(int32_t a, int32_t b, int32_t c, int32_t d) {
+    int32_t one_third = ((a * 8 - b * 15 + 6 * c + d) * 19) >> 9;
+    int32_t two_third = ((a + 6 * b - c * 15 + d * 8) * 19) >> 9;
+    return std::max(std::abs(one_third), std::abs(two_third));
+}
+static const int kMaxCoeffShift = 6;
+
+struct Edge {
+    int type = 0;  // 0 line, 1 quad, 2 cubic
+    int id = 0;
+    // current line piece
+    int32_t x = 0, dx = 0, first_y = 0, last_y = 0;
+    int32_t winding = 1;
+    // curve state
+    int32_t curve_count = 0, curve_shift = 0, dshift = 0;
+    int32_t qx = 0, qy = 0, qdx = 0, qdy = 0, qddx = 0, qddy = 0, qlx = 0, qly = 0;
+    int32_t cdx = 0, cdy = 0, cddx = 0, cddy = 0, cdddx = 0, cdddy = 0;
+    bool alive = true;
+};
+
+static bool line_update(Edge &e, int32_t x0, int32_t y0, int32_t x1, int32_t y1) {  // SkEdge::updateLine
+    y0 >>= 10;
+    y1 >>= 10;
+    int32_t top = fdot6_round(y0), bot = fdot6_round(y1);
+    if (top == bot) return false;
+    x0 >>= 10;
+    x1 >>= 10;
+    int32_t slope = fdot6_div(x1 - x0, y1 - y0);
+    int32_t dy = compute_dy(top, y0);
+    e.x = fdot6_to_fixed(x0 + fixed_mul(slope, dy));
+    e.dx = slope;
+    e.first_y = top;
+    e.last_y = bot - 1;
+    return true;
+}
+
+static bool quad_update(Edge &e) {  // SkQuadraticEdge::updateQuadratic
+    bool ok;
+    int32_t count = e.curve_count, oldx = e.qx, oldy = e.qy, dx = e.qdx, dy = e.qdy, newx, newy;
+    const int32_t shift = e.curve_shift;
+    do {
+        if (--count > 0) {
+            newx = oldx + (dx >> shift);
+            dx += e.qddx;
+            newy = oldy + (dy >> shift);
+            dy += e.qddy;
+        } else {
+            newx = e.qlx;
+            newy = e.qly;
+        }
+        ok = line_update(e, oldx, oldy, newx, newy);
+        oldx = newx;
+        oldy = newy;
+    } while (count > 0 && !ok);
+    e.qx = newx; e.qy = newy; e.qdx = dx; e.qdy = dy; e.curve_count = count;
+    return ok;
+}
+
+static bool cubic_update(Edge &e) {  // SkCubicEdge::updateCubic
+    bool ok;
+    int32_t count = e.curve_count, oldx = e.qx, oldy = e.qy, newx, newy;
+    const int32_t ddshift = e.curve_shift, dshift = e.dshift;
+    do {
+        if (++count < 0) {
+            newx = oldx + (e.cdx >> dshift);
+            e.cdx += e.cddx >> ddshift;
+            e.cddx += e.cdddx;
+            newy = oldy + (e.cdy >> dshift);
+            e.cdy += e.cddy >> ddshift;
+            e.cddy += e.cdddy;
+        } else {
+            newx = e.qlx;
+            newy = e.qly;
+        }
+        if (newy < oldy) newy = oldy;
+        ok = line_update(e, oldx, oldy, newx, newy);
+        oldx = newx;
+        oldy = newy;
+    } while (count < 0 && !ok);
+    e.qx = newx; e.qy = newy; e.curve_count = count;
+    return ok;
+}
+
+static inline int32_t to_fdot6(float v, float scale) { return as_i32(v * scale); }
+
+static bool make_line(Pt p0, Pt p1, int shift, Edge &e) {  // LineEdge::new
+    float scale = (float)(1 << (shift + 6));
+    int32_t x0 = to_fdot6(p0.x, scale), y0 = to_fdot6(p0.y, scale), x1 = to_fdot6(p1.x, scale),
+            y1 = to_fdot6(p1.y, scale);
+    int32_t w = 1;
+    if (y0 > y1) { std::swap(x0, x1); std::swap(y0, y1); w = -1; }
+    int32_t top = fdot6_round(y0), bot = fdot6_round(y1);
+    if (top == bot) return false;
+    int32_t slope = fdot6_div(x1 - x0, y1 - y0);
+    int32_t dy = compute_dy(top, y0);
+    e = Edge{};
+    e.type = 0;
+    e.x = fdot6_to_fixed(x0 + fixed_mul(slope, dy));
+    e.dx = slope;
+    e.first_y = top;
+    e.last_y = bot - 1;
+    e.winding = w;
+    return true;
+}
+
+static bool make_quad(const Pt *p, int shift, Edge &e) {  // QuadraticEdge::new
+    float scale = (float)(1 << (shift + 6));
+    int32_t x0 = to_fdot6(p[0].x, scale), y0 = to_fdot6(p[0].y, scale), x1 = to_fdot6(p[1].x, scale),
+            y1 = to_fdot6(p[1].y, scale), x2 = to_fdot6(p[2].x, scale), y2 = to_fdot6(p[2].y, scale);
+    int32_t w = 1;
+    if (y0 > y2) { std::swap(x0, x2); std::swap(y0, y2); w = -1; }
+    int32_t top = fdot6_round(y0), bot = fdot6_round(y2);
+    if (top == bot) return false;
+    int32_t dxx = ((int32_t)((uint32_t)x1 << 1) - x0 - x2) >> 2;
+    int32_t dyy = ((int32_t)((uint32_t)y1 << 1) - y0 - y2) >> 2;
+    int32_t sh = diff_to_shift(dxx, dyy, shift);
+    if (sh == 0) sh = 1;
+    else if (sh > kMaxCoeffShift) sh = kMaxCoeffShift;
+    e = Edge{};
+    e.type = 1;
+    e.winding = w;
+    e.curve_count = 1 << sh;
+    e.curve_shift = sh - 1;
+    int32_t A = (int32_t)((uint32_t)(x0 - x1 - x1 + x2) << 9);  // fdot6_to_fixed_div2
+    int32_t B = fdot6_to_fixed(x1 - x0);
+    e.qx = fdot6_to_fixed(x0);
+    e.qdx = B + (A >> sh);
+    e.qddx = A >> (sh - 1);
+    A = (int32_t)((uint32_t)(y0 - y1 - y1 + y2) << 9);
+    B = fdot6_to_fixed(y1 - y0);
+    e.qy = fdot6_to_fixed(y0);
+    e.qdy = B + (A >> sh);
+    e.qddy = A >> (sh - 1);
+    e.qlx = fdot6_to_fixed(x2);
+    e.qly = fdot6_to_fixed(y2);
+    return quad_update(e);
+}
+
+static bool make_cubic(const Pt *p, int shift, Edge &e) {  // CubicEdge::new
+    float scale = (float)(1 << (shift + 6));
+    int32_t x0 = to_fdot6(p[0].x, scale), y0 = to_fdot6(p[0].y, scale), x1 = to_fdot6(p[1].x, scale),
+            y1 = to_fdot6(p[1].y, scale), x2 = to_fdot6(p[2].x, scale), y2 = to_fdot6(p[2].y, scale),
+            x3 = to_fdot6(p[3].x, scale), y3 = to_fdot6(p[3].y, scale);
+    int32_t w = 1;
+    if (y0 > y3) {
+        std::swap(x0, x3); std::swap(x1, x2); std::swap(y0, y3); std::swap(y1, y2);
+        w = -1;
+    }
+    int32_t top = fdot6_round(y0), bot = fdot6_round(y3);
+    if (top == bot) return false;
+    int32_t ddx = cubic_delta_from_line(x0, x1, x2, x3), ddy = cubic_delta_from_line(y0, y1, y2, y3);
+    int32_t sh = diff_to_shift(ddx, ddy, shift) + 1;
+    if (sh > kMaxCoeffShift) sh = kMaxCoeffShift;
+    int32_t up = 6, down = sh + up - 10;
+    if (down < 0) { down = 0; up = 10 - sh; }
+    e = Edge{};
+    e.type = 2;
+    e.winding = w;
+    e.curve_count = (int32_t)((uint32_t)-1 << sh);
+    e.curve_shift = sh;
+    e.dshift = down;
+    auto upsh = [&](int32_t v) { return (int32_t)((uint32_t)v << up); };
+    int32_t B = upsh(3 * (x1 - x0)), C = upsh(3 * (x0 - x1 - x1 + x2)), D = upsh(x3 + 3 * (x1 - x2) - x0);
+    e.qx = fdot6_to_fixed(x0);
+    e.cdx = B + (C >> sh) + (D >> (2 * sh));
+    e.cddx = 2 * C + ((3 * D) >> (sh - 1));
+    e.cdddx = (3 * D) >> (sh - 1);
+    B = upsh(3 * (y1 - y0)); C = upsh(3 * (y0 - y1 - y1 + y2)); D = upsh(y3 + 3 * (y1 - y2) - y0);
+    e.qy = fdot6_to_fixed(y0);
+    e.cdy = B + (C >> sh) + (D >> (2 * sh));
+    e.cddy = 2 * C + ((3 * D) >> (sh - 1));
+    e.cdddy = (3 * D) >> (sh - 1);
+    e.qlx = fdot6_to_fixed(x3);
+    e.qly = fdot6_to_fixed(y3);
+    return cubic_update(e);
+}
+
+// ---- chopping at Y extrema (tiny-skia-path path_geometry) ----
+static bool valid_unit_divide(float numer, float denom, float &r) {
+    if (numer < 0.0f) { numer = -numer; denom = -denom; }
+    if (denom == 0.0f || numer == 0.0f || numer >= denom) return false;
+    r = numer / denom;
+    return std::isfinite(r) && r > 0.0f && r < 1.0f;
+}
+static inline Pt interp(Pt a, Pt b, float t) { return Pt{a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t}; }
+
+static int chop_quad_at_y_extrema(const Pt *src, Pt *dst) {
+    float a = src[0].y, b = src[1].y, c = src[2].y;
+    float ab = a - b, bc = b - c;
+    if (ab < 0.0f) bc = -bc;
+    if (ab == 0.0f || bc < 0.0f) {  // not monotonic
+        float t;
+        if (valid_unit_divide(a - b, a - b - b + c, t)) {
+            Pt p01 = interp(src[0], src[1], t), p12 = interp(src[1], src[2], t);
+            dst[0] = src[0]; dst[1] = p01; dst[2] = interp(p01, p12, t); dst[3] = p12; dst[4] = src[2];
+            dst[1].y = dst[2].y;
+            dst[3].y = dst[2].y;
+            return 1;
+        }
+        b = std::fabs(a - b) < std::fabs(b - c) ? a : c;
+    }
+    dst[0] = Pt{src[0].x, a}; dst[1] = Pt{src[1].x, b}; dst[2] = Pt{src[2].x, c};
+    return 0;
+}
+
+static int find_unit_quad_roots(float a, float b, float c, float *roots) {
+    if (a == 0.0f) {
+        float r;
+        if (valid_unit_divide(-c, b, r)) { roots[0] = r; return 1; }
+        return 0;
+    }
+    double dr = (double)b * (double)b - 4.0 * (double)a * (double)c;
+    if (dr < 0.0) return 0;
+    dr = std::sqrt(dr);
+    float r = (float)dr;
+    if (!std::isfinite(r)) return 0;
+    float q = b < 0.0f ? -(b - r) / 2.0f : -(b + r) / 2.0f;
+    int n = 0;
+    float t;
+    if (valid_unit_divide(q, a, t)) roots[n++] = t;
+    if (valid_unit_divide(c, q, t)) roots[n++] = t;
+    if (n == 2) {
+        if (roots[0] > roots[1]) std::swap(roots[0], roots[1]);
+        else if (roots[0] == roots[1]) n = 1;
+    }
+    return n;
+}
+
+static void chop_cubic_at2(const Pt *s, float t, Pt *d) {
+    Pt ab = interp(s[0], s[1], t), bc = interp(s[1], s[2], t), cd = interp(s[2], s[3], t);
+    Pt abc = interp(ab, bc, t), bcd = interp(bc, cd, t), abcd = interp(abc, bcd, t);
+    d[0] = s[0]; d[1] = ab; d[2] = abc; d[3] = abcd; d[4] = bcd; d[5] = cd; d[6] = s[3];
+}
+
+static int chop_cubic_at_y_extrema(const Pt *src, Pt *dst) {
+    float a = src[0].y, b = src[1].y, c = src[2].y, d = src[3].y;
+    float A = d - a + 3.0f * (b - c), B = 2.0f * (a - b - b + c), C = b - a;
+    float ts[3];
+    int n = find_unit_quad_roots(A, B, C, ts);
+    if (n == 0) {
+        for (int i = 0; i < 4; i++) dst[i] = src[i];
+        return 0;
+    }
+    Pt tmp[4];
+    const Pt *s = src;
+    float t = ts[0];
+    int off = 0;
+    for (int i = 0; i < n; i++) {
+        chop_cubic_at2(s, t, dst + off);
+        if (i == n - 1) break;
+        off += 3;
+        for (int k = 0; k < 4; k++) tmp[k] = dst[off + k];
+        s = tmp;
+        float nt;
+        if (valid_unit_divide(ts[i + 1] - ts[i], 1.0f - ts[i], nt)) t = nt;
+        else {
+            dst[off + 4] = s[3]; dst[off + 5] = s[3]; dst[off + 6] = s[3];
+            break;
+        }
+    }
+    dst[2].y = dst[3].y;
+    dst[4].y = dst[3].y;
+    if (n == 2) {
+        dst[5].y = dst[6].y;
+        dst[7].y = dst[6].y;
+    }
+    return n;
+}
+
+// combine_vertical (tiny-skia edge_builder.rs): merge a vertical line edge into the
+// previous vertical edge at the same x.
+enum Combine { kNo, kPartial, kTotal };
+static Combine combine_vertical(const Edge &e, Edge &last) {
+    if (last.type != 0 || last.dx != 0 || e.x != last.x) return kNo;
+    if (e.winding == last.winding) {
+        if (e.last_y + 1 == last.first_y) { last.first_y = e.first_y; return kPartial; }
+        if (e.first_y == last.last_y + 1) { last.last_y = e.last_y; return kPartial; }
+        return kNo;
+    }
+    if (e.first_y == last.first_y) {
+        if (e.last_y == last.last_y) return kTotal;
+        if (e.last_y < last.last_y) { last.first_y = e.last_y + 1; return kPartial; }
+        last.first_y = last.last_y + 1;
+        last.last_y = e.last_y;
+        last.winding = e.winding;
+        return kPartial;
+    }
+    if (e.last_y == last.last_y) {
+        if (e.first_y > last.first_y) { last.last_y = e.first_y - 1; return kPartial; }
+        last.last_y = last.first_y - 1;
+        last.first_y = e.first_y;
+        last.winding = e.winding;
+        return kPartial;
+    }
+    return kNo;
+}
+
+static void build_edges(const std::vector<Seg> &segs, int shift, std::vector<Edge> &edges) {
+    auto push_line = [&](Pt a, Pt b) {
+        Edge e;
+        if (!make_line(a, b, shift, e)) return;
+        if (e.dx == 0 && !edges.empty()) {
+            Combine c = combine_vertical(e, edges.back());
+            if (c == kTotal) { edges.pop_back(); return; }
+            if (c == kPartial) return;
+        }
+        edges.push_back(e);
+    };
+    for (const Seg &s : segs) {
+        if (s.n == 2) push_line(s.p[0], s.p[1]);
+        else if (s.n == 3) {
+            Pt mono[5];
+            int n = chop_quad_at_y_extrema(s.p, mono);
+            for (int i = 0; i <= n; i++) {
+                Edge e;
+                if (make_quad(mono + 2 * i, shift, e)) edges.push_back(e);
+            }
+        } else {
+            Pt mono[10];
+            int n = chop_cubic_at_y_extrema(s.p, mono);
+            for (int i = 0; i <= n; i++) {
+                Edge e;
+                if (make_cubic(mono + 3 * i, shift, e)) edges.push_back(e);
+            }
+        }
+    }
+    for (size_t i = 0; i < edges.size(); i++) edges[i].id = (int)i;
+}
+
+// Per-pixel coverage of `segs` (already in device space) inside a w x h pixmap:
+// the SuperBlitter accumulation of tiny-skia's scan/path_aa.rs for anti-aliased
+// fills, or full-pixel spans for non-AA fills.
+static void coverage(const std::vector<Seg> &segs, uint32_t rule, bool aa, uint32_t w, uint32_t h,
+                     std::vector<uint8_t> &cov) {
+    cov.assign((size_t)w * h, 0);
+    const int shift = aa ? 2 : 0;
+    std::vector<Edge> edges;
+    build_edges(segs, shift, edges);
+    if (edges.empty()) return;
+    // Curve edges expose only their current piece, so the walk runs until every
+    // edge has been used up (or the pixmap ends), not to the first pieces' last row.
+    int32_t ymin = INT32_MAX;
+    for (auto &e : edges) ymin = std::min(ymin, e.first_y);
+    const int32_t scale = 1 << shift;
+    const int32_t y0 = std::max(ymin, 0), y1 = (int32_t)h * scale - 1;
+    const int32_t wmax = (int32_t)w * scale;
+    std::vector<uint32_t> acc(w);
+    struct X { int32_t x, w, id; };
+    std::vector<X> xs;
+    int32_t cur_row = -1;
+    auto flush = [&]() {
+        if (cur_row < 0) return;
+        for (uint32_t i = 0; i < w; i++) cov[(size_t)cur_row * w + i] = (uint8_t)std::min<uint32_t>(255, acc[i]);
+        std::fill(acc.begin(), acc.end(), 0);
+    };
+    // advance curve edges to the piece containing y (they are walked in increasing y)
+    for (int32_t y = ymin; y <= y1; y++) {
+        xs.clear();
+        size_t alive = 0;
+        for (auto &e : edges) {
+            if (!e.alive) continue;
+            while (e.alive && e.last_y < y) {
+                bool ok = false;
+                if (e.type == 1 && e.curve_count > 0) ok = quad_update(e);
+                else if (e.type == 2 && e.curve_count < 0) ok = cubic_update(e);
+                if (!ok) e.alive = false;
+            }
+            if (!e.alive) continue;
+            alive++;
+            if (e.first_y > y) continue;
+            // walk_edges steps x by dx once per row with wrapping i32 adds
+            int32_t x = (int32_t)((uint32_t)e.x + (uint32_t)(y - e.first_y) * (uint32_t)e.dx);
+            xs.push_back(X{x, e.winding, e.id});
+        }
+        if (alive == 0) break;
+        if (y < y0) continue;
+        std::sort(xs.begin(), xs.end(), [](const X &a, const X &b) { return a.x != b.x ? a.x < b.x : a.id < b.id; });
+        int32_t row = y >> shift;
+        if (row != cur_row) { flush(); cur_row = row; }
+        const int32_t mask = rule == RVK_FILL_EVENODD ? 1 : -1;
+        int32_t wsum = 0, left = 0;
+        const uint32_t max_value = (1u << (8 - shift)) - ((((uint32_t)y & (uint32_t)(scale - 1)) + 1) >> shift);
+        for (const X &c : xs) {
+            int32_t xr = (c.x + 0x8000) >> 16;
+            if ((wsum & mask) == 0) left = xr;
+            wsum += c.w;
+            if ((wsum & mask) == 0 && xr > left) {
+                int32_t L = std::max(left, 0), R = std::min(xr, wmax);
+                if (R <= L) continue;
+                if (!aa) {
+                    for (int32_t px = L; px < R; px++) acc[px] = 255;
+                    continue;
+                }
+                // SuperBlitter::blit_h
+                int32_t fb = L & 3, fe = R & 3, n = (R >> 2) - (L >> 2) - 1;
+                if (n < 0) {
+                    acc[L >> 2] += (uint32_t)(fe - fb) * 16;
+                } else {
+                    int32_t start = L >> 2;
+                    if (fb == 0) n += 1;
+                    else { acc[start] += (uint32_t)(4 - fb) * 16; start += 1; }
+                    for (int32_t k = 0; k < n; k++) acc[start + k] += max_value;
+                    if (fe) acc[R >> 2] += (uint32_t)fe * 16;
+                }
+            }
+        }
+    }
+    flush();
+}
+
+// ---------------------------------------------------------------------------
+// Stroker (this build's polyline outline; see the file comment)
+// ---------------------------------------------------------------------------
+struct Poly { std::vector<Pt> pts; };
+
+static float stroke_res_scale(Ts ts) {  // tiny-skia compute_res_scale_for_stroking
+    float sx = std::sqrt(ts.sx * ts.sx + ts.kx * ts.kx), sy = std::sqrt(ts.ky * ts.ky + ts.sy * ts.sy);
+    if (std::isfinite(sx) && std::isfinite(sy)) {
+        float s = std::max(sx, sy);
+        if (s > 0.0f) return s;
+    }
+    return 1.0f;
+}
+
+// Number of uniform steps flattening a curve to within `tol` user units.
+static int flatten_steps(const Pt *p, int n, float tol) {
+    float dd = 0.0f;
+    for (int i = 0; i + 2 < n; i++) {
+        float dx = p[i].x - 2.0f * p[i + 1].x + p[i + 2].x, dy = p[i].y - 2.0f * p[i + 1].y + p[i + 2].y;
+        dd = std::max(dd, std::sqrt(dx * dx + dy * dy));
+    }
+    float k = n == 4 ? 0.75f : 0.25f;
+    int steps = (int)std::ceil(std::sqrt(k * dd / tol));
+    return std::max(1, std::min(steps, 256));
+}
+
+static Pt eval_curve(const Pt *p, int n, float t) {
+    float u = 1.0f - t;
+    if (n == 3) {
+        float a = u * u, b = 2.0f * u * t, c = t * t;
+        return Pt{a * p[0].x + b * p[1].x + c * p[2].x, a * p[0].y + b * p[1].y + c * p[2].y};
+    }
+    float a = u * u * u, b = 3.0f * u * u * t, c = 3.0f * u * t * t, d = t * t * t;
+    return Pt{a * p[0].x + b * p[1].x + c * p[2].x + d * p[3].x, a * p[0].y + b * p[1].y + c * p[2].y + d * p[3].y};
+}
+
+struct Contour { std::vector<Pt> pts; bool closed; };
+
+static void stroke_contours(const rvk_scene &sc, const rvk_path &path, float tol, std::vector<Contour> &out) {
+    const uint8_t *verbs = sc.verbs + path.verb_offset;
+    const float *pts = sc.points + 2 * (size_t)path.point_offset;
+    size_t pi = 0;
+    Contour cur{{}, false};
+    auto P = [&](size_t i) { return Pt{pts[2 * i], pts[2 * i + 1]}; };
+    auto add = [&](Pt p) {
+        if (cur.pts.empty() || cur.pts.back().x != p.x || cur.pts.back().y != p.y) cur.pts.push_back(p);
+    };
+    auto finish = [&]() {
+        if (cur.pts.size() >= 2) out.push_back(cur);
+        cur = Contour{{}, false};
+    };
+    for (uint32_t v = 0; v < path.verb_count; v++) {
+        switch (verbs[v]) {
+            case RVK_VERB_MOVE: finish(); add(P(pi++)); break;
+            case RVK_VERB_LINE: add(P(pi++)); break;
+            case RVK_VERB_QUAD:
+            case RVK_VERB_CUBIC: {
+                int n = verbs[v] == RVK_VERB_QUAD ? 3 : 4;
+                Pt c[4];
+                c[0] = cur.pts.empty() ? Pt{0, 0} : cur.pts.back();
+                for (int k = 1; k < n; k++) c[k] = P(pi++);
+                int steps = flatten_steps(c, n, tol);
+                for (int s = 1; s < steps; s++) add(eval_curve(c, n, (float)s / (float)steps));
+                add(c[n - 1]);
+                break;
+            }
+            case RVK_VERB_CLOSE:
+                if (!cur.pts.empty() && cur.pts.size() >= 2 &&
+                    (cur.pts.front().x == cur.pts.back().x && cur.pts.front().y == cur.pts.back().y))
+                    cur.pts.pop_back();
+                cur.closed = true;
+                {
+                    Pt start = cur.pts.empty() ? Pt{0, 0} : cur.pts.front();
+                    finish();
+                    add(start);  // a following segment without MoveTo starts here
+                }
+                break;
+        }
+    }
+    finish();
+}
+
+static float signed_area(const std::vector<Pt> &p) {
+    double a = 0.0;
+    for (size_t i = 0; i < p.size(); i++) {
+        const Pt &u = p[i], &v = p[(i + 1) % p.size()];
+        a += (double)u.x * v.y - (double)v.x * u.y;
+    }
+    return (float)a;
+}
+
+static void emit_piece(std::vector<Pt> p, std::vector<Poly> &out) {
+    if (p.size() < 3) return;
+    if (signed_area(p) > 0.0f) std::reverse(p.begin(), p.end());
+    out.push_back(Poly{std::move(p)});
+}
+
+static void arc_points(Pt c, float hw, float a0, float a1, float tol, std::vector<Pt> &out) {
+    // arc from angle a0 to a1 (radians, |a1-a0| <= pi), chord error <= tol
+    float span = a1 - a0;
+    float step = 2.0f * std::acos(std::max(-1.0f, std::min(1.0f, 1.0f - tol / std::max(hw, tol))));
+    int n = std::max(1, (int)std::ceil(std::fabs(span) / std::max(step, 1e-3f)));
+    n = std::min(n, 64);
+    for (int i = 0; i <= n; i++) {
+        float a = a0 + span * ((float)i / (float)n);
+        out.push_back(Pt{c.x + hw * std::cos(a), c.y + hw * std::sin(a)});
+    }
+}
+
+static void stroke_outline(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Poly> &out) {
+    const float hw = path.stroke_width * 0.5f;
+    if (!(hw > 0.0f)) return;
+    const float res = stroke_res_scale(ts);
+    const float tol = 0.25f / res;
+    std::vector<Contour> contours;
+    stroke_contours(sc, path, tol, contours);
+    for (const Contour &c : contours) {
+        const std::vector<Pt> &p = c.pts;
+        size_t n = p.size();
+        size_t nseg = c.closed ? n : n - 1;
+        if (nseg == 0) continue;
+        auto dir = [&](size_t i) {
+            Pt a = p[i], b = p[(i + 1) % n];
+            float dx = b.x - a.x, dy = b.y - a.y, l = std::sqrt(dx * dx + dy * dy);
+            return Pt{dx / l, dy / l};
+        };
+        for (size_t i = 0; i < nseg; i++) {
+            Pt a = p[i], b = p[(i + 1) % n], d = dir(i);
+            Pt nm{-d.y * hw, d.x * hw};
+            emit_piece({Pt{a.x + nm.x, a.y + nm.y}, Pt{b.x + nm.x, b.y + nm.y}, Pt{b.x - nm.x, b.y - nm.y},
+                        Pt{a.x - nm.x, a.y - nm.y}}, out);
+        }
+        // joins at interior vertices (and all vertices of a closed contour)
+        size_t first = c.closed ? 0 : 1, last = c.closed ? n : n - 1;
+        for (size_t v = first; v < last; v++) {
+            size_t i0 = (v + nseg - 1) % nseg, i1 = v % nseg;
+            if (!c.closed) { i0 = v - 1; i1 = v; }
+            Pt d0 = dir(i0), d1 = dir(i1), q = p[v];
+            float cross = d0.x * d1.y - d0.y * d1.x, dot = d0.x * d1.x + d0.y * d1.y;
+            if (std::fabs(cross) < 1e-6f && dot > 0.0f) continue;
+            // outer side: the side opposite to the turn
+            float s = cross > 0.0f ? -1.0f : 1.0f;
+            Pt n0{-d0.y * hw * s, d0.x * hw * s}, n1{-d1.y * hw * s, d1.x * hw * s};
+            Pt e0{q.x + n0.x, q.y + n0.y}, e1{q.x + n1.x, q.y + n1.y};
+            if (path.stroke_join == RVK_JOIN_ROUND) {
+                float a0 = std::atan2(n0.y, n0.x), a1 = std::atan2(n1.y, n1.x);
+                float da = a1 - a0;
+                while (da > 3.14159265f) da -= 6.2831853f;
+                while (da < -3.14159265f) da += 6.2831853f;
+                std::vector<Pt> fan{q};
+                arc_points(q, hw, a0, a0 + da, tol, fan);
+                emit_piece(fan, out);
+            } else {
+                bool miter = path.stroke_join == RVK_JOIN_MITER || path.stroke_join == RVK_JOIN_MITER_CLIP;
+                // 1/sin(theta/2) = sqrt(2 / (1 - cos(theta))) with theta the angle between the outer normals
+                float cos_t = n0.x * n1.x / (hw * hw) + n0.y * n1.y / (hw * hw);
+                float ratio2 = 2.0f / std::max(1e-12f, 1.0f + cos_t);
+                if (miter && ratio2 <= path.stroke_miter_limit * path.stroke_miter_limit) {
+                    Pt m{n0.x + n1.x, n0.y + n1.y};
+                    float ml = std::sqrt(m.x * m.x + m.y * m.y);
+                    float len = hw * std::sqrt(ratio2);
+                    Pt tip{q.x + m.x / ml * len, q.y + m.y / ml * len};
+                    emit_piece({q, e0, tip, e1}, out);
+                } else {
+                    emit_piece({q, e0, e1}, out);
+                }
+            }
+        }
+        if (!c.closed) {
+            for (int end = 0; end < 2; end++) {
+                Pt q = end == 0 ? p[0] : p[n - 1];
+                Pt d = end == 0 ? dir(0) : dir(n - 2);
+                if (end == 0) d = Pt{-d.x, -d.y};  // outward direction
+                Pt nm{-d.y * hw, d.x * hw};
+                if (path.stroke_cap == RVK_CAP_SQUARE) {
+                    Pt f{d.x * hw, d.y * hw};
+                    emit_piece({Pt{q.x + nm.x, q.y + nm.y}, Pt{q.x + nm.x + f.x, q.y + nm.y + f.y},
+                                Pt{q.x - nm.x + f.x, q.y - nm.y + f.y}, Pt{q.x - nm.x, q.y - nm.y}}, out);
+                } else if (path.stroke_cap == RVK_CAP_ROUND) {
+                    float a0 = std::atan2(nm.y, nm.x);
+                    std::vector<Pt> fan{q};
+                    arc_points(q, hw, a0, a0 - 3.14159265f, tol, fan);
+                    emit_piece(fan, out);
+                }
+            }
+        }
+    }
+    // map to device space
+    for (auto &poly : out)
+        for (auto &pt : poly.pts) ts_map(ts, pt.x, pt.y);
+}
+
+// ---------------------------------------------------------------------------
+// Shading (tiny-skia lowp): solid colours and linear / concentric radial gradients
+// ---------------------------------------------------------------------------
+static inline uint32_t div255(uint32_t v) { return (v + 255) >> 8; }
+
+struct Shader {
+    bool valid = false;
+    bool solid = true;
+    uint32_t rgba[4] = {0, 0, 0, 0};
+    Ts ts{};  // device -> unit space
+    bool radial = false;
+    float r_scale = 1.0f, r_bias = 0.0f;
+    bool r_twist = false;
+    uint32_t spread = RVK_SPREAD_PAD;
+    bool pad_clamp = false;
+    bool two_stop = false;
+    float factor[4]{}, bias[4]{};
+    std::vector<float> t;
+    std::vector<std::array<float, 4>> f, b;
+    bool opaque = true;
+    // Shader::is_opaque (solid colour with alpha 1, linear gradient with opaque stops;
+    // never a radial gradient): RasterPipelineBlitter::new then reduces SourceOver to
+    // Source, whose partial coverage is lerp_1_float instead of scale_1_float + over.
+    bool as_source = false;
+};
+
+// Transform::invert (tiny-skia-path transform.rs)
+static bool ts_invert(Ts t, Ts &out) {
+    if (ts_is_identity(t)) { out = t; return true; }
+    if (!ts_has_skew(t)) {
+        if (ts_has_scale(t)) {
+            float ix = 1.0f / t.sx, iy = 1.0f / t.sy;
+            out = Ts{ix, 0.0f, 0.0f, iy, -t.tx * ix, -t.ty * iy};
+        } else {
+            out = Ts{1.0f, 0.0f, 0.0f, 1.0f, -t.tx, -t.ty};
+        }
+        return std::isfinite(out.sx) && std::isfinite(out.sy) && std::isfinite(out.tx) && std::isfinite(out.ty);
+    }
+    double det = (double)t.sx * (double)t.sy - (double)t.kx * (double)t.ky;
+    const float nz = 1.0f / 4096.0f;
+    if (std::fabs((float)det) <= nz * nz * nz) return false;
+    double inv = 1.0 / det;
+    out.sx = (float)((double)t.sy * inv);
+    out.ky = (float)((double)-t.ky * inv);
+    out.kx = (float)((double)-t.kx * inv);
+    out.sy = (float)((double)t.sx * inv);
+    out.tx = (float)(((double)t.kx * (double)t.ty - (double)t.sy * (double)t.tx) * inv);
+    out.ty = (float)(((double)t.ky * (double)t.tx - (double)t.sx * (double)t.ty) * inv);
+    return std::isfinite(out.sx) && std::isfinite(out.ky) && std::isfinite(out.kx) && std::isfinite(out.sy) &&
+           std::isfinite(out.tx) && std::isfinite(out.ty);
+}
+
+// A solid colour shader (Shader::SolidColor): premultiplied lowp colour.
+static void set_solid(Shader &s, const float c[4]) {
+    uint32_t *out = s.rgba;
+    s.as_source = c[3] == 1.0f;
+    float r = c[0], g = c[1], b = c[2], a = c[3];
+    if (a != 1.0f) { r *= a; g *= a; b *= a; }
+    out[0] = (uint32_t)(r * 255.0f + 0.5f);
+    out[1] = (uint32_t)(g * 255.0f + 0.5f);
+    out[2] = (uint32_t)(b * 255.0f + 0.5f);
+    out[3] = (uint32_t)(a * 255.0f + 0.5f);
+}
+
+static Shader make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts) {
+    Shader s;
+    if (p.kind == RVK_PAINT_COLOR) {
+        float c[4] = {(float)p.color[0] / 255.0f, (float)p.color[1] / 255.0f, (float)p.color[2] / 255.0f,
+                      (float)as_u8(std::min(std::max(p.opacity, 0.0f), 1.0f) * 255.0f + 0.5f) / 255.0f};
+        set_solid(s, c);
+        s.valid = true;
+        return s;
+    }
+    if (p.kind != RVK_PAINT_LINEAR && p.kind != RVK_PAINT_RADIAL) return s;
+    // convert_base_gradient (path.rs:154-177)
+    struct GS { float pos; float c[4]; };
+    std::vector<GS> stops;
+    float fo = std::min(std::max(p.opacity, 0.0f), 1.0f);
+    for (uint32_t i = 0; i < p.stop_count; i++) {
+        const rvk_stop &st = sc.stops[p.stop_offset + i];
+        float a = std::min(std::max(st.opacity, 0.0f), 1.0f) * fo;
+        uint8_t a8 = as_u8(std::min(std::max(a, 0.0f), 1.0f) * 255.0f + 0.5f);
+        stops.push_back(GS{std::min(std::max(st.offset, 0.0f), 1.0f),
+                           {(float)st.color[0] / 255.0f, (float)st.color[1] / 255.0f, (float)st.color[2] / 255.0f,
+                            (float)a8 / 255.0f}});
+    }
+    if (stops.empty()) return s;
+    if (stops.size() == 1) {
+        set_solid(s, stops[0].c);
+        s.valid = true;
+        return s;
+    }
+    Ts unit;
+    if (p.kind == RVK_PAINT_LINEAR) {
+        float dx = p.x2 - p.x1, dy = p.y2 - p.y1, len = std::sqrt(dx * dx + dy * dy);
+        if (!std::isfinite(len)) return s;
+        if (len <= 1.0f / 32768.0f) {  // degenerate: last colour (pad) / average (repeat, reflect)
+            if (p.spread == RVK_SPREAD_PAD) set_solid(s, stops.back().c);
+            else {
+                float avg[4] = {0, 0, 0, 0};
+                for (size_t i = 0; i + 1 < stops.size(); i++) {
+                    float wgt = stops[i + 1].pos - stops[i].pos;
+                    for (int k = 0; k < 4; k++) avg[k] += wgt * (stops[i].c[k] + stops[i + 1].c[k]) * 0.5f;
+                }
+                set_solid(s, avg);
+            }
+            s.valid = true;
+            return s;
+        }
+        // points_to_unit_ts: rotate about p1 so that p1->p2 maps onto +x, translate
+        // p1 to the origin, scale by 1/len (ts_from_sin_cos_at, post_translate, post_scale)
+        float inv = 1.0f / len;
+        float vx = dx * inv, vy = dy * inv;
+        float sn = -vy, cs = vx, cinv = 1.0f - cs;
+        Ts rot{cs, sn, -sn, cs, sn * p.y1 + cinv * p.x1, -sn * p.x1 + cinv * p.y1};
+        Ts tr{1, 0, 0, 1, -p.x1, -p.y1};
+        Ts scl{inv, 0, 0, inv, 0, 0};
+        unit = ts_concat(scl, ts_concat(tr, rot));
+    } else {
+        float fdx = p.fx - p.cx, fdy = p.fy - p.cy;
+        if (!(std::sqrt(fdx * fdx + fdy * fdy) <= 1.0f / 4096.0f)) return s;  // focal gradients are outside this build
+        float r0 = p.fr, r1 = p.r;
+        // equal radii are not rejected: the twist below then divides by zero and every
+        // t becomes NaN, which the clamps map to 0 (the first stop), as tiny-skia draws
+        if (!(r1 >= 0.0f) || !(std::max(r0, r1) > 0.0f)) return s;
+        float scale = 1.0f / std::max(r0, r1);
+        unit = ts_concat(Ts{scale, 0, 0, scale, 0, 0}, Ts{1, 0, 0, 1, -p.cx, -p.cy});
+        s.radial = true;
+        if (r0 != 0.0f) {
+            // focal radius: t = (t - r0/m) / ((r1 - r0)/m), m = max(r0, r1); pinned by the
+            // reference fixtures radialGradient/fr=0.2, fr=0.5 and fr=0.7
+            const float m = std::max(r0, r1);
+            s.r_twist = true;
+            s.r_scale = r0 / m;
+            s.r_bias = (r1 - r0) / m;
+        }
+    }
+    // Gradient::new: dummy end stops, monotonic positions, uniform-step test
+    bool dummy_first = stops.front().pos != 0.0f, dummy_last = stops.back().pos != 1.0f;
+    if (dummy_first) stops.insert(stops.begin(), GS{0.0f, {stops.front().c[0], stops.front().c[1], stops.front().c[2], stops.front().c[3]}});
+    if (dummy_last) stops.push_back(GS{1.0f, {stops.back().c[0], stops.back().c[1], stops.back().c[2], stops.back().c[3]}});
+    s.opaque = true;
+    for (auto &g : stops) s.opaque = s.opaque && g.c[3] == 1.0f;
+    s.as_source = s.opaque && !s.radial;
+    size_t start = dummy_first ? 0 : 1;
+    float prev = 0.0f;
+    bool uniform = true;
+    float step = stops[start].pos - prev;
+    for (size_t i = start; i < stops.size(); i++) {
+        float curr = i + 1 == stops.size() ? 1.0f : std::max(prev, std::min(stops[i].pos, 1.0f));
+        uniform = uniform && std::fabs(step - (curr - prev)) <= 1.0f / 4096.0f;
+        stops[i].pos = std::min(std::max(curr, 0.0f), 1.0f);
+        prev = curr;
+    }
+    // total device -> unit transform: (ts * paint.transform)^-1 then unit
+    Ts total = ts_concat(ts, ts_from(p.transform));
+    Ts inv;
+    if (!ts_invert(total, inv)) return s;
+    s.ts = ts_concat(unit, inv);
+    s.spread = p.spread;
+    s.pad_clamp = p.spread == RVK_SPREAD_PAD && uniform;
+    s.solid = false;
+    if (stops.size() == 2 && uniform) {
+        s.two_stop = true;
+        for (int k = 0; k < 4; k++) { s.factor[k] = stops[1].c[k] - stops[0].c[k]; s.bias[k] = stops[0].c[k]; }
+    } else {
+        size_t first = 0, last = stops.size() - 1;
+        auto same = [](const GS &a, const GS &b) {
+            return a.c[0] == b.c[0] && a.c[1] == b.c[1] && a.c[2] == b.c[2] && a.c[3] == b.c[3];
+        };
+        if (stops.size() > 2) {
+            if (same(stops[0], stops[1])) first = 1;
+            if (same(stops[stops.size() - 2], stops[stops.size() - 1])) last = stops.size() - 2;
+        }
+        float tl = stops[first].pos;
+        const float *cl = stops[first].c;
+        s.t.push_back(0.0f);
+        s.f.push_back({0, 0, 0, 0});
+        s.b.push_back({cl[0], cl[1], cl[2], cl[3]});
+        for (size_t i = first; i < last; i++) {
+            float tr = stops[i + 1].pos;
+            const float *cr = stops[i + 1].c;
+            if (tl < tr) {
+                std::array<float, 4> f, b;
+                for (int k = 0; k < 4; k++) {
+                    f[k] = (cr[k] - cl[k]) / (tr - tl);
+                    b[k] = cl[k] - f[k] * tl;
+                }
+                s.t.push_back(tl);
+                s.f.push_back(f);
+                s.b.push_back(b);
+            }
+            tl = tr;
+            cl = cr;
+        }
+        s.t.push_back(tl);
+        s.f.push_back({0, 0, 0, 0});
+        s.b.push_back({cl[0], cl[1], cl[2], cl[3]});
+    }
+    s.valid = true;
+    return s;
+}
+
+// SIMD max(v, 0) / min(v, 1) semantics: NaN -> 0
+static inline float clamp01(float v) {
+    v = v > 0.0f ? v : 0.0f;
+    return v < 1.0f ? v : 1.0f;
+}
+
+static void shade(const Shader &s, uint32_t px, uint32_t py, uint32_t out[4]) {
+    if (s.solid) { for (int k = 0; k < 4; k++) out[k] = s.rgba[k]; return; }
+    float x = (float)px + 0.5f, y = (float)py + 0.5f;
+    float tx = x * s.ts.sx + (y * s.ts.kx + s.ts.tx);
+    float ty = x * s.ts.ky + (y * s.ts.sy + s.ts.ty);
+    float t = tx;
+    if (s.radial) {
+        t = std::sqrt(tx * tx + ty * ty);
+        if (s.r_twist) t = (t - s.r_scale) / s.r_bias;
+    }
+    if (s.spread == RVK_SPREAD_REFLECT) {
+        float u = t - 1.0f;
+        t = clamp01(std::fabs(u - 2.0f * std::floor(u * 0.5f) - 1.0f));
+    } else if (s.spread == RVK_SPREAD_REPEAT) {
+        t = clamp01(t - std::floor(t));
+    } else if (s.pad_clamp) {
+        t = clamp01(t);
+    }
+    float c[4];
+    if (s.two_stop) {
+        for (int k = 0; k < 4; k++) c[k] = t * s.factor[k] + s.bias[k];
+    } else {
+        size_t idx = 0;
+        for (size_t i = 1; i < s.t.size(); i++) idx += t >= s.t[i] ? 1 : 0;
+        for (int k = 0; k < 4; k++) c[k] = t * s.f[idx][k] + s.b[idx][k];
+    }
+    for (int k = 0; k < 4; k++) {
+        float v = c[k];
+        v = (v != v) ? 0.0f : clamp01(v);
+        out[k] = (uint32_t)(v * 255.0f + 0.5f);
+    }
+    if (!s.opaque) for (int k = 0; k < 3; k++) out[k] = div255(out[k] * out[3]);
+}
+
+// Composite `cov` x shader onto px: lowp SourceOver (scale_1_float, source_over), or
+// for an opaque shader Source (full coverage stores the colour, partial coverage is
+// lerp_1_float: div255(d * (255 - c) + s * c)).
+static void blit(const Shader &s, const std::vector<uint8_t> &cov, uint8_t *px, uint32_t w, uint32_t h) {
+    for (uint32_t y = 0; y < h; y++) {
+        for (uint32_t x = 0; x < w; x++) {
+            uint32_t c = cov[(size_t)y * w + x];
+            if (!c) continue;
+            uint32_t src[4];
+            shade(s, x, y, src);
+            uint8_t *d = px + 4 * ((size_t)y * w + x);
+            if (s.as_source) {
+                for (int k = 0; k < 4; k++)
+                    d[k] = (uint8_t)(c == 255 ? src[k] : div255(d[k] * (255 - c) + src[k] * c));
+                continue;
+            }
+            if (c != 255) for (int k = 0; k < 4; k++) src[k] = div255(src[k] * c);
+            uint32_t ia = 255 - src[3];
+            for (int k = 0; k < 4; k++) d[k] = (uint8_t)(src[k] + div255(d[k] * ia));
+        }
+    }
+}
+
+static bool path_bounds_nearly_zero(const std::vector<Seg> &segs) {
+    float mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+    for (auto &s : segs)
+        for (int i = 0; i < s.n; i++) {
+            mnx = std::min(mnx, s.p[i].x); mxx = std::max(mxx, s.p[i].x);
+            mny = std::min(mny, s.p[i].y); mxy = std::max(mxy, s.p[i].y);
+        }
+    const float nz = 1.0f / 4096.0f;
+    return !(mxx - mnx > nz) || !(mxy - mny > nz);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// entry points
+// ---------------------------------------------------------------------------
+namespace orc {
+
+// resvg path::fill_path (path.rs:26-75) + tiny-skia fill_path; returns false when nothing is drawn.
+bool fill_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h) {
+    if (path.fill.kind == RVK_PAINT_NONE) return false;
+    if (path.bounds.w == 0.0f || path.bounds.h == 0.0f) return false;
+    Shader s = make_shader(sc, path.fill, ts);
+    if (!s.valid) return false;
+    std::vector<Seg> segs;
+    fill_segments(sc, path, ts, segs);
+    if (segs.empty() || path_bounds_nearly_zero(segs)) return false;
+    std::vector<uint8_t> cov;
+    coverage(segs, path.fill_rule, path.anti_alias != 0, w, h, cov);
+    blit(s, cov, px, w, h);
+    return true;
+}
+
+// resvg path::stroke_path (path.rs:77-116) with this build's stroker
+bool stroke_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h) {
+    if (path.stroke.kind == RVK_PAINT_NONE) return false;
+    Shader s = make_shader(sc, path.stroke, ts);
+    if (!s.valid) return false;
+    std::vector<Poly> polys;
+    stroke_outline(sc, path, ts, polys);
+    std::vector<Seg> segs;
+    for (auto &p : polys)
+        for (size_t i = 0; i < p.pts.size(); i++)
+            segs.push_back(Seg{2, {p.pts[i], p.pts[(i + 1) % p.pts.size()]}});
+    if (segs.empty()) return false;
+    std::vector<uint8_t> cov;
+    coverage(segs, RVK_FILL_NONZERO, path.anti_alias != 0, w, h, cov);
+    blit(s, cov, px, w, h);
+    return true;
+}
+
+void path_coverage_of(const rvk_scene &sc, const rvk_path &path, Ts ts, bool stroke, uint8_t *cov_out, uint32_t w,
+                      uint32_t h) {
+    std::vector<Seg> segs;
+    uint32_t rule = path.fill_rule;
+    if (stroke) {
+        std::vector<Poly> polys;
+        stroke_outline(sc, path, ts, polys);
+        for (auto &p : polys)
+            for (size_t i = 0; i < p.pts.size(); i++)
+                segs.push_back(Seg{2, {p.pts[i], p.pts[(i + 1) % p.pts.size()]}});
+        rule = RVK_FILL_NONZERO;
+    } else {
+        fill_segments(sc, path, ts, segs);
+    }
+    std::vector<uint8_t> cov;
+    coverage(segs, rule, path.anti_alias != 0, w, h, cov);
+    std::memcpy(cov_out, cov.data(), cov.size());
+}
+
+}  // namespace orc
+
+int32_t orc_path_coverage(const rvk_scene *scene, uint32_t path_index, rvk_transform ts, uint32_t stroke,
+                          uint8_t *cov, uint32_t w, uint32_t h) {
+    orc::path_coverage_of(*scene, scene->paths[path_index], orc::ts_from(ts), stroke != 0, cov, w, h);
+    return 0;
+}
+
+int32_t orc_fill_path(const rvk_scene *scene, uint32_t path_index, rvk_transform ts, uint8_t *px, uint32_t w,
+                      uint32_t h) {
+    return orc::fill_path_into(*scene, scene->paths[path_index], orc::ts_from(ts), px, w, h) ? 1 : 0;
+}

@@ -1,0 +1,119 @@
+// Oracle restatement of resvg's render walk (crates/resvg/src/lib.rs:34-43,
+// render.rs:10-143, path.rs:6-24) over the flattened scene.  Test infrastructure only.
+#include "oracle.h"
+#include "common.h"
+#include "geom.h"
+#include <vector>
+
+using namespace orc;
+
+namespace orc {
+bool fill_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h);
+bool stroke_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h);
+}
+
+namespace {
+
+struct Ctx { IRect max_bbox; };
+
+// convert_blend_mode (render.rs:254-273)
+uint32_t convert_blend_mode(uint32_t m) {
+    static const uint32_t map[RVK_MIX_COUNT] = {
+        RVK_BLEND_SOURCE_OVER, RVK_BLEND_MULTIPLY, RVK_BLEND_SCREEN, RVK_BLEND_OVERLAY, RVK_BLEND_DARKEN,
+        RVK_BLEND_LIGHTEN, RVK_BLEND_COLOR_DODGE, RVK_BLEND_COLOR_BURN, RVK_BLEND_HARD_LIGHT, RVK_BLEND_SOFT_LIGHT,
+        RVK_BLEND_DIFFERENCE, RVK_BLEND_EXCLUSION, RVK_BLEND_HUE, RVK_BLEND_SATURATION, RVK_BLEND_COLOR,
+        RVK_BLEND_LUMINOSITY};
+    return m < RVK_MIX_COUNT ? map[m] : RVK_BLEND_SOURCE_OVER;
+}
+
+void render_nodes(const rvk_scene &sc, const rvk_group &g, const Ctx &ctx, Ts ts, uint8_t *px, uint32_t w,
+                  uint32_t h);
+
+// path::render (path.rs:6-24)
+void render_path(const rvk_scene &sc, const rvk_path &p, Ts ts, uint8_t *px, uint32_t w, uint32_t h) {
+    if (!p.visible) return;
+    if (!p.paint_order_stroke_first) {
+        fill_path_into(sc, p, ts, px, w, h);
+        stroke_path_into(sc, p, ts, px, w, h);
+    } else {
+        stroke_path_into(sc, p, ts, px, w, h);
+        fill_path_into(sc, p, ts, px, w, h);
+    }
+}
+
+bool int_rect_from_xywh(int64_t x, int64_t y, int64_t w, int64_t h, IRect &out) {
+    if (w <= 0 || h <= 0 || x < INT32_MIN || y < INT32_MIN || x + w > INT32_MAX || y + h > INT32_MAX ||
+        w > INT32_MAX || h > INT32_MAX)
+        return false;
+    out = IRect{(int32_t)x, (int32_t)y, (int32_t)w, (int32_t)h};
+    return true;
+}
+
+// render_group (render.rs:49-143)
+void render_group(const rvk_scene &sc, const rvk_group &g, const Ctx &ctx, Ts ts, uint8_t *px, uint32_t w,
+                  uint32_t h) {
+    ts = ts_concat(ts, ts_from(g.transform));
+    bool isolate = g.isolate || g.opacity != 1.0f || g.clip_path >= 0 || g.mask >= 0 || g.filter_count > 0 ||
+                   g.blend_mode != RVK_MIX_NORMAL;
+    if (!isolate) {
+        render_nodes(sc, g, ctx, ts, px, w, h);
+        return;
+    }
+    FRect bb = rect_transform(g.layer_bbox, ts);
+    if (!bb.ok) return;
+    IRect ib;
+    float bw = bb.r - bb.l, bh = bb.b - bb.t;
+    if (g.filter_count == 0) {
+        int64_t x = (int64_t)as_i32(std::floor(bb.l)) - 2, y = (int64_t)as_i32(std::floor(bb.t)) - 2;
+        if (x < INT32_MIN || y < INT32_MIN) return;
+        int64_t ww = (int64_t)as_u32(std::ceil(bw)) + 4, hh = (int64_t)as_u32(std::ceil(bh)) + 4;
+        if (ww > UINT32_MAX || hh > UINT32_MAX) return;
+        if (!int_rect_from_xywh(x, y, ww, hh, ib)) return;
+        if (!fit_to_rect(ib, ctx.max_bbox, ib)) return;
+    } else {
+        IRect t;
+        if (!int_rect_from_xywh(as_i32(std::floor(bb.l)), as_i32(std::floor(bb.t)),
+                                as_u32(std::max(std::ceil(bw), 1.0f)), as_u32(std::max(std::ceil(bh), 1.0f)), t))
+            return;
+        if (!fit_to_rect(t, ctx.max_bbox, ib)) return;
+    }
+    float dx = bb.l, dy = bb.t;
+    dx -= bb.l - (float)ib.x;
+    dy -= bb.t - (float)ib.y;
+    Ts lts = ts_concat(Ts{1, 0, 0, 1, -dx, -dy}, ts);
+    std::vector<uint8_t> layer((size_t)ib.w * ib.h * 4, 0);
+    render_nodes(sc, g, ctx, lts, layer.data(), ib.w, ib.h);
+    for (uint32_t i = 0; i < g.filter_count; i++)
+        orc_filter_apply(&sc.filters[g.filter_offset + i], &sc.filter_data,
+                         rvk_transform{lts.sx, lts.ky, lts.kx, lts.sy, lts.tx, lts.ty}, layer.data(), ib.w, ib.h);
+    orc_draw_pixmap(px, w, h, ib.x, ib.y, layer.data(), ib.w, ib.h, g.opacity, convert_blend_mode(g.blend_mode));
+}
+
+void render_nodes(const rvk_scene &sc, const rvk_group &g, const Ctx &ctx, Ts ts, uint8_t *px, uint32_t w,
+                  uint32_t h) {
+    for (uint32_t i = 0; i < g.child_count; i++) {
+        const rvk_node_ref &n = sc.children[g.child_offset + i];
+        if (n.kind == RVK_NODE_GROUP) render_group(sc, sc.groups[n.index], ctx, ts, px, w, h);
+        else render_path(sc, sc.paths[n.index], ts, px, w, h);
+    }
+}
+
+// max_filter_bbox (lib.rs:86-97)
+IRect max_filter_bbox(uint32_t w, uint32_t h) {
+    int64_t x = -2 * (int64_t)std::min<uint32_t>(w, INT32_MAX), y = -2 * (int64_t)std::min<uint32_t>(h, INT32_MAX);
+    int64_t ww = 5 * (int64_t)w, hh = 5 * (int64_t)h;
+    IRect r;
+    if (x >= INT32_MIN && y >= INT32_MIN && int_rect_from_xywh(x, y, std::min<int64_t>(ww, UINT32_MAX),
+                                                               std::min<int64_t>(hh, UINT32_MAX), r))
+        return r;
+    return IRect{INT32_MIN / 2, INT32_MIN / 2, INT32_MAX / 2 - INT32_MIN / 2, INT32_MAX / 2 - INT32_MIN / 2};
+}
+
+}  // namespace
+
+int32_t orc_render(const rvk_scene *scene, rvk_transform ts, uint32_t w, uint32_t h, uint8_t *px) {
+    if (!scene || scene->group_count == 0) return RVK_ERR_INVALID_ARGUMENT;
+    Ctx ctx{max_filter_bbox(w, h)};
+    render_nodes(*scene, scene->groups[0], ctx, ts_from(ts), px, w, h);
+    return RVK_OK;
+}

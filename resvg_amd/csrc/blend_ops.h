@@ -36,14 +36,15 @@ __device__ __forceinline__ void set_lum(float &r, float &g, float &b, float l) {
     float d = l - lum(r, g, b);
     r += d; g += d; b += d;
 }
-__device__ __forceinline__ float clip1(float c, float mn, float mx, float l, float a) {
-    if (mn < 0.0f && l - mn != 0.0f) c = l + (c - l) * l / (l - mn);
+// clip_color as tiny-skia evaluates it: only the upper clamp toward the luminance
+// (the reference's mix-blend-mode/hue fixture rules out a lower `min < 0` clamp)
+__device__ __forceinline__ float clip1(float c, float mx, float l, float a) {
     if (mx > a && mx - l != 0.0f) c = l + (c - l) * (a - l) / (mx - l);
     return fmaxf(c, 0.0f);
 }
 __device__ __forceinline__ void clip_color(float &r, float &g, float &b, float a) {
-    float mn = fminf(r, fminf(g, b)), mx = fmaxf(r, fmaxf(g, b)), l = lum(r, g, b);
-    r = clip1(r, mn, mx, l, a); g = clip1(g, mn, mx, l, a); b = clip1(b, mn, mx, l, a);
+    float mx = fmaxf(r, fmaxf(g, b)), l = lum(r, g, b);
+    r = clip1(r, mx, l, a); g = clip1(g, mx, l, a); b = clip1(b, mx, l, a);
 }
 
 __device__ __forceinline__ float sep_channel(uint32_t mode, float x, float y, float sa, float da) {

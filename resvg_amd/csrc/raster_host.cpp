@@ -1,0 +1,741 @@
+// Host front end of the rasterizer: path iteration, Skia edge stepping expanded
+// into line pieces, the polyline stroker and shader setup.
+//
+// Edge arithmetic restates tiny-skia 0.12 (Skia's SkEdge: FDot6 inputs on the 4x
+// supersampled grid, 16.16 slopes, quadratic and cubic forward differencing with
+// MAX_COEFF_SHIFT 6, curves chopped at Y extrema first).  Every update step of a
+// curve edge is run here once and each non-empty line piece is kept, which is the
+// sequence tiny-skia's scanline walk visits lazily.
+#include "raster.h"
+#include <algorithm>
+#include <array>
+#include <cmath>
+
+using namespace rvkh;
+
+namespace rvkr {
+
+// ---------------------------------------------------------------------------
+// path iteration
+// ---------------------------------------------------------------------------
+void fill_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
+    const uint8_t *verbs = sc.verbs + path.verb_offset;
+    const float *pts = sc.points + 2 * (size_t)path.point_offset;
+    size_t pi = 0;
+    Pt move{0, 0}, last{0, 0};
+    bool open = false;
+    auto next = [&]() {
+        Pt p{pts[2 * pi], pts[2 * pi + 1]};
+        pi++;
+        ts_map(ts, p.x, p.y);
+        return p;
+    };
+    auto close = [&]() {
+        if (open && (last.x != move.x || last.y != move.y)) out.push_back(Seg{2, {last, move}});
+        open = false;
+    };
+    for (uint32_t v = 0; v < path.verb_count; v++) {
+        const uint8_t verb = verbs[v];
+        if (verb == RVK_VERB_MOVE) {
+            close();
+            move = last = next();
+            open = true;
+        } else if (verb == RVK_VERB_LINE) {
+            Pt p = next();
+            out.push_back(Seg{2, {last, p}});
+            last = p;
+        } else if (verb == RVK_VERB_QUAD) {
+            Seg s{3, {last}};
+            s.p[1] = next();
+            s.p[2] = next();
+            out.push_back(s);
+            last = s.p[2];
+        } else if (verb == RVK_VERB_CUBIC) {
+            Seg s{4, {last}};
+            s.p[1] = next();
+            s.p[2] = next();
+            s.p[3] = next();
+            out.push_back(s);
+            last = s.p[3];
+        } else {  // close
+            close();
+            last = move;
+        }
+    }
+    close();
+}
+
+bool bounds_nearly_zero(const std::vector<Seg> &segs) {
+    float mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+    for (const Seg &s : segs)
+        for (int i = 0; i < s.n; i++) {
+            mnx = std::min(mnx, s.p[i].x); mxx = std::max(mxx, s.p[i].x);
+            mny = std::min(mny, s.p[i].y); mxy = std::max(mxy, s.p[i].y);
+        }
+    const float nz = 1.0f / 4096.0f;
+    return !(mxx - mnx > nz) || !(mxy - mny > nz);
+}
+
+// ---------------------------------------------------------------------------
+// fixed-point edge stepping
+// ---------------------------------------------------------------------------
+namespace {
+
+inline int32_t fdot6_round(int32_t x) { return (x + 32) >> 6; }
+inline int32_t fdot6_to_fixed(int32_t x) { return (int32_t)((uint32_t)x << 10); }
+inline int32_t fixed_mul(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)b) >> 16); }
+inline int32_t fdot6_div(int32_t a, int32_t b) {
+    if ((int32_t)(int16_t)a == a) return (int32_t)((uint32_t)a << 16) / b;
+    int64_t v = ((int64_t)a << 16) / (int64_t)b;
+    return (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, v));
+}
+inline int32_t compute_dy(int32_t top, int32_t y0) { return ((top << 6) + 32) - y0; }
+inline int32_t diff_to_shift(int32_t dx, int32_t dy, int32_t shift_aa) {
+    dx = std::abs(dx);
+    dy = std::abs(dy);
+    uint32_t dist = (uint32_t)(dx > dy ? dx + (dy >> 1) : dy + (dx >> 1));
+    dist = (dist + (1u << 4)) >> (3 + shift_aa);
+    return (32 - (dist ? __builtin_clz(dist) : 32)) >> 1;
+}
+inline int32_t cubic_delta_from_line(int32_t a, int32_t b, int32_t c, int32_t d) {
+    int32_t one_third = ((a * 8 - b * 15 + 6 * c + d) * 19) >> 9;
+    int32_t two_third = ((a + 6 * b - c * 15 + d * 8) * 19) >> 9;
+    return std::max(std::abs(one_third), std::abs(two_third));
+}
+inline int32_t fdot6_of(float v, float scale) { return as_i32(v * scale); }
+constexpr int kMaxCoeffShift = 6;
+
+// SkEdge::updateLine on 16.16 inputs; emits the piece when it spans a row
+bool line_piece(int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t winding, Piece &p) {
+    y0 >>= 10;
+    y1 >>= 10;
+    int32_t top = fdot6_round(y0), bot = fdot6_round(y1);
+    if (top == bot) return false;
+    x0 >>= 10;
+    x1 >>= 10;
+    int32_t slope = fdot6_div(x1 - x0, y1 - y0);
+    p.x = fdot6_to_fixed(x0 + fixed_mul(slope, compute_dy(top, y0)));
+    p.dx = slope;
+    p.first_y = top;
+    p.last_y = bot - 1;
+    p.winding = winding;
+    return true;
+}
+
+struct Builder {
+    int shift;
+    std::vector<Piece> &out;
+    int32_t ymin = INT32_MAX, ymax = INT32_MIN;
+    int32_t edge_id = 0;
+    // kind of every edge pushed so far (line edges own exactly one piece, the last
+    // one in `out` while they are the newest edge), for combine_vertical
+    std::vector<uint8_t> edge_is_line;
+    bool last_is_line = false;
+    size_t last_index = 0;
+
+    void emit(Piece p) {
+        p.id = edge_id;
+        out.push_back(p);
+    }
+    void finish_edge(bool is_line) {
+        edge_is_line.push_back(is_line ? 1 : 0);
+        edge_id++;
+        last_is_line = is_line;
+        last_index = out.size() - 1;
+    }
+
+    // LineEdge::new + push_line's combine_vertical (tiny-skia edge_builder.rs)
+    void line(Pt a, Pt b) {
+        float scale = (float)(1 << (shift + 6));
+        int32_t x0 = fdot6_of(a.x, scale), y0 = fdot6_of(a.y, scale), x1 = fdot6_of(b.x, scale),
+                y1 = fdot6_of(b.y, scale);
+        int32_t w = 1;
+        if (y0 > y1) { std::swap(x0, x1); std::swap(y0, y1); w = -1; }
+        int32_t top = fdot6_round(y0), bot = fdot6_round(y1);
+        if (top == bot) return;
+        int32_t slope = fdot6_div(x1 - x0, y1 - y0);
+        Piece e{fdot6_to_fixed(x0 + fixed_mul(slope, compute_dy(top, y0))), slope, top, bot - 1, w, 0};
+        if (e.dx == 0 && last_is_line) {
+            Piece &l = out[last_index];
+            if (l.dx == 0 && e.x == l.x) {
+                if (e.winding == l.winding) {
+                    if (e.last_y + 1 == l.first_y) { l.first_y = e.first_y; return; }
+                    if (e.first_y == l.last_y + 1) { l.last_y = e.last_y; return; }
+                } else if (e.first_y == l.first_y) {
+                    if (e.last_y == l.last_y) {  // the two cancel: drop the previous edge
+                        out.pop_back();
+                        edge_id--;
+                        edge_is_line.pop_back();
+                        last_is_line = !edge_is_line.empty() && edge_is_line.back();
+                        last_index = out.empty() ? 0 : out.size() - 1;
+                        return;
+                    }
+                    if (e.last_y < l.last_y) { l.first_y = e.last_y + 1; return; }
+                    l.first_y = l.last_y + 1;
+                    l.last_y = e.last_y;
+                    l.winding = e.winding;
+                    return;
+                } else if (e.last_y == l.last_y) {
+                    if (e.first_y > l.first_y) { l.last_y = e.first_y - 1; return; }
+                    l.last_y = l.first_y - 1;
+                    l.first_y = e.first_y;
+                    l.winding = e.winding;
+                    return;
+                }
+            }
+        }
+        emit(e);
+        finish_edge(true);
+    }
+
+    // QuadraticEdge::new + update() until exhausted
+    void quad(const Pt *p) {
+        float scale = (float)(1 << (shift + 6));
+        int32_t x0 = fdot6_of(p[0].x, scale), y0 = fdot6_of(p[0].y, scale), x1 = fdot6_of(p[1].x, scale),
+                y1 = fdot6_of(p[1].y, scale), x2 = fdot6_of(p[2].x, scale), y2 = fdot6_of(p[2].y, scale);
+        int32_t w = 1;
+        if (y0 > y2) { std::swap(x0, x2); std::swap(y0, y2); w = -1; }
+        if (fdot6_round(y0) == fdot6_round(y2)) return;
+        int32_t sh = diff_to_shift(((int32_t)((uint32_t)x1 << 1) - x0 - x2) >> 2,
+                                   ((int32_t)((uint32_t)y1 << 1) - y0 - y2) >> 2, shift);
+        if (sh == 0) sh = 1;
+        else if (sh > kMaxCoeffShift) sh = kMaxCoeffShift;
+        int32_t count = 1 << sh, cshift = sh - 1;
+        int32_t A = (int32_t)((uint32_t)(x0 - x1 - x1 + x2) << 9), B = fdot6_to_fixed(x1 - x0);
+        int32_t qx = fdot6_to_fixed(x0), qdx = B + (A >> sh), qddx = A >> (sh - 1);
+        A = (int32_t)((uint32_t)(y0 - y1 - y1 + y2) << 9);
+        B = fdot6_to_fixed(y1 - y0);
+        int32_t qy = fdot6_to_fixed(y0), qdy = B + (A >> sh), qddy = A >> (sh - 1);
+        const int32_t lx = fdot6_to_fixed(x2), ly = fdot6_to_fixed(y2);
+        bool any = false;
+        while (count > 0) {
+            int32_t nx, ny;
+            if (--count > 0) {
+                nx = qx + (qdx >> cshift);
+                qdx += qddx;
+                ny = qy + (qdy >> cshift);
+                qdy += qddy;
+            } else {
+                nx = lx;
+                ny = ly;
+            }
+            Piece pc;
+            if (line_piece(qx, qy, nx, ny, w, pc)) { emit(pc); any = true; }
+            qx = nx;
+            qy = ny;
+        }
+        if (any) finish_edge(false);
+    }
+
+    // CubicEdge::new + update() until exhausted
+    void cubic(const Pt *p) {
+        float scale = (float)(1 << (shift + 6));
+        int32_t x0 = fdot6_of(p[0].x, scale), y0 = fdot6_of(p[0].y, scale), x1 = fdot6_of(p[1].x, scale),
+                y1 = fdot6_of(p[1].y, scale), x2 = fdot6_of(p[2].x, scale), y2 = fdot6_of(p[2].y, scale),
+                x3 = fdot6_of(p[3].x, scale), y3 = fdot6_of(p[3].y, scale);
+        int32_t w = 1;
+        if (y0 > y3) {
+            std::swap(x0, x3); std::swap(x1, x2); std::swap(y0, y3); std::swap(y1, y2);
+            w = -1;
+        }
+        if (fdot6_round(y0) == fdot6_round(y3)) return;
+        int32_t sh = diff_to_shift(cubic_delta_from_line(x0, x1, x2, x3), cubic_delta_from_line(y0, y1, y2, y3),
+                                   shift) + 1;
+        if (sh > kMaxCoeffShift) sh = kMaxCoeffShift;
+        int32_t up = 6, down = sh + up - 10;
+        if (down < 0) { down = 0; up = 10 - sh; }
+        auto upsh = [&](int32_t v) { return (int32_t)((uint32_t)v << up); };
+        int32_t count = (int32_t)((uint32_t)-1 << sh);
+        int32_t B = upsh(3 * (x1 - x0)), C = upsh(3 * (x0 - x1 - x1 + x2)), D = upsh(x3 + 3 * (x1 - x2) - x0);
+        int32_t cx = fdot6_to_fixed(x0), cdx = B + (C >> sh) + (D >> (2 * sh)), cddx = 2 * C + ((3 * D) >> (sh - 1)),
+                cdddx = (3 * D) >> (sh - 1);
+        B = upsh(3 * (y1 - y0)); C = upsh(3 * (y0 - y1 - y1 + y2)); D = upsh(y3 + 3 * (y1 - y2) - y0);
+        int32_t cy = fdot6_to_fixed(y0), cdy = B + (C >> sh) + (D >> (2 * sh)), cddy = 2 * C + ((3 * D) >> (sh - 1)),
+                cdddy = (3 * D) >> (sh - 1);
+        const int32_t lx = fdot6_to_fixed(x3), ly = fdot6_to_fixed(y3);
+        bool any = false;
+        while (count < 0) {
+            int32_t nx, ny;
+            if (++count < 0) {
+                nx = cx + (cdx >> down);
+                cdx += cddx >> sh;
+                cddx += cdddx;
+                ny = cy + (cdy >> down);
+                cdy += cddy >> sh;
+                cddy += cdddy;
+            } else {
+                nx = lx;
+                ny = ly;
+            }
+            if (ny < cy) ny = cy;
+            Piece pc;
+            if (line_piece(cx, cy, nx, ny, w, pc)) { emit(pc); any = true; }
+            cx = nx;
+            cy = ny;
+        }
+        if (any) finish_edge(false);
+    }
+};
+
+// chopping at Y extrema (tiny-skia-path path_geometry)
+bool unit_divide(float numer, float denom, float &r) {
+    if (numer < 0.0f) { numer = -numer; denom = -denom; }
+    if (denom == 0.0f || numer == 0.0f || numer >= denom) return false;
+    r = numer / denom;
+    return std::isfinite(r) && r > 0.0f && r < 1.0f;
+}
+inline Pt lerp_pt(Pt a, Pt b, float t) { return Pt{a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t}; }
+
+int quad_y_monotonic(const Pt *s, Pt *d) {
+    float a = s[0].y, b = s[1].y, c = s[2].y;
+    float ab = a - b, bc = b - c;
+    if (ab < 0.0f) bc = -bc;
+    if (ab == 0.0f || bc < 0.0f) {
+        float t;
+        if (unit_divide(a - b, a - b - b + c, t)) {
+            Pt p01 = lerp_pt(s[0], s[1], t), p12 = lerp_pt(s[1], s[2], t);
+            d[0] = s[0]; d[1] = p01; d[2] = lerp_pt(p01, p12, t); d[3] = p12; d[4] = s[2];
+            d[1].y = d[2].y;
+            d[3].y = d[2].y;
+            return 1;
+        }
+        b = std::fabs(a - b) < std::fabs(b - c) ? a : c;
+    }
+    d[0] = Pt{s[0].x, a}; d[1] = Pt{s[1].x, b}; d[2] = Pt{s[2].x, c};
+    return 0;
+}
+
+int unit_quad_roots(float a, float b, float c, float *roots) {
+    if (a == 0.0f) {
+        float r;
+        if (unit_divide(-c, b, r)) { roots[0] = r; return 1; }
+        return 0;
+    }
+    double dr = (double)b * (double)b - 4.0 * (double)a * (double)c;
+    if (dr < 0.0) return 0;
+    float r = (float)std::sqrt(dr);
+    if (!std::isfinite(r)) return 0;
+    float q = b < 0.0f ? -(b - r) / 2.0f : -(b + r) / 2.0f;
+    int n = 0;
+    float t;
+    if (unit_divide(q, a, t)) roots[n++] = t;
+    if (unit_divide(c, q, t)) roots[n++] = t;
+    if (n == 2) {
+        if (roots[0] > roots[1]) std::swap(roots[0], roots[1]);
+        else if (roots[0] == roots[1]) n = 1;
+    }
+    return n;
+}
+
+void chop_cubic(const Pt *s, float t, Pt *d) {
+    Pt ab = lerp_pt(s[0], s[1], t), bc = lerp_pt(s[1], s[2], t), cd = lerp_pt(s[2], s[3], t);
+    Pt abc = lerp_pt(ab, bc, t), bcd = lerp_pt(bc, cd, t);
+    d[0] = s[0]; d[1] = ab; d[2] = abc; d[3] = lerp_pt(abc, bcd, t); d[4] = bcd; d[5] = cd; d[6] = s[3];
+}
+
+int cubic_y_monotonic(const Pt *src, Pt *dst) {
+    float a = src[0].y, b = src[1].y, c = src[2].y, d = src[3].y;
+    float ts[3];
+    int n = unit_quad_roots(d - a + 3.0f * (b - c), 2.0f * (a - b - b + c), b - a, ts);
+    if (n == 0) {
+        for (int i = 0; i < 4; i++) dst[i] = src[i];
+        return 0;
+    }
+    Pt tmp[4];
+    const Pt *s = src;
+    float t = ts[0];
+    int off = 0;
+    for (int i = 0; i < n; i++) {
+        chop_cubic(s, t, dst + off);
+        if (i == n - 1) break;
+        off += 3;
+        for (int k = 0; k < 4; k++) tmp[k] = dst[off + k];
+        s = tmp;
+        float nt;
+        if (unit_divide(ts[i + 1] - ts[i], 1.0f - ts[i], nt)) t = nt;
+        else {
+            dst[off + 4] = s[3]; dst[off + 5] = s[3]; dst[off + 6] = s[3];
+            break;
+        }
+    }
+    dst[2].y = dst[3].y;
+    dst[4].y = dst[3].y;
+    if (n == 2) { dst[5].y = dst[6].y; dst[7].y = dst[6].y; }
+    return n;
+}
+
+}  // namespace
+
+void build_pieces(const std::vector<Seg> &segs, int shift, std::vector<Piece> &out, int32_t &ymin, int32_t &ymax) {
+    Builder b{shift, out};
+    for (const Seg &s : segs) {
+        if (s.n == 2) {
+            b.line(s.p[0], s.p[1]);
+        } else if (s.n == 3) {
+            Pt mono[5];
+            int n = quad_y_monotonic(s.p, mono);
+            for (int i = 0; i <= n; i++) b.quad(mono + 2 * i);
+        } else {
+            Pt mono[10];
+            int n = cubic_y_monotonic(s.p, mono);
+            for (int i = 0; i <= n; i++) b.cubic(mono + 3 * i);
+        }
+    }
+    ymin = INT32_MAX;
+    ymax = INT32_MIN;
+    for (const Piece &p : out) {
+        if (p.first_y > p.last_y) continue;
+        ymin = std::min(ymin, p.first_y);
+        ymax = std::max(ymax, p.last_y);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stroker: polyline outline pieces, each oriented with negative signed area so
+// that their nonzero union is the stroke (same specification as the oracle).
+// ---------------------------------------------------------------------------
+namespace {
+
+float res_scale_for(Ts ts) {
+    float sx = std::sqrt(ts.sx * ts.sx + ts.kx * ts.kx), sy = std::sqrt(ts.ky * ts.ky + ts.sy * ts.sy);
+    if (std::isfinite(sx) && std::isfinite(sy)) {
+        float s = std::max(sx, sy);
+        if (s > 0.0f) return s;
+    }
+    return 1.0f;
+}
+
+int steps_for(const Pt *p, int n, float tol) {
+    float dd = 0.0f;
+    for (int i = 0; i + 2 < n; i++) {
+        float dx = p[i].x - 2.0f * p[i + 1].x + p[i + 2].x, dy = p[i].y - 2.0f * p[i + 1].y + p[i + 2].y;
+        dd = std::max(dd, std::sqrt(dx * dx + dy * dy));
+    }
+    float k = n == 4 ? 0.75f : 0.25f;
+    int steps = (int)std::ceil(std::sqrt(k * dd / tol));
+    return std::max(1, std::min(steps, 256));
+}
+
+Pt bezier(const Pt *p, int n, float t) {
+    float u = 1.0f - t;
+    if (n == 3) {
+        float a = u * u, b = 2.0f * u * t, c = t * t;
+        return Pt{a * p[0].x + b * p[1].x + c * p[2].x, a * p[0].y + b * p[1].y + c * p[2].y};
+    }
+    float a = u * u * u, b = 3.0f * u * u * t, c = 3.0f * u * t * t, d = t * t * t;
+    return Pt{a * p[0].x + b * p[1].x + c * p[2].x + d * p[3].x, a * p[0].y + b * p[1].y + c * p[2].y + d * p[3].y};
+}
+
+struct Polyline { std::vector<Pt> pts; bool closed; };
+
+void polylines(const rvk_scene &sc, const rvk_path &path, float tol, std::vector<Polyline> &out) {
+    const uint8_t *verbs = sc.verbs + path.verb_offset;
+    const float *pts = sc.points + 2 * (size_t)path.point_offset;
+    size_t pi = 0;
+    Polyline cur{{}, false};
+    auto add = [&](Pt p) {
+        if (cur.pts.empty() || cur.pts.back().x != p.x || cur.pts.back().y != p.y) cur.pts.push_back(p);
+    };
+    auto next = [&]() { Pt p{pts[2 * pi], pts[2 * pi + 1]}; pi++; return p; };
+    auto finish = [&]() {
+        if (cur.pts.size() >= 2) out.push_back(cur);
+        cur = Polyline{{}, false};
+    };
+    for (uint32_t v = 0; v < path.verb_count; v++) {
+        uint8_t verb = verbs[v];
+        if (verb == RVK_VERB_MOVE) {
+            finish();
+            add(next());
+        } else if (verb == RVK_VERB_LINE) {
+            add(next());
+        } else if (verb == RVK_VERB_QUAD || verb == RVK_VERB_CUBIC) {
+            int n = verb == RVK_VERB_QUAD ? 3 : 4;
+            Pt c[4];
+            c[0] = cur.pts.empty() ? Pt{0, 0} : cur.pts.back();
+            for (int k = 1; k < n; k++) c[k] = next();
+            int steps = steps_for(c, n, tol);
+            for (int s = 1; s < steps; s++) add(bezier(c, n, (float)s / (float)steps));
+            add(c[n - 1]);
+        } else {  // close
+            if (cur.pts.size() >= 2 && cur.pts.front().x == cur.pts.back().x && cur.pts.front().y == cur.pts.back().y)
+                cur.pts.pop_back();
+            cur.closed = true;
+            Pt start = cur.pts.empty() ? Pt{0, 0} : cur.pts.front();
+            finish();
+            add(start);
+        }
+    }
+    finish();
+}
+
+void piece(std::vector<Pt> &poly, Ts ts, std::vector<Seg> &out) {
+    if (poly.size() < 3) return;
+    double a = 0.0;
+    for (size_t i = 0; i < poly.size(); i++) {
+        const Pt &u = poly[i], &v = poly[(i + 1) % poly.size()];
+        a += (double)u.x * v.y - (double)v.x * u.y;
+    }
+    if ((float)a > 0.0f) std::reverse(poly.begin(), poly.end());
+    for (auto &p : poly) ts_map(ts, p.x, p.y);
+    for (size_t i = 0; i < poly.size(); i++) out.push_back(Seg{2, {poly[i], poly[(i + 1) % poly.size()]}});
+}
+
+void arc(Pt c, float hw, float a0, float a1, float tol, std::vector<Pt> &out) {
+    float span = a1 - a0;
+    float step = 2.0f * std::acos(std::max(-1.0f, std::min(1.0f, 1.0f - tol / std::max(hw, tol))));
+    int n = std::max(1, (int)std::ceil(std::fabs(span) / std::max(step, 1e-3f)));
+    n = std::min(n, 64);
+    for (int i = 0; i <= n; i++) {
+        float a = a0 + span * ((float)i / (float)n);
+        out.push_back(Pt{c.x + hw * std::cos(a), c.y + hw * std::sin(a)});
+    }
+}
+
+}  // namespace
+
+void stroke_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
+    const float hw = path.stroke_width * 0.5f;
+    if (!(hw > 0.0f)) return;
+    const float tol = 0.25f / res_scale_for(ts);
+    std::vector<Polyline> lines;
+    polylines(sc, path, tol, lines);
+    std::vector<Pt> poly;
+    for (const Polyline &c : lines) {
+        const std::vector<Pt> &p = c.pts;
+        const size_t n = p.size(), nseg = c.closed ? n : n - 1;
+        if (nseg == 0) continue;
+        auto dir = [&](size_t i) {
+            Pt a = p[i], b = p[(i + 1) % n];
+            float dx = b.x - a.x, dy = b.y - a.y, l = std::sqrt(dx * dx + dy * dy);
+            return Pt{dx / l, dy / l};
+        };
+        for (size_t i = 0; i < nseg; i++) {
+            Pt a = p[i], b = p[(i + 1) % n], d = dir(i), nm{-d.y * hw, d.x * hw};
+            poly = {Pt{a.x + nm.x, a.y + nm.y}, Pt{b.x + nm.x, b.y + nm.y}, Pt{b.x - nm.x, b.y - nm.y},
+                    Pt{a.x - nm.x, a.y - nm.y}};
+            piece(poly, ts, out);
+        }
+        size_t first = c.closed ? 0 : 1, last = c.closed ? n : n - 1;
+        for (size_t v = first; v < last; v++) {
+            size_t i0 = c.closed ? (v + nseg - 1) % nseg : v - 1, i1 = c.closed ? v % nseg : v;
+            Pt d0 = dir(i0), d1 = dir(i1), q = p[v];
+            float cross = d0.x * d1.y - d0.y * d1.x, dot = d0.x * d1.x + d0.y * d1.y;
+            if (std::fabs(cross) < 1e-6f && dot > 0.0f) continue;
+            float s = cross > 0.0f ? -1.0f : 1.0f;
+            Pt n0{-d0.y * hw * s, d0.x * hw * s}, n1{-d1.y * hw * s, d1.x * hw * s};
+            Pt e0{q.x + n0.x, q.y + n0.y}, e1{q.x + n1.x, q.y + n1.y};
+            if (path.stroke_join == RVK_JOIN_ROUND) {
+                float a0 = std::atan2(n0.y, n0.x), a1 = std::atan2(n1.y, n1.x), da = a1 - a0;
+                while (da > 3.14159265f) da -= 6.2831853f;
+                while (da < -3.14159265f) da += 6.2831853f;
+                poly = {q};
+                arc(q, hw, a0, a0 + da, tol, poly);
+                piece(poly, ts, out);
+            } else {
+                bool miter = path.stroke_join == RVK_JOIN_MITER || path.stroke_join == RVK_JOIN_MITER_CLIP;
+                float cos_t = n0.x * n1.x / (hw * hw) + n0.y * n1.y / (hw * hw);
+                float ratio2 = 2.0f / std::max(1e-12f, 1.0f + cos_t);
+                if (miter && ratio2 <= path.stroke_miter_limit * path.stroke_miter_limit) {
+                    Pt m{n0.x + n1.x, n0.y + n1.y};
+                    float ml = std::sqrt(m.x * m.x + m.y * m.y), len = hw * std::sqrt(ratio2);
+                    poly = {q, e0, Pt{q.x + m.x / ml * len, q.y + m.y / ml * len}, e1};
+                } else {
+                    poly = {q, e0, e1};
+                }
+                piece(poly, ts, out);
+            }
+        }
+        if (!c.closed) {
+            for (int end = 0; end < 2; end++) {
+                Pt q = end == 0 ? p[0] : p[n - 1];
+                Pt d = end == 0 ? dir(0) : dir(n - 2);
+                if (end == 0) d = Pt{-d.x, -d.y};
+                Pt nm{-d.y * hw, d.x * hw};
+                if (path.stroke_cap == RVK_CAP_SQUARE) {
+                    Pt f{d.x * hw, d.y * hw};
+                    poly = {Pt{q.x + nm.x, q.y + nm.y}, Pt{q.x + nm.x + f.x, q.y + nm.y + f.y},
+                            Pt{q.x - nm.x + f.x, q.y - nm.y + f.y}, Pt{q.x - nm.x, q.y - nm.y}};
+                    piece(poly, ts, out);
+                } else if (path.stroke_cap == RVK_CAP_ROUND) {
+                    float a0 = std::atan2(nm.y, nm.x);
+                    poly = {q};
+                    arc(q, hw, a0, a0 - 3.14159265f, tol, poly);
+                    piece(poly, ts, out);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Shader setup (tiny-skia LinearGradient / RadialGradient / Gradient::new and the
+// gradient stage's factor/bias tables)
+// ---------------------------------------------------------------------------
+namespace {
+
+bool invert(Ts t, Ts &o) {
+    if (ts_is_identity(t)) { o = t; return true; }
+    if (!ts_has_skew(t)) {
+        if (ts_has_scale(t)) {
+            float ix = 1.0f / t.sx, iy = 1.0f / t.sy;
+            o = Ts{ix, 0.0f, 0.0f, iy, -t.tx * ix, -t.ty * iy};
+        } else {
+            o = Ts{1.0f, 0.0f, 0.0f, 1.0f, -t.tx, -t.ty};
+        }
+        return std::isfinite(o.sx) && std::isfinite(o.sy) && std::isfinite(o.tx) && std::isfinite(o.ty);
+    }
+    double det = (double)t.sx * (double)t.sy - (double)t.kx * (double)t.ky;
+    const float nz = 1.0f / 4096.0f;
+    if (std::fabs((float)det) <= nz * nz * nz) return false;
+    double inv = 1.0 / det;
+    o.sx = (float)((double)t.sy * inv);
+    o.ky = (float)((double)-t.ky * inv);
+    o.kx = (float)((double)-t.kx * inv);
+    o.sy = (float)((double)t.sx * inv);
+    o.tx = (float)(((double)t.kx * (double)t.ty - (double)t.sy * (double)t.tx) * inv);
+    o.ty = (float)(((double)t.ky * (double)t.tx - (double)t.sx * (double)t.ty) * inv);
+    return std::isfinite(o.sx) && std::isfinite(o.ky) && std::isfinite(o.kx) && std::isfinite(o.sy) &&
+           std::isfinite(o.tx) && std::isfinite(o.ty);
+}
+
+// Shader::SolidColor: premultiplied lowp colour; opaque when alpha is exactly 1.
+void set_solid(ShaderParams &s, const float c[4]) {
+    float r = c[0], g = c[1], b = c[2], a = c[3];
+    if (a != 1.0f) { r *= a; g *= a; b *= a; }
+    uint32_t R = (uint32_t)(r * 255.0f + 0.5f), G = (uint32_t)(g * 255.0f + 0.5f), B = (uint32_t)(b * 255.0f + 0.5f),
+             A = (uint32_t)(a * 255.0f + 0.5f);
+    s.kind = 1;
+    s.rgba = R | (G << 8) | (B << 16) | (A << 24);
+    s.as_source = a == 1.0f;
+}
+
+}  // namespace
+
+bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s) {
+    s = ShaderParams{};
+    if (p.kind == RVK_PAINT_COLOR) {
+        float c[4] = {(float)p.color[0] / 255.0f, (float)p.color[1] / 255.0f, (float)p.color[2] / 255.0f,
+                      (float)opacity_to_u8(p.opacity) / 255.0f};
+        set_solid(s, c);
+        return true;
+    }
+    if (p.kind != RVK_PAINT_LINEAR && p.kind != RVK_PAINT_RADIAL) return false;
+    struct GS { float pos; std::array<float, 4> c; };
+    std::vector<GS> stops;
+    const float fo = std::min(std::max(p.opacity, 0.0f), 1.0f);
+    for (uint32_t i = 0; i < p.stop_count; i++) {
+        const rvk_stop &st = sc.stops[p.stop_offset + i];
+        float a = std::min(std::max(st.opacity, 0.0f), 1.0f) * fo;
+        stops.push_back(GS{std::min(std::max(st.offset, 0.0f), 1.0f),
+                           {(float)st.color[0] / 255.0f, (float)st.color[1] / 255.0f, (float)st.color[2] / 255.0f,
+                            (float)opacity_to_u8(a) / 255.0f}});
+    }
+    if (stops.empty()) return false;
+    if (stops.size() == 1) {
+        set_solid(s, stops[0].c.data());
+        return true;
+    }
+    if (stops.size() + 2 > (size_t)kMaxStops) return false;
+    Ts unit;
+    if (p.kind == RVK_PAINT_LINEAR) {
+        float dx = p.x2 - p.x1, dy = p.y2 - p.y1, len = std::sqrt(dx * dx + dy * dy);
+        if (!std::isfinite(len)) return false;
+        if (len <= 1.0f / 32768.0f) {
+            float c[4] = {0, 0, 0, 0};
+            if (p.spread == RVK_SPREAD_PAD) {
+                for (int k = 0; k < 4; k++) c[k] = stops.back().c[k];
+            } else {
+                for (size_t i = 0; i + 1 < stops.size(); i++) {
+                    float wgt = stops[i + 1].pos - stops[i].pos;
+                    for (int k = 0; k < 4; k++) c[k] += wgt * (stops[i].c[k] + stops[i + 1].c[k]) * 0.5f;
+                }
+            }
+            set_solid(s, c);
+            return true;
+        }
+        float inv = 1.0f / len, vx = dx * inv, vy = dy * inv;
+        float sn = -vy, cs = vx, cinv = 1.0f - cs;
+        Ts rot{cs, sn, -sn, cs, sn * p.y1 + cinv * p.x1, -sn * p.x1 + cinv * p.y1};
+        unit = ts_concat(Ts{inv, 0, 0, inv, 0, 0}, ts_concat(Ts{1, 0, 0, 1, -p.x1, -p.y1}, rot));
+    } else {
+        float fdx = p.fx - p.cx, fdy = p.fy - p.cy;
+        if (!(std::sqrt(fdx * fdx + fdy * fdy) <= 1.0f / 4096.0f)) return false;  // focal: not supported
+        float r0 = p.fr, r1 = p.r;
+        // equal radii make the twist divide by zero; the NaN t clamps to the first stop
+        if (!(r1 >= 0.0f) || !(std::max(r0, r1) > 0.0f)) return false;
+        float scale = 1.0f / std::max(r0, r1);
+        unit = ts_concat(Ts{scale, 0, 0, scale, 0, 0}, Ts{1, 0, 0, 1, -p.cx, -p.cy});
+        s.radial = 1;
+        if (r0 != 0.0f) {
+            // focal radius: t = (t - r0/m) / ((r1 - r0)/m), m = max(r0, r1)
+            const float m = std::max(r0, r1);
+            s.r_twist = 1;
+            s.r_scale = r0 / m;
+            s.r_bias = (r1 - r0) / m;
+        }
+    }
+    bool dummy_first = stops.front().pos != 0.0f, dummy_last = stops.back().pos != 1.0f;
+    if (dummy_first) stops.insert(stops.begin(), GS{0.0f, stops.front().c});
+    if (dummy_last) stops.push_back(GS{1.0f, stops.back().c});
+    bool opaque = true;
+    for (auto &g : stops) opaque = opaque && g.c[3] == 1.0f;
+    size_t start = dummy_first ? 0 : 1;
+    float prev = 0.0f;
+    bool uniform = true;
+    const float step = stops[start].pos - prev;
+    for (size_t i = start; i < stops.size(); i++) {
+        float curr = i + 1 == stops.size() ? 1.0f : std::max(prev, std::min(stops[i].pos, 1.0f));
+        uniform = uniform && std::fabs(step - (curr - prev)) <= 1.0f / 4096.0f;
+        stops[i].pos = std::min(std::max(curr, 0.0f), 1.0f);
+        prev = curr;
+    }
+    Ts inv;
+    if (!invert(ts_concat(ts, ts_from(p.transform)), inv)) return false;
+    Ts u = ts_concat(unit, inv);
+    s.kind = 2;
+    s.sx = u.sx; s.ky = u.ky; s.kx = u.kx; s.sy = u.sy; s.tx = u.tx; s.ty = u.ty;
+    s.spread = p.spread;
+    s.pad_clamp = p.spread == RVK_SPREAD_PAD && uniform;
+    s.opaque = opaque;
+    s.as_source = opaque && !s.radial;
+    if (stops.size() == 2 && uniform) {
+        s.two_stop = 1;
+        for (int k = 0; k < 4; k++) {
+            s.f[0][k] = stops[1].c[k] - stops[0].c[k];
+            s.b[0][k] = stops[0].c[k];
+        }
+        return true;
+    }
+    size_t first = 0, last = stops.size() - 1;
+    if (stops.size() > 2) {
+        if (stops[0].c == stops[1].c) first = 1;
+        if (stops[stops.size() - 2].c == stops[stops.size() - 1].c) last = stops.size() - 2;
+    }
+    uint32_t n = 0;
+    float tl = stops[first].pos;
+    std::array<float, 4> cl = stops[first].c;
+    s.t[n] = 0.0f;
+    for (int k = 0; k < 4; k++) { s.f[n][k] = 0.0f; s.b[n][k] = cl[k]; }
+    n++;
+    for (size_t i = first; i < last; i++) {
+        float tr = stops[i + 1].pos;
+        std::array<float, 4> cr = stops[i + 1].c;
+        if (tl < tr) {
+            s.t[n] = tl;
+            for (int k = 0; k < 4; k++) {
+                s.f[n][k] = (cr[k] - cl[k]) / (tr - tl);
+                s.b[n][k] = cl[k] - s.f[n][k] * tl;
+            }
+            n++;
+        }
+        tl = tr;
+        cl = cr;
+    }
+    s.t[n] = tl;
+    for (int k = 0; k < 4; k++) { s.f[n][k] = 0.0f; s.b[n][k] = cl[k]; }
+    n++;
+    s.nstops = n;
+    return true;
+}
+
+}  // namespace rvkr

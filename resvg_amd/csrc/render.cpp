@@ -1,22 +1,391 @@
 // resvg::render / resvg_render on the device (crates/resvg/src/lib.rs:34-43,
 // crates/c-api/lib.rs:874-897).
+//
+// The render walk restates render.rs:10-143 and path.rs:6-24 on the host; pixel
+// work is recorded into per-target draw lists and executed by two kernels per
+// flush (coverage, then fused shade/composite).  A group that isolates only for
+// opacity or blend mode and holds a single path is not materialised: its layer
+// pixel is formed in registers and composited like draw_pixmap would (the layer
+// is transparent wherever the path has no coverage, and every group blend mode
+// leaves the destination unchanged under a transparent source).  Groups with
+// filters, or with several children, get a real layer, the filter graph, and a
+// draw_pixmap into their parent.
 #include "rvk_internal.h"
 #include "host_geom.h"
+#include "raster.h"
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+using namespace rvkh;
+using namespace rvkr;
+
+namespace rvk {
+rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, uint32_t total_rows,
+                           const Piece *pieces, const uint32_t *band_start, const uint32_t *band_items, uint8_t *arena,
+                           int32_t *overflow);
+rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
+                             int32_t x1, int32_t y1, const DrawItem *items, uint32_t nitems, const CovJob *jobs,
+                             const uint8_t *arena, const ShaderParams *shaders);
+}
+
+namespace {
+
+constexpr uint64_t kArenaBudget = 1ull << 30;  // coverage bytes per flush
+
+struct DrawList {
+    uint8_t *px = nullptr;
+    uint32_t w = 0, h = 0;
+    std::vector<DrawItem> items;
+    std::vector<CovJob> jobs;
+    std::vector<Piece> pieces;
+    std::vector<uint32_t> band_start;
+    std::vector<uint32_t> band_items;
+    std::vector<ShaderParams> shaders;
+    uint64_t arena = 0;
+    uint32_t rows = 0;
+    int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN;
+};
+
+uint32_t convert_blend_mode(uint32_t m) {  // render.rs:254-273
+    static const uint32_t map[RVK_MIX_COUNT] = {
+        RVK_BLEND_SOURCE_OVER, RVK_BLEND_MULTIPLY, RVK_BLEND_SCREEN, RVK_BLEND_OVERLAY, RVK_BLEND_DARKEN,
+        RVK_BLEND_LIGHTEN, RVK_BLEND_COLOR_DODGE, RVK_BLEND_COLOR_BURN, RVK_BLEND_HARD_LIGHT, RVK_BLEND_SOFT_LIGHT,
+        RVK_BLEND_DIFFERENCE, RVK_BLEND_EXCLUSION, RVK_BLEND_HUE, RVK_BLEND_SATURATION, RVK_BLEND_COLOR,
+        RVK_BLEND_LUMINOSITY};
+    return m < RVK_MIX_COUNT ? map[m] : RVK_BLEND_SOURCE_OVER;
+}
+
+bool int_rect(int64_t x, int64_t y, int64_t w, int64_t h, IRect &out) {
+    if (w <= 0 || h <= 0 || x < INT32_MIN || y < INT32_MIN || x + w > INT32_MAX || y + h > INT32_MAX) return false;
+    out = IRect{(int32_t)x, (int32_t)y, (int32_t)w, (int32_t)h};
+    return true;
+}
+
+IRect max_filter_bbox(uint32_t w, uint32_t h) {  // lib.rs:86-97
+    int64_t x = -2 * (int64_t)std::min<uint32_t>(w, INT32_MAX), y = -2 * (int64_t)std::min<uint32_t>(h, INT32_MAX);
+    IRect r;
+    if (int_rect(x, y, std::min<int64_t>(5 * (int64_t)w, UINT32_MAX), std::min<int64_t>(5 * (int64_t)h, UINT32_MAX), r))
+        return r;
+    return IRect{INT32_MIN / 2, INT32_MIN / 2, INT32_MAX / 2 - INT32_MIN / 2, INT32_MAX / 2 - INT32_MIN / 2};
+}
+
+struct Renderer {
+    rvk_context *ctx;
+    const rvk_scene &sc;
+    IRect max_bbox;
+    rvk_status err = RVK_OK;
+    uint64_t overflow_rows = 0;
+
+    // Adds a coverage job for device-space segments, restricted to `clip` (pixels
+    // of the job's pixmap).  Returns the job index or -1 when nothing is covered.
+    int32_t add_job(DrawList &dl, const std::vector<Seg> &segs, uint32_t rule, bool aa, IRect clip) {
+        if (segs.empty()) return -1;
+        std::vector<Piece> pcs;
+        int32_t ymin, ymax;
+        const int shift = aa ? 2 : 0;
+        build_pieces(segs, shift, pcs, ymin, ymax);
+        if (pcs.empty() || ymin > ymax) return -1;
+        float mnx = INFINITY, mxx = -INFINITY;
+        for (const Seg &s : segs)
+            for (int i = 0; i < s.n; i++) { mnx = std::min(mnx, s.p[i].x); mxx = std::max(mxx, s.p[i].x); }
+        if (!std::isfinite(mnx) || !std::isfinite(mxx)) return -1;
+        int64_t x0 = std::max<int64_t>((int64_t)std::floor(mnx) - 1, clip.x);
+        int64_t x1 = std::min<int64_t>((int64_t)std::ceil(mxx) + 2, (int64_t)clip.x + clip.w);
+        int64_t y0 = std::max<int64_t>(ymin >> shift, clip.y);
+        int64_t y1 = std::min<int64_t>((ymax >> shift) + 1, (int64_t)clip.y + clip.h);
+        if (x1 <= x0 || y1 <= y0) return -1;
+        CovJob j{};
+        j.x0 = (int32_t)x0; j.y0 = (int32_t)y0; j.w = (int32_t)(x1 - x0); j.h = (int32_t)(y1 - y0);
+        j.rule = rule;
+        j.aa = aa ? 1 : 0;
+        j.piece_offset = (uint32_t)dl.pieces.size();
+        j.piece_count = (uint32_t)pcs.size();
+        j.band_y0 = j.y0 << shift;
+        const int32_t rows_ss = j.h << shift;
+        j.band_count = (uint32_t)((rows_ss + (1 << kBandShift) - 1) >> kBandShift);
+        j.band_offset = (uint32_t)dl.band_start.size();
+        // counting sort of piece indices into bands
+        std::vector<uint32_t> cnt(j.band_count + 1, 0);
+        auto band_range = [&](const Piece &p, int32_t &b0, int32_t &b1) {
+            int32_t a = std::max(p.first_y, j.band_y0), b = std::min(p.last_y, j.band_y0 + rows_ss - 1);
+            if (b < a) { b0 = 1; b1 = 0; return; }
+            b0 = (a - j.band_y0) >> kBandShift;
+            b1 = (b - j.band_y0) >> kBandShift;
+        };
+        for (const Piece &p : pcs) {
+            int32_t b0, b1;
+            band_range(p, b0, b1);
+            for (int32_t b = b0; b <= b1; b++) cnt[b + 1]++;
+        }
+        for (uint32_t b = 0; b < j.band_count; b++) cnt[b + 1] += cnt[b];
+        const uint32_t items_base = (uint32_t)dl.band_items.size();
+        dl.band_items.resize(items_base + cnt[j.band_count]);
+        std::vector<uint32_t> fillpos(cnt.begin(), cnt.end() - 1);
+        for (uint32_t i = 0; i < pcs.size(); i++) {
+            int32_t b0, b1;
+            band_range(pcs[i], b0, b1);
+            for (int32_t b = b0; b <= b1; b++) dl.band_items[items_base + fillpos[b]++] = i;
+        }
+        for (uint32_t b = 0; b <= j.band_count; b++) dl.band_start.push_back(items_base + cnt[b]);
+        dl.pieces.insert(dl.pieces.end(), pcs.begin(), pcs.end());
+        const uint64_t stride = ((uint64_t)j.w + 3) & ~3ull;
+        j.cov_offset = dl.arena;
+        dl.arena += (stride * (uint64_t)j.h + 255) & ~255ull;
+        j.row_start = dl.rows;
+        dl.rows += (uint32_t)j.h;
+        dl.jobs.push_back(j);
+        return (int32_t)dl.jobs.size() - 1;
+    }
+
+    // fill / stroke coverage jobs + shaders of one path drawn with `ts` into a
+    // pixmap whose visible pixels are `clip`.
+    void path_jobs(DrawList &dl, const rvk_path &p, Ts ts, IRect clip, DrawItem &it) {
+        it.fill_job = it.stroke_job = -1;
+        it.fill_shader = it.stroke_shader = -1;
+        it.stroke_first = p.paint_order_stroke_first ? 1 : 0;
+        if (!p.visible) return;
+        ShaderParams sp;
+        // path::fill_path (path.rs:26-75)
+        if (p.fill.kind != RVK_PAINT_NONE && p.bounds.w != 0.0f && p.bounds.h != 0.0f &&
+            make_shader(sc, p.fill, ts, sp)) {
+            std::vector<Seg> segs;
+            fill_segments(sc, p, ts, segs);
+            if (!segs.empty() && !bounds_nearly_zero(segs)) {
+                it.fill_job = add_job(dl, segs, p.fill_rule, p.anti_alias != 0, clip);
+                if (it.fill_job >= 0) {
+                    it.fill_shader = (int32_t)dl.shaders.size();
+                    dl.shaders.push_back(sp);
+                }
+            }
+        }
+        // path::stroke_path (path.rs:77-116), outline by this build's stroker
+        if (p.stroke.kind != RVK_PAINT_NONE && make_shader(sc, p.stroke, ts, sp)) {
+            std::vector<Seg> segs;
+            stroke_segments(sc, p, ts, segs);
+            it.stroke_job = add_job(dl, segs, RVK_FILL_NONZERO, p.anti_alias != 0, clip);
+            if (it.stroke_job >= 0) {
+                it.stroke_shader = (int32_t)dl.shaders.size();
+                dl.shaders.push_back(sp);
+            }
+        }
+    }
+
+    void push_item(DrawList &dl, DrawItem &it) {
+        if (it.fill_job < 0 && it.stroke_job < 0) return;
+        // the item's target rect: union of its jobs' rects moved to target space
+        int32_t x0 = INT32_MAX, y0 = INT32_MAX, x1 = INT32_MIN, y1 = INT32_MIN;
+        for (int32_t j : {it.fill_job, it.stroke_job}) {
+            if (j < 0) continue;
+            const CovJob &jb = dl.jobs[j];
+            x0 = std::min(x0, jb.x0 + it.lx); y0 = std::min(y0, jb.y0 + it.ly);
+            x1 = std::max(x1, jb.x0 + jb.w + it.lx); y1 = std::max(y1, jb.y0 + jb.h + it.ly);
+        }
+        it.x0 = std::max(x0, 0); it.y0 = std::max(y0, 0);
+        it.x1 = std::min(x1, (int32_t)dl.w); it.y1 = std::min(y1, (int32_t)dl.h);
+        if (it.x1 <= it.x0 || it.y1 <= it.y0) return;
+        dl.bx0 = std::min(dl.bx0, it.x0); dl.by0 = std::min(dl.by0, it.y0);
+        dl.bx1 = std::max(dl.bx1, it.x1); dl.by1 = std::max(dl.by1, it.y1);
+        dl.items.push_back(it);
+        ctx->stats.paths++;
+    }
+
+    rvk_status flush(DrawList &dl) {
+        if (dl.items.empty()) {
+            dl = DrawList{dl.px, dl.w, dl.h};
+            return RVK_OK;
+        }
+        // one device block: jobs | pieces | band_start | band_items | shaders | items | overflow | arena
+        auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+        const size_t o_jobs = 0;
+        const size_t o_pieces = al(o_jobs + dl.jobs.size() * sizeof(CovJob));
+        const size_t o_bs = al(o_pieces + dl.pieces.size() * sizeof(Piece));
+        const size_t o_bi = al(o_bs + dl.band_start.size() * 4);
+        const size_t o_sh = al(o_bi + dl.band_items.size() * 4);
+        const size_t o_it = al(o_sh + dl.shaders.size() * sizeof(ShaderParams));
+        const size_t o_ov = al(o_it + dl.items.size() * sizeof(DrawItem));
+        const size_t meta = al(o_ov + 4);
+        std::vector<uint8_t> host(meta, 0);
+        auto put = [&](size_t off, const void *src, size_t n) { if (n) std::memcpy(host.data() + off, src, n); };
+        put(o_jobs, dl.jobs.data(), dl.jobs.size() * sizeof(CovJob));
+        put(o_pieces, dl.pieces.data(), dl.pieces.size() * sizeof(Piece));
+        put(o_bs, dl.band_start.data(), dl.band_start.size() * 4);
+        put(o_bi, dl.band_items.data(), dl.band_items.size() * 4);
+        put(o_sh, dl.shaders.data(), dl.shaders.size() * sizeof(ShaderParams));
+        put(o_it, dl.items.data(), dl.items.size() * sizeof(DrawItem));
+        void *dev = nullptr;
+        RVK_TRY(rvk::scratch_alloc(ctx, meta + dl.arena, &dev));
+        uint8_t *d = (uint8_t *)dev;
+        rvk_status st = rvk::upload_sync(ctx, d, host.data(), meta);
+        if (st == RVK_OK)
+            st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)dl.jobs.size(), dl.rows,
+                                      (const Piece *)(d + o_pieces), (const uint32_t *)(d + o_bs),
+                                      (const uint32_t *)(d + o_bi), d + meta, (int32_t *)(d + o_ov));
+        if (st == RVK_OK)
+            st = rvk::launch_draw_items(ctx, dl.px, dl.w, dl.h, dl.bx0, dl.by0, dl.bx1, dl.by1,
+                                        (const DrawItem *)(d + o_it), (uint32_t)dl.items.size(),
+                                        (const CovJob *)(d + o_jobs), d + meta, (const ShaderParams *)(d + o_sh));
+        int32_t ov = 0;
+        if (st == RVK_OK) {
+            hipError_t e = hipMemcpyAsync(&ov, d + o_ov, 4, hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) st = RVK_ERR_HIP;
+        }
+        rvk::scratch_release(ctx, dev);
+        overflow_rows += (uint64_t)(ov > 0 ? ov : 0);
+        dl = DrawList{dl.px, dl.w, dl.h};
+        return st;
+    }
+
+    void fail(rvk_status s) {
+        if (err == RVK_OK) err = s;
+    }
+
+    void render_path(const rvk_path &p, Ts ts, DrawList &dl) {
+        if (dl.arena > kArenaBudget) fail(flush(dl));
+        DrawItem it{};
+        it.layer = 0;
+        it.opacity = 1.0f;
+        it.blend_mode = RVK_BLEND_SOURCE_OVER;
+        path_jobs(dl, p, ts, IRect{0, 0, (int32_t)dl.w, (int32_t)dl.h}, it);
+        push_item(dl, it);
+    }
+
+    void render_nodes(const rvk_group &g, Ts ts, DrawList &dl) {
+        for (uint32_t i = 0; i < g.child_count && err == RVK_OK; i++) {
+            const rvk_node_ref &n = sc.children[g.child_offset + i];
+            if (n.kind == RVK_NODE_GROUP) render_group(sc.groups[n.index], ts, dl);
+            else render_path(sc.paths[n.index], ts, dl);
+        }
+    }
+
+    // render_group (render.rs:49-143)
+    void render_group(const rvk_group &g, Ts ts, DrawList &dl) {
+        ts = ts_concat(ts, ts_from(g.transform));
+        const bool isolate = g.isolate || g.opacity != 1.0f || g.clip_path >= 0 || g.mask >= 0 ||
+                             g.filter_count > 0 || g.blend_mode != RVK_MIX_NORMAL;
+        if (!isolate) {
+            render_nodes(g, ts, dl);
+            return;
+        }
+        if (g.clip_path >= 0 || g.mask >= 0) {  // clip paths and masks are outside this scene format
+            fail(RVK_ERR_UNSUPPORTED);
+            return;
+        }
+        FRect bb = rect_transform(g.layer_bbox, ts);
+        if (!bb.ok) return;
+        const float bw = bb.r - bb.l, bh = bb.b - bb.t;
+        IRect ib;
+        if (g.filter_count == 0) {
+            int64_t x = (int64_t)as_i32(std::floor(bb.l)) - 2, y = (int64_t)as_i32(std::floor(bb.t)) - 2;
+            if (x < INT32_MIN || y < INT32_MIN) return;
+            if (!int_rect(x, y, (int64_t)as_u32(std::ceil(bw)) + 4, (int64_t)as_u32(std::ceil(bh)) + 4, ib)) return;
+            if (!fit_to_rect(ib, max_bbox, ib)) return;
+        } else {
+            IRect t;
+            if (!int_rect(as_i32(std::floor(bb.l)), as_i32(std::floor(bb.t)), as_u32(std::max(std::ceil(bw), 1.0f)),
+                          as_u32(std::max(std::ceil(bh), 1.0f)), t))
+                return;
+            if (!fit_to_rect(t, max_bbox, ib)) return;
+        }
+        float dx = bb.l, dy = bb.t;
+        dx -= bb.l - (float)ib.x;
+        dy -= bb.t - (float)ib.y;
+        const Ts lts = ts_concat(Ts{1, 0, 0, 1, -dx, -dy}, ts);
+        ctx->stats.layers++;
+        // elided layer: one path child, no filters
+        if (g.filter_count == 0 && g.child_count == 1 && sc.children[g.child_offset].kind == RVK_NODE_PATH) {
+            if (dl.arena > kArenaBudget) fail(flush(dl));
+            const rvk_path &p = sc.paths[sc.children[g.child_offset].index];
+            DrawItem it{};
+            it.layer = 1;
+            it.lx = ib.x;
+            it.ly = ib.y;
+            it.opacity = g.opacity;
+            it.blend_mode = convert_blend_mode(g.blend_mode);
+            // layer pixels that land on the target
+            int32_t cx0 = std::max(0, -ib.x), cy0 = std::max(0, -ib.y);
+            int32_t cx1 = std::min(ib.w, (int32_t)dl.w - ib.x), cy1 = std::min(ib.h, (int32_t)dl.h - ib.y);
+            if (cx1 <= cx0 || cy1 <= cy0) return;
+            path_jobs(dl, p, lts, IRect{cx0, cy0, cx1 - cx0, cy1 - cy0}, it);
+            push_item(dl, it);
+            return;
+        }
+        // materialised layer
+        void *buf = nullptr;
+        rvk_status s = rvk::scratch_alloc(ctx, (size_t)ib.w * ib.h * 4, &buf);
+        if (s != RVK_OK) return;  // Pixmap::new failure skips the group (render.rs:108-109)
+        uint8_t *layer = (uint8_t *)buf;
+        s = rvk::launch_fill_sub(ctx, layer, ib.w, ib.h, 0, 0, ib.w, ib.h, 0u);
+        DrawList child;
+        child.px = layer;
+        child.w = (uint32_t)ib.w;
+        child.h = (uint32_t)ib.h;
+        if (s == RVK_OK) {
+            render_nodes(g, lts, child);
+            s = flush(child);
+        }
+        for (uint32_t i = 0; s == RVK_OK && err == RVK_OK && i < g.filter_count; i++)
+            s = rvk::filter_apply(ctx, &sc.filters[g.filter_offset + i], &sc.filter_data, lts, layer, ib.w, ib.h,
+                                  nullptr);
+        if (s == RVK_OK) s = flush(dl);
+        if (s == RVK_OK)
+            s = rvk::launch_draw_pixmap(ctx, dl.px, dl.w, dl.h, ib.x, ib.y, layer, ib.w, ib.h, g.opacity,
+                                        convert_blend_mode(g.blend_mode));
+        rvk::scratch_release(ctx, buf);
+        fail(s);
+    }
+};
+
+}  // namespace
+
+namespace rvk {
+
+rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t w, uint32_t h, uint8_t *px,
+                        uint64_t *overflow_rows) {
+    Renderer r{ctx, *scene, max_filter_bbox(w, h)};
+    DrawList root;
+    root.px = px;
+    root.w = w;
+    root.h = h;
+    r.render_nodes(scene->groups[0], ts, root);
+    if (r.err == RVK_OK) r.fail(r.flush(root));
+    if (overflow_rows) *overflow_rows = r.overflow_rows;
+    if (r.err == RVK_OK && r.overflow_rows) return RVK_ERR_UNSUPPORTED;
+    return r.err;
+}
+
+}  // namespace rvk
 
 extern "C" {
 
+static bool scene_ok(const rvk_scene *s) {
+    return s && s->group_count > 0 && s->groups && (s->path_count == 0 || (s->paths && s->verbs && s->points));
+}
+
 rvk_status rvk_render(rvk_context *ctx, const rvk_scene *scene, rvk_transform transform, uint32_t width,
                       uint32_t height, uint8_t *dev_pixmap) {
-    if (!ctx || !scene || !dev_pixmap || !rvk::valid_dims(width, height)) return RVK_ERR_INVALID_ARGUMENT;
-    (void)transform;
-    return RVK_ERR_UNSUPPORTED;
+    if (!ctx || !scene_ok(scene) || !dev_pixmap || !rvk::valid_dims(width, height)) return RVK_ERR_INVALID_ARGUMENT;
+    ctx->stats = rvk_stats{};
+    return rvk::render_scene(ctx, scene, ts_from(transform), width, height, dev_pixmap, nullptr);
 }
 
 rvk_status rvk_render_host(rvk_context *ctx, const rvk_scene *scene, rvk_transform transform, uint32_t width,
                            uint32_t height, uint8_t *host_pixmap) {
-    if (!ctx || !scene || !host_pixmap || !rvk::valid_dims(width, height)) return RVK_ERR_INVALID_ARGUMENT;
-    (void)transform;
-    return RVK_ERR_UNSUPPORTED;
+    if (!ctx || !scene_ok(scene) || !host_pixmap || !rvk::valid_dims(width, height)) return RVK_ERR_INVALID_ARGUMENT;
+    ctx->stats = rvk_stats{};
+    const size_t bytes = (size_t)width * height * 4;
+    void *dev = nullptr;
+    RVK_TRY(rvk::scratch_alloc(ctx, bytes, &dev));
+    rvk_status s = rvk::upload_sync(ctx, dev, host_pixmap, bytes);
+    if (s == RVK_OK) s = rvk::render_scene(ctx, scene, ts_from(transform), width, height, (uint8_t *)dev, nullptr);
+    if (s == RVK_OK) {
+        hipError_t e = hipMemcpyAsync(host_pixmap, dev, bytes, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) s = RVK_ERR_HIP;
+    }
+    rvk::scratch_release(ctx, dev);
+    return s;
 }
 
 }  // extern "C"

@@ -1,0 +1,207 @@
+"""Seeded synthetic documents for the three plain-method workloads.
+
+* ``blur_offset_composite_doc`` — 512x512: one closed nonzero path of 64 cubics with
+  control points uniform in the canvas, filled with a 4-stop linear gradient, inside
+  a group filtered by feGaussianBlur(4) -> feOffset(3,3) -> feComposite(over
+  SourceGraphic) with the default filter region.
+* ``full_filter_doc`` — 4096x4096 by default: N paths (3-200 cubic/quad segments,
+  points uniform in the canvas, evenodd/nonzero 50/50, 40% also stroked 0.5-8 px with
+  round/miter/bevel joins), each with a 2-8 stop linear or radial gradient and in its
+  own group with opacity U[0.2, 1]; 8 filter groups with userSpaceOnUse regions
+  covering blur (IIR and box), morphology, 3x3/5x5 convolution, colour matrix,
+  component transfer, turbulence, distant/point/spot lighting and multiply/screen.
+* ``document_batch`` — many documents of log-uniform sizes for the batched method.
+
+Random numbers come from numpy's PCG64 seeded per document; every float is rounded
+to f32 before use, so the same document is produced on every machine.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import abi
+from .filters import SOURCE_ALPHA, SOURCE_GRAPHIC, FilterGraph, distant_light, point_light, spot_light
+from .scene import PaintSpec, Scene
+
+F32 = np.float32
+
+
+def _f(v):
+    return float(F32(v))
+
+
+def _stops(rng, n):
+    offs = np.sort(rng.uniform(0.0, 1.0, size=n)).astype(F32)
+    cols = rng.integers(0, 256, size=(n, 3))
+    return [(_f(o), tuple(int(c) for c in col), 1.0) for o, col in zip(offs, cols)]
+
+
+def _cubic_extrema_bounds(p0, p1, p2, p3):
+    xs, ys = [p0[0], p3[0]], [p0[1], p3[1]]
+    for k in range(2):
+        a = -p0[k] + 3 * p1[k] - 3 * p2[k] + p3[k]
+        b = 2 * (p0[k] - 2 * p1[k] + p2[k])
+        c = p1[k] - p0[k]
+        roots = []
+        if abs(a) < 1e-12:
+            if abs(b) > 1e-12:
+                roots.append(-c / b)
+        else:
+            disc = b * b - 4 * a * c
+            if disc >= 0:
+                s = np.sqrt(disc)
+                roots += [(-b + s) / (2 * a), (-b - s) / (2 * a)]
+        for t in roots:
+            if 0 < t < 1:
+                u = 1 - t
+                v = u ** 3 * p0[k] + 3 * u * u * t * p1[k] + 3 * u * t * t * p2[k] + t ** 3 * p3[k]
+                (xs if k == 0 else ys).append(v)
+    return min(xs), min(ys), max(xs), max(ys)
+
+
+def blur_offset_composite_doc(seed: int = 0, size: int = 512):
+    """The 512x512 single-path document (returns the Scene and its canvas size)."""
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(0, size, size=(1 + 64 * 3, 2)).astype(F32)
+    pts[-1] = pts[0]  # p64 = p0
+    verbs = [abi.VERB_MOVE] + [abi.VERB_CUBIC] * 64 + [abi.VERB_CLOSE]
+    # tight bounding box (usvg's objectBoundingBox) for the default filter region
+    bx0 = by0 = np.inf
+    bx1 = by1 = -np.inf
+    for i in range(64):
+        q = [tuple(map(float, pts[3 * i + k])) for k in range(4)]
+        a, b, c, d = _cubic_extrema_bounds(*q)
+        bx0, by0, bx1, by1 = min(bx0, a), min(by0, b), max(bx1, c), max(by1, d)
+    bw, bh = bx1 - bx0, by1 - by0
+    region = (_f(bx0 - 0.1 * bw), _f(by0 - 0.1 * bh), _f(1.2 * bw), _f(1.2 * bh))
+    g = FilterGraph(region)
+    blur = g.gaussian_blur(4.0)
+    off = g.offset(3.0, 3.0, in1=blur)
+    g.composite(abi.COMPOSITE_OVER, in1=off, in2=SOURCE_GRAPHIC)
+    e = rng.uniform(0, size, size=4).astype(F32)
+    paint = PaintSpec.linear(_f(e[0]), _f(e[1]), _f(e[2]), _f(e[3]), _stops(rng, 4))
+    sc = Scene(size, size)
+    grp = sc.group(sc.root, filters=[g])
+    sc.path(grp, verbs, pts, fill=paint, fill_rule=abi.FILL_NONZERO)
+    return sc, (size, size)
+
+
+def _random_path(rng, size, nseg):
+    verbs = [abi.VERB_MOVE]
+    kinds = rng.integers(0, 2, size=nseg)
+    npts = 1 + int(sum(3 if k else 2 for k in kinds))
+    pts = rng.uniform(0, size, size=(npts, 2)).astype(F32)
+    for k in kinds:
+        verbs.append(abi.VERB_CUBIC if k else abi.VERB_QUAD)
+    verbs.append(abi.VERB_CLOSE)
+    return verbs, pts
+
+
+def _random_paint(rng, size, nstops):
+    stops = _stops(rng, nstops)
+    if rng.random() < 0.5:
+        e = rng.uniform(0, size, size=4).astype(F32)
+        return PaintSpec.linear(_f(e[0]), _f(e[1]), _f(e[2]), _f(e[3]), stops)
+    c = rng.uniform(0, size, size=2).astype(F32)
+    r = _f(rng.uniform(0.1, 0.6) * size)
+    return PaintSpec.radial(_f(c[0]), _f(c[1]), r, stops)
+
+
+def _filter_chains(rng, size):
+    """Eight filter graphs whose primitives together cover the full-filter workload."""
+    out = []
+
+    def region():
+        w, h = rng.uniform(0.2, 0.45, size=2) * size
+        x, y = rng.uniform(0, size - w), rng.uniform(0, size - h)
+        return (_f(x), _f(y), _f(w), _f(h))
+
+    g = FilterGraph(region())  # IIR blur + multiply
+    b = g.gaussian_blur(0.5)
+    g.blend("multiply", in1=b, in2=SOURCE_GRAPHIC)
+    out.append(g)
+    g = FilterGraph(region())  # IIR blur + colour matrix
+    b = g.gaussian_blur(1.5)
+    g.color_matrix(abi.COLOR_MATRIX_MATRIX, rng.uniform(-1, 1, 20).astype(F32).tolist(), in1=b)
+    out.append(g)
+    g = FilterGraph(region())  # box blur + dilate
+    b = g.gaussian_blur(6.0)
+    g.morphology(abi.MORPH_DILATE, _f(rng.uniform(1, 8)), in1=b)
+    out.append(g)
+    g = FilterGraph(region())  # box blur sigma 24 + screen
+    b = g.gaussian_blur(24.0)
+    g.blend("screen", in1=b, in2=SOURCE_GRAPHIC)
+    out.append(g)
+    g = FilterGraph(region())  # erode + 3x3 convolution
+    m = g.morphology(abi.MORPH_ERODE, _f(rng.uniform(1, 8)))
+    g.convolve_matrix(3, 3, rng.integers(-4, 5, 9).astype(float), in1=m)
+    out.append(g)
+    g = FilterGraph(region())  # 5x5 convolution + component transfer
+    cv = g.convolve_matrix(5, 5, rng.integers(-4, 5, 25).astype(float))
+    g.component_transfer([("table", rng.uniform(0, 1, 5).astype(F32).tolist()), ("gamma", 1.2, 0.8, 0.05),
+                          ("gamma", 0.9, 2.2, 0.0), None], in1=cv)
+    out.append(g)
+    g = FilterGraph(region())  # fractal turbulence composited into the source
+    t = g.turbulence(_f(rng.uniform(0.01, 0.1)), _f(rng.uniform(0.01, 0.1)), 4, int(rng.integers(0, 1000)),
+                     False, True)
+    g.composite(abi.COMPOSITE_IN, in1=t, in2=SOURCE_GRAPHIC)
+    out.append(g)
+    x, y, w, h = region()
+    g = FilterGraph((x, y, w, h))  # distant / point / spot lighting on the source alpha
+    d = g.lighting(False, distant_light(_f(rng.uniform(0, 360)), _f(rng.uniform(10, 80))), 2.0, 1.0,
+                   in1=SOURCE_ALPHA)
+    s = g.lighting(True, point_light(_f(x + w / 2), _f(y + h / 2), _f(0.2 * w)), 2.0, 1.0, 12.0, in1=SOURCE_ALPHA)
+    ar = g.composite(abi.COMPOSITE_ARITHMETIC, k=(0.0, 1.0, 1.0, 0.0), in1=d, in2=s)
+    sp = g.lighting(False, spot_light(_f(x), _f(y), _f(0.3 * w), _f(x + w), _f(y + h), 0.0, 2.0, 35.0), 2.0, 1.0,
+                    in1=SOURCE_ALPHA)
+    g.blend("multiply", in1=sp, in2=ar)
+    out.append(g)
+    return out
+
+
+def full_filter_doc(seed: int = 0, size: int = 4096, n_paths: int = 2000, paths_per_filter: int = 25,
+                    max_segments: int = 200):
+    rng = np.random.default_rng(seed)
+    sc = Scene(size, size)
+    chains = _filter_chains(rng, size)
+    n_filtered = min(n_paths, paths_per_filter * len(chains))
+    # filter groups are spread evenly through the document order
+    starts = {int(i * n_paths / len(chains)): k for k, i in enumerate(range(len(chains)))}
+    i = 0
+    while i < n_paths:
+        parent = sc.root
+        count = 1
+        if i in starts and n_filtered > 0:
+            parent = sc.group(sc.root, filters=[chains[starts[i]]])
+            count = min(paths_per_filter, n_paths - i)
+        for _ in range(count):
+            nseg = int(rng.integers(3, max_segments + 1))
+            verbs, pts = _random_path(rng, size, nseg)
+            fill = _random_paint(rng, size, int(rng.integers(2, 9)))
+            rule = abi.FILL_EVENODD if rng.random() < 0.5 else abi.FILL_NONZERO
+            stroke, width, join = None, 1.0, abi.JOIN_MITER
+            if rng.random() < 0.4:
+                stroke = _random_paint(rng, size, int(rng.integers(2, 9)))
+                width = _f(rng.uniform(0.5, 8.0))
+                join = [abi.JOIN_ROUND, abi.JOIN_MITER, abi.JOIN_BEVEL][int(rng.integers(3))]
+            grp = sc.group(parent, opacity=_f(rng.uniform(0.2, 1.0)))
+            sc.path(grp, verbs, pts, fill=fill, fill_rule=rule, stroke=stroke, stroke_width=width, stroke_join=join)
+            i += 1
+    return sc, (size, size)
+
+
+def document_batch(n_docs: int, seed: int = 0, min_side: int = 64, max_side: int = 2048, min_paths: int = 5,
+                   max_paths: int = 200):
+    """Documents with log-uniform sides and aspect 0.5-2 (the batched workload)."""
+    rng = np.random.default_rng(seed)
+    docs = []
+    for d in range(n_docs):
+        side = float(np.exp(rng.uniform(np.log(min_side), np.log(max_side))))
+        aspect = float(np.exp(rng.uniform(np.log(0.5), np.log(2.0))))
+        w = int(max(min_side, min(max_side, round(side * np.sqrt(aspect)))))
+        h = int(max(min_side, min(max_side, round(side / np.sqrt(aspect)))))
+        n = int(rng.integers(min_paths, max_paths + 1))
+        sc, _ = full_filter_doc(seed=int(rng.integers(0, 2**31)), size=min(w, h), n_paths=n,
+                                paths_per_filter=max(1, n // 16), max_segments=64)
+        docs.append((sc, (w, h)))
+    return docs

@@ -25,12 +25,32 @@ NAMES = [
     "feTurbulence/type=fractalNoise", "feTurbulence/no-attributes", "feTurbulence/seed=20",
     "feTurbulence/stitchTiles=stitch", "feTurbulence/numOctaves=5", "feTurbulence/baseFrequency=0.05-0.01",
 ]
+# fixtures outside filters/ (paths relative to crates/resvg/tests/tests)
+OTHER = [
+    "painting/fill-rule/evenodd", "painting/fill-rule/nonzero",
+    "paint-servers/linearGradient/default-attributes", "paint-servers/linearGradient/gradientUnits=userSpaceOnUse",
+    "paint-servers/linearGradient/spreadMethod=reflect", "paint-servers/linearGradient/spreadMethod=repeat",
+    "paint-servers/linearGradient/many-stops", "paint-servers/linearGradient/gradientTransform",
+    "paint-servers/radialGradient/default-attributes", "paint-servers/radialGradient/gradientUnits=userSpaceOnUse",
+    "paint-servers/radialGradient/spreadMethod=reflect", "paint-servers/radialGradient/fr=0.5",
+    "paint-servers/radialGradient/fr=0.2", "paint-servers/radialGradient/fr=0.7",
+    "painting/opacity/50percent",
+    "painting/mix-blend-mode/multiply", "painting/mix-blend-mode/screen", "painting/mix-blend-mode/hue",
+    "painting/mix-blend-mode/darken", "painting/mix-blend-mode/color-burn", "painting/mix-blend-mode/normal",
+    "shapes/circle/simple-case", "shapes/ellipse/simple-case", "shapes/rect/rounded-rect",
+]
 
 
 def main() -> int:
     DST.mkdir(exist_ok=True)
     for n in NAMES:
         src = SRC / f"{n}.png"
+        if not src.exists():
+            print("missing", src, file=sys.stderr)
+            return 1
+        shutil.copyfile(src, DST / (n.replace("/", "__") + ".png"))
+    for n in OTHER:
+        src = SRC.parent / f"{n}.png"
         if not src.exists():
             print("missing", src, file=sys.stderr)
             return 1

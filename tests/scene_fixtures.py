@@ -1,0 +1,292 @@
+"""Reference fixtures restated as scenes for the render walk (rvk_render / the oracle).
+
+Each entry rebuilds the usvg tree of a fixture SVG from crates/resvg/tests/tests
+(rects become M/L/L/L/Z paths, objectBoundingBox gradients get the bbox transform
+usvg computes in paint_server.rs:947-1000, elements with opacity or mix-blend-mode
+become groups) and renders it at 300 px (scale 1.5).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from resvg_amd import abi
+from resvg_amd.scene import PaintSpec, Scene, rect_path
+
+from fixture_scenes import fixtures as filter_fixtures
+
+SCALE = 1.5
+SIZE = 300
+GREEN = (0, 128, 0)
+WHITE = (255, 255, 255)
+BLACK = (0, 0, 0)
+RED = (255, 0, 0)
+YELLOW = (255, 255, 0)
+
+R = (20.0, 20.0, 160.0, 160.0)
+
+
+def _bbox_ts(x, y, w, h, inner=(1.0, 0.0, 0.0, 1.0, 0.0, 0.0)):
+    """Transform::from_bbox(bbox) post-concatenated after the gradient transform."""
+    a, b, c, d, e, f = inner
+    return (w * a, h * b, w * c, h * d, w * e + x, h * f + y)
+
+
+def _rect_scene(paint, rect=R):
+    sc = Scene(200, 200)
+    v, p = rect_path(*rect)
+    sc.path(sc.root, v, p, fill=paint)
+    return sc
+
+
+def _star(rule):
+    sc = Scene(200, 200)
+    pts = [(100, 20), (150, 180), (20, 80), (180, 80), (50, 180)]
+    verbs = [abi.VERB_MOVE] + [abi.VERB_LINE] * 4 + [abi.VERB_CLOSE]
+    sc.path(sc.root, verbs, pts, fill=PaintSpec.solid(GREEN), fill_rule=rule)
+    return sc
+
+
+def _wb(offsets=(0.0, 1.0)):
+    return [(offsets[0], WHITE, 1.0), (offsets[1], BLACK, 1.0)]
+
+
+def _mix(mode, color=YELLOW):
+    sc = Scene(200, 200)
+    v, p = rect_path(20.0, 20.0, 140.0, 140.0)
+    sc.path(sc.root, v, p, fill=PaintSpec.solid(GREEN))
+    g = sc.group(sc.root, blend=mode)
+    v, p = rect_path(40.0, 40.0, 140.0, 140.0)
+    stops = [(0.0, WHITE, 0.0), (1.0, color, 1.0)]
+    sc.path(g, v, p, fill=PaintSpec.linear(0.0, 0.0, 1.0, 0.0, stops, transform=_bbox_ts(40.0, 40.0, 140.0, 140.0)))
+    return sc
+
+
+def shape_fixtures():
+    cos30, sin30 = math.cos(math.radians(30)), math.sin(math.radians(30))
+    rot = (float(np.float32(cos30)), float(np.float32(sin30)), float(np.float32(-sin30)), float(np.float32(cos30)),
+           0.0, 0.0)
+    bb = _bbox_ts(*R)
+    out = {
+        "painting/fill-rule/evenodd": _star(abi.FILL_EVENODD),
+        "painting/fill-rule/nonzero": _star(abi.FILL_NONZERO),
+        "paint-servers/linearGradient/default-attributes": _rect_scene(PaintSpec.linear(0, 0, 1, 0, _wb(),
+                                                                                         transform=bb)),
+        "paint-servers/linearGradient/gradientUnits=userSpaceOnUse": _rect_scene(PaintSpec.linear(0, 0, 200, 0,
+                                                                                                   _wb())),
+        "paint-servers/linearGradient/spreadMethod=reflect": _rect_scene(
+            PaintSpec.linear(0.4, 0, 0.6, 0, _wb(), spread=abi.SPREAD_REFLECT, transform=bb)),
+        "paint-servers/linearGradient/spreadMethod=repeat": _rect_scene(
+            PaintSpec.linear(0.4, 0, 0.6, 0, _wb(), spread=abi.SPREAD_REPEAT, transform=bb)),
+        "paint-servers/linearGradient/many-stops": _rect_scene(PaintSpec.linear(
+            0, 0, 1, 0, [(0.0, WHITE, 1.0), (0.25, BLACK, 1.0), (0.5, RED, 1.0), (0.9, GREEN, 1.0)], transform=bb)),
+        "paint-servers/linearGradient/gradientTransform": _rect_scene(
+            PaintSpec.linear(0, 0, 1, 0, _wb(), transform=_bbox_ts(*R, inner=rot))),
+        "paint-servers/radialGradient/default-attributes": _rect_scene(PaintSpec.radial(0.5, 0.5, 0.5, _wb(),
+                                                                                         transform=bb)),
+        "paint-servers/radialGradient/gradientUnits=userSpaceOnUse": _rect_scene(PaintSpec.radial(100, 100, 100,
+                                                                                                   _wb())),
+        "paint-servers/radialGradient/spreadMethod=reflect": _rect_scene(
+            PaintSpec.radial(0.5, 0.5, 0.5, _wb(), spread=abi.SPREAD_REFLECT, transform=bb)),
+    }
+    for fr in ("0.2", "0.5", "0.7"):
+        out[f"paint-servers/radialGradient/fr={fr}"] = _rect_scene(
+            PaintSpec.radial(0.5, 0.5, 0.5, [(0.0, BLACK, 1.0), (1.0, WHITE, 1.0)], fr=float(fr), transform=bb))
+    sc = Scene(200, 200)
+    g = sc.group(sc.root, opacity=0.5)
+    v, p = rect_path(*R)
+    sc.path(g, v, p, fill=PaintSpec.solid(GREEN))
+    out["painting/opacity/50percent"] = sc
+    out.update(_curve_fixtures())
+    for mode, name, col in [("multiply", "multiply", YELLOW), ("screen", "screen", (0, 0, 255)),
+                            ("hue", "hue", YELLOW), ("darken", "darken", YELLOW),
+                            ("color_burn", "color-burn", YELLOW), ("normal", "normal", YELLOW)]:
+        out[f"painting/mix-blend-mode/{name}"] = _mix(mode, col)
+    return out
+
+
+# ---- SVG arcs as usvg emits them (shapes.rs:290-330 via kurbo 0.13.1) ----
+# kurbo is a Cargo.lock dependency that the reference does not vendor; this restates
+# its published Arc::from_svg_arc (SVG implementation notes F.6.5/F.6.6) and
+# Arc::append_iter (n = ceil(max(3.999999, (1.1163 * r / tol)^(1/6)) * sweep / 2pi)
+# cubics with arm length 4/3 tan(step / 4)), in f64 with the f32 casts of the caller.
+
+def _svg_arc_cubics(p0, rx, ry, large_arc, sweep, to, tol=0.1):
+    x0, y0 = p0
+    x1, y1 = to
+    if (x0, y0) == (x1, y1) or rx == 0 or ry == 0:
+        return None
+    rx, ry = abs(rx), abs(ry)
+    hd_x, hd_y = (x0 - x1) * 0.5, (y0 - y1) * 0.5
+    hs_x, hs_y = (x0 + x1) * 0.5, (y0 + y1) * 0.5
+    px, py = hd_x, hd_y  # x_rotation 0: cos 1, sin 0
+    rf = px * px / (rx * rx) + py * py / (ry * ry)
+    if rf > 1.0:
+        rx *= math.sqrt(rf)
+        ry *= math.sqrt(rf)
+    rxry, rxpy, rypx = rx * ry, rx * py, ry * px
+    sum_sq = rxpy * rxpy + rypx * rypx
+    coe = (-1.0 if large_arc == sweep else 1.0) * math.sqrt(abs((rxry * rxry - sum_sq) / sum_sq))
+    tcx, tcy = coe * rxpy / ry, -coe * rypx / rx
+    cx, cy = tcx + hs_x, tcy + hs_y
+    start = math.atan2((py - tcy) / ry, (px - tcx) / rx)
+    end = math.atan2((-py - tcy) / ry, (-px - tcx) / rx)
+    sweep_angle = math.fmod(end - start, 2.0 * math.pi)
+    if sweep and sweep_angle < 0.0:
+        sweep_angle += 2.0 * math.pi
+    elif not sweep and sweep_angle > 0.0:
+        sweep_angle -= 2.0 * math.pi
+    sign = math.copysign(1.0, sweep_angle)
+    n_err = max((1.1163 * (max(rx, ry) / tol)) ** (1.0 / 6.0), 3.999999)
+    n = math.ceil(n_err * abs(sweep_angle) * (1.0 / (2.0 * math.pi)))
+    step = sweep_angle / n
+    arm = (4.0 / 3.0) * abs(math.tan(step / 4.0)) * sign
+
+    def sample(a):
+        return rx * math.cos(a), ry * math.sin(a)
+
+    out = []
+    a0 = start
+    q0 = sample(a0)
+    for _ in range(int(n)):
+        a1 = a0 + step
+        t0 = sample(a0 + math.pi / 2.0)
+        q3 = sample(a1)
+        t1 = sample(a1 + math.pi / 2.0)
+        q1 = (q0[0] + arm * t0[0], q0[1] + arm * t0[1])
+        q2 = (q3[0] - arm * t1[0], q3[1] - arm * t1[1])
+        out.append([(cx + q1[0], cy + q1[1]), (cx + q2[0], cy + q2[1]), (cx + q3[0], cy + q3[1])])
+        a0, q0 = a1, q3
+    return out
+
+
+class _PathBuilder:
+    def __init__(self):
+        self.verbs, self.pts = [], []
+
+    def move_to(self, x, y):
+        self.verbs.append(abi.VERB_MOVE)
+        self.pts.append((x, y))
+
+    def line_to(self, x, y):
+        self.verbs.append(abi.VERB_LINE)
+        self.pts.append((x, y))
+
+    def cubic_to(self, *c):
+        self.verbs.append(abi.VERB_CUBIC)
+        self.pts += [(c[0], c[1]), (c[2], c[3]), (c[4], c[5])]
+
+    def arc_to(self, rx, ry, x, y):  # large_arc false, sweep true, as usvg's shapes use
+        p0 = tuple(float(np.float32(v)) for v in self.pts[-1])
+        cubics = _svg_arc_cubics(p0, rx, ry, False, True, (x, y))
+        if cubics is None:
+            self.line_to(x, y)
+            return
+        for c in cubics:
+            self.verbs.append(abi.VERB_CUBIC)
+            self.pts += c
+
+    def close(self):
+        self.verbs.append(abi.VERB_CLOSE)
+
+
+def ellipse_path_usvg(cx, cy, rx, ry):
+    """usvg shapes.rs:265-274 ellipse_to_path."""
+    b = _PathBuilder()
+    b.move_to(cx + rx, cy)
+    b.arc_to(rx, ry, cx, cy + ry)
+    b.arc_to(rx, ry, cx - rx, cy)
+    b.arc_to(rx, ry, cx, cy - ry)
+    b.arc_to(rx, ry, cx + rx, cy)
+    b.close()
+    return b.verbs, b.pts
+
+
+def rounded_rect_usvg(x, y, w, h, rx, ry):
+    """usvg shapes.rs:107-122 convert_rect with rx/ry."""
+    b = _PathBuilder()
+    b.move_to(x + rx, y)
+    b.line_to(x + w - rx, y)
+    b.arc_to(rx, ry, x + w, y + ry)
+    b.line_to(x + w, y + h - ry)
+    b.arc_to(rx, ry, x + w - rx, y + h)
+    b.line_to(x + rx, y + h)
+    b.arc_to(rx, ry, x, y + h - ry)
+    b.line_to(x, y + ry)
+    b.arc_to(rx, ry, x + rx, y)
+    b.close()
+    return b.verbs, b.pts
+
+
+def _curve_fixtures():
+    """shapes/{circle,ellipse}/simple-case and shapes/rect/rounded-rect: a red path
+    of explicit cubics under the green shape."""
+    out = {}
+    sc = Scene(200, 200)
+    b = _PathBuilder()
+    b.move_to(180, 100)
+    b.cubic_to(180, 55.82, 144.18, 20, 100, 20)
+    b.cubic_to(55.82, 20, 20, 55.82, 20, 100)
+    b.cubic_to(20, 144.18, 55.82, 180, 100, 180)
+    b.cubic_to(144.18, 180, 180, 144.18, 180, 100)
+    b.close()
+    sc.path(sc.root, b.verbs, b.pts, fill=PaintSpec.solid(RED))
+    v, p = ellipse_path_usvg(100.0, 100.0, 80.0, 80.0)
+    sc.path(sc.root, v, p, fill=PaintSpec.solid(GREEN))
+    out["shapes/circle/simple-case"] = sc
+
+    sc = Scene(200, 200)
+    b = _PathBuilder()
+    b.move_to(170, 100)
+    b.cubic_to(170, 66.86, 134.18, 40, 90, 40)
+    b.cubic_to(45.82, 40, 10, 66.86, 10, 100)
+    b.cubic_to(10, 133.14, 45.82, 160, 90, 160)
+    b.cubic_to(134.18, 160, 170, 133.14, 170, 100)
+    b.close()
+    sc.path(sc.root, b.verbs, b.pts, fill=PaintSpec.solid(RED))
+    v, p = ellipse_path_usvg(90.0, 100.0, 80.0, 60.0)
+    sc.path(sc.root, v, p, fill=PaintSpec.solid(GREEN))
+    out["shapes/ellipse/simple-case"] = sc
+
+    sc = Scene(200, 200)
+    b = _PathBuilder()
+    b.move_to(40, 20)
+    b.line_to(160, 20)
+    b.cubic_to(165.52, 20, 170.24, 21.95, 174.14, 25.86)
+    b.cubic_to(178.05, 29.76, 180, 34.48, 180, 40)
+    b.line_to(180, 160)
+    b.cubic_to(180, 165.52, 178.05, 170.24, 174.14, 174.14)
+    b.cubic_to(170.24, 178.05, 165.52, 180, 160, 180)
+    b.line_to(40, 180)
+    b.cubic_to(34.48, 180, 29.76, 178.05, 25.86, 174.14)
+    b.cubic_to(21.95, 170.24, 20, 165.52, 20, 160)
+    b.line_to(20, 40)
+    b.cubic_to(20, 36.43, 20.89, 33.09, 22.68, 30)
+    b.cubic_to(24.47, 26.91, 26.91, 24.47, 30, 22.68)
+    b.cubic_to(33.09, 20.89, 36.43, 20, 40, 20)
+    sc.path(sc.root, b.verbs, b.pts, fill=PaintSpec.solid(RED))
+    v, p = rounded_rect_usvg(20.0, 20.0, 160.0, 160.0, 20.0, 20.0)
+    sc.path(sc.root, v, p, fill=PaintSpec.solid(GREEN))
+    out["shapes/rect/rounded-rect"] = sc
+    return out
+
+
+def filter_scene(fx) -> Scene:
+    """A filter fixture (fixture_scenes.Fixture) as a scene: a filtered group with a rect."""
+    sc = Scene(200, 200)
+    g = sc.group(sc.root, filters=[fx.graph])
+    v, p = rect_path(*fx.rect)
+    sc.path(g, v, p, fill=PaintSpec.solid(fx.color))
+    return sc
+
+
+def all_scene_fixtures():
+    out = dict(shape_fixtures())
+    for fx in filter_fixtures():
+        out[fx.name] = filter_scene(fx)
+    return out
+
+
+def render_ts():
+    return abi.Transform(SCALE, 0.0, 0.0, SCALE, 0.0, 0.0)

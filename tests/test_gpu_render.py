@@ -1,0 +1,178 @@
+"""rvk_render (the product's device pipeline) against the oracle's render walk,
+bit-exact on every channel, on the reference's fixture scenes and on seeded
+random scenes that exercise the rasterizer, shaders, groups and filters."""
+import numpy as np
+import pytest
+
+from oracle import bindings as orc
+from resvg_amd import abi
+from resvg_amd.scene import PaintSpec, Scene, ellipse_path, rect_path
+from resvg_amd.workloads import blur_offset_composite_doc, full_filter_doc
+
+from fixture_scenes import compare_png, load_png
+from scene_fixtures import all_scene_fixtures, render_ts
+
+pytestmark = pytest.mark.gpu
+
+SCENES = all_scene_fixtures()
+
+
+def _gpu(ctx, scene, ts, w, h, base=None):
+    import torch
+    px = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda") if base is None else \
+        torch.from_numpy(base.copy()).cuda()
+    ctx.render(scene, ts, px)
+    ctx.synchronize()
+    return px.cpu().numpy()
+
+
+def _oracle(scene, ts, w, h, base=None):
+    px = np.zeros((h, w, 4), np.uint8) if base is None else base.copy()
+    orc.render(scene.to_abi().abi, ts, px)
+    return px
+
+
+def _assert_same(got, exp, what):
+    diff = np.abs(got.astype(np.int32) - exp.astype(np.int32))
+    if diff.any():
+        ys, xs = np.nonzero(diff.max(-1))
+        hist = np.bincount(diff.reshape(-1), minlength=2)[1:]
+        raise AssertionError(f"{what}: {len(ys)} pixels differ, max {diff.max()}, histogram {hist[:8].tolist()}, "
+                             f"first at (y={ys[0]}, x={xs[0]}) got {got[ys[0], xs[0]]} exp {exp[ys[0], xs[0]]}")
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_scene_fixture(ctx, name):
+    got = _gpu(ctx, SCENES[name], render_ts(), 300, 300)
+    _assert_same(got, _oracle(SCENES[name], render_ts(), 300, 300), name)
+    assert compare_png(got, load_png(name)) == 0
+
+
+# ---------------------------------------------------------------------------
+# seeded random scenes
+# ---------------------------------------------------------------------------
+def _stops(rng, n, opaque=True):
+    offs = np.sort(rng.uniform(0, 1, n)).astype(np.float32)
+    return [(float(o), tuple(int(c) for c in rng.integers(0, 256, 3)), 1.0 if opaque else float(rng.uniform(0, 1)))
+            for o in offs]
+
+
+def _paint(rng, size, kind=None):
+    kind = kind if kind is not None else int(rng.integers(0, 3))
+    op = 1.0 if rng.random() < 0.6 else float(np.float32(rng.uniform(0.1, 1.0)))
+    spread = int(rng.integers(0, 3))
+    if kind == 0:
+        return PaintSpec.solid(tuple(int(c) for c in rng.integers(0, 256, 3)), opacity=op)
+    opaque = rng.random() < 0.6
+    stops = _stops(rng, int(rng.integers(2, 7)), opaque)
+    if kind == 1:
+        e = rng.uniform(0, size, 4).astype(np.float32)
+        return PaintSpec.linear(*map(float, e), stops, opacity=op, spread=spread)
+    c = rng.uniform(0, size, 2).astype(np.float32)
+    fr = float(np.float32(rng.uniform(0, 0.3) * size)) if rng.random() < 0.3 else 0.0
+    return PaintSpec.radial(float(c[0]), float(c[1]), float(np.float32(rng.uniform(0.1, 0.7) * size)), stops,
+                            opacity=op, spread=spread, fr=fr)
+
+
+def _random_path(rng, size, nseg, kinds=(abi.VERB_LINE, abi.VERB_QUAD, abi.VERB_CUBIC), closed=True, margin=0.0):
+    verbs = [abi.VERB_MOVE]
+    n = 1
+    for _ in range(nseg):
+        v = kinds[int(rng.integers(len(kinds)))]
+        verbs.append(v)
+        n += {abi.VERB_LINE: 1, abi.VERB_QUAD: 2, abi.VERB_CUBIC: 3}[v]
+    if closed:
+        verbs.append(abi.VERB_CLOSE)
+    pts = rng.uniform(-margin * size, (1 + margin) * size, (n, 2)).astype(np.float32)
+    return verbs, pts
+
+
+def _random_scene(seed, size, n_paths, strokes=True, groups=True, aa_off=False, margin=0.0):
+    rng = np.random.default_rng(seed)
+    sc = Scene(size, size)
+    for _ in range(n_paths):
+        parent = sc.root
+        if groups and rng.random() < 0.4:
+            blend = ["normal", "multiply", "screen", "overlay", "darken", "difference", "hue", "luminosity"][
+                int(rng.integers(8))]
+            parent = sc.group(sc.root, opacity=float(np.float32(rng.uniform(0.2, 1.0))), blend=blend)
+        verbs, pts = _random_path(rng, size, int(rng.integers(1, 12)), margin=margin)
+        stroke = _paint(rng, size) if strokes and rng.random() < 0.5 else None
+        sc.path(parent, verbs, pts, fill=_paint(rng, size) if rng.random() < 0.9 else None,
+                fill_rule=abi.FILL_EVENODD if rng.random() < 0.5 else abi.FILL_NONZERO, stroke=stroke,
+                stroke_width=float(np.float32(rng.uniform(0.3, 12.0))),
+                stroke_join=[abi.JOIN_MITER, abi.JOIN_ROUND, abi.JOIN_BEVEL][int(rng.integers(3))],
+                stroke_cap=[abi.CAP_BUTT, abi.CAP_ROUND, abi.CAP_SQUARE][int(rng.integers(3))],
+                miter_limit=float(np.float32(rng.uniform(1.0, 10.0))),
+                anti_alias=not (aa_off and rng.random() < 0.5), stroke_first=rng.random() < 0.3)
+    return sc
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_fills(ctx, seed):
+    sc = _random_scene(seed, 160, 12, strokes=False, groups=False)
+    ts = abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, 160, 160), _oracle(sc, ts, 160, 160), f"fills seed {seed}")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_strokes_and_groups(ctx, seed):
+    sc = _random_scene(100 + seed, 200, 16, aa_off=True)
+    ts = abi.Transform(1.25, 0.1, -0.2, 0.9, 7.0, -3.0)
+    _assert_same(_gpu(ctx, sc, ts, 220, 190), _oracle(sc, ts, 220, 190), f"strokes/groups seed {seed}")
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_paths_beyond_the_pixmap(ctx, seed):
+    sc = _random_scene(200 + seed, 128, 10, margin=0.5)
+    ts = abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, 128, 128), _oracle(sc, ts, 128, 128), f"margins seed {seed}")
+
+
+def test_composites_onto_existing_pixels(ctx):
+    rng = np.random.default_rng(7)
+    base = rng.integers(0, 256, (96, 128, 4)).astype(np.uint8)
+    base[..., :3] = np.minimum(base[..., :3], base[..., 3:4])  # premultiplied
+    sc = _random_scene(300, 128, 8)
+    ts = abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, 128, 96, base), _oracle(sc, ts, 128, 96, base), "onto base")
+
+
+def test_nested_and_multi_child_groups(ctx):
+    rng = np.random.default_rng(11)
+    sc = Scene(256, 256)
+    g1 = sc.group(sc.root, opacity=0.7, blend="multiply")
+    for _ in range(3):
+        v, p = _random_path(rng, 256, 6)
+        sc.path(g1, v, p, fill=_paint(rng, 256))
+    g2 = sc.group(g1, transform=(0.8, 0.1, -0.1, 0.9, 20.0, 10.0), opacity=0.5, blend="screen", isolate=True)
+    v, p = ellipse_path(120.0, 110.0, 70.0, 40.0)
+    sc.path(g2, v, p, fill=_paint(rng, 256, 1), stroke=_paint(rng, 256, 0), stroke_width=5.0)
+    g3 = sc.group(sc.root, blend="color_dodge")
+    v, p = rect_path(30.0, 40.0, 150.0, 120.0)
+    sc.path(g3, v, p, fill=_paint(rng, 256, 2))
+    ts = abi.Transform(1.1, 0.0, 0.0, 1.1, -5.0, 3.0)
+    _assert_same(_gpu(ctx, sc, ts, 256, 256), _oracle(sc, ts, 256, 256), "nested groups")
+
+
+def test_blur_offset_composite_document(ctx):
+    sc, (w, h) = blur_offset_composite_doc(seed=0, size=512)
+    ts = abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, w, h), _oracle(sc, ts, w, h), "blur-offset-composite 512")
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_small_full_filter_document(ctx, seed):
+    sc, (w, h) = full_filter_doc(seed=seed, size=384, n_paths=32, paths_per_filter=3, max_segments=24)
+    ts = abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, w, h), _oracle(sc, ts, w, h), f"full-filter 384 seed {seed}")
+
+
+def test_render_host_matches_device(ctx):
+    import resvg_amd
+    sc = _random_scene(400, 150, 10)
+    ts = abi.Transform.identity()
+    host = np.zeros((150, 150, 4), np.uint8)
+    ctx.render_host(sc, ts, host)
+    _assert_same(host, _gpu(ctx, sc, ts, 150, 150), "render_host")
+    assert resvg_amd.lib() is not None

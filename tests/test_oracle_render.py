@@ -1,0 +1,76 @@
+"""Pins the oracle's render walk and rasterizer against the reference's PNG fixtures
+(CPU only), plus coverage invariants that the mathematics fixes."""
+import numpy as np
+import pytest
+
+from oracle import bindings as orc
+from resvg_amd import abi
+from resvg_amd.scene import PaintSpec, Scene, ellipse_path, rect_path
+
+from fixture_scenes import compare_png, load_png
+from scene_fixtures import all_scene_fixtures, render_ts
+
+SCENES = all_scene_fixtures()
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_scene_fixture_png(name):
+    px = np.zeros((300, 300, 4), np.uint8)
+    orc.render(SCENES[name].to_abi().abi, render_ts(), px)
+    assert compare_png(px, load_png(name)) == 0
+
+
+def _cov(sc, w, h, ts=None, stroke=False):
+    return orc.path_coverage(sc.to_abi().abi, 0, ts or abi.Transform.identity(), stroke, w, h)
+
+
+def test_pixel_aligned_rect_coverage_is_exact():
+    sc = Scene(32, 32)
+    v, p = rect_path(4.0, 6.0, 10.0, 7.0)
+    sc.path(sc.root, v, p, fill=PaintSpec.solid((0, 0, 0)))
+    cov = _cov(sc, 32, 32)
+    exp = np.zeros((32, 32), np.uint8)
+    exp[6:13, 4:14] = 255
+    assert np.array_equal(cov, exp)
+
+
+def test_half_pixel_edge_is_half_coverage():
+    # an edge at x = 4.5: two of four supersampled columns -> 16*2 per sub-scanline
+    sc = Scene(16, 16)
+    v, p = rect_path(4.5, 2.0, 5.0, 4.0)
+    sc.path(sc.root, v, p, fill=PaintSpec.solid((0, 0, 0)))
+    cov = _cov(sc, 16, 16)
+    assert (cov[2:6, 4] == 128).all() and (cov[2:6, 9] == 128).all()
+    assert (cov[2:6, 5:9] == 255).all()
+
+
+def test_evenodd_vs_nonzero_on_a_doubly_wound_square():
+    sc = Scene(24, 24)
+    verbs = [abi.VERB_MOVE, abi.VERB_LINE, abi.VERB_LINE, abi.VERB_LINE, abi.VERB_CLOSE] * 2
+    sq = [(2, 2), (20, 2), (20, 20), (2, 20)]
+    for rule in (abi.FILL_NONZERO, abi.FILL_EVENODD):
+        sc = Scene(24, 24)
+        sc.path(sc.root, verbs, sq + sq, fill=PaintSpec.solid((0, 0, 0)), fill_rule=rule)
+        cov = _cov(sc, 24, 24)
+        if rule == abi.FILL_NONZERO:
+            assert (cov[2:20, 2:20] == 255).all()
+        else:
+            assert not cov.any()
+
+
+def test_circle_coverage_area_matches_pi_r2():
+    sc = Scene(64, 64)
+    v, p = ellipse_path(32.0, 32.0, 20.0, 20.0)
+    sc.path(sc.root, v, p, fill=PaintSpec.solid((0, 0, 0)))
+    cov = _cov(sc, 64, 64).astype(np.float64) / 255.0
+    assert abs(cov.sum() - np.pi * 400) / (np.pi * 400) < 0.01
+    # symmetry of the 4x4 supersampling grid under a vertical flip about the centre row pair
+    assert np.abs(cov - cov[::-1, :]).max() <= 0.07
+
+
+def test_stroke_outline_covers_line_width():
+    sc = Scene(40, 40)
+    sc.path(sc.root, [abi.VERB_MOVE, abi.VERB_LINE], [(5, 20), (35, 20)], stroke=PaintSpec.solid((0, 0, 0)),
+            stroke_width=4.0)
+    cov = _cov(sc, 40, 40, stroke=True)
+    assert (cov[18:22, 5:35] == 255).all() and not cov[:18].any() and not cov[22:].any()
