@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Throughput of resvg's pixel pipeline (rvk_render, the resvg::render twin) on one
+MI355X, on the 4096x4096 full-filter scene.
+
+One step renders the whole seeded document onto a cleared 4096x4096 RGBA8
+premultiplied device pixmap: 2000 paths (3-200 cubic/quad segments, evenodd and
+nonzero, 40% also stroked) with 2-8 stop linear/radial gradients, each in its own
+opacity group, and 8 filter regions covering blur (IIR and box), morphology,
+convolution, colour matrix, component transfer, turbulence, lighting and blend.
+The scene tree is built before the timed region (usvg's job); host-side edge
+building, uploads and every kernel are inside it.
+
+Prints one JSON line: Mpixel/s of the document, the roofline of the dominant kernel
+(algorithmic bytes / its HIP-event time over the timed steps, against 8 TB/s HBM)
+and the oracle's CPU throughput on a bounded sample of the same document.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def _workload(size, paths, seed):
+    from resvg_amd.workloads import full_filter_doc
+    return full_filter_doc(seed=seed, size=size, n_paths=paths)
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the oracle (a scalar C++ restatement of the reference path) on a
+# stratified sample of the same document's top-level items.  resvg sizes a group's
+# layer by the group's bounds (up to 5x the canvas), not by the visible canvas, so a
+# crop of the canvas does not shrink the work; a subset of the items does.
+# ---------------------------------------------------------------------------
+class _Keep:
+    """Keeps filter group number `fg` (-1: none) and, of the plain items, every
+    `stride`-th one, dealt round-robin over `parts` jobs (this is job `part`)."""
+
+    def __init__(self, fg, stride, parts, part):
+        self.fg, self.stride, self.parts, self.part = fg, stride, parts, part
+        self.nf = self.np = 0
+
+    def __call__(self, item, is_filter):
+        if is_filter:
+            self.nf += 1
+            return self.nf - 1 == self.fg
+        self.np += 1
+        k = self.np - 1
+        return k % self.stride == 0 and (k // self.stride) % self.parts == self.part
+
+
+def _cpu_job(job):
+    size, paths, seed, fg, stride, parts, part = job
+    import numpy as np
+    from oracle import bindings as orc
+    from resvg_amd import abi
+    from resvg_amd.workloads import full_filter_doc
+    sc, _ = full_filter_doc(seed=seed, size=size, n_paths=paths, keep=_Keep(fg, stride, parts, part))
+    s = sc.to_abi()
+    px = np.zeros((size, size, 4), np.uint8)
+    t = time.perf_counter()
+    orc.render(s.abi, abi.Transform.identity(), px)
+    return time.perf_counter() - t, len(sc.paths)
+
+
+def cpu_baseline(size, paths, seed, workers, plain_stride=64, filter_group=1, n_filter_groups=8):
+    """One filter group (of 8) on one core and every `plain_stride`-th plain path spread
+    over the other cores; the whole-document time per core is estimated as
+    8 x (filter group time) + plain_stride x (sum of the plain jobs' times)."""
+    from oracle import bindings as orc
+    orc.lib()  # builds the oracle if it is missing
+    parts = max(1, workers - 1)
+    jobs = [(size, paths, seed, filter_group, 1 << 30, 1, 1)]  # the filter group alone
+    jobs += [(size, paths, seed, -1, plain_stride, parts, p) for p in range(parts)]
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(min(workers, len(jobs))) as pool:
+        res = pool.map(_cpu_job, jobs)
+    wall = time.perf_counter() - t0
+    t_filter = res[0][0]
+    t_plain = sum(r[0] for r in res[1:])
+    n_plain = sum(r[1] for r in res[1:])
+    t_doc = n_filter_groups * t_filter + plain_stride * t_plain
+    return {
+        "value": round(workers * size * size / 1e6 / t_doc, 4),
+        "unit": "Mpixel/s",
+        "cores": workers,
+        "kind": "port",
+        "sample": f"oracle (scalar C++ restatement, one document per core) on a stratified sample of the same "
+                  f"{size}x{size} {paths}-path document: filter group {filter_group} of {n_filter_groups} "
+                  f"({res[0][1]} paths, {t_filter:.1f} s) and every {plain_stride}th plain path ({n_plain} paths, "
+                  f"{t_plain:.1f} s over {parts} cores); estimated {t_doc:.0f} s per document per core; "
+                  f"wall {wall:.1f} s",
+    }
+
+
+# ---------------------------------------------------------------------------
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--paths", type=int, default=2000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-stride", type=int, default=64, help="CPU sample: every n-th plain path")
+    ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline processes (0: min(16, cores))")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        workers = args.cpu_workers or max(1, min(16, len(os.sched_getaffinity(0))))
+        print(f"[bench] cpu baseline: {workers} workers", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.size, args.paths, args.seed, workers, plain_stride=args.cpu_stride)
+        print(f"[bench] cpu baseline {cpu['value']} Mpixel/s", file=sys.stderr, flush=True)
+
+    import torch
+    import resvg_amd
+    from resvg_amd import abi
+
+    print(f"[bench] building the {args.size}x{args.size} document ({args.paths} paths)", file=sys.stderr, flush=True)
+    sc, (w, h) = _workload(args.size, args.paths, args.seed)
+    scene = sc.to_abi()
+    ts = abi.Transform.identity()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    ctx = resvg_amd.Context(0)
+    px = torch.zeros((h, w, 4), dtype=torch.uint8, device=dev)
+
+    def step():
+        px.zero_()
+        ctx.render(scene, ts, px)
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        print(f"[bench] warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
+
+    stream = torch.cuda.current_stream(dev)  # the context renders on torch's current stream
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ctx.profile_begin()
+    t_wall = time.perf_counter()
+    start.record(stream)
+    for i in range(args.steps):
+        step()
+        print(f"[bench] step {i + 1}/{args.steps} queued", file=sys.stderr, flush=True)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    kernels = ctx.profile_end()
+    ms = start.elapsed_time(stop) / args.steps
+    mpix = w * h / 1e6
+    value = mpix / (ms / 1e3)
+
+    dom = kernels[0] if kernels else None
+    roofline = None
+    if dom:
+        achieved = dom["bytes"] / (dom["total_ms"] / 1e3) / 1e9
+        traffic = None
+        tf = ROOT / "profiles" / "traffic.json"
+        if tf.exists():
+            rec = json.loads(tf.read_text()).get(dom["name"])
+            if rec and rec.get("workload") == f"full-filter {args.size} {args.paths} {args.seed}":
+                traffic = rec.get("bytes_per_launch")
+        roofline = {
+            "bound": "hbm",
+            "kernel": dom["name"],
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(dom["bytes"] / dom["launches"], 1),
+            "avg_launch_ms": round(dom["total_ms"] / dom["launches"], 4),
+            "kernel_share_of_step": round(dom["total_ms"] / (ms * args.steps), 4),
+        }
+
+    line = {
+        "metric": "Mpixel/s at 4096x4096 full-filter scene (resvg::render pixel pipeline)",
+        "value": round(value, 3),
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded document generator, resvg_amd/workloads.py)",
+        "config": {"workload": f"full-filter scene {args.size}x{args.size}, {args.paths} paths, 8 filter regions",
+                   "size": args.size, "paths": args.paths, "seed": args.seed},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()} for d in kernels[:8]],
+        "wall_s_per_step": round(t_wall / args.steps, 3),
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

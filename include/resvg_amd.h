@@ -345,6 +345,21 @@ typedef struct rvk_stats {
 } rvk_stats;
 rvk_status rvk_get_stats(rvk_context *ctx, rvk_stats *out);
 
+/* Kernel timing (no counterpart in the reference, which has no device kernels):
+ * between rvk_profile_begin and rvk_profile_end every kernel launch on the
+ * context's stream is bracketed by a pair of HIP events and charged with the
+ * algorithmic bytes it must move (the bytes of its inputs and outputs, each read or
+ * written once).  rvk_profile_end waits for the stream and returns one record per
+ * kernel name, sorted by total time (largest first). */
+typedef struct rvk_kernel_stat {
+    char name[40];
+    uint64_t launches;
+    double total_ms;
+    double bytes;
+} rvk_kernel_stat;
+rvk_status rvk_profile_begin(rvk_context *ctx);
+rvk_status rvk_profile_end(rvk_context *ctx, rvk_kernel_stat *out, uint32_t capacity, uint32_t *count);
+
 #ifdef __cplusplus
 }
 #endif

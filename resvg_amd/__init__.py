@@ -70,6 +70,8 @@ def _declare(L: C.CDLL) -> None:
         "rvk_filter_apply": ([P, C.POINTER(abi.Filter), C.POINTER(abi.FilterData), abi.Transform, P, u32, u32,
                               C.POINTER(i32)], i32),
         "rvk_get_stats": ([P, C.POINTER(abi.Stats)], i32),
+        "rvk_profile_begin": ([P], i32),
+        "rvk_profile_end": ([P, C.POINTER(abi.KernelStat), C.c_uint32, C.POINTER(C.c_uint32)], i32),
     }
     optional = {
         "rvk_render": ([P, C.POINTER(abi.Scene), abi.Transform, u32, u32, P], i32),
@@ -148,6 +150,19 @@ class Context:
         s = abi.Stats()
         _check(lib().rvk_get_stats(self._h, C.byref(s)), "rvk_get_stats")
         return s
+
+    def profile_begin(self) -> None:
+        """Start timing every kernel launch with HIP events on the context stream."""
+        _check(lib().rvk_profile_begin(self._h), "rvk_profile_begin")
+
+    def profile_end(self) -> list[dict]:
+        """Wait for the stream; per-kernel launches, total ms and algorithmic bytes."""
+        cap = 64
+        out = (abi.KernelStat * cap)()
+        n = C.c_uint32(0)
+        _check(lib().rvk_profile_end(self._h, out, cap, C.byref(n)), "rvk_profile_end")
+        return [dict(name=out[i].name.decode(), launches=int(out[i].launches), total_ms=float(out[i].total_ms),
+                     bytes=float(out[i].bytes)) for i in range(n.value)]
 
     # --- filter primitives (in place on a (H, W, 4) uint8 device tensor) ---
     def into_linear_rgb(self, px):

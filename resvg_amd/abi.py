@@ -203,6 +203,10 @@ class Stats(C.Structure):
                 ("filters", C.c_uint64), ("scratch_bytes", C.c_uint64)]
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 40), ("launches", C.c_uint64), ("total_ms", C.c_double), ("bytes", C.c_double)]
+
+
 def array(ctype, items):
     """A ctypes array from a sequence (keeps an explicit reference to the buffer)."""
     items = list(items)

@@ -46,14 +46,16 @@ rvk_status launch_draw_pixmap(rvk_context *ctx, uint8_t *dst, uint32_t dw, uint3
     size_t n = (size_t)rw * rh;
     size_t g = (n + 255) / 256;
     if (g > 65535u * 4u) g = 65535u * 4u;
-    if (mode == RVK_BLEND_SOURCE_OVER)
-        hipLaunchKernelGGL(k_draw<RVK_BLEND_SOURCE_OVER>, dim3((unsigned)g), dim3(256), 0, ctx->stream,
-                           (uint32_t *)dst, dw, (const uint32_t *)src, sw, x, y, (uint32_t)x0, (uint32_t)y0, rw, rh,
-                           opacity, mode);
-    else
-        hipLaunchKernelGGL(k_draw<RVK_BLEND_COUNT>, dim3((unsigned)g), dim3(256), 0, ctx->stream, (uint32_t *)dst, dw,
-                           (const uint32_t *)src, sw, x, y, (uint32_t)x0, (uint32_t)y0, rw, rh, opacity, mode);
-    ctx->stats.kernel_launches++;
+    // source read + destination read and write: 12 bytes per pixel of the rect
+    timed_launch(ctx, "draw_pixmap", 12.0 * (double)n, [&] {
+        if (mode == RVK_BLEND_SOURCE_OVER)
+            hipLaunchKernelGGL(k_draw<RVK_BLEND_SOURCE_OVER>, dim3((unsigned)g), dim3(256), 0, ctx->stream,
+                               (uint32_t *)dst, dw, (const uint32_t *)src, sw, x, y, (uint32_t)x0, (uint32_t)y0, rw,
+                               rh, opacity, mode);
+        else
+            hipLaunchKernelGGL(k_draw<RVK_BLEND_COUNT>, dim3((unsigned)g), dim3(256), 0, ctx->stream, (uint32_t *)dst,
+                               dw, (const uint32_t *)src, sw, x, y, (uint32_t)x0, (uint32_t)y0, rw, rh, opacity, mode);
+    });
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;
 }

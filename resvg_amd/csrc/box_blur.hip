@@ -189,8 +189,9 @@ rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
         if (rh <= kMaxFusedR && rv <= kMaxFusedR) {
             size_t lds = (size_t)((TW + 2 * rh) * (TH + 2 * rv) + (TW + 2 * rh) * TH) * 4;
             dim3 grid(div_up(w, TW), div_up(h, TH));
-            hipLaunchKernelGGL(k_box_round, grid, dim3(kThreads), lds, ctx->stream, a, b, w, h, rh, rv);
-            ctx->stats.kernel_launches++;
+            timed_launch(ctx, "box_blur_round", 8.0 * (double)w * h, [&] {
+                hipLaunchKernelGGL(k_box_round, grid, dim3(kThreads), lds, ctx->stream, a, b, w, h, rh, rv);
+            });
             if (hipGetLastError() != hipSuccess) { st = RVK_ERR_HIP; break; }
             std::swap(a, b);
         } else {
@@ -199,16 +200,18 @@ rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
             if (rv == 0) {
                 if (hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) { st = RVK_ERR_HIP; break; }
             } else {
-                hipLaunchKernelGGL(k_box_vert, dim3(div_up(w, 64), div_up(h, chunk)), dim3(64), 0, ctx->stream, a,
-                                   b, w, h, rv, chunk);
-                ctx->stats.kernel_launches++;
+                timed_launch(ctx, "box_blur_vert", 8.0 * (double)w * h, [&] {
+                    hipLaunchKernelGGL(k_box_vert, dim3(div_up(w, 64), div_up(h, chunk)), dim3(64), 0, ctx->stream,
+                                       a, b, w, h, rv, chunk);
+                });
             }
             if (rh == 0) {
                 if (hipMemcpyAsync(a, b, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) { st = RVK_ERR_HIP; break; }
             } else {
-                hipLaunchKernelGGL(k_box_horz, dim3(div_up(h, 64), div_up(w, chunk)), dim3(64), 0, ctx->stream, b,
-                                   a, w, h, rh, chunk);
-                ctx->stats.kernel_launches++;
+                timed_launch(ctx, "box_blur_horz", 8.0 * (double)w * h, [&] {
+                    hipLaunchKernelGGL(k_box_horz, dim3(div_up(h, 64), div_up(w, chunk)), dim3(64), 0, ctx->stream,
+                                       b, a, w, h, rh, chunk);
+                });
             }
             if (hipGetLastError() != hipSuccess) { st = RVK_ERR_HIP; break; }
         }

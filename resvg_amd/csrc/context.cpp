@@ -12,6 +12,36 @@ void set_last_error(const char *what, const char *where) {
     if (std::getenv("RVK_VERBOSE")) std::fprintf(stderr, "[rvk] HIP error: %s\n", g_last_error.c_str());
 }
 
+size_t prof_start(rvk_context *ctx, const char *name, double bytes) {
+    if (!ctx->profiling) return SIZE_MAX;
+    hipEvent_t ev[2];
+    for (auto &e : ev) {
+        if (!ctx->event_pool.empty()) {
+            e = ctx->event_pool.back();
+            ctx->event_pool.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            (void)hipGetLastError();
+            return SIZE_MAX;
+        }
+    }
+    if (hipEventRecord(ev[0], ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->event_pool.push_back(ev[0]);
+        ctx->event_pool.push_back(ev[1]);
+        return SIZE_MAX;
+    }
+    ctx->prof.push_back(rvk_prof_rec{name, ev[0], ev[1], bytes});
+    return ctx->prof.size() - 1;
+}
+
+void prof_stop(rvk_context *ctx, size_t slot) {
+    if (slot == SIZE_MAX || slot >= ctx->prof.size()) return;
+    if (hipEventRecord(ctx->prof[slot].stop, ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->prof[slot].name = nullptr;  // dropped from the summary
+    }
+}
+
 rvk_status scratch_alloc(rvk_context *ctx, size_t bytes, void **out) {
     if (bytes == 0) bytes = 256;
     bytes = (bytes + 255) & ~(size_t)255;
@@ -114,6 +144,8 @@ rvk_status rvk_context_destroy(rvk_context *ctx) {
     if (!ctx) return RVK_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &r : ctx->prof) { (void)hipEventDestroy(r.start); (void)hipEventDestroy(r.stop); }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto &f : ctx->free_list) (void)hipFree(f.second);
     for (auto &f : ctx->live) (void)hipFree(f.second);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -170,6 +202,52 @@ rvk_status rvk_get_stats(rvk_context *ctx, rvk_stats *out) {
     if (!ctx || !out) return RVK_ERR_INVALID_ARGUMENT;
     *out = ctx->stats;
     return RVK_OK;
+}
+
+rvk_status rvk_profile_begin(rvk_context *ctx) {
+    if (!ctx) return RVK_ERR_INVALID_ARGUMENT;
+    ctx->profiling = true;
+    return RVK_OK;
+}
+
+rvk_status rvk_profile_end(rvk_context *ctx, rvk_kernel_stat *out, uint32_t capacity, uint32_t *count) {
+    if (!ctx || (!out && capacity) || !count) return RVK_ERR_INVALID_ARGUMENT;
+    ctx->profiling = false;
+    RVK_HIP_CHECK(hipSetDevice(ctx->device));
+    RVK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::vector<rvk_kernel_stat> agg;
+    rvk_status st = RVK_OK;
+    for (auto &r : ctx->prof) {
+        float ms = 0.0f;
+        if (r.name && st == RVK_OK) {
+            hipError_t e = hipEventElapsedTime(&ms, r.start, r.stop);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                rvk::set_last_error(hipGetErrorString(e), "hipEventElapsedTime");
+                st = RVK_ERR_HIP;
+            } else {
+                auto it = std::find_if(agg.begin(), agg.end(),
+                                       [&](const rvk_kernel_stat &k) { return std::strcmp(k.name, r.name) == 0; });
+                if (it == agg.end()) {
+                    rvk_kernel_stat k{};
+                    std::snprintf(k.name, sizeof(k.name), "%s", r.name);
+                    agg.push_back(k);
+                    it = agg.end() - 1;
+                }
+                it->launches++;
+                it->total_ms += ms;
+                it->bytes += r.bytes;
+            }
+        }
+        ctx->event_pool.push_back(r.start);
+        ctx->event_pool.push_back(r.stop);
+    }
+    ctx->prof.clear();
+    std::sort(agg.begin(), agg.end(),
+              [](const rvk_kernel_stat &a, const rvk_kernel_stat &b) { return a.total_ms > b.total_ms; });
+    *count = (uint32_t)std::min<size_t>(agg.size(), capacity);
+    for (uint32_t i = 0; i < *count; i++) out[i] = agg[i];
+    return st;
 }
 
 }  // extern "C"

@@ -235,9 +235,10 @@ rvk_status launch_convolve(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
     size_t bytes = (size_t)w * h * 4;
     rvk_status st = scratch_alloc(ctx, bytes, &tmp);
     if (st == RVK_OK) {
-        hipLaunchKernelGGL(k_convolve, dim3(grid_for((size_t)w * h)), dim3(256), 0, ctx->stream,
-                           (const uint32_t *)px, (uint32_t *)tmp, w, h, a, (const float *)bigk);
-        ctx->stats.kernel_launches++;
+        timed_launch(ctx, "convolve_matrix", 2.0 * (double)bytes, [&] {
+            hipLaunchKernelGGL(k_convolve, dim3(grid_for((size_t)w * h)), dim3(256), 0, ctx->stream,
+                               (const uint32_t *)px, (uint32_t *)tmp, w, h, a, (const float *)bigk);
+        });
         if (hipGetLastError() != hipSuccess) st = RVK_ERR_HIP;
         if (st == RVK_OK) st = launch_copy(ctx, px, (const uint8_t *)tmp, bytes);
         scratch_release(ctx, tmp);
@@ -291,9 +292,10 @@ rvk_status launch_lighting(rvk_context *ctx, const uint8_t *src, uint8_t *dst, u
         a.has_cone = light->has_limiting_cone_angle;
         a.cos_cone = std::cos(light->limiting_cone_angle * rads_per_deg);
     }
-    hipLaunchKernelGGL(k_lighting, dim3(grid_for((size_t)w * h)), dim3(256), 0, ctx->stream, (const uint32_t *)src,
-                       (uint32_t *)dst, w, h, a);
-    ctx->stats.kernel_launches++;
+    timed_launch(ctx, "lighting", 8.0 * (double)w * h, [&] {  // source read + destination write
+        hipLaunchKernelGGL(k_lighting, dim3(grid_for((size_t)w * h)), dim3(256), 0, ctx->stream,
+                           (const uint32_t *)src, (uint32_t *)dst, w, h, a);
+    });
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;
 }

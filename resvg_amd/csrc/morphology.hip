@@ -123,30 +123,35 @@ rvk_status launch_morphology(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t
     if ((int)columns <= kMaxFusedN && (int)rows <= kMaxFusedN) {
         size_t lds = (size_t)((TW + columns - 1) * (TH + rows - 1) + (TH + rows - 1) * TW) * sizeof(uint2);
         dim3 grid(div_up(w, TW), div_up(h, TH));
-        if (erode)
-            hipLaunchKernelGGL(k_morph<true>, grid, dim3(kThreads), lds, ctx->stream, (const uint32_t *)px,
-                               (uint32_t *)tmp, w, h, (int)columns, (int)tx0, (int)rows, (int)ty0);
-        else
-            hipLaunchKernelGGL(k_morph<false>, grid, dim3(kThreads), lds, ctx->stream, (const uint32_t *)px,
-                               (uint32_t *)tmp, w, h, (int)columns, (int)tx0, (int)rows, (int)ty0);
-        ctx->stats.kernel_launches++;
+        timed_launch(ctx, "morphology", 2.0 * (double)bytes, [&] {
+            if (erode)
+                hipLaunchKernelGGL(k_morph<true>, grid, dim3(kThreads), lds, ctx->stream, (const uint32_t *)px,
+                                   (uint32_t *)tmp, w, h, (int)columns, (int)tx0, (int)rows, (int)ty0);
+            else
+                hipLaunchKernelGGL(k_morph<false>, grid, dim3(kThreads), lds, ctx->stream, (const uint32_t *)px,
+                                   (uint32_t *)tmp, w, h, (int)columns, (int)tx0, (int)rows, (int)ty0);
+        });
         if (hipGetLastError() != hipSuccess) st = RVK_ERR_HIP;
         if (st == RVK_OK) st = launch_copy(ctx, px, (const uint8_t *)tmp, bytes);
     } else {
         size_t n = (size_t)w * h;
         unsigned g = (unsigned)((n + 255) / 256);
-        if (erode) {
-            hipLaunchKernelGGL(k_morph_h<true>, dim3(g), dim3(256), 0, ctx->stream, (const uint32_t *)px,
-                               (uint32_t *)tmp, w, h, (int)columns, (int)tx0);
-            hipLaunchKernelGGL(k_morph_v<true>, dim3(g), dim3(256), 0, ctx->stream, (const uint32_t *)tmp,
-                               (uint32_t *)px, w, h, (int)rows, (int)ty0);
-        } else {
-            hipLaunchKernelGGL(k_morph_h<false>, dim3(g), dim3(256), 0, ctx->stream, (const uint32_t *)px,
-                               (uint32_t *)tmp, w, h, (int)columns, (int)tx0);
-            hipLaunchKernelGGL(k_morph_v<false>, dim3(g), dim3(256), 0, ctx->stream, (const uint32_t *)tmp,
-                               (uint32_t *)px, w, h, (int)rows, (int)ty0);
-        }
-        ctx->stats.kernel_launches += 2;
+        timed_launch(ctx, "morphology_h", 8.0 * (double)n, [&] {
+            if (erode)
+                hipLaunchKernelGGL(k_morph_h<true>, dim3(g), dim3(256), 0, ctx->stream, (const uint32_t *)px,
+                                   (uint32_t *)tmp, w, h, (int)columns, (int)tx0);
+            else
+                hipLaunchKernelGGL(k_morph_h<false>, dim3(g), dim3(256), 0, ctx->stream, (const uint32_t *)px,
+                                   (uint32_t *)tmp, w, h, (int)columns, (int)tx0);
+        });
+        timed_launch(ctx, "morphology_v", 8.0 * (double)n, [&] {
+            if (erode)
+                hipLaunchKernelGGL(k_morph_v<true>, dim3(g), dim3(256), 0, ctx->stream, (const uint32_t *)tmp,
+                                   (uint32_t *)px, w, h, (int)rows, (int)ty0);
+            else
+                hipLaunchKernelGGL(k_morph_v<false>, dim3(g), dim3(256), 0, ctx->stream, (const uint32_t *)tmp,
+                                   (uint32_t *)px, w, h, (int)rows, (int)ty0);
+        });
         if (hipGetLastError() != hipSuccess) st = RVK_ERR_HIP;
     }
     scratch_release(ctx, tmp);

@@ -203,27 +203,35 @@ __global__ void k_zero_rgb(uint32_t *px, size_t n) {
 
 namespace rvk {
 
-static rvk_status post_launch(rvk_context *ctx) {
-    ctx->stats.kernel_launches++;
+static rvk_status post_launch(rvk_context *) {
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;
 }
 
+// in-place per-pixel kernels read and write 4 bytes per pixel
+static inline double rw_bytes(size_t n) { return 8.0 * (double)n; }
+
 rvk_status launch_into_cs(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, int to_linear) {
     size_t n = (size_t)w * h;
-    hipLaunchKernelGGL(k_into_cs, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n, to_linear);
+    timed_launch(ctx, "into_color_space", rw_bytes(n), [&] {
+        hipLaunchKernelGGL(k_into_cs, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n, to_linear);
+    });
     return post_launch(ctx);
 }
 
 rvk_status launch_demultiply(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h) {
     size_t n = (size_t)w * h;
-    hipLaunchKernelGGL(k_demul, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n);
+    timed_launch(ctx, "demultiply", rw_bytes(n), [&] {
+        hipLaunchKernelGGL(k_demul, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n);
+    });
     return post_launch(ctx);
 }
 
 rvk_status launch_multiply(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h) {
     size_t n = (size_t)w * h;
-    hipLaunchKernelGGL(k_premul, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n);
+    timed_launch(ctx, "premultiply", rw_bytes(n), [&] {
+        hipLaunchKernelGGL(k_premul, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n);
+    });
     return post_launch(ctx);
 }
 
@@ -255,8 +263,10 @@ rvk_status launch_color_matrix(rvk_context *ctx, uint8_t *px, uint32_t w, uint32
         return RVK_ERR_INVALID_ARGUMENT;
     }
     size_t n = (size_t)w * h;
-    hipLaunchKernelGGL(k_color_matrix, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n, a,
-                       fuse_alpha);
+    timed_launch(ctx, "color_matrix", rw_bytes(n), [&] {
+        hipLaunchKernelGGL(k_color_matrix, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n, a,
+                           fuse_alpha);
+    });
     return post_launch(ctx);
 }
 
@@ -318,15 +328,19 @@ rvk_status launch_component_transfer(rvk_context *ctx, uint8_t *px, uint32_t w, 
     }
     size_t n = (size_t)w * h;
     // the four tables travel by value in the kernel arguments (1040 bytes)
-    hipLaunchKernelGGL(k_component_transfer, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n,
-                       lut, fuse_alpha);
+    timed_launch(ctx, "component_transfer", rw_bytes(n), [&] {
+        hipLaunchKernelGGL(k_component_transfer, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n,
+                           lut, fuse_alpha);
+    });
     return post_launch(ctx);
 }
 
 rvk_status launch_arithmetic(rvk_context *ctx, const uint8_t *s1, const uint8_t *s2, uint8_t *dst, size_t n,
                              float k1, float k2, float k3, float k4) {
-    hipLaunchKernelGGL(k_arithmetic, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (const uint32_t *)s1,
-                       (const uint32_t *)s2, (uint32_t *)dst, n, k1, k2, k3, k4);
+    timed_launch(ctx, "composite_arithmetic", 12.0 * (double)n, [&] {  // two sources, one destination
+        hipLaunchKernelGGL(k_arithmetic, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (const uint32_t *)s1,
+                           (const uint32_t *)s2, (uint32_t *)dst, n, k1, k2, k3, k4);
+    });
     return post_launch(ctx);
 }
 
@@ -335,8 +349,10 @@ rvk_status launch_displacement(rvk_context *ctx, const uint8_t *src, const uint8
                                uint32_t ych) {
     if (xch > 3 || ych > 3) return RVK_ERR_INVALID_ARGUMENT;
     size_t n = (size_t)w * h;
-    hipLaunchKernelGGL(k_displacement, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (const uint32_t *)src,
-                       (const uint32_t *)map, (uint32_t *)dst, w, h, sx, sy, scale, xch, ych);
+    timed_launch(ctx, "displacement_map", 12.0 * (double)n, [&] {  // source, map, destination
+        hipLaunchKernelGGL(k_displacement, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (const uint32_t *)src,
+                           (const uint32_t *)map, (uint32_t *)dst, w, h, sx, sy, scale, xch, ych);
+    });
     return post_launch(ctx);
 }
 
@@ -354,16 +370,20 @@ rvk_status launch_fill_sub(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
     if (y1 > (int64_t)h) y1 = h;
     if (x1 <= x0 || y1 <= y0) return RVK_OK;
     size_t n = (size_t)(x1 - x0) * (size_t)(y1 - y0);
-    hipLaunchKernelGGL(k_fill_rect, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, w,
-                       (uint32_t)x0, (uint32_t)y0, (uint32_t)(x1 - x0), (uint32_t)(y1 - y0), value);
+    timed_launch(ctx, "fill_rect", 4.0 * (double)n, [&] {  // writes only
+        hipLaunchKernelGGL(k_fill_rect, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, w,
+                           (uint32_t)x0, (uint32_t)y0, (uint32_t)(x1 - x0), (uint32_t)(y1 - y0), value);
+    });
     return post_launch(ctx);
 }
 
 rvk_status launch_copy(rvk_context *ctx, uint8_t *dst, const uint8_t *src, size_t bytes) {
     if (bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0) {
         size_t n = bytes / 16;
-        hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint4 *)dst,
-                           (const uint4 *)src, n);
+        timed_launch(ctx, "copy", 2.0 * (double)bytes, [&] {
+            hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint4 *)dst,
+                               (const uint4 *)src, n);
+        });
         return post_launch(ctx);
     }
     RVK_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
@@ -376,10 +396,11 @@ namespace rvkh {
 rvk_status launch_shadow_flood(rvk_context *ctx, uint8_t *px, size_t n, const uint8_t color[4], float opacity) {
     float op = opacity < 0.0f ? 0.0f : (opacity > 1.0f ? 1.0f : opacity);
     uint8_t a8 = rvk::host_as_u8(op * 255.0f + 0.5f);
-    hipLaunchKernelGGL(k_shadow_flood, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n,
-                       (float)color[0] / 255.0f, (float)color[1] / 255.0f, (float)color[2] / 255.0f,
-                       (float)a8 / 255.0f);
-    ctx->stats.kernel_launches++;
+    rvk::timed_launch(ctx, "shadow_flood", 8.0 * (double)n, [&] {
+        hipLaunchKernelGGL(k_shadow_flood, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n,
+                           (float)color[0] / 255.0f, (float)color[1] / 255.0f, (float)color[2] / 255.0f,
+                           (float)a8 / 255.0f);
+    });
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;
 }
@@ -388,7 +409,9 @@ rvk_status launch_shadow_flood(rvk_context *ctx, uint8_t *px, size_t n, const ui
 namespace rvk {
 
 rvk_status launch_zero_rgb(rvk_context *ctx, uint8_t *px, size_t n) {
-    hipLaunchKernelGGL(k_zero_rgb, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n);
+    timed_launch(ctx, "zero_rgb", rw_bytes(n), [&] {
+        hipLaunchKernelGGL(k_zero_rgb, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, n);
+    });
     return post_launch(ctx);
 }
 

@@ -337,23 +337,24 @@ namespace rvk {
 
 rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, uint32_t total_rows,
                            const Piece *pieces, const uint32_t *band_start, const uint32_t *band_items, uint8_t *arena,
-                           int32_t *overflow) {
+                           int32_t *overflow, double algo_bytes) {
     if (total_rows == 0) return RVK_OK;
     CovArgs a{jobs, njobs, pieces, band_start, band_items, arena, overflow};
-    hipLaunchKernelGGL(k_coverage, dim3(total_rows), dim3(kWave), 0, ctx->stream, a);
-    ctx->stats.kernel_launches++;
+    timed_launch(ctx, "coverage", algo_bytes,
+                 [&] { hipLaunchKernelGGL(k_coverage, dim3(total_rows), dim3(kWave), 0, ctx->stream, a); });
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;
 }
 
 rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
                              int32_t x1, int32_t y1, const DrawItem *items, uint32_t nitems, const CovJob *jobs,
-                             const uint8_t *arena, const ShaderParams *shaders) {
+                             const uint8_t *arena, const ShaderParams *shaders, double algo_bytes) {
     if (nitems == 0 || x1 <= x0 || y1 <= y0) return RVK_OK;
     dim3 grid(div_up((uint32_t)(x1 - x0), TX), div_up((uint32_t)(y1 - y0), TY));
-    hipLaunchKernelGGL(k_draw, grid, dim3(TX * TY), 0, ctx->stream, (uint32_t *)target, w, h, x0, y0, items, nitems,
-                       jobs, arena, shaders);
-    ctx->stats.kernel_launches++;
+    timed_launch(ctx, "draw_items", algo_bytes, [&] {
+        hipLaunchKernelGGL(k_draw, grid, dim3(TX * TY), 0, ctx->stream, (uint32_t *)target, w, h, x0, y0, items, nitems,
+                           jobs, arena, shaders);
+    });
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;
 }

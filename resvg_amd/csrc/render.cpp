@@ -23,10 +23,10 @@ using namespace rvkr;
 namespace rvk {
 rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, uint32_t total_rows,
                            const Piece *pieces, const uint32_t *band_start, const uint32_t *band_items, uint8_t *arena,
-                           int32_t *overflow);
+                           int32_t *overflow, double algo_bytes);
 rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
                              int32_t x1, int32_t y1, const DrawItem *items, uint32_t nitems, const CovJob *jobs,
-                             const uint8_t *arena, const ShaderParams *shaders);
+                             const uint8_t *arena, const ShaderParams *shaders, double algo_bytes);
 }
 
 namespace {
@@ -216,15 +216,23 @@ struct Renderer {
         void *dev = nullptr;
         RVK_TRY(rvk::scratch_alloc(ctx, meta + dl.arena, &dev));
         uint8_t *d = (uint8_t *)dev;
+        // algorithmic bytes: coverage writes one byte per job pixel and reads each piece
+        // and band entry once; the compositor reads and writes each target pixel of its
+        // rect once and reads each coverage byte once
+        double cov_px = 0.0;
+        for (const CovJob &j : dl.jobs) cov_px += (double)j.w * (double)j.h;
+        const double cov_bytes = cov_px + (double)(dl.pieces.size() * sizeof(Piece) + dl.band_items.size() * 4);
+        const double draw_bytes = 8.0 * (double)(dl.bx1 - dl.bx0) * (double)(dl.by1 - dl.by0) + cov_px;
         rvk_status st = rvk::upload_sync(ctx, d, host.data(), meta);
         if (st == RVK_OK)
             st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)dl.jobs.size(), dl.rows,
                                       (const Piece *)(d + o_pieces), (const uint32_t *)(d + o_bs),
-                                      (const uint32_t *)(d + o_bi), d + meta, (int32_t *)(d + o_ov));
+                                      (const uint32_t *)(d + o_bi), d + meta, (int32_t *)(d + o_ov), cov_bytes);
         if (st == RVK_OK)
             st = rvk::launch_draw_items(ctx, dl.px, dl.w, dl.h, dl.bx0, dl.by0, dl.bx1, dl.by1,
                                         (const DrawItem *)(d + o_it), (uint32_t)dl.items.size(),
-                                        (const CovJob *)(d + o_jobs), d + meta, (const ShaderParams *)(d + o_sh));
+                                        (const CovJob *)(d + o_jobs), d + meta, (const ShaderParams *)(d + o_sh),
+                                        draw_bytes);
         int32_t ov = 0;
         if (st == RVK_OK) {
             hipError_t e = hipMemcpyAsync(&ov, d + o_ov, 4, hipMemcpyDeviceToHost, ctx->stream);

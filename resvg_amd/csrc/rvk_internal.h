@@ -23,11 +23,21 @@
         if (s_ != RVK_OK) return s_;          \
     } while (0)
 
+// One profiled launch: events around it on the context stream.
+struct rvk_prof_rec {
+    const char *name;
+    hipEvent_t start, stop;
+    double bytes;
+};
+
 struct rvk_context {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     rvk_stats stats{};
+    bool profiling = false;
+    std::vector<rvk_prof_rec> prof;
+    std::vector<hipEvent_t> event_pool;
     // Simple size-bucketed cache of device scratch buffers (layers, filter images).
     std::vector<std::pair<size_t, void *>> free_list;
     std::vector<std::pair<size_t, void *>> live;
@@ -46,6 +56,20 @@ void scratch_release(rvk_context *ctx, void *p);
 rvk_status upload_sync(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes);
 
 static inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// Profiling hooks around one kernel launch (no-ops unless rvk_profile_begin ran).
+size_t prof_start(rvk_context *ctx, const char *name, double bytes);
+void prof_stop(rvk_context *ctx, size_t slot);
+
+// Launches `fn` (a lambda holding one hipLaunchKernelGGL) on the context stream,
+// counting it and, while profiling, timing it with HIP events.
+template <class F>
+static inline void timed_launch(rvk_context *ctx, const char *name, double bytes, F &&fn) {
+    const size_t slot = prof_start(ctx, name, bytes);
+    fn();
+    prof_stop(ctx, slot);
+    ctx->stats.kernel_launches++;
+}
 
 // Upper bound on one pixmap side: tiny-skia's Pixmap size limit is i32::MAX / 4 bytes per row.
 constexpr uint32_t kMaxSide = 1u << 16;

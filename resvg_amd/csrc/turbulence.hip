@@ -182,9 +182,10 @@ rvk_status launch_turbulence(rvk_context *ctx, uint8_t *dst, uint32_t w, uint32_
     size_t n = (size_t)w * h;
     size_t g = (n + 255) / 256;
     if (g > 4096) g = 4096;  // grid-stride: amortise the LDS table load
-    hipLaunchKernelGGL(k_turbulence, dim3((unsigned)g), dim3(256), 0, ctx->stream, (uint32_t *)dst, w, h, a,
-                       (const int32_t *)dlat, (const double *)dgrad);
-    ctx->stats.kernel_launches++;
+    timed_launch(ctx, "turbulence", 4.0 * (double)n, [&] {  // writes 4 bytes per pixel
+        hipLaunchKernelGGL(k_turbulence, dim3((unsigned)g), dim3(256), 0, ctx->stream, (uint32_t *)dst, w, h, a,
+                           (const int32_t *)dlat, (const double *)dgrad);
+    });
     e = hipGetLastError();
     scratch_release(ctx, buf);
     RVK_HIP_CHECK(e);

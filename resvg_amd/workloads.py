@@ -160,7 +160,11 @@ def _filter_chains(rng, size):
 
 
 def full_filter_doc(seed: int = 0, size: int = 4096, n_paths: int = 2000, paths_per_filter: int = 25,
-                    max_segments: int = 200):
+                    max_segments: int = 200, keep=None):
+    """The full-filter document.  `keep(item, is_filter_group) -> bool`, when given,
+    selects top-level items (a filter group with its paths, or one path in its opacity
+    group, numbered in document order) without changing the random stream, so a kept
+    item is identical to the same item of the whole document."""
     rng = np.random.default_rng(seed)
     sc = Scene(size, size)
     chains = _filter_chains(rng, size)
@@ -168,11 +172,16 @@ def full_filter_doc(seed: int = 0, size: int = 4096, n_paths: int = 2000, paths_
     # filter groups are spread evenly through the document order
     starts = {int(i * n_paths / len(chains)): k for k, i in enumerate(range(len(chains)))}
     i = 0
+    item = -1
     while i < n_paths:
         parent = sc.root
         count = 1
-        if i in starts and n_filtered > 0:
-            parent = sc.group(sc.root, filters=[chains[starts[i]]])
+        item += 1
+        is_filter = i in starts and n_filtered > 0
+        kept = keep is None or keep(item, is_filter)
+        if is_filter:
+            if kept:
+                parent = sc.group(sc.root, filters=[chains[starts[i]]])
             count = min(paths_per_filter, n_paths - i)
         for _ in range(count):
             nseg = int(rng.integers(3, max_segments + 1))
@@ -184,8 +193,11 @@ def full_filter_doc(seed: int = 0, size: int = 4096, n_paths: int = 2000, paths_
                 stroke = _random_paint(rng, size, int(rng.integers(2, 9)))
                 width = _f(rng.uniform(0.5, 8.0))
                 join = [abi.JOIN_ROUND, abi.JOIN_MITER, abi.JOIN_BEVEL][int(rng.integers(3))]
-            grp = sc.group(parent, opacity=_f(rng.uniform(0.2, 1.0)))
-            sc.path(grp, verbs, pts, fill=fill, fill_rule=rule, stroke=stroke, stroke_width=width, stroke_join=join)
+            opacity = _f(rng.uniform(0.2, 1.0))
+            if kept:
+                grp = sc.group(parent, opacity=opacity)
+                sc.path(grp, verbs, pts, fill=fill, fill_rule=rule, stroke=stroke, stroke_width=width,
+                        stroke_join=join)
             i += 1
     return sc, (size, size)
 
