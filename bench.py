@@ -115,6 +115,8 @@ def main() -> int:
     ap.add_argument("--cpu-stride", type=int, default=64, help="CPU sample: every n-th plain path")
     ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline processes (0: min(16, cores))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy a 1 GiB device buffer once (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -139,6 +141,12 @@ def main() -> int:
     torch.cuda.set_device(dev)
     ctx = resvg_amd.Context(0)
     px = torch.zeros((h, w, 4), dtype=torch.uint8, device=dev)
+    if args.copy_calibration:
+        a = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     def step():
         px.zero_()

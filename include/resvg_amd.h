@@ -342,6 +342,12 @@ typedef struct rvk_stats {
     uint64_t paths;
     uint64_t filters;
     uint64_t scratch_bytes;
+    uint64_t edge_pieces;   /* line pieces built for coverage */
+    uint64_t band_entries;  /* piece references in the row bands */
+    uint64_t upload_bytes;  /* draw-list metadata sent to the device */
+    uint64_t flushes;       /* draw-list submissions */
+    double host_build_ms;   /* host time building edges, bands and shaders */
+    double host_wait_ms;    /* host time waiting for the device */
 } rvk_stats;
 rvk_status rvk_get_stats(rvk_context *ctx, rvk_stats *out);
 
