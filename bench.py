@@ -103,6 +103,25 @@ def cpu_baseline(size, paths, seed, workers, plain_stride=64, filter_group=1, n_
     }
 
 
+def _primitive_rooflines(kernels, steps):
+    """Per filter primitive / pipeline stage: its kernels' time per step and the
+    achieved fraction of HBM peak on the primitive's algorithmic bytes."""
+    groups = {}
+    for k in kernels:
+        n = k["name"]
+        p = ("iir_blur" if n.startswith("iir_") else "box_blur" if n.startswith("box_blur") else
+             "morphology" if n.startswith("morphology") else n)
+        g = groups.setdefault(p, {"ms_per_step": 0.0, "bytes": 0.0})
+        g["ms_per_step"] += k["total_ms"] / steps
+        g["bytes"] += k["bytes"] / steps
+    out = {}
+    for p, g in sorted(groups.items(), key=lambda kv: -kv[1]["ms_per_step"]):
+        gbs = g["bytes"] / (g["ms_per_step"] / 1e3) / 1e9 if g["ms_per_step"] > 0 else 0.0
+        out[p] = {"ms_per_step": round(g["ms_per_step"], 3), "GB/s": round(gbs, 1),
+                  "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 # ---------------------------------------------------------------------------
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
@@ -215,6 +234,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()} for d in kernels[:8]],
+        "primitives": _primitive_rooflines(kernels, args.steps),
         "wall_s_per_step": round(t_wall / args.steps, 3),
     }
     print(json.dumps(line), flush=True)

@@ -184,12 +184,15 @@ rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
     RVK_TRY(scratch_alloc(ctx, bytes, &tmp));
     uint32_t *a = (uint32_t *)px, *b = (uint32_t *)tmp;
     rvk_status st = RVK_OK;
+    // algorithmic bytes of the whole blur (read 4 + write 4 per pixel), shared by its
+    // five rounds (ten launches on the unfused path)
+    const double round_bytes = 8.0 * (double)w * h / 5.0;
     for (int i = 0; i < 5 && st == RVK_OK; i++) {
         int rh = (bh[i] - 1) / 2, rv = (bv[i] - 1) / 2;
         if (rh <= kMaxFusedR && rv <= kMaxFusedR) {
             size_t lds = (size_t)((TW + 2 * rh) * (TH + 2 * rv) + (TW + 2 * rh) * TH) * 4;
             dim3 grid(div_up(w, TW), div_up(h, TH));
-            timed_launch(ctx, "box_blur_round", 8.0 * (double)w * h, [&] {
+            timed_launch(ctx, "box_blur_round", round_bytes, [&] {
                 hipLaunchKernelGGL(k_box_round, grid, dim3(kThreads), lds, ctx->stream, a, b, w, h, rh, rv);
             });
             if (hipGetLastError() != hipSuccess) { st = RVK_ERR_HIP; break; }
@@ -200,7 +203,7 @@ rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
             if (rv == 0) {
                 if (hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) { st = RVK_ERR_HIP; break; }
             } else {
-                timed_launch(ctx, "box_blur_vert", 8.0 * (double)w * h, [&] {
+                timed_launch(ctx, "box_blur_vert", round_bytes / 2.0, [&] {
                     hipLaunchKernelGGL(k_box_vert, dim3(div_up(w, 64), div_up(h, chunk)), dim3(64), 0, ctx->stream,
                                        a, b, w, h, rv, chunk);
                 });
@@ -208,7 +211,7 @@ rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
             if (rh == 0) {
                 if (hipMemcpyAsync(a, b, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) { st = RVK_ERR_HIP; break; }
             } else {
-                timed_launch(ctx, "box_blur_horz", 8.0 * (double)w * h, [&] {
+                timed_launch(ctx, "box_blur_horz", round_bytes / 2.0, [&] {
                     hipLaunchKernelGGL(k_box_horz, dim3(div_up(h, 64), div_up(w, chunk)), dim3(64), 0, ctx->stream,
                                        b, a, w, h, rh, chunk);
                 });
