@@ -11,12 +11,12 @@
 // orders differ only between edges with exactly equal x, which can change one
 // coverage value by 1/255 on the last sub-scanline of a pixel.
 //
-// The stroker is not tiny-skia's (its source is unavailable): strokes are this
-// build's polyline outline (see stroke_outline) filled with the nonzero rule.
-// Test infrastructure only.
+// Strokes use resvg_amd/csrc/stroker.cpp (tiny-skia's PathStroker restated), the one
+// piece of geometry shared with the product.  Test infrastructure only.
 #include "oracle.h"
 #include "common.h"
 #include "geom.h"
+#include "../resvg_amd/csrc/stroker.h"
 #include <algorithm>
 #include <array>
 #include <vector>
@@ -517,195 +517,49 @@ static void coverage(const std::vector<Seg> &segs, uint32_t rule, bool aa, uint3
 }
 
 // ---------------------------------------------------------------------------
-// Stroker (this build's polyline outline; see the file comment)
+// Stroking: tiny-skia's PathStroker (resvg_amd/csrc/stroker.cpp, one restatement
+// shared with the product; its geometry is pinned by the reference's stroke
+// fixtures).  The outline is built in user space, mapped with the transform and
+// filled nonzero (tiny-skia Pixmap::stroke_path -> fill_path).
 // ---------------------------------------------------------------------------
-struct Poly { std::vector<Pt> pts; };
-
-static float stroke_res_scale(Ts ts) {  // tiny-skia compute_res_scale_for_stroking
-    float sx = std::sqrt(ts.sx * ts.sx + ts.kx * ts.kx), sy = std::sqrt(ts.ky * ts.ky + ts.sy * ts.sy);
-    if (std::isfinite(sx) && std::isfinite(sy)) {
-        float s = std::max(sx, sy);
-        if (s > 0.0f) return s;
-    }
-    return 1.0f;
-}
-
-// Number of uniform steps flattening a curve to within `tol` user units.
-static int flatten_steps(const Pt *p, int n, float tol) {
-    float dd = 0.0f;
-    for (int i = 0; i + 2 < n; i++) {
-        float dx = p[i].x - 2.0f * p[i + 1].x + p[i + 2].x, dy = p[i].y - 2.0f * p[i + 1].y + p[i + 2].y;
-        dd = std::max(dd, std::sqrt(dx * dx + dy * dy));
-    }
-    float k = n == 4 ? 0.75f : 0.25f;
-    int steps = (int)std::ceil(std::sqrt(k * dd / tol));
-    return std::max(1, std::min(steps, 256));
-}
-
-static Pt eval_curve(const Pt *p, int n, float t) {
-    float u = 1.0f - t;
-    if (n == 3) {
-        float a = u * u, b = 2.0f * u * t, c = t * t;
-        return Pt{a * p[0].x + b * p[1].x + c * p[2].x, a * p[0].y + b * p[1].y + c * p[2].y};
-    }
-    float a = u * u * u, b = 3.0f * u * u * t, c = 3.0f * u * t * t, d = t * t * t;
-    return Pt{a * p[0].x + b * p[1].x + c * p[2].x + d * p[3].x, a * p[0].y + b * p[1].y + c * p[2].y + d * p[3].y};
-}
-
-struct Contour { std::vector<Pt> pts; bool closed; };
-
-static void stroke_contours(const rvk_scene &sc, const rvk_path &path, float tol, std::vector<Contour> &out) {
+static bool stroke_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
+    rvks::StrokeParams sp{path.stroke_width, path.stroke_miter_limit, path.stroke_cap, path.stroke_join};
+    const float res = rvks::resolution_scale(ts.sx, ts.ky, ts.kx, ts.sy);
+    rvks::PathBuilder outline;
     const uint8_t *verbs = sc.verbs + path.verb_offset;
     const float *pts = sc.points + 2 * (size_t)path.point_offset;
+    size_t npts = 0;
+    for (uint32_t v = 0; v < path.verb_count; v++)
+        npts += verbs[v] == RVK_VERB_MOVE || verbs[v] == RVK_VERB_LINE ? 1 : verbs[v] == RVK_VERB_QUAD ? 2
+              : verbs[v] == RVK_VERB_CUBIC ? 3 : 0;
+    if (!rvks::stroke(verbs, path.verb_count, pts, npts, sp, res, outline)) return false;
+    for (auto &p : outline.pts) ts_map(ts, p.x, p.y);
+    // contours of the outline, each closed for filling
     size_t pi = 0;
-    Contour cur{{}, false};
-    auto P = [&](size_t i) { return Pt{pts[2 * i], pts[2 * i + 1]}; };
-    auto add = [&](Pt p) {
-        if (cur.pts.empty() || cur.pts.back().x != p.x || cur.pts.back().y != p.y) cur.pts.push_back(p);
+    Pt move{0, 0}, last{0, 0};
+    bool open = false;
+    auto P = [&](size_t i) { return Pt{outline.pts[i].x, outline.pts[i].y}; };
+    auto close = [&]() {
+        if (open && (last.x != move.x || last.y != move.y)) out.push_back(Seg{2, {last, move}});
+        open = false;
     };
-    auto finish = [&]() {
-        if (cur.pts.size() >= 2) out.push_back(cur);
-        cur = Contour{{}, false};
-    };
-    for (uint32_t v = 0; v < path.verb_count; v++) {
-        switch (verbs[v]) {
-            case RVK_VERB_MOVE: finish(); add(P(pi++)); break;
-            case RVK_VERB_LINE: add(P(pi++)); break;
-            case RVK_VERB_QUAD:
-            case RVK_VERB_CUBIC: {
-                int n = verbs[v] == RVK_VERB_QUAD ? 3 : 4;
-                Pt c[4];
-                c[0] = cur.pts.empty() ? Pt{0, 0} : cur.pts.back();
-                for (int k = 1; k < n; k++) c[k] = P(pi++);
-                int steps = flatten_steps(c, n, tol);
-                for (int s = 1; s < steps; s++) add(eval_curve(c, n, (float)s / (float)steps));
-                add(c[n - 1]);
+    for (uint8_t v : outline.verbs) {
+        switch (v) {
+            case rvks::kMove: close(); move = last = P(pi++); open = true; break;
+            case rvks::kLine: { Pt p = P(pi++); out.push_back(Seg{2, {last, p}}); last = p; break; }
+            case rvks::kQuad: { Pt a = P(pi), b = P(pi + 1); pi += 2; out.push_back(Seg{3, {last, a, b}}); last = b; break; }
+            case rvks::kCubic: {
+                Pt a = P(pi), b = P(pi + 1), c = P(pi + 2);
+                pi += 3;
+                out.push_back(Seg{4, {last, a, b, c}});
+                last = c;
                 break;
             }
-            case RVK_VERB_CLOSE:
-                if (!cur.pts.empty() && cur.pts.size() >= 2 &&
-                    (cur.pts.front().x == cur.pts.back().x && cur.pts.front().y == cur.pts.back().y))
-                    cur.pts.pop_back();
-                cur.closed = true;
-                {
-                    Pt start = cur.pts.empty() ? Pt{0, 0} : cur.pts.front();
-                    finish();
-                    add(start);  // a following segment without MoveTo starts here
-                }
-                break;
+            default: close(); last = move; break;
         }
     }
-    finish();
-}
-
-static float signed_area(const std::vector<Pt> &p) {
-    double a = 0.0;
-    for (size_t i = 0; i < p.size(); i++) {
-        const Pt &u = p[i], &v = p[(i + 1) % p.size()];
-        a += (double)u.x * v.y - (double)v.x * u.y;
-    }
-    return (float)a;
-}
-
-static void emit_piece(std::vector<Pt> p, std::vector<Poly> &out) {
-    if (p.size() < 3) return;
-    if (signed_area(p) > 0.0f) std::reverse(p.begin(), p.end());
-    out.push_back(Poly{std::move(p)});
-}
-
-static void arc_points(Pt c, float hw, float a0, float a1, float tol, std::vector<Pt> &out) {
-    // arc from angle a0 to a1 (radians, |a1-a0| <= pi), chord error <= tol
-    float span = a1 - a0;
-    float step = 2.0f * std::acos(std::max(-1.0f, std::min(1.0f, 1.0f - tol / std::max(hw, tol))));
-    int n = std::max(1, (int)std::ceil(std::fabs(span) / std::max(step, 1e-3f)));
-    n = std::min(n, 64);
-    for (int i = 0; i <= n; i++) {
-        float a = a0 + span * ((float)i / (float)n);
-        out.push_back(Pt{c.x + hw * std::cos(a), c.y + hw * std::sin(a)});
-    }
-}
-
-static void stroke_outline(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Poly> &out) {
-    const float hw = path.stroke_width * 0.5f;
-    if (!(hw > 0.0f)) return;
-    const float res = stroke_res_scale(ts);
-    const float tol = 0.25f / res;
-    std::vector<Contour> contours;
-    stroke_contours(sc, path, tol, contours);
-    for (const Contour &c : contours) {
-        const std::vector<Pt> &p = c.pts;
-        size_t n = p.size();
-        size_t nseg = c.closed ? n : n - 1;
-        if (nseg == 0) continue;
-        auto dir = [&](size_t i) {
-            Pt a = p[i], b = p[(i + 1) % n];
-            float dx = b.x - a.x, dy = b.y - a.y, l = std::sqrt(dx * dx + dy * dy);
-            return Pt{dx / l, dy / l};
-        };
-        for (size_t i = 0; i < nseg; i++) {
-            Pt a = p[i], b = p[(i + 1) % n], d = dir(i);
-            Pt nm{-d.y * hw, d.x * hw};
-            emit_piece({Pt{a.x + nm.x, a.y + nm.y}, Pt{b.x + nm.x, b.y + nm.y}, Pt{b.x - nm.x, b.y - nm.y},
-                        Pt{a.x - nm.x, a.y - nm.y}}, out);
-        }
-        // joins at interior vertices (and all vertices of a closed contour)
-        size_t first = c.closed ? 0 : 1, last = c.closed ? n : n - 1;
-        for (size_t v = first; v < last; v++) {
-            size_t i0 = (v + nseg - 1) % nseg, i1 = v % nseg;
-            if (!c.closed) { i0 = v - 1; i1 = v; }
-            Pt d0 = dir(i0), d1 = dir(i1), q = p[v];
-            float cross = d0.x * d1.y - d0.y * d1.x, dot = d0.x * d1.x + d0.y * d1.y;
-            if (std::fabs(cross) < 1e-6f && dot > 0.0f) continue;
-            // outer side: the side opposite to the turn
-            float s = cross > 0.0f ? -1.0f : 1.0f;
-            Pt n0{-d0.y * hw * s, d0.x * hw * s}, n1{-d1.y * hw * s, d1.x * hw * s};
-            Pt e0{q.x + n0.x, q.y + n0.y}, e1{q.x + n1.x, q.y + n1.y};
-            if (path.stroke_join == RVK_JOIN_ROUND) {
-                float a0 = std::atan2(n0.y, n0.x), a1 = std::atan2(n1.y, n1.x);
-                float da = a1 - a0;
-                while (da > 3.14159265f) da -= 6.2831853f;
-                while (da < -3.14159265f) da += 6.2831853f;
-                std::vector<Pt> fan{q};
-                arc_points(q, hw, a0, a0 + da, tol, fan);
-                emit_piece(fan, out);
-            } else {
-                bool miter = path.stroke_join == RVK_JOIN_MITER || path.stroke_join == RVK_JOIN_MITER_CLIP;
-                // 1/sin(theta/2) = sqrt(2 / (1 - cos(theta))) with theta the angle between the outer normals
-                float cos_t = n0.x * n1.x / (hw * hw) + n0.y * n1.y / (hw * hw);
-                float ratio2 = 2.0f / std::max(1e-12f, 1.0f + cos_t);
-                if (miter && ratio2 <= path.stroke_miter_limit * path.stroke_miter_limit) {
-                    Pt m{n0.x + n1.x, n0.y + n1.y};
-                    float ml = std::sqrt(m.x * m.x + m.y * m.y);
-                    float len = hw * std::sqrt(ratio2);
-                    Pt tip{q.x + m.x / ml * len, q.y + m.y / ml * len};
-                    emit_piece({q, e0, tip, e1}, out);
-                } else {
-                    emit_piece({q, e0, e1}, out);
-                }
-            }
-        }
-        if (!c.closed) {
-            for (int end = 0; end < 2; end++) {
-                Pt q = end == 0 ? p[0] : p[n - 1];
-                Pt d = end == 0 ? dir(0) : dir(n - 2);
-                if (end == 0) d = Pt{-d.x, -d.y};  // outward direction
-                Pt nm{-d.y * hw, d.x * hw};
-                if (path.stroke_cap == RVK_CAP_SQUARE) {
-                    Pt f{d.x * hw, d.y * hw};
-                    emit_piece({Pt{q.x + nm.x, q.y + nm.y}, Pt{q.x + nm.x + f.x, q.y + nm.y + f.y},
-                                Pt{q.x - nm.x + f.x, q.y - nm.y + f.y}, Pt{q.x - nm.x, q.y - nm.y}}, out);
-                } else if (path.stroke_cap == RVK_CAP_ROUND) {
-                    float a0 = std::atan2(nm.y, nm.x);
-                    std::vector<Pt> fan{q};
-                    arc_points(q, hw, a0, a0 - 3.14159265f, tol, fan);
-                    emit_piece(fan, out);
-                }
-            }
-        }
-    }
-    // map to device space
-    for (auto &poly : out)
-        for (auto &pt : poly.pts) ts_map(ts, pt.x, pt.y);
+    close();
+    return !out.empty();
 }
 
 // ---------------------------------------------------------------------------
@@ -1007,18 +861,14 @@ bool fill_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *p
     return true;
 }
 
-// resvg path::stroke_path (path.rs:77-116) with this build's stroker
+// resvg path::stroke_path (path.rs:77-116) -> tiny-skia Pixmap::stroke_path
 bool stroke_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h) {
     if (path.stroke.kind == RVK_PAINT_NONE) return false;
+    if (!(path.stroke_width >= 0.0f)) return false;
     Shader s = make_shader(sc, path.stroke, ts);
     if (!s.valid) return false;
-    std::vector<Poly> polys;
-    stroke_outline(sc, path, ts, polys);
     std::vector<Seg> segs;
-    for (auto &p : polys)
-        for (size_t i = 0; i < p.pts.size(); i++)
-            segs.push_back(Seg{2, {p.pts[i], p.pts[(i + 1) % p.pts.size()]}});
-    if (segs.empty()) return false;
+    if (!stroke_segments(sc, path, ts, segs) || path_bounds_nearly_zero(segs)) return false;
     std::vector<uint8_t> cov;
     coverage(segs, RVK_FILL_NONZERO, path.anti_alias != 0, w, h, cov);
     blit(s, cov, px, w, h);
@@ -1030,11 +880,7 @@ void path_coverage_of(const rvk_scene &sc, const rvk_path &path, Ts ts, bool str
     std::vector<Seg> segs;
     uint32_t rule = path.fill_rule;
     if (stroke) {
-        std::vector<Poly> polys;
-        stroke_outline(sc, path, ts, polys);
-        for (auto &p : polys)
-            for (size_t i = 0; i < p.pts.size(); i++)
-                segs.push_back(Seg{2, {p.pts[i], p.pts[(i + 1) % p.pts.size()]}});
+        stroke_segments(sc, path, ts, segs);
         rule = RVK_FILL_NONZERO;
     } else {
         fill_segments(sc, path, ts, segs);

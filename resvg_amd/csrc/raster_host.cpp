@@ -1,5 +1,5 @@
 // Host front end of the rasterizer: path iteration, Skia edge stepping expanded
-// into line pieces, the polyline stroker and shader setup.
+// into line pieces, stroke outlines (stroker.cpp) and shader setup.
 //
 // Edge arithmetic restates tiny-skia 0.12 (Skia's SkEdge: FDot6 inputs on the 4x
 // supersampled grid, 16.16 slopes, quadratic and cubic forward differencing with
@@ -7,6 +7,7 @@
 // curve edge is run here once and each non-empty line piece is kept, which is the
 // sequence tiny-skia's scanline walk visits lazily.
 #include "raster.h"
+#include "stroker.h"
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -391,180 +392,66 @@ void build_pieces(const std::vector<Seg> &segs, int shift, std::vector<Piece> &o
 }
 
 // ---------------------------------------------------------------------------
-// Stroker: polyline outline pieces, each oriented with negative signed area so
-// that their nonzero union is the stroke (same specification as the oracle).
+// Strokes: tiny-skia's PathStroker (stroker.cpp) in user space, the outline mapped
+// with the transform and handed to the fill path (Pixmap::stroke_path -> fill_path).
 // ---------------------------------------------------------------------------
-namespace {
-
-float res_scale_for(Ts ts) {
-    float sx = std::sqrt(ts.sx * ts.sx + ts.kx * ts.kx), sy = std::sqrt(ts.ky * ts.ky + ts.sy * ts.sy);
-    if (std::isfinite(sx) && std::isfinite(sy)) {
-        float s = std::max(sx, sy);
-        if (s > 0.0f) return s;
-    }
-    return 1.0f;
-}
-
-int steps_for(const Pt *p, int n, float tol) {
-    float dd = 0.0f;
-    for (int i = 0; i + 2 < n; i++) {
-        float dx = p[i].x - 2.0f * p[i + 1].x + p[i + 2].x, dy = p[i].y - 2.0f * p[i + 1].y + p[i + 2].y;
-        dd = std::max(dd, std::sqrt(dx * dx + dy * dy));
-    }
-    float k = n == 4 ? 0.75f : 0.25f;
-    int steps = (int)std::ceil(std::sqrt(k * dd / tol));
-    return std::max(1, std::min(steps, 256));
-}
-
-Pt bezier(const Pt *p, int n, float t) {
-    float u = 1.0f - t;
-    if (n == 3) {
-        float a = u * u, b = 2.0f * u * t, c = t * t;
-        return Pt{a * p[0].x + b * p[1].x + c * p[2].x, a * p[0].y + b * p[1].y + c * p[2].y};
-    }
-    float a = u * u * u, b = 3.0f * u * u * t, c = 3.0f * u * t * t, d = t * t * t;
-    return Pt{a * p[0].x + b * p[1].x + c * p[2].x + d * p[3].x, a * p[0].y + b * p[1].y + c * p[2].y + d * p[3].y};
-}
-
-struct Polyline { std::vector<Pt> pts; bool closed; };
-
-void polylines(const rvk_scene &sc, const rvk_path &path, float tol, std::vector<Polyline> &out) {
+void stroke_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
+    rvks::StrokeParams sp{path.stroke_width, path.stroke_miter_limit, path.stroke_cap, path.stroke_join};
+    const float res = rvks::resolution_scale(ts.sx, ts.ky, ts.kx, ts.sy);
     const uint8_t *verbs = sc.verbs + path.verb_offset;
     const float *pts = sc.points + 2 * (size_t)path.point_offset;
+    size_t npts = 0;
+    for (uint32_t v = 0; v < path.verb_count; v++)
+        npts += verbs[v] == RVK_VERB_MOVE || verbs[v] == RVK_VERB_LINE ? 1 : verbs[v] == RVK_VERB_QUAD ? 2
+              : verbs[v] == RVK_VERB_CUBIC ? 3 : 0;
+    rvks::PathBuilder outline;
+    if (!rvks::stroke(verbs, path.verb_count, pts, npts, sp, res, outline)) return;
     size_t pi = 0;
-    Polyline cur{{}, false};
-    auto add = [&](Pt p) {
-        if (cur.pts.empty() || cur.pts.back().x != p.x || cur.pts.back().y != p.y) cur.pts.push_back(p);
+    Pt move{0, 0}, last{0, 0};
+    bool open = false;
+    auto P = [&](size_t i) {
+        float x = outline.pts[i].x, y = outline.pts[i].y;
+        ts_map(ts, x, y);
+        return Pt{x, y};
     };
-    auto next = [&]() { Pt p{pts[2 * pi], pts[2 * pi + 1]}; pi++; return p; };
-    auto finish = [&]() {
-        if (cur.pts.size() >= 2) out.push_back(cur);
-        cur = Polyline{{}, false};
+    auto close = [&]() {
+        if (open && (last.x != move.x || last.y != move.y)) out.push_back(Seg{2, {last, move}});
+        open = false;
     };
-    for (uint32_t v = 0; v < path.verb_count; v++) {
-        uint8_t verb = verbs[v];
-        if (verb == RVK_VERB_MOVE) {
-            finish();
-            add(next());
-        } else if (verb == RVK_VERB_LINE) {
-            add(next());
-        } else if (verb == RVK_VERB_QUAD || verb == RVK_VERB_CUBIC) {
-            int n = verb == RVK_VERB_QUAD ? 3 : 4;
-            Pt c[4];
-            c[0] = cur.pts.empty() ? Pt{0, 0} : cur.pts.back();
-            for (int k = 1; k < n; k++) c[k] = next();
-            int steps = steps_for(c, n, tol);
-            for (int s = 1; s < steps; s++) add(bezier(c, n, (float)s / (float)steps));
-            add(c[n - 1]);
-        } else {  // close
-            if (cur.pts.size() >= 2 && cur.pts.front().x == cur.pts.back().x && cur.pts.front().y == cur.pts.back().y)
-                cur.pts.pop_back();
-            cur.closed = true;
-            Pt start = cur.pts.empty() ? Pt{0, 0} : cur.pts.front();
-            finish();
-            add(start);
-        }
-    }
-    finish();
-}
-
-void piece(std::vector<Pt> &poly, Ts ts, std::vector<Seg> &out) {
-    if (poly.size() < 3) return;
-    double a = 0.0;
-    for (size_t i = 0; i < poly.size(); i++) {
-        const Pt &u = poly[i], &v = poly[(i + 1) % poly.size()];
-        a += (double)u.x * v.y - (double)v.x * u.y;
-    }
-    if ((float)a > 0.0f) std::reverse(poly.begin(), poly.end());
-    for (auto &p : poly) ts_map(ts, p.x, p.y);
-    for (size_t i = 0; i < poly.size(); i++) out.push_back(Seg{2, {poly[i], poly[(i + 1) % poly.size()]}});
-}
-
-void arc(Pt c, float hw, float a0, float a1, float tol, std::vector<Pt> &out) {
-    float span = a1 - a0;
-    float step = 2.0f * std::acos(std::max(-1.0f, std::min(1.0f, 1.0f - tol / std::max(hw, tol))));
-    int n = std::max(1, (int)std::ceil(std::fabs(span) / std::max(step, 1e-3f)));
-    n = std::min(n, 64);
-    for (int i = 0; i <= n; i++) {
-        float a = a0 + span * ((float)i / (float)n);
-        out.push_back(Pt{c.x + hw * std::cos(a), c.y + hw * std::sin(a)});
-    }
-}
-
-}  // namespace
-
-void stroke_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
-    const float hw = path.stroke_width * 0.5f;
-    if (!(hw > 0.0f)) return;
-    const float tol = 0.25f / res_scale_for(ts);
-    std::vector<Polyline> lines;
-    polylines(sc, path, tol, lines);
-    std::vector<Pt> poly;
-    for (const Polyline &c : lines) {
-        const std::vector<Pt> &p = c.pts;
-        const size_t n = p.size(), nseg = c.closed ? n : n - 1;
-        if (nseg == 0) continue;
-        auto dir = [&](size_t i) {
-            Pt a = p[i], b = p[(i + 1) % n];
-            float dx = b.x - a.x, dy = b.y - a.y, l = std::sqrt(dx * dx + dy * dy);
-            return Pt{dx / l, dy / l};
-        };
-        for (size_t i = 0; i < nseg; i++) {
-            Pt a = p[i], b = p[(i + 1) % n], d = dir(i), nm{-d.y * hw, d.x * hw};
-            poly = {Pt{a.x + nm.x, a.y + nm.y}, Pt{b.x + nm.x, b.y + nm.y}, Pt{b.x - nm.x, b.y - nm.y},
-                    Pt{a.x - nm.x, a.y - nm.y}};
-            piece(poly, ts, out);
-        }
-        size_t first = c.closed ? 0 : 1, last = c.closed ? n : n - 1;
-        for (size_t v = first; v < last; v++) {
-            size_t i0 = c.closed ? (v + nseg - 1) % nseg : v - 1, i1 = c.closed ? v % nseg : v;
-            Pt d0 = dir(i0), d1 = dir(i1), q = p[v];
-            float cross = d0.x * d1.y - d0.y * d1.x, dot = d0.x * d1.x + d0.y * d1.y;
-            if (std::fabs(cross) < 1e-6f && dot > 0.0f) continue;
-            float s = cross > 0.0f ? -1.0f : 1.0f;
-            Pt n0{-d0.y * hw * s, d0.x * hw * s}, n1{-d1.y * hw * s, d1.x * hw * s};
-            Pt e0{q.x + n0.x, q.y + n0.y}, e1{q.x + n1.x, q.y + n1.y};
-            if (path.stroke_join == RVK_JOIN_ROUND) {
-                float a0 = std::atan2(n0.y, n0.x), a1 = std::atan2(n1.y, n1.x), da = a1 - a0;
-                while (da > 3.14159265f) da -= 6.2831853f;
-                while (da < -3.14159265f) da += 6.2831853f;
-                poly = {q};
-                arc(q, hw, a0, a0 + da, tol, poly);
-                piece(poly, ts, out);
-            } else {
-                bool miter = path.stroke_join == RVK_JOIN_MITER || path.stroke_join == RVK_JOIN_MITER_CLIP;
-                float cos_t = n0.x * n1.x / (hw * hw) + n0.y * n1.y / (hw * hw);
-                float ratio2 = 2.0f / std::max(1e-12f, 1.0f + cos_t);
-                if (miter && ratio2 <= path.stroke_miter_limit * path.stroke_miter_limit) {
-                    Pt m{n0.x + n1.x, n0.y + n1.y};
-                    float ml = std::sqrt(m.x * m.x + m.y * m.y), len = hw * std::sqrt(ratio2);
-                    poly = {q, e0, Pt{q.x + m.x / ml * len, q.y + m.y / ml * len}, e1};
-                } else {
-                    poly = {q, e0, e1};
-                }
-                piece(poly, ts, out);
+    for (uint8_t v : outline.verbs) {
+        switch (v) {
+            case rvks::kMove:
+                close();
+                move = last = P(pi++);
+                open = true;
+                break;
+            case rvks::kLine: {
+                const Pt p = P(pi++);
+                out.push_back(Seg{2, {last, p}});
+                last = p;
+                break;
             }
-        }
-        if (!c.closed) {
-            for (int end = 0; end < 2; end++) {
-                Pt q = end == 0 ? p[0] : p[n - 1];
-                Pt d = end == 0 ? dir(0) : dir(n - 2);
-                if (end == 0) d = Pt{-d.x, -d.y};
-                Pt nm{-d.y * hw, d.x * hw};
-                if (path.stroke_cap == RVK_CAP_SQUARE) {
-                    Pt f{d.x * hw, d.y * hw};
-                    poly = {Pt{q.x + nm.x, q.y + nm.y}, Pt{q.x + nm.x + f.x, q.y + nm.y + f.y},
-                            Pt{q.x - nm.x + f.x, q.y - nm.y + f.y}, Pt{q.x - nm.x, q.y - nm.y}};
-                    piece(poly, ts, out);
-                } else if (path.stroke_cap == RVK_CAP_ROUND) {
-                    float a0 = std::atan2(nm.y, nm.x);
-                    poly = {q};
-                    arc(q, hw, a0, a0 - 3.14159265f, tol, poly);
-                    piece(poly, ts, out);
-                }
+            case rvks::kQuad: {
+                const Pt a = P(pi), b = P(pi + 1);
+                pi += 2;
+                out.push_back(Seg{3, {last, a, b}});
+                last = b;
+                break;
             }
+            case rvks::kCubic: {
+                const Pt a = P(pi), b = P(pi + 1), c = P(pi + 2);
+                pi += 3;
+                out.push_back(Seg{4, {last, a, b, c}});
+                last = c;
+                break;
+            }
+            default:
+                close();
+                last = move;
+                break;
         }
     }
+    close();
 }
 
 // ---------------------------------------------------------------------------

@@ -159,10 +159,11 @@ struct Renderer {
                 }
             }
         }
-        // path::stroke_path (path.rs:77-116), outline by this build's stroker
-        if (p.stroke.kind != RVK_PAINT_NONE && make_shader(sc, p.stroke, ts, sp)) {
+        // path::stroke_path (path.rs:77-116): tiny-skia's stroker outline, filled nonzero
+        if (p.stroke.kind != RVK_PAINT_NONE && p.stroke_width >= 0.0f && make_shader(sc, p.stroke, ts, sp)) {
             std::vector<Seg> segs;
             stroke_segments(sc, p, ts, segs);
+            if (segs.empty() || bounds_nearly_zero(segs)) return;
             it.stroke_job = add_job(dl, segs, RVK_FILL_NONZERO, p.anti_alias != 0, clip);
             if (it.stroke_job >= 0) {
                 it.stroke_shader = (int32_t)dl.shaders.size();

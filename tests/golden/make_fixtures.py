@@ -38,6 +38,14 @@ OTHER = [
     "painting/mix-blend-mode/multiply", "painting/mix-blend-mode/screen", "painting/mix-blend-mode/hue",
     "painting/mix-blend-mode/darken", "painting/mix-blend-mode/color-burn", "painting/mix-blend-mode/normal",
     "shapes/circle/simple-case", "shapes/ellipse/simple-case", "shapes/rect/rounded-rect",
+    "painting/stroke-linejoin/miter", "painting/stroke-linejoin/round", "painting/stroke-linejoin/bevel",
+    "painting/stroke-linejoin/arcs", "painting/stroke-linejoin/miter-clip",
+    "painting/stroke-linecap/butt", "painting/stroke-linecap/round", "painting/stroke-linecap/square",
+    "painting/stroke-linecap/open-path-with-butt", "painting/stroke-linecap/open-path-with-round",
+    "painting/stroke-linecap/open-path-with-square", "painting/stroke-linecap/zero-length-path-with-butt",
+    "painting/stroke-linecap/zero-length-path-with-round", "painting/stroke-linecap/zero-length-path-with-square",
+    "shapes/path/M-C", "shapes/path/M-Q", "shapes/path/M-C-S", "shapes/path/M-Q-T", "shapes/path/M-L-L-Z",
+    "painting/stroke-miterlimit/default", "painting/stroke-miterlimit/valid-value",
 ]
 
 

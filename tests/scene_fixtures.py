@@ -99,6 +99,7 @@ def shape_fixtures():
     sc.path(g, v, p, fill=PaintSpec.solid(GREEN))
     out["painting/opacity/50percent"] = sc
     out.update(_curve_fixtures())
+    out.update(_stroke_fixtures())
     for mode, name, col in [("multiply", "multiply", YELLOW), ("screen", "screen", (0, 0, 255)),
                             ("hue", "hue", YELLOW), ("darken", "darken", YELLOW),
                             ("color_burn", "color-burn", YELLOW), ("normal", "normal", YELLOW)]:
@@ -269,6 +270,67 @@ def _curve_fixtures():
     v, p = rounded_rect_usvg(20.0, 20.0, 160.0, 160.0, 20.0, 20.0)
     sc.path(sc.root, v, p, fill=PaintSpec.solid(GREEN))
     out["shapes/rect/rounded-rect"] = sc
+    return out
+
+
+def _stroke_fixtures():
+    """painting/stroke-linejoin/* and painting/stroke-linecap/*: the stroker's geometry."""
+    out = {}
+    M, L, C, Z = abi.VERB_MOVE, abi.VERB_LINE, abi.VERB_CUBIC, abi.VERB_CLOSE
+    # M 25 130 c 0 -40 50 -75 100 -75 c -35 20 -50 50 -50 75 (relative cubics made absolute)
+    jv = [M, C, C]
+    jp = [(25, 130), (25, 90), (75, 55), (125, 55), (90, 75), (75, 105), (75, 130)]
+    for name, join in (("miter", abi.JOIN_MITER), ("round", abi.JOIN_ROUND), ("bevel", abi.JOIN_BEVEL),
+                       ("arcs", abi.JOIN_MITER)):  # usvg parses the unknown "arcs" as the default
+        sc = Scene(200, 200)
+        sc.path(sc.root, jv, jp, stroke=PaintSpec.solid(GREEN), stroke_width=8.0, stroke_join=join)
+        out[f"painting/stroke-linejoin/{name}"] = sc
+    sc = Scene(200, 200)
+    sc.path(sc.root, jv, jp, stroke=PaintSpec.solid(GREEN), stroke_width=8.0, stroke_join=abi.JOIN_MITER_CLIP,
+            miter_limit=2.0)
+    sc.path(sc.root, jv, jp, stroke=PaintSpec.solid(BLACK), stroke_width=8.0, stroke_join=abi.JOIN_MITER,
+            miter_limit=2.0)
+    out["painting/stroke-linejoin/miter-clip"] = sc
+    for name, cap in (("butt", abi.CAP_BUTT), ("round", abi.CAP_ROUND), ("square", abi.CAP_SQUARE)):
+        sc = Scene(200, 200)
+        sc.path(sc.root, [M, L], [(40, 40), (160, 160)], stroke=PaintSpec.solid(GREEN), stroke_width=10.0,
+                stroke_cap=cap)
+        out[f"painting/stroke-linecap/{name}"] = sc
+        sc = Scene(200, 200)
+        sc.path(sc.root, [M, L, L, L], [(150, 50), (150, 130), (50, 90), (150, 50)], fill=PaintSpec.solid(GREEN),
+                stroke=PaintSpec.solid((0x33, 0x99, 0xDD)), stroke_width=20.0, stroke_cap=cap)
+        out[f"painting/stroke-linecap/open-path-with-{name}"] = sc
+    Q = abi.VERB_QUAD
+    green5 = dict(stroke=PaintSpec.solid(GREEN), stroke_width=5.0)
+    for name, verbs, pts in (
+            ("M-C", [M, C], [(30, 40), (16, 137), (171, 45), (180, 155)]),
+            ("M-Q", [M, Q], [(30, 40), (171, 45), (180, 155)]),
+            # S and T reflect the previous control point about the current point
+            ("M-C-S", [M, C, C], [(30, 40), (16, 137), (171, 45), (100, 90), (29, 135), (171, 45), (180, 155)]),
+            ("M-Q-T", [M, Q, Q], [(30, 40), (171, 45), (100, 90), (29, 135), (160, 180)]),
+            ("M-L-L-Z", [M, L, L, Z], [(30, 40), (100, 150), (160, 110)])):
+        sc = Scene(200, 200)
+        sc.path(sc.root, verbs, pts, **green5)
+        out[f"shapes/path/{name}"] = sc
+    for name, limit in (("default", 4.0), ("valid-value", 10.0)):
+        sc = Scene(200, 200)
+        sc.path(sc.root, [M, L, L], [(40, 70), (120, 90), (40, 110)], stroke=PaintSpec.solid(GREEN),
+                stroke_width=30.0, miter_limit=limit)
+        out[f"painting/stroke-miterlimit/{name}"] = sc
+    sc = Scene(200, 200)
+    sc.path(sc.root, [M, L], [(100, 100), (100, 100)], fill=PaintSpec.solid(RED), stroke=PaintSpec.solid(RED),
+            stroke_width=20.0, stroke_cap=abi.CAP_BUTT)
+    out["painting/stroke-linecap/zero-length-path-with-butt"] = sc
+    for name, cap in (("round", abi.CAP_ROUND), ("square", abi.CAP_SQUARE)):
+        sc = Scene(200, 200)
+        kw = dict(fill=PaintSpec.solid(RED), stroke=PaintSpec.solid(GREEN), stroke_width=20.0, stroke_cap=cap)
+        sc.path(sc.root, [M, L], [(100, 70), (100, 70)], **kw)
+        sc.path(sc.root, [M, L, M, L], [(70, 100), (70, 100), (130, 100), (130, 100)], **kw)
+        if cap == abi.CAP_SQUARE:
+            sc.path(sc.root, [M, L, L], [(100, 130), (100, 130), (100, 130)], **kw)
+        else:
+            sc.path(sc.root, [M, L], [(100, 130), (100, 130)], **kw)
+        out[f"painting/stroke-linecap/zero-length-path-with-{name}"] = sc
     return out
 
 
