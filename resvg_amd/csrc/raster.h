@@ -1,10 +1,10 @@
 // Host-side rasterizer front end and device data layouts shared with raster.hip.
 //
-// A fill or stroke becomes a coverage job: a list of line pieces in 16.16 fixed
-// point on the supersampled grid (Skia's edge stepping, tiny-skia edge.rs, run to
-// completion on the host), bucketed by bands of supersampled rows, plus the pixel
-// rectangle whose coverage the device computes.  Shading parameters are plain
-// structs evaluated per pixel by the compositor.
+// A fill or stroke becomes a coverage job: the Skia edges of its path on the
+// supersampled grid (tiny-skia edge.rs), set up on the host up to their first line
+// piece, and the pixel rectangle whose coverage the device computes.  The device
+// expands every curve edge into its remaining line pieces (k_expand, the
+// forward-differencing update loop) and then computes coverage row by row.
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -12,27 +12,47 @@
 
 namespace rvkr {
 
-// One line piece of an edge: active on supersampled rows [first_y, last_y]; its
-// 16.16 x at row y is x + (y - first_y) * dx.  `id` orders pieces with equal x.
-struct Piece {
+// One edge as the host hands it over: its first line piece (16.16 x and slope on
+// the supersampled rows first_y..last_y) and, for curves, the stepping state after
+// that piece (tiny-skia QuadraticEdge / CubicEdge fields).
+struct EdgeRec {
     int32_t x, dx, first_y, last_y;
-    int32_t winding, id;
+    int32_t type;                 // 0 line, 1 quadratic, 2 cubic
+    int32_t count;                // curve_count after the first piece
+    int32_t shift, dshift;        // curve_shift; cubic_dshift
+    int32_t cx, cy, lx, ly;       // current point, last point (16.16)
+    int32_t d1x, d1y, d2x, d2y;   // quad: qdx qdy qddx qddy; cubic: cdx cdy cddx cddy
+    int32_t d3x, d3y;             // cubic: cdddx cdddy
+    uint32_t piece_offset;        // first slot of this edge's pieces
+    int32_t winding;              // +1 downwards, -1 upwards
 };
 
-constexpr int kBandShift = 4;  // 16 supersampled rows (4 pixel rows) per band
+// What the coverage kernel reads per edge.  Edges of a job are sorted by first_y;
+// ymax is an upper bound of the last covered row.
+struct EdgeCov {
+    int32_t first_y, ymax;
+    int32_t winding, id;          // id: position in tiny-skia's edge list (orders equal x)
+    uint32_t piece_offset, pad_;
+};
+
+// A line piece written by k_expand.
+struct PieceDev {
+    int32_t x, dx, first_y, last_y;
+};
+
+constexpr int kGroupRows = 4;  // pixel rows per entry of a job's edge-window table
 
 // A coverage job computes u8 coverage for the pixel rect [x0, x0+w) x [y0, y0+h)
-// of its pixmap (coordinates of the edge space) into cov[(y - y0) * w + (x - x0)].
+// of its pixmap (coordinates of the edge space) into cov[(y - y0) * stride + (x - x0)],
+// stride = w rounded up to 4.  For every group of kGroupRows pixel rows the window
+// table holds the range of the job's edges that can touch the group.
 struct CovJob {
     int32_t x0, y0, w, h;
-    uint32_t piece_offset, piece_count;
-    uint32_t band_offset;  // into band_start: bands cover rows y0*scale .. (y0+h)*scale
-    uint32_t band_count;
-    int32_t band_y0;       // first supersampled row of band 0
+    uint32_t edge_offset, edge_count;
+    uint32_t window_offset;  // into the window table (uint2 lo, hi per row group)
     uint32_t rule, aa;
-    uint64_t cov_offset;   // byte offset into the coverage arena
-    uint32_t row_start;    // prefix over jobs of h (wave index of the first row)
-    uint32_t pad_;
+    uint32_t row_start;      // prefix over jobs of h (wave index of the first row)
+    uint64_t cov_offset;     // byte offset into the coverage arena
 };
 
 // Shader parameters (tiny-skia lowp stages for solid, linear and concentric radial).
@@ -71,11 +91,13 @@ struct Seg { int n; Pt p[4]; };
 
 // Path segments in device space with every contour closed (fill semantics).
 void fill_segments(const rvk_scene &sc, const rvk_path &path, rvkh::Ts ts, std::vector<Seg> &out);
-// Stroke outline polygons (this build's polyline stroker) in device space.
+// Stroke outline (tiny-skia's stroker, stroker.cpp) in device space, contours closed.
 void stroke_segments(const rvk_scene &sc, const rvk_path &path, rvkh::Ts ts, std::vector<Seg> &out);
-// Skia edges of `segs` on the supersampled grid, expanded into line pieces; returns
-// the supersampled row range [ymin, ymax] (ymin > ymax when empty).
-void build_pieces(const std::vector<Seg> &segs, int shift, std::vector<Piece> &out, int32_t &ymin, int32_t &ymax);
+// Skia edges of `segs` on the supersampled grid (lines combined like tiny-skia's
+// edge builder, curves chopped at Y extrema), each set up to its first piece, in
+// tiny-skia's edge-list order (ids 0..n-1).  Returns the maximum number of pieces
+// each edge can produce in `caps`.
+void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &out, std::vector<uint32_t> &caps);
 // tiny-skia shader setup; false when the paint draws nothing.
 bool make_shader(const rvk_scene &sc, const rvk_paint &paint, rvkh::Ts ts, ShaderParams &out);
 // Control-point bounds of device-space segments nearly zero (tiny-skia skips such fills).

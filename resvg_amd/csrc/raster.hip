@@ -1,13 +1,18 @@
-// Device rasterizer: anti-aliased coverage and the fused shade/composite of draw
-// lists.
+// Device rasterizer: curve expansion, anti-aliased coverage and the fused
+// shade/composite of draw lists.
 //
-// Coverage (k_coverage): one wave per (coverage job, pixel row).  For each of the
-// row's four sub-scanlines the wave gathers the active line pieces of the row's
-// band, computes each crossing's 16.16 x, sorts the crossings by (x, edge id) with
-// a bitonic sort in LDS, scans the windings to find the spans tiny-skia's walk
-// would blit, and accumulates the SuperBlitter contributions (16 per covered
-// sub-column of a partial pixel, 64 or 63 for a fully covered pixel) into an LDS
-// row; full-pixel runs go through a difference array.  The row is then prefix-summed,
+// Expansion (k_expand): one thread per edge runs tiny-skia's quadratic / cubic
+// update loop (Skia's forward differencing in 16.16 fixed point) from the state the
+// host left after the first piece, writing every non-empty line piece in order.
+//
+// Coverage (k_coverage): one 64-lane wave per (coverage job, pixel row).  The row's
+// edge window comes from the job's table; each lane takes an edge, binary-searches
+// its pieces for the row and produces the crossings of the row's sub-scanlines.
+// Per sub-scanline the crossings (keys: x, edge id, winding) are sorted with a
+// register-resident bitonic network (up to 8 keys per lane), the windings are
+// scanned into span events (tiny-skia's walk_edges spans), and the SuperBlitter
+// contributions (16 per covered sub-column of a partial pixel, 64 or 63 for a fully
+// covered pixel) are accumulated per 1024-pixel window in LDS, prefix-summed,
 // clamped to 255 and written as u8.
 //
 // Compositing (k_draw): one thread per target pixel of a 64x4 tile, looping over
@@ -22,21 +27,123 @@
 using namespace rvkd;
 using rvkr::CovJob;
 using rvkr::DrawItem;
-using rvkr::Piece;
+using rvkr::EdgeCov;
+using rvkr::EdgeRec;
+using rvkr::PieceDev;
 using rvkr::ShaderParams;
 
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kMaxX = 1024;       // crossings per sub-scanline a wave can sort
-constexpr int kRowWindow = 4608;  // pixels accumulated per pass over a row
+constexpr int kMaxKeys = 512;     // crossings per sub-scanline (8 per lane)
+constexpr int kWin = 1024;        // pixels accumulated per window
+constexpr int kMaxSub = 4;        // sub-scanlines per pixel row
 
+// ---------------------------------------------------------------------------
+// expansion
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t fdot6_round(int32_t x) { return (x + 32) >> 6; }
+__device__ __forceinline__ int32_t fdot6_to_fixed(int32_t x) { return (int32_t)((uint32_t)x << 10); }
+__device__ __forceinline__ int32_t fixed_mul(int32_t a, int32_t b) {
+    return (int32_t)(((int64_t)a * (int64_t)b) >> 16);
+}
+__device__ __forceinline__ int32_t fdot6_div(int32_t a, int32_t b) {
+    if ((int32_t)(int16_t)a == a) return (int32_t)((uint32_t)a << 16) / b;
+    int64_t v = ((int64_t)a << 16) / (int64_t)b;
+    return (int32_t)max((int64_t)INT32_MIN, min((int64_t)INT32_MAX, v));
+}
+
+// SkEdge::updateLine on 16.16 inputs
+__device__ __forceinline__ bool line_piece(int32_t x0, int32_t y0, int32_t x1, int32_t y1, PieceDev &p) {
+    y0 >>= 10;
+    y1 >>= 10;
+    const int32_t top = fdot6_round(y0), bot = fdot6_round(y1);
+    if (top == bot) return false;
+    x0 >>= 10;
+    x1 >>= 10;
+    const int32_t slope = fdot6_div(x1 - x0, y1 - y0);
+    const int32_t dy = ((top << 6) + 32) - y0;
+    p.x = fdot6_to_fixed(x0 + fixed_mul(slope, dy));
+    p.dx = slope;
+    p.first_y = top;
+    p.last_y = bot - 1;
+    return true;
+}
+
+struct ExpandArgs {
+    const EdgeRec *edges;
+    uint32_t nedges;
+    const EdgeCov *cov;
+    PieceDev *pieces;
+    uint32_t *npieces;
+    int32_t *overflow;
+};
+
+__global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nedges) return;
+    EdgeRec e = a.edges[i];
+    PieceDev *out = a.pieces + e.piece_offset;
+    const int32_t ymax = a.cov[i].ymax;
+    uint32_t n = 0;
+    out[n++] = PieceDev{e.x, e.dx, e.first_y, e.last_y};
+    int32_t oldx = e.cx, oldy = e.cy;
+    if (e.type == 1) {  // SkQuadraticEdge::updateQuadratic, repeated until exhausted
+        int32_t count = e.count, dx = e.d1x, dy = e.d1y;
+        while (count > 0) {
+            int32_t newx, newy;
+            if (--count > 0) {
+                newx = oldx + (dx >> e.shift);
+                dx += e.d2x;
+                newy = oldy + (dy >> e.shift);
+                dy += e.d2y;
+            } else {
+                newx = e.lx;
+                newy = e.ly;
+            }
+            PieceDev p;
+            if (line_piece(oldx, oldy, newx, newy, p)) out[n++] = p;
+            oldx = newx;
+            oldy = newy;
+        }
+    } else if (e.type == 2) {  // SkCubicEdge::updateCubic
+        int32_t count = e.count;
+        int32_t d1x = e.d1x, d1y = e.d1y, d2x = e.d2x, d2y = e.d2y;
+        while (count < 0) {
+            int32_t newx, newy;
+            if (++count < 0) {
+                newx = oldx + (d1x >> e.dshift);
+                d1x += d2x >> e.shift;
+                d2x += e.d3x;
+                newy = oldy + (d1y >> e.dshift);
+                d1y += d2y >> e.shift;
+                d2y += e.d3y;
+            } else {
+                newx = e.lx;
+                newy = e.ly;
+            }
+            if (newy < oldy) newy = oldy;
+            PieceDev p;
+            if (line_piece(oldx, oldy, newx, newy, p)) out[n++] = p;
+            oldx = newx;
+            oldy = newy;
+        }
+    }
+    a.npieces[i] = n;
+    // the host's row bound must cover every piece (else rows would miss the edge)
+    if (out[n - 1].last_y > ymax) atomicAdd(a.overflow, 1);
+}
+
+// ---------------------------------------------------------------------------
+// coverage
+// ---------------------------------------------------------------------------
 struct CovArgs {
     const CovJob *jobs;
     uint32_t njobs;
-    const Piece *pieces;
-    const uint32_t *band_start;
-    const uint32_t *band_items;
+    const EdgeCov *edges;
+    const uint2 *windows;
+    const PieceDev *pieces;
+    const uint32_t *npieces;
     uint8_t *arena;
     int32_t *overflow;
 };
@@ -59,10 +166,92 @@ __device__ __forceinline__ uint32_t find_job(const CovJob *jobs, uint32_t n, uin
     return lo;
 }
 
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, kWave);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Ascending bitonic sort of the 64*K keys held K per lane (element lane*K + j).
+template <int K>
+__device__ __forceinline__ void wave_sort(uint64_t (&v)[K], int lane) {
+    constexpr int N = kWave * K;
+    for (int k = 2; k <= N; k <<= 1) {
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            if (jj >= K) {
+                const int ld = jj / K;
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    const int e = lane * K + j;
+                    const uint64_t o = shfl_xor64(v[j], ld);
+                    const bool up = (e & k) == 0, lower = (e & jj) == 0;
+                    const uint64_t mn = v[j] < o ? v[j] : o, mx = v[j] < o ? o : v[j];
+                    v[j] = (lower == up) ? mn : mx;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    const int p = j ^ jj;
+                    if (p > j) {
+                        const int e = lane * K + j;
+                        const bool up = (e & k) == 0;
+                        const uint64_t x = v[j], y = v[p];
+                        const bool sw = (x > y) == up;
+                        v[j] = sw ? y : x;
+                        v[p] = sw ? x : y;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Sorts the n keys of one sub-scanline and writes its span events (pixel-rounded
+// supersampled columns; starts and ends alternate) to ev.  Returns the count.
+template <int K>
+__device__ __forceinline__ int spans_of(const uint64_t *keys, int n, bool evenodd, int32_t *ev, int lane) {
+    uint64_t v[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int e = lane * K + j;
+        v[j] = e < n ? keys[e] : ~0ull;
+    }
+    __syncthreads();  // every key is in registers before the key area is reused for events
+    wave_sort<K>(v, lane);
+    int w[K], local = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int e = lane * K + j;
+        w[j] = e < n ? ((v[j] & 1) ? 1 : -1) : 0;
+        local += w[j];
+    }
+    const int incl = wave_incl_scan(local, lane);
+    int before = incl - local;
+    bool evt[K];
+    int nevt = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int after = before + w[j];
+        const bool in_b = evenodd ? (before & 1) != 0 : before != 0;
+        const bool in_a = evenodd ? (after & 1) != 0 : after != 0;
+        evt[j] = w[j] != 0 && in_b != in_a;
+        nevt += evt[j] ? 1 : 0;
+        before = after;
+    }
+    const int eincl = wave_incl_scan(nevt, lane);
+    int pos = eincl - nevt;
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        if (evt[j]) {
+            const int32_t x = (int32_t)((uint32_t)(v[j] >> 32) ^ 0x80000000u);
+            ev[pos++] = (x + 0x8000) >> 16;
+        }
+    return __shfl(eincl, kWave - 1, kWave);
+}
+
 __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
-    __shared__ uint64_t keys[kMaxX];
-    __shared__ int32_t ev[kMaxX];
-    __shared__ int32_t row[kRowWindow + 1];  // acc (low 16 bits) + diff (high 16 bits, signed)
+    __shared__ uint64_t keys[kMaxSub][kMaxKeys];  // crossings; reused for span events
+    __shared__ int32_t acc[kWin + 1];             // acc (low 16 bits) + run difference (high 16 bits)
     const int lane = threadIdx.x;
     const uint32_t gr = blockIdx.x;
     const uint32_t ji = find_job(a.jobs, a.njobs, gr);
@@ -70,129 +259,113 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
     const int32_t r = (int32_t)(gr - job.row_start);
     if (r < 0 || r >= job.h) return;
     const int shift = job.aa ? 2 : 0;
-    const int32_t scale = 1 << shift;
-    const int32_t py = job.y0 + r;
+    const int nsub = 1 << shift;
+    const int32_t sy0 = (job.y0 + r) << shift;
     const uint32_t stride = ((uint32_t)job.w + 3u) & ~3u;
     uint8_t *out = a.arena + job.cov_offset + (size_t)r * stride;
     const bool evenodd = job.rule == RVK_FILL_EVENODD;
 
-    for (int32_t wx0 = 0; wx0 < job.w; wx0 += kRowWindow) {
-        const int32_t ww = min(kRowWindow, job.w - wx0);
-        for (int i = lane; i <= ww; i += kWave) row[i] = 0;
+    // ---- gather the crossings of the row's sub-scanlines ----
+    const uint2 win = a.windows[job.window_offset + (uint32_t)r / rvkr::kGroupRows];
+    int cnt[kMaxSub] = {0, 0, 0, 0};
+    for (uint32_t base = win.x; base < win.y; base += kWave) {
+        const uint32_t i = base + lane;
+        uint64_t key[kMaxSub];
+        bool act[kMaxSub] = {false, false, false, false};
+        if (i < win.y) {
+            const EdgeCov e = a.edges[job.edge_offset + i];
+            if (e.first_y <= sy0 + nsub - 1 && e.ymax >= sy0) {
+                const PieceDev *pc = a.pieces + e.piece_offset;
+                const uint32_t n = a.npieces[job.edge_offset + i];
+                uint32_t lo = 0, hi = n;  // first piece with last_y >= sy0
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (pc[mid].last_y < sy0) lo = mid + 1;
+                    else hi = mid;
+                }
+                uint32_t k = lo;
+                for (int s = 0; s < nsub; s++) {
+                    const int32_t sy = sy0 + s;
+                    while (k < n && pc[k].last_y < sy) k++;
+                    if (k < n && pc[k].first_y <= sy) {
+                        const PieceDev p = pc[k];
+                        const int32_t x = (int32_t)((uint32_t)p.x + (uint32_t)(sy - p.first_y) * (uint32_t)p.dx);
+                        key[s] = ((uint64_t)((uint32_t)x ^ 0x80000000u) << 32) | ((uint64_t)(uint32_t)e.id << 1) |
+                                 (e.winding > 0 ? 1u : 0u);
+                        act[s] = true;
+                    }
+                }
+            }
+        }
+        for (int s = 0; s < nsub; s++) {
+            const uint64_t m = __ballot(act[s]);
+            const int pos = cnt[s] + __popcll(m & ((1ull << lane) - 1ull));
+            if (act[s] && pos < kMaxKeys) keys[s][pos] = key[s];
+            cnt[s] += __popcll(m);
+        }
+    }
+    __syncthreads();
+    // ---- sort each sub-scanline and turn it into span events ----
+    int nev[kMaxSub] = {0, 0, 0, 0};
+    for (int s = 0; s < nsub; s++) {
+        int n = cnt[s];
+        if (n > kMaxKeys) {
+            if (lane == 0) atomicAdd(a.overflow, 1);
+            n = kMaxKeys;
+        }
+        int32_t *ev = reinterpret_cast<int32_t *>(keys[s]);
+        if (n == 0) nev[s] = 0;
+        else if (n <= 64) nev[s] = spans_of<1>(keys[s], n, evenodd, ev, lane);
+        else if (n <= 128) nev[s] = spans_of<2>(keys[s], n, evenodd, ev, lane);
+        else if (n <= 256) nev[s] = spans_of<4>(keys[s], n, evenodd, ev, lane);
+        else nev[s] = spans_of<8>(keys[s], n, evenodd, ev, lane);
+    }
+    __syncthreads();
+    // ---- accumulate per window, prefix-sum, clamp, write u8 ----
+    for (int32_t wx0 = 0; wx0 < job.w; wx0 += kWin) {
+        const int32_t ww = min(kWin, job.w - wx0);
+        for (int i = lane; i <= ww; i += kWave) acc[i] = 0;
         __syncthreads();
-        // absolute supersampled column of the window's left edge
-        const int32_t colbase = (job.x0 + wx0) * scale;
-        const int32_t colmax = ww * scale;
-        for (int s = 0; s < scale; s++) {
-            const int32_t sy = py * scale + s;
-            // ---- gather active pieces of the row's band ----
-            uint32_t n = 0;
-            const int32_t band = (sy - job.band_y0) >> rvkr::kBandShift;
-            if (band >= 0 && (uint32_t)band < job.band_count) {
-                const uint32_t b0 = a.band_start[job.band_offset + band], b1 = a.band_start[job.band_offset + band + 1];
-                for (uint32_t base = b0; base < b1; base += kWave) {
-                    const uint32_t i = base + lane;
-                    bool act = false;
-                    uint64_t key = 0;
-                    if (i < b1) {
-                        const Piece p = a.pieces[job.piece_offset + a.band_items[i]];
-                        if (p.first_y <= sy && sy <= p.last_y) {
-                            int32_t x = (int32_t)((uint32_t)p.x + (uint32_t)(sy - p.first_y) * (uint32_t)p.dx);
-                            key = ((uint64_t)((uint32_t)x ^ 0x80000000u) << 32) | ((uint64_t)(uint32_t)p.id << 1) |
-                                  (p.winding > 0 ? 1u : 0u);
-                            act = true;
-                        }
-                    }
-                    const uint64_t m = __ballot(act);
-                    const uint32_t pos = n + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                    if (act && pos < (uint32_t)kMaxX) keys[pos] = key;
-                    n += (uint32_t)__popcll(m);
-                }
-            }
-            if (n > (uint32_t)kMaxX) {
-                if (lane == 0) atomicAdd(a.overflow, 1);
-                n = kMaxX;
-            }
-            if (n == 0) continue;
-            // ---- bitonic sort of keys[0, npow) ----
-            uint32_t npow = 1;
-            while (npow < n) npow <<= 1;
-            for (uint32_t i = n + lane; i < npow; i += kWave) keys[i] = ~0ull;
-            __syncthreads();
-            for (uint32_t k = 2; k <= npow; k <<= 1) {
-                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                    for (uint32_t i = lane; i < npow; i += kWave) {
-                        const uint32_t ixj = i ^ j;
-                        if (ixj > i) {
-                            const uint64_t x0 = keys[i], x1 = keys[ixj];
-                            const bool up = (i & k) == 0;
-                            if ((x0 > x1) == up) { keys[i] = x1; keys[ixj] = x0; }
-                        }
-                    }
-                    __syncthreads();
-                }
-            }
-            // ---- winding scan -> span boundary events (start, end, start, end, ...) ----
-            int carry = 0;
-            uint32_t nev = 0;
-            for (uint32_t base = 0; base < n; base += kWave) {
-                const uint32_t i = base + lane;
-                int w = 0;
-                uint64_t key = 0;
-                if (i < n) { key = keys[i]; w = (key & 1) ? 1 : -1; }
-                const int incl = wave_incl_scan(w, lane);
-                const int before = carry + incl - w, after = carry + incl;
-                const bool in_b = evenodd ? (before & 1) != 0 : before != 0;
-                const bool in_a = evenodd ? (after & 1) != 0 : after != 0;
-                const bool evt = i < n && (in_b != in_a);
-                const uint64_t m = __ballot(evt);
-                if (evt) {
-                    const int32_t x = (int32_t)((uint32_t)(key >> 32) ^ 0x80000000u);
-                    ev[nev + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (x + 0x8000) >> 16;
-                }
-                nev += (uint32_t)__popcll(m);
-                carry += __shfl(incl, kWave - 1, kWave);
-            }
-            __syncthreads();
-            // ---- SuperBlitter contributions of each span ----
+        const int32_t colbase = (job.x0 + wx0) << shift;
+        const int32_t colmax = ww << shift;
+        for (int s = 0; s < nsub; s++) {
+            const int32_t *ev = reinterpret_cast<const int32_t *>(keys[s]);
             const int32_t maxv = job.aa ? (64 - (((s & 3) + 1) >> 2)) : 255;
-            for (uint32_t k = lane; 2 * k + 1 < nev; k += kWave) {
+            for (int k = lane; 2 * k + 1 < nev[s]; k += kWave) {
                 int32_t L = ev[2 * k] - colbase, R = ev[2 * k + 1] - colbase;
                 if (R <= L) continue;
                 L = max(L, 0);
                 R = min(R, colmax);
                 if (R <= L) continue;
                 if (!job.aa) {
-                    atomicAdd(&row[L], 255 << 16);
-                    atomicAdd(&row[R], -(255 << 16));
+                    atomicAdd(&acc[L], 255 << 16);
+                    atomicAdd(&acc[R], -(255 << 16));
                     continue;
                 }
                 const int32_t fb = L & 3, fe = R & 3;
-                int32_t cnt = (R >> 2) - (L >> 2) - 1;
-                if (cnt < 0) {
-                    atomicAdd(&row[L >> 2], (fe - fb) * 16);
+                int32_t c = (R >> 2) - (L >> 2) - 1;
+                if (c < 0) {
+                    atomicAdd(&acc[L >> 2], (fe - fb) * 16);
                 } else {
                     int32_t st = L >> 2;
-                    if (fb == 0) cnt += 1;
-                    else { atomicAdd(&row[st], (4 - fb) * 16); st += 1; }
-                    if (cnt > 0) {
-                        atomicAdd(&row[st], maxv << 16);
-                        atomicAdd(&row[st + cnt], -(maxv << 16));
+                    if (fb == 0) c += 1;
+                    else { atomicAdd(&acc[st], (4 - fb) * 16); st += 1; }
+                    if (c > 0) {
+                        atomicAdd(&acc[st], maxv << 16);
+                        atomicAdd(&acc[st + c], -(maxv << 16));
                     }
-                    if (fe) atomicAdd(&row[R >> 2], fe * 16);
+                    if (fe) atomicAdd(&acc[R >> 2], fe * 16);
                 }
             }
-            __syncthreads();
         }
-        // ---- prefix-sum the run differences, clamp, write u8 ----
+        __syncthreads();
         int carry = 0;
         for (int32_t base = 0; base < ww; base += 4 * kWave) {
             const int32_t p0 = base + 4 * lane;
-            int v[4], d[4];
-            int loc = 0;
+            int v[4], d[4], loc = 0;
             for (int q = 0; q < 4; q++) {
                 const int32_t p = p0 + q;
-                const int32_t packed = p < ww ? row[p] : 0;
+                const int32_t packed = p < ww ? acc[p] : 0;
                 v[q] = packed & 0xffff;
                 d[q] = packed >> 16;
                 loc += d[q];
@@ -202,11 +375,10 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
             uint32_t word = 0;
             for (int q = 0; q < 4; q++) {
                 run += d[q];
-                const int c = min(255, v[q] + run);
-                word |= (uint32_t)c << (8 * q);
+                word |= (uint32_t)min(255, v[q] + run) << (8 * q);
             }
             const int32_t gx = wx0 + p0;
-            if (gx + 3 < job.w && ((wx0 + p0) & 3) == 0) {
+            if (gx + 3 < job.w) {
                 *(uint32_t *)(out + gx) = word;
             } else {
                 for (int q = 0; q < 4; q++)
@@ -335,11 +507,21 @@ __global__ __launch_bounds__(TX * TY) void k_draw(uint32_t *px, uint32_t w, uint
 
 namespace rvk {
 
+rvk_status launch_expand(rvk_context *ctx, const EdgeRec *edges, uint32_t nedges, const EdgeCov *cov,
+                         PieceDev *pieces, uint32_t *npieces, int32_t *overflow, double algo_bytes) {
+    if (nedges == 0) return RVK_OK;
+    ExpandArgs a{edges, nedges, cov, pieces, npieces, overflow};
+    timed_launch(ctx, "expand", algo_bytes,
+                 [&] { hipLaunchKernelGGL(k_expand, dim3(div_up(nedges, 256)), dim3(256), 0, ctx->stream, a); });
+    RVK_HIP_CHECK(hipGetLastError());
+    return RVK_OK;
+}
+
 rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, uint32_t total_rows,
-                           const Piece *pieces, const uint32_t *band_start, const uint32_t *band_items, uint8_t *arena,
-                           int32_t *overflow, double algo_bytes) {
+                           const EdgeCov *edges, const uint2 *windows, const PieceDev *pieces, const uint32_t *npieces,
+                           uint8_t *arena, int32_t *overflow, double algo_bytes) {
     if (total_rows == 0) return RVK_OK;
-    CovArgs a{jobs, njobs, pieces, band_start, band_items, arena, overflow};
+    CovArgs a{jobs, njobs, edges, windows, pieces, npieces, arena, overflow};
     timed_launch(ctx, "coverage", algo_bytes,
                  [&] { hipLaunchKernelGGL(k_coverage, dim3(total_rows), dim3(kWave), 0, ctx->stream, a); });
     RVK_HIP_CHECK(hipGetLastError());

@@ -107,7 +107,7 @@ inline int32_t fdot6_of(float v, float scale) { return as_i32(v * scale); }
 constexpr int kMaxCoeffShift = 6;
 
 // SkEdge::updateLine on 16.16 inputs; emits the piece when it spans a row
-bool line_piece(int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t winding, Piece &p) {
+bool line_piece(int32_t x0, int32_t y0, int32_t x1, int32_t y1, PieceDev &p) {
     y0 >>= 10;
     y1 >>= 10;
     int32_t top = fdot6_round(y0), bot = fdot6_round(y1);
@@ -119,30 +119,70 @@ bool line_piece(int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t winding,
     p.dx = slope;
     p.first_y = top;
     p.last_y = bot - 1;
-    p.winding = winding;
     return true;
+}
+
+// SkQuadraticEdge::updateQuadratic: the next non-empty piece, or false when done
+bool quad_update(EdgeRec &e) {
+    int32_t count = e.count, oldx = e.cx, oldy = e.cy, dx = e.d1x, dy = e.d1y, newx = oldx, newy = oldy;
+    PieceDev pc;
+    bool ok;
+    do {
+        if (--count > 0) {
+            newx = oldx + (dx >> e.shift);
+            dx += e.d2x;
+            newy = oldy + (dy >> e.shift);
+            dy += e.d2y;
+        } else {
+            newx = e.lx;
+            newy = e.ly;
+        }
+        ok = line_piece(oldx, oldy, newx, newy, pc);
+        oldx = newx;
+        oldy = newy;
+    } while (count > 0 && !ok);
+    e.cx = newx; e.cy = newy; e.d1x = dx; e.d1y = dy; e.count = count;
+    if (ok) { e.x = pc.x; e.dx = pc.dx; e.first_y = pc.first_y; e.last_y = pc.last_y; }
+    return ok;
+}
+
+// SkCubicEdge::updateCubic
+bool cubic_update(EdgeRec &e) {
+    int32_t count = e.count, oldx = e.cx, oldy = e.cy, newx = oldx, newy = oldy;
+    PieceDev pc;
+    bool ok;
+    do {
+        if (++count < 0) {
+            newx = oldx + (e.d1x >> e.dshift);
+            e.d1x += e.d2x >> e.shift;
+            e.d2x += e.d3x;
+            newy = oldy + (e.d1y >> e.dshift);
+            e.d1y += e.d2y >> e.shift;
+            e.d2y += e.d3y;
+        } else {
+            newx = e.lx;
+            newy = e.ly;
+        }
+        if (newy < oldy) newy = oldy;
+        ok = line_piece(oldx, oldy, newx, newy, pc);
+        oldx = newx;
+        oldy = newy;
+    } while (count < 0 && !ok);
+    e.cx = newx; e.cy = newy; e.count = count;
+    if (ok) { e.x = pc.x; e.dx = pc.dx; e.first_y = pc.first_y; e.last_y = pc.last_y; }
+    return ok;
 }
 
 struct Builder {
     int shift;
-    std::vector<Piece> &out;
-    int32_t ymin = INT32_MAX, ymax = INT32_MIN;
-    int32_t edge_id = 0;
-    // kind of every edge pushed so far (line edges own exactly one piece, the last
-    // one in `out` while they are the newest edge), for combine_vertical
-    std::vector<uint8_t> edge_is_line;
-    bool last_is_line = false;
-    size_t last_index = 0;
+    std::vector<EdgeRec> &out;
+    std::vector<uint32_t> &caps;
+    std::vector<int32_t> winding;  // per edge
 
-    void emit(Piece p) {
-        p.id = edge_id;
-        out.push_back(p);
-    }
-    void finish_edge(bool is_line) {
-        edge_is_line.push_back(is_line ? 1 : 0);
-        edge_id++;
-        last_is_line = is_line;
-        last_index = out.size() - 1;
+    void push(const EdgeRec &e, int32_t w, uint32_t cap) {
+        out.push_back(e);
+        winding.push_back(w);
+        caps.push_back(cap);
     }
 
     // LineEdge::new + push_line's combine_vertical (tiny-skia edge_builder.rs)
@@ -155,41 +195,44 @@ struct Builder {
         int32_t top = fdot6_round(y0), bot = fdot6_round(y1);
         if (top == bot) return;
         int32_t slope = fdot6_div(x1 - x0, y1 - y0);
-        Piece e{fdot6_to_fixed(x0 + fixed_mul(slope, compute_dy(top, y0))), slope, top, bot - 1, w, 0};
-        if (e.dx == 0 && last_is_line) {
-            Piece &l = out[last_index];
+        EdgeRec e{};
+        e.x = fdot6_to_fixed(x0 + fixed_mul(slope, compute_dy(top, y0)));
+        e.dx = slope;
+        e.first_y = top;
+        e.last_y = bot - 1;
+        e.type = 0;
+        if (e.dx == 0 && !out.empty() && out.back().type == 0) {
+            EdgeRec &l = out.back();
+            int32_t &lw = winding.back();
             if (l.dx == 0 && e.x == l.x) {
-                if (e.winding == l.winding) {
+                if (w == lw) {
                     if (e.last_y + 1 == l.first_y) { l.first_y = e.first_y; return; }
                     if (e.first_y == l.last_y + 1) { l.last_y = e.last_y; return; }
                 } else if (e.first_y == l.first_y) {
                     if (e.last_y == l.last_y) {  // the two cancel: drop the previous edge
                         out.pop_back();
-                        edge_id--;
-                        edge_is_line.pop_back();
-                        last_is_line = !edge_is_line.empty() && edge_is_line.back();
-                        last_index = out.empty() ? 0 : out.size() - 1;
+                        winding.pop_back();
+                        caps.pop_back();
                         return;
                     }
                     if (e.last_y < l.last_y) { l.first_y = e.last_y + 1; return; }
                     l.first_y = l.last_y + 1;
                     l.last_y = e.last_y;
-                    l.winding = e.winding;
+                    lw = w;
                     return;
                 } else if (e.last_y == l.last_y) {
                     if (e.first_y > l.first_y) { l.last_y = e.first_y - 1; return; }
                     l.last_y = l.first_y - 1;
                     l.first_y = e.first_y;
-                    l.winding = e.winding;
+                    lw = w;
                     return;
                 }
             }
         }
-        emit(e);
-        finish_edge(true);
+        push(e, w, 1);
     }
 
-    // QuadraticEdge::new + update() until exhausted
+    // QuadraticEdge::new (setup + first update)
     void quad(const Pt *p) {
         float scale = (float)(1 << (shift + 6));
         int32_t x0 = fdot6_of(p[0].x, scale), y0 = fdot6_of(p[0].y, scale), x1 = fdot6_of(p[1].x, scale),
@@ -201,34 +244,26 @@ struct Builder {
                                    ((int32_t)((uint32_t)y1 << 1) - y0 - y2) >> 2, shift);
         if (sh == 0) sh = 1;
         else if (sh > kMaxCoeffShift) sh = kMaxCoeffShift;
-        int32_t count = 1 << sh, cshift = sh - 1;
+        EdgeRec e{};
+        e.type = 1;
+        e.count = 1 << sh;
+        e.shift = sh - 1;
         int32_t A = (int32_t)((uint32_t)(x0 - x1 - x1 + x2) << 9), B = fdot6_to_fixed(x1 - x0);
-        int32_t qx = fdot6_to_fixed(x0), qdx = B + (A >> sh), qddx = A >> (sh - 1);
+        e.cx = fdot6_to_fixed(x0);
+        e.d1x = B + (A >> sh);
+        e.d2x = A >> (sh - 1);
         A = (int32_t)((uint32_t)(y0 - y1 - y1 + y2) << 9);
         B = fdot6_to_fixed(y1 - y0);
-        int32_t qy = fdot6_to_fixed(y0), qdy = B + (A >> sh), qddy = A >> (sh - 1);
-        const int32_t lx = fdot6_to_fixed(x2), ly = fdot6_to_fixed(y2);
-        bool any = false;
-        while (count > 0) {
-            int32_t nx, ny;
-            if (--count > 0) {
-                nx = qx + (qdx >> cshift);
-                qdx += qddx;
-                ny = qy + (qdy >> cshift);
-                qdy += qddy;
-            } else {
-                nx = lx;
-                ny = ly;
-            }
-            Piece pc;
-            if (line_piece(qx, qy, nx, ny, w, pc)) { emit(pc); any = true; }
-            qx = nx;
-            qy = ny;
-        }
-        if (any) finish_edge(false);
+        e.cy = fdot6_to_fixed(y0);
+        e.d1y = B + (A >> sh);
+        e.d2y = A >> (sh - 1);
+        e.lx = fdot6_to_fixed(x2);
+        e.ly = fdot6_to_fixed(y2);
+        if (!quad_update(e)) return;
+        push(e, w, 1u + (uint32_t)std::max(e.count, 0));
     }
 
-    // CubicEdge::new + update() until exhausted
+    // CubicEdge::new (setup + first update)
     void cubic(const Pt *p) {
         float scale = (float)(1 << (shift + 6));
         int32_t x0 = fdot6_of(p[0].x, scale), y0 = fdot6_of(p[0].y, scale), x1 = fdot6_of(p[1].x, scale),
@@ -246,35 +281,25 @@ struct Builder {
         int32_t up = 6, down = sh + up - 10;
         if (down < 0) { down = 0; up = 10 - sh; }
         auto upsh = [&](int32_t v) { return (int32_t)((uint32_t)v << up); };
-        int32_t count = (int32_t)((uint32_t)-1 << sh);
+        EdgeRec e{};
+        e.type = 2;
+        e.count = (int32_t)((uint32_t)-1 << sh);
+        e.shift = sh;
+        e.dshift = down;
         int32_t B = upsh(3 * (x1 - x0)), C = upsh(3 * (x0 - x1 - x1 + x2)), D = upsh(x3 + 3 * (x1 - x2) - x0);
-        int32_t cx = fdot6_to_fixed(x0), cdx = B + (C >> sh) + (D >> (2 * sh)), cddx = 2 * C + ((3 * D) >> (sh - 1)),
-                cdddx = (3 * D) >> (sh - 1);
+        e.cx = fdot6_to_fixed(x0);
+        e.d1x = B + (C >> sh) + (D >> (2 * sh));
+        e.d2x = 2 * C + ((3 * D) >> (sh - 1));
+        e.d3x = (3 * D) >> (sh - 1);
         B = upsh(3 * (y1 - y0)); C = upsh(3 * (y0 - y1 - y1 + y2)); D = upsh(y3 + 3 * (y1 - y2) - y0);
-        int32_t cy = fdot6_to_fixed(y0), cdy = B + (C >> sh) + (D >> (2 * sh)), cddy = 2 * C + ((3 * D) >> (sh - 1)),
-                cdddy = (3 * D) >> (sh - 1);
-        const int32_t lx = fdot6_to_fixed(x3), ly = fdot6_to_fixed(y3);
-        bool any = false;
-        while (count < 0) {
-            int32_t nx, ny;
-            if (++count < 0) {
-                nx = cx + (cdx >> down);
-                cdx += cddx >> sh;
-                cddx += cdddx;
-                ny = cy + (cdy >> down);
-                cdy += cddy >> sh;
-                cddy += cdddy;
-            } else {
-                nx = lx;
-                ny = ly;
-            }
-            if (ny < cy) ny = cy;
-            Piece pc;
-            if (line_piece(cx, cy, nx, ny, w, pc)) { emit(pc); any = true; }
-            cx = nx;
-            cy = ny;
-        }
-        if (any) finish_edge(false);
+        e.cy = fdot6_to_fixed(y0);
+        e.d1y = B + (C >> sh) + (D >> (2 * sh));
+        e.d2y = 2 * C + ((3 * D) >> (sh - 1));
+        e.d3y = (3 * D) >> (sh - 1);
+        e.lx = fdot6_to_fixed(x3);
+        e.ly = fdot6_to_fixed(y3);
+        if (!cubic_update(e)) return;
+        push(e, w, 1u + (uint32_t)std::max(-e.count, 0));
     }
 };
 
@@ -367,8 +392,8 @@ int cubic_y_monotonic(const Pt *src, Pt *dst) {
 
 }  // namespace
 
-void build_pieces(const std::vector<Seg> &segs, int shift, std::vector<Piece> &out, int32_t &ymin, int32_t &ymax) {
-    Builder b{shift, out};
+void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &out, std::vector<uint32_t> &caps) {
+    Builder b{shift, out, caps, {}};
     for (const Seg &s : segs) {
         if (s.n == 2) {
             b.line(s.p[0], s.p[1]);
@@ -382,13 +407,7 @@ void build_pieces(const std::vector<Seg> &segs, int shift, std::vector<Piece> &o
             for (int i = 0; i <= n; i++) b.cubic(mono + 3 * i);
         }
     }
-    ymin = INT32_MAX;
-    ymax = INT32_MIN;
-    for (const Piece &p : out) {
-        if (p.first_y > p.last_y) continue;
-        ymin = std::min(ymin, p.first_y);
-        ymax = std::max(ymax, p.last_y);
-    }
+    for (size_t i = 0; i < out.size(); i++) out[i].winding = b.winding[i];
 }
 
 // ---------------------------------------------------------------------------

@@ -14,6 +14,7 @@
 #include "host_geom.h"
 #include "raster.h"
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <memory>
 
@@ -21,9 +22,11 @@ using namespace rvkh;
 using namespace rvkr;
 
 namespace rvk {
+rvk_status launch_expand(rvk_context *ctx, const EdgeRec *edges, uint32_t nedges, const EdgeCov *cov,
+                         PieceDev *pieces, uint32_t *npieces, int32_t *overflow, double algo_bytes);
 rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, uint32_t total_rows,
-                           const Piece *pieces, const uint32_t *band_start, const uint32_t *band_items, uint8_t *arena,
-                           int32_t *overflow, double algo_bytes);
+                           const EdgeCov *edges, const uint2 *windows, const PieceDev *pieces, const uint32_t *npieces,
+                           uint8_t *arena, int32_t *overflow, double algo_bytes);
 rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
                              int32_t x1, int32_t y1, const DrawItem *items, uint32_t nitems, const CovJob *jobs,
                              const uint8_t *arena, const ShaderParams *shaders, double algo_bytes);
@@ -38,9 +41,10 @@ struct DrawList {
     uint32_t w = 0, h = 0;
     std::vector<DrawItem> items;
     std::vector<CovJob> jobs;
-    std::vector<Piece> pieces;
-    std::vector<uint32_t> band_start;
-    std::vector<uint32_t> band_items;
+    std::vector<EdgeRec> edges;       // every job's edges, each job's sorted by first row
+    std::vector<EdgeCov> ecov;        // the coverage kernel's view of the same edges
+    std::vector<uint2> windows;       // per job and row group: [lo, hi) of the job's edges
+    uint64_t piece_slots = 0;         // piece capacity of all edges
     std::vector<ShaderParams> shaders;
     uint64_t arena = 0;
     uint32_t rows = 0;
@@ -81,11 +85,22 @@ struct Renderer {
     // of the job's pixmap).  Returns the job index or -1 when nothing is covered.
     int32_t add_job(DrawList &dl, const std::vector<Seg> &segs, uint32_t rule, bool aa, IRect clip) {
         if (segs.empty()) return -1;
-        std::vector<Piece> pcs;
-        int32_t ymin, ymax;
         const int shift = aa ? 2 : 0;
-        build_pieces(segs, shift, pcs, ymin, ymax);
-        if (pcs.empty() || ymin > ymax) return -1;
+        std::vector<EdgeRec> er;
+        std::vector<uint32_t> caps;
+        build_edges(segs, shift, er, caps);
+        if (er.empty()) return -1;
+        // last covered row of each edge: exact for lines; for curves the row of the
+        // end point plus one (the stepped points stay within a fraction of a row of
+        // the monotonic curve; the expansion kernel checks the bound)
+        std::vector<int32_t> ylast(er.size());
+        int32_t ymin = INT32_MAX, ymax = INT32_MIN;
+        for (size_t i = 0; i < er.size(); i++) {
+            const EdgeRec &e = er[i];
+            ylast[i] = e.type == 0 ? e.last_y : (((e.ly >> 10) + 32) >> 6) + 1;
+            ymin = std::min(ymin, e.first_y);
+            ymax = std::max(ymax, ylast[i]);
+        }
         float mnx = INFINITY, mxx = -INFINITY;
         for (const Seg &s : segs)
             for (int i = 0; i < s.n; i++) { mnx = std::min(mnx, s.p[i].x); mxx = std::max(mxx, s.p[i].x); }
@@ -99,36 +114,37 @@ struct Renderer {
         j.x0 = (int32_t)x0; j.y0 = (int32_t)y0; j.w = (int32_t)(x1 - x0); j.h = (int32_t)(y1 - y0);
         j.rule = rule;
         j.aa = aa ? 1 : 0;
-        j.piece_offset = (uint32_t)dl.pieces.size();
-        j.piece_count = (uint32_t)pcs.size();
-        j.band_y0 = j.y0 << shift;
-        const int32_t rows_ss = j.h << shift;
-        j.band_count = (uint32_t)((rows_ss + (1 << kBandShift) - 1) >> kBandShift);
-        j.band_offset = (uint32_t)dl.band_start.size();
-        // counting sort of piece indices into bands
-        std::vector<uint32_t> cnt(j.band_count + 1, 0);
-        auto band_range = [&](const Piece &p, int32_t &b0, int32_t &b1) {
-            int32_t a = std::max(p.first_y, j.band_y0), b = std::min(p.last_y, j.band_y0 + rows_ss - 1);
-            if (b < a) { b0 = 1; b1 = 0; return; }
-            b0 = (a - j.band_y0) >> kBandShift;
-            b1 = (b - j.band_y0) >> kBandShift;
-        };
-        for (const Piece &p : pcs) {
-            int32_t b0, b1;
-            band_range(p, b0, b1);
-            for (int32_t b = b0; b <= b1; b++) cnt[b + 1]++;
+        // edges sorted by first row (ids keep tiny-skia's edge-list order)
+        std::vector<uint32_t> order(er.size());
+        for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return er[a].first_y < er[b].first_y; });
+        j.edge_offset = (uint32_t)dl.edges.size();
+        j.edge_count = (uint32_t)er.size();
+        std::vector<int32_t> prefmax(er.size());
+        int32_t pm = INT32_MIN;
+        for (size_t k = 0; k < order.size(); k++) {
+            const uint32_t i = order[k];
+            EdgeRec e = er[i];
+            e.piece_offset = (uint32_t)dl.piece_slots;
+            dl.piece_slots += caps[i];
+            dl.edges.push_back(e);
+            dl.ecov.push_back(EdgeCov{e.first_y, ylast[i], e.winding, (int32_t)i, e.piece_offset, 0});
+            pm = std::max(pm, ylast[i]);
+            prefmax[k] = pm;
         }
-        for (uint32_t b = 0; b < j.band_count; b++) cnt[b + 1] += cnt[b];
-        const uint32_t items_base = (uint32_t)dl.band_items.size();
-        dl.band_items.resize(items_base + cnt[j.band_count]);
-        std::vector<uint32_t> fillpos(cnt.begin(), cnt.end() - 1);
-        for (uint32_t i = 0; i < pcs.size(); i++) {
-            int32_t b0, b1;
-            band_range(pcs[i], b0, b1);
-            for (int32_t b = b0; b <= b1; b++) dl.band_items[items_base + fillpos[b]++] = i;
+        // per group of kGroupRows pixel rows: the edges that start at or before the
+        // group's last row, minus the leading ones that all end before its first row
+        j.window_offset = (uint32_t)dl.windows.size();
+        const int32_t groups = (j.h + kGroupRows - 1) / kGroupRows;
+        uint32_t lo = 0, hi = 0;
+        for (int32_t g = 0; g < groups; g++) {
+            const int32_t r0 = j.y0 + g * kGroupRows, r1 = std::min(j.y0 + j.h, r0 + kGroupRows);
+            const int32_t s0 = r0 << shift, s1 = (r1 << shift) - 1;
+            while (lo < j.edge_count && prefmax[lo] < s0) lo++;
+            while (hi < j.edge_count && dl.ecov[j.edge_offset + hi].first_y <= s1) hi++;
+            dl.windows.push_back(uint2{lo, std::max(lo, hi)});
         }
-        for (uint32_t b = 0; b <= j.band_count; b++) dl.band_start.push_back(items_base + cnt[b]);
-        dl.pieces.insert(dl.pieces.end(), pcs.begin(), pcs.end());
         const uint64_t stride = ((uint64_t)j.w + 3) & ~3ull;
         j.cov_offset = dl.arena;
         dl.arena += (stride * (uint64_t)j.h + 255) & ~255ull;
@@ -196,50 +212,70 @@ struct Renderer {
             dl = DrawList{dl.px, dl.w, dl.h};
             return RVK_OK;
         }
-        // one device block: jobs | pieces | band_start | band_items | shaders | items | overflow | arena
+        // one device block: jobs | edges | ecov | windows | shaders | items | overflow ||
+        // npieces | pieces | arena (the part after || is written by the device)
         auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
         const size_t o_jobs = 0;
-        const size_t o_pieces = al(o_jobs + dl.jobs.size() * sizeof(CovJob));
-        const size_t o_bs = al(o_pieces + dl.pieces.size() * sizeof(Piece));
-        const size_t o_bi = al(o_bs + dl.band_start.size() * 4);
-        const size_t o_sh = al(o_bi + dl.band_items.size() * 4);
+        const size_t o_edges = al(o_jobs + dl.jobs.size() * sizeof(CovJob));
+        const size_t o_ecov = al(o_edges + dl.edges.size() * sizeof(EdgeRec));
+        const size_t o_win = al(o_ecov + dl.ecov.size() * sizeof(EdgeCov));
+        const size_t o_sh = al(o_win + dl.windows.size() * sizeof(uint2));
         const size_t o_it = al(o_sh + dl.shaders.size() * sizeof(ShaderParams));
         const size_t o_ov = al(o_it + dl.items.size() * sizeof(DrawItem));
         const size_t meta = al(o_ov + 4);
+        const size_t o_np = meta;
+        const size_t o_pc = al(o_np + dl.edges.size() * 4);
+        const size_t o_arena = al(o_pc + (size_t)dl.piece_slots * sizeof(PieceDev));
         std::vector<uint8_t> host(meta, 0);
         auto put = [&](size_t off, const void *src, size_t n) { if (n) std::memcpy(host.data() + off, src, n); };
         put(o_jobs, dl.jobs.data(), dl.jobs.size() * sizeof(CovJob));
-        put(o_pieces, dl.pieces.data(), dl.pieces.size() * sizeof(Piece));
-        put(o_bs, dl.band_start.data(), dl.band_start.size() * 4);
-        put(o_bi, dl.band_items.data(), dl.band_items.size() * 4);
+        put(o_edges, dl.edges.data(), dl.edges.size() * sizeof(EdgeRec));
+        put(o_ecov, dl.ecov.data(), dl.ecov.size() * sizeof(EdgeCov));
+        put(o_win, dl.windows.data(), dl.windows.size() * sizeof(uint2));
         put(o_sh, dl.shaders.data(), dl.shaders.size() * sizeof(ShaderParams));
         put(o_it, dl.items.data(), dl.items.size() * sizeof(DrawItem));
         void *dev = nullptr;
-        RVK_TRY(rvk::scratch_alloc(ctx, meta + dl.arena, &dev));
+        RVK_TRY(rvk::scratch_alloc(ctx, o_arena + dl.arena, &dev));
         uint8_t *d = (uint8_t *)dev;
-        // algorithmic bytes: coverage writes one byte per job pixel and reads each piece
-        // and band entry once; the compositor reads and writes each target pixel of its
-        // rect once and reads each coverage byte once
+        // algorithmic bytes: expansion reads each edge once and writes its pieces;
+        // coverage writes one byte per job pixel and reads edges, windows and pieces
+        // once; the compositor reads and writes each target pixel of its rect once
+        // and reads each coverage byte once
         double cov_px = 0.0;
         for (const CovJob &j : dl.jobs) cov_px += (double)j.w * (double)j.h;
-        const double cov_bytes = cov_px + (double)(dl.pieces.size() * sizeof(Piece) + dl.band_items.size() * 4);
+        const double piece_bytes = (double)dl.piece_slots * sizeof(PieceDev);
+        const double exp_bytes = (double)dl.edges.size() * (sizeof(EdgeRec) + 4) + piece_bytes;
+        const double cov_bytes = cov_px + (double)dl.ecov.size() * sizeof(EdgeCov) +
+                                 (double)dl.windows.size() * sizeof(uint2) + piece_bytes;
         const double draw_bytes = 8.0 * (double)(dl.bx1 - dl.bx0) * (double)(dl.by1 - dl.by0) + cov_px;
+        ctx->stats.edge_pieces += dl.piece_slots;
+        ctx->stats.band_entries += dl.windows.size();
+        ctx->stats.upload_bytes += meta;
         rvk_status st = rvk::upload_sync(ctx, d, host.data(), meta);
         if (st == RVK_OK)
+            st = rvk::launch_expand(ctx, (const EdgeRec *)(d + o_edges), (uint32_t)dl.edges.size(),
+                                    (const EdgeCov *)(d + o_ecov), (PieceDev *)(d + o_pc), (uint32_t *)(d + o_np),
+                                    (int32_t *)(d + o_ov), exp_bytes);
+        if (st == RVK_OK)
             st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)dl.jobs.size(), dl.rows,
-                                      (const Piece *)(d + o_pieces), (const uint32_t *)(d + o_bs),
-                                      (const uint32_t *)(d + o_bi), d + meta, (int32_t *)(d + o_ov), cov_bytes);
+                                      (const EdgeCov *)(d + o_ecov), (const uint2 *)(d + o_win),
+                                      (const PieceDev *)(d + o_pc), (const uint32_t *)(d + o_np), d + o_arena,
+                                      (int32_t *)(d + o_ov), cov_bytes);
         if (st == RVK_OK)
             st = rvk::launch_draw_items(ctx, dl.px, dl.w, dl.h, dl.bx0, dl.by0, dl.bx1, dl.by1,
                                         (const DrawItem *)(d + o_it), (uint32_t)dl.items.size(),
-                                        (const CovJob *)(d + o_jobs), d + meta, (const ShaderParams *)(d + o_sh),
+                                        (const CovJob *)(d + o_jobs), d + o_arena, (const ShaderParams *)(d + o_sh),
                                         draw_bytes);
         int32_t ov = 0;
         if (st == RVK_OK) {
+            const auto t0 = std::chrono::steady_clock::now();
             hipError_t e = hipMemcpyAsync(&ov, d + o_ov, 4, hipMemcpyDeviceToHost, ctx->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
             if (e != hipSuccess) st = RVK_ERR_HIP;
+            ctx->stats.host_wait_ms +=
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
+        ctx->stats.flushes++;
         rvk::scratch_release(ctx, dev);
         overflow_rows += (uint64_t)(ov > 0 ? ov : 0);
         dl = DrawList{dl.px, dl.w, dl.h};
@@ -376,7 +412,12 @@ rvk_status rvk_render(rvk_context *ctx, const rvk_scene *scene, rvk_transform tr
                       uint32_t height, uint8_t *dev_pixmap) {
     if (!ctx || !scene_ok(scene) || !dev_pixmap || !rvk::valid_dims(width, height)) return RVK_ERR_INVALID_ARGUMENT;
     ctx->stats = rvk_stats{};
-    return rvk::render_scene(ctx, scene, ts_from(transform), width, height, dev_pixmap, nullptr);
+    const auto t0 = std::chrono::steady_clock::now();
+    const rvk_status s = rvk::render_scene(ctx, scene, ts_from(transform), width, height, dev_pixmap, nullptr);
+    ctx->stats.host_build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() -
+        ctx->stats.host_wait_ms;
+    return s;
 }
 
 rvk_status rvk_render_host(rvk_context *ctx, const rvk_scene *scene, rvk_transform transform, uint32_t width,
