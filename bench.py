@@ -189,6 +189,10 @@ def main() -> int:
     torch.cuda.synchronize()
     t_wall = time.perf_counter() - t_wall
     kernels = ctx.profile_end()
+    st = ctx.stats()  # of the last render call
+    host = {"build_ms": round(st.host_build_ms, 1), "wait_ms": round(st.host_wait_ms, 1), "flushes": st.flushes,
+            "edge_piece_slots": st.edge_pieces, "upload_MB": round(st.upload_bytes / 1e6, 1),
+            "kernel_launches": st.kernel_launches, "layers": st.layers}
     ms = start.elapsed_time(stop) / args.steps
     mpix = w * h / 1e6
     value = mpix / (ms / 1e3)
@@ -236,6 +240,7 @@ def main() -> int:
         "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()} for d in kernels[:8]],
         "primitives": _primitive_rooflines(kernels, args.steps),
         "wall_s_per_step": round(t_wall / args.steps, 3),
+        "host_last_step": host,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -14,9 +14,12 @@
 #include "host_geom.h"
 #include "raster.h"
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
+#include <thread>
 
 using namespace rvkh;
 using namespace rvkr;
@@ -74,6 +77,74 @@ IRect max_filter_bbox(uint32_t w, uint32_t h) {  // lib.rs:86-97
     return IRect{INT32_MIN / 2, INT32_MIN / 2, INT32_MAX / 2 - INT32_MIN / 2, INT32_MAX / 2 - INT32_MIN / 2};
 }
 
+// Edges of one fill or stroke, computed off the walk (the expensive host part).
+struct EdgeSet {
+    bool ok = false;
+    std::vector<EdgeRec> edges;
+    std::vector<uint32_t> caps;
+    float mnx = 0.0f, mxx = 0.0f;  // control-point x range of the device-space segments
+};
+
+struct PathGeom {
+    EdgeSet fill, stroke;
+};
+
+// Device-space geometry of one path: tiny-skia's fill_path preconditions and edges,
+// and the stroke outline's edges.
+void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, PathGeom &g) {
+    auto finish = [&](std::vector<Seg> &segs, EdgeSet &es) {
+        if (segs.empty() || bounds_nearly_zero(segs)) return;
+        float mnx = INFINITY, mxx = -INFINITY;
+        for (const Seg &s : segs)
+            for (int i = 0; i < s.n; i++) { mnx = std::min(mnx, s.p[i].x); mxx = std::max(mxx, s.p[i].x); }
+        if (!std::isfinite(mnx) || !std::isfinite(mxx)) return;
+        build_edges(segs, p.anti_alias ? 2 : 0, es.edges, es.caps);
+        es.mnx = mnx;
+        es.mxx = mxx;
+        es.ok = !es.edges.empty();
+    };
+    if (!p.visible) return;
+    if (p.fill.kind != RVK_PAINT_NONE && p.bounds.w != 0.0f && p.bounds.h != 0.0f) {
+        std::vector<Seg> segs;
+        fill_segments(sc, p, ts, segs);
+        finish(segs, g.fill);
+    }
+    if (p.stroke.kind != RVK_PAINT_NONE && p.stroke_width >= 0.0f) {
+        std::vector<Seg> segs;
+        stroke_segments(sc, p, ts, segs);
+        finish(segs, g.stroke);
+    }
+}
+
+unsigned host_threads() {
+    static const unsigned n = [] {
+        if (const char *e = std::getenv("RVK_HOST_THREADS")) {
+            const int v = std::atoi(e);
+            if (v > 0) return (unsigned)std::min(v, 64);
+        }
+        const unsigned hc = std::thread::hardware_concurrency();
+        return std::max(1u, std::min(hc ? hc : 1u, 16u));
+    }();
+    return n;
+}
+
+template <class F>
+void parallel_for(size_t n, F &&fn) {
+    const unsigned t = (unsigned)std::min<size_t>(host_threads(), n);
+    if (t <= 1) {
+        for (size_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
+    };
+    std::vector<std::thread> pool;
+    for (unsigned k = 1; k < t; k++) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+}
+
 struct Renderer {
     rvk_context *ctx;
     const rvk_scene &sc;
@@ -81,15 +152,37 @@ struct Renderer {
     rvk_status err = RVK_OK;
     uint64_t overflow_rows = 0;
 
+    // The walk runs twice: kCollect records every path drawn (with its device
+    // transform), the geometry is computed in parallel, and kExecute replays the
+    // walk consuming it in the same order (kSkip steps over a skipped subtree).
+    enum Mode { kCollect, kExecute, kSkip } mode = kExecute;
+    struct Task {
+        const rvk_path *path;
+        Ts ts;
+    };
+    std::vector<Task> tasks;
+    std::vector<PathGeom> geoms;
+    size_t next_task = 0, ready_end = 0;
+    static constexpr size_t kChunk = 256;
+
+    PathGeom take_geom() {
+        const size_t k = next_task++;
+        if (k >= tasks.size()) return PathGeom{};
+        if (k >= ready_end) {
+            const size_t end = std::min(tasks.size(), k + kChunk);
+            parallel_for(end - k, [&](size_t i) { compute_geom(sc, *tasks[k + i].path, tasks[k + i].ts, geoms[k + i]); });
+            ready_end = end;
+        }
+        return std::move(geoms[k]);
+    }
+
     // Adds a coverage job for device-space segments, restricted to `clip` (pixels
     // of the job's pixmap).  Returns the job index or -1 when nothing is covered.
-    int32_t add_job(DrawList &dl, const std::vector<Seg> &segs, uint32_t rule, bool aa, IRect clip) {
-        if (segs.empty()) return -1;
+    int32_t add_job(DrawList &dl, const EdgeSet &es, uint32_t rule, bool aa, IRect clip) {
+        if (!es.ok) return -1;
         const int shift = aa ? 2 : 0;
-        std::vector<EdgeRec> er;
-        std::vector<uint32_t> caps;
-        build_edges(segs, shift, er, caps);
-        if (er.empty()) return -1;
+        const std::vector<EdgeRec> &er = es.edges;
+        const std::vector<uint32_t> &caps = es.caps;
         // last covered row of each edge: exact for lines; for curves the row of the
         // end point plus one (the stepped points stay within a fraction of a row of
         // the monotonic curve; the expansion kernel checks the bound)
@@ -101,12 +194,8 @@ struct Renderer {
             ymin = std::min(ymin, e.first_y);
             ymax = std::max(ymax, ylast[i]);
         }
-        float mnx = INFINITY, mxx = -INFINITY;
-        for (const Seg &s : segs)
-            for (int i = 0; i < s.n; i++) { mnx = std::min(mnx, s.p[i].x); mxx = std::max(mxx, s.p[i].x); }
-        if (!std::isfinite(mnx) || !std::isfinite(mxx)) return -1;
-        int64_t x0 = std::max<int64_t>((int64_t)std::floor(mnx) - 1, clip.x);
-        int64_t x1 = std::min<int64_t>((int64_t)std::ceil(mxx) + 2, (int64_t)clip.x + clip.w);
+        int64_t x0 = std::max<int64_t>((int64_t)std::floor(es.mnx) - 1, clip.x);
+        int64_t x1 = std::min<int64_t>((int64_t)std::ceil(es.mxx) + 2, (int64_t)clip.x + clip.w);
         int64_t y0 = std::max<int64_t>(ymin >> shift, clip.y);
         int64_t y1 = std::min<int64_t>((ymax >> shift) + 1, (int64_t)clip.y + clip.h);
         if (x1 <= x0 || y1 <= y0) return -1;
@@ -160,27 +249,28 @@ struct Renderer {
         it.fill_job = it.stroke_job = -1;
         it.fill_shader = it.stroke_shader = -1;
         it.stroke_first = p.paint_order_stroke_first ? 1 : 0;
+        if (mode == kCollect) {
+            tasks.push_back(Task{&p, ts});
+            return;
+        }
+        if (mode == kSkip) {
+            next_task++;
+            return;
+        }
+        const PathGeom g = take_geom();
         if (!p.visible) return;
         ShaderParams sp;
         // path::fill_path (path.rs:26-75)
-        if (p.fill.kind != RVK_PAINT_NONE && p.bounds.w != 0.0f && p.bounds.h != 0.0f &&
-            make_shader(sc, p.fill, ts, sp)) {
-            std::vector<Seg> segs;
-            fill_segments(sc, p, ts, segs);
-            if (!segs.empty() && !bounds_nearly_zero(segs)) {
-                it.fill_job = add_job(dl, segs, p.fill_rule, p.anti_alias != 0, clip);
-                if (it.fill_job >= 0) {
-                    it.fill_shader = (int32_t)dl.shaders.size();
-                    dl.shaders.push_back(sp);
-                }
+        if (g.fill.ok && make_shader(sc, p.fill, ts, sp)) {
+            it.fill_job = add_job(dl, g.fill, p.fill_rule, p.anti_alias != 0, clip);
+            if (it.fill_job >= 0) {
+                it.fill_shader = (int32_t)dl.shaders.size();
+                dl.shaders.push_back(sp);
             }
         }
         // path::stroke_path (path.rs:77-116): tiny-skia's stroker outline, filled nonzero
-        if (p.stroke.kind != RVK_PAINT_NONE && p.stroke_width >= 0.0f && make_shader(sc, p.stroke, ts, sp)) {
-            std::vector<Seg> segs;
-            stroke_segments(sc, p, ts, segs);
-            if (segs.empty() || bounds_nearly_zero(segs)) return;
-            it.stroke_job = add_job(dl, segs, RVK_FILL_NONZERO, p.anti_alias != 0, clip);
+        if (g.stroke.ok && make_shader(sc, p.stroke, ts, sp)) {
+            it.stroke_job = add_job(dl, g.stroke, RVK_FILL_NONZERO, p.anti_alias != 0, clip);
             if (it.stroke_job >= 0) {
                 it.stroke_shader = (int32_t)dl.shaders.size();
                 dl.shaders.push_back(sp);
@@ -296,6 +386,14 @@ struct Renderer {
         push_item(dl, it);
     }
 
+    // steps over the paths of a subtree that is not drawn
+    void skip_nodes(const rvk_group &g, Ts ts, DrawList &dl) {
+        const Mode saved = mode;
+        mode = kSkip;
+        render_nodes(g, ts, dl);
+        mode = saved;
+    }
+
     void render_nodes(const rvk_group &g, Ts ts, DrawList &dl) {
         for (uint32_t i = 0; i < g.child_count && err == RVK_OK; i++) {
             const rvk_node_ref &n = sc.children[g.child_offset + i];
@@ -337,7 +435,7 @@ struct Renderer {
         dx -= bb.l - (float)ib.x;
         dy -= bb.t - (float)ib.y;
         const Ts lts = ts_concat(Ts{1, 0, 0, 1, -dx, -dy}, ts);
-        ctx->stats.layers++;
+        if (mode == kExecute) ctx->stats.layers++;
         // elided layer: one path child, no filters
         if (g.filter_count == 0 && g.child_count == 1 && sc.children[g.child_offset].kind == RVK_NODE_PATH) {
             if (dl.arena > kArenaBudget) fail(flush(dl));
@@ -357,18 +455,27 @@ struct Renderer {
             return;
         }
         // materialised layer
-        void *buf = nullptr;
-        rvk_status s = rvk::scratch_alloc(ctx, (size_t)ib.w * ib.h * 4, &buf);
-        if (s != RVK_OK) return;  // Pixmap::new failure skips the group (render.rs:108-109)
-        uint8_t *layer = (uint8_t *)buf;
-        s = rvk::launch_fill_sub(ctx, layer, ib.w, ib.h, 0, 0, ib.w, ib.h, 0u);
         DrawList child;
-        child.px = layer;
         child.w = (uint32_t)ib.w;
         child.h = (uint32_t)ib.h;
+        if (mode != kExecute) {
+            render_nodes(g, lts, child);
+            return;
+        }
+        void *buf = nullptr;
+        rvk_status s = rvk::scratch_alloc(ctx, (size_t)ib.w * ib.h * 4, &buf);
+        if (s != RVK_OK) {  // Pixmap::new failure skips the group (render.rs:108-109)
+            skip_nodes(g, lts, child);
+            return;
+        }
+        uint8_t *layer = (uint8_t *)buf;
+        child.px = layer;
+        s = rvk::launch_fill_sub(ctx, layer, ib.w, ib.h, 0, 0, ib.w, ib.h, 0u);
         if (s == RVK_OK) {
             render_nodes(g, lts, child);
             s = flush(child);
+        } else {
+            skip_nodes(g, lts, child);
         }
         for (uint32_t i = 0; s == RVK_OK && err == RVK_OK && i < g.filter_count; i++)
             s = rvk::filter_apply(ctx, &sc.filters[g.filter_offset + i], &sc.filter_data, lts, layer, ib.w, ib.h,
@@ -390,6 +497,13 @@ rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_
                         uint64_t *overflow_rows) {
     Renderer r{ctx, *scene, max_filter_bbox(w, h)};
     DrawList root;
+    root.w = w;
+    root.h = h;
+    r.mode = Renderer::kCollect;
+    r.render_nodes(scene->groups[0], ts, root);
+    r.geoms.resize(r.tasks.size());
+    r.mode = Renderer::kExecute;
+    root = DrawList{};
     root.px = px;
     root.w = w;
     root.h = h;
