@@ -1,6 +1,7 @@
 // Context, stream and device-memory management of librvk.
 #include "rvk_internal.h"
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 
 namespace rvk {
@@ -102,6 +103,52 @@ rvk_status upload_sync(rvk_context *ctx, void *dev_dst, const void *host_src, si
     return RVK_OK;
 }
 
+rvk_status staging_acquire(rvk_context *ctx, size_t bytes, void **host, uint32_t *slot) {
+    const uint32_t k = ctx->staging_next;
+    ctx->staging_next = (k + 1) % 3;
+    rvk_staging &sb = ctx->staging[k];
+    if (!sb.done) RVK_HIP_CHECK(hipEventCreateWithFlags(&sb.done, hipEventDisableTiming));
+    if (sb.pending) {
+        const auto t0 = std::chrono::steady_clock::now();
+        RVK_HIP_CHECK(hipEventSynchronize(sb.done));
+        ctx->stats.host_wait_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        sb.pending = false;
+    }
+    if (sb.cap < bytes) {
+        if (sb.host) RVK_HIP_CHECK(hipHostFree(sb.host));
+        sb.host = nullptr;
+        sb.cap = 0;
+        size_t cap = 1u << 20;
+        while (cap < bytes) cap <<= 1;
+        if (hipHostMalloc(&sb.host, cap, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            sb.host = nullptr;
+            return RVK_ERR_OUT_OF_MEMORY;
+        }
+        sb.cap = cap;
+    }
+    *host = sb.host;
+    *slot = k;
+    return RVK_OK;
+}
+
+rvk_status upload_staged(rvk_context *ctx, uint32_t slot, void *dev_dst, size_t bytes) {
+    rvk_staging &sb = ctx->staging[slot];
+    RVK_HIP_CHECK(hipMemcpyAsync(dev_dst, sb.host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    RVK_HIP_CHECK(hipEventRecord(sb.done, ctx->stream));
+    sb.pending = true;
+    return RVK_OK;
+}
+
+rvk_status upload_async(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes) {
+    void *h = nullptr;
+    uint32_t slot = 0;
+    RVK_TRY(staging_acquire(ctx, bytes, &h, &slot));
+    std::memcpy(h, host_src, bytes);
+    return upload_staged(ctx, slot, dev_dst, bytes);
+}
+
 }  // namespace rvk
 
 extern "C" {
@@ -146,6 +193,10 @@ rvk_status rvk_context_destroy(rvk_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &r : ctx->prof) { (void)hipEventDestroy(r.start); (void)hipEventDestroy(r.stop); }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (auto &sb : ctx->staging) {
+        if (sb.host) (void)hipHostFree(sb.host);
+        if (sb.done) (void)hipEventDestroy(sb.done);
+    }
     for (auto &f : ctx->free_list) (void)hipFree(f.second);
     for (auto &f : ctx->live) (void)hipFree(f.second);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);

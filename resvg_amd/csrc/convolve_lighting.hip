@@ -228,7 +228,7 @@ rvk_status launch_convolve(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
     if (nk <= (size_t)kMaxKernel) std::memcpy(a.k, flipped.data(), nk * sizeof(float));
     else {
         RVK_TRY(scratch_alloc(ctx, nk * sizeof(float), &bigk));
-        rvk_status ust = upload_sync(ctx, bigk, flipped.data(), nk * sizeof(float));
+        rvk_status ust = upload_async(ctx, bigk, flipped.data(), nk * sizeof(float));
         if (ust != RVK_OK) { scratch_release(ctx, bigk); return ust; }
     }
     void *tmp = nullptr;

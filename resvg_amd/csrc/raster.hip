@@ -172,86 +172,113 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Ascending bitonic sort of the 64*K keys held K per lane (element lane*K + j).
-template <int K>
-__device__ __forceinline__ void wave_sort(uint64_t (&v)[K], int lane) {
-    constexpr int N = kWave * K;
-    for (int k = 2; k <= N; k <<= 1) {
-        for (int jj = k >> 1; jj > 0; jj >>= 1) {
-            if (jj >= K) {
-                const int ld = jj / K;
+// Lane exchange across xor distance LD (1..32) for the bitonic network: an
+// immediate-pattern ds_swizzle within 32-lane halves, ds_bpermute across them.
+template <int LD>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+    if constexpr (LD < 32) {
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (LD << 10) | 0x1F);
+    } else {
+        return (uint32_t)__shfl_xor((int)v, LD, kWave);
+    }
+}
+
+// One stage (k, jj) of an ascending bitonic network over S independent arrays of
+// 64*K keys, element e = lane*K + j of array s in v[s][j].
+template <int K, int S, int k, int jj>
+__device__ __forceinline__ void bitonic_stage(uint64_t (&v)[S][K], int lane) {
+    if constexpr (jj >= K) {
+        constexpr int ld = jj / K;
+        // e & k and e & jj depend on the lane only; keep the smaller key when the pair
+        // is ascending and this is its lower element, or descending and upper
+        const bool up = k >= kWave * K ? true : (lane & (k / K)) == 0;
+        const bool keep_min = up == ((lane & ld) == 0);
 #pragma unroll
-                for (int j = 0; j < K; j++) {
-                    const int e = lane * K + j;
-                    const uint64_t o = shfl_xor64(v[j], ld);
-                    const bool up = (e & k) == 0, lower = (e & jj) == 0;
-                    const uint64_t mn = v[j] < o ? v[j] : o, mx = v[j] < o ? o : v[j];
-                    v[j] = (lower == up) ? mn : mx;
-                }
-            } else {
+        for (int s = 0; s < S; s++)
 #pragma unroll
-                for (int j = 0; j < K; j++) {
-                    const int p = j ^ jj;
-                    if (p > j) {
-                        const int e = lane * K + j;
-                        const bool up = (e & k) == 0;
-                        const uint64_t x = v[j], y = v[p];
-                        const bool sw = (x > y) == up;
-                        v[j] = sw ? y : x;
-                        v[p] = sw ? x : y;
-                    }
+            for (int j = 0; j < K; j++) {
+                const uint64_t x = v[s][j];
+                const uint64_t o = ((uint64_t)xor_lane<ld>((uint32_t)(x >> 32)) << 32) | xor_lane<ld>((uint32_t)x);
+                v[s][j] = ((o < x) == keep_min) ? o : x;
+            }
+    } else {
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            if ((j & jj) == 0) {
+                const int p = j | jj;
+                const bool up = k >= K ? (k >= kWave * K ? true : (lane & (k / K)) == 0) : (j & k) == 0;
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    const uint64_t x = v[s][j], y = v[s][p];
+                    const bool sw = (x > y) == up;
+                    v[s][j] = sw ? y : x;
+                    v[s][p] = sw ? x : y;
                 }
             }
         }
     }
 }
 
-// Sorts the n keys of one sub-scanline and writes its span events (pixel-rounded
-// supersampled columns; starts and ends alternate) to ev.  Returns the count.
-template <int K>
-__device__ __forceinline__ int spans_of(const uint64_t *keys, int n, bool evenodd, int32_t *ev, int lane) {
-    uint64_t v[K];
+template <int K, int S, int k, int jj>
+__device__ __forceinline__ void bitonic_from(uint64_t (&v)[S][K], int lane) {
+    bitonic_stage<K, S, k, jj>(v, lane);
+    if constexpr (jj > 1) bitonic_from<K, S, k, jj / 2>(v, lane);
+    else if constexpr (k < kWave * K) bitonic_from<K, S, k * 2, k>(v, lane);
+}
+
+// Sorts the S sub-scanlines' keys (n[s] each, in keys[s]) together and writes each
+// one's span events (pixel-rounded supersampled columns; starts and ends alternate)
+// over its own key area.  Returns the event counts in nev.
+template <int K, int S>
+__device__ __forceinline__ void spans_of(uint64_t (*keys)[kMaxKeys], const int *n, bool evenodd, int *nev, int lane) {
+    uint64_t v[S][K];
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int e = lane * K + j;
-        v[j] = e < n ? keys[e] : ~0ull;
-    }
-    __syncthreads();  // every key is in registers before the key area is reused for events
-    wave_sort<K>(v, lane);
-    int w[K], local = 0;
+    for (int s = 0; s < S; s++)
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int e = lane * K + j;
-        w[j] = e < n ? ((v[j] & 1) ? 1 : -1) : 0;
-        local += w[j];
-    }
-    const int incl = wave_incl_scan(local, lane);
-    int before = incl - local;
-    bool evt[K];
-    int nevt = 0;
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int after = before + w[j];
-        const bool in_b = evenodd ? (before & 1) != 0 : before != 0;
-        const bool in_a = evenodd ? (after & 1) != 0 : after != 0;
-        evt[j] = w[j] != 0 && in_b != in_a;
-        nevt += evt[j] ? 1 : 0;
-        before = after;
-    }
-    const int eincl = wave_incl_scan(nevt, lane);
-    int pos = eincl - nevt;
-#pragma unroll
-    for (int j = 0; j < K; j++)
-        if (evt[j]) {
-            const int32_t x = (int32_t)((uint32_t)(v[j] >> 32) ^ 0x80000000u);
-            ev[pos++] = (x + 0x8000) >> 16;
+        for (int j = 0; j < K; j++) {
+            const int e = lane * K + j;
+            v[s][j] = e < n[s] ? keys[s][e] : ~0ull;
         }
-    return __shfl(eincl, kWave - 1, kWave);
+    __syncthreads();  // every key is in registers before the key areas are reused for events
+    bitonic_from<K, S, 2, 1>(v, lane);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        int w[K], local = 0;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const int e = lane * K + j;
+            w[j] = e < n[s] ? ((v[s][j] & 1) ? 1 : -1) : 0;
+            local += w[j];
+        }
+        const int incl = wave_incl_scan(local, lane);
+        int before = incl - local;
+        bool evt[K];
+        int nevt = 0;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const int after = before + w[j];
+            const bool in_b = evenodd ? (before & 1) != 0 : before != 0;
+            const bool in_a = evenodd ? (after & 1) != 0 : after != 0;
+            evt[j] = w[j] != 0 && in_b != in_a;
+            nevt += evt[j] ? 1 : 0;
+            before = after;
+        }
+        const int eincl = wave_incl_scan(nevt, lane);
+        int pos = eincl - nevt;
+        int32_t *ev = reinterpret_cast<int32_t *>(keys[s]);
+#pragma unroll
+        for (int j = 0; j < K; j++)
+            if (evt[j]) {
+                const int32_t x = (int32_t)((uint32_t)(v[s][j] >> 32) ^ 0x80000000u);
+                ev[pos++] = (x + 0x8000) >> 16;
+            }
+        nev[s] = __shfl(eincl, kWave - 1, kWave);
+    }
 }
 
 __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
     __shared__ uint64_t keys[kMaxSub][kMaxKeys];  // crossings; reused for span events
-    __shared__ int32_t acc[kWin + 1];             // acc (low 16 bits) + run difference (high 16 bits)
+    __shared__ __align__(16) int32_t acc[kWin + 4];  // acc (low 16 bits) + run difference (high 16 bits)
     const int lane = threadIdx.x;
     const uint32_t gr = blockIdx.x;
     const uint32_t ji = find_job(a.jobs, a.njobs, gr);
@@ -305,20 +332,28 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
         }
     }
     __syncthreads();
-    // ---- sort each sub-scanline and turn it into span events ----
+    // ---- sort the sub-scanlines together and turn them into span events ----
     int nev[kMaxSub] = {0, 0, 0, 0};
+    int nmax = 0;
     for (int s = 0; s < nsub; s++) {
-        int n = cnt[s];
-        if (n > kMaxKeys) {
+        if (cnt[s] > kMaxKeys) {
             if (lane == 0) atomicAdd(a.overflow, 1);
-            n = kMaxKeys;
+            cnt[s] = kMaxKeys;
         }
-        int32_t *ev = reinterpret_cast<int32_t *>(keys[s]);
-        if (n == 0) nev[s] = 0;
-        else if (n <= 64) nev[s] = spans_of<1>(keys[s], n, evenodd, ev, lane);
-        else if (n <= 128) nev[s] = spans_of<2>(keys[s], n, evenodd, ev, lane);
-        else if (n <= 256) nev[s] = spans_of<4>(keys[s], n, evenodd, ev, lane);
-        else nev[s] = spans_of<8>(keys[s], n, evenodd, ev, lane);
+        nmax = max(nmax, cnt[s]);
+    }
+    if (nsub == 1) {
+        if (nmax == 0) {}
+        else if (nmax <= 64) spans_of<1, 1>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 128) spans_of<2, 1>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 256) spans_of<4, 1>(keys, cnt, evenodd, nev, lane);
+        else spans_of<8, 1>(keys, cnt, evenodd, nev, lane);
+    } else {
+        if (nmax == 0) {}
+        else if (nmax <= 64) spans_of<1, kMaxSub>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 128) spans_of<2, kMaxSub>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 256) spans_of<4, kMaxSub>(keys, cnt, evenodd, nev, lane);
+        else spans_of<8, kMaxSub>(keys, cnt, evenodd, nev, lane);
     }
     __syncthreads();
     // ---- accumulate per window, prefix-sum, clamp, write u8 ----
@@ -362,10 +397,13 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
         int carry = 0;
         for (int32_t base = 0; base < ww; base += 4 * kWave) {
             const int32_t p0 = base + 4 * lane;
+            // one 16-byte LDS read per lane (acc is 16-byte aligned; p0 + 3 <= kWin + 3
+            // stays inside the padded array, entries past ww are zeroed below)
+            const int4 q4 = *reinterpret_cast<const int4 *>(&acc[p0]);
+            const int32_t packed4[4] = {q4.x, q4.y, q4.z, q4.w};
             int v[4], d[4], loc = 0;
             for (int q = 0; q < 4; q++) {
-                const int32_t p = p0 + q;
-                const int32_t packed = p < ww ? acc[p] : 0;
+                const int32_t packed = p0 + q < ww ? packed4[q] : 0;
                 v[q] = packed & 0xffff;
                 d[q] = packed >> 16;
                 loc += d[q];

@@ -37,7 +37,15 @@ rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint
 
 namespace {
 
-constexpr uint64_t kArenaBudget = 1ull << 30;  // coverage bytes per flush
+// Coverage bytes per flush (1 GiB); RVK_FLUSH_BYTES lowers it, which tests use to
+// drive many flushes through small documents.
+uint64_t arena_budget() {
+    if (const char *e = std::getenv("RVK_FLUSH_BYTES")) {
+        const long long v = std::atoll(e);
+        if (v > 0) return (uint64_t)v;
+    }
+    return 1ull << 30;
+}
 
 struct DrawList {
     uint8_t *px = nullptr;
@@ -151,6 +159,8 @@ struct Renderer {
     IRect max_bbox;
     rvk_status err = RVK_OK;
     uint64_t overflow_rows = 0;
+    int32_t *dev_overflow = nullptr;  // per render: crossings or rows the kernels could not hold
+    const uint64_t budget = arena_budget();
 
     // The walk runs twice: kCollect records every path drawn (with its device
     // transform), the geometry is computed in parallel, and kExecute replays the
@@ -302,8 +312,10 @@ struct Renderer {
             dl = DrawList{dl.px, dl.w, dl.h};
             return RVK_OK;
         }
-        // one device block: jobs | edges | ecov | windows | shaders | items | overflow ||
-        // npieces | pieces | arena (the part after || is written by the device)
+        // one device block: jobs | edges | ecov | windows | shaders | items || npieces |
+        // pieces | arena (the part after || is written by the device).  The metadata is
+        // packed straight into a pinned staging buffer and copied without a host wait;
+        // device buffers are reused only by later work on the same stream.
         auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
         const size_t o_jobs = 0;
         const size_t o_edges = al(o_jobs + dl.jobs.size() * sizeof(CovJob));
@@ -311,13 +323,15 @@ struct Renderer {
         const size_t o_win = al(o_ecov + dl.ecov.size() * sizeof(EdgeCov));
         const size_t o_sh = al(o_win + dl.windows.size() * sizeof(uint2));
         const size_t o_it = al(o_sh + dl.shaders.size() * sizeof(ShaderParams));
-        const size_t o_ov = al(o_it + dl.items.size() * sizeof(DrawItem));
-        const size_t meta = al(o_ov + 4);
+        const size_t meta = al(o_it + dl.items.size() * sizeof(DrawItem));
         const size_t o_np = meta;
         const size_t o_pc = al(o_np + dl.edges.size() * 4);
         const size_t o_arena = al(o_pc + (size_t)dl.piece_slots * sizeof(PieceDev));
-        std::vector<uint8_t> host(meta, 0);
-        auto put = [&](size_t off, const void *src, size_t n) { if (n) std::memcpy(host.data() + off, src, n); };
+        void *hostv = nullptr;
+        uint32_t slot = 0;
+        RVK_TRY(rvk::staging_acquire(ctx, meta, &hostv, &slot));
+        uint8_t *host = (uint8_t *)hostv;
+        auto put = [&](size_t off, const void *src, size_t n) { if (n) std::memcpy(host + off, src, n); };
         put(o_jobs, dl.jobs.data(), dl.jobs.size() * sizeof(CovJob));
         put(o_edges, dl.edges.data(), dl.edges.size() * sizeof(EdgeRec));
         put(o_ecov, dl.ecov.data(), dl.ecov.size() * sizeof(EdgeCov));
@@ -341,33 +355,23 @@ struct Renderer {
         ctx->stats.edge_pieces += dl.piece_slots;
         ctx->stats.band_entries += dl.windows.size();
         ctx->stats.upload_bytes += meta;
-        rvk_status st = rvk::upload_sync(ctx, d, host.data(), meta);
+        rvk_status st = rvk::upload_staged(ctx, slot, d, meta);
         if (st == RVK_OK)
             st = rvk::launch_expand(ctx, (const EdgeRec *)(d + o_edges), (uint32_t)dl.edges.size(),
                                     (const EdgeCov *)(d + o_ecov), (PieceDev *)(d + o_pc), (uint32_t *)(d + o_np),
-                                    (int32_t *)(d + o_ov), exp_bytes);
+                                    dev_overflow, exp_bytes);
         if (st == RVK_OK)
             st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)dl.jobs.size(), dl.rows,
                                       (const EdgeCov *)(d + o_ecov), (const uint2 *)(d + o_win),
                                       (const PieceDev *)(d + o_pc), (const uint32_t *)(d + o_np), d + o_arena,
-                                      (int32_t *)(d + o_ov), cov_bytes);
+                                      dev_overflow, cov_bytes);
         if (st == RVK_OK)
             st = rvk::launch_draw_items(ctx, dl.px, dl.w, dl.h, dl.bx0, dl.by0, dl.bx1, dl.by1,
                                         (const DrawItem *)(d + o_it), (uint32_t)dl.items.size(),
                                         (const CovJob *)(d + o_jobs), d + o_arena, (const ShaderParams *)(d + o_sh),
                                         draw_bytes);
-        int32_t ov = 0;
-        if (st == RVK_OK) {
-            const auto t0 = std::chrono::steady_clock::now();
-            hipError_t e = hipMemcpyAsync(&ov, d + o_ov, 4, hipMemcpyDeviceToHost, ctx->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-            if (e != hipSuccess) st = RVK_ERR_HIP;
-            ctx->stats.host_wait_ms +=
-                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        }
-        ctx->stats.flushes++;
         rvk::scratch_release(ctx, dev);
-        overflow_rows += (uint64_t)(ov > 0 ? ov : 0);
+        ctx->stats.flushes++;
         dl = DrawList{dl.px, dl.w, dl.h};
         return st;
     }
@@ -377,7 +381,7 @@ struct Renderer {
     }
 
     void render_path(const rvk_path &p, Ts ts, DrawList &dl) {
-        if (dl.arena > kArenaBudget) fail(flush(dl));
+        if (dl.arena > budget) fail(flush(dl));
         DrawItem it{};
         it.layer = 0;
         it.opacity = 1.0f;
@@ -438,7 +442,7 @@ struct Renderer {
         if (mode == kExecute) ctx->stats.layers++;
         // elided layer: one path child, no filters
         if (g.filter_count == 0 && g.child_count == 1 && sc.children[g.child_offset].kind == RVK_NODE_PATH) {
-            if (dl.arena > kArenaBudget) fail(flush(dl));
+            if (dl.arena > budget) fail(flush(dl));
             const rvk_path &p = sc.paths[sc.children[g.child_offset].index];
             DrawItem it{};
             it.layer = 1;
@@ -496,6 +500,16 @@ namespace rvk {
 rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t w, uint32_t h, uint8_t *px,
                         uint64_t *overflow_rows) {
     Renderer r{ctx, *scene, max_filter_bbox(w, h)};
+    void *ovbuf = nullptr;
+    RVK_TRY(rvk::scratch_alloc(ctx, 256, &ovbuf));
+    r.dev_overflow = (int32_t *)ovbuf;
+    {
+        const rvk_status s = rvk::launch_fill_sub(ctx, (uint8_t *)ovbuf, 1, 1, 0, 0, 1, 1, 0u);  // counter = 0
+        if (s != RVK_OK) {
+            rvk::scratch_release(ctx, ovbuf);
+            return s;
+        }
+    }
     DrawList root;
     root.w = w;
     root.h = h;
@@ -509,6 +523,21 @@ rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_
     root.h = h;
     r.render_nodes(scene->groups[0], ts, root);
     if (r.err == RVK_OK) r.fail(r.flush(root));
+    // the one wait of a render: read the overflow counter
+    int32_t ov = 0;
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        hipError_t e = hipMemcpyAsync(&ov, ovbuf, 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            r.fail(RVK_ERR_HIP);
+        }
+        ctx->stats.host_wait_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    rvk::scratch_release(ctx, ovbuf);
+    r.overflow_rows = (uint64_t)(ov > 0 ? ov : 0);
     if (overflow_rows) *overflow_rows = r.overflow_rows;
     if (r.err == RVK_OK && r.overflow_rows) return RVK_ERR_UNSUPPORTED;
     return r.err;

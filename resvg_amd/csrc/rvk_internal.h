@@ -30,6 +30,14 @@ struct rvk_prof_rec {
     double bytes;
 };
 
+// A pinned host staging buffer; `done` is recorded after the copy that reads it.
+struct rvk_staging {
+    void *host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+
 struct rvk_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -38,6 +46,8 @@ struct rvk_context {
     bool profiling = false;
     std::vector<rvk_prof_rec> prof;
     std::vector<hipEvent_t> event_pool;
+    rvk_staging staging[3];  // ring used by upload_async
+    uint32_t staging_next = 0;
     // Simple size-bucketed cache of device scratch buffers (layers, filter images).
     std::vector<std::pair<size_t, void *>> free_list;
     std::vector<std::pair<size_t, void *>> live;
@@ -54,6 +64,13 @@ void scratch_release(rvk_context *ctx, void *p);
 
 // Copies host memory to the device on the context stream and waits for it.
 rvk_status upload_sync(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes);
+// A pinned staging buffer of at least `bytes` from the context's ring (waits until
+// the copy that last used it has finished); fill it, then call upload_staged.
+rvk_status staging_acquire(rvk_context *ctx, size_t bytes, void **host, uint32_t *slot);
+// Enqueues the copy of `bytes` of staging slot `slot` to `dev_dst` (no host wait).
+rvk_status upload_staged(rvk_context *ctx, uint32_t slot, void *dev_dst, size_t bytes);
+// Copies host memory to the device through the staging ring without waiting.
+rvk_status upload_async(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes);
 
 static inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 

@@ -173,9 +173,9 @@ rvk_status launch_turbulence(rvk_context *ctx, uint8_t *dst, uint32_t w, uint32_
     RVK_TRY(scratch_alloc(ctx, lat_bytes + 64 + grad_bytes, &buf));
     int32_t *dlat = (int32_t *)buf;
     double *dgrad = (double *)((uint8_t *)buf + ((lat_bytes + 63) / 64) * 64);
-    // the host tables die with this call, so the uploads complete before returning
-    rvk_status ust = upload_sync(ctx, dlat, lat.data(), lat_bytes);
-    if (ust == RVK_OK) ust = upload_sync(ctx, dgrad, grad.data(), grad_bytes);
+    // the tables are copied into the pinned staging ring, so they may die with this call
+    rvk_status ust = upload_async(ctx, dlat, lat.data(), lat_bytes);
+    if (ust == RVK_OK) ust = upload_async(ctx, dgrad, grad.data(), grad_bytes);
     if (ust != RVK_OK) { scratch_release(ctx, buf); return ust; }
     hipError_t e;
     TurbArgs a{ox, oy, sx, sy, bfx, bfy, octaves, stitch ? 1u : 0u, fractal ? 1u : 0u, premultiply ? 1u : 0u};

@@ -168,6 +168,16 @@ def test_small_full_filter_document(ctx, seed):
     _assert_same(_gpu(ctx, sc, ts, w, h), _oracle(sc, ts, w, h), f"full-filter 384 seed {seed}")
 
 
+def test_many_small_flushes(ctx, monkeypatch):
+    # a tiny flush budget sends every path in its own flush through the staging ring
+    monkeypatch.setenv("RVK_FLUSH_BYTES", "4096")
+    sc, (w, h) = full_filter_doc(seed=3, size=256, n_paths=24, paths_per_filter=2, max_segments=16)
+    ts = abi.Transform.identity()
+    got = _gpu(ctx, sc, ts, w, h)
+    assert ctx.stats().flushes >= 10
+    _assert_same(got, _oracle(sc, ts, w, h), "many flushes")
+
+
 def test_render_host_matches_device(ctx):
     import resvg_amd
     sc = _random_scene(400, 150, 10)
