@@ -46,6 +46,7 @@ OTHER = [
     "painting/stroke-linecap/zero-length-path-with-round", "painting/stroke-linecap/zero-length-path-with-square",
     "shapes/path/M-C", "shapes/path/M-Q", "shapes/path/M-C-S", "shapes/path/M-Q-T", "shapes/path/M-L-L-Z",
     "painting/stroke-miterlimit/default", "painting/stroke-miterlimit/valid-value",
+    "painting/stroke/control-points-clamping-1",
 ]
 
 

@@ -312,6 +312,15 @@ def _stroke_fixtures():
         sc = Scene(200, 200)
         sc.path(sc.root, verbs, pts, **green5)
         out[f"shapes/path/{name}"] = sc
+    # painting/stroke/control-points-clamping-1: nearly straight cubics (control points
+    # 0.1 off the line) with bevel joins, widths 20 .. 0.5 (0.75 px: a hairline)
+    sc = Scene(200, 200)
+    g = sc.group(sc.root)
+    for x, y, wdt in ((175, 175, 20.0), (140, 140, 10.0), (120, 115, 5.0), (100, 95, 2.5), (85, 80, 1.0),
+                      (70, 65, 0.5)):
+        sc.path(g, [M, C, L], [(x, 15), (x, 30), (x + 0.1, y), (x, y), (15, y)], stroke=PaintSpec.solid(GREEN),
+                stroke_width=wdt, stroke_join=abi.JOIN_BEVEL)
+    out["painting/stroke/control-points-clamping-1"] = sc
     for name, limit in (("default", 4.0), ("valid-value", 10.0)):
         sc = Scene(200, 200)
         sc.path(sc.root, [M, L, L], [(40, 70), (120, 90), (40, 110)], stroke=PaintSpec.solid(GREEN),
