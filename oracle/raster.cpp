@@ -16,6 +16,7 @@
 #include "oracle.h"
 #include "common.h"
 #include "geom.h"
+#include "../resvg_amd/csrc/hairline.h"
 #include "../resvg_amd/csrc/stroker.h"
 #include <algorithm>
 #include <array>
@@ -626,11 +627,16 @@ static void set_solid(Shader &s, const float c[4]) {
     out[3] = (uint32_t)(a * 255.0f + 0.5f);
 }
 
-static Shader make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts) {
+// `opacity` is Shader::apply_opacity's factor (a hairline's coverage), applied to the
+// colour's alpha or to every stop's.
+static Shader make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, float opacity = 1.0f) {
     Shader s;
+    const float op = std::min(std::max(opacity, 0.0f), 1.0f);
+    auto scale_alpha = [op](float &a) { a = std::min(std::max(a * op, 0.0f), 1.0f); };
     if (p.kind == RVK_PAINT_COLOR) {
         float c[4] = {(float)p.color[0] / 255.0f, (float)p.color[1] / 255.0f, (float)p.color[2] / 255.0f,
                       (float)as_u8(std::min(std::max(p.opacity, 0.0f), 1.0f) * 255.0f + 0.5f) / 255.0f};
+        scale_alpha(c[3]);
         set_solid(s, c);
         s.valid = true;
         return s;
@@ -650,6 +656,7 @@ static Shader make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts) {
     }
     if (stops.empty()) return s;
     if (stops.size() == 1) {
+        scale_alpha(stops[0].c[3]);
         set_solid(s, stops[0].c);
         s.valid = true;
         return s;
@@ -659,13 +666,16 @@ static Shader make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts) {
         float dx = p.x2 - p.x1, dy = p.y2 - p.y1, len = std::sqrt(dx * dx + dy * dy);
         if (!std::isfinite(len)) return s;
         if (len <= 1.0f / 32768.0f) {  // degenerate: last colour (pad) / average (repeat, reflect)
-            if (p.spread == RVK_SPREAD_PAD) set_solid(s, stops.back().c);
-            else {
+            if (p.spread == RVK_SPREAD_PAD) {
+                scale_alpha(stops.back().c[3]);
+                set_solid(s, stops.back().c);
+            } else {
                 float avg[4] = {0, 0, 0, 0};
                 for (size_t i = 0; i + 1 < stops.size(); i++) {
                     float wgt = stops[i + 1].pos - stops[i].pos;
                     for (int k = 0; k < 4; k++) avg[k] += wgt * (stops[i].c[k] + stops[i + 1].c[k]) * 0.5f;
                 }
+                scale_alpha(avg[3]);
                 set_solid(s, avg);
             }
             s.valid = true;
@@ -703,6 +713,7 @@ static Shader make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts) {
     bool dummy_first = stops.front().pos != 0.0f, dummy_last = stops.back().pos != 1.0f;
     if (dummy_first) stops.insert(stops.begin(), GS{0.0f, {stops.front().c[0], stops.front().c[1], stops.front().c[2], stops.front().c[3]}});
     if (dummy_last) stops.push_back(GS{1.0f, {stops.back().c[0], stops.back().c[1], stops.back().c[2], stops.back().c[3]}});
+    for (auto &g : stops) scale_alpha(g.c[3]);
     s.opaque = true;
     for (auto &g : stops) s.opaque = s.opaque && g.c[3] == 1.0f;
     s.as_source = s.opaque && !s.radial;
@@ -805,26 +816,251 @@ static void shade(const Shader &s, uint32_t px, uint32_t py, uint32_t out[4]) {
     if (!s.opaque) for (int k = 0; k < 3; k++) out[k] = div255(out[k] * out[3]);
 }
 
-// Composite `cov` x shader onto px: lowp SourceOver (scale_1_float, source_over), or
-// for an opaque shader Source (full coverage stores the colour, partial coverage is
-// lerp_1_float: div255(d * (255 - c) + s * c)).
+// Composite coverage `c` (1..255) x shader onto one pixel: lowp SourceOver
+// (scale_1_float, source_over), or for an opaque shader Source (full coverage stores
+// the colour, partial coverage is lerp_1_float: div255(d * (255 - c) + s * c)).
+static void composite_px(const Shader &s, uint32_t c, uint32_t x, uint32_t y, uint8_t *d) {
+    uint32_t src[4];
+    shade(s, x, y, src);
+    if (s.as_source) {
+        for (int k = 0; k < 4; k++) d[k] = (uint8_t)(c == 255 ? src[k] : div255(d[k] * (255 - c) + src[k] * c));
+        return;
+    }
+    if (c != 255) for (int k = 0; k < 4; k++) src[k] = div255(src[k] * c);
+    uint32_t ia = 255 - src[3];
+    for (int k = 0; k < 4; k++) d[k] = (uint8_t)(src[k] + div255(d[k] * ia));
+}
+
 static void blit(const Shader &s, const std::vector<uint8_t> &cov, uint8_t *px, uint32_t w, uint32_t h) {
     for (uint32_t y = 0; y < h; y++) {
         for (uint32_t x = 0; x < w; x++) {
             uint32_t c = cov[(size_t)y * w + x];
-            if (!c) continue;
-            uint32_t src[4];
-            shade(s, x, y, src);
-            uint8_t *d = px + 4 * ((size_t)y * w + x);
-            if (s.as_source) {
-                for (int k = 0; k < 4; k++)
-                    d[k] = (uint8_t)(c == 255 ? src[k] : div255(d[k] * (255 - c) + src[k] * c));
-                continue;
-            }
-            if (c != 255) for (int k = 0; k < 4; k++) src[k] = div255(src[k] * c);
-            uint32_t ia = 255 - src[3];
-            for (int k = 0; k < 4; k++) d[k] = (uint8_t)(src[k] + div255(d[k] * ia));
+            if (c) composite_px(s, c, x, y, px + 4 * ((size_t)y * w + x));
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Anti-aliased hairlines: tiny-skia scan/hairline_aa.rs do_anti_hairline (Skia's
+// SkScan_Antihair.cpp) with its four blitters (HLine, Horish, VLine, Vertish), each
+// blit composited at once in drawing order; the segments come from
+// resvg_amd/csrc/hairline.cpp (shared geometry).
+// ---------------------------------------------------------------------------
+struct HairTarget {
+    const Shader &s;
+    uint8_t *px;
+    int32_t w, h;
+    bool clipped;  // RectClipBlitter
+    int32_t cl, ct, cr, cb;
+    void blit(int32_t x, int32_t y, int32_t a) {
+        if (a <= 0) return;  // zero coverage leaves the pixel unchanged
+        if (clipped && (x < cl || x >= cr || y < ct || y >= cb)) return;
+        if (x < 0 || y < 0 || x >= w || y >= h) return;
+        composite_px(s, (uint32_t)a, (uint32_t)x, (uint32_t)y, px + 4 * ((size_t)y * w + x));
+    }
+};
+
+enum class HairKind { HLine, Horish, VLine, Vertish };
+
+static inline int32_t small_dot6_scale(int32_t value, int32_t dot6) { return (value * dot6) >> 6; }
+static inline int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
+// draw_cap: one step at `i` scaled by mod64/64; returns the next position
+static int32_t hair_draw_cap(HairKind k, HairTarget &t, int32_t i, int32_t f, int32_t slope, int32_t mod64) {
+    f = wrap_add(f, 0x8000);
+    const int32_t lo = f >> 16;
+    const int32_t a = (f >> 8) & 0xff;
+    switch (k) {
+        case HairKind::HLine:
+            t.blit(i, lo, small_dot6_scale(a, mod64));
+            t.blit(i, lo - 1, small_dot6_scale(255 - a, mod64));
+            return wrap_add(f, -0x8000);
+        case HairKind::Horish:
+            t.blit(i, lo - 1, small_dot6_scale(255 - a, mod64));
+            t.blit(i, lo, small_dot6_scale(a, mod64));
+            return wrap_add(wrap_add(f, slope), -0x8000);
+        case HairKind::VLine:
+            t.blit(lo, i, small_dot6_scale(a, mod64));
+            t.blit(lo - 1, i, small_dot6_scale(255 - a, mod64));
+            return wrap_add(f, -0x8000);
+        case HairKind::Vertish:
+        default:
+            t.blit(lo - 1, i, small_dot6_scale(255 - a, mod64));
+            t.blit(lo, i, small_dot6_scale(a, mod64));
+            return wrap_add(wrap_add(f, slope), -0x8000);
+    }
+}
+
+// draw_line: full steps [i, stop)
+static int32_t hair_draw_line(HairKind k, HairTarget &t, int32_t i, int32_t stop, int32_t f, int32_t slope) {
+    f = wrap_add(f, 0x8000);
+    if (k == HairKind::HLine || k == HairKind::VLine) {
+        const int32_t lo = f >> 16;
+        const int32_t a = (f >> 8) & 0xff;
+        for (int32_t j = i; j < stop; j++) k == HairKind::HLine ? t.blit(j, lo, a) : t.blit(lo, j, a);
+        for (int32_t j = i; j < stop; j++) k == HairKind::HLine ? t.blit(j, lo - 1, 255 - a) : t.blit(lo - 1, j, 255 - a);
+        return wrap_add(f, -0x8000);
+    }
+    do {
+        const int32_t lo = f >> 16;
+        const int32_t a = (f >> 8) & 0xff;
+        if (k == HairKind::Horish) {
+            t.blit(i, lo - 1, 255 - a);
+            t.blit(i, lo, a);
+        } else {
+            t.blit(lo - 1, i, 255 - a);
+            t.blit(lo, i, a);
+        }
+        f = wrap_add(f, slope);
+    } while (++i < stop);
+    return wrap_add(f, -0x8000);
+}
+
+static void do_anti_hairline(int32_t x0, int32_t y0, int32_t x1, int32_t y1, const int32_t *clip, HairTarget &t) {
+    auto bad = [](int32_t v) { return v == INT32_MIN; };  // integer NaN
+    if (bad(x0) || bad(y0) || bad(x1) || bad(y1)) return;
+    if (std::abs(x1 - x0) > 511 * 64 || std::abs(y1 - y0) > 511 * 64) {
+        const int32_t hx = (x0 >> 1) + (x1 >> 1), hy = (y0 >> 1) + (y1 >> 1);
+        do_anti_hairline(x0, y0, hx, hy, clip, t);
+        do_anti_hairline(hx, hy, x1, y1, clip, t);
+        return;
+    }
+    auto fdot6_floor = [](int32_t v) { return v >> 6; };
+    auto fdot6_ceil = [](int32_t v) { return (v + 63) >> 6; };
+    auto contribution_64 = [](int32_t o) { return ((o - 1) & 63) + 1; };
+    auto fixed_floor = [](int32_t v) { return v >> 16; };
+    auto fixed_ceil = [](int32_t v) { return (int32_t)(((int64_t)v + 0xffff) >> 16); };
+    int32_t scale_start, scale_stop, istart, istop, fstart, slope;
+    HairKind kind;
+    bool use_clip = clip != nullptr;
+    if (std::abs(x1 - x0) > std::abs(y1 - y0)) {  // mostly horizontal
+        if (x0 > x1) { std::swap(x0, x1); std::swap(y0, y1); }
+        istart = fdot6_floor(x0);
+        istop = fdot6_ceil(x1);
+        fstart = (int32_t)((uint32_t)y0 << 10);
+        if (y0 == y1) {
+            slope = 0;
+            kind = HairKind::HLine;
+        } else {
+            slope = (int32_t)((uint32_t)(y1 - y0) << 16) / (x1 - x0);
+            fstart += (slope * (32 - (x0 & 63)) + 32) >> 6;
+            kind = HairKind::Horish;
+        }
+        if (istop - istart == 1) {
+            scale_start = x1 - x0;
+            scale_stop = 0;
+        } else {
+            scale_start = 64 - (x0 & 63);
+            scale_stop = x1 & 63;
+        }
+        if (use_clip) {
+            const int32_t cl = clip[0], ct = clip[1], cr = clip[2], cb = clip[3];
+            if (istart >= cr || istop <= cl) return;
+            if (istart < cl) {
+                fstart += slope * (cl - istart);
+                istart = cl;
+                scale_start = 64;
+                if (istop - istart == 1) {
+                    scale_start = contribution_64(x1);
+                    scale_stop = 0;
+                }
+            }
+            if (istop > cr) {
+                istop = cr;
+                scale_stop = 0;
+            }
+            if (istart == istop) return;
+            int32_t top, bottom;
+            if (slope >= 0) {
+                top = fixed_floor(fstart - 0x8000);
+                bottom = fixed_ceil(fstart + (istop - istart - 1) * slope + 0x8000);
+            } else {
+                bottom = fixed_ceil(fstart + 0x8000);
+                top = fixed_floor(fstart + (istop - istart - 1) * slope - 0x8000);
+            }
+            if (top >= cb || bottom <= ct) return;
+            if (ct <= top && cb >= bottom) use_clip = false;
+        }
+    } else {  // mostly vertical
+        if (y0 > y1) { std::swap(x0, x1); std::swap(y0, y1); }
+        istart = fdot6_floor(y0);
+        istop = fdot6_ceil(y1);
+        fstart = (int32_t)((uint32_t)x0 << 10);
+        if (x0 == x1) {
+            if (y0 == y1) return;
+            slope = 0;
+            kind = HairKind::VLine;
+        } else {
+            slope = (int32_t)((uint32_t)(x1 - x0) << 16) / (y1 - y0);
+            fstart += (slope * (32 - (y0 & 63)) + 32) >> 6;
+            kind = HairKind::Vertish;
+        }
+        if (istop - istart == 1) {
+            scale_start = y1 - y0;
+            scale_stop = 0;
+        } else {
+            scale_start = 64 - (y0 & 63);
+            scale_stop = y1 & 63;
+        }
+        if (use_clip) {
+            const int32_t cl = clip[0], ct = clip[1], cr = clip[2], cb = clip[3];
+            if (istart >= cb || istop <= ct) return;
+            if (istart < ct) {
+                fstart += slope * (ct - istart);
+                istart = ct;
+                scale_start = 64;
+                if (istop - istart == 1) {
+                    scale_start = contribution_64(y1);
+                    scale_stop = 0;
+                }
+            }
+            if (istop > cb) {
+                istop = cb;
+                scale_stop = 0;
+            }
+            if (istart == istop) return;
+            int32_t left, right;
+            if (slope >= 0) {
+                left = fixed_floor(fstart - 0x8000);
+                right = fixed_ceil(fstart + (istop - istart - 1) * slope + 0x8000);
+            } else {
+                right = fixed_ceil(fstart + 0x8000);
+                left = fixed_floor(fstart + (istop - istart - 1) * slope - 0x8000);
+            }
+            if (left >= cr || right <= cl) return;
+            if (cl <= left && cr >= right) use_clip = false;
+        }
+    }
+    t.clipped = use_clip;
+    if (use_clip) { t.cl = clip[0]; t.ct = clip[1]; t.cr = clip[2]; t.cb = clip[3]; }
+    fstart = hair_draw_cap(kind, t, istart, fstart, slope, scale_start);
+    istart += 1;
+    const int32_t full_spans = istop - istart - (scale_stop > 0 ? 1 : 0);
+    if (full_spans > 0) fstart = hair_draw_line(kind, t, istart, istart + full_spans, fstart, slope);
+    if (scale_stop > 0) hair_draw_cap(kind, t, istop - 1, fstart, slope, scale_stop);
+}
+
+// tiny-skia Pixmap::stroke_path's hairline branch -> stroke_hairline -> hairline_aa::stroke_path
+static void stroke_hairline(const rvk_scene &sc, const rvk_path &path, Ts ts, const Shader &s, uint8_t *px,
+                            uint32_t w, uint32_t h) {
+    const uint8_t *verbs = sc.verbs + path.verb_offset;
+    const float *pts = sc.points + 2 * (size_t)path.point_offset;
+    size_t npts = 0;
+    for (uint32_t v = 0; v < path.verb_count; v++)
+        npts += verbs[v] == RVK_VERB_MOVE || verbs[v] == RVK_VERB_LINE ? 1 : verbs[v] == RVK_VERB_QUAD ? 2
+              : verbs[v] == RVK_VERB_CUBIC ? 3 : 0;
+    std::vector<rvks::Pt> dev(npts);
+    for (size_t i = 0; i < npts; i++) {
+        float x = pts[2 * i], y = pts[2 * i + 1];
+        ts_map(ts, x, y);  // Path::transform
+        dev[i] = rvks::Pt{x, y};
+    }
+    std::vector<rvks::HairSeg> segs;
+    rvks::anti_hairline_segments(verbs, path.verb_count, dev.data(), npts, path.stroke_cap, w, h, segs);
+    HairTarget t{s, px, (int32_t)w, (int32_t)h, false, 0, 0, 0, 0};
+    for (const rvks::HairSeg &g : segs) {
+        const int32_t c[4] = {g.cl, g.ct, g.cr, g.cb};
+        do_anti_hairline(g.x0, g.y0, g.x1, g.y1, g.clipped ? c : nullptr, t);
     }
 }
 
@@ -865,6 +1101,13 @@ bool fill_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *p
 bool stroke_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h) {
     if (path.stroke.kind == RVK_PAINT_NONE) return false;
     if (!(path.stroke_width >= 0.0f)) return false;
+    float hair_cov;
+    if (rvks::treat_as_hairline(path.stroke_width, path.anti_alias != 0, ts.sx, ts.ky, ts.kx, ts.sy, hair_cov)) {
+        Shader s = make_shader(sc, path.stroke, ts, rvks::hairline_paint_opacity(hair_cov));
+        if (!s.valid) return false;
+        stroke_hairline(sc, path, ts, s, px, w, h);
+        return true;
+    }
     Shader s = make_shader(sc, path.stroke, ts);
     if (!s.valid) return false;
     std::vector<Seg> segs;

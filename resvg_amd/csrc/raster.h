@@ -55,6 +55,33 @@ struct CovJob {
     uint64_t cov_offset;     // byte offset into the coverage arena
 };
 
+// An anti-aliased hairline segment set up for per-pixel evaluation: the state of
+// tiny-skia's do_anti_hairline (scan/hairline_aa.rs) after its setup and clipping.
+// Along the major axis (x, or y with kHairYMajor), pixel i in [istart, istop) sits at
+// the 16.16 minor position f = fstart + (i - istart) * slope + 1/2; it blits coverage
+// a = (f >> 8) & 255 to minor pixel f >> 16 and 255 - a to the pixel before, both
+// scaled by scale_start / 64 at istart and by scale_stop / 64 at istop - 1 when
+// scale_stop > 0.  With kHairClipped, blits outside [cl, cr) x [ct, cb) are dropped.
+struct HairLine {
+    int32_t istart, istop, fstart, slope;
+    int32_t scale_start, scale_stop;
+    uint32_t flags;
+    int32_t cl, ct, cr, cb;
+    int32_t pad_;
+};
+constexpr uint32_t kHairYMajor = 1, kHairClipped = 2;
+constexpr int kHairTile = 32;  // pixels per side of a hairline bin tile
+
+// A hairline job: the lines of one hairline stroke binned, in drawing order, into
+// kHairTile x kHairTile tiles of the rect [x0, x0 + w) x [y0, y0 + h) (edge space).
+// Tile t's line indices are bins[tiles[tile_offset + t] .. tiles[tile_offset + t + 1]).
+struct HairJob {
+    int32_t x0, y0, w, h;
+    uint32_t tiles_x, tiles_y;
+    uint32_t tile_offset;
+    uint32_t pad_;
+};
+
 // Shader parameters (tiny-skia lowp stages for solid, linear and concentric radial).
 constexpr int kMaxStops = 16;
 struct ShaderParams {
@@ -84,6 +111,8 @@ struct DrawItem {
     uint32_t layer;             // 0 direct, 1 elided layer
     float opacity;
     uint32_t blend_mode;
+    uint32_t stroke_hair;       // 1: stroke_job is a hairline job
+    uint32_t pad_;
 };
 
 struct Pt { float x, y; };
@@ -98,8 +127,14 @@ void stroke_segments(const rvk_scene &sc, const rvk_path &path, rvkh::Ts ts, std
 // tiny-skia's edge-list order (ids 0..n-1).  Returns the maximum number of pieces
 // each edge can produce in `caps`.
 void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &out, std::vector<uint32_t> &caps);
-// tiny-skia shader setup; false when the paint draws nothing.
-bool make_shader(const rvk_scene &sc, const rvk_paint &paint, rvkh::Ts ts, ShaderParams &out);
+// tiny-skia's hairline stroke of a path drawn with `ts` into a w x h pixmap
+// (Pixmap::stroke_path's hairline branch): the prepared lines in drawing order.
+void hairline_lines(const rvk_scene &sc, const rvk_path &path, rvkh::Ts ts, uint32_t w, uint32_t h,
+                    std::vector<HairLine> &out);
+// tiny-skia shader setup; false when the paint draws nothing.  `extra_opacity`
+// scales the colour's or every stop's alpha (Shader::apply_opacity).
+bool make_shader(const rvk_scene &sc, const rvk_paint &paint, rvkh::Ts ts, ShaderParams &out,
+                 float extra_opacity = 1.0f);
 // Control-point bounds of device-space segments nearly zero (tiny-skia skips such fills).
 bool bounds_nearly_zero(const std::vector<Seg> &segs);
 

@@ -29,6 +29,8 @@ using rvkr::CovJob;
 using rvkr::DrawItem;
 using rvkr::EdgeCov;
 using rvkr::EdgeRec;
+using rvkr::HairJob;
+using rvkr::HairLine;
 using rvkr::PieceDev;
 using rvkr::ShaderParams;
 
@@ -499,32 +501,88 @@ __device__ __forceinline__ uint32_t job_cov(const CovJob *jobs, const uint8_t *a
     return arena[jb.cov_offset + (size_t)cy * stride + cx];
 }
 
+// Coverage of one hairline blit at pixel (x, y) (see HairLine); 0 when the line does
+// not blit the pixel.
+__device__ __forceinline__ uint32_t hair_alpha(const HairLine &L, int32_t x, int32_t y) {
+    const bool ymaj = (L.flags & rvkr::kHairYMajor) != 0;
+    const int32_t major = ymaj ? y : x, minor = ymaj ? x : y;
+    if (major < L.istart || major >= L.istop) return 0;
+    if ((L.flags & rvkr::kHairClipped) && (x < L.cl || x >= L.cr || y < L.ct || y >= L.cb)) return 0;
+    const int32_t k = major - L.istart;
+    int32_t mod = 64;
+    if (k == 0) mod = L.scale_start;
+    else if (L.scale_stop > 0 && major == L.istop - 1) mod = L.scale_stop;
+    const int32_t f = (int32_t)((uint32_t)L.fstart + (uint32_t)k * (uint32_t)L.slope + 0x8000u);
+    const int32_t lo = f >> 16;
+    const int32_t a = (f >> 8) & 0xff;
+    if (minor == lo) return (uint32_t)((a * mod) >> 6);
+    if (minor == lo - 1) return (uint32_t)(((255 - a) * mod) >> 6);
+    return 0;
+}
+
+struct DrawArgs {
+    const DrawItem *items;
+    uint32_t nitems;
+    const CovJob *jobs;
+    const uint8_t *arena;
+    const ShaderParams *shaders;
+    const HairJob *hair_jobs;
+    const uint32_t *hair_tiles, *hair_bins;
+    const HairLine *hair_lines;
+};
+
 constexpr int TX = 64, TY = 4;
 
 __global__ __launch_bounds__(TX * TY) void k_draw(uint32_t *px, uint32_t w, uint32_t h, int32_t ox, int32_t oy,
-                                                 const DrawItem *items, uint32_t nitems, const CovJob *jobs,
-                                                 const uint8_t *arena, const ShaderParams *shaders) {
+                                                 DrawArgs a) {
     const int32_t x = ox + (int32_t)(blockIdx.x * TX + threadIdx.x % TX);
     const int32_t y = oy + (int32_t)(blockIdx.y * TY + threadIdx.x / TX);
     const int32_t tx0 = ox + (int32_t)(blockIdx.x * TX), ty0 = oy + (int32_t)(blockIdx.y * TY);
     const bool inside = x < (int32_t)w && y < (int32_t)h;
     uint32_t d = inside ? px[(size_t)y * w + x] : 0u;
     const uint32_t d0 = d;
-    for (uint32_t it = 0; it < nitems; it++) {
-        const DrawItem item = items[it];
+    for (uint32_t it = 0; it < a.nitems; it++) {
+        const DrawItem item = a.items[it];
         if (item.x1 <= tx0 || item.x0 >= tx0 + TX || item.y1 <= ty0 || item.y0 >= ty0 + TY) continue;
         if (!inside || x < item.x0 || x >= item.x1 || y < item.y0 || y >= item.y1) continue;
         const int32_t lx = x - item.lx, ly = y - item.ly;
-        const uint32_t cf = job_cov(jobs, arena, item.fill_job, lx, ly);
-        const uint32_t cs = job_cov(jobs, arena, item.stroke_job, lx, ly);
-        if ((cf | cs) == 0) continue;
+        const uint32_t cf = job_cov(a.jobs, a.arena, item.fill_job, lx, ly);
+        uint32_t cs = 0, hb = 0, he = 0;
+        if (item.stroke_hair) {
+            // the hairline lines binned to this pixel's tile, in drawing order
+            const HairJob &hj = a.hair_jobs[item.stroke_job];
+            const int32_t cx = lx - hj.x0, cy = ly - hj.y0;
+            if (cx >= 0 && cy >= 0 && cx < hj.w && cy < hj.h) {
+                const uint32_t t = hj.tile_offset + (uint32_t)(cy / rvkr::kHairTile) * hj.tiles_x +
+                                   (uint32_t)(cx / rvkr::kHairTile);
+                hb = a.hair_tiles[t];
+                he = a.hair_tiles[t + 1];
+            }
+        } else {
+            cs = job_cov(a.jobs, a.arena, item.stroke_job, lx, ly);
+        }
+        if ((cf | cs) == 0 && hb == he) continue;
         uint32_t target = item.layer ? 0u : d;
         for (int pass = 0; pass < 2; pass++) {
             const bool stroke = (pass == 0) == (item.stroke_first != 0);
+            const ShaderParams &sh = a.shaders[stroke ? item.stroke_shader : item.fill_shader];
+            uint32_t src[4];
+            if (stroke && item.stroke_hair) {
+                // every blit of the hairline at this pixel, composited in order
+                bool shaded = false;
+                for (uint32_t k = hb; k < he; k++) {
+                    const uint32_t c = hair_alpha(a.hair_lines[a.hair_bins[k]], lx, ly);
+                    if (!c) continue;
+                    if (!shaded) {
+                        shade(sh, lx, ly, src);
+                        shaded = true;
+                    }
+                    target = over_lowp(src, c, target, sh.as_source != 0);
+                }
+                continue;
+            }
             const uint32_t c = stroke ? cs : cf;
             if (!c) continue;
-            uint32_t src[4];
-            const ShaderParams &sh = shaders[stroke ? item.stroke_shader : item.fill_shader];
             shade(sh, lx, ly, src);
             target = over_lowp(src, c, target, sh.as_source != 0);
         }
@@ -568,12 +626,14 @@ rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs,
 
 rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
                              int32_t x1, int32_t y1, const DrawItem *items, uint32_t nitems, const CovJob *jobs,
-                             const uint8_t *arena, const ShaderParams *shaders, double algo_bytes) {
+                             const uint8_t *arena, const ShaderParams *shaders, const HairJob *hair_jobs,
+                             const uint32_t *hair_tiles, const uint32_t *hair_bins, const HairLine *hair_lines,
+                             double algo_bytes) {
     if (nitems == 0 || x1 <= x0 || y1 <= y0) return RVK_OK;
     dim3 grid(div_up((uint32_t)(x1 - x0), TX), div_up((uint32_t)(y1 - y0), TY));
+    const DrawArgs a{items, nitems, jobs, arena, shaders, hair_jobs, hair_tiles, hair_bins, hair_lines};
     timed_launch(ctx, "draw_items", algo_bytes, [&] {
-        hipLaunchKernelGGL(k_draw, grid, dim3(TX * TY), 0, ctx->stream, (uint32_t *)target, w, h, x0, y0, items, nitems,
-                           jobs, arena, shaders);
+        hipLaunchKernelGGL(k_draw, grid, dim3(TX * TY), 0, ctx->stream, (uint32_t *)target, w, h, x0, y0, a);
     });
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;

@@ -7,6 +7,7 @@
 // curve edge is run here once and each non-empty line piece is kept, which is the
 // sequence tiny-skia's scanline walk visits lazily.
 #include "raster.h"
+#include "hairline.h"
 #include "stroker.h"
 #include <algorithm>
 #include <array>
@@ -474,6 +475,110 @@ void stroke_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vect
 }
 
 // ---------------------------------------------------------------------------
+// Hairlines: the shared segment walk (hairline.cpp), then do_anti_hairline's setup
+// (tiny-skia scan/hairline_aa.rs): halving of lines longer than 511 px, the major
+// axis, the start position and slope, the cap scales and the clip adjustments.  The
+// per-pixel blits are evaluated on the device from the resulting HairLine.
+// ---------------------------------------------------------------------------
+namespace {
+
+void prepare_anti_hairline(int32_t x0, int32_t y0, int32_t x1, int32_t y1, const rvks::HairSeg &seg,
+                           std::vector<HairLine> &out) {
+    if (x0 == INT32_MIN || y0 == INT32_MIN || x1 == INT32_MIN || y1 == INT32_MIN) return;
+    if (std::abs(x1 - x0) > 511 * 64 || std::abs(y1 - y0) > 511 * 64) {
+        const int32_t hx = (x0 >> 1) + (x1 >> 1), hy = (y0 >> 1) + (y1 >> 1);
+        prepare_anti_hairline(x0, y0, hx, hy, seg, out);
+        prepare_anti_hairline(hx, hy, x1, y1, seg, out);
+        return;
+    }
+    auto fdot6_floor = [](int32_t v) { return v >> 6; };
+    auto fdot6_ceil = [](int32_t v) { return (v + 63) >> 6; };
+    auto fixed_floor = [](int32_t v) { return v >> 16; };
+    auto fixed_ceil = [](int32_t v) { return (int32_t)(((int64_t)v + 0xffff) >> 16); };
+    HairLine L{};
+    const bool ymajor = !(std::abs(x1 - x0) > std::abs(y1 - y0));
+    // work in (major, minor) coordinates
+    int32_t a0 = ymajor ? y0 : x0, b0 = ymajor ? x0 : y0, a1 = ymajor ? y1 : x1, b1 = ymajor ? x1 : y1;
+    if (a0 > a1) { std::swap(a0, a1); std::swap(b0, b1); }
+    if (ymajor && x0 == x1 && y0 == y1) return;  // zero length
+    int32_t istart = fdot6_floor(a0), istop = fdot6_ceil(a1);
+    int32_t fstart = (int32_t)((uint32_t)b0 << 10), slope = 0;
+    if (b0 != b1) {
+        slope = (int32_t)((uint32_t)(b1 - b0) << 16) / (a1 - a0);
+        fstart += (slope * (32 - (a0 & 63)) + 32) >> 6;
+    }
+    int32_t scale_start, scale_stop;
+    if (istop - istart == 1) {
+        scale_start = a1 - a0;
+        scale_stop = 0;
+    } else {
+        scale_start = 64 - (a0 & 63);
+        scale_stop = a1 & 63;
+    }
+    bool clipped = seg.clipped;
+    if (clipped) {
+        // the clip along (major, minor)
+        const int32_t lo = ymajor ? seg.ct : seg.cl, hi = ymajor ? seg.cb : seg.cr;
+        const int32_t mlo = ymajor ? seg.cl : seg.ct, mhi = ymajor ? seg.cr : seg.cb;
+        if (istart >= hi || istop <= lo) return;
+        if (istart < lo) {
+            fstart += slope * (lo - istart);
+            istart = lo;
+            scale_start = 64;
+            if (istop - istart == 1) {
+                scale_start = ((a1 - 1) & 63) + 1;  // contribution_64
+                scale_stop = 0;
+            }
+        }
+        if (istop > hi) {
+            istop = hi;
+            scale_stop = 0;
+        }
+        if (istart == istop) return;
+        int32_t first, last;  // minor extent estimate
+        if (slope >= 0) {
+            first = fixed_floor(fstart - 0x8000);
+            last = fixed_ceil(fstart + (istop - istart - 1) * slope + 0x8000);
+        } else {
+            last = fixed_ceil(fstart + 0x8000);
+            first = fixed_floor(fstart + (istop - istart - 1) * slope - 0x8000);
+        }
+        if (first >= mhi || last <= mlo) return;
+        if (mlo <= first && mhi >= last) clipped = false;
+    }
+    L.istart = istart;
+    L.istop = istop;
+    L.fstart = fstart;
+    L.slope = slope;
+    L.scale_start = scale_start;
+    L.scale_stop = scale_stop;
+    L.flags = (ymajor ? kHairYMajor : 0u) | (clipped ? kHairClipped : 0u);
+    if (clipped) { L.cl = seg.cl; L.ct = seg.ct; L.cr = seg.cr; L.cb = seg.cb; }
+    out.push_back(L);
+}
+
+}  // namespace
+
+void hairline_lines(const rvk_scene &sc, const rvk_path &path, Ts ts, uint32_t w, uint32_t h,
+                    std::vector<HairLine> &out) {
+    const uint8_t *verbs = sc.verbs + path.verb_offset;
+    const float *pts = sc.points + 2 * (size_t)path.point_offset;
+    size_t npts = 0;
+    for (uint32_t v = 0; v < path.verb_count; v++)
+        npts += verbs[v] == RVK_VERB_MOVE || verbs[v] == RVK_VERB_LINE ? 1 : verbs[v] == RVK_VERB_QUAD ? 2
+              : verbs[v] == RVK_VERB_CUBIC ? 3 : 0;
+    std::vector<rvks::Pt> dev(npts);
+    for (size_t i = 0; i < npts; i++) {
+        float x = pts[2 * i], y = pts[2 * i + 1];
+        ts_map(ts, x, y);  // Path::transform
+        dev[i] = rvks::Pt{x, y};
+    }
+    std::vector<rvks::HairSeg> segs;
+    rvks::anti_hairline_segments(verbs, path.verb_count, dev.data(), npts, path.stroke_cap, w, h, segs);
+    for (const rvks::HairSeg &g : segs) prepare_anti_hairline(g.x0, g.y0, g.x1, g.y1, g, out);
+}
+
+// ---------------------------------------------------------------------------
 // Shader setup (tiny-skia LinearGradient / RadialGradient / Gradient::new and the
 // gradient stage's factor/bias tables)
 // ---------------------------------------------------------------------------
@@ -517,11 +622,15 @@ void set_solid(ShaderParams &s, const float c[4]) {
 
 }  // namespace
 
-bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s) {
+bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s, float extra_opacity) {
     s = ShaderParams{};
+    // Shader::apply_opacity (Color::apply_opacity on the colour or on every stop)
+    const float xo = std::min(std::max(extra_opacity, 0.0f), 1.0f);
+    auto apply = [&](float &a) { a = std::min(std::max(a * xo, 0.0f), 1.0f); };
     if (p.kind == RVK_PAINT_COLOR) {
         float c[4] = {(float)p.color[0] / 255.0f, (float)p.color[1] / 255.0f, (float)p.color[2] / 255.0f,
                       (float)opacity_to_u8(p.opacity) / 255.0f};
+        if (xo != 1.0f) apply(c[3]);
         set_solid(s, c);
         return true;
     }
@@ -538,6 +647,7 @@ bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s
     }
     if (stops.empty()) return false;
     if (stops.size() == 1) {
+        if (xo != 1.0f) apply(stops[0].c[3]);
         set_solid(s, stops[0].c.data());
         return true;
     }
@@ -556,6 +666,7 @@ bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s
                     for (int k = 0; k < 4; k++) c[k] += wgt * (stops[i].c[k] + stops[i + 1].c[k]) * 0.5f;
                 }
             }
+            if (xo != 1.0f) apply(c[3]);
             set_solid(s, c);
             return true;
         }
@@ -583,6 +694,8 @@ bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s
     bool dummy_first = stops.front().pos != 0.0f, dummy_last = stops.back().pos != 1.0f;
     if (dummy_first) stops.insert(stops.begin(), GS{0.0f, stops.front().c});
     if (dummy_last) stops.push_back(GS{1.0f, stops.back().c});
+    if (xo != 1.0f)
+        for (auto &g : stops) apply(g.c[3]);
     bool opaque = true;
     for (auto &g : stops) opaque = opaque && g.c[3] == 1.0f;
     size_t start = dummy_first ? 0 : 1;

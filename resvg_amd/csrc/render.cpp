@@ -12,7 +12,9 @@
 // draw_pixmap into their parent.
 #include "rvk_internal.h"
 #include "host_geom.h"
+#include "hairline.h"
 #include "raster.h"
+#include "stroker.h"
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -32,7 +34,9 @@ rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs,
                            uint8_t *arena, int32_t *overflow, double algo_bytes);
 rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
                              int32_t x1, int32_t y1, const DrawItem *items, uint32_t nitems, const CovJob *jobs,
-                             const uint8_t *arena, const ShaderParams *shaders, double algo_bytes);
+                             const uint8_t *arena, const ShaderParams *shaders, const HairJob *hair_jobs,
+                             const uint32_t *hair_tiles, const uint32_t *hair_bins, const HairLine *hair_lines,
+                             double algo_bytes);
 }
 
 namespace {
@@ -57,6 +61,10 @@ struct DrawList {
     std::vector<uint2> windows;       // per job and row group: [lo, hi) of the job's edges
     uint64_t piece_slots = 0;         // piece capacity of all edges
     std::vector<ShaderParams> shaders;
+    std::vector<HairJob> hair_jobs;   // hairline strokes
+    std::vector<uint32_t> hair_tiles; // per job: bin offsets of its tiles (+1 end)
+    std::vector<uint32_t> hair_bins;  // line indices, each tile's in drawing order
+    std::vector<HairLine> hair_lines;
     uint64_t arena = 0;
     uint32_t rows = 0;
     int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN;
@@ -95,11 +103,15 @@ struct EdgeSet {
 
 struct PathGeom {
     EdgeSet fill, stroke;
+    bool hair = false;               // the stroke is an anti-aliased hairline
+    float hair_opacity = 1.0f;       // folded into its paint
+    std::vector<HairLine> hair_lines;
 };
 
-// Device-space geometry of one path: tiny-skia's fill_path preconditions and edges,
-// and the stroke outline's edges.
-void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, PathGeom &g) {
+// Device-space geometry of one path drawn into a pw x ph pixmap: tiny-skia's
+// fill_path preconditions and edges, and the stroke outline's edges or, for a stroke
+// at most one device pixel wide, its hairline segments.
+void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, uint32_t pw, uint32_t ph, PathGeom &g) {
     auto finish = [&](std::vector<Seg> &segs, EdgeSet &es) {
         if (segs.empty() || bounds_nearly_zero(segs)) return;
         float mnx = INFINITY, mxx = -INFINITY;
@@ -118,6 +130,13 @@ void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, PathGeom &g) {
         finish(segs, g.fill);
     }
     if (p.stroke.kind != RVK_PAINT_NONE && p.stroke_width >= 0.0f) {
+        float cov = 1.0f;
+        if (rvks::treat_as_hairline(p.stroke_width, p.anti_alias != 0, ts.sx, ts.ky, ts.kx, ts.sy, cov)) {
+            g.hair = true;
+            g.hair_opacity = rvks::hairline_paint_opacity(cov);
+            hairline_lines(sc, p, ts, pw, ph, g.hair_lines);
+            return;
+        }
         std::vector<Seg> segs;
         stroke_segments(sc, p, ts, segs);
         finish(segs, g.stroke);
@@ -169,6 +188,7 @@ struct Renderer {
     struct Task {
         const rvk_path *path;
         Ts ts;
+        uint32_t pw, ph;  // size of the pixmap drawn into
     };
     std::vector<Task> tasks;
     std::vector<PathGeom> geoms;
@@ -180,7 +200,10 @@ struct Renderer {
         if (k >= tasks.size()) return PathGeom{};
         if (k >= ready_end) {
             const size_t end = std::min(tasks.size(), k + kChunk);
-            parallel_for(end - k, [&](size_t i) { compute_geom(sc, *tasks[k + i].path, tasks[k + i].ts, geoms[k + i]); });
+            parallel_for(end - k, [&](size_t i) {
+                const Task &t = tasks[k + i];
+                compute_geom(sc, *t.path, t.ts, t.pw, t.ph, geoms[k + i]);
+            });
             ready_end = end;
         }
         return std::move(geoms[k]);
@@ -253,14 +276,77 @@ struct Renderer {
         return (int32_t)dl.jobs.size() - 1;
     }
 
+    // Adds a hairline job for prepared lines, binned into tiles of the part of their
+    // pixel bounds inside `clip`.  Returns the job index or -1 when nothing is drawn.
+    int32_t add_hair_job(DrawList &dl, const std::vector<HairLine> &lines, IRect clip) {
+        if (lines.empty()) return -1;
+        struct Box { int32_t x0, y0, x1, y1; };
+        std::vector<Box> boxes(lines.size());
+        int32_t jx0 = INT32_MAX, jy0 = INT32_MAX, jx1 = INT32_MIN, jy1 = INT32_MIN;
+        for (size_t i = 0; i < lines.size(); i++) {
+            const HairLine &L = lines[i];
+            // minor pixels touched: f >> 16 and the one before, f at both ends
+            const int64_t fa = (int64_t)L.fstart + 0x8000;
+            const int64_t fb = fa + (int64_t)(L.istop - L.istart - 1) * L.slope;
+            const int32_t m0 = (int32_t)(std::min(fa, fb) >> 16) - 1, m1 = (int32_t)(std::max(fa, fb) >> 16) + 1;
+            Box b = (L.flags & kHairYMajor) ? Box{m0, L.istart, m1, L.istop} : Box{L.istart, m0, L.istop, m1};
+            if (L.flags & kHairClipped) {
+                b.x0 = std::max(b.x0, L.cl); b.y0 = std::max(b.y0, L.ct);
+                b.x1 = std::min(b.x1, L.cr); b.y1 = std::min(b.y1, L.cb);
+            }
+            b.x0 = std::max(b.x0, clip.x); b.y0 = std::max(b.y0, clip.y);
+            b.x1 = std::min(b.x1, clip.x + clip.w); b.y1 = std::min(b.y1, clip.y + clip.h);
+            boxes[i] = b;
+            if (b.x0 >= b.x1 || b.y0 >= b.y1) continue;
+            jx0 = std::min(jx0, b.x0); jy0 = std::min(jy0, b.y0);
+            jx1 = std::max(jx1, b.x1); jy1 = std::max(jy1, b.y1);
+        }
+        if (jx0 >= jx1 || jy0 >= jy1) return -1;
+        HairJob j{};
+        j.x0 = jx0; j.y0 = jy0; j.w = jx1 - jx0; j.h = jy1 - jy0;
+        j.tiles_x = (uint32_t)((j.w + kHairTile - 1) / kHairTile);
+        j.tiles_y = (uint32_t)((j.h + kHairTile - 1) / kHairTile);
+        const size_t nt = (size_t)j.tiles_x * j.tiles_y;
+        std::vector<uint32_t> count(nt + 1, 0);
+        auto tiles_of = [&](const Box &b, auto &&fn) {
+            const int32_t tx0 = (b.x0 - j.x0) / kHairTile, tx1 = (b.x1 - 1 - j.x0) / kHairTile;
+            const int32_t ty0 = (b.y0 - j.y0) / kHairTile, ty1 = (b.y1 - 1 - j.y0) / kHairTile;
+            for (int32_t ty = ty0; ty <= ty1; ty++)
+                for (int32_t tx = tx0; tx <= tx1; tx++) fn((size_t)ty * j.tiles_x + tx);
+        };
+        for (const Box &b : boxes)
+            if (b.x0 < b.x1 && b.y0 < b.y1) tiles_of(b, [&](size_t t) { count[t]++; });
+        j.tile_offset = (uint32_t)dl.hair_tiles.size();
+        const uint32_t base = (uint32_t)dl.hair_bins.size();
+        uint32_t run = base;
+        for (size_t t = 0; t < nt; t++) {
+            dl.hair_tiles.push_back(run);
+            const uint32_t c = count[t];
+            count[t] = run;  // becomes the fill cursor
+            run += c;
+        }
+        dl.hair_tiles.push_back(run);
+        dl.hair_bins.resize(run);
+        const uint32_t line_base = (uint32_t)dl.hair_lines.size();
+        for (size_t i = 0; i < lines.size(); i++) {
+            const Box &b = boxes[i];
+            if (b.x0 < b.x1 && b.y0 < b.y1)
+                tiles_of(b, [&](size_t t) { dl.hair_bins[count[t]++] = line_base + (uint32_t)i; });
+        }
+        dl.hair_lines.insert(dl.hair_lines.end(), lines.begin(), lines.end());
+        dl.hair_jobs.push_back(j);
+        return (int32_t)dl.hair_jobs.size() - 1;
+    }
+
     // fill / stroke coverage jobs + shaders of one path drawn with `ts` into a
-    // pixmap whose visible pixels are `clip`.
-    void path_jobs(DrawList &dl, const rvk_path &p, Ts ts, IRect clip, DrawItem &it) {
+    // pw x ph pixmap whose visible pixels are `clip`.
+    void path_jobs(DrawList &dl, const rvk_path &p, Ts ts, uint32_t pw, uint32_t ph, IRect clip, DrawItem &it) {
         it.fill_job = it.stroke_job = -1;
         it.fill_shader = it.stroke_shader = -1;
+        it.stroke_hair = 0;
         it.stroke_first = p.paint_order_stroke_first ? 1 : 0;
         if (mode == kCollect) {
-            tasks.push_back(Task{&p, ts});
+            tasks.push_back(Task{&p, ts, pw, ph});
             return;
         }
         if (mode == kSkip) {
@@ -278,8 +364,18 @@ struct Renderer {
                 dl.shaders.push_back(sp);
             }
         }
-        // path::stroke_path (path.rs:77-116): tiny-skia's stroker outline, filled nonzero
-        if (g.stroke.ok && make_shader(sc, p.stroke, ts, sp)) {
+        // path::stroke_path (path.rs:77-116): a hairline, or tiny-skia's stroker outline
+        // filled nonzero
+        if (g.hair) {
+            if (make_shader(sc, p.stroke, ts, sp, g.hair_opacity)) {
+                it.stroke_job = add_hair_job(dl, g.hair_lines, clip);
+                if (it.stroke_job >= 0) {
+                    it.stroke_hair = 1;
+                    it.stroke_shader = (int32_t)dl.shaders.size();
+                    dl.shaders.push_back(sp);
+                }
+            }
+        } else if (g.stroke.ok && make_shader(sc, p.stroke, ts, sp)) {
             it.stroke_job = add_job(dl, g.stroke, RVK_FILL_NONZERO, p.anti_alias != 0, clip);
             if (it.stroke_job >= 0) {
                 it.stroke_shader = (int32_t)dl.shaders.size();
@@ -292,11 +388,19 @@ struct Renderer {
         if (it.fill_job < 0 && it.stroke_job < 0) return;
         // the item's target rect: union of its jobs' rects moved to target space
         int32_t x0 = INT32_MAX, y0 = INT32_MAX, x1 = INT32_MIN, y1 = INT32_MIN;
-        for (int32_t j : {it.fill_job, it.stroke_job}) {
+        for (int k = 0; k < 2; k++) {
+            const int32_t j = k == 0 ? it.fill_job : it.stroke_job;
             if (j < 0) continue;
-            const CovJob &jb = dl.jobs[j];
-            x0 = std::min(x0, jb.x0 + it.lx); y0 = std::min(y0, jb.y0 + it.ly);
-            x1 = std::max(x1, jb.x0 + jb.w + it.lx); y1 = std::max(y1, jb.y0 + jb.h + it.ly);
+            int32_t jx, jy, jw, jh;
+            if (k == 1 && it.stroke_hair) {
+                const HairJob &hj = dl.hair_jobs[j];
+                jx = hj.x0; jy = hj.y0; jw = hj.w; jh = hj.h;
+            } else {
+                const CovJob &jb = dl.jobs[j];
+                jx = jb.x0; jy = jb.y0; jw = jb.w; jh = jb.h;
+            }
+            x0 = std::min(x0, jx + it.lx); y0 = std::min(y0, jy + it.ly);
+            x1 = std::max(x1, jx + jw + it.lx); y1 = std::max(y1, jy + jh + it.ly);
         }
         it.x0 = std::max(x0, 0); it.y0 = std::max(y0, 0);
         it.x1 = std::min(x1, (int32_t)dl.w); it.y1 = std::min(y1, (int32_t)dl.h);
@@ -323,7 +427,11 @@ struct Renderer {
         const size_t o_win = al(o_ecov + dl.ecov.size() * sizeof(EdgeCov));
         const size_t o_sh = al(o_win + dl.windows.size() * sizeof(uint2));
         const size_t o_it = al(o_sh + dl.shaders.size() * sizeof(ShaderParams));
-        const size_t meta = al(o_it + dl.items.size() * sizeof(DrawItem));
+        const size_t o_hj = al(o_it + dl.items.size() * sizeof(DrawItem));
+        const size_t o_ht = al(o_hj + dl.hair_jobs.size() * sizeof(HairJob));
+        const size_t o_hb = al(o_ht + dl.hair_tiles.size() * 4);
+        const size_t o_hl = al(o_hb + dl.hair_bins.size() * 4);
+        const size_t meta = al(o_hl + dl.hair_lines.size() * sizeof(HairLine));
         const size_t o_np = meta;
         const size_t o_pc = al(o_np + dl.edges.size() * 4);
         const size_t o_arena = al(o_pc + (size_t)dl.piece_slots * sizeof(PieceDev));
@@ -338,6 +446,10 @@ struct Renderer {
         put(o_win, dl.windows.data(), dl.windows.size() * sizeof(uint2));
         put(o_sh, dl.shaders.data(), dl.shaders.size() * sizeof(ShaderParams));
         put(o_it, dl.items.data(), dl.items.size() * sizeof(DrawItem));
+        put(o_hj, dl.hair_jobs.data(), dl.hair_jobs.size() * sizeof(HairJob));
+        put(o_ht, dl.hair_tiles.data(), dl.hair_tiles.size() * 4);
+        put(o_hb, dl.hair_bins.data(), dl.hair_bins.size() * 4);
+        put(o_hl, dl.hair_lines.data(), dl.hair_lines.size() * sizeof(HairLine));
         void *dev = nullptr;
         RVK_TRY(rvk::scratch_alloc(ctx, o_arena + dl.arena, &dev));
         uint8_t *d = (uint8_t *)dev;
@@ -351,7 +463,8 @@ struct Renderer {
         const double exp_bytes = (double)dl.edges.size() * (sizeof(EdgeRec) + 4) + piece_bytes;
         const double cov_bytes = cov_px + (double)dl.ecov.size() * sizeof(EdgeCov) +
                                  (double)dl.windows.size() * sizeof(uint2) + piece_bytes;
-        const double draw_bytes = 8.0 * (double)(dl.bx1 - dl.bx0) * (double)(dl.by1 - dl.by0) + cov_px;
+        const double draw_bytes = 8.0 * (double)(dl.bx1 - dl.bx0) * (double)(dl.by1 - dl.by0) + cov_px +
+                                  (double)(dl.hair_lines.size() * sizeof(HairLine) + dl.hair_bins.size() * 4);
         ctx->stats.edge_pieces += dl.piece_slots;
         ctx->stats.band_entries += dl.windows.size();
         ctx->stats.upload_bytes += meta;
@@ -369,7 +482,8 @@ struct Renderer {
             st = rvk::launch_draw_items(ctx, dl.px, dl.w, dl.h, dl.bx0, dl.by0, dl.bx1, dl.by1,
                                         (const DrawItem *)(d + o_it), (uint32_t)dl.items.size(),
                                         (const CovJob *)(d + o_jobs), d + o_arena, (const ShaderParams *)(d + o_sh),
-                                        draw_bytes);
+                                        (const HairJob *)(d + o_hj), (const uint32_t *)(d + o_ht),
+                                        (const uint32_t *)(d + o_hb), (const HairLine *)(d + o_hl), draw_bytes);
         rvk::scratch_release(ctx, dev);
         ctx->stats.flushes++;
         dl = DrawList{dl.px, dl.w, dl.h};
@@ -386,7 +500,7 @@ struct Renderer {
         it.layer = 0;
         it.opacity = 1.0f;
         it.blend_mode = RVK_BLEND_SOURCE_OVER;
-        path_jobs(dl, p, ts, IRect{0, 0, (int32_t)dl.w, (int32_t)dl.h}, it);
+        path_jobs(dl, p, ts, dl.w, dl.h, IRect{0, 0, (int32_t)dl.w, (int32_t)dl.h}, it);
         push_item(dl, it);
     }
 
@@ -454,7 +568,7 @@ struct Renderer {
             int32_t cx0 = std::max(0, -ib.x), cy0 = std::max(0, -ib.y);
             int32_t cx1 = std::min(ib.w, (int32_t)dl.w - ib.x), cy1 = std::min(ib.h, (int32_t)dl.h - ib.y);
             if (cx1 <= cx0 || cy1 <= cy0) return;
-            path_jobs(dl, p, lts, IRect{cx0, cy0, cx1 - cx0, cy1 - cy0}, it);
+            path_jobs(dl, p, lts, (uint32_t)ib.w, (uint32_t)ib.h, IRect{cx0, cy0, cx1 - cx0, cy1 - cy0}, it);
             push_item(dl, it);
             return;
         }

@@ -236,7 +236,7 @@ int solve_cubic_poly(const float coeff[4], float t[3]) {
         return collapse_duplicates(t, 3);
     }
     float aa = std::fabs(r) + std::sqrt(r2_minus_q3);
-    aa = std::cbrt(aa);
+    aa = std::pow(aa, 0.3333333f);  // scalar_cube_root
     if (r > 0.0f) aa = -aa;
     if (aa != 0.0f) aa += q / aa;
     t[0] = pin01(aa - adiv3);
@@ -1256,6 +1256,13 @@ void PathBuilder::push_circle(float x, float y, float r) {
 }
 
 // ---------------------------------------------------------------------------
+int cubic_max_curvature(const Pt src[4], float t[3]) { return find_cubic_max_curvature(src, t); }
+void chop_cubic_at(const Pt src[4], float t, Pt dst[7]) { chop_cubic_at2(src, t, dst); }
+bool unit_divide(float numer, float denom, float &r) { return valid_unit_divide(numer, denom, r); }
+void transform_points(float sx, float ky, float kx, float sy, float tx, float ty, Pt *p, size_t n) {
+    map_points(Ts{sx, ky, kx, sy, tx, ty}, p, n);
+}
+
 float resolution_scale(float sx, float ky, float kx, float sy) {
     auto length = [](float x, float y) {
         const float m2 = x * x + y * y;

@@ -72,4 +72,12 @@ bool stroke(const uint8_t *verbs, size_t nverbs, const float *points, size_t npo
 // is at most one device pixel wide under the 2x2 part of the transform.
 bool treat_as_hairline(float width, bool anti_alias, float sx, float ky, float kx, float sy, float &coverage);
 
+// path_geometry helpers shared with the hairline code: the t values (clamped to
+// [0, 1], sorted, distinct) where F'(t) . F''(t) = 0; de Casteljau split at t;
+// valid_unit_divide; Transform::map_points.
+int cubic_max_curvature(const Pt src[4], float t[3]);
+void chop_cubic_at(const Pt src[4], float t, Pt dst[7]);
+bool unit_divide(float numer, float denom, float &r);
+void transform_points(float sx, float ky, float kx, float sy, float tx, float ty, Pt *p, size_t n);
+
 }  // namespace rvks

@@ -24,6 +24,7 @@ NAMES = [
     "feComponentTransfer/type=linear-on-blue", "feComponentTransfer/type=discrete-on-blue",
     "feTurbulence/type=fractalNoise", "feTurbulence/no-attributes", "feTurbulence/seed=20",
     "feTurbulence/stitchTiles=stitch", "feTurbulence/numOctaves=5", "feTurbulence/baseFrequency=0.05-0.01",
+    "filter/subregion-and-primitiveUnits=objectBoundingBox-1", "filter/subregion-and-primitiveUnits=objectBoundingBox-2",
 ]
 # fixtures outside filters/ (paths relative to crates/resvg/tests/tests)
 OTHER = [

@@ -12,6 +12,7 @@ import math
 import numpy as np
 
 from resvg_amd import abi
+from resvg_amd.filters import FilterGraph
 from resvg_amd.scene import PaintSpec, Scene, rect_path
 
 from fixture_scenes import fixtures as filter_fixtures
@@ -343,6 +344,28 @@ def _stroke_fixtures():
     return out
 
 
+def _hairline_fixtures():
+    """filters/filter/subregion-and-primitiveUnits=objectBoundingBox-1/-2: a blurred rect
+    cut to a primitive subregion, under a 0.5-wide grey crosshair (0.75 px: hairlines).
+    primitiveUnits=objectBoundingBox resolves the blur's width/height (0.5 or 50%) as
+    the filter region scaled by the subregion box (usvg parser/filter.rs:423-432):
+    (4, 4, 192, 192) -> (2, 2, 96, 96); stdDeviation 0.01 x bbox 160 = 1.6."""
+    out = {}
+    M, L = abi.VERB_MOVE, abi.VERB_LINE
+    for k in (1, 2):
+        sc = Scene(200, 200)
+        g = FilterGraph((4.0, 4.0, 192.0, 192.0))
+        g.gaussian_blur(1.6, rect=(2.0, 2.0, 96.0, 96.0))
+        grp = sc.group(sc.root, filters=[g])
+        v, p = rect_path(20.0, 20.0, 160.0, 160.0)
+        sc.path(grp, v, p, fill=PaintSpec.solid((46, 139, 87)))
+        # the crosshair keeps the default black fill (two zero-area contours)
+        sc.path(sc.root, [M, L, M, L], [(20, 100), (180, 100), (100, 20), (100, 180)],
+                fill=PaintSpec.solid(BLACK), stroke=PaintSpec.solid((128, 128, 128)), stroke_width=0.5)
+        out[f"filter/subregion-and-primitiveUnits=objectBoundingBox-{k}"] = sc
+    return out
+
+
 def filter_scene(fx) -> Scene:
     """A filter fixture (fixture_scenes.Fixture) as a scene: a filtered group with a rect."""
     sc = Scene(200, 200)
@@ -354,6 +377,7 @@ def filter_scene(fx) -> Scene:
 
 def all_scene_fixtures():
     out = dict(shape_fixtures())
+    out.update(_hairline_fixtures())
     for fx in filter_fixtures():
         out[fx.name] = filter_scene(fx)
     return out

@@ -41,16 +41,11 @@ def _assert_same(got, exp, what):
                              f"first at (y={ys[0]}, x={xs[0]}) got {got[ys[0], xs[0]]} exp {exp[ys[0], xs[0]]}")
 
 
-# strokes thinner than a device pixel use tiny-skia's hairline blitter, not restated
-HAIRLINE = {"painting/stroke/control-points-clamping-1"}
-
-
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_scene_fixture(ctx, name):
     got = _gpu(ctx, SCENES[name], render_ts(), 300, 300)
     _assert_same(got, _oracle(SCENES[name], render_ts(), 300, 300), name)
-    if name not in HAIRLINE:
-        assert compare_png(got, load_png(name)) == 0
+    assert compare_png(got, load_png(name)) == 0
 
 
 # ---------------------------------------------------------------------------
@@ -132,6 +127,41 @@ def test_paths_beyond_the_pixmap(ctx, seed):
     sc = _random_scene(200 + seed, 128, 10, margin=0.5)
     ts = abi.Transform.identity()
     _assert_same(_gpu(ctx, sc, ts, 128, 128), _oracle(sc, ts, 128, 128), f"margins seed {seed}")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_hairlines(ctx, seed):
+    # strokes 0.2-1 px wide: tiny-skia's anti-aliased hairlines, some in elided opacity
+    # layers, some with round/square caps, some running off the pixmap
+    rng = np.random.default_rng(500 + seed)
+    size = 192
+    sc = Scene(size, size)
+    for i in range(14):
+        parent = sc.root
+        if rng.random() < 0.4:
+            parent = sc.group(sc.root, opacity=float(np.float32(rng.uniform(0.3, 1.0))))
+        verbs, pts = _random_path(rng, size, int(rng.integers(1, 10)), closed=rng.random() < 0.5,
+                                  margin=0.3 if i % 3 == 0 else 0.0)
+        sc.path(parent, verbs, pts, fill=_paint(rng, size) if rng.random() < 0.3 else None,
+                stroke=_paint(rng, size), stroke_width=float(np.float32(rng.uniform(0.2, 1.0))),
+                stroke_cap=[abi.CAP_BUTT, abi.CAP_ROUND, abi.CAP_SQUARE][int(rng.integers(3))],
+                stroke_first=rng.random() < 0.3)
+    ts = abi.Transform(1.0, 0.0, 0.0, 1.0, 0.5, -0.25) if seed % 2 else abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, size, size), _oracle(sc, ts, size, size), f"hairlines seed {seed}")
+
+
+def test_hairline_many_lines_per_tile(ctx):
+    # a long zig-zag through one small region: dozens of lines share each bin tile and
+    # cross each other, so the per-pixel blit order matters
+    sc = Scene(96, 96)
+    rng = np.random.default_rng(9)
+    pts = rng.uniform(30, 66, (80, 2)).astype(np.float32)
+    verbs = [abi.VERB_MOVE] + [abi.VERB_LINE] * 79
+    sc.path(sc.root, verbs, pts, stroke=PaintSpec.solid((200, 40, 90), opacity=0.6), stroke_width=0.8)
+    sc.path(sc.root, verbs, pts[::-1].copy(), stroke=PaintSpec.linear(30.0, 30.0, 66.0, 66.0, _stops(rng, 3)),
+            stroke_width=1.0)
+    ts = abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, 96, 96), _oracle(sc, ts, 96, 96), "dense hairlines")
 
 
 def test_composites_onto_existing_pixels(ctx):

@@ -13,16 +13,8 @@ from scene_fixtures import all_scene_fixtures, render_ts
 SCENES = all_scene_fixtures()
 
 
-# Fixtures that exercise a tiny-skia path this build does not restate yet: strokes at
-# most one device pixel wide are drawn by tiny-skia's anti-aliased hairline blitter,
-# here by the stroker (the 0.5-wide path of this fixture; every other pixel matches).
-HAIRLINE = {"painting/stroke/control-points-clamping-1"}
-
-
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_scene_fixture_png(name):
-    if name in HAIRLINE:
-        pytest.xfail("anti-aliased hairline strokes are not restated")
     px = np.zeros((300, 300, 4), np.uint8)
     orc.render(SCENES[name].to_abi().abi, render_ts(), px)
     assert compare_png(px, load_png(name)) == 0
@@ -82,3 +74,57 @@ def test_stroke_outline_covers_line_width():
             stroke_width=4.0)
     cov = _cov(sc, 40, 40, stroke=True)
     assert (cov[18:22, 5:35] == 255).all() and not cov[:18].any() and not cov[22:].any()
+
+
+# ---------------------------------------------------------------------------
+# hairlines (strokes at most one device pixel wide): closed forms of tiny-skia's
+# anti-aliased hairline blitter
+# ---------------------------------------------------------------------------
+def _hair_alpha(verbs, pts, w=64, h=48, width=1.0, cap=abi.CAP_BUTT):
+    """Alpha of an opaque black hairline on a transparent pixmap (Source lerp: each
+    blit's coverage lands in alpha unchanged)."""
+    sc = Scene(w, h)
+    sc.path(sc.root, verbs, pts, stroke=PaintSpec.solid((0, 0, 0)), stroke_width=width, stroke_cap=cap)
+    px = np.zeros((h, w, 4), np.uint8)
+    orc.render(sc.to_abi().abi, abi.Transform.identity(), px)
+    return px[..., 3].astype(np.int32)
+
+
+def test_hairline_on_half_pixel_row_fills_one_row():
+    # y = 20.5: the minor position lands on a pixel boundary, so the whole blit goes to
+    # row 20 with coverage 255 and its neighbour gets 0; integer end points give full caps
+    a = _hair_alpha([abi.VERB_MOVE, abi.VERB_LINE], [(10.0, 20.5), (50.0, 20.5)])
+    exp = np.zeros_like(a)
+    exp[20, 10:50] = 255
+    assert np.array_equal(a, exp)
+    a = _hair_alpha([abi.VERB_MOVE, abi.VERB_LINE], [(7.5, 5.0), (7.5, 30.0)])
+    exp = np.zeros_like(a)
+    exp[5:30, 7] = 255
+    assert np.array_equal(a, exp)
+
+
+def test_hairline_column_pairs_sum_to_full_coverage():
+    # away from the caps every column of an x-major line gets a and 255 - a
+    a = _hair_alpha([abi.VERB_MOVE, abi.VERB_LINE], [(4.25, 10.3), (60.75, 31.7)])
+    cols = a[:, 6:58].sum(axis=0)
+    assert (cols == 255).all()
+    assert ((a[:, 6:58] > 0).sum(axis=0) <= 2).all()
+    # y-major: rows
+    a = _hair_alpha([abi.VERB_MOVE, abi.VERB_LINE], [(10.3, 2.2), (30.9, 45.6)])
+    rows = a[4:44, :].sum(axis=1)
+    assert (rows == 255).all()
+
+
+def test_hairline_cap_scales_partial_end_pixels():
+    # x from 10.25 to 20.75 on a half-pixel row: the first pixel is covered for 48/64
+    # of its width and the last for 48/64; (255 * 48) >> 6 = 191
+    a = _hair_alpha([abi.VERB_MOVE, abi.VERB_LINE], [(10.25, 8.5), (20.75, 8.5)])
+    assert a[8, 10] == 191 and a[8, 20] == 191
+    assert (a[8, 11:20] == 255).all()
+    assert a.sum() == 191 * 2 + 255 * 9
+
+
+def test_thin_stroke_coverage_scales_the_paint():
+    # width 0.5 under identity: coverage 0.5 -> paint opacity (255 * 128 >> 8) / 255
+    a = _hair_alpha([abi.VERB_MOVE, abi.VERB_LINE], [(10.0, 20.5), (50.0, 20.5)], width=0.5)
+    assert (a[20, 10:50] == 127).all() and a.sum() == 127 * 40
