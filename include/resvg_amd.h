@@ -222,13 +222,30 @@ typedef struct rvk_group {                 /* usvg::Group (mod.rs:1035-1225) */
     float opacity;
     uint32_t blend_mode;                   /* RVK_MIX_* */
     uint32_t isolate;
-    int32_t clip_path, mask;               /* -1: none (clip paths and masks are not in this scene format) */
+    int32_t clip_path, mask;               /* into rvk_scene.clip_paths / masks; -1: none */
     uint32_t filter_offset, filter_count;
     rvk_rect layer_bbox;                   /* layer_bounding_box(), object coordinates */
     uint32_t child_offset, child_count;    /* into rvk_scene.children */
 } rvk_group;
 
 typedef struct rvk_node_ref { uint32_t kind, index; } rvk_node_ref;
+
+enum { RVK_MASK_LUMINANCE = 0, RVK_MASK_ALPHA };
+
+typedef struct rvk_clip_path {             /* usvg::ClipPath (mod.rs:793-840) */
+    rvk_transform transform;               /* ClipPath::transform (objectBoundingBox units folded in) */
+    int32_t clip_path;                     /* the clip path's own clip-path; -1: none */
+    uint32_t root;                         /* group holding its children (drawn with their
+                                              transforms only: fills of paths, nested groups) */
+} rvk_clip_path;
+
+typedef struct rvk_mask {                  /* usvg::Mask (mod.rs:857-910) */
+    rvk_rect rect;                         /* Mask::rect, user space */
+    uint32_t kind;                         /* RVK_MASK_* */
+    int32_t mask;                          /* the mask's own mask; -1: none */
+    uint32_t root;                         /* group holding its content (rendered normally;
+                                              the group's own transform is not applied) */
+} rvk_mask;
 
 typedef struct rvk_scene {
     const rvk_group *groups;    uint32_t group_count;   /* groups[0] is the root */
@@ -239,6 +256,8 @@ typedef struct rvk_scene {
     const rvk_stop *stops;
     const rvk_filter *filters;
     rvk_filter_data filter_data;
+    const rvk_clip_path *clip_paths;  uint32_t clip_path_count;
+    const rvk_mask *masks;            uint32_t mask_count;
 } rvk_scene;
 
 /* ------------------------------------------------------------------------- */

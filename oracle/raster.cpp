@@ -1118,6 +1118,39 @@ bool stroke_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t 
     return true;
 }
 
+// resvg path::fill_path with BlendMode::Clear (clip path children, clip.rs:35-56):
+// lowp Clear with coverage lerps every channel toward zero, div255(d * (255 - c)).
+bool clear_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h) {
+    if (path.fill.kind == RVK_PAINT_NONE) return false;
+    if (path.bounds.w == 0.0f || path.bounds.h == 0.0f) return false;
+    Shader s = make_shader(sc, path.fill, ts);
+    if (!s.valid) return false;
+    std::vector<Seg> segs;
+    fill_segments(sc, path, ts, segs);
+    if (segs.empty() || path_bounds_nearly_zero(segs)) return false;
+    std::vector<uint8_t> cov;
+    coverage(segs, path.fill_rule, path.anti_alias != 0, w, h, cov);
+    for (size_t i = 0; i < cov.size(); i++) {
+        const uint32_t c = cov[i];
+        if (!c) continue;
+        uint8_t *d = px + 4 * i;
+        for (int k = 0; k < 4; k++) d[k] = (uint8_t)div255(d[k] * (255 - c));
+    }
+    return true;
+}
+
+// tiny-skia Mask::fill_path(PathBuilder::from_rect(rect), Winding, anti_alias = true,
+// ts) into a zeroed mask (mask.rs:20-30): the rect's coverage per pixel.
+void rect_mask(const rvk_rect &r, Ts ts, uint32_t w, uint32_t h, std::vector<uint8_t> &mask) {
+    mask.assign((size_t)w * h, 0);
+    const float l = r.x, t = r.y, rr = r.x + r.w, b = r.y + r.h;
+    Pt p[4] = {{l, t}, {rr, t}, {rr, b}, {l, b}};
+    for (auto &q : p) ts_map(ts, q.x, q.y);
+    std::vector<Seg> segs;
+    for (int i = 0; i < 4; i++) segs.push_back(Seg{2, {p[i], p[(i + 1) % 4]}});
+    if (path_bounds_nearly_zero(segs)) return;
+    coverage(segs, RVK_FILL_NONZERO, true, w, h, mask);
+}
 void path_coverage_of(const rvk_scene &sc, const rvk_path &path, Ts ts, bool stroke, uint8_t *cov_out, uint32_t w,
                       uint32_t h) {
     std::vector<Seg> segs;

@@ -3,6 +3,8 @@
 #include "oracle.h"
 #include "common.h"
 #include "geom.h"
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 using namespace orc;
@@ -10,6 +12,8 @@ using namespace orc;
 namespace orc {
 bool fill_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h);
 bool stroke_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h);
+bool clear_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h);
+void rect_mask(const rvk_rect &r, Ts ts, uint32_t w, uint32_t h, std::vector<uint8_t> &mask);
 }
 
 namespace {
@@ -40,6 +44,92 @@ void render_path(const rvk_scene &sc, const rvk_path &p, Ts ts, uint8_t *px, uin
         fill_path_into(sc, p, ts, px, w, h);
     }
 }
+
+// ---------------------------------------------------------------------------
+// clip paths (clip.rs) and masks (mask.rs)
+// ---------------------------------------------------------------------------
+constexpr int kMaxNesting = 64;  // clip/mask chains deeper than this are malformed input
+
+inline uint32_t div255(uint32_t v) { return (v + 255) >> 8; }
+
+// Pixmap::apply_mask: lowp LoadMaskU8 + DestinationIn on every channel
+void apply_mask(uint8_t *px, const std::vector<uint8_t> &m) {
+    for (size_t i = 0; i < m.size(); i++)
+        for (int k = 0; k < 4; k++) px[4 * i + k] = (uint8_t)div255(px[4 * i + k] * m[i]);
+}
+
+// Mask::from_pixmap (tiny-skia mask.rs): the alpha channel, or the luminance of the
+// demultiplied colour (Rec. 709 coefficients 0.2126, 0.7152, 0.0722) times alpha,
+// scaled to 255 and rounded up; pinned by the masking/mask fixtures (only this form
+// reproduces masking/mask/simple-case exactly)
+void mask_from_pixmap(const std::vector<uint8_t> &px, uint32_t kind, std::vector<uint8_t> &m) {
+    m.resize(px.size() / 4);
+    for (size_t i = 0; i < m.size(); i++) {
+        const uint8_t *p = &px[4 * i];
+        if (kind == RVK_MASK_ALPHA) {
+            m[i] = p[3];
+            continue;
+        }
+        float r = (float)p[0] / 255.0f, g = (float)p[1] / 255.0f, b = (float)p[2] / 255.0f;
+        const float a = (float)p[3] / 255.0f;
+        if (p[3] != 0) {
+            r /= a;
+            g /= a;
+            b /= a;
+        }
+        const float luma = r * 0.2126f + g * 0.7152f + b * 0.0722f;
+        float v = (luma * a) * 255.0f;
+        v = std::min(std::max(v, 0.0f), 255.0f);
+        m[i] = (uint8_t)std::ceil(v);
+    }
+}
+
+void clip_apply(const rvk_scene &sc, int32_t idx, Ts ts, uint8_t *px, uint32_t w, uint32_t h, int depth);
+
+// clip::draw_children (clip.rs:30-70): fills of the paths, groups with their
+// transforms (a group with its own clip-path goes through clip_group)
+void clip_draw_children(const rvk_scene &sc, const rvk_group &parent, bool clear, Ts ts, uint8_t *px, uint32_t w,
+                        uint32_t h, int depth) {
+    for (uint32_t i = 0; i < parent.child_count; i++) {
+        const rvk_node_ref &n = sc.children[parent.child_offset + i];
+        if (n.kind == RVK_NODE_PATH) {
+            const rvk_path &p = sc.paths[n.index];
+            if (!p.visible) continue;
+            if (clear) clear_path_into(sc, p, ts, px, w, h);
+            else fill_path_into(sc, p, ts, px, w, h);
+        } else {
+            const rvk_group &g = sc.groups[n.index];
+            const Ts gts = ts_concat(ts, ts_from(g.transform));
+            if (g.clip_path >= 0) {
+                // clip_group (clip.rs:72-98): the children on their own canvas, clipped,
+                // then Xor-ed into the clip pixmap
+                std::vector<uint8_t> tmp((size_t)w * h * 4, 0);
+                clip_draw_children(sc, g, false, gts, tmp.data(), w, h, depth + 1);
+                clip_apply(sc, g.clip_path, gts, tmp.data(), w, h, depth + 1);
+                orc_draw_pixmap(px, w, h, 0, 0, tmp.data(), w, h, 1.0f, RVK_BLEND_XOR);
+            } else {
+                clip_draw_children(sc, g, clear, gts, px, w, h, depth + 1);
+            }
+        }
+    }
+}
+
+// clip::apply (clip.rs:6-28)
+void clip_apply(const rvk_scene &sc, int32_t idx, Ts ts, uint8_t *px, uint32_t w, uint32_t h, int depth) {
+    if (idx < 0 || (uint32_t)idx >= sc.clip_path_count || depth > kMaxNesting) return;
+    const rvk_clip_path &cp = sc.clip_paths[idx];
+    std::vector<uint8_t> clip((size_t)w * h * 4, 0);
+    for (size_t i = 0; i < (size_t)w * h; i++) clip[4 * i + 3] = 255;  // Color::BLACK
+    clip_draw_children(sc, sc.groups[cp.root], true, ts_concat(ts, ts_from(cp.transform)), clip.data(), w, h, depth);
+    if (cp.clip_path >= 0) clip_apply(sc, cp.clip_path, ts, px, w, h, depth + 1);
+    std::vector<uint8_t> m;
+    mask_from_pixmap(clip, RVK_MASK_ALPHA, m);
+    for (auto &v : m) v = (uint8_t)(255 - v);  // Mask::invert
+    apply_mask(px, m);
+}
+
+void mask_apply(const rvk_scene &sc, const Ctx &ctx, int32_t idx, Ts ts, uint8_t *px, uint32_t w, uint32_t h,
+                int depth);
 
 bool int_rect_from_xywh(int64_t x, int64_t y, int64_t w, int64_t h, IRect &out) {
     if (w <= 0 || h <= 0 || x < INT32_MIN || y < INT32_MIN || x + w > INT32_MAX || y + h > INT32_MAX ||
@@ -86,7 +176,30 @@ void render_group(const rvk_scene &sc, const rvk_group &g, const Ctx &ctx, Ts ts
     for (uint32_t i = 0; i < g.filter_count; i++)
         orc_filter_apply(&sc.filters[g.filter_offset + i], &sc.filter_data,
                          rvk_transform{lts.sx, lts.ky, lts.kx, lts.sy, lts.tx, lts.ty}, layer.data(), ib.w, ib.h);
+    if (g.clip_path >= 0) clip_apply(sc, g.clip_path, lts, layer.data(), ib.w, ib.h, 0);
+    if (g.mask >= 0) mask_apply(sc, ctx, g.mask, lts, layer.data(), ib.w, ib.h, 0);
     orc_draw_pixmap(px, w, h, ib.x, ib.y, layer.data(), ib.w, ib.h, g.opacity, convert_blend_mode(g.blend_mode));
+}
+
+// mask::apply (mask.rs:6-46)
+void mask_apply(const rvk_scene &sc, const Ctx &ctx, int32_t idx, Ts ts, uint8_t *px, uint32_t w, uint32_t h,
+                int depth) {
+    if (idx < 0 || (uint32_t)idx >= sc.mask_count || depth > kMaxNesting) return;
+    const rvk_mask &mk = sc.masks[idx];
+    const rvk_group &root = sc.groups[mk.root];
+    if (root.child_count == 0) {
+        std::fill(px, px + (size_t)w * h * 4, (uint8_t)0);
+        return;
+    }
+    std::vector<uint8_t> mp((size_t)w * h * 4, 0);
+    std::vector<uint8_t> region;
+    rect_mask(mk.rect, ts, w, h, region);
+    render_nodes(sc, root, ctx, ts, mp.data(), w, h);
+    apply_mask(mp.data(), region);
+    if (mk.mask >= 0) mask_apply(sc, ctx, mk.mask, ts, px, w, h, depth + 1);
+    std::vector<uint8_t> m;
+    mask_from_pixmap(mp, mk.kind, m);
+    apply_mask(px, m);
 }
 
 void render_nodes(const rvk_scene &sc, const rvk_group &g, const Ctx &ctx, Ts ts, uint8_t *px, uint32_t w,

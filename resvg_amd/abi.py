@@ -191,11 +191,23 @@ class NodeRef(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("index", C.c_uint32)]
 
 
+MASK_LUMINANCE, MASK_ALPHA = 0, 1
+
+
+class ClipPath(C.Structure):
+    _fields_ = [("transform", Transform), ("clip_path", C.c_int32), ("root", C.c_uint32)]
+
+
+class Mask(C.Structure):
+    _fields_ = [("rect", Rect), ("kind", C.c_uint32), ("mask", C.c_int32), ("root", C.c_uint32)]
+
+
 class Scene(C.Structure):
     _fields_ = [("groups", C.POINTER(Group)), ("group_count", C.c_uint32), ("paths", C.POINTER(Path)),
                 ("path_count", C.c_uint32), ("children", C.POINTER(NodeRef)), ("verbs", C.POINTER(C.c_uint8)),
                 ("points", C.POINTER(C.c_float)), ("stops", C.POINTER(Stop)), ("filters", C.POINTER(Filter)),
-                ("filter_data", FilterData)]
+                ("filter_data", FilterData), ("clip_paths", C.POINTER(ClipPath)), ("clip_path_count", C.c_uint32),
+                ("masks", C.POINTER(Mask)), ("mask_count", C.c_uint32)]
 
 
 class Stats(C.Structure):

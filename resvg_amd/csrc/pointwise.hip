@@ -194,6 +194,41 @@ __global__ void k_shadow_flood(uint32_t *px, size_t n, float cr, float cg, float
     }
 }
 
+// Pixmap::apply_mask with a mask made from a second pixmap (Mask::from_pixmap):
+// every channel becomes div255(c * m) (lowp LoadMaskU8 + DestinationIn).  m is the
+// source's alpha (kMaskAlpha), 255 minus it (kMaskInvAlpha: an inverted clip mask,
+// clip.rs:25-27) or its luminance: the demultiplied colour with Rec. 709 weights,
+// times alpha, scaled to 255 and rounded up (kMaskLuminance).
+constexpr uint32_t kMaskAlpha = 0, kMaskInvAlpha = 1, kMaskLuminance = 2;
+
+__device__ __forceinline__ uint32_t mask_value(uint32_t s, uint32_t mode) {
+    const uint32_t a8 = s >> 24;
+    if (mode == kMaskAlpha) return a8;
+    if (mode == kMaskInvAlpha) return 255u - a8;
+    float r = (float)(s & 0xff) / 255.0f, g = (float)((s >> 8) & 0xff) / 255.0f,
+          b = (float)((s >> 16) & 0xff) / 255.0f;
+    const float a = (float)a8 / 255.0f;
+    if (a8 != 0) {
+        r = r / a;
+        g = g / a;
+        b = b / a;
+    }
+    const float luma = r * 0.2126f + g * 0.7152f + b * 0.0722f;
+    const float v = f32_bound(0.0f, (luma * a) * 255.0f, 255.0f);
+    return (uint32_t)ceilf(v);
+}
+
+__global__ void k_apply_mask(uint32_t *px, const uint32_t *src, size_t n, uint32_t mode) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t m = mask_value(src[i], mode);
+        const uint32_t d = px[i];
+        if (m == 255u) continue;  // div255(c * 255) = c
+        uint32_t o = 0;
+        for (int k = 0; k < 4; k++) o |= ((((d >> (8 * k)) & 0xffu) * m + 255u) >> 8) << (8 * k);
+        px[i] = o;
+    }
+}
+
 __global__ void k_zero_rgb(uint32_t *px, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         px[i] &= 0xff000000u;
@@ -407,6 +442,14 @@ rvk_status launch_shadow_flood(rvk_context *ctx, uint8_t *px, size_t n, const ui
 }  // namespace rvkh
 
 namespace rvk {
+
+rvk_status launch_apply_mask(rvk_context *ctx, uint8_t *px, const uint8_t *src, size_t n, uint32_t mode) {
+    timed_launch(ctx, "apply_mask", 12.0 * (double)n, [&] {  // reads both pixmaps, writes one
+        hipLaunchKernelGGL(k_apply_mask, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px,
+                           (const uint32_t *)src, n, mode);
+    });
+    return post_launch(ctx);
+}
 
 rvk_status launch_zero_rgb(rvk_context *ctx, uint8_t *px, size_t n) {
     timed_launch(ctx, "zero_rgb", rw_bytes(n), [&] {

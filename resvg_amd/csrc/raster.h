@@ -91,6 +91,9 @@ struct ShaderParams {
     // Opaque shader (solid alpha 1, linear gradient with opaque stops): tiny-skia
     // reduces SourceOver to Source, so partial coverage lerps towards the colour.
     uint32_t as_source;
+    // BlendMode::Clear (clip path children): partial coverage lerps every channel
+    // toward zero, div255(d * (255 - c)); the colour is not used
+    uint32_t clear;
     float sx, ky, kx, sy, tx, ty;  // device -> unit transform
     float r_scale, r_bias;  // focal radius: t = (t - r_scale) / r_bias
     float t[kMaxStops + 2];

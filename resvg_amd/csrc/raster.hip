@@ -491,6 +491,23 @@ __device__ __forceinline__ uint32_t over_lowp(const uint32_t src_in[4], uint32_t
     return o;
 }
 
+// one covered pixel of a draw item: Clear, or the shaded colour composited with
+// over_lowp (the colour is shaded only when needed)
+__device__ __forceinline__ uint32_t paint_px(const ShaderParams &sh, int32_t lx, int32_t ly, uint32_t c, uint32_t d,
+                                             uint32_t src[4], bool &shaded) {
+    if (sh.clear) {
+        const uint32_t ic = 255u - c;
+        uint32_t o = 0;
+        for (int k = 0; k < 4; k++) o |= div255(((d >> (8 * k)) & 0xffu) * ic) << (8 * k);
+        return o;
+    }
+    if (!shaded) {
+        shade(sh, lx, ly, src);
+        shaded = true;
+    }
+    return over_lowp(src, c, d, sh.as_source != 0);
+}
+
 __device__ __forceinline__ uint32_t job_cov(const CovJob *jobs, const uint8_t *arena, int32_t j, int32_t x,
                                             int32_t y) {
     if (j < 0) return 0;
@@ -567,24 +584,17 @@ __global__ __launch_bounds__(TX * TY) void k_draw(uint32_t *px, uint32_t w, uint
             const bool stroke = (pass == 0) == (item.stroke_first != 0);
             const ShaderParams &sh = a.shaders[stroke ? item.stroke_shader : item.fill_shader];
             uint32_t src[4];
+            bool shaded = false;
             if (stroke && item.stroke_hair) {
                 // every blit of the hairline at this pixel, composited in order
-                bool shaded = false;
                 for (uint32_t k = hb; k < he; k++) {
                     const uint32_t c = hair_alpha(a.hair_lines[a.hair_bins[k]], lx, ly);
-                    if (!c) continue;
-                    if (!shaded) {
-                        shade(sh, lx, ly, src);
-                        shaded = true;
-                    }
-                    target = over_lowp(src, c, target, sh.as_source != 0);
+                    if (c) target = paint_px(sh, lx, ly, c, target, src, shaded);
                 }
                 continue;
             }
             const uint32_t c = stroke ? cs : cf;
-            if (!c) continue;
-            shade(sh, lx, ly, src);
-            target = over_lowp(src, c, target, sh.as_source != 0);
+            if (c) target = paint_px(sh, lx, ly, c, target, src, shaded);
         }
         if (!item.layer) {
             d = target;

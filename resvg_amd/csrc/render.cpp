@@ -109,9 +109,10 @@ struct PathGeom {
 };
 
 // Device-space geometry of one path drawn into a pw x ph pixmap: tiny-skia's
-// fill_path preconditions and edges, and the stroke outline's edges or, for a stroke
-// at most one device pixel wide, its hairline segments.
-void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, uint32_t pw, uint32_t ph, PathGeom &g) {
+// fill_path preconditions and edges, and (unless `fill_only`) the stroke outline's
+// edges or, for a stroke at most one device pixel wide, its hairline segments.
+void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, uint32_t pw, uint32_t ph, bool fill_only,
+                  PathGeom &g) {
     auto finish = [&](std::vector<Seg> &segs, EdgeSet &es) {
         if (segs.empty() || bounds_nearly_zero(segs)) return;
         float mnx = INFINITY, mxx = -INFINITY;
@@ -129,7 +130,7 @@ void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, uint32_t pw, ui
         fill_segments(sc, p, ts, segs);
         finish(segs, g.fill);
     }
-    if (p.stroke.kind != RVK_PAINT_NONE && p.stroke_width >= 0.0f) {
+    if (!fill_only && p.stroke.kind != RVK_PAINT_NONE && p.stroke_width >= 0.0f) {
         float cov = 1.0f;
         if (rvks::treat_as_hairline(p.stroke_width, p.anti_alias != 0, ts.sx, ts.ky, ts.kx, ts.sy, cov)) {
             g.hair = true;
@@ -186,9 +187,11 @@ struct Renderer {
     // walk consuming it in the same order (kSkip steps over a skipped subtree).
     enum Mode { kCollect, kExecute, kSkip } mode = kExecute;
     struct Task {
+        const rvk_scene *scene;
         const rvk_path *path;
         Ts ts;
         uint32_t pw, ph;  // size of the pixmap drawn into
+        bool fill_only;
     };
     std::vector<Task> tasks;
     std::vector<PathGeom> geoms;
@@ -202,7 +205,7 @@ struct Renderer {
             const size_t end = std::min(tasks.size(), k + kChunk);
             parallel_for(end - k, [&](size_t i) {
                 const Task &t = tasks[k + i];
-                compute_geom(sc, *t.path, t.ts, t.pw, t.ph, geoms[k + i]);
+                compute_geom(*t.scene, *t.path, t.ts, t.pw, t.ph, t.fill_only, geoms[k + i]);
             });
             ready_end = end;
         }
@@ -339,14 +342,18 @@ struct Renderer {
     }
 
     // fill / stroke coverage jobs + shaders of one path drawn with `ts` into a
-    // pw x ph pixmap whose visible pixels are `clip`.
-    void path_jobs(DrawList &dl, const rvk_path &p, Ts ts, uint32_t pw, uint32_t ph, IRect clip, DrawItem &it) {
+    // pw x ph pixmap whose visible pixels are `clip`.  `fill_only` draws no stroke;
+    // `clear` fills with BlendMode::Clear; `psc` is the scene holding the path (the
+    // render's scene unless given).
+    void path_jobs(DrawList &dl, const rvk_path &p, Ts ts, uint32_t pw, uint32_t ph, IRect clip, DrawItem &it,
+                   bool fill_only = false, bool clear = false, const rvk_scene *psc = nullptr) {
+        const rvk_scene &S = psc ? *psc : sc;
         it.fill_job = it.stroke_job = -1;
         it.fill_shader = it.stroke_shader = -1;
         it.stroke_hair = 0;
         it.stroke_first = p.paint_order_stroke_first ? 1 : 0;
         if (mode == kCollect) {
-            tasks.push_back(Task{&p, ts, pw, ph});
+            tasks.push_back(Task{&S, &p, ts, pw, ph, fill_only});
             return;
         }
         if (mode == kSkip) {
@@ -357,17 +364,19 @@ struct Renderer {
         if (!p.visible) return;
         ShaderParams sp;
         // path::fill_path (path.rs:26-75)
-        if (g.fill.ok && make_shader(sc, p.fill, ts, sp)) {
+        if (g.fill.ok && make_shader(S, p.fill, ts, sp)) {
             it.fill_job = add_job(dl, g.fill, p.fill_rule, p.anti_alias != 0, clip);
             if (it.fill_job >= 0) {
+                sp.clear = clear ? 1 : 0;
                 it.fill_shader = (int32_t)dl.shaders.size();
                 dl.shaders.push_back(sp);
             }
         }
+        if (fill_only) return;
         // path::stroke_path (path.rs:77-116): a hairline, or tiny-skia's stroker outline
         // filled nonzero
         if (g.hair) {
-            if (make_shader(sc, p.stroke, ts, sp, g.hair_opacity)) {
+            if (make_shader(S, p.stroke, ts, sp, g.hair_opacity)) {
                 it.stroke_job = add_hair_job(dl, g.hair_lines, clip);
                 if (it.stroke_job >= 0) {
                     it.stroke_hair = 1;
@@ -375,7 +384,7 @@ struct Renderer {
                     dl.shaders.push_back(sp);
                 }
             }
-        } else if (g.stroke.ok && make_shader(sc, p.stroke, ts, sp)) {
+        } else if (g.stroke.ok && make_shader(S, p.stroke, ts, sp)) {
             it.stroke_job = add_job(dl, g.stroke, RVK_FILL_NONZERO, p.anti_alias != 0, clip);
             if (it.stroke_job >= 0) {
                 it.stroke_shader = (int32_t)dl.shaders.size();
@@ -504,12 +513,205 @@ struct Renderer {
         push_item(dl, it);
     }
 
-    // steps over the paths of a subtree that is not drawn
+    // steps over the paths of a group's content, clip path and mask when the group
+    // is not drawn
     void skip_nodes(const rvk_group &g, Ts ts, DrawList &dl) {
         const Mode saved = mode;
         mode = kSkip;
         render_nodes(g, ts, dl);
+        walk_clip_mask(g, ts, (int32_t)dl.w, (int32_t)dl.h);
         mode = saved;
+    }
+
+    // collect / skip passes: the clip path and mask paths in drawing order
+    void walk_clip_mask(const rvk_group &g, Ts ts, int32_t w, int32_t h) {
+        if (g.clip_path >= 0) fail(apply_clip(g.clip_path, ts, nullptr, w, h, 0));
+        if (g.mask >= 0) fail(apply_mask(g.mask, ts, nullptr, w, h, 0));
+    }
+
+    // ---- clip paths (crates/resvg/src/clip.rs) ----
+    static constexpr int kMaxNesting = 64;  // deeper clip / mask chains are malformed input
+
+    // clip::draw_children (clip.rs:30-70) into `dl`: fills of the visible paths (Clear
+    // when `clear`), groups with only their transforms; a group with its own clip path
+    // is drawn through clip_group
+    void clip_children(const rvk_group &parent, bool clear, Ts ts, DrawList &dl, int depth) {
+        if (depth > kMaxNesting) {
+            fail(RVK_ERR_INVALID_ARGUMENT);
+            return;
+        }
+        for (uint32_t i = 0; i < parent.child_count && err == RVK_OK; i++) {
+            const rvk_node_ref &n = sc.children[parent.child_offset + i];
+            if (n.kind == RVK_NODE_PATH) {
+                const rvk_path &p = sc.paths[n.index];
+                if (!p.visible) continue;
+                if (dl.arena > budget) fail(flush(dl));
+                DrawItem it{};
+                it.layer = 0;
+                it.opacity = 1.0f;
+                it.blend_mode = RVK_BLEND_SOURCE_OVER;
+                path_jobs(dl, p, ts, dl.w, dl.h, IRect{0, 0, (int32_t)dl.w, (int32_t)dl.h}, it, true, clear);
+                push_item(dl, it);
+            } else {
+                const rvk_group &g = sc.groups[n.index];
+                const Ts gts = ts_concat(ts, ts_from(g.transform));
+                if (g.clip_path >= 0) clip_group(g, gts, dl, depth + 1);
+                else clip_children(g, clear, gts, dl, depth + 1);
+            }
+        }
+    }
+
+    // clip_group (clip.rs:72-98): the group's fills on their own canvas, clipped, then
+    // Xor-ed into the clip pixmap
+    void clip_group(const rvk_group &g, Ts ts, DrawList &dl, int depth) {
+        DrawList tdl;
+        tdl.w = dl.w;
+        tdl.h = dl.h;
+        if (mode != kExecute) {
+            clip_children(g, false, ts, tdl, depth);
+            fail(apply_clip(g.clip_path, ts, nullptr, (int32_t)dl.w, (int32_t)dl.h, depth + 1));
+            return;
+        }
+        void *buf = nullptr;
+        rvk_status s = rvk::scratch_alloc(ctx, (size_t)dl.w * dl.h * 4, &buf);
+        if (s != RVK_OK) {
+            fail(s);
+            return;
+        }
+        tdl.px = (uint8_t *)buf;
+        s = rvk::launch_fill_sub(ctx, tdl.px, dl.w, dl.h, 0, 0, dl.w, dl.h, 0u);
+        if (s == RVK_OK) {
+            clip_children(g, false, ts, tdl, depth);
+            s = err != RVK_OK ? err : flush(tdl);
+        }
+        if (s == RVK_OK) s = apply_clip(g.clip_path, ts, tdl.px, (int32_t)dl.w, (int32_t)dl.h, depth + 1);
+        if (s == RVK_OK) s = flush(dl);  // earlier fills of the clip pixmap come first
+        if (s == RVK_OK)
+            s = rvk::launch_draw_pixmap(ctx, dl.px, dl.w, dl.h, 0, 0, tdl.px, dl.w, dl.h, 1.0f, RVK_BLEND_XOR);
+        rvk::scratch_release(ctx, buf);
+        fail(s);
+    }
+
+    // clip::apply (clip.rs:6-28) on a w x h layer: a black pixmap with the clip's
+    // fills cleared out of it, the clip's own clip path applied to the layer, then the
+    // layer scaled by the inverted alpha
+    rvk_status apply_clip(int32_t idx, Ts ts, uint8_t *layer, int32_t w, int32_t h, int depth) {
+        if (idx < 0 || (uint32_t)idx >= sc.clip_path_count || depth > kMaxNesting) return RVK_ERR_INVALID_ARGUMENT;
+        const rvk_clip_path &cp = sc.clip_paths[idx];
+        if (cp.root >= sc.group_count) return RVK_ERR_INVALID_ARGUMENT;
+        const Ts cts = ts_concat(ts, ts_from(cp.transform));
+        DrawList cdl;
+        cdl.w = (uint32_t)w;
+        cdl.h = (uint32_t)h;
+        if (mode != kExecute) {
+            clip_children(sc.groups[cp.root], true, cts, cdl, depth);
+            return cp.clip_path >= 0 ? apply_clip(cp.clip_path, ts, nullptr, w, h, depth + 1) : RVK_OK;
+        }
+        void *buf = nullptr;
+        RVK_TRY(rvk::scratch_alloc(ctx, (size_t)w * h * 4, &buf));
+        cdl.px = (uint8_t *)buf;
+        rvk_status s = rvk::launch_fill_sub(ctx, cdl.px, w, h, 0, 0, w, h, 0xff000000u);  // Color::BLACK
+        if (s == RVK_OK) {
+            clip_children(sc.groups[cp.root], true, cts, cdl, depth);
+            s = err != RVK_OK ? err : flush(cdl);
+        }
+        if (s == RVK_OK && cp.clip_path >= 0) s = apply_clip(cp.clip_path, ts, layer, w, h, depth + 1);
+        if (s == RVK_OK) s = rvk::launch_apply_mask(ctx, layer, cdl.px, (size_t)w * h, 1u);
+        rvk::scratch_release(ctx, buf);
+        return s;
+    }
+
+    // ---- masks (crates/resvg/src/mask.rs) ----
+    // mask::apply (mask.rs:6-46): the mask content rendered into its own pixmap and
+    // limited to the mask region (Mask::fill_path of the region rect), the mask's own
+    // mask applied to the layer, then the layer scaled by the content's luminance or
+    // alpha
+    rvk_status apply_mask(int32_t idx, Ts ts, uint8_t *layer, int32_t w, int32_t h, int depth) {
+        if (idx < 0 || (uint32_t)idx >= sc.mask_count || depth > kMaxNesting) return RVK_ERR_INVALID_ARGUMENT;
+        const rvk_mask &mk = sc.masks[idx];
+        if (mk.root >= sc.group_count) return RVK_ERR_INVALID_ARGUMENT;
+        const rvk_group &root = sc.groups[mk.root];
+        if (root.child_count == 0)  // no children: the whole element is masked out
+            return mode == kExecute ? rvk::launch_fill_sub(ctx, layer, w, h, 0, 0, w, h, 0u) : RVK_OK;
+        DrawList rdl, mdl;
+        rdl.w = mdl.w = (uint32_t)w;
+        rdl.h = mdl.h = (uint32_t)h;
+        auto region = [&] {
+            DrawItem it{};
+            it.layer = 0;
+            it.opacity = 1.0f;
+            it.blend_mode = RVK_BLEND_SOURCE_OVER;
+            path_jobs(rdl, aux.paths[idx], ts, rdl.w, rdl.h, IRect{0, 0, w, h}, it, true, false, &aux.scene);
+            push_item(rdl, it);
+        };
+        if (mode != kExecute) {
+            region();
+            render_nodes(root, ts, mdl);
+            return mk.mask >= 0 ? apply_mask(mk.mask, ts, nullptr, w, h, depth + 1) : RVK_OK;
+        }
+        const size_t bytes = (size_t)w * h * 4;
+        void *rbuf = nullptr, *mbuf = nullptr;
+        RVK_TRY(rvk::scratch_alloc(ctx, bytes, &rbuf));
+        rvk_status s = rvk::scratch_alloc(ctx, bytes, &mbuf);
+        if (s != RVK_OK) {
+            rvk::scratch_release(ctx, rbuf);
+            return s;
+        }
+        rdl.px = (uint8_t *)rbuf;
+        mdl.px = (uint8_t *)mbuf;
+        s = rvk::launch_fill_sub(ctx, rdl.px, w, h, 0, 0, w, h, 0u);
+        if (s == RVK_OK) s = rvk::launch_fill_sub(ctx, mdl.px, w, h, 0, 0, w, h, 0u);
+        // the region's coverage: an opaque white fill (Source) leaves alpha = coverage
+        if (s == RVK_OK) {
+            region();
+            s = err != RVK_OK ? err : flush(rdl);
+        }
+        if (s == RVK_OK) {
+            render_nodes(root, ts, mdl);
+            s = err != RVK_OK ? err : flush(mdl);
+        }
+        if (s == RVK_OK) s = rvk::launch_apply_mask(ctx, mdl.px, rdl.px, (size_t)w * h, 0u);
+        if (s == RVK_OK && mk.mask >= 0) s = apply_mask(mk.mask, ts, layer, w, h, depth + 1);
+        if (s == RVK_OK)
+            s = rvk::launch_apply_mask(ctx, layer, mdl.px, (size_t)w * h, mk.kind == RVK_MASK_ALPHA ? 0u : 2u);
+        rvk::scratch_release(ctx, mbuf);
+        rvk::scratch_release(ctx, rbuf);
+        return s;
+    }
+
+    // the region rect of every mask as a path (PathBuilder::from_rect) filled opaque white
+    struct Aux {
+        std::vector<uint8_t> verbs;
+        std::vector<float> points;
+        std::vector<rvk_path> paths;
+        rvk_scene scene{};
+    } aux;
+
+    void build_aux() {
+        for (uint32_t i = 0; i < sc.mask_count; i++) {
+            const rvk_rect r = sc.masks[i].rect;
+            const float l = r.x, t = r.y, rr = r.x + r.w, b = r.y + r.h;
+            rvk_path p{};
+            p.verb_offset = (uint32_t)aux.verbs.size();
+            p.verb_count = 5;
+            p.point_offset = (uint32_t)(aux.points.size() / 2);
+            p.visible = 1;
+            p.anti_alias = 1;
+            p.fill.kind = RVK_PAINT_COLOR;
+            p.fill.color[0] = p.fill.color[1] = p.fill.color[2] = 255;
+            p.fill.opacity = 1.0f;
+            p.fill_rule = RVK_FILL_NONZERO;
+            p.stroke.kind = RVK_PAINT_NONE;
+            p.bounds = r;
+            for (uint8_t v : {RVK_VERB_MOVE, RVK_VERB_LINE, RVK_VERB_LINE, RVK_VERB_LINE, RVK_VERB_CLOSE})
+                aux.verbs.push_back(v);
+            for (float v : {l, t, rr, t, rr, b, l, b}) aux.points.push_back(v);
+            aux.paths.push_back(p);
+        }
+        aux.scene.verbs = aux.verbs.data();
+        aux.scene.points = aux.points.data();
+        aux.scene.paths = aux.paths.data();
+        aux.scene.path_count = (uint32_t)aux.paths.size();
     }
 
     void render_nodes(const rvk_group &g, Ts ts, DrawList &dl) {
@@ -527,10 +729,6 @@ struct Renderer {
                              g.filter_count > 0 || g.blend_mode != RVK_MIX_NORMAL;
         if (!isolate) {
             render_nodes(g, ts, dl);
-            return;
-        }
-        if (g.clip_path >= 0 || g.mask >= 0) {  // clip paths and masks are outside this scene format
-            fail(RVK_ERR_UNSUPPORTED);
             return;
         }
         FRect bb = rect_transform(g.layer_bbox, ts);
@@ -554,8 +752,9 @@ struct Renderer {
         dy -= bb.t - (float)ib.y;
         const Ts lts = ts_concat(Ts{1, 0, 0, 1, -dx, -dy}, ts);
         if (mode == kExecute) ctx->stats.layers++;
-        // elided layer: one path child, no filters
-        if (g.filter_count == 0 && g.child_count == 1 && sc.children[g.child_offset].kind == RVK_NODE_PATH) {
+        // elided layer: one path child, no filters, clip path or mask
+        if (g.filter_count == 0 && g.clip_path < 0 && g.mask < 0 && g.child_count == 1 &&
+            sc.children[g.child_offset].kind == RVK_NODE_PATH) {
             if (dl.arena > budget) fail(flush(dl));
             const rvk_path &p = sc.paths[sc.children[g.child_offset].index];
             DrawItem it{};
@@ -578,6 +777,7 @@ struct Renderer {
         child.h = (uint32_t)ib.h;
         if (mode != kExecute) {
             render_nodes(g, lts, child);
+            walk_clip_mask(g, lts, ib.w, ib.h);
             return;
         }
         void *buf = nullptr;
@@ -598,6 +798,8 @@ struct Renderer {
         for (uint32_t i = 0; s == RVK_OK && err == RVK_OK && i < g.filter_count; i++)
             s = rvk::filter_apply(ctx, &sc.filters[g.filter_offset + i], &sc.filter_data, lts, layer, ib.w, ib.h,
                                   nullptr);
+        if (s == RVK_OK && err == RVK_OK && g.clip_path >= 0) s = apply_clip(g.clip_path, lts, layer, ib.w, ib.h, 0);
+        if (s == RVK_OK && err == RVK_OK && g.mask >= 0) s = apply_mask(g.mask, lts, layer, ib.w, ib.h, 0);
         if (s == RVK_OK) s = flush(dl);
         if (s == RVK_OK)
             s = rvk::launch_draw_pixmap(ctx, dl.px, dl.w, dl.h, ib.x, ib.y, layer, ib.w, ib.h, g.opacity,
@@ -614,6 +816,7 @@ namespace rvk {
 rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t w, uint32_t h, uint8_t *px,
                         uint64_t *overflow_rows) {
     Renderer r{ctx, *scene, max_filter_bbox(w, h)};
+    r.build_aux();
     void *ovbuf = nullptr;
     RVK_TRY(rvk::scratch_alloc(ctx, 256, &ovbuf));
     r.dev_overflow = (int32_t *)ovbuf;
@@ -662,7 +865,8 @@ rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_
 extern "C" {
 
 static bool scene_ok(const rvk_scene *s) {
-    return s && s->group_count > 0 && s->groups && (s->path_count == 0 || (s->paths && s->verbs && s->points));
+    return s && s->group_count > 0 && s->groups && (s->path_count == 0 || (s->paths && s->verbs && s->points)) &&
+           (s->clip_path_count == 0 || s->clip_paths) && (s->mask_count == 0 || s->masks);
 }
 
 rvk_status rvk_render(rvk_context *ctx, const rvk_scene *scene, rvk_transform transform, uint32_t width,

@@ -127,5 +127,7 @@ rvk_status launch_fill_sub(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
                            int64_t x1, int64_t y1, uint32_t value);
 rvk_status launch_copy(rvk_context *ctx, uint8_t *dst, const uint8_t *src, size_t bytes);
 rvk_status launch_zero_rgb(rvk_context *ctx, uint8_t *px, size_t n);
+// px = div255(px * m) per channel, m from `src` (0 alpha, 1 inverted alpha, 2 luminance)
+rvk_status launch_apply_mask(rvk_context *ctx, uint8_t *px, const uint8_t *src, size_t n, uint32_t mode);
 
 }  // namespace rvk

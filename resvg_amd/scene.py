@@ -75,21 +75,39 @@ class Scene:
         self.primitives: list = []
         self.values: list[float] = []
         self.inputs: list[int] = []
+        self.clip_paths: list[dict] = []
+        self.masks: list[dict] = []
         self.root = self._new_group(None, (1.0, 0.0, 0.0, 1.0, 0.0, 0.0), 1.0, abi.MIX["normal"], False, [])
         self._abi = None
 
     # ---- construction ----
-    def _new_group(self, parent, transform, opacity, blend, isolate, filters):
+    def _new_group(self, parent, transform, opacity, blend, isolate, filters, clip_path=-1, mask=-1):
         gid = len(self.groups)
         self.groups.append(dict(transform=tuple(transform), opacity=opacity, blend=blend, isolate=isolate,
-                                filters=list(filters), children=[], bbox=None))
+                                filters=list(filters), children=[], bbox=None, clip_path=clip_path, mask=mask))
         if parent is not None:
             self.groups[parent]["children"].append((abi.NODE_GROUP, gid))
         self._abi = None
         return gid
 
+    def clip_path(self, transform=(1.0, 0.0, 0.0, 1.0, 0.0, 0.0), clip_path: int = -1) -> tuple[int, int]:
+        """A usvg::ClipPath; returns (clip path index, group to add its children to).
+        Children are drawn as fills (with the clip rule as their fill rule) and groups."""
+        root = self._new_group(None, (1.0, 0.0, 0.0, 1.0, 0.0, 0.0), 1.0, abi.MIX["normal"], False, [])
+        self.clip_paths.append(dict(transform=tuple(transform), clip_path=clip_path, root=root))
+        self._abi = None
+        return len(self.clip_paths) - 1, root
+
+    def mask(self, rect, kind: int = abi.MASK_LUMINANCE, mask: int = -1) -> tuple[int, int]:
+        """A usvg::Mask with its region in user space; returns (mask index, content group)."""
+        root = self._new_group(None, (1.0, 0.0, 0.0, 1.0, 0.0, 0.0), 1.0, abi.MIX["normal"], False, [])
+        self.masks.append(dict(rect=tuple(f32(v) for v in rect), kind=kind, mask=mask, root=root))
+        self._abi = None
+        return len(self.masks) - 1, root
+
     def group(self, parent: int, transform=(1.0, 0.0, 0.0, 1.0, 0.0, 0.0), opacity: float = 1.0,
-              blend: str | int = "normal", isolate: bool = False, filters: list[FilterGraph] = ()) -> int:
+              blend: str | int = "normal", isolate: bool = False, filters: list[FilterGraph] = (),
+              clip_path: int = -1, mask: int = -1) -> int:
         mode = abi.MIX[blend] if isinstance(blend, str) else blend
         fids = []
         for g in filters:
@@ -109,7 +127,7 @@ class Scene:
             self.inputs.extend(g.inputs)
             fids.append(len(self.filters))
             self.filters.append((tuple(g.rect), off, len(g.primitives)))
-        return self._new_group(parent, transform, opacity, mode, isolate, fids)
+        return self._new_group(parent, transform, opacity, mode, isolate, fids, clip_path, mask)
 
     def path(self, parent: int, verbs, points, fill: PaintSpec | None = None, fill_rule: int = abi.FILL_NONZERO,
              stroke: PaintSpec | None = None, stroke_width: float = 1.0, stroke_join: int = abi.JOIN_MITER,
@@ -196,8 +214,8 @@ class Scene:
             G.opacity = g["opacity"]
             G.blend_mode = g["blend"]
             G.isolate = int(g["isolate"])
-            G.clip_path = -1
-            G.mask = -1
+            G.clip_path = g["clip_path"]
+            G.mask = g["mask"]
             G.filter_offset = g["filters"][0] if g["filters"] else 0
             G.filter_count = len(g["filters"])
             bb = self._group_layer_bounds(i)
@@ -262,13 +280,20 @@ class Scene:
         sc.filters = C.cast(filt, C.POINTER(abi.Filter))
         sc.filter_data = abi.FilterData(C.cast(prims, C.POINTER(abi.Primitive)), C.cast(vals, C.POINTER(C.c_float)),
                                         C.cast(ins, C.POINTER(C.c_int32)))
+        cps = abi.array(abi.ClipPath, [abi.ClipPath(abi.Transform(*c["transform"]), c["clip_path"], c["root"])
+                                       for c in self.clip_paths])
+        msk = abi.array(abi.Mask, [abi.Mask(abi.Rect(*m["rect"]), m["kind"], m["mask"], m["root"]) for m in self.masks])
+        sc.clip_paths = C.cast(cps, C.POINTER(abi.ClipPath))
+        sc.clip_path_count = len(self.clip_paths)
+        sc.masks = C.cast(msk, C.POINTER(abi.Mask))
+        sc.mask_count = len(self.masks)
 
         class _SceneABI:
             pass
 
         out = _SceneABI()
         out.abi = sc
-        out.keep = (groups, paths, ch, verbs, pts, st, filt, prims, vals, ins)
+        out.keep = (groups, paths, ch, verbs, pts, st, filt, prims, vals, ins, cps, msk)
         self._abi = out
         return out
 

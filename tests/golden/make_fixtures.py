@@ -48,6 +48,15 @@ OTHER = [
     "shapes/path/M-C", "shapes/path/M-Q", "shapes/path/M-C-S", "shapes/path/M-Q-T", "shapes/path/M-L-L-Z",
     "painting/stroke-miterlimit/default", "painting/stroke-miterlimit/valid-value",
     "painting/stroke/control-points-clamping-1",
+    "masking/clipPath/simple-case", "masking/clipPath/clip-rule=evenodd", "masking/clipPath/multiple-children",
+    "masking/clipPath/overlapped-shapes-with-evenodd", "masking/clipPath/clip-path-on-child",
+    "masking/clipPath/clip-path-on-children", "masking/clipPath/nested-clip-path",
+    "masking/clipPath/clipPathUnits=objectBoundingBox", "masking/clipPath/transform-on-clipPath",
+    "masking/clipPath/clip-path-with-transform",
+    "masking/mask/simple-case", "masking/mask/color-interpolation=linearRGB", "masking/mask/mask-type=alpha",
+    "masking/mask/maskUnits=userSpaceOnUse-with-rect", "masking/mask/maskUnits=userSpaceOnUse-with-percent",
+    "masking/mask/nested-objectBoundingBox", "masking/mask/mask-on-self", "masking/mask/mask-on-child",
+    "masking/mask/on-group-with-transform",
 ]
 
 

@@ -193,30 +193,37 @@ class _PathBuilder:
         self.verbs.append(abi.VERB_CLOSE)
 
 
+def _f32s(*v):
+    return [np.float32(x) for x in v]
+
+
 def ellipse_path_usvg(cx, cy, rx, ry):
-    """usvg shapes.rs:265-274 ellipse_to_path."""
+    """usvg shapes.rs:265-274 ellipse_to_path (end points in f32 arithmetic)."""
+    cx, cy, rx, ry = _f32s(cx, cy, rx, ry)
     b = _PathBuilder()
-    b.move_to(cx + rx, cy)
-    b.arc_to(rx, ry, cx, cy + ry)
-    b.arc_to(rx, ry, cx - rx, cy)
-    b.arc_to(rx, ry, cx, cy - ry)
-    b.arc_to(rx, ry, cx + rx, cy)
+    b.move_to(float(cx + rx), float(cy))
+    b.arc_to(float(rx), float(ry), float(cx), float(cy + ry))
+    b.arc_to(float(rx), float(ry), float(cx - rx), float(cy))
+    b.arc_to(float(rx), float(ry), float(cx), float(cy - ry))
+    b.arc_to(float(rx), float(ry), float(cx + rx), float(cy))
     b.close()
     return b.verbs, b.pts
 
 
 def rounded_rect_usvg(x, y, w, h, rx, ry):
-    """usvg shapes.rs:107-122 convert_rect with rx/ry."""
+    """usvg shapes.rs:107-122 convert_rect with rx/ry (end points in f32 arithmetic)."""
+    x, y, w, h, rx, ry = _f32s(x, y, w, h, rx, ry)
+    F = float
     b = _PathBuilder()
-    b.move_to(x + rx, y)
-    b.line_to(x + w - rx, y)
-    b.arc_to(rx, ry, x + w, y + ry)
-    b.line_to(x + w, y + h - ry)
-    b.arc_to(rx, ry, x + w - rx, y + h)
-    b.line_to(x + rx, y + h)
-    b.arc_to(rx, ry, x, y + h - ry)
-    b.line_to(x, y + ry)
-    b.arc_to(rx, ry, x + rx, y)
+    b.move_to(F(x + rx), F(y))
+    b.line_to(F((x + w) - rx), F(y))
+    b.arc_to(F(rx), F(ry), F(x + w), F(y + ry))
+    b.line_to(F(x + w), F((y + h) - ry))
+    b.arc_to(F(rx), F(ry), F((x + w) - rx), F(y + h))
+    b.line_to(F(x + rx), F(y + h))
+    b.arc_to(F(rx), F(ry), F(x), F((y + h) - ry))
+    b.line_to(F(x), F(y + ry))
+    b.arc_to(F(rx), F(ry), F(x + rx), F(y))
     b.close()
     return b.verbs, b.pts
 
@@ -366,6 +373,168 @@ def _hairline_fixtures():
     return out
 
 
+def _ts_mul(a, b):
+    """a.pre_concat(b) in f32 (tiny-skia Transform::concat; f64 products in the skew case)."""
+    a = [float(np.float32(v)) for v in a]
+    b = [float(np.float32(v)) for v in b]
+    f = lambda v: float(np.float32(v))
+    if a[1] == 0 and a[2] == 0 and b[1] == 0 and b[2] == 0:
+        return (f(a[0] * b[0]), 0.0, 0.0, f(a[3] * b[3]), f(f(a[0] * b[4]) + a[4]), f(f(a[3] * b[5]) + a[5]))
+    return (f(a[0] * b[0] + a[2] * b[1]), f(a[1] * b[0] + a[3] * b[1]),
+            f(a[0] * b[2] + a[2] * b[3]), f(a[1] * b[2] + a[3] * b[3]),
+            f(f(a[0] * b[4] + a[2] * b[5]) + a[4]), f(f(a[1] * b[4] + a[3] * b[5]) + a[5]))
+
+
+def _rotate(deg):
+    v = np.float32(deg) * np.float32(np.pi / 180.0)
+    c, s = float(np.cos(v, dtype=np.float32)), float(np.sin(v, dtype=np.float32))
+    return (c, s, -s, c, 0.0, 0.0)
+
+
+def _masking_fixtures():
+    """masking/clipPath/* and masking/mask/*: clip paths (clip.rs) and masks (mask.rs)
+    as usvg resolves them (clip children are black fills with the clip rule;
+    objectBoundingBox units become transforms; default mask regions are -10%/120%)."""
+    out = {}
+    M, L, Z = abi.VERB_MOVE, abi.VERB_LINE, abi.VERB_CLOSE
+    ident = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
+    star = ([M, L, L, L, L, Z], [(100, 15), (150, 175), (20, 75), (180, 75), (50, 175)])
+    full = rect_path(0.0, 0.0, 200.0, 200.0)
+    black = PaintSpec.solid(BLACK)
+    green = PaintSpec.solid(GREEN)
+
+    def clipped(sc, clip, parent=None, transform=ident):
+        g = sc.group(sc.root if parent is None else parent, transform=transform, clip_path=clip)
+        sc.path(g, *full, fill=green)
+        return g
+
+    def circle(cx, cy, r):
+        return ellipse_path_usvg(cx, cy, r, r)
+
+    sc = Scene(200, 200)
+    c, root = sc.clip_path()
+    sc.path(root, *star, fill=black)
+    clipped(sc, c)
+    out["masking/clipPath/simple-case"] = sc
+
+    sc = Scene(200, 200)
+    c, root = sc.clip_path()
+    sc.path(root, *star, fill=black, fill_rule=abi.FILL_EVENODD)
+    clipped(sc, c)
+    out["masking/clipPath/clip-rule=evenodd"] = sc
+
+    sc = Scene(200, 200)
+    c, root = sc.clip_path()
+    sc.path(root, *star, fill=black, fill_rule=abi.FILL_EVENODD)
+    sc.path(root, *circle(100, 100, 10), fill=black)
+    clipped(sc, c)
+    out["masking/clipPath/multiple-children"] = sc
+
+    sc = Scene(200, 200)
+    c, root = sc.clip_path()
+    for cx, cy in ((100, 70), (70, 110), (130, 110)):
+        sc.path(root, *circle(cx, cy, 40), fill=black, fill_rule=abi.FILL_EVENODD)
+    clipped(sc, c)
+    out["masking/clipPath/overlapped-shapes-with-evenodd"] = sc
+
+    sc = Scene(200, 200)
+    c2, r2 = sc.clip_path()
+    sc.path(r2, *circle(100, 100, 60), fill=black)
+    c1, r1 = sc.clip_path()
+    g = sc.group(r1, clip_path=c2)
+    sc.path(g, *star, fill=black, fill_rule=abi.FILL_EVENODD)
+    clipped(sc, c1)
+    out["masking/clipPath/clip-path-on-child"] = sc
+
+    sc = Scene(200, 200)
+    c2, r2 = sc.clip_path()
+    sc.path(r2, *circle(100, 100, 60), fill=black)
+    c1, r1 = sc.clip_path()
+    g = sc.group(r1, clip_path=c2)
+    sc.path(g, *star, fill=black, fill_rule=abi.FILL_EVENODD)
+    g = sc.group(r1, transform=_ts_mul((0.25, 0.0, 0.0, 0.25, 0.0, 0.0), (1.0, 0.0, 0.0, 1.0, 300.0, 300.0)),
+                 clip_path=c2)
+    sc.path(g, *rect_path(50.0, 50.0, 100.0, 100.0), fill=black)
+    clipped(sc, c1)
+    out["masking/clipPath/clip-path-on-children"] = sc
+
+    sc = Scene(200, 200)
+    c1, r1 = sc.clip_path()
+    sc.path(r1, *star, fill=black, fill_rule=abi.FILL_EVENODD)
+    c2, r2 = sc.clip_path()
+    sc.path(r2, *circle(100, 100, 60), fill=black)
+    g1 = sc.group(sc.root, clip_path=c2)
+    clipped(sc, c1, parent=g1)
+    out["masking/clipPath/nested-clip-path"] = sc
+
+    bbox200 = (200.0, 0.0, 0.0, 200.0, 0.0, 0.0)
+    trr = _ts_mul((1.0, 0.0, 0.0, 1.0, 100.0, -40.0), _rotate(45.0))
+    for name, clip_ts, elem_ts in (("clipPathUnits=objectBoundingBox", bbox200, ident),
+                                   ("transform-on-clipPath", _ts_mul(trr, bbox200), ident),
+                                   ("clip-path-with-transform", bbox200, trr)):
+        sc = Scene(200, 200)
+        c, root = sc.clip_path(transform=clip_ts)
+        sc.path(root, *rounded_rect_usvg(0.2, 0.2, 0.6, 0.6, 0.1, 0.1), fill=black)
+        clipped(sc, c, transform=elem_ts)
+        out[f"masking/clipPath/{name}"] = sc
+
+    # masks: content a rect with a white (transparent) -> black gradient in its bbox
+    WHITE_T = [(0.0, WHITE, 0.0), (1.0, BLACK, 1.0)]
+    lg = PaintSpec.linear(0.0, 0.0, 1.0, 0.0, WHITE_T, transform=_bbox_ts(20.0, 20.0, 160.0, 160.0))
+
+    def masked(sc, m, rect=(0.0, 0.0, 200.0, 200.0), parent=None):
+        g = sc.group(sc.root if parent is None else parent, mask=m)
+        sc.path(g, *rect_path(*rect), fill=green)
+        return g
+
+    for name, region, kind in (("simple-case", (-20.0, -20.0, 240.0, 240.0), abi.MASK_LUMINANCE),
+                               ("color-interpolation=linearRGB", (-20.0, -20.0, 240.0, 240.0), abi.MASK_LUMINANCE),
+                               ("mask-type=alpha", (-20.0, -20.0, 240.0, 240.0), abi.MASK_ALPHA),
+                               ("maskUnits=userSpaceOnUse-with-rect", (30.0, 50.0, 100.0, 120.0), abi.MASK_LUMINANCE),
+                               ("maskUnits=userSpaceOnUse-with-percent", (60.0, -20.0, 80.0, 240.0),
+                                abi.MASK_LUMINANCE)):
+        sc = Scene(200, 200)
+        m, root = sc.mask(region, kind)
+        sc.path(root, *rect_path(20.0, 20.0, 160.0, 160.0), fill=lg)
+        masked(sc, m)
+        out[f"masking/mask/{name}"] = sc
+
+    sc = Scene(200, 200)
+    m, root = sc.mask((-20.0, -20.0, 240.0, 240.0))
+    g = sc.group(root, transform=bbox200)
+    sc.path(g, *rect_path(0.2, 0.2, 0.6, 0.6),
+            fill=PaintSpec.linear(0.0, 0.0, 1.0, 0.0, WHITE_T, transform=_bbox_ts(0.2, 0.2, 0.6, 0.6)))
+    masked(sc, m)
+    out["masking/mask/nested-objectBoundingBox"] = sc
+
+    sc = Scene(200, 200)
+    m1, r1 = sc.mask((4.0, 4.0, 192.0, 192.0))
+    sc.path(r1, *rect_path(40.0, 40.0, 120.0, 120.0), fill=PaintSpec.solid((128, 128, 128)))
+    m2, r2 = sc.mask((-20.0, -20.0, 240.0, 240.0), mask=m1)
+    sc.path(r2, *rect_path(20.0, 20.0, 160.0, 160.0), fill=PaintSpec.solid(WHITE))
+    masked(sc, m2, rect=(20.0, 20.0, 160.0, 160.0))
+    out["masking/mask/mask-on-self"] = sc
+
+    sc = Scene(200, 200)
+    m1, r1 = sc.mask((4.0, 4.0, 192.0, 192.0))
+    sc.path(r1, *rect_path(20.0, 20.0, 160.0, 160.0), fill=lg)
+    m2, r2 = sc.mask((4.0, 4.0, 192.0, 192.0))
+    masked(sc, m1, rect=(20.0, 20.0, 160.0, 160.0), parent=r2)
+    masked(sc, m2, rect=(20.0, 20.0, 160.0, 160.0))
+    out["masking/mask/mask-on-child"] = sc
+
+    sc = Scene(200, 200)
+    m, root = sc.mask((45.0, -10.0, 60.0, 120.0))
+    sc.path(root, *rect_path(0.0, 0.0, 100.0, 50.0), fill=PaintSpec.solid(WHITE))
+    gts = _ts_mul(_ts_mul(_ts_mul((1.0, 0.0, 0.0, 1.0, 100.0, 100.0), _rotate(45.0)),
+                          (1.0, 0.0, 0.0, 1.0, -100.0, -100.0)),
+                  _ts_mul((2.0, 0.0, 0.0, 2.0, 0.0, 0.0), (1.0, 0.0, 0.0, 1.0, -30.0, 20.0)))
+    g = sc.group(sc.root, transform=gts, mask=m)
+    sc.path(g, *rect_path(50.0, 0.0, 50.0, 100.0), fill=green)
+    out["masking/mask/on-group-with-transform"] = sc
+    return out
+
+
 def filter_scene(fx) -> Scene:
     """A filter fixture (fixture_scenes.Fixture) as a scene: a filtered group with a rect."""
     sc = Scene(200, 200)
@@ -378,6 +547,7 @@ def filter_scene(fx) -> Scene:
 def all_scene_fixtures():
     out = dict(shape_fixtures())
     out.update(_hairline_fixtures())
+    out.update(_masking_fixtures())
     for fx in filter_fixtures():
         out[fx.name] = filter_scene(fx)
     return out

@@ -164,6 +164,59 @@ def test_hairline_many_lines_per_tile(ctx):
     _assert_same(_gpu(ctx, sc, ts, 96, 96), _oracle(sc, ts, 96, 96), "dense hairlines")
 
 
+def _clip_or_mask_scene(seed, size=160):
+    """Random content in groups carrying random clip paths (with nested clip paths and
+    clip-path'd children) and masks (luminance / alpha, nested masks, gradients)."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(size, size)
+
+    def rand_ts():
+        a = float(np.float32(rng.uniform(0.6, 1.3)))
+        k = float(np.float32(rng.uniform(-0.3, 0.3)))
+        return (a, k, -k, a, float(np.float32(rng.uniform(-20, 20))), float(np.float32(rng.uniform(-20, 20))))
+
+    clips = []
+    for i in range(3):
+        nested = clips[-1] if clips and rng.random() < 0.5 else -1
+        c, root = sc.clip_path(transform=rand_ts() if rng.random() < 0.5 else (1.0, 0.0, 0.0, 1.0, 0.0, 0.0),
+                               clip_path=nested)
+        for _ in range(int(rng.integers(1, 4))):
+            parent = root
+            if clips and rng.random() < 0.3:
+                parent = sc.group(root, transform=rand_ts(), clip_path=clips[int(rng.integers(len(clips)))])
+            v, p = _random_path(rng, size, int(rng.integers(2, 8)))
+            sc.path(parent, v, p, fill=PaintSpec.solid((0, 0, 0)),
+                    fill_rule=abi.FILL_EVENODD if rng.random() < 0.5 else abi.FILL_NONZERO)
+        clips.append(c)
+    masks = []
+    for i in range(3):
+        nested = masks[-1] if masks and rng.random() < 0.5 else -1
+        x, y = rng.uniform(-10, size * 0.4, 2)
+        m, root = sc.mask((float(x), float(y), float(rng.uniform(size * 0.4, size)), float(rng.uniform(size * 0.4, size))),
+                          kind=abi.MASK_ALPHA if rng.random() < 0.4 else abi.MASK_LUMINANCE, mask=nested)
+        for _ in range(int(rng.integers(1, 4))):
+            v, p = _random_path(rng, size, int(rng.integers(2, 8)))
+            sc.path(root, v, p, fill=_paint(rng, size))
+        masks.append(m)
+    for i in range(8):
+        clip = clips[int(rng.integers(len(clips)))] if rng.random() < 0.5 else -1
+        mask = masks[int(rng.integers(len(masks)))] if rng.random() < 0.5 else -1
+        g = sc.group(sc.root, transform=rand_ts() if rng.random() < 0.3 else (1.0, 0.0, 0.0, 1.0, 0.0, 0.0),
+                     opacity=float(np.float32(rng.uniform(0.5, 1.0))), clip_path=clip, mask=mask)
+        for _ in range(int(rng.integers(1, 3))):
+            v, p = _random_path(rng, size, int(rng.integers(2, 9)))
+            sc.path(g, v, p, fill=_paint(rng, size), stroke=_paint(rng, size) if rng.random() < 0.3 else None,
+                    stroke_width=float(np.float32(rng.uniform(0.5, 4.0))))
+    return sc
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_clip_paths_and_masks(ctx, seed):
+    sc = _clip_or_mask_scene(700 + seed)
+    ts = abi.Transform(1.1, 0.0, 0.0, 1.1, -4.0, 3.0) if seed % 2 else abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, 176, 176), _oracle(sc, ts, 176, 176), f"clip/mask seed {seed}")
+
+
 def test_composites_onto_existing_pixels(ctx):
     rng = np.random.default_rng(7)
     base = rng.integers(0, 256, (96, 128, 4)).astype(np.uint8)

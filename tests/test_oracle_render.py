@@ -128,3 +128,88 @@ def test_thin_stroke_coverage_scales_the_paint():
     # width 0.5 under identity: coverage 0.5 -> paint opacity (255 * 128 >> 8) / 255
     a = _hair_alpha([abi.VERB_MOVE, abi.VERB_LINE], [(10.0, 20.5), (50.0, 20.5)], width=0.5)
     assert (a[20, 10:50] == 127).all() and a.sum() == 127 * 40
+
+
+# ---------------------------------------------------------------------------
+# clip paths and masks: closed forms on pixel-aligned shapes
+# ---------------------------------------------------------------------------
+def _content_with(sc, clip=-1, mask=-1):
+    g = sc.group(sc.root, clip_path=clip, mask=mask)
+    v, p = rect_path(0.0, 0.0, 40.0, 30.0)
+    sc.path(g, v, p, fill=PaintSpec.linear(0.0, 0.0, 40.0, 0.0, [(0.0, (255, 0, 0), 1.0), (1.0, (0, 0, 255), 0.5)]))
+
+
+def _render(sc, w=40, h=30):
+    px = np.zeros((h, w, 4), np.uint8)
+    orc.render(sc.to_abi().abi, abi.Transform.identity(), px)
+    return px
+
+
+def test_pixel_aligned_clip_keeps_inside_and_clears_outside():
+    plain = Scene(40, 30)
+    _content_with(plain)
+    ref = _render(plain)
+    sc = Scene(40, 30)
+    c, root = sc.clip_path()
+    v, p = rect_path(5.0, 4.0, 20.0, 10.0)
+    sc.path(root, v, p, fill=PaintSpec.solid((0, 0, 0)))
+    _content_with(sc, clip=c)
+    got = _render(sc)
+    exp = np.zeros_like(ref)
+    exp[4:14, 5:25] = ref[4:14, 5:25]
+    assert np.array_equal(got, exp)
+
+
+def test_clip_children_union_and_evenodd_rule():
+    # two overlapping aligned rects in one clip path: their union clips (each fill clears
+    # the black clip pixmap); one evenodd child with a doubly wound square clips nothing
+    plain = Scene(40, 30)
+    _content_with(plain)
+    ref = _render(plain)
+    sc = Scene(40, 30)
+    c, root = sc.clip_path()
+    for r in ((2.0, 2.0, 10.0, 10.0), (8.0, 6.0, 10.0, 10.0)):
+        v, p = rect_path(*r)
+        sc.path(root, v, p, fill=PaintSpec.solid((0, 0, 0)))
+    _content_with(sc, clip=c)
+    got = _render(sc)
+    inside = np.zeros((30, 40), bool)
+    inside[2:12, 2:12] = True
+    inside[6:16, 8:18] = True
+    exp = np.where(inside[..., None], ref, 0).astype(np.uint8)
+    assert np.array_equal(got, exp)
+
+
+def test_alpha_and_luminance_masks_of_solid_content():
+    plain = Scene(40, 30)
+    _content_with(plain)
+    ref = _render(plain).astype(np.uint32)
+    for kind, color, m in ((abi.MASK_ALPHA, (0, 0, 0), 255), (abi.MASK_LUMINANCE, (255, 255, 255), 255),
+                           (abi.MASK_LUMINANCE, (0, 0, 0), 0), (abi.MASK_LUMINANCE, (128, 128, 128), 128)):
+        sc = Scene(40, 30)
+        mk, root = sc.mask((-4.0, -3.0, 48.0, 36.0), kind)
+        v, p = rect_path(10.0, 5.0, 20.0, 20.0)
+        sc.path(root, v, p, fill=PaintSpec.solid(color))
+        _content_with(sc, mask=mk)
+        got = _render(sc)
+        exp = np.zeros_like(ref)
+        exp[5:25, 10:30] = (ref[5:25, 10:30] * m + 255) >> 8  # div255(c * m)
+        assert np.array_equal(got, exp.astype(np.uint8)), (kind, color)
+
+
+def test_mask_region_limits_and_empty_mask_clears():
+    plain = Scene(40, 30)
+    _content_with(plain)
+    ref = _render(plain)
+    sc = Scene(40, 30)
+    mk, root = sc.mask((6.0, 3.0, 10.0, 12.0), abi.MASK_ALPHA)  # region inside the white content
+    v, p = rect_path(0.0, 0.0, 40.0, 30.0)
+    sc.path(root, v, p, fill=PaintSpec.solid((255, 255, 255)))
+    _content_with(sc, mask=mk)
+    exp = np.zeros_like(ref)
+    exp[3:15, 6:16] = ref[3:15, 6:16]
+    assert np.array_equal(_render(sc), exp)
+    sc = Scene(40, 30)
+    mk, _ = sc.mask((0.0, 0.0, 40.0, 30.0))
+    _content_with(sc, mask=mk)
+    assert not _render(sc).any()
