@@ -215,6 +215,8 @@ typedef struct rvk_path {                  /* usvg::Path (mod.rs:1282-1440) */
     float stroke_width, stroke_miter_limit;
     uint32_t stroke_cap, stroke_join;
     rvk_rect bounds;                       /* path.data().bounds(): bounds of all points */
+    uint32_t dash_offset, dash_count;      /* Stroke::dasharray in rvk_scene.dashes (count 0: none) */
+    float dash_phase;                      /* Stroke::dashoffset */
 } rvk_path;
 
 typedef struct rvk_group {                 /* usvg::Group (mod.rs:1035-1225) */
@@ -258,6 +260,7 @@ typedef struct rvk_scene {
     rvk_filter_data filter_data;
     const rvk_clip_path *clip_paths;  uint32_t clip_path_count;
     const rvk_mask *masks;            uint32_t mask_count;
+    const float *dashes;              /* dash arrays of the paths' strokes */
 } rvk_scene;
 
 /* ------------------------------------------------------------------------- */

@@ -523,17 +523,38 @@ static void coverage(const std::vector<Seg> &segs, uint32_t rule, bool aa, uint3
 // fixtures).  The outline is built in user space, mapped with the transform and
 // filled nonzero (tiny-skia Pixmap::stroke_path -> fill_path).
 // ---------------------------------------------------------------------------
-static bool stroke_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
-    rvks::StrokeParams sp{path.stroke_width, path.stroke_miter_limit, path.stroke_cap, path.stroke_join};
-    const float res = rvks::resolution_scale(ts.sx, ts.ky, ts.kx, ts.sy);
-    rvks::PathBuilder outline;
-    const uint8_t *verbs = sc.verbs + path.verb_offset;
-    const float *pts = sc.points + 2 * (size_t)path.point_offset;
-    size_t npts = 0;
+// Pixmap::stroke_path (tiny-skia painter.rs): the path, or its dashes (Path::dash,
+// resvg_amd/csrc/dash.cpp) when StrokeDash::new accepts the stroke's dash array;
+// false when dashing leaves nothing
+static bool stroke_input(const rvk_scene &sc, const rvk_path &path, float res, rvks::PathBuilder &dashed,
+                         const uint8_t *&verbs, size_t &nverbs, const float *&pts, size_t &npts) {
+    verbs = sc.verbs + path.verb_offset;
+    nverbs = path.verb_count;
+    pts = sc.points + 2 * (size_t)path.point_offset;
+    npts = 0;
     for (uint32_t v = 0; v < path.verb_count; v++)
         npts += verbs[v] == RVK_VERB_MOVE || verbs[v] == RVK_VERB_LINE ? 1 : verbs[v] == RVK_VERB_QUAD ? 2
               : verbs[v] == RVK_VERB_CUBIC ? 3 : 0;
-    if (!rvks::stroke(verbs, path.verb_count, pts, npts, sp, res, outline)) return false;
+    if (path.dash_count == 0 || !sc.dashes) return true;
+    rvks::DashParams dp;
+    if (!rvks::dash_params(sc.dashes + path.dash_offset, path.dash_count, path.dash_phase, dp)) return true;
+    if (!rvks::dash(verbs, nverbs, pts, npts, dp, res, dashed)) return false;
+    verbs = dashed.verbs.data();
+    nverbs = dashed.verbs.size();
+    pts = &dashed.pts[0].x;
+    npts = dashed.pts.size();
+    return true;
+}
+
+static bool stroke_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
+    rvks::StrokeParams sp{path.stroke_width, path.stroke_miter_limit, path.stroke_cap, path.stroke_join};
+    const float res = rvks::resolution_scale(ts.sx, ts.ky, ts.kx, ts.sy);
+    rvks::PathBuilder outline, dashed;
+    const uint8_t *verbs;
+    const float *pts;
+    size_t nverbs, npts;
+    if (!stroke_input(sc, path, res, dashed, verbs, nverbs, pts, npts)) return false;
+    if (!rvks::stroke(verbs, nverbs, pts, npts, sp, res, outline)) return false;
     for (auto &p : outline.pts) ts_map(ts, p.x, p.y);
     // contours of the outline, each closed for filling
     size_t pi = 0;
@@ -1043,12 +1064,12 @@ static void do_anti_hairline(int32_t x0, int32_t y0, int32_t x1, int32_t y1, con
 // tiny-skia Pixmap::stroke_path's hairline branch -> stroke_hairline -> hairline_aa::stroke_path
 static void stroke_hairline(const rvk_scene &sc, const rvk_path &path, Ts ts, const Shader &s, uint8_t *px,
                             uint32_t w, uint32_t h) {
-    const uint8_t *verbs = sc.verbs + path.verb_offset;
-    const float *pts = sc.points + 2 * (size_t)path.point_offset;
-    size_t npts = 0;
-    for (uint32_t v = 0; v < path.verb_count; v++)
-        npts += verbs[v] == RVK_VERB_MOVE || verbs[v] == RVK_VERB_LINE ? 1 : verbs[v] == RVK_VERB_QUAD ? 2
-              : verbs[v] == RVK_VERB_CUBIC ? 3 : 0;
+    rvks::PathBuilder dashed;
+    const uint8_t *verbs;
+    const float *pts;
+    size_t nverbs, npts;
+    if (!stroke_input(sc, path, rvks::resolution_scale(ts.sx, ts.ky, ts.kx, ts.sy), dashed, verbs, nverbs, pts, npts))
+        return;
     std::vector<rvks::Pt> dev(npts);
     for (size_t i = 0; i < npts; i++) {
         float x = pts[2 * i], y = pts[2 * i + 1];
@@ -1056,7 +1077,7 @@ static void stroke_hairline(const rvk_scene &sc, const rvk_path &path, Ts ts, co
         dev[i] = rvks::Pt{x, y};
     }
     std::vector<rvks::HairSeg> segs;
-    rvks::anti_hairline_segments(verbs, path.verb_count, dev.data(), npts, path.stroke_cap, w, h, segs);
+    rvks::anti_hairline_segments(verbs, nverbs, dev.data(), npts, path.stroke_cap, w, h, segs);
     HairTarget t{s, px, (int32_t)w, (int32_t)h, false, 0, 0, 0, 0};
     for (const rvks::HairSeg &g : segs) {
         const int32_t c[4] = {g.cl, g.ct, g.cr, g.cb};

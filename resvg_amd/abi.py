@@ -177,7 +177,7 @@ class Path(C.Structure):
                 ("visible", C.c_uint32), ("anti_alias", C.c_uint32), ("paint_order_stroke_first", C.c_uint32),
                 ("fill", Paint), ("fill_rule", C.c_uint32), ("stroke", Paint), ("stroke_width", C.c_float),
                 ("stroke_miter_limit", C.c_float), ("stroke_cap", C.c_uint32), ("stroke_join", C.c_uint32),
-                ("bounds", Rect)]
+                ("bounds", Rect), ("dash_offset", C.c_uint32), ("dash_count", C.c_uint32), ("dash_phase", C.c_float)]
 
 
 class Group(C.Structure):
@@ -207,7 +207,7 @@ class Scene(C.Structure):
                 ("path_count", C.c_uint32), ("children", C.POINTER(NodeRef)), ("verbs", C.POINTER(C.c_uint8)),
                 ("points", C.POINTER(C.c_float)), ("stops", C.POINTER(Stop)), ("filters", C.POINTER(Filter)),
                 ("filter_data", FilterData), ("clip_paths", C.POINTER(ClipPath)), ("clip_path_count", C.c_uint32),
-                ("masks", C.POINTER(Mask)), ("mask_count", C.c_uint32)]
+                ("masks", C.POINTER(Mask)), ("mask_count", C.c_uint32), ("dashes", C.POINTER(C.c_float))]
 
 
 class Stats(C.Structure):

@@ -41,6 +41,7 @@ struct PieceDev {
 };
 
 constexpr int kGroupRows = 4;  // pixel rows per entry of a job's edge-window table
+constexpr uint32_t kCoverageSmallKeys = 128;  // crossings per sub-scanline of the low-LDS coverage variant
 
 // A coverage job computes u8 coverage for the pixel rect [x0, x0+w) x [y0, y0+h)
 // of its pixmap (coordinates of the edge space) into cov[(y - y0) * stride + (x - x0)],

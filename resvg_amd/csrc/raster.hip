@@ -38,6 +38,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kMaxKeys = 512;     // crossings per sub-scanline (8 per lane)
+constexpr int kSmallKeys = (int)rvkr::kCoverageSmallKeys;  // the low-LDS variant's limit (2 per lane)
 constexpr int kWin = 1024;        // pixels accumulated per window
 constexpr int kMaxSub = 4;        // sub-scanlines per pixel row
 
@@ -140,6 +141,7 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
 // coverage
 // ---------------------------------------------------------------------------
 struct CovArgs {
+    const uint32_t *groups;  // first global row of each row group this launch handles
     const CovJob *jobs;
     uint32_t njobs;
     const EdgeCov *edges;
@@ -231,8 +233,8 @@ __device__ __forceinline__ void bitonic_from(uint64_t (&v)[S][K], int lane) {
 // Sorts the S sub-scanlines' keys (n[s] each, in keys[s]) together and writes each
 // one's span events (pixel-rounded supersampled columns; starts and ends alternate)
 // over its own key area.  Returns the event counts in nev.
-template <int K, int S>
-__device__ __forceinline__ void spans_of(uint64_t (*keys)[kMaxKeys], const int *n, bool evenodd, int *nev, int lane) {
+template <int K, int S, int MK>
+__device__ __forceinline__ void spans_of(uint64_t (*keys)[MK], const int *n, bool evenodd, int *nev, int lane) {
     uint64_t v[S][K];
 #pragma unroll
     for (int s = 0; s < S; s++)
@@ -278,14 +280,21 @@ __device__ __forceinline__ void spans_of(uint64_t (*keys)[kMaxKeys], const int *
     }
 }
 
+// MK: crossings per sub-scanline the launch can hold.  The host sends a row group to
+// the kSmallKeys variant (a quarter of the LDS, so about three times the waves per
+// CU) when the group's edge window, an upper bound on any of its sub-scanlines'
+// crossings, fits; otherwise to the kMaxKeys variant.  Block b handles row
+// groups[b / kGroupRows] + b % kGroupRows.
+template <int MK>
 __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
-    __shared__ uint64_t keys[kMaxSub][kMaxKeys];  // crossings; reused for span events
+    __shared__ uint64_t keys[kMaxSub][MK];  // crossings; reused for span events
     __shared__ __align__(16) int32_t acc[kWin + 4];  // acc (low 16 bits) + run difference (high 16 bits)
     const int lane = threadIdx.x;
-    const uint32_t gr = blockIdx.x;
-    const uint32_t ji = find_job(a.jobs, a.njobs, gr);
+    // the group's first row picks the job (a job's last group can be partial)
+    const uint32_t gs = a.groups[blockIdx.x / rvkr::kGroupRows];
+    const uint32_t ji = find_job(a.jobs, a.njobs, gs);
     const CovJob job = a.jobs[ji];
-    const int32_t r = (int32_t)(gr - job.row_start);
+    const int32_t r = (int32_t)(gs - job.row_start) + (int32_t)(blockIdx.x % rvkr::kGroupRows);
     if (r < 0 || r >= job.h) return;
     const int shift = job.aa ? 2 : 0;
     const int nsub = 1 << shift;
@@ -329,7 +338,7 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
         for (int s = 0; s < nsub; s++) {
             const uint64_t m = __ballot(act[s]);
             const int pos = cnt[s] + __popcll(m & ((1ull << lane) - 1ull));
-            if (act[s] && pos < kMaxKeys) keys[s][pos] = key[s];
+            if (act[s] && pos < MK) keys[s][pos] = key[s];
             cnt[s] += __popcll(m);
         }
     }
@@ -338,24 +347,24 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
     int nev[kMaxSub] = {0, 0, 0, 0};
     int nmax = 0;
     for (int s = 0; s < nsub; s++) {
-        if (cnt[s] > kMaxKeys) {
+        if (cnt[s] > MK) {
             if (lane == 0) atomicAdd(a.overflow, 1);
-            cnt[s] = kMaxKeys;
+            cnt[s] = MK;
         }
         nmax = max(nmax, cnt[s]);
     }
     if (nsub == 1) {
         if (nmax == 0) {}
-        else if (nmax <= 64) spans_of<1, 1>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 128) spans_of<2, 1>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 256) spans_of<4, 1>(keys, cnt, evenodd, nev, lane);
-        else spans_of<8, 1>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 64) spans_of<1, 1, MK>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 128 || MK <= 128) spans_of<2, 1, MK>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 256) spans_of<(MK > 128 ? 4 : 2), 1, MK>(keys, cnt, evenodd, nev, lane);
+        else spans_of<(MK > 128 ? 8 : 2), 1, MK>(keys, cnt, evenodd, nev, lane);
     } else {
         if (nmax == 0) {}
-        else if (nmax <= 64) spans_of<1, kMaxSub>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 128) spans_of<2, kMaxSub>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 256) spans_of<4, kMaxSub>(keys, cnt, evenodd, nev, lane);
-        else spans_of<8, kMaxSub>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 64) spans_of<1, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 128 || MK <= 128) spans_of<2, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 256) spans_of<(MK > 128 ? 4 : 2), kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+        else spans_of<(MK > 128 ? 8 : 2), kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
     }
     __syncthreads();
     // ---- accumulate per window, prefix-sum, clamp, write u8 ----
@@ -548,19 +557,26 @@ struct DrawArgs {
     const HairLine *hair_lines;
 };
 
-constexpr int TX = 64, TY = 4;
+// One thread per target pixel; each 64-lane wave covers an 8 x 8 pixel square and a
+// block of 4 waves a 16 x 16 tile.  Items are culled per block and per wave; a wave
+// runs an item's shading only when one of its pixels is covered, and compact squares
+// cross path boundaries far less often than 64-pixel row segments do.
+constexpr int kWaveSide = 8;
+constexpr int TX = 2 * kWaveSide, TY = 2 * kWaveSide;
 
-__global__ __launch_bounds__(TX * TY) void k_draw(uint32_t *px, uint32_t w, uint32_t h, int32_t ox, int32_t oy,
-                                                 DrawArgs a) {
-    const int32_t x = ox + (int32_t)(blockIdx.x * TX + threadIdx.x % TX);
-    const int32_t y = oy + (int32_t)(blockIdx.y * TY + threadIdx.x / TX);
+__global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t h, int32_t ox, int32_t oy,
+                                              DrawArgs a) {
+    const int lane = (int)(threadIdx.x % 64), wv = (int)(threadIdx.x / 64);
     const int32_t tx0 = ox + (int32_t)(blockIdx.x * TX), ty0 = oy + (int32_t)(blockIdx.y * TY);
+    const int32_t wx0 = tx0 + (wv & 1) * kWaveSide, wy0 = ty0 + (wv >> 1) * kWaveSide;
+    const int32_t x = wx0 + lane % kWaveSide, y = wy0 + lane / kWaveSide;
     const bool inside = x < (int32_t)w && y < (int32_t)h;
     uint32_t d = inside ? px[(size_t)y * w + x] : 0u;
     const uint32_t d0 = d;
     for (uint32_t it = 0; it < a.nitems; it++) {
         const DrawItem item = a.items[it];
         if (item.x1 <= tx0 || item.x0 >= tx0 + TX || item.y1 <= ty0 || item.y0 >= ty0 + TY) continue;
+        if (item.x1 <= wx0 || item.x0 >= wx0 + kWaveSide || item.y1 <= wy0 || item.y0 >= wy0 + kWaveSide) continue;
         if (!inside || x < item.x0 || x >= item.x1 || y < item.y0 || y >= item.y1) continue;
         const int32_t lx = x - item.lx, ly = y - item.ly;
         const uint32_t cf = job_cov(a.jobs, a.arena, item.fill_job, lx, ly);
@@ -623,14 +639,27 @@ rvk_status launch_expand(rvk_context *ctx, const EdgeRec *edges, uint32_t nedges
     return RVK_OK;
 }
 
-rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, uint32_t total_rows,
-                           const EdgeCov *edges, const uint2 *windows, const PieceDev *pieces, const uint32_t *npieces,
-                           uint8_t *arena, int32_t *overflow, double algo_bytes) {
-    if (total_rows == 0) return RVK_OK;
-    CovArgs a{jobs, njobs, edges, windows, pieces, npieces, arena, overflow};
-    timed_launch(ctx, "coverage", algo_bytes,
-                 [&] { hipLaunchKernelGGL(k_coverage, dim3(total_rows), dim3(kWave), 0, ctx->stream, a); });
-    RVK_HIP_CHECK(hipGetLastError());
+rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint32_t *small_groups,
+                           uint32_t n_small, const uint32_t *big_groups, uint32_t n_big, const EdgeCov *edges,
+                           const uint2 *windows, const PieceDev *pieces, const uint32_t *npieces, uint8_t *arena,
+                           int32_t *overflow, double algo_bytes) {
+    // the algorithmic bytes are split between the two launches by their row counts
+    const double frac = n_small + n_big ? (double)n_small / (double)(n_small + n_big) : 0.0;
+    if (n_small) {
+        CovArgs a{small_groups, jobs, njobs, edges, windows, pieces, npieces, arena, overflow};
+        timed_launch(ctx, "coverage", algo_bytes * frac, [&] {
+            hipLaunchKernelGGL(k_coverage<kSmallKeys>, dim3(n_small * rvkr::kGroupRows), dim3(kWave), 0, ctx->stream,
+                               a);
+        });
+        RVK_HIP_CHECK(hipGetLastError());
+    }
+    if (n_big) {
+        CovArgs a{big_groups, jobs, njobs, edges, windows, pieces, npieces, arena, overflow};
+        timed_launch(ctx, "coverage", algo_bytes * (1.0 - frac), [&] {
+            hipLaunchKernelGGL(k_coverage<kMaxKeys>, dim3(n_big * rvkr::kGroupRows), dim3(kWave), 0, ctx->stream, a);
+        });
+        RVK_HIP_CHECK(hipGetLastError());
+    }
     return RVK_OK;
 }
 
@@ -643,7 +672,8 @@ rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint
     dim3 grid(div_up((uint32_t)(x1 - x0), TX), div_up((uint32_t)(y1 - y0), TY));
     const DrawArgs a{items, nitems, jobs, arena, shaders, hair_jobs, hair_tiles, hair_bins, hair_lines};
     timed_launch(ctx, "draw_items", algo_bytes, [&] {
-        hipLaunchKernelGGL(k_draw, grid, dim3(TX * TY), 0, ctx->stream, (uint32_t *)target, w, h, x0, y0, a);
+        // four 8 x 8 waves per 16 x 16 tile
+        hipLaunchKernelGGL(k_draw, grid, dim3(256), 0, ctx->stream, (uint32_t *)target, w, h, x0, y0, a);
     });
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;

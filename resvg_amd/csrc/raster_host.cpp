@@ -412,20 +412,46 @@ void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &
 }
 
 // ---------------------------------------------------------------------------
-// Strokes: tiny-skia's PathStroker (stroker.cpp) in user space, the outline mapped
-// with the transform and handed to the fill path (Pixmap::stroke_path -> fill_path).
+// Strokes: the path or its dashes (dash.cpp), tiny-skia's PathStroker (stroker.cpp)
+// in user space, the outline mapped with the transform and handed to the fill path
+// (Pixmap::stroke_path -> fill_path).
 // ---------------------------------------------------------------------------
-void stroke_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
-    rvks::StrokeParams sp{path.stroke_width, path.stroke_miter_limit, path.stroke_cap, path.stroke_join};
-    const float res = rvks::resolution_scale(ts.sx, ts.ky, ts.kx, ts.sy);
-    const uint8_t *verbs = sc.verbs + path.verb_offset;
-    const float *pts = sc.points + 2 * (size_t)path.point_offset;
-    size_t npts = 0;
+namespace {
+
+// Pixmap::stroke_path's input: the path, or its dashes when the stroke has a dash
+// array that StrokeDash::new accepts.  False when dashing leaves nothing to draw.
+bool stroke_input(const rvk_scene &sc, const rvk_path &path, float res, rvks::PathBuilder &dashed,
+                  const uint8_t *&verbs, size_t &nverbs, const float *&pts, size_t &npts) {
+    verbs = sc.verbs + path.verb_offset;
+    nverbs = path.verb_count;
+    pts = sc.points + 2 * (size_t)path.point_offset;
+    npts = 0;
     for (uint32_t v = 0; v < path.verb_count; v++)
         npts += verbs[v] == RVK_VERB_MOVE || verbs[v] == RVK_VERB_LINE ? 1 : verbs[v] == RVK_VERB_QUAD ? 2
               : verbs[v] == RVK_VERB_CUBIC ? 3 : 0;
+    if (path.dash_count == 0 || !sc.dashes) return true;
+    rvks::DashParams dp;
+    if (!rvks::dash_params(sc.dashes + path.dash_offset, path.dash_count, path.dash_phase, dp)) return true;
+    if (!rvks::dash(verbs, nverbs, pts, npts, dp, res, dashed)) return false;
+    verbs = dashed.verbs.data();
+    nverbs = dashed.verbs.size();
+    pts = &dashed.pts[0].x;
+    npts = dashed.pts.size();
+    return true;
+}
+
+}  // namespace
+
+void stroke_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
+    rvks::StrokeParams sp{path.stroke_width, path.stroke_miter_limit, path.stroke_cap, path.stroke_join};
+    const float res = rvks::resolution_scale(ts.sx, ts.ky, ts.kx, ts.sy);
+    rvks::PathBuilder dashed;
+    const uint8_t *verbs;
+    const float *pts;
+    size_t nverbs, npts;
+    if (!stroke_input(sc, path, res, dashed, verbs, nverbs, pts, npts)) return;
     rvks::PathBuilder outline;
-    if (!rvks::stroke(verbs, path.verb_count, pts, npts, sp, res, outline)) return;
+    if (!rvks::stroke(verbs, nverbs, pts, npts, sp, res, outline)) return;
     size_t pi = 0;
     Pt move{0, 0}, last{0, 0};
     bool open = false;
@@ -561,12 +587,13 @@ void prepare_anti_hairline(int32_t x0, int32_t y0, int32_t x1, int32_t y1, const
 
 void hairline_lines(const rvk_scene &sc, const rvk_path &path, Ts ts, uint32_t w, uint32_t h,
                     std::vector<HairLine> &out) {
-    const uint8_t *verbs = sc.verbs + path.verb_offset;
-    const float *pts = sc.points + 2 * (size_t)path.point_offset;
-    size_t npts = 0;
-    for (uint32_t v = 0; v < path.verb_count; v++)
-        npts += verbs[v] == RVK_VERB_MOVE || verbs[v] == RVK_VERB_LINE ? 1 : verbs[v] == RVK_VERB_QUAD ? 2
-              : verbs[v] == RVK_VERB_CUBIC ? 3 : 0;
+    rvks::PathBuilder dashed;
+    const uint8_t *verbs;
+    const float *pts;
+    size_t nverbs, npts;
+    if (!stroke_input(sc, path, rvks::resolution_scale(ts.sx, ts.ky, ts.kx, ts.sy), dashed, verbs, nverbs, pts,
+                      npts))
+        return;
     std::vector<rvks::Pt> dev(npts);
     for (size_t i = 0; i < npts; i++) {
         float x = pts[2 * i], y = pts[2 * i + 1];
@@ -574,7 +601,7 @@ void hairline_lines(const rvk_scene &sc, const rvk_path &path, Ts ts, uint32_t w
         dev[i] = rvks::Pt{x, y};
     }
     std::vector<rvks::HairSeg> segs;
-    rvks::anti_hairline_segments(verbs, path.verb_count, dev.data(), npts, path.stroke_cap, w, h, segs);
+    rvks::anti_hairline_segments(verbs, nverbs, dev.data(), npts, path.stroke_cap, w, h, segs);
     for (const rvks::HairSeg &g : segs) prepare_anti_hairline(g.x0, g.y0, g.x1, g.y1, g, out);
 }
 

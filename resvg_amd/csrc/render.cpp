@@ -29,9 +29,10 @@ using namespace rvkr;
 namespace rvk {
 rvk_status launch_expand(rvk_context *ctx, const EdgeRec *edges, uint32_t nedges, const EdgeCov *cov,
                          PieceDev *pieces, uint32_t *npieces, int32_t *overflow, double algo_bytes);
-rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, uint32_t total_rows,
-                           const EdgeCov *edges, const uint2 *windows, const PieceDev *pieces, const uint32_t *npieces,
-                           uint8_t *arena, int32_t *overflow, double algo_bytes);
+rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint32_t *small_groups,
+                           uint32_t n_small, const uint32_t *big_groups, uint32_t n_big, const EdgeCov *edges,
+                           const uint2 *windows, const PieceDev *pieces, const uint32_t *npieces, uint8_t *arena,
+                           int32_t *overflow, double algo_bytes);
 rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
                              int32_t x1, int32_t y1, const DrawItem *items, uint32_t nitems, const CovJob *jobs,
                              const uint8_t *arena, const ShaderParams *shaders, const HairJob *hair_jobs,
@@ -429,6 +430,17 @@ struct Renderer {
         // pieces | arena (the part after || is written by the device).  The metadata is
         // packed straight into a pinned staging buffer and copied without a host wait;
         // device buffers are reused only by later work on the same stream.
+        // row groups by the variant of the coverage kernel that handles them: a
+        // group's edge window bounds the crossings of any of its sub-scanlines
+        std::vector<uint32_t> groups_small, groups_big;
+        for (const CovJob &j : dl.jobs) {
+            const int32_t ng = (j.h + kGroupRows - 1) / kGroupRows;
+            for (int32_t g = 0; g < ng; g++) {
+                const uint2 wnd = dl.windows[j.window_offset + g];
+                (wnd.y - wnd.x <= kCoverageSmallKeys ? groups_small : groups_big)
+                    .push_back(j.row_start + (uint32_t)(g * kGroupRows));
+            }
+        }
         auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
         const size_t o_jobs = 0;
         const size_t o_edges = al(o_jobs + dl.jobs.size() * sizeof(CovJob));
@@ -440,7 +452,9 @@ struct Renderer {
         const size_t o_ht = al(o_hj + dl.hair_jobs.size() * sizeof(HairJob));
         const size_t o_hb = al(o_ht + dl.hair_tiles.size() * 4);
         const size_t o_hl = al(o_hb + dl.hair_bins.size() * 4);
-        const size_t meta = al(o_hl + dl.hair_lines.size() * sizeof(HairLine));
+        const size_t o_gs = al(o_hl + dl.hair_lines.size() * sizeof(HairLine));
+        const size_t o_gb = al(o_gs + groups_small.size() * 4);
+        const size_t meta = al(o_gb + groups_big.size() * 4);
         const size_t o_np = meta;
         const size_t o_pc = al(o_np + dl.edges.size() * 4);
         const size_t o_arena = al(o_pc + (size_t)dl.piece_slots * sizeof(PieceDev));
@@ -459,6 +473,8 @@ struct Renderer {
         put(o_ht, dl.hair_tiles.data(), dl.hair_tiles.size() * 4);
         put(o_hb, dl.hair_bins.data(), dl.hair_bins.size() * 4);
         put(o_hl, dl.hair_lines.data(), dl.hair_lines.size() * sizeof(HairLine));
+        put(o_gs, groups_small.data(), groups_small.size() * 4);
+        put(o_gb, groups_big.data(), groups_big.size() * 4);
         void *dev = nullptr;
         RVK_TRY(rvk::scratch_alloc(ctx, o_arena + dl.arena, &dev));
         uint8_t *d = (uint8_t *)dev;
@@ -483,7 +499,9 @@ struct Renderer {
                                     (const EdgeCov *)(d + o_ecov), (PieceDev *)(d + o_pc), (uint32_t *)(d + o_np),
                                     dev_overflow, exp_bytes);
         if (st == RVK_OK)
-            st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)dl.jobs.size(), dl.rows,
+            st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)dl.jobs.size(),
+                                      (const uint32_t *)(d + o_gs), (uint32_t)groups_small.size(),
+                                      (const uint32_t *)(d + o_gb), (uint32_t)groups_big.size(),
                                       (const EdgeCov *)(d + o_ecov), (const uint2 *)(d + o_win),
                                       (const PieceDev *)(d + o_pc), (const uint32_t *)(d + o_np), d + o_arena,
                                       dev_overflow, cov_bytes);

@@ -72,6 +72,22 @@ bool stroke(const uint8_t *verbs, size_t nverbs, const float *points, size_t npo
 // is at most one device pixel wide under the 2x2 part of the transform.
 bool treat_as_hairline(float width, bool anti_alias, float sx, float ky, float kx, float sy, float &coverage);
 
+// tiny-skia's StrokeDash::new: the dash array (even length, no negative entry,
+// positive finite sum), the offset reduced into [0, interval length), and the
+// first interval the offset falls in.  False when tiny-skia rejects the dash.
+struct DashParams {
+    std::vector<float> array;
+    float offset = 0.0f, interval_len = 0.0f, first_len = 0.0f;
+    size_t first_index = 0;
+};
+bool dash_params(const float *array, size_t n, float offset, DashParams &out);
+
+// Path::dash (dash.rs): the dashed path of verbs/points at resolution scale
+// `res_scale` into `out`.  False when no dash is produced (tiny-skia returns None and
+// stroke_path draws nothing).
+bool dash(const uint8_t *verbs, size_t nverbs, const float *points, size_t npoints, const DashParams &d,
+          float res_scale, PathBuilder &out);
+
 // path_geometry helpers shared with the hairline code: the t values (clamped to
 // [0, 1], sorted, distinct) where F'(t) . F''(t) = 0; de Casteljau split at t;
 // valid_unit_divide; Transform::map_points.

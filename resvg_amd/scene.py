@@ -75,6 +75,7 @@ class Scene:
         self.primitives: list = []
         self.values: list[float] = []
         self.inputs: list[int] = []
+        self.dashes: list[float] = []
         self.clip_paths: list[dict] = []
         self.masks: list[dict] = []
         self.root = self._new_group(None, (1.0, 0.0, 0.0, 1.0, 0.0, 0.0), 1.0, abi.MIX["normal"], False, [])
@@ -132,14 +133,18 @@ class Scene:
     def path(self, parent: int, verbs, points, fill: PaintSpec | None = None, fill_rule: int = abi.FILL_NONZERO,
              stroke: PaintSpec | None = None, stroke_width: float = 1.0, stroke_join: int = abi.JOIN_MITER,
              stroke_cap: int = abi.CAP_BUTT, miter_limit: float = 4.0, anti_alias: bool = True,
-             visible: bool = True, stroke_first: bool = False) -> int:
+             visible: bool = True, stroke_first: bool = False, dasharray=None, dashoffset: float = 0.0) -> int:
+        """`dasharray` is usvg's resolved list (non-negative, even length, non-zero sum)."""
         pts = [f32(v) for v in np.asarray(points, dtype=np.float64).reshape(-1)]
         pid = len(self.paths)
         self.paths.append(dict(verb_offset=len(self.verbs), verb_count=len(verbs), point_offset=len(self.points) // 2,
                                fill=fill or PaintSpec(), fill_rule=fill_rule, stroke=stroke or PaintSpec(),
                                stroke_width=stroke_width, join=stroke_join, cap=stroke_cap, miter=miter_limit,
                                aa=anti_alias, visible=visible, stroke_first=stroke_first,
-                               npts=len(pts) // 2, pts=np.array(pts, np.float32).reshape(-1, 2)))
+                               npts=len(pts) // 2, pts=np.array(pts, np.float32).reshape(-1, 2),
+                               dash_offset=len(self.dashes), dash_count=len(dasharray or ()),
+                               dash_phase=f32(dashoffset)))
+        self.dashes.extend(f32(v) for v in (dasharray or ()))
         self.verbs.extend(int(v) for v in verbs)
         self.points.extend(pts)
         self.groups[parent]["children"].append((abi.NODE_PATH, pid))
@@ -260,6 +265,7 @@ class Scene:
             Q.stroke_cap, Q.stroke_join = p["cap"], p["join"]
             b = self._path_bounds(p) or (0.0, 0.0, 0.0, 0.0)
             Q.bounds = abi.Rect(b[0], b[1], f32(b[2] - b[0]), f32(b[3] - b[1]))
+            Q.dash_offset, Q.dash_count, Q.dash_phase = p["dash_offset"], p["dash_count"], p["dash_phase"]
         ch = abi.array(abi.NodeRef, children)
         verbs = abi.array(C.c_uint8, self.verbs)
         pts = np.ascontiguousarray(np.array(self.points if self.points else [0.0], dtype=np.float32))
@@ -287,13 +293,15 @@ class Scene:
         sc.clip_path_count = len(self.clip_paths)
         sc.masks = C.cast(msk, C.POINTER(abi.Mask))
         sc.mask_count = len(self.masks)
+        dsh = abi.array(C.c_float, self.dashes)
+        sc.dashes = C.cast(dsh, C.POINTER(C.c_float))
 
         class _SceneABI:
             pass
 
         out = _SceneABI()
         out.abi = sc
-        out.keep = (groups, paths, ch, verbs, pts, st, filt, prims, vals, ins, cps, msk)
+        out.keep = (groups, paths, ch, verbs, pts, st, filt, prims, vals, ins, cps, msk, dsh)
         self._abi = out
         return out
 

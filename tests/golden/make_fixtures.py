@@ -57,6 +57,18 @@ OTHER = [
     "masking/mask/maskUnits=userSpaceOnUse-with-rect", "masking/mask/maskUnits=userSpaceOnUse-with-percent",
     "masking/mask/nested-objectBoundingBox", "masking/mask/mask-on-self", "masking/mask/mask-on-child",
     "masking/mask/on-group-with-transform",
+    "painting/stroke-dasharray/0-n-with-butt-caps", "painting/stroke-dasharray/0-n-with-round-caps",
+    "painting/stroke-dasharray/0-n-with-square-caps", "painting/stroke-dasharray/n-0",
+    "painting/stroke-dasharray/negative-sum", "painting/stroke-dasharray/negative-values",
+    "painting/stroke-dasharray/zero-sum", "painting/stroke-dasharray/none",
+    "painting/stroke-dasharray/comma-ws-separator", "painting/stroke-dasharray/even-count",
+    "painting/stroke-dasharray/odd-count", "painting/stroke-dasharray/ws-separator",
+    "painting/stroke-dasharray/on-a-circle", "painting/stroke-dasharray/mm-units",
+    "painting/stroke-dasharray/em-units", "painting/stroke-dasharray/percent-units",
+    "painting/stroke-dasharray/multiple-subpaths",
+    "painting/stroke-dashoffset/default", "painting/stroke-dashoffset/em-units",
+    "painting/stroke-dashoffset/mm-units", "painting/stroke-dashoffset/negative-value",
+    "painting/stroke-dashoffset/percent-units", "painting/stroke-dashoffset/px-units",
 ]
 
 

@@ -535,6 +535,52 @@ def _masking_fixtures():
     return out
 
 
+def _dash_fixtures():
+    """painting/stroke-dasharray/* and painting/stroke-dashoffset/*: usvg resolves the
+    list (a negative entry or a zero sum drops it, an odd count repeats it; mm =
+    96/25.4 px, em = font-size, % of the viewport diagonal sqrt((w^2 + h^2) / 2))."""
+    out = {}
+    M, L = abi.VERB_MOVE, abi.VERB_LINE
+    green = PaintSpec.solid(GREEN)
+    mm = 96.0 / 25.4
+
+    def rect(dash, width, cap=abi.CAP_BUTT):
+        sc = Scene(200, 200)
+        sc.path(sc.root, *rect_path(40.0, 40.0, 120.0, 120.0), stroke=green, stroke_width=width, stroke_cap=cap,
+                dasharray=dash)
+        return sc
+
+    def circle(dash, r=70.0, offset=0.0):
+        sc = Scene(200, 200)
+        sc.path(sc.root, *ellipse_path_usvg(100.0, 100.0, r, r), stroke=green, stroke_width=10.0, dasharray=dash,
+                dashoffset=offset)
+        return sc
+
+    for name, cap in (("butt", abi.CAP_BUTT), ("round", abi.CAP_ROUND), ("square", abi.CAP_SQUARE)):
+        out[f"painting/stroke-dasharray/0-n-with-{name}-caps"] = rect([0.0, 40.0], 20.0, cap)
+    out["painting/stroke-dasharray/n-0"] = rect([40.0, 0.0], 20.0)
+    for name in ("negative-sum", "negative-values", "zero-sum", "none"):  # dropped: a solid stroke
+        out[f"painting/stroke-dasharray/{name}"] = rect(None, 10.0)
+    out["painting/stroke-dasharray/comma-ws-separator"] = circle([30.0, 20.0, 10.0, 30.0, 20.0, 10.0])
+    out["painting/stroke-dasharray/even-count"] = circle([10.0, 20.0, 30.0, 40.0])
+    out["painting/stroke-dasharray/odd-count"] = circle([10.0, 20.0, 30.0, 10.0, 20.0, 30.0])
+    out["painting/stroke-dasharray/ws-separator"] = circle([10.0, 20.0])
+    out["painting/stroke-dasharray/on-a-circle"] = circle([10.0, 20.0], r=60.0)
+    out["painting/stroke-dasharray/mm-units"] = circle([5.0 * mm, 2.5 * mm])
+    out["painting/stroke-dasharray/em-units"] = circle([40.0, 20.0])
+    out["painting/stroke-dasharray/percent-units"] = circle([30.0, 60.0])
+    for name, off in (("default", 0.0), ("em-units", 30.0), ("mm-units", 1.5 * mm), ("negative-value", -15.0),
+                      ("percent-units", 40.0), ("px-units", 15.0)):
+        out[f"painting/stroke-dashoffset/{name}"] = circle([10.0, 20.0], offset=off)
+    sc = Scene(200, 200)
+    for r in ((20.0, 65.0, 160.0, 10.0), (95.0, 90.0, 10.0, 70.0)):
+        sc.path(sc.root, *rect_path(*r), stroke=PaintSpec.solid(BLACK), stroke_width=0.5)
+    sc.path(sc.root, [M, L, M, L], [(20, 70), (160, 70), (100, 90), (100, 160)], stroke=green, stroke_width=10.0,
+            dasharray=[15.0, 15.0])
+    out["painting/stroke-dasharray/multiple-subpaths"] = sc
+    return out
+
+
 def filter_scene(fx) -> Scene:
     """A filter fixture (fixture_scenes.Fixture) as a scene: a filtered group with a rect."""
     sc = Scene(200, 200)
@@ -548,6 +594,7 @@ def all_scene_fixtures():
     out = dict(shape_fixtures())
     out.update(_hairline_fixtures())
     out.update(_masking_fixtures())
+    out.update(_dash_fixtures())
     for fx in filter_fixtures():
         out[fx.name] = filter_scene(fx)
     return out
