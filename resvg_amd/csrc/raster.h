@@ -41,7 +41,9 @@ struct PieceDev {
 };
 
 constexpr int kGroupRows = 4;  // pixel rows per entry of a job's edge-window table
-constexpr uint32_t kCoverageSmallKeys = 128;  // crossings per sub-scanline of the low-LDS coverage variant
+// crossings per sub-scanline that the coverage kernel's variants hold (more keys,
+// more LDS per wave, fewer waves per CU)
+constexpr uint32_t kCoverageKeys[3] = {128, 256, 512};
 
 // A coverage job computes u8 coverage for the pixel rect [x0, x0+w) x [y0, y0+h)
 // of its pixmap (coordinates of the edge space) into cov[(y - y0) * stride + (x - x0)],

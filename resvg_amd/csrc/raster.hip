@@ -37,8 +37,7 @@ using rvkr::ShaderParams;
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kMaxKeys = 512;     // crossings per sub-scanline (8 per lane)
-constexpr int kSmallKeys = (int)rvkr::kCoverageSmallKeys;  // the low-LDS variant's limit (2 per lane)
+constexpr int kMaxKeys = (int)rvkr::kCoverageKeys[2];  // crossings per sub-scanline of the widest variant
 constexpr int kWin = 1024;        // pixels accumulated per window
 constexpr int kMaxSub = 4;        // sub-scanlines per pixel row
 
@@ -280,11 +279,11 @@ __device__ __forceinline__ void spans_of(uint64_t (*keys)[MK], const int *n, boo
     }
 }
 
-// MK: crossings per sub-scanline the launch can hold.  The host sends a row group to
-// the kSmallKeys variant (a quarter of the LDS, so about three times the waves per
-// CU) when the group's edge window, an upper bound on any of its sub-scanlines'
-// crossings, fits; otherwise to the kMaxKeys variant.  Block b handles row
-// groups[b / kGroupRows] + b % kGroupRows.
+// MK: crossings per sub-scanline the launch can hold (128, 256 or 512; the keys take
+// 4, 8 or 16 KB of LDS, so the smaller variants run two to three times the waves per
+// CU).  The host sends a row group to the smallest variant that holds the number of
+// edges whose rows meet the group, an upper bound on any of its sub-scanlines'
+// crossings.  Block b handles row groups[b / kGroupRows] + b % kGroupRows.
 template <int MK>
 __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
     __shared__ uint64_t keys[kMaxSub][MK];  // crossings; reused for span events
@@ -353,18 +352,21 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
         }
         nmax = max(nmax, cnt[s]);
     }
+    // keys per lane: the smallest network that holds the fullest sub-scanline (cnt is
+    // at most MK, so the branches past MK are never taken)
+    constexpr int K4 = MK >= 256 ? 4 : 2, K8 = MK >= 512 ? 8 : K4;
     if (nsub == 1) {
         if (nmax == 0) {}
         else if (nmax <= 64) spans_of<1, 1, MK>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 128 || MK <= 128) spans_of<2, 1, MK>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 256) spans_of<(MK > 128 ? 4 : 2), 1, MK>(keys, cnt, evenodd, nev, lane);
-        else spans_of<(MK > 128 ? 8 : 2), 1, MK>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 128) spans_of<2, 1, MK>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 256) spans_of<K4, 1, MK>(keys, cnt, evenodd, nev, lane);
+        else spans_of<K8, 1, MK>(keys, cnt, evenodd, nev, lane);
     } else {
         if (nmax == 0) {}
         else if (nmax <= 64) spans_of<1, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 128 || MK <= 128) spans_of<2, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 256) spans_of<(MK > 128 ? 4 : 2), kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
-        else spans_of<(MK > 128 ? 8 : 2), kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 128) spans_of<2, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+        else if (nmax <= 256) spans_of<K4, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+        else spans_of<K8, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
     }
     __syncthreads();
     // ---- accumulate per window, prefix-sum, clamp, write u8 ----
@@ -639,27 +641,24 @@ rvk_status launch_expand(rvk_context *ctx, const EdgeRec *edges, uint32_t nedges
     return RVK_OK;
 }
 
-rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint32_t *small_groups,
-                           uint32_t n_small, const uint32_t *big_groups, uint32_t n_big, const EdgeCov *edges,
-                           const uint2 *windows, const PieceDev *pieces, const uint32_t *npieces, uint8_t *arena,
-                           int32_t *overflow, double algo_bytes) {
-    // the algorithmic bytes are split between the two launches by their row counts
-    const double frac = n_small + n_big ? (double)n_small / (double)(n_small + n_big) : 0.0;
-    if (n_small) {
-        CovArgs a{small_groups, jobs, njobs, edges, windows, pieces, npieces, arena, overflow};
-        timed_launch(ctx, "coverage", algo_bytes * frac, [&] {
-            hipLaunchKernelGGL(k_coverage<kSmallKeys>, dim3(n_small * rvkr::kGroupRows), dim3(kWave), 0, ctx->stream,
-                               a);
+rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint32_t *groups128,
+                           uint32_t n128, const uint32_t *groups256, uint32_t n256, const uint32_t *groups512,
+                           uint32_t n512, const EdgeCov *edges, const uint2 *windows, const PieceDev *pieces,
+                           const uint32_t *npieces, uint8_t *arena, int32_t *overflow, double algo_bytes) {
+    // the algorithmic bytes are split between the launches by their row-group counts
+    const double total = (double)n128 + (double)n256 + (double)n512;
+    auto launch = [&](auto kernel, const uint32_t *groups, uint32_t n) -> rvk_status {
+        if (n == 0) return RVK_OK;
+        CovArgs a{groups, jobs, njobs, edges, windows, pieces, npieces, arena, overflow};
+        timed_launch(ctx, "coverage", algo_bytes * (double)n / total, [&] {
+            hipLaunchKernelGGL(kernel, dim3(n * rvkr::kGroupRows), dim3(kWave), 0, ctx->stream, a);
         });
         RVK_HIP_CHECK(hipGetLastError());
-    }
-    if (n_big) {
-        CovArgs a{big_groups, jobs, njobs, edges, windows, pieces, npieces, arena, overflow};
-        timed_launch(ctx, "coverage", algo_bytes * (1.0 - frac), [&] {
-            hipLaunchKernelGGL(k_coverage<kMaxKeys>, dim3(n_big * rvkr::kGroupRows), dim3(kWave), 0, ctx->stream, a);
-        });
-        RVK_HIP_CHECK(hipGetLastError());
-    }
+        return RVK_OK;
+    };
+    RVK_TRY(launch(k_coverage<(int)rvkr::kCoverageKeys[0]>, groups128, n128));
+    RVK_TRY(launch(k_coverage<(int)rvkr::kCoverageKeys[1]>, groups256, n256));
+    RVK_TRY(launch(k_coverage<(int)rvkr::kCoverageKeys[2]>, groups512, n512));
     return RVK_OK;
 }
 

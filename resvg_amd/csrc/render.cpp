@@ -29,10 +29,10 @@ using namespace rvkr;
 namespace rvk {
 rvk_status launch_expand(rvk_context *ctx, const EdgeRec *edges, uint32_t nedges, const EdgeCov *cov,
                          PieceDev *pieces, uint32_t *npieces, int32_t *overflow, double algo_bytes);
-rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint32_t *small_groups,
-                           uint32_t n_small, const uint32_t *big_groups, uint32_t n_big, const EdgeCov *edges,
-                           const uint2 *windows, const PieceDev *pieces, const uint32_t *npieces, uint8_t *arena,
-                           int32_t *overflow, double algo_bytes);
+rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint32_t *groups128,
+                           uint32_t n128, const uint32_t *groups256, uint32_t n256, const uint32_t *groups512,
+                           uint32_t n512, const EdgeCov *edges, const uint2 *windows, const PieceDev *pieces,
+                           const uint32_t *npieces, uint8_t *arena, int32_t *overflow, double algo_bytes);
 rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
                              int32_t x1, int32_t y1, const DrawItem *items, uint32_t nitems, const CovJob *jobs,
                              const uint8_t *arena, const ShaderParams *shaders, const HairJob *hair_jobs,
@@ -60,6 +60,7 @@ struct DrawList {
     std::vector<EdgeRec> edges;       // every job's edges, each job's sorted by first row
     std::vector<EdgeCov> ecov;        // the coverage kernel's view of the same edges
     std::vector<uint2> windows;       // per job and row group: [lo, hi) of the job's edges
+    std::vector<uint32_t> active;     // per job and row group: edges whose rows meet the group
     uint64_t piece_slots = 0;         // piece capacity of all edges
     std::vector<ShaderParams> shaders;
     std::vector<HairJob> hair_jobs;   // hairline strokes
@@ -271,6 +272,20 @@ struct Renderer {
             while (hi < j.edge_count && dl.ecov[j.edge_offset + hi].first_y <= s1) hi++;
             dl.windows.push_back(uint2{lo, std::max(lo, hi)});
         }
+        // edges whose row range meets each group (difference array): an edge crosses a
+        // sub-scanline at most once, so this bounds every sub-scanline's crossings
+        std::vector<int32_t> diff((size_t)groups + 1, 0);
+        for (size_t i = 0; i < er.size(); i++) {
+            const int64_t fr = (int64_t)(er[i].first_y >> shift) - j.y0, lr = (int64_t)(ylast[i] >> shift) - j.y0;
+            if (lr < 0 || fr >= j.h) continue;
+            const int64_t ga = std::max<int64_t>(fr, 0) / kGroupRows, gb = std::min<int64_t>(lr, j.h - 1) / kGroupRows;
+            diff[ga] += 1;
+            diff[gb + 1] -= 1;
+        }
+        for (int32_t g = 0, run = 0; g < groups; g++) {
+            run += diff[g];
+            dl.active.push_back((uint32_t)run);
+        }
         const uint64_t stride = ((uint64_t)j.w + 3) & ~3ull;
         j.cov_offset = dl.arena;
         dl.arena += (stride * (uint64_t)j.h + 255) & ~255ull;
@@ -432,13 +447,13 @@ struct Renderer {
         // device buffers are reused only by later work on the same stream.
         // row groups by the variant of the coverage kernel that handles them: a
         // group's edge window bounds the crossings of any of its sub-scanlines
-        std::vector<uint32_t> groups_small, groups_big;
+        std::vector<uint32_t> groups_tier[3];
         for (const CovJob &j : dl.jobs) {
             const int32_t ng = (j.h + kGroupRows - 1) / kGroupRows;
             for (int32_t g = 0; g < ng; g++) {
-                const uint2 wnd = dl.windows[j.window_offset + g];
-                (wnd.y - wnd.x <= kCoverageSmallKeys ? groups_small : groups_big)
-                    .push_back(j.row_start + (uint32_t)(g * kGroupRows));
+                const uint32_t n = dl.active[j.window_offset + g];
+                const int tier = n <= kCoverageKeys[0] ? 0 : n <= kCoverageKeys[1] ? 1 : 2;
+                groups_tier[tier].push_back(j.row_start + (uint32_t)(g * kGroupRows));
             }
         }
         auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
@@ -452,9 +467,11 @@ struct Renderer {
         const size_t o_ht = al(o_hj + dl.hair_jobs.size() * sizeof(HairJob));
         const size_t o_hb = al(o_ht + dl.hair_tiles.size() * 4);
         const size_t o_hl = al(o_hb + dl.hair_bins.size() * 4);
-        const size_t o_gs = al(o_hl + dl.hair_lines.size() * sizeof(HairLine));
-        const size_t o_gb = al(o_gs + groups_small.size() * 4);
-        const size_t meta = al(o_gb + groups_big.size() * 4);
+        size_t o_grp[3];
+        o_grp[0] = al(o_hl + dl.hair_lines.size() * sizeof(HairLine));
+        o_grp[1] = al(o_grp[0] + groups_tier[0].size() * 4);
+        o_grp[2] = al(o_grp[1] + groups_tier[1].size() * 4);
+        const size_t meta = al(o_grp[2] + groups_tier[2].size() * 4);
         const size_t o_np = meta;
         const size_t o_pc = al(o_np + dl.edges.size() * 4);
         const size_t o_arena = al(o_pc + (size_t)dl.piece_slots * sizeof(PieceDev));
@@ -473,8 +490,7 @@ struct Renderer {
         put(o_ht, dl.hair_tiles.data(), dl.hair_tiles.size() * 4);
         put(o_hb, dl.hair_bins.data(), dl.hair_bins.size() * 4);
         put(o_hl, dl.hair_lines.data(), dl.hair_lines.size() * sizeof(HairLine));
-        put(o_gs, groups_small.data(), groups_small.size() * 4);
-        put(o_gb, groups_big.data(), groups_big.size() * 4);
+        for (int t = 0; t < 3; t++) put(o_grp[t], groups_tier[t].data(), groups_tier[t].size() * 4);
         void *dev = nullptr;
         RVK_TRY(rvk::scratch_alloc(ctx, o_arena + dl.arena, &dev));
         uint8_t *d = (uint8_t *)dev;
@@ -500,8 +516,9 @@ struct Renderer {
                                     dev_overflow, exp_bytes);
         if (st == RVK_OK)
             st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)dl.jobs.size(),
-                                      (const uint32_t *)(d + o_gs), (uint32_t)groups_small.size(),
-                                      (const uint32_t *)(d + o_gb), (uint32_t)groups_big.size(),
+                                      (const uint32_t *)(d + o_grp[0]), (uint32_t)groups_tier[0].size(),
+                                      (const uint32_t *)(d + o_grp[1]), (uint32_t)groups_tier[1].size(),
+                                      (const uint32_t *)(d + o_grp[2]), (uint32_t)groups_tier[2].size(),
                                       (const EdgeCov *)(d + o_ecov), (const uint2 *)(d + o_win),
                                       (const PieceDev *)(d + o_pc), (const uint32_t *)(d + o_np), d + o_arena,
                                       dev_overflow, cov_bytes);
