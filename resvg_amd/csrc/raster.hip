@@ -566,10 +566,22 @@ struct DrawArgs {
 constexpr int kWaveSide = 8;
 constexpr int TX = 2 * kWaveSide, TY = 2 * kWaveSide;
 
+// Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.  Block b is
+// mapped to a tile so that each XCD gets runs of kChunk horizontally adjacent
+// row-major tiles (which read the same coverage and pixel cache lines), the runs
+// interleaved over the draw rect so the XCDs share the work evenly; tiles past the
+// last whole round of runs keep their own index.
+constexpr uint32_t kXcds = 8, kChunk = 8;
+
 __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t h, int32_t ox, int32_t oy,
-                                              DrawArgs a) {
+                                              uint32_t tiles_x, DrawArgs a) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t round = kXcds * kChunk, full = gridDim.x / round * round;
+    const uint32_t xcd = b % kXcds, k = b / kXcds;
+    const uint32_t tile = b < full ? (k / kChunk) * round + xcd * kChunk + k % kChunk : b;
+    const uint32_t bx = tile % tiles_x, by = tile / tiles_x;
     const int lane = (int)(threadIdx.x % 64), wv = (int)(threadIdx.x / 64);
-    const int32_t tx0 = ox + (int32_t)(blockIdx.x * TX), ty0 = oy + (int32_t)(blockIdx.y * TY);
+    const int32_t tx0 = ox + (int32_t)(bx * TX), ty0 = oy + (int32_t)(by * TY);
     const int32_t wx0 = tx0 + (wv & 1) * kWaveSide, wy0 = ty0 + (wv >> 1) * kWaveSide;
     const int32_t x = wx0 + lane % kWaveSide, y = wy0 + lane / kWaveSide;
     const bool inside = x < (int32_t)w && y < (int32_t)h;
@@ -668,11 +680,12 @@ rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint
                              const uint32_t *hair_tiles, const uint32_t *hair_bins, const HairLine *hair_lines,
                              double algo_bytes) {
     if (nitems == 0 || x1 <= x0 || y1 <= y0) return RVK_OK;
-    dim3 grid(div_up((uint32_t)(x1 - x0), TX), div_up((uint32_t)(y1 - y0), TY));
+    const uint32_t tiles_x = div_up((uint32_t)(x1 - x0), TX), tiles_y = div_up((uint32_t)(y1 - y0), TY);
+    const dim3 grid(tiles_x * tiles_y);
     const DrawArgs a{items, nitems, jobs, arena, shaders, hair_jobs, hair_tiles, hair_bins, hair_lines};
     timed_launch(ctx, "draw_items", algo_bytes, [&] {
         // four 8 x 8 waves per 16 x 16 tile
-        hipLaunchKernelGGL(k_draw, grid, dim3(256), 0, ctx->stream, (uint32_t *)target, w, h, x0, y0, a);
+        hipLaunchKernelGGL(k_draw, grid, dim3(256), 0, ctx->stream, (uint32_t *)target, w, h, x0, y0, tiles_x, a);
     });
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;
