@@ -11,8 +11,10 @@ The scene tree is built before the timed region (usvg's job); host-side edge
 building, uploads and every kernel are inside it.
 
 Prints one JSON line: Mpixel/s of the document, the roofline of the dominant kernel
-(algorithmic bytes / its HIP-event time over the timed steps, against 8 TB/s HBM)
-and the oracle's CPU throughput on a bounded sample of the same document.
+(algorithmic bytes / its HIP-event time over the timed steps, against 8 TB/s HBM),
+the oracle's CPU throughput on a bounded sample of the same document, and under
+"batched" the documents/s of rvk_render_batch against a loop of rvk_render calls on a
+sample of the batched workload.
 """
 from __future__ import annotations
 
@@ -103,6 +105,50 @@ def cpu_baseline(size, paths, seed, workers, plain_stride=64, filter_group=1, n_
     }
 
 
+def batched(ctx, n_docs, seed, steps):
+    """The batched form: `n_docs` documents of the batched distribution (sides
+    log-uniform 64-2048, aspect 0.5-2, 5-200 paths, a 1-6 primitive filter chain each)
+    rendered by one rvk_render_batch call, against the same documents rendered by a
+    loop of rvk_render calls (each ends with its own wait).  Documents are built before
+    the timed region; both forms composite onto the same resident device pixmaps."""
+    import torch
+    from resvg_amd import abi
+    from resvg_amd.workloads import document_batch
+    t0 = time.perf_counter()
+    docs = document_batch(n_docs, seed=seed)
+    scenes = [sc.to_abi() for sc, _ in docs]
+    pxs = [torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda") for _, (w, h) in docs]
+    print(f"[bench] batched: {n_docs} documents built in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    ts = abi.Transform.identity()
+
+    def loop():
+        for sc, px in zip(scenes, pxs):
+            ctx.render(sc, ts, px)
+
+    def batch():
+        ctx.render_batch(scenes, ts, pxs)
+
+    out = {}
+    for name, fn in (("loop", loop), ("batch", batch)):
+        fn()  # warm-up: scratch buffers and staging reach their working sizes
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t) / steps
+        print(f"[bench] batched: {name} {out[name] * 1e3:.1f} ms per pass", file=sys.stderr, flush=True)
+    mpix = sum(w * h for _, (w, h) in docs) / 1e6
+    return {"docs": n_docs, "seed": seed, "mpixels": round(mpix, 2),
+            "paths": sum(len(sc.paths) for sc, _ in docs),
+            "docs_per_s": round(n_docs / out["batch"], 1), "loop_docs_per_s": round(n_docs / out["loop"], 1),
+            "speedup_vs_loop": round(out["loop"] / out["batch"], 3),
+            "mpixel_per_s": round(mpix / out["batch"], 2), "ms_per_pass": round(out["batch"] * 1e3, 2),
+            "loop_ms_per_pass": round(out["loop"] * 1e3, 2), "passes": steps,
+            "sample": f"{n_docs} documents of the 4096-document batched distribution (seed {seed})"}
+
+
 def _primitive_rooflines(kernels, steps):
     """Per filter primitive / pipeline stage: its kernels' time per step and the
     achieved fraction of HBM peak on the primitive's algorithmic bytes."""
@@ -134,6 +180,8 @@ def main() -> int:
     ap.add_argument("--cpu-stride", type=int, default=64, help="CPU sample: every n-th plain path")
     ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline processes (0: min(16, cores))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-docs", type=int, default=512, help="documents of the batched leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 1 GiB device buffer once (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -190,6 +238,7 @@ def main() -> int:
     t_wall = time.perf_counter() - t_wall
     kernels = ctx.profile_end()
     st = ctx.stats()  # of the last render call
+    bat = batched(ctx, args.batch_docs, args.seed, args.batch_steps) if args.batch_docs > 0 else None
     host = {"build_ms": round(st.host_build_ms, 1), "wait_ms": round(st.host_wait_ms, 1), "flushes": st.flushes,
             "edge_piece_slots": st.edge_pieces, "upload_MB": round(st.upload_bytes / 1e6, 1),
             "kernel_launches": st.kernel_launches, "layers": st.layers}
@@ -241,6 +290,7 @@ def main() -> int:
         "primitives": _primitive_rooflines(kernels, args.steps),
         "wall_s_per_step": round(t_wall / args.steps, 3),
         "host_last_step": host,
+        "batched": bat,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -356,6 +356,17 @@ rvk_status rvk_render(rvk_context *ctx, const rvk_scene *scene, rvk_transform tr
  * renders, downloads and synchronizes. */
 rvk_status rvk_render_host(rvk_context *ctx, const rvk_scene *scene, rvk_transform transform,
                            uint32_t width, uint32_t height, uint8_t *host_pixmap);
+/* A loop of resvg::render calls (lib.rs:34-43), one per document, in one call: document i
+ * is composited onto dev_pixmaps[i] (widths[i] x heights[i]) with transforms[i].  The
+ * documents' kernels run back to back on the context stream with one wait at the end.
+ * statuses[i] (if non-null) receives each document's status; the return value is the
+ * first non-OK status, RVK_OK when every document rendered.  Arguments are checked for
+ * every document before any is drawn.  Documents are spread over up to 16 internal
+ * streams (RVK_BATCH_LANES) that start after, and are joined back into, the context
+ * stream; documents whose pixmaps overlap are drawn in order, as the loop would. */
+rvk_status rvk_render_batch(rvk_context *ctx, uint32_t count, const rvk_scene *const *scenes,
+                            const rvk_transform *transforms, const uint32_t *widths, const uint32_t *heights,
+                            uint8_t *const *dev_pixmaps, rvk_status *statuses);
 
 /* Per-call statistics of the last rvk_render / rvk_filter_apply on this context. */
 typedef struct rvk_stats {
@@ -370,6 +381,7 @@ typedef struct rvk_stats {
     uint64_t flushes;       /* draw-list submissions */
     double host_build_ms;   /* host time building edges, bands and shaders */
     double host_wait_ms;    /* host time waiting for the device */
+    double host_geom_ms;    /* part of host_build_ms: path geometry (edges, strokes), on all host threads */
 } rvk_stats;
 rvk_status rvk_get_stats(rvk_context *ctx, rvk_stats *out);
 

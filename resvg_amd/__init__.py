@@ -76,6 +76,8 @@ def _declare(L: C.CDLL) -> None:
     optional = {
         "rvk_render": ([P, C.POINTER(abi.Scene), abi.Transform, u32, u32, P], i32),
         "rvk_render_host": ([P, C.POINTER(abi.Scene), abi.Transform, u32, u32, P], i32),
+        "rvk_render_batch": ([P, u32, C.POINTER(C.POINTER(abi.Scene)), C.POINTER(abi.Transform), C.POINTER(u32),
+                              C.POINTER(u32), C.POINTER(C.c_void_p), C.POINTER(i32)], i32),
     }
     for name, (args, res) in {**sigs, **optional}.items():
         fn = getattr(L, name, None)
@@ -256,6 +258,27 @@ class Context:
         w, h = _dims(pixmap)
         sc = scene.to_abi() if hasattr(scene, "to_abi") else scene
         _check(lib().rvk_render(self._h, C.byref(sc.abi), ts, w, h, _ptr(pixmap)), "rvk_render")
+
+    def render_batch(self, scenes, transforms, pixmaps, check: bool = True) -> list[int]:
+        """Several resvg::render calls in one: scenes[i] onto the (H, W, 4) uint8 device
+        tensor pixmaps[i] with transforms[i] (one Transform for all, or a list).  Returns
+        the per-document statuses; raises on the first failure when `check`."""
+        n = len(scenes)
+        if isinstance(transforms, abi.Transform):
+            transforms = [transforms] * n
+        assert len(transforms) == n and len(pixmaps) == n
+        abis = [sc.to_abi() if hasattr(sc, "to_abi") else sc for sc in scenes]
+        dims = [_dims(px) for px in pixmaps]
+        sc_arr = (C.POINTER(abi.Scene) * n)(*[C.pointer(a.abi) for a in abis])
+        ts_arr = (abi.Transform * n)(*transforms)
+        w_arr = (C.c_uint32 * n)(*[d[0] for d in dims])
+        h_arr = (C.c_uint32 * n)(*[d[1] for d in dims])
+        px_arr = (C.c_void_p * n)(*[_ptr(px) for px in pixmaps])
+        st_arr = (C.c_int32 * n)()
+        status = lib().rvk_render_batch(self._h, n, sc_arr, ts_arr, w_arr, h_arr, px_arr, st_arr)
+        if check:
+            _check(status, "rvk_render_batch")
+        return list(st_arr)
 
     def render_host(self, scene, ts: abi.Transform, pixmap_np):
         """resvg_render twin: host numpy (H, W, 4) uint8 in and out."""

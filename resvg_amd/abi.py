@@ -214,7 +214,8 @@ class Stats(C.Structure):
     _fields_ = [("kernel_launches", C.c_uint64), ("layers", C.c_uint64), ("paths", C.c_uint64),
                 ("filters", C.c_uint64), ("scratch_bytes", C.c_uint64), ("edge_pieces", C.c_uint64),
                 ("band_entries", C.c_uint64), ("upload_bytes", C.c_uint64), ("flushes", C.c_uint64),
-                ("host_build_ms", C.c_double), ("host_wait_ms", C.c_double)]
+                ("host_build_ms", C.c_double), ("host_wait_ms", C.c_double),
+                ("host_geom_ms", C.c_double)]
 
 
 class KernelStat(C.Structure):

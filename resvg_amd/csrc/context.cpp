@@ -97,6 +97,46 @@ void scratch_release(rvk_context *ctx, void *p) {
     }
 }
 
+rvk_status lanes_prepare(rvk_context *ctx, uint32_t n) {
+    while (ctx->lanes.size() < n) {
+        rvk_context *l = new rvk_context();
+        l->device = ctx->device;
+        if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            delete l;
+            return RVK_ERR_HIP;
+        }
+        l->own_stream = true;
+        ctx->lanes.push_back(l);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        ctx->lanes[i]->stats = rvk_stats{};
+        ctx->lanes[i]->profiling = ctx->profiling;
+    }
+    return RVK_OK;
+}
+
+void lanes_collect(rvk_context *ctx, uint32_t n) {
+    for (uint32_t i = 0; i < n && i < ctx->lanes.size(); i++) {
+        rvk_context *l = ctx->lanes[i];
+        rvk_stats &a = ctx->stats;
+        const rvk_stats &b = l->stats;
+        a.kernel_launches += b.kernel_launches;
+        a.layers += b.layers;
+        a.paths += b.paths;
+        a.filters += b.filters;
+        a.scratch_bytes += b.scratch_bytes;
+        a.edge_pieces += b.edge_pieces;
+        a.band_entries += b.band_entries;
+        a.upload_bytes += b.upload_bytes;
+        a.flushes += b.flushes;
+        a.host_geom_ms += b.host_geom_ms;  // host_wait_ms of a lane is its own thread's
+        for (auto &r : l->prof) ctx->prof.push_back(r);
+        l->prof.clear();
+        l->profiling = false;
+    }
+}
+
 rvk_status upload_sync(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes) {
     RVK_HIP_CHECK(hipMemcpyAsync(dev_dst, host_src, bytes, hipMemcpyHostToDevice, ctx->stream));
     RVK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -190,6 +230,8 @@ rvk_status rvk_context_create(int device, rvk_context **out) {
 rvk_status rvk_context_destroy(rvk_context *ctx) {
     if (!ctx) return RVK_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
+    for (rvk_context *l : ctx->lanes) (void)rvk_context_destroy(l);
+    ctx->lanes.clear();
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &r : ctx->prof) { (void)hipEventDestroy(r.start); (void)hipEventDestroy(r.stop); }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);

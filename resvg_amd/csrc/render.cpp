@@ -19,8 +19,12 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdlib>
+#include <cstring>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <thread>
 
 using namespace rvkh;
@@ -52,21 +56,44 @@ uint64_t arena_budget() {
     return 1ull << 30;
 }
 
+// A coverage job prepared off the walk: the CovJob (offsets relative to the job),
+// its edges sorted by first row with piece slots relative to the job, the coverage
+// kernel's view of them, and per group of kGroupRows rows the edge window and the
+// coverage kernel variant.  add_job places it in a draw list.
+struct JobPrep {
+    bool ok = false;
+    CovJob j{};
+    std::vector<EdgeRec> edges;
+    std::vector<EdgeCov> ecov;
+    std::vector<uint2> windows;
+    std::vector<uint8_t> tier;       // per row group: index into kCoverageKeys
+    uint32_t tier_count[3] = {0, 0, 0};
+    uint64_t pieces = 0;             // piece slots of all edges
+    uint64_t piece_base = 0;         // first piece slot in the draw list (add_job)
+};
+
+// A hairline job prepared off the walk: per tile the offset of its bin (relative to
+// the job's bins, tiles_x * tiles_y + 1 entries), bins of line indices relative to the
+// job's lines, each tile's in drawing order.
+struct HairPrep {
+    bool ok = false;
+    HairJob j{};
+    std::vector<uint32_t> tiles;
+    std::vector<uint32_t> bins;
+    std::vector<HairLine> lines;
+    uint32_t bin_base = 0, line_base = 0;  // positions in the draw list (add_hair_job)
+};
+
 struct DrawList {
     uint8_t *px = nullptr;
     uint32_t w = 0, h = 0;
     std::vector<DrawItem> items;
-    std::vector<CovJob> jobs;
-    std::vector<EdgeRec> edges;       // every job's edges, each job's sorted by first row
-    std::vector<EdgeCov> ecov;        // the coverage kernel's view of the same edges
-    std::vector<uint2> windows;       // per job and row group: [lo, hi) of the job's edges
-    std::vector<uint32_t> active;     // per job and row group: edges whose rows meet the group
-    uint64_t piece_slots = 0;         // piece capacity of all edges
+    std::vector<JobPrep> jobs;
+    std::vector<HairPrep> hair;       // hairline strokes
     std::vector<ShaderParams> shaders;
-    std::vector<HairJob> hair_jobs;   // hairline strokes
-    std::vector<uint32_t> hair_tiles; // per job: bin offsets of its tiles (+1 end)
-    std::vector<uint32_t> hair_bins;  // line indices, each tile's in drawing order
-    std::vector<HairLine> hair_lines;
+    uint64_t n_edges = 0, n_windows = 0, piece_slots = 0;
+    uint64_t n_groups[3] = {0, 0, 0};  // row groups per coverage kernel variant
+    uint64_t n_hair_tiles = 0, n_hair_bins = 0, n_hair_lines = 0;
     uint64_t arena = 0;
     uint32_t rows = 0;
     int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN;
@@ -104,19 +131,163 @@ struct EdgeSet {
 };
 
 struct PathGeom {
-    EdgeSet fill, stroke;
+    JobPrep fill, stroke;
     bool hair = false;               // the stroke is an anti-aliased hairline
     float hair_opacity = 1.0f;       // folded into its paint
-    std::vector<HairLine> hair_lines;
+    HairPrep hair_job;
 };
 
-// Device-space geometry of one path drawn into a pw x ph pixmap: tiny-skia's
-// fill_path preconditions and edges, and (unless `fill_only`) the stroke outline's
-// edges or, for a stroke at most one device pixel wide, its hairline segments.
+// The coverage job of an edge set drawn into the pixels `clip` of its pixmap.
+void prep_job(const EdgeSet &es, uint32_t rule, bool aa, IRect clip, JobPrep &P) {
+    if (!es.ok) return;
+    const int shift = aa ? 2 : 0;
+    const std::vector<EdgeRec> &er = es.edges;
+    const std::vector<uint32_t> &caps = es.caps;
+    // last covered row of each edge: exact for lines; for curves the row of the
+    // end point plus one (the stepped points stay within a fraction of a row of
+    // the monotonic curve; the expansion kernel checks the bound)
+    std::vector<int32_t> ylast(er.size());
+    int32_t ymin = INT32_MAX, ymax = INT32_MIN;
+    for (size_t i = 0; i < er.size(); i++) {
+        const EdgeRec &e = er[i];
+        ylast[i] = e.type == 0 ? e.last_y : (((e.ly >> 10) + 32) >> 6) + 1;
+        ymin = std::min(ymin, e.first_y);
+        ymax = std::max(ymax, ylast[i]);
+    }
+    int64_t x0 = std::max<int64_t>((int64_t)std::floor(es.mnx) - 1, clip.x);
+    int64_t x1 = std::min<int64_t>((int64_t)std::ceil(es.mxx) + 2, (int64_t)clip.x + clip.w);
+    int64_t y0 = std::max<int64_t>(ymin >> shift, clip.y);
+    int64_t y1 = std::min<int64_t>((ymax >> shift) + 1, (int64_t)clip.y + clip.h);
+    if (x1 <= x0 || y1 <= y0) return;
+    CovJob &j = P.j;
+    j = CovJob{};
+    j.x0 = (int32_t)x0; j.y0 = (int32_t)y0; j.w = (int32_t)(x1 - x0); j.h = (int32_t)(y1 - y0);
+    j.rule = rule;
+    j.aa = aa ? 1 : 0;
+    // edges sorted by first row (ids keep tiny-skia's edge-list order)
+    std::vector<uint32_t> order(er.size());
+    for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return er[a].first_y < er[b].first_y; });
+    j.edge_count = (uint32_t)er.size();
+    P.edges.resize(er.size());
+    P.ecov.resize(er.size());
+    std::vector<int32_t> prefmax(er.size());
+    int32_t pm = INT32_MIN;
+    uint64_t slots = 0;
+    for (size_t k = 0; k < order.size(); k++) {
+        const uint32_t i = order[k];
+        EdgeRec e = er[i];
+        e.piece_offset = (uint32_t)slots;
+        slots += caps[i];
+        P.edges[k] = e;
+        P.ecov[k] = EdgeCov{e.first_y, ylast[i], e.winding, (int32_t)i, e.piece_offset, 0};
+        pm = std::max(pm, ylast[i]);
+        prefmax[k] = pm;
+    }
+    P.pieces = slots;
+    // per group of kGroupRows pixel rows: the edges that start at or before the
+    // group's last row, minus the leading ones that all end before its first row
+    const int32_t groups = (j.h + kGroupRows - 1) / kGroupRows;
+    P.windows.resize((size_t)groups);
+    uint32_t lo = 0, hi = 0;
+    for (int32_t g = 0; g < groups; g++) {
+        const int32_t r0 = j.y0 + g * kGroupRows, r1 = std::min(j.y0 + j.h, r0 + kGroupRows);
+        const int32_t s0 = r0 << shift, s1 = (r1 << shift) - 1;
+        while (lo < j.edge_count && prefmax[lo] < s0) lo++;
+        while (hi < j.edge_count && P.ecov[hi].first_y <= s1) hi++;
+        P.windows[(size_t)g] = uint2{lo, std::max(lo, hi)};
+    }
+    // edges whose row range meets each group (difference array): an edge crosses a
+    // sub-scanline at most once, so this bounds every sub-scanline's crossings and
+    // picks the coverage kernel variant that holds them
+    std::vector<int32_t> diff((size_t)groups + 1, 0);
+    for (size_t i = 0; i < er.size(); i++) {
+        const int64_t fr = (int64_t)(er[i].first_y >> shift) - j.y0, lr = (int64_t)(ylast[i] >> shift) - j.y0;
+        if (lr < 0 || fr >= j.h) continue;
+        const int64_t ga = std::max<int64_t>(fr, 0) / kGroupRows, gb = std::min<int64_t>(lr, j.h - 1) / kGroupRows;
+        diff[ga] += 1;
+        diff[gb + 1] -= 1;
+    }
+    P.tier.resize((size_t)groups);
+    for (int32_t g = 0, run = 0; g < groups; g++) {
+        run += diff[g];
+        const uint32_t n = (uint32_t)run;
+        const uint8_t t = n <= kCoverageKeys[0] ? 0 : n <= kCoverageKeys[1] ? 1 : 2;
+        P.tier[(size_t)g] = t;
+        P.tier_count[t]++;
+    }
+    P.ok = true;
+}
+
+// The hairline job of prepared lines, binned into tiles of the part of their pixel
+// bounds inside `clip`.
+void prep_hair(std::vector<HairLine> &&lines, IRect clip, HairPrep &P) {
+    if (lines.empty()) return;
+    struct Box { int32_t x0, y0, x1, y1; };
+    std::vector<Box> boxes(lines.size());
+    int32_t jx0 = INT32_MAX, jy0 = INT32_MAX, jx1 = INT32_MIN, jy1 = INT32_MIN;
+    for (size_t i = 0; i < lines.size(); i++) {
+        const HairLine &L = lines[i];
+        // minor pixels touched: f >> 16 and the one before, f at both ends
+        const int64_t fa = (int64_t)L.fstart + 0x8000;
+        const int64_t fb = fa + (int64_t)(L.istop - L.istart - 1) * L.slope;
+        const int32_t m0 = (int32_t)(std::min(fa, fb) >> 16) - 1, m1 = (int32_t)(std::max(fa, fb) >> 16) + 1;
+        Box b = (L.flags & kHairYMajor) ? Box{m0, L.istart, m1, L.istop} : Box{L.istart, m0, L.istop, m1};
+        if (L.flags & kHairClipped) {
+            b.x0 = std::max(b.x0, L.cl); b.y0 = std::max(b.y0, L.ct);
+            b.x1 = std::min(b.x1, L.cr); b.y1 = std::min(b.y1, L.cb);
+        }
+        b.x0 = std::max(b.x0, clip.x); b.y0 = std::max(b.y0, clip.y);
+        b.x1 = std::min(b.x1, clip.x + clip.w); b.y1 = std::min(b.y1, clip.y + clip.h);
+        boxes[i] = b;
+        if (b.x0 >= b.x1 || b.y0 >= b.y1) continue;
+        jx0 = std::min(jx0, b.x0); jy0 = std::min(jy0, b.y0);
+        jx1 = std::max(jx1, b.x1); jy1 = std::max(jy1, b.y1);
+    }
+    if (jx0 >= jx1 || jy0 >= jy1) return;
+    HairJob &j = P.j;
+    j = HairJob{};
+    j.x0 = jx0; j.y0 = jy0; j.w = jx1 - jx0; j.h = jy1 - jy0;
+    j.tiles_x = (uint32_t)((j.w + kHairTile - 1) / kHairTile);
+    j.tiles_y = (uint32_t)((j.h + kHairTile - 1) / kHairTile);
+    const size_t nt = (size_t)j.tiles_x * j.tiles_y;
+    std::vector<uint32_t> count(nt + 1, 0);
+    auto tiles_of = [&](const Box &b, auto &&fn) {
+        const int32_t tx0 = (b.x0 - j.x0) / kHairTile, tx1 = (b.x1 - 1 - j.x0) / kHairTile;
+        const int32_t ty0 = (b.y0 - j.y0) / kHairTile, ty1 = (b.y1 - 1 - j.y0) / kHairTile;
+        for (int32_t ty = ty0; ty <= ty1; ty++)
+            for (int32_t tx = tx0; tx <= tx1; tx++) fn((size_t)ty * j.tiles_x + tx);
+    };
+    for (const Box &b : boxes)
+        if (b.x0 < b.x1 && b.y0 < b.y1) tiles_of(b, [&](size_t t) { count[t]++; });
+    P.tiles.resize(nt + 1);
+    uint32_t run = 0;
+    for (size_t t = 0; t < nt; t++) {
+        P.tiles[t] = run;
+        const uint32_t c = count[t];
+        count[t] = run;  // becomes the fill cursor
+        run += c;
+    }
+    P.tiles[nt] = run;
+    P.bins.resize(run);
+    for (size_t i = 0; i < lines.size(); i++) {
+        const Box &b = boxes[i];
+        if (b.x0 < b.x1 && b.y0 < b.y1) tiles_of(b, [&](size_t t) { P.bins[count[t]++] = (uint32_t)i; });
+    }
+    P.lines = std::move(lines);
+    P.ok = true;
+}
+
+// Device-space geometry of one path drawn into a pw x ph pixmap whose visible
+// pixels are `clip`: tiny-skia's fill_path preconditions and edges, and (unless
+// `fill_only`) the stroke outline's edges or, for a stroke at most one device pixel
+// wide, its hairline segments, each prepared as a job.
 void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, uint32_t pw, uint32_t ph, bool fill_only,
-                  PathGeom &g) {
-    auto finish = [&](std::vector<Seg> &segs, EdgeSet &es) {
+                  IRect clip, PathGeom &g) {
+    auto finish = [&](std::vector<Seg> &segs, uint32_t rule, JobPrep &out) {
         if (segs.empty() || bounds_nearly_zero(segs)) return;
+        EdgeSet es;
         float mnx = INFINITY, mxx = -INFINITY;
         for (const Seg &s : segs)
             for (int i = 0; i < s.n; i++) { mnx = std::min(mnx, s.p[i].x); mxx = std::max(mxx, s.p[i].x); }
@@ -125,24 +296,27 @@ void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, uint32_t pw, ui
         es.mnx = mnx;
         es.mxx = mxx;
         es.ok = !es.edges.empty();
+        prep_job(es, rule, p.anti_alias != 0, clip, out);
     };
     if (!p.visible) return;
     if (p.fill.kind != RVK_PAINT_NONE && p.bounds.w != 0.0f && p.bounds.h != 0.0f) {
         std::vector<Seg> segs;
         fill_segments(sc, p, ts, segs);
-        finish(segs, g.fill);
+        finish(segs, p.fill_rule, g.fill);
     }
     if (!fill_only && p.stroke.kind != RVK_PAINT_NONE && p.stroke_width >= 0.0f) {
         float cov = 1.0f;
         if (rvks::treat_as_hairline(p.stroke_width, p.anti_alias != 0, ts.sx, ts.ky, ts.kx, ts.sy, cov)) {
             g.hair = true;
             g.hair_opacity = rvks::hairline_paint_opacity(cov);
-            hairline_lines(sc, p, ts, pw, ph, g.hair_lines);
+            std::vector<HairLine> lines;
+            hairline_lines(sc, p, ts, pw, ph, lines);
+            prep_hair(std::move(lines), clip, g.hair_job);
             return;
         }
         std::vector<Seg> segs;
         stroke_segments(sc, p, ts, segs);
-        finish(segs, g.stroke);
+        finish(segs, RVK_FILL_NONZERO, g.stroke);
     }
 }
 
@@ -158,21 +332,75 @@ unsigned host_threads() {
     return n;
 }
 
-template <class F>
-void parallel_for(size_t n, F &&fn) {
-    const unsigned t = (unsigned)std::min<size_t>(host_threads(), n);
-    if (t <= 1) {
-        for (size_t i = 0; i < n; i++) fn(i);
-        return;
+// Host worker threads shared by every context: parallel_for(n, fn) runs fn(0..n-1)
+// on the calling thread and the workers.  One parallel_for runs at a time; a call
+// made while another is running (another context on another thread) runs serially.
+class Pool {
+  public:
+    Pool() {
+        const unsigned n = host_threads();
+        for (unsigned k = 1; k < n; k++) threads_.emplace_back([this] { worker(); });
     }
-    std::atomic<size_t> next{0};
-    auto work = [&] {
-        for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
-    };
-    std::vector<std::thread> pool;
-    for (unsigned k = 1; k < t; k++) pool.emplace_back(work);
-    work();
-    for (auto &th : pool) th.join();
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_) t.join();
+    }
+    void run(size_t n, const std::function<void(size_t)> &fn) {
+        std::unique_lock<std::mutex> busy(run_mu_, std::try_to_lock);
+        if (n < 2 || threads_.empty() || !busy.owns_lock()) {
+            for (size_t i = 0; i < n; i++) fn(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            fn_ = &fn;
+            n_ = n;
+            next_.store(0);
+            active_ = (unsigned)threads_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> g(mu_);
+        done_.wait(g, [&] { return active_ == 0; });
+        fn_ = nullptr;
+    }
+
+  private:
+    void work() {
+        for (size_t i; (i = next_.fetch_add(1)) < n_;) (*fn_)(i);
+    }
+    void worker() {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> g(mu_);
+        for (;;) {
+            cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            g.unlock();
+            work();
+            g.lock();
+            if (--active_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex run_mu_, mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(size_t)> *fn_ = nullptr;
+    size_t n_ = 0;
+    std::atomic<size_t> next_{0};
+    unsigned active_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+void parallel_for(size_t n, const std::function<void(size_t)> &fn) {
+    static Pool pool;
+    pool.run(n, fn);
 }
 
 struct Renderer {
@@ -183,6 +411,7 @@ struct Renderer {
     uint64_t overflow_rows = 0;
     int32_t *dev_overflow = nullptr;  // per render: crossings or rows the kernels could not hold
     const uint64_t budget = arena_budget();
+    bool parallel_pack = true;  // flush packs large draw lists on the host thread pool
 
     // The walk runs twice: kCollect records every path drawn (with its device
     // transform), the geometry is computed in parallel, and kExecute replays the
@@ -194,6 +423,7 @@ struct Renderer {
         Ts ts;
         uint32_t pw, ph;  // size of the pixmap drawn into
         bool fill_only;
+        IRect clip;       // its visible pixels
     };
     std::vector<Task> tasks;
     std::vector<PathGeom> geoms;
@@ -204,157 +434,52 @@ struct Renderer {
         const size_t k = next_task++;
         if (k >= tasks.size()) return PathGeom{};
         if (k >= ready_end) {
+            const auto t0 = std::chrono::steady_clock::now();
             const size_t end = std::min(tasks.size(), k + kChunk);
             parallel_for(end - k, [&](size_t i) {
                 const Task &t = tasks[k + i];
-                compute_geom(*t.scene, *t.path, t.ts, t.pw, t.ph, t.fill_only, geoms[k + i]);
+                compute_geom(*t.scene, *t.path, t.ts, t.pw, t.ph, t.fill_only, t.clip, geoms[k + i]);
             });
             ready_end = end;
+            ctx->stats.host_geom_ms +=
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
         return std::move(geoms[k]);
     }
 
-    // Adds a coverage job for device-space segments, restricted to `clip` (pixels
-    // of the job's pixmap).  Returns the job index or -1 when nothing is covered.
-    int32_t add_job(DrawList &dl, const EdgeSet &es, uint32_t rule, bool aa, IRect clip) {
-        if (!es.ok) return -1;
-        const int shift = aa ? 2 : 0;
-        const std::vector<EdgeRec> &er = es.edges;
-        const std::vector<uint32_t> &caps = es.caps;
-        // last covered row of each edge: exact for lines; for curves the row of the
-        // end point plus one (the stepped points stay within a fraction of a row of
-        // the monotonic curve; the expansion kernel checks the bound)
-        std::vector<int32_t> ylast(er.size());
-        int32_t ymin = INT32_MAX, ymax = INT32_MIN;
-        for (size_t i = 0; i < er.size(); i++) {
-            const EdgeRec &e = er[i];
-            ylast[i] = e.type == 0 ? e.last_y : (((e.ly >> 10) + 32) >> 6) + 1;
-            ymin = std::min(ymin, e.first_y);
-            ymax = std::max(ymax, ylast[i]);
-        }
-        int64_t x0 = std::max<int64_t>((int64_t)std::floor(es.mnx) - 1, clip.x);
-        int64_t x1 = std::min<int64_t>((int64_t)std::ceil(es.mxx) + 2, (int64_t)clip.x + clip.w);
-        int64_t y0 = std::max<int64_t>(ymin >> shift, clip.y);
-        int64_t y1 = std::min<int64_t>((ymax >> shift) + 1, (int64_t)clip.y + clip.h);
-        if (x1 <= x0 || y1 <= y0) return -1;
-        CovJob j{};
-        j.x0 = (int32_t)x0; j.y0 = (int32_t)y0; j.w = (int32_t)(x1 - x0); j.h = (int32_t)(y1 - y0);
-        j.rule = rule;
-        j.aa = aa ? 1 : 0;
-        // edges sorted by first row (ids keep tiny-skia's edge-list order)
-        std::vector<uint32_t> order(er.size());
-        for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](uint32_t a, uint32_t b) { return er[a].first_y < er[b].first_y; });
-        j.edge_offset = (uint32_t)dl.edges.size();
-        j.edge_count = (uint32_t)er.size();
-        std::vector<int32_t> prefmax(er.size());
-        int32_t pm = INT32_MIN;
-        for (size_t k = 0; k < order.size(); k++) {
-            const uint32_t i = order[k];
-            EdgeRec e = er[i];
-            e.piece_offset = (uint32_t)dl.piece_slots;
-            dl.piece_slots += caps[i];
-            dl.edges.push_back(e);
-            dl.ecov.push_back(EdgeCov{e.first_y, ylast[i], e.winding, (int32_t)i, e.piece_offset, 0});
-            pm = std::max(pm, ylast[i]);
-            prefmax[k] = pm;
-        }
-        // per group of kGroupRows pixel rows: the edges that start at or before the
-        // group's last row, minus the leading ones that all end before its first row
-        j.window_offset = (uint32_t)dl.windows.size();
-        const int32_t groups = (j.h + kGroupRows - 1) / kGroupRows;
-        uint32_t lo = 0, hi = 0;
-        for (int32_t g = 0; g < groups; g++) {
-            const int32_t r0 = j.y0 + g * kGroupRows, r1 = std::min(j.y0 + j.h, r0 + kGroupRows);
-            const int32_t s0 = r0 << shift, s1 = (r1 << shift) - 1;
-            while (lo < j.edge_count && prefmax[lo] < s0) lo++;
-            while (hi < j.edge_count && dl.ecov[j.edge_offset + hi].first_y <= s1) hi++;
-            dl.windows.push_back(uint2{lo, std::max(lo, hi)});
-        }
-        // edges whose row range meets each group (difference array): an edge crosses a
-        // sub-scanline at most once, so this bounds every sub-scanline's crossings
-        std::vector<int32_t> diff((size_t)groups + 1, 0);
-        for (size_t i = 0; i < er.size(); i++) {
-            const int64_t fr = (int64_t)(er[i].first_y >> shift) - j.y0, lr = (int64_t)(ylast[i] >> shift) - j.y0;
-            if (lr < 0 || fr >= j.h) continue;
-            const int64_t ga = std::max<int64_t>(fr, 0) / kGroupRows, gb = std::min<int64_t>(lr, j.h - 1) / kGroupRows;
-            diff[ga] += 1;
-            diff[gb + 1] -= 1;
-        }
-        for (int32_t g = 0, run = 0; g < groups; g++) {
-            run += diff[g];
-            dl.active.push_back((uint32_t)run);
-        }
+    // Places a prepared coverage job in the draw list; returns its index or -1 when
+    // it covers nothing.
+    int32_t add_job(DrawList &dl, JobPrep &P) {
+        if (!P.ok) return -1;
+        CovJob &j = P.j;
+        j.edge_offset = (uint32_t)dl.n_edges;
+        dl.n_edges += j.edge_count;
+        j.window_offset = (uint32_t)dl.n_windows;
+        dl.n_windows += P.windows.size();
+        P.piece_base = dl.piece_slots;
+        dl.piece_slots += P.pieces;
+        for (int t = 0; t < 3; t++) dl.n_groups[t] += P.tier_count[t];
         const uint64_t stride = ((uint64_t)j.w + 3) & ~3ull;
         j.cov_offset = dl.arena;
         dl.arena += (stride * (uint64_t)j.h + 255) & ~255ull;
         j.row_start = dl.rows;
         dl.rows += (uint32_t)j.h;
-        dl.jobs.push_back(j);
+        dl.jobs.push_back(std::move(P));
         return (int32_t)dl.jobs.size() - 1;
     }
 
-    // Adds a hairline job for prepared lines, binned into tiles of the part of their
-    // pixel bounds inside `clip`.  Returns the job index or -1 when nothing is drawn.
-    int32_t add_hair_job(DrawList &dl, const std::vector<HairLine> &lines, IRect clip) {
-        if (lines.empty()) return -1;
-        struct Box { int32_t x0, y0, x1, y1; };
-        std::vector<Box> boxes(lines.size());
-        int32_t jx0 = INT32_MAX, jy0 = INT32_MAX, jx1 = INT32_MIN, jy1 = INT32_MIN;
-        for (size_t i = 0; i < lines.size(); i++) {
-            const HairLine &L = lines[i];
-            // minor pixels touched: f >> 16 and the one before, f at both ends
-            const int64_t fa = (int64_t)L.fstart + 0x8000;
-            const int64_t fb = fa + (int64_t)(L.istop - L.istart - 1) * L.slope;
-            const int32_t m0 = (int32_t)(std::min(fa, fb) >> 16) - 1, m1 = (int32_t)(std::max(fa, fb) >> 16) + 1;
-            Box b = (L.flags & kHairYMajor) ? Box{m0, L.istart, m1, L.istop} : Box{L.istart, m0, L.istop, m1};
-            if (L.flags & kHairClipped) {
-                b.x0 = std::max(b.x0, L.cl); b.y0 = std::max(b.y0, L.ct);
-                b.x1 = std::min(b.x1, L.cr); b.y1 = std::min(b.y1, L.cb);
-            }
-            b.x0 = std::max(b.x0, clip.x); b.y0 = std::max(b.y0, clip.y);
-            b.x1 = std::min(b.x1, clip.x + clip.w); b.y1 = std::min(b.y1, clip.y + clip.h);
-            boxes[i] = b;
-            if (b.x0 >= b.x1 || b.y0 >= b.y1) continue;
-            jx0 = std::min(jx0, b.x0); jy0 = std::min(jy0, b.y0);
-            jx1 = std::max(jx1, b.x1); jy1 = std::max(jy1, b.y1);
-        }
-        if (jx0 >= jx1 || jy0 >= jy1) return -1;
-        HairJob j{};
-        j.x0 = jx0; j.y0 = jy0; j.w = jx1 - jx0; j.h = jy1 - jy0;
-        j.tiles_x = (uint32_t)((j.w + kHairTile - 1) / kHairTile);
-        j.tiles_y = (uint32_t)((j.h + kHairTile - 1) / kHairTile);
-        const size_t nt = (size_t)j.tiles_x * j.tiles_y;
-        std::vector<uint32_t> count(nt + 1, 0);
-        auto tiles_of = [&](const Box &b, auto &&fn) {
-            const int32_t tx0 = (b.x0 - j.x0) / kHairTile, tx1 = (b.x1 - 1 - j.x0) / kHairTile;
-            const int32_t ty0 = (b.y0 - j.y0) / kHairTile, ty1 = (b.y1 - 1 - j.y0) / kHairTile;
-            for (int32_t ty = ty0; ty <= ty1; ty++)
-                for (int32_t tx = tx0; tx <= tx1; tx++) fn((size_t)ty * j.tiles_x + tx);
-        };
-        for (const Box &b : boxes)
-            if (b.x0 < b.x1 && b.y0 < b.y1) tiles_of(b, [&](size_t t) { count[t]++; });
-        j.tile_offset = (uint32_t)dl.hair_tiles.size();
-        const uint32_t base = (uint32_t)dl.hair_bins.size();
-        uint32_t run = base;
-        for (size_t t = 0; t < nt; t++) {
-            dl.hair_tiles.push_back(run);
-            const uint32_t c = count[t];
-            count[t] = run;  // becomes the fill cursor
-            run += c;
-        }
-        dl.hair_tiles.push_back(run);
-        dl.hair_bins.resize(run);
-        const uint32_t line_base = (uint32_t)dl.hair_lines.size();
-        for (size_t i = 0; i < lines.size(); i++) {
-            const Box &b = boxes[i];
-            if (b.x0 < b.x1 && b.y0 < b.y1)
-                tiles_of(b, [&](size_t t) { dl.hair_bins[count[t]++] = line_base + (uint32_t)i; });
-        }
-        dl.hair_lines.insert(dl.hair_lines.end(), lines.begin(), lines.end());
-        dl.hair_jobs.push_back(j);
-        return (int32_t)dl.hair_jobs.size() - 1;
+    // Places a prepared hairline job in the draw list; returns its index or -1 when
+    // nothing is drawn.
+    int32_t add_hair_job(DrawList &dl, HairPrep &P) {
+        if (!P.ok) return -1;
+        P.j.tile_offset = (uint32_t)dl.n_hair_tiles;
+        dl.n_hair_tiles += P.tiles.size();
+        P.bin_base = (uint32_t)dl.n_hair_bins;
+        dl.n_hair_bins += P.bins.size();
+        P.line_base = (uint32_t)dl.n_hair_lines;
+        dl.n_hair_lines += P.lines.size();
+        dl.hair.push_back(std::move(P));
+        return (int32_t)dl.hair.size() - 1;
     }
 
     // fill / stroke coverage jobs + shaders of one path drawn with `ts` into a
@@ -369,19 +494,19 @@ struct Renderer {
         it.stroke_hair = 0;
         it.stroke_first = p.paint_order_stroke_first ? 1 : 0;
         if (mode == kCollect) {
-            tasks.push_back(Task{&S, &p, ts, pw, ph, fill_only});
+            tasks.push_back(Task{&S, &p, ts, pw, ph, fill_only, clip});
             return;
         }
         if (mode == kSkip) {
             next_task++;
             return;
         }
-        const PathGeom g = take_geom();
+        PathGeom g = take_geom();
         if (!p.visible) return;
         ShaderParams sp;
         // path::fill_path (path.rs:26-75)
         if (g.fill.ok && make_shader(S, p.fill, ts, sp)) {
-            it.fill_job = add_job(dl, g.fill, p.fill_rule, p.anti_alias != 0, clip);
+            it.fill_job = add_job(dl, g.fill);
             if (it.fill_job >= 0) {
                 sp.clear = clear ? 1 : 0;
                 it.fill_shader = (int32_t)dl.shaders.size();
@@ -393,7 +518,7 @@ struct Renderer {
         // filled nonzero
         if (g.hair) {
             if (make_shader(S, p.stroke, ts, sp, g.hair_opacity)) {
-                it.stroke_job = add_hair_job(dl, g.hair_lines, clip);
+                it.stroke_job = add_hair_job(dl, g.hair_job);
                 if (it.stroke_job >= 0) {
                     it.stroke_hair = 1;
                     it.stroke_shader = (int32_t)dl.shaders.size();
@@ -401,7 +526,7 @@ struct Renderer {
                 }
             }
         } else if (g.stroke.ok && make_shader(S, p.stroke, ts, sp)) {
-            it.stroke_job = add_job(dl, g.stroke, RVK_FILL_NONZERO, p.anti_alias != 0, clip);
+            it.stroke_job = add_job(dl, g.stroke);
             if (it.stroke_job >= 0) {
                 it.stroke_shader = (int32_t)dl.shaders.size();
                 dl.shaders.push_back(sp);
@@ -418,10 +543,10 @@ struct Renderer {
             if (j < 0) continue;
             int32_t jx, jy, jw, jh;
             if (k == 1 && it.stroke_hair) {
-                const HairJob &hj = dl.hair_jobs[j];
+                const HairJob &hj = dl.hair[j].j;
                 jx = hj.x0; jy = hj.y0; jw = hj.w; jh = hj.h;
             } else {
-                const CovJob &jb = dl.jobs[j];
+                const CovJob &jb = dl.jobs[j].j;
                 jx = jb.x0; jy = jb.y0; jw = jb.w; jh = jb.h;
             }
             x0 = std::min(x0, jx + it.lx); y0 = std::min(y0, jy + it.ly);
@@ -441,56 +566,87 @@ struct Renderer {
             dl = DrawList{dl.px, dl.w, dl.h};
             return RVK_OK;
         }
-        // one device block: jobs | edges | ecov | windows | shaders | items || npieces |
+        // one device block: jobs | edges | ecov | windows | shaders | items | hairline
+        // jobs, tiles, bins, lines | row groups per coverage variant || npieces |
         // pieces | arena (the part after || is written by the device).  The metadata is
-        // packed straight into a pinned staging buffer and copied without a host wait;
-        // device buffers are reused only by later work on the same stream.
-        // row groups by the variant of the coverage kernel that handles them: a
-        // group's edge window bounds the crossings of any of its sub-scanlines
-        std::vector<uint32_t> groups_tier[3];
-        for (const CovJob &j : dl.jobs) {
-            const int32_t ng = (j.h + kGroupRows - 1) / kGroupRows;
-            for (int32_t g = 0; g < ng; g++) {
-                const uint32_t n = dl.active[j.window_offset + g];
-                const int tier = n <= kCoverageKeys[0] ? 0 : n <= kCoverageKeys[1] ? 1 : 2;
-                groups_tier[tier].push_back(j.row_start + (uint32_t)(g * kGroupRows));
-            }
-        }
+        // packed straight into a pinned staging buffer (the jobs in parallel, each
+        // moved to its place in the block) and copied without a host wait; device
+        // buffers are reused only by later work on the same stream.
+        const size_t njobs = dl.jobs.size(), nhair = dl.hair.size();
         auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
         const size_t o_jobs = 0;
-        const size_t o_edges = al(o_jobs + dl.jobs.size() * sizeof(CovJob));
-        const size_t o_ecov = al(o_edges + dl.edges.size() * sizeof(EdgeRec));
-        const size_t o_win = al(o_ecov + dl.ecov.size() * sizeof(EdgeCov));
-        const size_t o_sh = al(o_win + dl.windows.size() * sizeof(uint2));
+        const size_t o_edges = al(o_jobs + njobs * sizeof(CovJob));
+        const size_t o_ecov = al(o_edges + dl.n_edges * sizeof(EdgeRec));
+        const size_t o_win = al(o_ecov + dl.n_edges * sizeof(EdgeCov));
+        const size_t o_sh = al(o_win + dl.n_windows * sizeof(uint2));
         const size_t o_it = al(o_sh + dl.shaders.size() * sizeof(ShaderParams));
         const size_t o_hj = al(o_it + dl.items.size() * sizeof(DrawItem));
-        const size_t o_ht = al(o_hj + dl.hair_jobs.size() * sizeof(HairJob));
-        const size_t o_hb = al(o_ht + dl.hair_tiles.size() * 4);
-        const size_t o_hl = al(o_hb + dl.hair_bins.size() * 4);
+        const size_t o_ht = al(o_hj + nhair * sizeof(HairJob));
+        const size_t o_hb = al(o_ht + dl.n_hair_tiles * 4);
+        const size_t o_hl = al(o_hb + dl.n_hair_bins * 4);
         size_t o_grp[3];
-        o_grp[0] = al(o_hl + dl.hair_lines.size() * sizeof(HairLine));
-        o_grp[1] = al(o_grp[0] + groups_tier[0].size() * 4);
-        o_grp[2] = al(o_grp[1] + groups_tier[1].size() * 4);
-        const size_t meta = al(o_grp[2] + groups_tier[2].size() * 4);
+        o_grp[0] = al(o_hl + dl.n_hair_lines * sizeof(HairLine));
+        o_grp[1] = al(o_grp[0] + dl.n_groups[0] * 4);
+        o_grp[2] = al(o_grp[1] + dl.n_groups[1] * 4);
+        const size_t meta = al(o_grp[2] + dl.n_groups[2] * 4);
         const size_t o_np = meta;
-        const size_t o_pc = al(o_np + dl.edges.size() * 4);
+        const size_t o_pc = al(o_np + dl.n_edges * 4);
         const size_t o_arena = al(o_pc + (size_t)dl.piece_slots * sizeof(PieceDev));
         void *hostv = nullptr;
         uint32_t slot = 0;
         RVK_TRY(rvk::staging_acquire(ctx, meta, &hostv, &slot));
         uint8_t *host = (uint8_t *)hostv;
-        auto put = [&](size_t off, const void *src, size_t n) { if (n) std::memcpy(host + off, src, n); };
-        put(o_jobs, dl.jobs.data(), dl.jobs.size() * sizeof(CovJob));
-        put(o_edges, dl.edges.data(), dl.edges.size() * sizeof(EdgeRec));
-        put(o_ecov, dl.ecov.data(), dl.ecov.size() * sizeof(EdgeCov));
-        put(o_win, dl.windows.data(), dl.windows.size() * sizeof(uint2));
-        put(o_sh, dl.shaders.data(), dl.shaders.size() * sizeof(ShaderParams));
-        put(o_it, dl.items.data(), dl.items.size() * sizeof(DrawItem));
-        put(o_hj, dl.hair_jobs.data(), dl.hair_jobs.size() * sizeof(HairJob));
-        put(o_ht, dl.hair_tiles.data(), dl.hair_tiles.size() * 4);
-        put(o_hb, dl.hair_bins.data(), dl.hair_bins.size() * 4);
-        put(o_hl, dl.hair_lines.data(), dl.hair_lines.size() * sizeof(HairLine));
-        for (int t = 0; t < 3; t++) put(o_grp[t], groups_tier[t].data(), groups_tier[t].size() * 4);
+        if (!dl.shaders.empty()) std::memcpy(host + o_sh, dl.shaders.data(), dl.shaders.size() * sizeof(ShaderParams));
+        std::memcpy(host + o_it, dl.items.data(), dl.items.size() * sizeof(DrawItem));
+        // each job's first entry in the row-group lists
+        std::vector<uint64_t> gbase(njobs * 3);
+        {
+            uint64_t run[3] = {0, 0, 0};
+            for (size_t i = 0; i < njobs; i++)
+                for (int t = 0; t < 3; t++) {
+                    gbase[i * 3 + t] = run[t];
+                    run[t] += dl.jobs[i].tier_count[t];
+                }
+        }
+        auto pack = [&](size_t i) {
+            if (i < njobs) {
+                const JobPrep &P = dl.jobs[i];
+                const CovJob &j = P.j;
+                std::memcpy(host + o_jobs + i * sizeof(CovJob), &j, sizeof(CovJob));
+                EdgeRec *er = (EdgeRec *)(host + o_edges) + j.edge_offset;
+                EdgeCov *ec = (EdgeCov *)(host + o_ecov) + j.edge_offset;
+                const uint32_t base = (uint32_t)P.piece_base;
+                for (uint32_t k = 0; k < j.edge_count; k++) {
+                    EdgeRec e = P.edges[k];
+                    e.piece_offset += base;
+                    er[k] = e;
+                    EdgeCov c = P.ecov[k];
+                    c.piece_offset += base;
+                    ec[k] = c;
+                }
+                if (!P.windows.empty())
+                    std::memcpy((uint2 *)(host + o_win) + j.window_offset, P.windows.data(),
+                                P.windows.size() * sizeof(uint2));
+                uint32_t *gl[3];
+                for (int t = 0; t < 3; t++) gl[t] = (uint32_t *)(host + o_grp[t]) + gbase[i * 3 + t];
+                for (size_t g = 0; g < P.tier.size(); g++)
+                    *gl[P.tier[g]]++ = j.row_start + (uint32_t)(g * kGroupRows);
+            } else {
+                const size_t k = i - njobs;
+                const HairPrep &P = dl.hair[k];
+                std::memcpy(host + o_hj + k * sizeof(HairJob), &P.j, sizeof(HairJob));
+                uint32_t *tl = (uint32_t *)(host + o_ht) + P.j.tile_offset;
+                for (size_t t = 0; t < P.tiles.size(); t++) tl[t] = P.tiles[t] + P.bin_base;
+                uint32_t *bl = (uint32_t *)(host + o_hb) + P.bin_base;
+                for (size_t b = 0; b < P.bins.size(); b++) bl[b] = P.bins[b] + P.line_base;
+                std::memcpy((HairLine *)(host + o_hl) + P.line_base, P.lines.data(), P.lines.size() * sizeof(HairLine));
+            }
+        };
+        if (parallel_pack && dl.n_edges >= 16384) {
+            parallel_for(njobs + nhair, pack);
+        } else {
+            for (size_t i = 0; i < njobs + nhair; i++) pack(i);
+        }
         void *dev = nullptr;
         RVK_TRY(rvk::scratch_alloc(ctx, o_arena + dl.arena, &dev));
         uint8_t *d = (uint8_t *)dev;
@@ -499,26 +655,26 @@ struct Renderer {
         // once; the compositor reads and writes each target pixel of its rect once
         // and reads each coverage byte once
         double cov_px = 0.0;
-        for (const CovJob &j : dl.jobs) cov_px += (double)j.w * (double)j.h;
+        for (const JobPrep &P : dl.jobs) cov_px += (double)P.j.w * (double)P.j.h;
         const double piece_bytes = (double)dl.piece_slots * sizeof(PieceDev);
-        const double exp_bytes = (double)dl.edges.size() * (sizeof(EdgeRec) + 4) + piece_bytes;
-        const double cov_bytes = cov_px + (double)dl.ecov.size() * sizeof(EdgeCov) +
-                                 (double)dl.windows.size() * sizeof(uint2) + piece_bytes;
+        const double exp_bytes = (double)dl.n_edges * (sizeof(EdgeRec) + 4) + piece_bytes;
+        const double cov_bytes = cov_px + (double)dl.n_edges * sizeof(EdgeCov) +
+                                 (double)dl.n_windows * sizeof(uint2) + piece_bytes;
         const double draw_bytes = 8.0 * (double)(dl.bx1 - dl.bx0) * (double)(dl.by1 - dl.by0) + cov_px +
-                                  (double)(dl.hair_lines.size() * sizeof(HairLine) + dl.hair_bins.size() * 4);
+                                  (double)(dl.n_hair_lines * sizeof(HairLine) + dl.n_hair_bins * 4);
         ctx->stats.edge_pieces += dl.piece_slots;
-        ctx->stats.band_entries += dl.windows.size();
+        ctx->stats.band_entries += dl.n_windows;
         ctx->stats.upload_bytes += meta;
         rvk_status st = rvk::upload_staged(ctx, slot, d, meta);
         if (st == RVK_OK)
-            st = rvk::launch_expand(ctx, (const EdgeRec *)(d + o_edges), (uint32_t)dl.edges.size(),
+            st = rvk::launch_expand(ctx, (const EdgeRec *)(d + o_edges), (uint32_t)dl.n_edges,
                                     (const EdgeCov *)(d + o_ecov), (PieceDev *)(d + o_pc), (uint32_t *)(d + o_np),
                                     dev_overflow, exp_bytes);
         if (st == RVK_OK)
-            st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)dl.jobs.size(),
-                                      (const uint32_t *)(d + o_grp[0]), (uint32_t)groups_tier[0].size(),
-                                      (const uint32_t *)(d + o_grp[1]), (uint32_t)groups_tier[1].size(),
-                                      (const uint32_t *)(d + o_grp[2]), (uint32_t)groups_tier[2].size(),
+            st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)njobs,
+                                      (const uint32_t *)(d + o_grp[0]), (uint32_t)dl.n_groups[0],
+                                      (const uint32_t *)(d + o_grp[1]), (uint32_t)dl.n_groups[1],
+                                      (const uint32_t *)(d + o_grp[2]), (uint32_t)dl.n_groups[2],
                                       (const EdgeCov *)(d + o_ecov), (const uint2 *)(d + o_win),
                                       (const PieceDev *)(d + o_pc), (const uint32_t *)(d + o_np), d + o_arena,
                                       dev_overflow, cov_bytes);
@@ -895,6 +1051,207 @@ rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_
     return r.err;
 }
 
+// Streams a batch spreads its documents over (RVK_BATCH_LANES overrides, 1-32).
+uint32_t batch_lanes() {
+    if (const char *e = std::getenv("RVK_BATCH_LANES")) {
+        const int v = std::atoi(e);
+        if (v > 0) return (uint32_t)std::min(v, 32);
+    }
+    return 16;
+}
+
+// A batch of documents.  Document d is drawn on lane d % lanes (sub-contexts with
+// their own streams, so the kernels of several documents run at once) by a host
+// thread of that lane, which runs the documents' execute walks in order.  The
+// calling thread meanwhile prepares chunks of documents: every collect walk of the
+// chunk, then the geometry of all their paths in one parallel pass, handing each
+// document to its lane when done.  The lanes start after the work already queued on
+// the context stream and the context stream continues after all of them; each
+// document has its own overflow counter and the batch ends with the one wait.
+// Documents whose pixmaps overlap are drawn in order on a single lane.
+rvk_status render_batch(rvk_context *ctx, uint32_t count, const rvk_scene *const *scenes, const rvk_transform *tss,
+                        const uint32_t *ws, const uint32_t *hs, uint8_t *const *pxs, rvk_status *statuses) {
+    uint32_t nl = std::min(batch_lanes(), count);
+    {
+        std::vector<std::pair<uintptr_t, uintptr_t>> spans(count);
+        for (uint32_t d = 0; d < count; d++)
+            spans[d] = {(uintptr_t)pxs[d], (uintptr_t)pxs[d] + (uintptr_t)ws[d] * hs[d] * 4};
+        std::sort(spans.begin(), spans.end());
+        for (uint32_t d = 1; d < count; d++)
+            if (spans[d].first < spans[d - 1].second) nl = 1;
+    }
+    RVK_TRY(rvk::lanes_prepare(ctx, nl));
+    void *ovbuf = nullptr;
+    RVK_TRY(rvk::scratch_alloc(ctx, (size_t)count * 4, &ovbuf));
+    int32_t *dev_ov = (int32_t *)ovbuf;
+    std::vector<hipEvent_t> evs(nl + 1, nullptr);
+    rvk_status s = rvk::launch_fill_sub(ctx, (uint8_t *)ovbuf, count, 1, 0, 0, count, 1, 0u);  // counters = 0
+    for (auto &e : evs)
+        if (s == RVK_OK && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) s = RVK_ERR_HIP;
+    if (s == RVK_OK && hipEventRecord(evs[nl], ctx->stream) != hipSuccess) s = RVK_ERR_HIP;
+    for (uint32_t l = 0; s == RVK_OK && l < nl; l++)
+        if (hipStreamWaitEvent(ctx->lanes[l]->stream, evs[nl], 0) != hipSuccess) s = RVK_ERR_HIP;
+    if (s != RVK_OK) (void)hipGetLastError();
+
+    std::vector<std::unique_ptr<Renderer>> rs(count);
+    std::vector<rvk_status> st(count, RVK_OK);
+    std::atomic<int32_t> fatal{s};
+    // the hand-over: per lane a queue of prepared documents; `pending` counts the
+    // paths of prepared documents not yet drawn (the calling thread waits while it
+    // exceeds two chunks, which bounds the host memory held by prepared geometry)
+    struct Lane {
+        std::mutex mu;
+        std::condition_variable cv;
+        std::vector<uint32_t> q;
+        size_t head = 0;
+        bool closed = false;
+    };
+    std::vector<Lane> lanes(nl);
+    std::mutex pmu;
+    std::condition_variable pcv;
+    size_t pending = 0;
+    constexpr uint32_t kChunkDocs = 64;
+    constexpr size_t kChunkTasks = 2048;
+    auto draw_doc = [&](uint32_t d) {
+        Renderer &r = *rs[d];
+        const size_t nt = r.tasks.size();
+        if (fatal.load() == RVK_OK) {
+            r.ready_end = nt;
+            r.mode = Renderer::kExecute;
+            r.parallel_pack = false;  // the pool belongs to the preparing thread
+            DrawList root;
+            root.px = pxs[d];
+            root.w = ws[d];
+            root.h = hs[d];
+            r.render_nodes(scenes[d]->groups[0], ts_from(tss[d]), root);
+            if (r.err == RVK_OK) r.fail(r.flush(root));
+            st[d] = r.err;
+            if (r.err == RVK_ERR_HIP || r.err == RVK_ERR_OUT_OF_MEMORY) {  // the stream is unusable
+                int32_t expect = RVK_OK;
+                fatal.compare_exchange_strong(expect, r.err);
+            }
+        }
+        rs[d].reset();
+        {
+            std::lock_guard<std::mutex> g(pmu);
+            pending -= nt;
+        }
+        pcv.notify_all();
+    };
+    std::vector<std::thread> threads;
+    for (uint32_t l = 0; l < nl; l++)
+        threads.emplace_back([&, l] {
+            (void)hipSetDevice(ctx->device);
+            Lane &ln = lanes[l];
+            for (;;) {
+                uint32_t d;
+                {
+                    std::unique_lock<std::mutex> g(ln.mu);
+                    ln.cv.wait(g, [&] { return ln.closed || ln.head < ln.q.size(); });
+                    if (ln.head == ln.q.size()) return;  // closed and drained
+                    d = ln.q[ln.head++];
+                }
+                draw_doc(d);
+            }
+        });
+    for (uint32_t d0 = 0; d0 < count && fatal.load() == RVK_OK;) {
+        {
+            const auto t0 = std::chrono::steady_clock::now();
+            std::unique_lock<std::mutex> g(pmu);
+            pcv.wait(g, [&] { return pending <= 2 * kChunkTasks || fatal.load() != RVK_OK; });
+            ctx->stats.host_wait_ms +=
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        size_t ntasks = 0;
+        uint32_t d1 = d0;
+        for (; d1 < count && d1 - d0 < kChunkDocs && ntasks < kChunkTasks; d1++) {
+            rs[d1].reset(new Renderer{ctx->lanes[d1 % nl], *scenes[d1], max_filter_bbox(ws[d1], hs[d1])});
+            Renderer &r = *rs[d1];
+            r.build_aux();
+            r.dev_overflow = dev_ov + d1;
+            DrawList root;
+            root.w = ws[d1];
+            root.h = hs[d1];
+            r.mode = Renderer::kCollect;
+            r.render_nodes(scenes[d1]->groups[0], ts_from(tss[d1]), root);
+            r.geoms.resize(r.tasks.size());
+            ntasks += r.tasks.size();
+        }
+        std::vector<std::pair<Renderer *, uint32_t>> all;
+        all.reserve(ntasks);
+        for (uint32_t d = d0; d < d1; d++)
+            for (size_t k = 0; k < rs[d]->tasks.size(); k++) all.emplace_back(rs[d].get(), (uint32_t)k);
+        const auto t0 = std::chrono::steady_clock::now();
+        parallel_for(all.size(), [&](size_t i) {
+            Renderer &r = *all[i].first;
+            const auto &t = r.tasks[all[i].second];
+            compute_geom(*t.scene, *t.path, t.ts, t.pw, t.ph, t.fill_only, t.clip, r.geoms[all[i].second]);
+        });
+        ctx->stats.host_geom_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        {
+            std::lock_guard<std::mutex> g(pmu);
+            pending += ntasks;
+        }
+        for (uint32_t d = d0; d < d1; d++) {
+            Lane &ln = lanes[d % nl];
+            {
+                std::lock_guard<std::mutex> g(ln.mu);
+                ln.q.push_back(d);
+            }
+            ln.cv.notify_one();
+        }
+        d0 = d1;
+    }
+    for (Lane &ln : lanes) {
+        {
+            std::lock_guard<std::mutex> g(ln.mu);
+            ln.closed = true;
+        }
+        ln.cv.notify_one();
+    }
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (auto &t : threads) t.join();
+        ctx->stats.host_wait_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    s = fatal.load();
+    // join: the context stream waits for every lane
+    for (uint32_t l = 0; l < nl; l++) {
+        if (!evs[l] || hipEventRecord(evs[l], ctx->lanes[l]->stream) != hipSuccess ||
+            hipStreamWaitEvent(ctx->stream, evs[l], 0) != hipSuccess) {
+            (void)hipGetLastError();
+            if (s == RVK_OK) s = RVK_ERR_HIP;
+            (void)hipStreamSynchronize(ctx->lanes[l]->stream);
+        }
+    }
+    std::vector<int32_t> ov(count, 0);
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        hipError_t e = hipMemcpyAsync(ov.data(), ovbuf, (size_t)count * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            if (s == RVK_OK) s = RVK_ERR_HIP;
+        }
+        ctx->stats.host_wait_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    for (auto e : evs)
+        if (e) (void)hipEventDestroy(e);
+    rvk::scratch_release(ctx, ovbuf);
+    rvk::lanes_collect(ctx, nl);
+    rvk_status first = s;
+    for (uint32_t d = 0; d < count; d++) {
+        if (s != RVK_OK && st[d] == RVK_OK) st[d] = s;
+        if (st[d] == RVK_OK && ov[d] > 0) st[d] = RVK_ERR_UNSUPPORTED;
+        if (first == RVK_OK) first = st[d];
+        if (statuses) statuses[d] = st[d];
+    }
+    return first;
+}
+
 }  // namespace rvk
 
 extern "C" {
@@ -910,6 +1267,25 @@ rvk_status rvk_render(rvk_context *ctx, const rvk_scene *scene, rvk_transform tr
     ctx->stats = rvk_stats{};
     const auto t0 = std::chrono::steady_clock::now();
     const rvk_status s = rvk::render_scene(ctx, scene, ts_from(transform), width, height, dev_pixmap, nullptr);
+    ctx->stats.host_build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() -
+        ctx->stats.host_wait_ms;
+    return s;
+}
+
+rvk_status rvk_render_batch(rvk_context *ctx, uint32_t count, const rvk_scene *const *scenes,
+                            const rvk_transform *transforms, const uint32_t *widths, const uint32_t *heights,
+                            uint8_t *const *dev_pixmaps, rvk_status *statuses) {
+    if (!ctx || (count > 0 && (!scenes || !transforms || !widths || !heights || !dev_pixmaps)))
+        return RVK_ERR_INVALID_ARGUMENT;
+    for (uint32_t i = 0; i < count; i++)
+        if (!scene_ok(scenes[i]) || !dev_pixmaps[i] || !rvk::valid_dims(widths[i], heights[i]))
+            return RVK_ERR_INVALID_ARGUMENT;
+    ctx->stats = rvk_stats{};
+    if (count == 0) return RVK_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    const rvk_status s =
+        rvk::render_batch(ctx, count, scenes, transforms, widths, heights, dev_pixmaps, statuses);
     ctx->stats.host_build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() -
         ctx->stats.host_wait_ms;

@@ -52,6 +52,9 @@ struct rvk_context {
     std::vector<std::pair<size_t, void *>> free_list;
     std::vector<std::pair<size_t, void *>> live;
     std::mutex mu;
+    // Sub-contexts with their own streams, scratch caches and staging rings on the
+    // same device; rvk_render_batch spreads documents over them.
+    std::vector<rvk_context *> lanes;
 };
 
 namespace rvk {
@@ -71,6 +74,13 @@ rvk_status staging_acquire(rvk_context *ctx, size_t bytes, void **host, uint32_t
 rvk_status upload_staged(rvk_context *ctx, uint32_t slot, void *dev_dst, size_t bytes);
 // Copies host memory to the device through the staging ring without waiting.
 rvk_status upload_async(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes);
+
+// Makes `n` lanes available (created on first use with non-blocking streams of the
+// default priority), resets their statistics and mirrors the profiling switch.
+rvk_status lanes_prepare(rvk_context *ctx, uint32_t n);
+// Adds the statistics of the first `n` lanes to the context's and moves their
+// profiled launches into the context's records.
+void lanes_collect(rvk_context *ctx, uint32_t n);
 
 static inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 

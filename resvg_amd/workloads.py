@@ -10,7 +10,8 @@
   own group with opacity U[0.2, 1]; 8 filter groups with userSpaceOnUse regions
   covering blur (IIR and box), morphology, 3x3/5x5 convolution, colour matrix,
   component transfer, turbulence, distant/point/spot lighting and multiply/screen.
-* ``document_batch`` — many documents of log-uniform sizes for the batched method.
+* ``document_batch`` — documents with log-uniform sides 64-2048 (aspect 0.5-2), 5-200
+  paths of the full-filter distributions and a 1-6 primitive filter chain each.
 
 Random numbers come from numpy's PCG64 seeded per document; every float is rounded
 to f32 before use, so the same document is produced on every machine.
@@ -202,18 +203,96 @@ def full_filter_doc(seed: int = 0, size: int = 4096, n_paths: int = 2000, paths_
     return sc, (size, size)
 
 
+def _sampled_chain(rng, w, h, n_prims):
+    """A filter of `n_prims` primitives drawn from the full-filter primitive set, each
+    reading the previous one's result, over a region covering 40-100% of each side."""
+    rw, rh = rng.uniform(0.4, 1.0, size=2) * (w, h)
+    x, y = rng.uniform(0, w - rw), rng.uniform(0, h - rh)
+    g = FilterGraph((_f(x), _f(y), _f(rw), _f(rh)))
+    prev = SOURCE_GRAPHIC
+    for _ in range(n_prims):
+        k = int(rng.integers(0, 10))
+        if k == 0:
+            prev = g.gaussian_blur(float(rng.choice([0.5, 1.5, 6.0, 24.0])), in1=prev)
+        elif k == 1:
+            op = abi.MORPH_ERODE if rng.random() < 0.5 else abi.MORPH_DILATE
+            prev = g.morphology(op, _f(rng.uniform(1, 8)), in1=prev)
+        elif k == 2:
+            n = 3 if rng.random() < 0.5 else 5
+            prev = g.convolve_matrix(n, n, rng.integers(-4, 5, n * n).astype(float), in1=prev)
+        elif k == 3:
+            prev = g.color_matrix(abi.COLOR_MATRIX_MATRIX, rng.uniform(-1, 1, 20).astype(F32).tolist(), in1=prev)
+        elif k == 4:
+            prev = g.component_transfer([("table", rng.uniform(0, 1, 5).astype(F32).tolist()),
+                                         ("gamma", 1.2, 0.8, 0.05), ("gamma", 0.9, 2.2, 0.0), None], in1=prev)
+        elif k == 5:
+            t = g.turbulence(_f(rng.uniform(0.01, 0.1)), _f(rng.uniform(0.01, 0.1)), 4,
+                             int(rng.integers(0, 1000)), False, True)
+            prev = g.composite(abi.COMPOSITE_IN, in1=t, in2=prev)
+        elif k == 6:
+            light = distant_light(_f(rng.uniform(0, 360)), _f(rng.uniform(10, 80)))
+            prev = g.lighting(False, light, 2.0, 1.0, in1=prev)
+        elif k == 7:
+            light = point_light(_f(x + rw / 2), _f(y + rh / 2), _f(0.2 * rw))
+            prev = g.lighting(True, light, 2.0, 1.0, 12.0, in1=prev)
+        elif k == 8:
+            light = spot_light(_f(x), _f(y), _f(0.3 * rw), _f(x + rw), _f(y + rh), 0.0, 2.0, 35.0)
+            prev = g.lighting(False, light, 2.0, 1.0, in1=prev)
+        else:
+            prev = g.blend("multiply" if rng.random() < 0.5 else "screen", in1=prev, in2=SOURCE_GRAPHIC)
+    return g
+
+
+def batch_doc(seed: int, w: int, h: int, n_paths: int, n_prims: int):
+    """One document of the batched workload: `n_paths` paths of the full-filter
+    distributions (control points and gradient geometry uniform in the w x h canvas),
+    each in its own opacity group; the paths of the middle third sit in a group with a
+    `n_prims`-primitive filter chain."""
+    rng = np.random.default_rng(seed)
+    ext = np.array([w, h], dtype=np.float64)
+    ext4 = np.array([w, h, w, h], dtype=np.float64)
+    sc = Scene(w, h)
+    fgroup = sc.group(sc.root, filters=[_sampled_chain(rng, w, h, n_prims)])
+    lo, hi = n_paths // 3, max(n_paths // 3 + 1, 2 * n_paths // 3)
+    for i in range(n_paths):
+        parent = fgroup if lo <= i < hi else sc.root
+        nseg = int(rng.integers(3, 201))
+        verbs, pts = _random_path(rng, ext, nseg)
+        stops = _stops(rng, int(rng.integers(2, 9)))
+
+        def paint():
+            if rng.random() < 0.5:
+                e = rng.uniform(0, ext4).astype(F32)
+                return PaintSpec.linear(_f(e[0]), _f(e[1]), _f(e[2]), _f(e[3]), stops)
+            c = rng.uniform(0, ext).astype(F32)
+            return PaintSpec.radial(_f(c[0]), _f(c[1]), _f(rng.uniform(0.1, 0.6) * min(w, h)), stops)
+
+        fill = paint()
+        rule = abi.FILL_EVENODD if rng.random() < 0.5 else abi.FILL_NONZERO
+        stroke, width, join = None, 1.0, abi.JOIN_MITER
+        if rng.random() < 0.4:
+            stops = _stops(rng, int(rng.integers(2, 9)))
+            stroke = paint()
+            width = _f(rng.uniform(0.5, 8.0))
+            join = [abi.JOIN_ROUND, abi.JOIN_MITER, abi.JOIN_BEVEL][int(rng.integers(3))]
+        grp = sc.group(parent, opacity=_f(rng.uniform(0.2, 1.0)))
+        sc.path(grp, verbs, pts, fill=fill, fill_rule=rule, stroke=stroke, stroke_width=width, stroke_join=join)
+    return sc
+
+
 def document_batch(n_docs: int, seed: int = 0, min_side: int = 64, max_side: int = 2048, min_paths: int = 5,
                    max_paths: int = 200):
-    """Documents with log-uniform sides and aspect 0.5-2 (the batched workload)."""
+    """The batched workload: `n_docs` documents with sides log-uniform in
+    [min_side, max_side] and aspect 0.5-2, 5-200 paths and a 1-6 primitive filter
+    chain each.  Returns [(Scene, (w, h))]."""
     rng = np.random.default_rng(seed)
     docs = []
-    for d in range(n_docs):
+    for _ in range(n_docs):
         side = float(np.exp(rng.uniform(np.log(min_side), np.log(max_side))))
         aspect = float(np.exp(rng.uniform(np.log(0.5), np.log(2.0))))
         w = int(max(min_side, min(max_side, round(side * np.sqrt(aspect)))))
         h = int(max(min_side, min(max_side, round(side / np.sqrt(aspect)))))
         n = int(rng.integers(min_paths, max_paths + 1))
-        sc, _ = full_filter_doc(seed=int(rng.integers(0, 2**31)), size=min(w, h), n_paths=n,
-                                paths_per_filter=max(1, n // 16), max_segments=64)
-        docs.append((sc, (w, h)))
+        k = int(rng.integers(1, 7))
+        docs.append((batch_doc(int(rng.integers(0, 2**31)), w, h, n, k), (w, h)))
     return docs

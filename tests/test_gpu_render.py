@@ -7,7 +7,7 @@ import pytest
 from oracle import bindings as orc
 from resvg_amd import abi
 from resvg_amd.scene import PaintSpec, Scene, ellipse_path, rect_path
-from resvg_amd.workloads import blur_offset_composite_doc, full_filter_doc
+from resvg_amd.workloads import blur_offset_composite_doc, document_batch, full_filter_doc
 
 from fixture_scenes import compare_png, load_png
 from scene_fixtures import all_scene_fixtures, render_ts
@@ -274,3 +274,71 @@ def test_render_host_matches_device(ctx):
     ctx.render_host(sc, ts, host)
     _assert_same(host, _gpu(ctx, sc, ts, 150, 150), "render_host")
     assert resvg_amd.lib() is not None
+
+
+def _batch(ctx, docs, ts, bases=None):
+    import torch
+    pxs = [torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda") if bases is None else
+           torch.from_numpy(bases[i].copy()).cuda() for i, (_, (w, h)) in enumerate(docs)]
+    st = ctx.render_batch([sc for sc, _ in docs], ts, pxs)
+    ctx.synchronize()
+    return st, [px.cpu().numpy() for px in pxs]
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_render_batch_matches_oracle(ctx, seed):
+    docs = document_batch(12, seed=seed, max_side=256, max_paths=40)
+    ts = abi.Transform.identity()
+    st, got = _batch(ctx, docs, ts)
+    assert st == [abi.OK] * len(docs)
+    for i, ((sc, (w, h)), g) in enumerate(zip(docs, got)):
+        _assert_same(g, _oracle(sc, ts, w, h), f"batch seed {seed} document {i} ({w}x{h})")
+
+
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_render_batch_matches_render_loop(ctx, monkeypatch, lanes):
+    # more documents than one host chunk, onto existing pixels, with a flush budget small
+    # enough that the staging ring turns over inside and across documents; documents
+    # spread over `lanes` streams
+    monkeypatch.setenv("RVK_FLUSH_BYTES", "65536")
+    monkeypatch.setenv("RVK_BATCH_LANES", str(lanes))
+    docs = document_batch(70, seed=7, max_side=160, max_paths=12)
+    ts = abi.Transform(1.0, 0.1, -0.1, 1.0, 3.0, -2.0)
+    rng = np.random.default_rng(7)
+    bases = []
+    for _, (w, h) in docs:
+        a = rng.integers(0, 256, (h, w, 1), dtype=np.uint8)
+        bases.append(np.concatenate([np.minimum(rng.integers(0, 256, (h, w, 3), dtype=np.uint8), a), a], -1))
+    st, got = _batch(ctx, docs, ts, bases)
+    assert st == [abi.OK] * len(docs)
+    for i, ((sc, (w, h)), g) in enumerate(zip(docs, got)):
+        _assert_same(g, _gpu(ctx, sc, ts, w, h, bases[i]), f"batch vs loop document {i}")
+
+
+def test_render_batch_shared_pixmap_in_order(ctx):
+    # documents composited onto one pixmap are drawn in batch order, as the loop draws them
+    import torch
+    docs = document_batch(6, seed=11, min_side=96, max_side=96, max_paths=10)
+    ts = abi.Transform.identity()
+    px = torch.zeros((96, 96, 4), dtype=torch.uint8, device="cuda")
+    st = ctx.render_batch([sc for sc, _ in docs], ts, [px] * len(docs))
+    assert st == [abi.OK] * len(docs)
+    ref = torch.zeros_like(px)
+    for sc, _ in docs:
+        ctx.render(sc, ts, ref)
+    ctx.synchronize()
+    _assert_same(px.cpu().numpy(), ref.cpu().numpy(), "shared pixmap")
+
+
+def test_render_batch_arguments(ctx):
+    import torch
+    import resvg_amd
+    assert ctx.render_batch([], abi.Transform.identity(), []) == []
+    docs = document_batch(2, seed=3, max_side=96, max_paths=6)
+    pxs = [torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda") for _, (w, h) in docs]
+    pxs[1] = torch.zeros((0, 8, 4), dtype=torch.uint8, device="cuda")  # zero height: rejected
+    with pytest.raises(resvg_amd.RvkError) as e:
+        ctx.render_batch([sc for sc, _ in docs], abi.Transform.identity(), pxs)
+    assert e.value.status == abi.ERR_INVALID_ARGUMENT
+    ctx.synchronize()
+    assert not pxs[0].any()  # arguments are checked before any document is drawn
