@@ -144,4 +144,49 @@ bool make_shader(const rvk_scene &sc, const rvk_paint &paint, rvkh::Ts ts, Shade
 // Control-point bounds of device-space segments nearly zero (tiny-skia skips such fills).
 bool bounds_nearly_zero(const std::vector<Seg> &segs);
 
+// Edges of one fill or stroke, computed off the walk (the expensive host part).
+struct EdgeSet {
+    bool ok = false;
+    std::vector<EdgeRec> edges;
+    std::vector<uint32_t> caps;
+    float mnx = 0.0f, mxx = 0.0f;  // control-point x range of the device-space segments
+};
+
+// A coverage job prepared off the walk: the CovJob (offsets relative to the job),
+// its edges sorted by first row with piece slots relative to the job, the coverage
+// kernel's view of them, and per group of kGroupRows rows the edge window and the
+// coverage kernel variant.  add_job places it in a draw list.
+struct JobPrep {
+    bool ok = false;
+    CovJob j{};
+    std::vector<EdgeRec> edges;
+    std::vector<EdgeCov> ecov;
+    std::vector<uint2> windows;
+    std::vector<uint8_t> tier;       // per row group: index into kCoverageKeys
+    uint32_t tier_count[3] = {0, 0, 0};
+    uint64_t pieces = 0;             // piece slots of all edges
+    uint64_t piece_base = 0;         // first piece slot in the draw list (add_job)
+};
+
+// A hairline job prepared off the walk: per tile the offset of its bin (relative to
+// the job's bins, tiles_x * tiles_y + 1 entries), bins of line indices relative to the
+// job's lines, each tile's in drawing order.
+struct HairPrep {
+    bool ok = false;
+    HairJob j{};
+    std::vector<uint32_t> tiles;
+    std::vector<uint32_t> bins;
+    std::vector<HairLine> lines;
+    uint32_t bin_base = 0, line_base = 0;  // positions in the draw list (add_hair_job)
+};
+
+// The coverage job of an edge set drawn into the pixels `clip` of its pixmap: the
+// edges sorted by first row (stable, so equal rows keep tiny-skia's edge order), the
+// edge window of each row group and the coverage kernel variant that holds its
+// crossings.  Leaves P.ok false when nothing is covered.
+void prep_job(const EdgeSet &es, uint32_t rule, bool aa, rvkh::IRect clip, JobPrep &P);
+// The hairline job of prepared lines, binned into tiles of the part of their pixel
+// bounds inside `clip`.  Leaves P.ok false when nothing is drawn.
+void prep_hair(std::vector<HairLine> &&lines, rvkh::IRect clip, HairPrep &P);
+
 }  // namespace rvkr

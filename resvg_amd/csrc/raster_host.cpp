@@ -23,6 +23,7 @@ namespace rvkr {
 void fill_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector<Seg> &out) {
     const uint8_t *verbs = sc.verbs + path.verb_offset;
     const float *pts = sc.points + 2 * (size_t)path.point_offset;
+    out.reserve(out.size() + path.verb_count);
     size_t pi = 0;
     Pt move{0, 0}, last{0, 0};
     bool open = false;
@@ -178,11 +179,10 @@ struct Builder {
     int shift;
     std::vector<EdgeRec> &out;
     std::vector<uint32_t> &caps;
-    std::vector<int32_t> winding;  // per edge
 
-    void push(const EdgeRec &e, int32_t w, uint32_t cap) {
+    void push(EdgeRec &e, int32_t w, uint32_t cap) {
+        e.winding = w;
         out.push_back(e);
-        winding.push_back(w);
         caps.push_back(cap);
     }
 
@@ -204,7 +204,7 @@ struct Builder {
         e.type = 0;
         if (e.dx == 0 && !out.empty() && out.back().type == 0) {
             EdgeRec &l = out.back();
-            int32_t &lw = winding.back();
+            int32_t &lw = l.winding;
             if (l.dx == 0 && e.x == l.x) {
                 if (w == lw) {
                     if (e.last_y + 1 == l.first_y) { l.first_y = e.first_y; return; }
@@ -212,7 +212,6 @@ struct Builder {
                 } else if (e.first_y == l.first_y) {
                     if (e.last_y == l.last_y) {  // the two cancel: drop the previous edge
                         out.pop_back();
-                        winding.pop_back();
                         caps.pop_back();
                         return;
                     }
@@ -394,7 +393,12 @@ int cubic_y_monotonic(const Pt *src, Pt *dst) {
 }  // namespace
 
 void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &out, std::vector<uint32_t> &caps) {
-    Builder b{shift, out, caps, {}};
+    Builder b{shift, out, caps};
+    // a line gives at most one edge, a quadratic two, a cubic three
+    size_t bound = 0;
+    for (const Seg &s : segs) bound += (size_t)(s.n - 1);
+    out.reserve(out.size() + bound);
+    caps.reserve(caps.size() + bound);
     for (const Seg &s : segs) {
         if (s.n == 2) {
             b.line(s.p[0], s.p[1]);
@@ -408,7 +412,6 @@ void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &
             for (int i = 0; i <= n; i++) b.cubic(mono + 3 * i);
         }
     }
-    for (size_t i = 0; i < out.size(); i++) out[i].winding = b.winding[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -452,6 +455,7 @@ void stroke_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vect
     if (!stroke_input(sc, path, res, dashed, verbs, nverbs, pts, npts)) return;
     rvks::PathBuilder outline;
     if (!rvks::stroke(verbs, nverbs, pts, npts, sp, res, outline)) return;
+    out.reserve(out.size() + outline.verbs.size());
     size_t pi = 0;
     Pt move{0, 0}, last{0, 0};
     bool open = false;
@@ -782,6 +786,162 @@ bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s
     n++;
     s.nstops = n;
     return true;
+}
+
+// ---------------------------------------------------------------------------
+// Coverage and hairline jobs of one path, prepared off the render walk
+// ---------------------------------------------------------------------------
+void prep_job(const EdgeSet &es, uint32_t rule, bool aa, IRect clip, JobPrep &P) {
+    if (!es.ok) return;
+    const int shift = aa ? 2 : 0;
+    const std::vector<EdgeRec> &er = es.edges;
+    const std::vector<uint32_t> &caps = es.caps;
+    // last covered row of each edge: exact for lines; for curves the row of the
+    // end point plus one (the stepped points stay within a fraction of a row of
+    // the monotonic curve; the expansion kernel checks the bound)
+    std::vector<int32_t> ylast(er.size());
+    int32_t ymin = INT32_MAX, ymax = INT32_MIN;
+    for (size_t i = 0; i < er.size(); i++) {
+        const EdgeRec &e = er[i];
+        ylast[i] = e.type == 0 ? e.last_y : (((e.ly >> 10) + 32) >> 6) + 1;
+        ymin = std::min(ymin, e.first_y);
+        ymax = std::max(ymax, ylast[i]);
+    }
+    int64_t x0 = std::max<int64_t>((int64_t)std::floor(es.mnx) - 1, clip.x);
+    int64_t x1 = std::min<int64_t>((int64_t)std::ceil(es.mxx) + 2, (int64_t)clip.x + clip.w);
+    int64_t y0 = std::max<int64_t>(ymin >> shift, clip.y);
+    int64_t y1 = std::min<int64_t>((ymax >> shift) + 1, (int64_t)clip.y + clip.h);
+    if (x1 <= x0 || y1 <= y0) return;
+    CovJob &j = P.j;
+    j = CovJob{};
+    j.x0 = (int32_t)x0; j.y0 = (int32_t)y0; j.w = (int32_t)(x1 - x0); j.h = (int32_t)(y1 - y0);
+    j.rule = rule;
+    j.aa = aa ? 1 : 0;
+    // edges sorted by first row, stably (ids keep tiny-skia's edge-list order): a
+    // counting sort over the rows when they are few against the edges, else a sort
+    // of (row, index) keys
+    const size_t n = er.size();
+    std::vector<uint32_t> order(n);
+    int32_t fmin = INT32_MAX, fmax = INT32_MIN;
+    for (const EdgeRec &e : er) { fmin = std::min(fmin, e.first_y); fmax = std::max(fmax, e.first_y); }
+    const uint64_t range = (uint64_t)((int64_t)fmax - fmin) + 1;
+    if (range <= 8 * (uint64_t)n + 256) {
+        std::vector<uint32_t> start((size_t)range + 1, 0);
+        for (const EdgeRec &e : er) start[(size_t)(e.first_y - fmin) + 1]++;
+        for (size_t r = 1; r <= range; r++) start[r] += start[r - 1];
+        for (uint32_t i = 0; i < n; i++) order[start[(size_t)(er[i].first_y - fmin)]++] = i;
+    } else {
+        std::vector<uint64_t> keys(n);
+        for (uint32_t i = 0; i < n; i++) keys[i] = ((uint64_t)(uint32_t)(er[i].first_y - fmin) << 32) | i;
+        std::sort(keys.begin(), keys.end());
+        for (uint32_t i = 0; i < n; i++) order[i] = (uint32_t)keys[i];
+    }
+    j.edge_count = (uint32_t)er.size();
+    P.edges.resize(er.size());
+    P.ecov.resize(er.size());
+    std::vector<int32_t> prefmax(er.size());
+    int32_t pm = INT32_MIN;
+    uint64_t slots = 0;
+    for (size_t k = 0; k < order.size(); k++) {
+        const uint32_t i = order[k];
+        EdgeRec e = er[i];
+        e.piece_offset = (uint32_t)slots;
+        slots += caps[i];
+        P.edges[k] = e;
+        P.ecov[k] = EdgeCov{e.first_y, ylast[i], e.winding, (int32_t)i, e.piece_offset, 0};
+        pm = std::max(pm, ylast[i]);
+        prefmax[k] = pm;
+    }
+    P.pieces = slots;
+    // per group of kGroupRows pixel rows: the edges that start at or before the
+    // group's last row, minus the leading ones that all end before its first row
+    const int32_t groups = (j.h + kGroupRows - 1) / kGroupRows;
+    P.windows.resize((size_t)groups);
+    uint32_t lo = 0, hi = 0;
+    for (int32_t g = 0; g < groups; g++) {
+        const int32_t r0 = j.y0 + g * kGroupRows, r1 = std::min(j.y0 + j.h, r0 + kGroupRows);
+        const int32_t s0 = r0 << shift, s1 = (r1 << shift) - 1;
+        while (lo < j.edge_count && prefmax[lo] < s0) lo++;
+        while (hi < j.edge_count && P.ecov[hi].first_y <= s1) hi++;
+        P.windows[(size_t)g] = uint2{lo, std::max(lo, hi)};
+    }
+    // edges whose row range meets each group (difference array): an edge crosses a
+    // sub-scanline at most once, so this bounds every sub-scanline's crossings and
+    // picks the coverage kernel variant that holds them
+    std::vector<int32_t> diff((size_t)groups + 1, 0);
+    for (size_t i = 0; i < er.size(); i++) {
+        const int64_t fr = (int64_t)(er[i].first_y >> shift) - j.y0, lr = (int64_t)(ylast[i] >> shift) - j.y0;
+        if (lr < 0 || fr >= j.h) continue;
+        const int64_t ga = std::max<int64_t>(fr, 0) / kGroupRows, gb = std::min<int64_t>(lr, j.h - 1) / kGroupRows;
+        diff[ga] += 1;
+        diff[gb + 1] -= 1;
+    }
+    P.tier.resize((size_t)groups);
+    for (int32_t g = 0, run = 0; g < groups; g++) {
+        run += diff[g];
+        const uint32_t n = (uint32_t)run;
+        const uint8_t t = n <= kCoverageKeys[0] ? 0 : n <= kCoverageKeys[1] ? 1 : 2;
+        P.tier[(size_t)g] = t;
+        P.tier_count[t]++;
+    }
+    P.ok = true;
+}
+
+void prep_hair(std::vector<HairLine> &&lines, IRect clip, HairPrep &P) {
+    if (lines.empty()) return;
+    struct Box { int32_t x0, y0, x1, y1; };
+    std::vector<Box> boxes(lines.size());
+    int32_t jx0 = INT32_MAX, jy0 = INT32_MAX, jx1 = INT32_MIN, jy1 = INT32_MIN;
+    for (size_t i = 0; i < lines.size(); i++) {
+        const HairLine &L = lines[i];
+        // minor pixels touched: f >> 16 and the one before, f at both ends
+        const int64_t fa = (int64_t)L.fstart + 0x8000;
+        const int64_t fb = fa + (int64_t)(L.istop - L.istart - 1) * L.slope;
+        const int32_t m0 = (int32_t)(std::min(fa, fb) >> 16) - 1, m1 = (int32_t)(std::max(fa, fb) >> 16) + 1;
+        Box b = (L.flags & kHairYMajor) ? Box{m0, L.istart, m1, L.istop} : Box{L.istart, m0, L.istop, m1};
+        if (L.flags & kHairClipped) {
+            b.x0 = std::max(b.x0, L.cl); b.y0 = std::max(b.y0, L.ct);
+            b.x1 = std::min(b.x1, L.cr); b.y1 = std::min(b.y1, L.cb);
+        }
+        b.x0 = std::max(b.x0, clip.x); b.y0 = std::max(b.y0, clip.y);
+        b.x1 = std::min(b.x1, clip.x + clip.w); b.y1 = std::min(b.y1, clip.y + clip.h);
+        boxes[i] = b;
+        if (b.x0 >= b.x1 || b.y0 >= b.y1) continue;
+        jx0 = std::min(jx0, b.x0); jy0 = std::min(jy0, b.y0);
+        jx1 = std::max(jx1, b.x1); jy1 = std::max(jy1, b.y1);
+    }
+    if (jx0 >= jx1 || jy0 >= jy1) return;
+    HairJob &j = P.j;
+    j = HairJob{};
+    j.x0 = jx0; j.y0 = jy0; j.w = jx1 - jx0; j.h = jy1 - jy0;
+    j.tiles_x = (uint32_t)((j.w + kHairTile - 1) / kHairTile);
+    j.tiles_y = (uint32_t)((j.h + kHairTile - 1) / kHairTile);
+    const size_t nt = (size_t)j.tiles_x * j.tiles_y;
+    std::vector<uint32_t> count(nt + 1, 0);
+    auto tiles_of = [&](const Box &b, auto &&fn) {
+        const int32_t tx0 = (b.x0 - j.x0) / kHairTile, tx1 = (b.x1 - 1 - j.x0) / kHairTile;
+        const int32_t ty0 = (b.y0 - j.y0) / kHairTile, ty1 = (b.y1 - 1 - j.y0) / kHairTile;
+        for (int32_t ty = ty0; ty <= ty1; ty++)
+            for (int32_t tx = tx0; tx <= tx1; tx++) fn((size_t)ty * j.tiles_x + tx);
+    };
+    for (const Box &b : boxes)
+        if (b.x0 < b.x1 && b.y0 < b.y1) tiles_of(b, [&](size_t t) { count[t]++; });
+    P.tiles.resize(nt + 1);
+    uint32_t run = 0;
+    for (size_t t = 0; t < nt; t++) {
+        P.tiles[t] = run;
+        const uint32_t c = count[t];
+        count[t] = run;  // becomes the fill cursor
+        run += c;
+    }
+    P.tiles[nt] = run;
+    P.bins.resize(run);
+    for (size_t i = 0; i < lines.size(); i++) {
+        const Box &b = boxes[i];
+        if (b.x0 < b.x1 && b.y0 < b.y1) tiles_of(b, [&](size_t t) { P.bins[count[t]++] = (uint32_t)i; });
+    }
+    P.lines = std::move(lines);
+    P.ok = true;
 }
 
 }  // namespace rvkr

@@ -56,34 +56,6 @@ uint64_t arena_budget() {
     return 1ull << 30;
 }
 
-// A coverage job prepared off the walk: the CovJob (offsets relative to the job),
-// its edges sorted by first row with piece slots relative to the job, the coverage
-// kernel's view of them, and per group of kGroupRows rows the edge window and the
-// coverage kernel variant.  add_job places it in a draw list.
-struct JobPrep {
-    bool ok = false;
-    CovJob j{};
-    std::vector<EdgeRec> edges;
-    std::vector<EdgeCov> ecov;
-    std::vector<uint2> windows;
-    std::vector<uint8_t> tier;       // per row group: index into kCoverageKeys
-    uint32_t tier_count[3] = {0, 0, 0};
-    uint64_t pieces = 0;             // piece slots of all edges
-    uint64_t piece_base = 0;         // first piece slot in the draw list (add_job)
-};
-
-// A hairline job prepared off the walk: per tile the offset of its bin (relative to
-// the job's bins, tiles_x * tiles_y + 1 entries), bins of line indices relative to the
-// job's lines, each tile's in drawing order.
-struct HairPrep {
-    bool ok = false;
-    HairJob j{};
-    std::vector<uint32_t> tiles;
-    std::vector<uint32_t> bins;
-    std::vector<HairLine> lines;
-    uint32_t bin_base = 0, line_base = 0;  // positions in the draw list (add_hair_job)
-};
-
 struct DrawList {
     uint8_t *px = nullptr;
     uint32_t w = 0, h = 0;
@@ -122,162 +94,12 @@ IRect max_filter_bbox(uint32_t w, uint32_t h) {  // lib.rs:86-97
     return IRect{INT32_MIN / 2, INT32_MIN / 2, INT32_MAX / 2 - INT32_MIN / 2, INT32_MAX / 2 - INT32_MIN / 2};
 }
 
-// Edges of one fill or stroke, computed off the walk (the expensive host part).
-struct EdgeSet {
-    bool ok = false;
-    std::vector<EdgeRec> edges;
-    std::vector<uint32_t> caps;
-    float mnx = 0.0f, mxx = 0.0f;  // control-point x range of the device-space segments
-};
-
 struct PathGeom {
     JobPrep fill, stroke;
     bool hair = false;               // the stroke is an anti-aliased hairline
     float hair_opacity = 1.0f;       // folded into its paint
     HairPrep hair_job;
 };
-
-// The coverage job of an edge set drawn into the pixels `clip` of its pixmap.
-void prep_job(const EdgeSet &es, uint32_t rule, bool aa, IRect clip, JobPrep &P) {
-    if (!es.ok) return;
-    const int shift = aa ? 2 : 0;
-    const std::vector<EdgeRec> &er = es.edges;
-    const std::vector<uint32_t> &caps = es.caps;
-    // last covered row of each edge: exact for lines; for curves the row of the
-    // end point plus one (the stepped points stay within a fraction of a row of
-    // the monotonic curve; the expansion kernel checks the bound)
-    std::vector<int32_t> ylast(er.size());
-    int32_t ymin = INT32_MAX, ymax = INT32_MIN;
-    for (size_t i = 0; i < er.size(); i++) {
-        const EdgeRec &e = er[i];
-        ylast[i] = e.type == 0 ? e.last_y : (((e.ly >> 10) + 32) >> 6) + 1;
-        ymin = std::min(ymin, e.first_y);
-        ymax = std::max(ymax, ylast[i]);
-    }
-    int64_t x0 = std::max<int64_t>((int64_t)std::floor(es.mnx) - 1, clip.x);
-    int64_t x1 = std::min<int64_t>((int64_t)std::ceil(es.mxx) + 2, (int64_t)clip.x + clip.w);
-    int64_t y0 = std::max<int64_t>(ymin >> shift, clip.y);
-    int64_t y1 = std::min<int64_t>((ymax >> shift) + 1, (int64_t)clip.y + clip.h);
-    if (x1 <= x0 || y1 <= y0) return;
-    CovJob &j = P.j;
-    j = CovJob{};
-    j.x0 = (int32_t)x0; j.y0 = (int32_t)y0; j.w = (int32_t)(x1 - x0); j.h = (int32_t)(y1 - y0);
-    j.rule = rule;
-    j.aa = aa ? 1 : 0;
-    // edges sorted by first row (ids keep tiny-skia's edge-list order)
-    std::vector<uint32_t> order(er.size());
-    for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return er[a].first_y < er[b].first_y; });
-    j.edge_count = (uint32_t)er.size();
-    P.edges.resize(er.size());
-    P.ecov.resize(er.size());
-    std::vector<int32_t> prefmax(er.size());
-    int32_t pm = INT32_MIN;
-    uint64_t slots = 0;
-    for (size_t k = 0; k < order.size(); k++) {
-        const uint32_t i = order[k];
-        EdgeRec e = er[i];
-        e.piece_offset = (uint32_t)slots;
-        slots += caps[i];
-        P.edges[k] = e;
-        P.ecov[k] = EdgeCov{e.first_y, ylast[i], e.winding, (int32_t)i, e.piece_offset, 0};
-        pm = std::max(pm, ylast[i]);
-        prefmax[k] = pm;
-    }
-    P.pieces = slots;
-    // per group of kGroupRows pixel rows: the edges that start at or before the
-    // group's last row, minus the leading ones that all end before its first row
-    const int32_t groups = (j.h + kGroupRows - 1) / kGroupRows;
-    P.windows.resize((size_t)groups);
-    uint32_t lo = 0, hi = 0;
-    for (int32_t g = 0; g < groups; g++) {
-        const int32_t r0 = j.y0 + g * kGroupRows, r1 = std::min(j.y0 + j.h, r0 + kGroupRows);
-        const int32_t s0 = r0 << shift, s1 = (r1 << shift) - 1;
-        while (lo < j.edge_count && prefmax[lo] < s0) lo++;
-        while (hi < j.edge_count && P.ecov[hi].first_y <= s1) hi++;
-        P.windows[(size_t)g] = uint2{lo, std::max(lo, hi)};
-    }
-    // edges whose row range meets each group (difference array): an edge crosses a
-    // sub-scanline at most once, so this bounds every sub-scanline's crossings and
-    // picks the coverage kernel variant that holds them
-    std::vector<int32_t> diff((size_t)groups + 1, 0);
-    for (size_t i = 0; i < er.size(); i++) {
-        const int64_t fr = (int64_t)(er[i].first_y >> shift) - j.y0, lr = (int64_t)(ylast[i] >> shift) - j.y0;
-        if (lr < 0 || fr >= j.h) continue;
-        const int64_t ga = std::max<int64_t>(fr, 0) / kGroupRows, gb = std::min<int64_t>(lr, j.h - 1) / kGroupRows;
-        diff[ga] += 1;
-        diff[gb + 1] -= 1;
-    }
-    P.tier.resize((size_t)groups);
-    for (int32_t g = 0, run = 0; g < groups; g++) {
-        run += diff[g];
-        const uint32_t n = (uint32_t)run;
-        const uint8_t t = n <= kCoverageKeys[0] ? 0 : n <= kCoverageKeys[1] ? 1 : 2;
-        P.tier[(size_t)g] = t;
-        P.tier_count[t]++;
-    }
-    P.ok = true;
-}
-
-// The hairline job of prepared lines, binned into tiles of the part of their pixel
-// bounds inside `clip`.
-void prep_hair(std::vector<HairLine> &&lines, IRect clip, HairPrep &P) {
-    if (lines.empty()) return;
-    struct Box { int32_t x0, y0, x1, y1; };
-    std::vector<Box> boxes(lines.size());
-    int32_t jx0 = INT32_MAX, jy0 = INT32_MAX, jx1 = INT32_MIN, jy1 = INT32_MIN;
-    for (size_t i = 0; i < lines.size(); i++) {
-        const HairLine &L = lines[i];
-        // minor pixels touched: f >> 16 and the one before, f at both ends
-        const int64_t fa = (int64_t)L.fstart + 0x8000;
-        const int64_t fb = fa + (int64_t)(L.istop - L.istart - 1) * L.slope;
-        const int32_t m0 = (int32_t)(std::min(fa, fb) >> 16) - 1, m1 = (int32_t)(std::max(fa, fb) >> 16) + 1;
-        Box b = (L.flags & kHairYMajor) ? Box{m0, L.istart, m1, L.istop} : Box{L.istart, m0, L.istop, m1};
-        if (L.flags & kHairClipped) {
-            b.x0 = std::max(b.x0, L.cl); b.y0 = std::max(b.y0, L.ct);
-            b.x1 = std::min(b.x1, L.cr); b.y1 = std::min(b.y1, L.cb);
-        }
-        b.x0 = std::max(b.x0, clip.x); b.y0 = std::max(b.y0, clip.y);
-        b.x1 = std::min(b.x1, clip.x + clip.w); b.y1 = std::min(b.y1, clip.y + clip.h);
-        boxes[i] = b;
-        if (b.x0 >= b.x1 || b.y0 >= b.y1) continue;
-        jx0 = std::min(jx0, b.x0); jy0 = std::min(jy0, b.y0);
-        jx1 = std::max(jx1, b.x1); jy1 = std::max(jy1, b.y1);
-    }
-    if (jx0 >= jx1 || jy0 >= jy1) return;
-    HairJob &j = P.j;
-    j = HairJob{};
-    j.x0 = jx0; j.y0 = jy0; j.w = jx1 - jx0; j.h = jy1 - jy0;
-    j.tiles_x = (uint32_t)((j.w + kHairTile - 1) / kHairTile);
-    j.tiles_y = (uint32_t)((j.h + kHairTile - 1) / kHairTile);
-    const size_t nt = (size_t)j.tiles_x * j.tiles_y;
-    std::vector<uint32_t> count(nt + 1, 0);
-    auto tiles_of = [&](const Box &b, auto &&fn) {
-        const int32_t tx0 = (b.x0 - j.x0) / kHairTile, tx1 = (b.x1 - 1 - j.x0) / kHairTile;
-        const int32_t ty0 = (b.y0 - j.y0) / kHairTile, ty1 = (b.y1 - 1 - j.y0) / kHairTile;
-        for (int32_t ty = ty0; ty <= ty1; ty++)
-            for (int32_t tx = tx0; tx <= tx1; tx++) fn((size_t)ty * j.tiles_x + tx);
-    };
-    for (const Box &b : boxes)
-        if (b.x0 < b.x1 && b.y0 < b.y1) tiles_of(b, [&](size_t t) { count[t]++; });
-    P.tiles.resize(nt + 1);
-    uint32_t run = 0;
-    for (size_t t = 0; t < nt; t++) {
-        P.tiles[t] = run;
-        const uint32_t c = count[t];
-        count[t] = run;  // becomes the fill cursor
-        run += c;
-    }
-    P.tiles[nt] = run;
-    P.bins.resize(run);
-    for (size_t i = 0; i < lines.size(); i++) {
-        const Box &b = boxes[i];
-        if (b.x0 < b.x1 && b.y0 < b.y1) tiles_of(b, [&](size_t t) { P.bins[count[t]++] = (uint32_t)i; });
-    }
-    P.lines = std::move(lines);
-    P.ok = true;
-}
 
 // Device-space geometry of one path drawn into a pw x ph pixmap whose visible
 // pixels are `clip`: tiny-skia's fill_path preconditions and edges, and (unless

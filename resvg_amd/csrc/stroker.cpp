@@ -1321,7 +1321,11 @@ bool stroke(const uint8_t *verbs, size_t nverbs, const float *points, size_t npo
     s.cap_kind = sp.cap;
     s.join_kind = join;
     const Pt *pts = reinterpret_cast<const Pt *>(points);
-    (void)npoints;
+    // the outline holds a few quadratics per input segment and side
+    s.outer.verbs.reserve(nverbs * 8);
+    s.outer.pts.reserve(npoints * 16);
+    s.inner.verbs.reserve(nverbs * 4);
+    s.inner.pts.reserve(npoints * 8);
     bool last_is_line = false;
     size_t pi = 0;
     Pt move_pt{0, 0}, last{0, 0};
