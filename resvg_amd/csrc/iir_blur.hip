@@ -2,9 +2,10 @@
 // channel, 4 steps of a causal then an anti-causal first-order recurrence along every
 // row and then along every column, in f64, and truncates (v*255) to u8.  The chains
 // are kept sequential (bit-exact order of f64 operations); parallelism comes from
-// running every (row, channel) and (column, channel) chain in its own lane.  The f64
-// planes are laid out so that lanes of a wave touch consecutive addresses at every
-// step: column-major for the row sweeps, row-major for the column sweeps.
+// running every (row, channel) and (column, channel) chain in its own lane, and each
+// lane streams its chain a block of values ahead of the recurrence.  The f64 planes
+// are laid out so that lanes of a wave touch consecutive addresses at every step:
+// column-major for the row sweeps, row-major for the column sweeps.
 #include "rvk_internal.h"
 #include "device_math.h"
 
@@ -33,27 +34,63 @@ __global__ void k_to_f64_colmajor(const uint32_t *px, double *planes, uint32_t w
     }
 }
 
+// The eight sweeps of one chain b[0], b[s], ..., b[(n-1) s] in the reference's order
+// of f64 operations: per step, the product dnu * previous, then the sum (no fused
+// multiply-add).  Values ahead of the sweep are not written by it, so each lane loads
+// the next kBlock values while it computes the current kBlock: the recurrence then
+// waits for memory once per block instead of once per step.
+constexpr int kBlock = 32;
+__device__ __forceinline__ void iir_chain(double *b, size_t s, uint32_t n, double dnu) {
+    if (n < 2) return;
+    double cur[kBlock], nxt[kBlock];
+    for (int step = 0; step < 4; step++) {
+        // causal: b[x] += dnu * b[x - 1] for x = 1 .. n-1
+        double prev = b[0];
+#pragma unroll
+        for (int k = 0; k < kBlock; k++) cur[k] = 1 + k < (int)n ? b[(size_t)(1 + k) * s] : 0.0;
+        for (uint32_t x0 = 1; x0 < n; x0 += kBlock) {
+            const uint32_t x1 = x0 + kBlock;
+#pragma unroll
+            for (int k = 0; k < kBlock; k++) nxt[k] = x1 + k < n ? b[(size_t)(x1 + k) * s] : 0.0;
+#pragma unroll
+            for (int k = 0; k < kBlock; k++) {
+                if (x0 + k < n) {
+                    const double v = cur[k] + dnu * prev;
+                    b[(size_t)(x0 + k) * s] = v;
+                    prev = v;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kBlock; k++) cur[k] = nxt[k];
+        }
+        // anti-causal: b[x - 1] += dnu * b[x] for x = n-1 .. 1
+        prev = b[(size_t)(n - 1) * s];
+#pragma unroll
+        for (int k = 0; k < kBlock; k++) cur[k] = (int)n - 2 - k >= 0 ? b[(size_t)(n - 2 - k) * s] : 0.0;
+        for (int64_t x0 = (int64_t)n - 2; x0 >= 0; x0 -= kBlock) {
+            const int64_t x1 = x0 - kBlock;
+#pragma unroll
+            for (int k = 0; k < kBlock; k++) nxt[k] = x1 - k >= 0 ? b[(size_t)(x1 - k) * s] : 0.0;
+#pragma unroll
+            for (int k = 0; k < kBlock; k++) {
+                if (x0 - k >= 0) {
+                    const double v = cur[k] + dnu * prev;
+                    b[(size_t)(x0 - k) * s] = v;
+                    prev = v;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kBlock; k++) cur[k] = nxt[k];
+        }
+    }
+}
+
 // Row sweeps on column-major planes: lane = (row y, channel c).
-__global__ void k_iir_rows(double *planes, uint32_t w, uint32_t h, double dnu) {
+__global__ __launch_bounds__(64) void k_iir_rows(double *planes, uint32_t w, uint32_t h, double dnu) {
     uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t c = blockIdx.y;
     if (y >= h) return;
-    double *b = planes + (size_t)c * w * h + y;
-    const size_t s = h;  // stride between consecutive x
-    for (int step = 0; step < 4; step++) {
-        double prev = b[0];
-        for (uint32_t x = 1; x < w; x++) {
-            double v = b[x * s] + dnu * prev;
-            b[x * s] = v;
-            prev = v;
-        }
-        prev = b[(size_t)(w - 1) * s];
-        for (uint32_t x = w - 1; x > 0; x--) {
-            double v = b[(x - 1) * s] + dnu * prev;
-            b[(x - 1) * s] = v;
-            prev = v;
-        }
-    }
+    iir_chain(planes + (size_t)c * w * h + y, h, w, dnu);
 }
 
 // column-major planes -> row-major planes
@@ -76,26 +113,11 @@ __global__ void k_transpose_planes(const double *src, double *dst, uint32_t w, u
 }
 
 // Column sweeps on row-major planes: lane = (column x, channel c).
-__global__ void k_iir_cols(double *planes, uint32_t w, uint32_t h, double dnu) {
+__global__ __launch_bounds__(64) void k_iir_cols(double *planes, uint32_t w, uint32_t h, double dnu) {
     uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t c = blockIdx.y;
     if (x >= w) return;
-    double *b = planes + (size_t)c * w * h + x;
-    const size_t s = w;
-    for (int step = 0; step < 4; step++) {
-        double prev = b[0];
-        for (uint32_t y = 1; y < h; y++) {
-            double v = b[y * s] + dnu * prev;
-            b[y * s] = v;
-            prev = v;
-        }
-        prev = b[(size_t)(h - 1) * s];
-        for (uint32_t y = h - 1; y > 0; y--) {
-            double v = b[(y - 1) * s] + dnu * prev;
-            b[(y - 1) * s] = v;
-            prev = v;
-        }
-    }
+    iir_chain(planes + (size_t)c * w * h + x, w, h, dnu);
 }
 
 // scale + `(v * 255.0) as u8`, from row-major planes
