@@ -87,7 +87,8 @@ struct HairJob {
 
 // Shader parameters (tiny-skia lowp stages for solid, linear and concentric radial).
 constexpr int kMaxStops = 16;
-struct ShaderParams {
+// The scalar part of a shader, read by the compositor in one piece.
+struct ShaderHead {
     uint32_t kind;       // 0 none, 1 solid, 2 gradient
     uint32_t radial, spread, pad_clamp, two_stop, opaque, r_twist, nstops;
     uint32_t rgba;       // solid premultiplied colour
@@ -99,6 +100,9 @@ struct ShaderParams {
     uint32_t clear;
     float sx, ky, kx, sy, tx, ty;  // device -> unit transform
     float r_scale, r_bias;  // focal radius: t = (t - r_scale) / r_bias
+    uint32_t pad_;
+};
+struct ShaderParams : ShaderHead {
     float t[kMaxStops + 2];
     float f[kMaxStops + 2][4];
     float b[kMaxStops + 2][4];
@@ -108,6 +112,16 @@ struct ShaderParams {
 // into the target, either directly (lowp SourceOver) or through an elided layer
 // (the layer pixel is built in registers, then composited with opacity and blend
 // mode in f32 like draw_pixmap).
+// What the compositor needs of a job, copied into the draw item so that one load of
+// the item brings it: the rect (layer space; w = h = 0 for no job) and where the
+// coverage bytes are, or for a hairline job its tile grid.
+struct JobRef {
+    int32_t x0, y0, w, h;
+    uint32_t stride;       // coverage row stride (w rounded up to 4); hairline: tiles_x
+    uint32_t tile_offset;  // hairline: first entry of the job's tile offsets
+    uint64_t cov_offset;   // byte offset of the coverage in the arena
+};
+
 struct DrawItem {
     int32_t x0, y0, x1, y1;     // target-pixel rect touched (clipped to the target)
     int32_t fill_job, stroke_job;  // coverage jobs (-1: none); their rects are in layer space
@@ -119,6 +133,7 @@ struct DrawItem {
     uint32_t blend_mode;
     uint32_t stroke_hair;       // 1: stroke_job is a hairline job
     uint32_t pad_;
+    JobRef fj, sj;              // the fill and stroke jobs
 };
 
 struct Pt { float x, y; };

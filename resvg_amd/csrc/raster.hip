@@ -33,6 +33,9 @@ using rvkr::HairJob;
 using rvkr::HairLine;
 using rvkr::PieceDev;
 using rvkr::ShaderParams;
+using rvkr::ShaderHead;
+using rvkr::JobRef;
+using rvkr::kMaxStops;
 
 namespace {
 
@@ -448,7 +451,8 @@ __device__ __forceinline__ uint32_t div255(uint32_t v) { return (v + 255u) >> 8;
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
 // premultiplied u8 source colour of a shader at pixel (px, py)
-__device__ __forceinline__ void shade(const ShaderParams &s, int32_t px, int32_t py, uint32_t o[4]) {
+__device__ __forceinline__ void shade(const ShaderParams &sp, const ShaderHead &s, int32_t px, int32_t py,
+                                      uint32_t o[4]) {
     if (s.kind == 1) {
         o[0] = s.rgba & 0xff; o[1] = (s.rgba >> 8) & 0xff; o[2] = (s.rgba >> 16) & 0xff; o[3] = s.rgba >> 24;
         return;
@@ -471,11 +475,18 @@ __device__ __forceinline__ void shade(const ShaderParams &s, int32_t px, int32_t
     }
     float c[4];
     if (s.two_stop) {
-        for (int k = 0; k < 4; k++) c[k] = t * s.f[0][k] + s.b[0][k];
+        for (int k = 0; k < 4; k++) c[k] = t * sp.f[0][k] + sp.b[0][k];
     } else {
+        // the thresholds are read four at a time (t has kMaxStops + 2 entries)
         uint32_t idx = 0;
-        for (uint32_t i = 1; i < s.nstops; i++) idx += t >= s.t[i] ? 1u : 0u;
-        for (int k = 0; k < 4; k++) c[k] = t * s.f[idx][k] + s.b[idx][k];
+        for (uint32_t i = 1; i < s.nstops; i += 4) {
+            float q[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) q[k] = sp.t[min(i + (uint32_t)k, (uint32_t)kMaxStops + 1u)];
+#pragma unroll
+            for (int k = 0; k < 4; k++) idx += i + (uint32_t)k < s.nstops && t >= q[k] ? 1u : 0u;
+        }
+        for (int k = 0; k < 4; k++) c[k] = t * sp.f[idx][k] + sp.b[idx][k];
     }
     for (int k = 0; k < 4; k++) {
         const float v = (c[k] != c[k]) ? 0.0f : clamp01(c[k]);
@@ -504,8 +515,8 @@ __device__ __forceinline__ uint32_t over_lowp(const uint32_t src_in[4], uint32_t
 
 // one covered pixel of a draw item: Clear, or the shaded colour composited with
 // over_lowp (the colour is shaded only when needed)
-__device__ __forceinline__ uint32_t paint_px(const ShaderParams &sh, int32_t lx, int32_t ly, uint32_t c, uint32_t d,
-                                             uint32_t src[4], bool &shaded) {
+__device__ __forceinline__ uint32_t paint_px(const ShaderParams &sp, const ShaderHead &sh, int32_t lx, int32_t ly,
+                                             uint32_t c, uint32_t d, uint32_t src[4], bool &shaded) {
     if (sh.clear) {
         const uint32_t ic = 255u - c;
         uint32_t o = 0;
@@ -513,20 +524,17 @@ __device__ __forceinline__ uint32_t paint_px(const ShaderParams &sh, int32_t lx,
         return o;
     }
     if (!shaded) {
-        shade(sh, lx, ly, src);
+        shade(sp, sh, lx, ly, src);
         shaded = true;
     }
     return over_lowp(src, c, d, sh.as_source != 0);
 }
 
-__device__ __forceinline__ uint32_t job_cov(const CovJob *jobs, const uint8_t *arena, int32_t j, int32_t x,
-                                            int32_t y) {
-    if (j < 0) return 0;
-    const CovJob &jb = jobs[j];
-    const int32_t cx = x - jb.x0, cy = y - jb.y0;
-    if (cx < 0 || cy < 0 || cx >= jb.w || cy >= jb.h) return 0;
-    const uint32_t stride = ((uint32_t)jb.w + 3u) & ~3u;
-    return arena[jb.cov_offset + (size_t)cy * stride + cx];
+// coverage of a job at layer pixel (x, y); 0 outside its rect (and for no job, whose
+// rect is empty)
+__device__ __forceinline__ uint32_t ref_cov(const JobRef &r, const uint8_t *arena, int32_t x, int32_t y) {
+    const uint32_t cx = (uint32_t)(x - r.x0), cy = (uint32_t)(y - r.y0);
+    return cx < (uint32_t)r.w && cy < (uint32_t)r.h ? arena[r.cov_offset + (size_t)cy * r.stride + cx] : 0u;
 }
 
 // Coverage of one hairline blit at pixel (x, y) (see HairLine); 0 when the line does
@@ -588,52 +596,54 @@ __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t
     uint32_t d = inside ? px[(size_t)y * w + x] : 0u;
     const uint32_t d0 = d;
     for (uint32_t it = 0; it < a.nitems; it++) {
-        const DrawItem item = a.items[it];
-        if (item.x1 <= tx0 || item.x0 >= tx0 + TX || item.y1 <= ty0 || item.y0 >= ty0 + TY) continue;
-        if (item.x1 <= wx0 || item.x0 >= wx0 + kWaveSide || item.y1 <= wy0 || item.y0 >= wy0 + kWaveSide) continue;
-        if (!inside || x < item.x0 || x >= item.x1 || y < item.y0 || y >= item.y1) continue;
-        const int32_t lx = x - item.lx, ly = y - item.ly;
-        const uint32_t cf = job_cov(a.jobs, a.arena, item.fill_job, lx, ly);
-        uint32_t cs = 0, hb = 0, he = 0;
-        if (item.stroke_hair) {
-            // the hairline lines binned to this pixel's tile, in drawing order
-            const HairJob &hj = a.hair_jobs[item.stroke_job];
-            const int32_t cx = lx - hj.x0, cy = ly - hj.y0;
-            if (cx >= 0 && cy >= 0 && cx < hj.w && cy < hj.h) {
-                const uint32_t t = hj.tile_offset + (uint32_t)(cy / rvkr::kHairTile) * hj.tiles_x +
-                                   (uint32_t)(cx / rvkr::kHairTile);
-                hb = a.hair_tiles[t];
-                he = a.hair_tiles[t + 1];
-            }
-        } else {
-            cs = job_cov(a.jobs, a.arena, item.stroke_job, lx, ly);
-        }
-        if ((cf | cs) == 0 && hb == he) continue;
-        uint32_t target = item.layer ? 0u : d;
-        for (int pass = 0; pass < 2; pass++) {
-            const bool stroke = (pass == 0) == (item.stroke_first != 0);
-            const ShaderParams &sh = a.shaders[stroke ? item.stroke_shader : item.fill_shader];
-            uint32_t src[4];
-            bool shaded = false;
-            if (stroke && item.stroke_hair) {
-                // every blit of the hairline at this pixel, composited in order
-                for (uint32_t k = hb; k < he; k++) {
-                    const uint32_t c = hair_alpha(a.hair_lines[a.hair_bins[k]], lx, ly);
-                    if (c) target = paint_px(sh, lx, ly, c, target, src, shaded);
+        const DrawItem item = a.items[it];  // with its jobs' rects and coverage offsets
+        const bool hit = item.x1 > tx0 && item.x0 < tx0 + TX && item.y1 > ty0 && item.y0 < ty0 + TY &&
+                         item.x1 > wx0 && item.x0 < wx0 + kWaveSide && item.y1 > wy0 && item.y0 < wy0 + kWaveSide;
+        if (hit && inside && x >= item.x0 && x < item.x1 && y >= item.y0 && y < item.y1) {
+            const int32_t lx = x - item.lx, ly = y - item.ly;
+            const uint32_t cf = ref_cov(item.fj, a.arena, lx, ly);
+            uint32_t cs = 0, hb = 0, he = 0;
+            if (item.stroke_hair) {
+                // the hairline lines binned to this pixel's tile, in drawing order
+                const uint32_t cx = (uint32_t)(lx - item.sj.x0), cy = (uint32_t)(ly - item.sj.y0);
+                if (cx < (uint32_t)item.sj.w && cy < (uint32_t)item.sj.h) {
+                    const uint32_t t = item.sj.tile_offset + cy / rvkr::kHairTile * item.sj.stride +
+                                       cx / rvkr::kHairTile;
+                    hb = a.hair_tiles[t];
+                    he = a.hair_tiles[t + 1];
                 }
-                continue;
+            } else {
+                cs = ref_cov(item.sj, a.arena, lx, ly);
             }
-            const uint32_t c = stroke ? cs : cf;
-            if (c) target = paint_px(sh, lx, ly, c, target, src, shaded);
-        }
-        if (!item.layer) {
-            d = target;
-        } else {
-            F4 s = load_px(target);
-            if (item.opacity != 1.0f) {
-                s.r *= item.opacity; s.g *= item.opacity; s.b *= item.opacity; s.a *= item.opacity;
+            if ((cf | cs) != 0 || hb != he) {
+                uint32_t target = item.layer ? 0u : d;
+                for (int pass = 0; pass < 2; pass++) {
+                    const bool stroke = (pass == 0) == (item.stroke_first != 0);
+                    const ShaderParams &sp = a.shaders[stroke ? item.stroke_shader : item.fill_shader];
+                    const ShaderHead &sh = sp;
+                    uint32_t src[4];
+                    bool shaded = false;
+                    if (stroke && item.stroke_hair) {
+                        // every blit of the hairline at this pixel, composited in order
+                        for (uint32_t k = hb; k < he; k++) {
+                            const uint32_t c = hair_alpha(a.hair_lines[a.hair_bins[k]], lx, ly);
+                            if (c) target = paint_px(sp, sh, lx, ly, c, target, src, shaded);
+                        }
+                        continue;
+                    }
+                    const uint32_t c = stroke ? cs : cf;
+                    if (c) target = paint_px(sp, sh, lx, ly, c, target, src, shaded);
+                }
+                if (!item.layer) {
+                    d = target;
+                } else {
+                    F4 s = load_px(target);
+                    if (item.opacity != 1.0f) {
+                        s.r *= item.opacity; s.g *= item.opacity; s.b *= item.opacity; s.a *= item.opacity;
+                    }
+                    d = store_px(blend(item.blend_mode, s, load_px(d)));
+                }
             }
-            d = store_px(blend(item.blend_mode, s, load_px(d)));
         }
     }
     if (inside && d != d0) px[(size_t)y * w + x] = d;

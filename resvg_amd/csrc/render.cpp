@@ -377,6 +377,25 @@ struct Renderer {
         it.x0 = std::max(x0, 0); it.y0 = std::max(y0, 0);
         it.x1 = std::min(x1, (int32_t)dl.w); it.y1 = std::min(y1, (int32_t)dl.h);
         if (it.x1 <= it.x0 || it.y1 <= it.y0) return;
+        auto cov_ref = [&](int32_t j) {
+            JobRef r{};
+            if (j < 0) return r;
+            const CovJob &jb = dl.jobs[j].j;
+            r.x0 = jb.x0; r.y0 = jb.y0; r.w = jb.w; r.h = jb.h;
+            r.stride = ((uint32_t)jb.w + 3u) & ~3u;
+            r.cov_offset = jb.cov_offset;
+            return r;
+        };
+        it.fj = cov_ref(it.fill_job);
+        if (it.stroke_hair) {
+            const HairJob &hj = dl.hair[it.stroke_job].j;
+            it.sj = JobRef{};
+            it.sj.x0 = hj.x0; it.sj.y0 = hj.y0; it.sj.w = hj.w; it.sj.h = hj.h;
+            it.sj.stride = hj.tiles_x;
+            it.sj.tile_offset = hj.tile_offset;
+        } else {
+            it.sj = cov_ref(it.stroke_job);
+        }
         dl.bx0 = std::min(dl.bx0, it.x0); dl.by0 = std::min(dl.by0, it.y0);
         dl.bx1 = std::max(dl.bx1, it.x1); dl.by1 = std::max(dl.by1, it.y1);
         dl.items.push_back(it);
