@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
 // coverage
 // ---------------------------------------------------------------------------
 struct CovArgs {
-    const uint32_t *groups;  // first global row of each row group this launch handles
+    const uint2 *groups;  // (job, first row in the job) of each row group this launch handles
     const CovJob *jobs;
     uint32_t njobs;
     const EdgeCov *edges;
@@ -160,16 +160,6 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
         if (lane >= d) v += t;
     }
     return v;
-}
-
-__device__ __forceinline__ uint32_t find_job(const CovJob *jobs, uint32_t n, uint32_t row) {
-    uint32_t lo = 0, hi = n;  // last job with row_start <= row
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].row_start <= row) lo = mid;
-        else hi = mid;
-    }
-    return lo;
 }
 
 __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
@@ -286,17 +276,17 @@ __device__ __forceinline__ void spans_of(uint64_t (*keys)[MK], const int *n, boo
 // 4, 8 or 16 KB of LDS, so the smaller variants run two to three times the waves per
 // CU).  The host sends a row group to the smallest variant that holds the number of
 // edges whose rows meet the group, an upper bound on any of its sub-scanlines'
-// crossings.  Block b handles row groups[b / kGroupRows] + b % kGroupRows.
+// crossings.  Block b handles row groups[b / kGroupRows].y + b % kGroupRows of job
+// groups[b / kGroupRows].x.
 template <int MK>
 __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
     __shared__ uint64_t keys[kMaxSub][MK];  // crossings; reused for span events
     __shared__ __align__(16) int32_t acc[kWin + 4];  // acc (low 16 bits) + run difference (high 16 bits)
     const int lane = threadIdx.x;
     // the group's first row picks the job (a job's last group can be partial)
-    const uint32_t gs = a.groups[blockIdx.x / rvkr::kGroupRows];
-    const uint32_t ji = find_job(a.jobs, a.njobs, gs);
-    const CovJob job = a.jobs[ji];
-    const int32_t r = (int32_t)(gs - job.row_start) + (int32_t)(blockIdx.x % rvkr::kGroupRows);
+    const uint2 gs = a.groups[blockIdx.x / rvkr::kGroupRows];
+    const CovJob job = a.jobs[gs.x];
+    const int32_t r = (int32_t)gs.y + (int32_t)(blockIdx.x % rvkr::kGroupRows);
     if (r < 0 || r >= job.h) return;
     const int shift = job.aa ? 2 : 0;
     const int nsub = 1 << shift;
@@ -663,13 +653,13 @@ rvk_status launch_expand(rvk_context *ctx, const EdgeRec *edges, uint32_t nedges
     return RVK_OK;
 }
 
-rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint32_t *groups128,
-                           uint32_t n128, const uint32_t *groups256, uint32_t n256, const uint32_t *groups512,
+rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint2 *groups128,
+                           uint32_t n128, const uint2 *groups256, uint32_t n256, const uint2 *groups512,
                            uint32_t n512, const EdgeCov *edges, const uint2 *windows, const PieceDev *pieces,
                            const uint32_t *npieces, uint8_t *arena, int32_t *overflow, double algo_bytes) {
     // the algorithmic bytes are split between the launches by their row-group counts
     const double total = (double)n128 + (double)n256 + (double)n512;
-    auto launch = [&](auto kernel, const uint32_t *groups, uint32_t n) -> rvk_status {
+    auto launch = [&](auto kernel, const uint2 *groups, uint32_t n) -> rvk_status {
         if (n == 0) return RVK_OK;
         CovArgs a{groups, jobs, njobs, edges, windows, pieces, npieces, arena, overflow};
         timed_launch(ctx, "coverage", algo_bytes * (double)n / total, [&] {

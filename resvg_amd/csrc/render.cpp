@@ -33,8 +33,8 @@ using namespace rvkr;
 namespace rvk {
 rvk_status launch_expand(rvk_context *ctx, const EdgeRec *edges, uint32_t nedges, const EdgeCov *cov,
                          PieceDev *pieces, uint32_t *npieces, int32_t *overflow, double algo_bytes);
-rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint32_t *groups128,
-                           uint32_t n128, const uint32_t *groups256, uint32_t n256, const uint32_t *groups512,
+rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint2 *groups128,
+                           uint32_t n128, const uint2 *groups256, uint32_t n256, const uint2 *groups512,
                            uint32_t n512, const EdgeCov *edges, const uint2 *windows, const PieceDev *pieces,
                            const uint32_t *npieces, uint8_t *arena, int32_t *overflow, double algo_bytes);
 rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
@@ -427,9 +427,9 @@ struct Renderer {
         const size_t o_hl = al(o_hb + dl.n_hair_bins * 4);
         size_t o_grp[3];
         o_grp[0] = al(o_hl + dl.n_hair_lines * sizeof(HairLine));
-        o_grp[1] = al(o_grp[0] + dl.n_groups[0] * 4);
-        o_grp[2] = al(o_grp[1] + dl.n_groups[1] * 4);
-        const size_t meta = al(o_grp[2] + dl.n_groups[2] * 4);
+        o_grp[1] = al(o_grp[0] + dl.n_groups[0] * sizeof(uint2));
+        o_grp[2] = al(o_grp[1] + dl.n_groups[1] * sizeof(uint2));
+        const size_t meta = al(o_grp[2] + dl.n_groups[2] * sizeof(uint2));
         const size_t o_np = meta;
         const size_t o_pc = al(o_np + dl.n_edges * 4);
         const size_t o_arena = al(o_pc + (size_t)dl.piece_slots * sizeof(PieceDev));
@@ -468,10 +468,10 @@ struct Renderer {
                 if (!P.windows.empty())
                     std::memcpy((uint2 *)(host + o_win) + j.window_offset, P.windows.data(),
                                 P.windows.size() * sizeof(uint2));
-                uint32_t *gl[3];
-                for (int t = 0; t < 3; t++) gl[t] = (uint32_t *)(host + o_grp[t]) + gbase[i * 3 + t];
+                uint2 *gl[3];
+                for (int t = 0; t < 3; t++) gl[t] = (uint2 *)(host + o_grp[t]) + gbase[i * 3 + t];
                 for (size_t g = 0; g < P.tier.size(); g++)
-                    *gl[P.tier[g]]++ = j.row_start + (uint32_t)(g * kGroupRows);
+                    *gl[P.tier[g]]++ = uint2{(uint32_t)i, (uint32_t)(g * kGroupRows)};
             } else {
                 const size_t k = i - njobs;
                 const HairPrep &P = dl.hair[k];
@@ -513,9 +513,9 @@ struct Renderer {
                                     dev_overflow, exp_bytes);
         if (st == RVK_OK)
             st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)njobs,
-                                      (const uint32_t *)(d + o_grp[0]), (uint32_t)dl.n_groups[0],
-                                      (const uint32_t *)(d + o_grp[1]), (uint32_t)dl.n_groups[1],
-                                      (const uint32_t *)(d + o_grp[2]), (uint32_t)dl.n_groups[2],
+                                      (const uint2 *)(d + o_grp[0]), (uint32_t)dl.n_groups[0],
+                                      (const uint2 *)(d + o_grp[1]), (uint32_t)dl.n_groups[1],
+                                      (const uint2 *)(d + o_grp[2]), (uint32_t)dl.n_groups[2],
                                       (const EdgeCov *)(d + o_ecov), (const uint2 *)(d + o_win),
                                       (const PieceDev *)(d + o_pc), (const uint32_t *)(d + o_np), d + o_arena,
                                       dev_overflow, cov_bytes);
