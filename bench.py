@@ -14,7 +14,8 @@ Prints one JSON line: Mpixel/s of the document, the roofline of the dominant ker
 (algorithmic bytes / its HIP-event time over the timed steps, against 8 TB/s HBM),
 the oracle's CPU throughput on a bounded sample of the same document, and under
 "batched" the documents/s of rvk_render_batch against a loop of rvk_render calls on a
-sample of the batched workload.
+sample of the batched workload, and under "differentiable" the f32 forward and
+forward+backward times of the differentiable form against the plain u8 forward.
 """
 from __future__ import annotations
 
@@ -149,6 +150,56 @@ def batched(ctx, n_docs, seed, steps):
             "sample": f"{n_docs} documents of the 4096-document batched distribution (seed {seed})"}
 
 
+def differentiable(ctx, size, n_paths, seed, reps):
+    """The differentiable f32 form at `size`² with `n_paths` paths: forward and
+    forward+backward of rvk_diff_forward / rvk_diff_backward (the coverage planes are
+    computed once, before the timed region, since the geometry is not learned),
+    against the plain u8 forward (rvk_render) of the same document."""
+    import torch
+    from resvg_amd import abi, diff
+    from resvg_amd.workloads import diff_scene
+    t0 = time.perf_counter()
+    w, h, paths, ends, p0, pt = diff_scene(seed=seed, size=size, n_paths=n_paths)
+    r = diff.Renderer(ctx, w, h, paths, ends)
+    params = torch.as_tensor(p0, device="cuda")
+    target = r.forward(torch.as_tensor(pt, device="cuda")).clone()
+    u8 = diff.u8_scene(w, h, paths, ends, p0).to_abi()
+    px = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    print(f"[bench] differentiable: scene and coverage ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+
+    def fwd():
+        return r.forward(params)
+
+    def fwd_bwd():
+        out = r.forward(params)
+        return r.backward(params, 2.0 * (out - target) / out.numel())
+
+    def plain():
+        px.zero_()
+        ctx.render(u8, abi.Transform.identity(), px)
+
+    res = {}
+    for name, fn in (("fwd", fwd), ("fwd_bwd", fwd_bwd), ("plain", plain)):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t) / reps * 1e3
+    loss = float(((r.forward(params) - target) ** 2).mean())
+    ctx.profile_begin()
+    fwd_bwd()
+    kern = [{"name": k["name"], "launches": k["launches"], "ms": round(k["total_ms"], 3)} for k in ctx.profile_end()]
+    return {"size": size, "paths": n_paths, "params": int(params.numel()), "f32_forward_ms": round(res["fwd"], 3),
+            "f32_forward_backward_ms": round(res["fwd_bwd"], 3), "plain_u8_forward_ms": round(res["plain"], 3),
+            "fwd_bwd_over_plain": round(res["fwd_bwd"] / res["plain"], 3), "loss": loss, "reps": reps,
+            "gradient_flows_to": "paint, opacity, blur sigma, colour matrix (not the geometry)",
+            "kernels_fwd_bwd": kern[:8]}
+
+
 def _primitive_rooflines(kernels, steps):
     """Per filter primitive / pipeline stage: its kernels' time per step and the
     achieved fraction of HBM peak on the primitive's algorithmic bytes."""
@@ -182,6 +233,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-docs", type=int, default=512, help="documents of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--diff-size", type=int, default=1024, help="canvas side of the differentiable leg (0: skip)")
+    ap.add_argument("--diff-paths", type=int, default=256)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 1 GiB device buffer once (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -239,6 +292,7 @@ def main() -> int:
     kernels = ctx.profile_end()
     st = ctx.stats()  # of the last render call
     bat = batched(ctx, args.batch_docs, args.seed, args.batch_steps) if args.batch_docs > 0 else None
+    dif = differentiable(ctx, args.diff_size, args.diff_paths, args.seed, 5) if args.diff_size > 0 else None
     host = {"build_ms": round(st.host_build_ms, 1), "wait_ms": round(st.host_wait_ms, 1), "flushes": st.flushes,
             "edge_piece_slots": st.edge_pieces, "upload_MB": round(st.upload_bytes / 1e6, 1),
             "kernel_launches": st.kernel_launches, "layers": st.layers}
@@ -291,6 +345,7 @@ def main() -> int:
         "wall_s_per_step": round(t_wall / args.steps, 3),
         "host_last_step": host,
         "batched": bat,
+        "differentiable": dif,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

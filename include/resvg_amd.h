@@ -368,6 +368,39 @@ rvk_status rvk_render_batch(rvk_context *ctx, uint32_t count, const rvk_scene *c
                             const rvk_transform *transforms, const uint32_t *widths, const uint32_t *heights,
                             uint8_t *const *dev_pixmaps, rvk_status *statuses);
 
+/* ------------------------------------------------------------------------- */
+/* Differentiable f32 form (no counterpart in the reference, which renders u8 only) */
+/* ------------------------------------------------------------------------- */
+/* A scene of the differentiable renderer: per path a u8 coverage plane (the
+ * anti-aliased fill coverage the u8 pipeline computes for the path, width*height
+ * bytes, path order) and the end points (x0, y0, x1, y1) of its pad-spread linear
+ * gradient in pixels.  The paint, group opacity, blur σ and colour matrix are
+ * parameters.  Model: paths composite SourceOver (path.rs:26-75, render.rs:133) with
+ * source (rgb·tint·A, A), A = coverage·opacity·stop alpha, the stop colour
+ * interpolated between 4 stops (path.rs:118-177); then feGaussianBlur (separable
+ * Gaussian, radius ceil(3σ) <= 48, zero outside) -> feColorMatrix on the
+ * demultiplied colour (color_matrix.rs:11-110) -> composite over the paths' canvas
+ * (filter/mod.rs:731-793).  Everything is f32 premultiplied RGBA. */
+typedef struct rvk_diff_scene {
+    uint32_t width, height, path_count;
+    const uint8_t *dev_coverage;      /* path_count x height x width */
+    const float *dev_gradient_ends;   /* path_count x 4 floats, 16-byte aligned */
+} rvk_diff_scene;
+/* Parameters per path: tint r, g, b; opacity; 4 stop colours (r, g, b, a); 4 stop
+ * offsets (ascending).  After the paths: σ, then the 4x5 colour matrix row-major. */
+#define RVK_DIFF_PATH_PARAMS 24
+rvk_status rvk_diff_workspace_size(uint32_t width, uint32_t height, uint32_t path_count, size_t *bytes);
+/* Renders the scene with `dev_params` into dev_out (width x height x 4 floats) and
+ * keeps what the backward pass needs (canvas checkpoints every 8 paths, the blur
+ * intermediates) in dev_workspace (rvk_diff_workspace_size bytes). */
+rvk_status rvk_diff_forward(rvk_context *ctx, const rvk_diff_scene *scene, const float *dev_params,
+                            void *dev_workspace, float *dev_out);
+/* From dL/d(out) (width x height x 4 floats) writes dL/d(params) (same layout as the
+ * parameters; no gradient flows to the coverage).  Must follow rvk_diff_forward
+ * with the same scene, parameters and workspace. */
+rvk_status rvk_diff_backward(rvk_context *ctx, const rvk_diff_scene *scene, const float *dev_params,
+                             void *dev_workspace, const float *dev_grad_out, float *dev_grad_params);
+
 /* Per-call statistics of the last rvk_render / rvk_filter_apply on this context. */
 typedef struct rvk_stats {
     uint64_t kernel_launches;

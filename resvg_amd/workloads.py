@@ -296,3 +296,34 @@ def document_batch(n_docs: int, seed: int = 0, min_side: int = 64, max_side: int
         k = int(rng.integers(1, 7))
         docs.append((batch_doc(int(rng.integers(0, 2**31)), w, h, n, k), (w, h)))
     return docs
+
+
+def diff_scene(seed: int = 0, size: int = 1024, n_paths: int = 256, segments: int = 8, sigma: float = 4.0):
+    """The differentiable workload: `n_paths` closed paths of `segments` cubics with
+    points uniform in the canvas, each with a linear gradient between two uniform
+    points.  Returns (width, height, paths, ends, params, target_params): parameters
+    in resvg_amd.diff's layout (tint, opacity, 4 stops, 4 sorted offsets per path,
+    then σ and a colour matrix near the identity), and the target's parameters
+    perturbed by N(0, 0.05)."""
+    from .diff import PATH_PARAMS
+    rng = np.random.default_rng(seed)
+    paths, ends, params = [], [], []
+    for _ in range(n_paths):
+        pts = rng.uniform(0, size, size=(1 + 3 * segments, 2)).astype(F32)
+        pts[-1] = pts[0]
+        verbs = [abi.VERB_MOVE] + [abi.VERB_CUBIC] * segments + [abi.VERB_CLOSE]
+        paths.append((verbs, pts))
+        ends.append(rng.uniform(0, size, size=4).astype(F32))
+        q = np.empty(PATH_PARAMS, np.float64)
+        q[0:3] = rng.uniform(0.5, 1.0, 3)
+        q[3] = rng.uniform(0.3, 1.0)
+        q[4:20] = rng.uniform(0.1, 0.9, 16)
+        q[20:24] = np.sort(rng.uniform(0.05, 0.95, 4))
+        params.append(q)
+    m = np.eye(4, 5) + rng.normal(0.0, 0.1, (4, 5))
+    flat = np.concatenate(params + [np.array([sigma]), m.reshape(-1)]).astype(np.float32)
+    target = (flat.astype(np.float64) + rng.normal(0.0, 0.05, flat.shape)).astype(np.float32)
+    for i in range(n_paths):  # keep the target's offsets ascending
+        o = target[i * PATH_PARAMS + 20:i * PATH_PARAMS + 24]
+        target[i * PATH_PARAMS + 20:i * PATH_PARAMS + 24] = np.sort(o)
+    return size, size, paths, ends, flat, target

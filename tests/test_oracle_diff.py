@@ -1,0 +1,106 @@
+"""The FP64 oracle of the differentiable form against closed forms (CPU only)."""
+import numpy as np
+
+from oracle import diff_ref as D
+
+P = D.PATH_PARAMS
+
+
+def _params(n, sigma, m=None):
+    p = np.zeros(n * P + 21)
+    for i in range(n):
+        q = p[i * P:(i + 1) * P]
+        q[0:3] = 1.0
+        q[3] = 1.0
+        q[20:24] = [0.1, 0.4, 0.6, 0.9]
+    p[n * P] = sigma
+    p[n * P + 1:] = (np.eye(4, 5) if m is None else m).reshape(-1)
+    return p
+
+
+def test_blur_weights_normalised_and_small_sigma_is_identity():
+    for s in (0.05, 0.7, 3.3, 16.0, 40.0):
+        r, w = D.blur_weights(s)
+        assert abs(w.sum() - 1.0) < 1e-12 and r <= D.MAX_RADIUS
+    r, w = D.blur_weights(0.05)
+    assert r == 1 and w[1] > 1.0 - 1e-12
+
+
+def test_single_solid_path_closed_form():
+    # one fully covering path, all stops the same colour: the canvas is that colour
+    # premultiplied by alpha·opacity; with a tiny σ and the identity matrix the
+    # output is F + (1 - F_a) S with F = S
+    h, w = 6, 7
+    cov = np.full((1, h, w), 255, np.uint8)
+    p = _params(1, 0.05)
+    p[4:20] = np.tile([0.2, 0.4, 0.8, 0.5], 4)
+    p[3] = 0.6
+    out = D.forward(cov, np.array([[0, 0, 7, 0]], np.float32), p, w, h)
+    a = 0.5 * 0.6
+    s = np.array([0.2 * a, 0.4 * a, 0.8 * a, a])
+    np.testing.assert_allclose(out, np.broadcast_to(s + (1 - a) * s, out.shape), atol=1e-12)
+
+
+def test_gradient_interpolates_between_stops():
+    # t = (x + 0.5) / w along the row: at each pixel the colour is the lerp of the
+    # two stops around t
+    h, w = 1, 10
+    cov = np.full((1, h, w), 255, np.uint8)
+    p = _params(1, 0.05)
+    cols = np.array([[1, 0, 0, 1], [0, 1, 0, 1], [0, 0, 1, 1], [1, 1, 1, 1]], float)
+    p[4:20] = cols.reshape(-1)
+    s = D._paths(cov, np.array([[0, 0, w, 0]], np.float32), p, w, h)
+    off = p[20:24]
+    for x in range(w):
+        t = (x + 0.5) / w
+        if t <= off[0]:
+            exp = cols[0]
+        elif t >= off[3]:
+            exp = cols[3]
+        else:
+            k = int(np.searchsorted(off, t, side="right")) - 1
+            u = (t - off[k]) / (off[k + 1] - off[k])
+            exp = cols[k] + u * (cols[k + 1] - cols[k])
+        np.testing.assert_allclose(s[0, x], np.concatenate([exp[:3] * exp[3], exp[3:]]), atol=1e-12)
+
+
+def test_opacity_gradient_matches_closed_form():
+    # S = c·op·a·rgb over nothing, then out = S + (1 - S_a)·S (tiny σ, identity
+    # matrix); d(sum out)/d(op) in closed form
+    h, w = 2, 3
+    cov = np.full((1, h, w), 128, np.uint8)
+    p = _params(1, 0.05)
+    p[4:20] = np.tile([0.3, 0.6, 0.9, 0.8], 4)
+    op = 0.7
+    p[3] = op
+    c = 128 / 255
+    ends = np.array([[0, 0, 3, 0]], np.float32)
+    g = D.grad_fd(cov, ends, p, w, h, np.ones((h, w, 4)), indices=[3])
+    # per pixel: A = c op a ; S = (rgb A, A) ; out = S (2 - A) ; sum = (sum rgb + 1) A (2 - A)
+    a, k = 0.8, 0.3 + 0.6 + 0.9 + 1.0
+    dA = c * a
+    A = c * op * a
+    exp = h * w * k * dA * (2.0 - 2.0 * A)
+    assert abs(g[3] - exp) < 1e-6 * abs(exp)
+
+
+def test_matrix_identity_and_clamp():
+    b = np.array([[[0.2, 0.1, 0.4, 0.5], [0.0, 0.0, 0.0, 0.0], [0.9, 0.9, 0.9, 0.9]]])
+    f = D._color_matrix(b, np.eye(4, 5))
+    np.testing.assert_allclose(f, b, atol=1e-12)
+    m = np.eye(4, 5)
+    m[0, 4] = 2.0  # red saturates at 1
+    f = D._color_matrix(b, m)
+    np.testing.assert_allclose(f[..., 0], b[..., 3])
+
+
+def test_blur_of_an_impulse_is_the_kernel_outer_product():
+    h, w, s = 31, 29, 2.2
+    img = np.zeros((h, w, 4))
+    img[15, 14] = [1.0, 0.5, 0.25, 1.0]
+    out = D._blur(img, s)
+    r, k = D.blur_weights(s)
+    exp = np.zeros((h, w))
+    exp[15 - r:15 + r + 1, 14 - r:14 + r + 1] = np.outer(k, k)
+    np.testing.assert_allclose(out[..., 0], exp, atol=1e-15)
+    np.testing.assert_allclose(out[..., 1], 0.5 * exp, atol=1e-15)
