@@ -152,14 +152,16 @@ def batched(ctx, n_docs, seed, steps):
 
 def differentiable(ctx, size, n_paths, seed, reps):
     """The differentiable f32 form at `size`² with `n_paths` paths: forward and
-    forward+backward of rvk_diff_forward / rvk_diff_backward (the coverage planes are
-    computed once, before the timed region, since the geometry is not learned),
-    against the plain u8 forward (rvk_render) of the same document."""
+    forward+backward of rvk_diff_forward / rvk_diff_backward against the plain u8
+    forward (rvk_render) of the same document; in the planes mode (coverage computed
+    once, before the timed region) and in the analytic mode (coverage from the
+    geometry on every forward, geometry gradients in the backward)."""
     import torch
     from resvg_amd import abi, diff
     from resvg_amd.workloads import diff_scene
     t0 = time.perf_counter()
-    w, h, paths, ends, p0, pt = diff_scene(seed=seed, size=size, n_paths=n_paths)
+    segments = 8
+    w, h, paths, ends, p0, pt = diff_scene(seed=seed, size=size, n_paths=n_paths, segments=segments)
     r = diff.Renderer(ctx, w, h, paths, ends)
     params = torch.as_tensor(p0, device="cuda")
     target = r.forward(torch.as_tensor(pt, device="cuda")).clone()
@@ -193,11 +195,46 @@ def differentiable(ctx, size, n_paths, seed, reps):
     ctx.profile_begin()
     fwd_bwd()
     kern = [{"name": k["name"], "launches": k["launches"], "ms": round(k["total_ms"], 3)} for k in ctx.profile_end()]
+    del r
+    torch.cuda.empty_cache()
+
+    # the analytic mode: the same paths as closed chains of cubics whose control points
+    # and affine are parameters too; the coverage is rebuilt from the geometry on every
+    # forward and the backward also returns dL/d(geometry)
+    ra = diff.Renderer(ctx, w, h, paths, ends, cubics=segments)
+    geom = ra.geom0.clone()
+    target_a = ra.forward(torch.as_tensor(pt, device="cuda"), geom).clone()
+
+    def fwd_a():
+        return ra.forward(params, geom)
+
+    def fwd_bwd_a():
+        out = ra.forward(params, geom)
+        return ra.backward(params, 2.0 * (out - target_a) / out.numel(), geom)
+
+    for name, fn in (("fwd_a", fwd_a), ("fwd_bwd_a", fwd_bwd_a)):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t) / reps * 1e3
+    ctx.profile_begin()
+    fwd_bwd_a()
+    kern_a = [{"name": k["name"], "launches": k["launches"], "ms": round(k["total_ms"], 3)} for k in ctx.profile_end()]
+    analytic = {"cubics_per_path": segments, "geometry_params": int(geom.numel()),
+                "f32_forward_ms": round(res["fwd_a"], 3), "f32_forward_backward_ms": round(res["fwd_bwd_a"], 3),
+                "fwd_bwd_over_plain": round(res["fwd_bwd_a"] / res["plain"], 3),
+                "gradient_flows_to": "paint, opacity, blur sigma, colour matrix, control points, affine",
+                "kernels_fwd_bwd": kern_a[:10]}
+    del ra
+    torch.cuda.empty_cache()
     return {"size": size, "paths": n_paths, "params": int(params.numel()), "f32_forward_ms": round(res["fwd"], 3),
             "f32_forward_backward_ms": round(res["fwd_bwd"], 3), "plain_u8_forward_ms": round(res["plain"], 3),
             "fwd_bwd_over_plain": round(res["fwd_bwd"] / res["plain"], 3), "loss": loss, "reps": reps,
-            "gradient_flows_to": "paint, opacity, blur sigma, colour matrix (not the geometry)",
-            "kernels_fwd_bwd": kern[:8]}
+            "gradient_flows_to": "paint, opacity, blur sigma, colour matrix (coverage planes fixed)",
+            "kernels_fwd_bwd": kern[:8], "analytic_geometry": analytic}
 
 
 def _primitive_rooflines(kernels, steps):

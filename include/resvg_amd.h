@@ -383,23 +383,35 @@ rvk_status rvk_render_batch(rvk_context *ctx, uint32_t count, const rvk_scene *c
  * (filter/mod.rs:731-793).  Everything is f32 premultiplied RGBA. */
 typedef struct rvk_diff_scene {
     uint32_t width, height, path_count;
-    const uint8_t *dev_coverage;      /* path_count x height x width */
+    /* 0: coverage from dev_coverage; k > 0: analytic mode, every path a closed chain
+     * of k cubics given by the geometry parameters (coverage = min(1, |winding
+     * area|) of the path flattened into 16 segments per cubic), with gradients to
+     * the control points and the path's affine transform */
+    uint32_t cubics;
+    const uint8_t *dev_coverage;      /* cubics == 0: path_count x height x width */
     const float *dev_gradient_ends;   /* path_count x 4 floats, 16-byte aligned */
 } rvk_diff_scene;
 /* Parameters per path: tint r, g, b; opacity; 4 stop colours (r, g, b, a); 4 stop
  * offsets (ascending).  After the paths: σ, then the 4x5 colour matrix row-major. */
 #define RVK_DIFF_PATH_PARAMS 24
-rvk_status rvk_diff_workspace_size(uint32_t width, uint32_t height, uint32_t path_count, size_t *bytes);
-/* Renders the scene with `dev_params` into dev_out (width x height x 4 floats) and
- * keeps what the backward pass needs (canvas checkpoints every 8 paths, the blur
- * intermediates) in dev_workspace (rvk_diff_workspace_size bytes). */
+/* Geometry parameters per path (analytic mode): the affine (sx, ky, kx, sy, tx, ty;
+ * device = (sx x + kx y + tx, ky x + sy y + ty)) and the 3k + 1 user-space points
+ * P0 .. P3k (x, y); the polygon closes from P3k back to P0. */
+#define RVK_DIFF_GEOM_FLOATS(k) (6 + 2 * (3 * (k) + 1))
+rvk_status rvk_diff_workspace_size(uint32_t width, uint32_t height, uint32_t path_count, uint32_t cubics,
+                                   size_t *bytes);
+/* Renders the scene with `dev_params` (and `dev_geom` in the analytic mode, else
+ * NULL) into dev_out (width x height x 4 floats) and keeps what the backward pass
+ * needs (canvas checkpoints every 8 paths, the blur intermediates, the coverage
+ * planes) in dev_workspace (rvk_diff_workspace_size bytes). */
 rvk_status rvk_diff_forward(rvk_context *ctx, const rvk_diff_scene *scene, const float *dev_params,
-                            void *dev_workspace, float *dev_out);
-/* From dL/d(out) (width x height x 4 floats) writes dL/d(params) (same layout as the
- * parameters; no gradient flows to the coverage).  Must follow rvk_diff_forward
- * with the same scene, parameters and workspace. */
+                            const float *dev_geom, void *dev_workspace, float *dev_out);
+/* From dL/d(out) (width x height x 4 floats) writes dL/d(params) and, in the
+ * analytic mode, dL/d(geom).  Must follow rvk_diff_forward with the same scene,
+ * parameters, geometry and workspace. */
 rvk_status rvk_diff_backward(rvk_context *ctx, const rvk_diff_scene *scene, const float *dev_params,
-                             void *dev_workspace, const float *dev_grad_out, float *dev_grad_params);
+                             const float *dev_geom, void *dev_workspace, const float *dev_grad_out,
+                             float *dev_grad_params, float *dev_grad_geom);
 
 /* Per-call statistics of the last rvk_render / rvk_filter_apply on this context. */
 typedef struct rvk_stats {

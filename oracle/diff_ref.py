@@ -73,11 +73,15 @@ def _blur(img, sigma):
     return out2
 
 
+ALPHA_FLOOR = 1.0 / 255.0
+
+
 def _color_matrix(b, m):
+    # demultiplied by max(a, 1/255): exact for the alphas of a u8 pixmap (non-zero ones
+    # are >= 1/255), a continuous ramp to zero below
     a = b[..., 3]
-    safe = np.where(a > 0, a, 1.0)
-    v = np.stack([np.where(a > 0, b[..., 0] / safe, 0.0), np.where(a > 0, b[..., 1] / safe, 0.0),
-                  np.where(a > 0, b[..., 2] / safe, 0.0), a], -1)
+    den = np.maximum(a, ALPHA_FLOOR)
+    v = np.stack([b[..., 0] / den, b[..., 1] / den, b[..., 2] / den, a], -1)
     o = v @ m[:, :4].T + m[:, 4]
     q = np.clip(o, 0.0, 1.0)
     return np.concatenate([q[..., :3] * q[..., 3:4], q[..., 3:4]], -1)
@@ -107,4 +111,123 @@ def grad_fd(cov, ends, params, width, height, grad_out, indices=None, h=1e-6):
         b[i] -= step
         g[i] = (np.sum(go * forward(cov, ends, a, width, height)) -
                 np.sum(go * forward(cov, ends, b, width, height))) / (2.0 * step)
+    return g
+
+
+# ---------------------------------------------------------------------------
+# Analytic coverage: closed chains of k cubics (user-space points P0 .. P3k after a
+# path's affine sx, ky, kx, sy, tx, ty), flattened into FLAT segments per cubic at
+# uniform t and closed from P3k to P0; per pixel, the integral of the winding number
+# over the pixel (signed area accumulation along rows), coverage min(1, |area|).
+# ---------------------------------------------------------------------------
+FLAT = 16
+
+
+def geom_floats(k):
+    return 6 + 2 * (3 * k + 1)
+
+
+def flatten(g, k):
+    a = np.asarray(g[:6], np.float64)
+    pts = np.asarray(g[6:6 + 2 * (3 * k + 1)], np.float64).reshape(-1, 2)
+    out = []
+    for j in range(k * FLAT + 1):
+        c = min(j // FLAT, k - 1)
+        t = (j - c * FLAT) / FLAT
+        u = 1.0 - t
+        b = (u * u * u, 3 * u * u * t, 3 * u * t * t, t * t * t)
+        x = sum(b[m] * pts[3 * c + m, 0] for m in range(4))
+        y = sum(b[m] * pts[3 * c + m, 1] for m in range(4))
+        out.append((a[0] * x + a[2] * y + a[4], a[1] * x + a[3] * y + a[5]))
+    return out
+
+
+def winding_area(verts, w, h):
+    """Per pixel, the integral of the polygon's winding number (rows x columns)."""
+    acc = np.zeros((h, w + 2))
+    nv = len(verts)
+    for j in range(nv):
+        (ax, ay), (bx, by) = verts[j], verts[(j + 1) % nv]
+        if ay == by:
+            continue
+        d = 1.0
+        if ay > by:
+            ax, ay, bx, by, d = bx, by, ax, ay, -1.0
+        dxdy = (bx - ax) / (by - ay)
+        for y in range(max(0, math.floor(ay)), min(h, math.ceil(by))):
+            ya, yb = max(ay, y), min(by, y + 1)
+            dy = yb - ya
+            if dy <= 0:
+                continue
+            xa = min(max(ax + dxdy * (ya - ay), 0.0), w)
+            xb = min(max(ax + dxdy * (yb - ay), 0.0), w)
+            dd = dy * d
+            x0, x1 = min(xa, xb), max(xa, xb)
+            x0f = math.floor(x0)
+            x0i = int(x0f)
+            x1c = math.ceil(x1)
+            if x1c <= x0f + 1:
+                xm = 0.5 * (xa + xb) - x0f
+                acc[y, x0i] += dd * (1 - xm)
+                acc[y, x0i + 1] += dd * xm
+            else:
+                # the row's trapezoid between x0 and x1, split by the pixel columns
+                s = 1.0 / (x1 - x0)
+                x1i = int(x1c)
+                for xi in range(x0i, x1i + 1):
+                    # area, within [xi, xi + 1), of the part of the row right of the edge
+                    # = integral over the row of clamp(xi + 1 - x(y), 0, 1) minus the
+                    # same for xi - 1 (the accumulation's difference form)
+                    acc[y, xi] += dd * (_ramp(xi + 1, x0, x1, s) - _ramp(xi, x0, x1, s))
+    return np.cumsum(acc[:, :w], axis=1)
+
+
+def _ramp(c, x0, x1, s):
+    """Mean over the row of clamp(c - x, 0, 1) for x uniform in [x0, x1] (s = 1 / (x1 - x0))."""
+    return (_antider(c, x1) - _antider(c, x0)) * s
+
+
+def _antider(c, x):
+    # G(x) = ∫_{-inf..x} clamp(c - t, 0, 1) dt up to a constant: slope 1 for t < c - 1,
+    # a parabola on [c - 1, c], flat after c
+    if x <= c - 1:
+        return x
+    if x >= c:
+        return c - 0.5
+    u = x - (c - 1)
+    return (c - 1) + u - 0.5 * u * u
+
+
+def analytic_coverage(geom, k, w, h):
+    g = np.asarray(geom, np.float64).reshape(-1, geom_floats(k))
+    return np.stack([np.minimum(1.0, np.abs(winding_area(flatten(gi, k), w, h))) for gi in g])
+
+
+def forward_geom(geom, k, ends, params, width, height):
+    """The analytic mode's output: coverage from the geometry, the rest as forward()."""
+    cov = analytic_coverage(geom, k, width, height)
+    return _forward_cov(cov, ends, params, width, height)
+
+
+def _forward_cov(cov_f, ends, params, width, height):
+    p = np.asarray(params, np.float64)
+    n = cov_f.shape[0]
+    s = _paths(cov_f * 255.0, ends, p, width, height)
+    b = _blur(s, float(p[n * PATH_PARAMS]))
+    m = p[n * PATH_PARAMS + 1:n * PATH_PARAMS + 21].reshape(4, 5)
+    f = _color_matrix(b, m)
+    return f + (1.0 - f[..., 3:4]) * s
+
+
+def grad_fd_geom(geom, k, ends, params, width, height, grad_out, indices, h=1e-6):
+    g0 = np.asarray(geom, np.float64).reshape(-1)
+    go = np.asarray(grad_out, np.float64)
+    g = np.zeros(len(g0))
+    for i in indices:
+        step = h * max(1.0, abs(g0[i]))
+        a, b = g0.copy(), g0.copy()
+        a[i] += step
+        b[i] -= step
+        g[i] = (np.sum(go * forward_geom(a, k, ends, params, width, height)) -
+                np.sum(go * forward_geom(b, k, ends, params, width, height))) / (2.0 * step)
     return g

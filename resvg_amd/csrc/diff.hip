@@ -4,9 +4,12 @@
 // thread per pixel.
 //
 // Model (the reference's operations restated in f32, resvg 0.48.1):
-//   * path i covers pixel p with c_i(p) = coverage_u8 / 255, the anti-aliased
-//     coverage of its outline (tiny-skia fill_path, path.rs:26-75), computed by the
-//     u8 rasterizer and fixed here (no gradient flows to the geometry);
+//   * path i covers pixel p with c_i(p): either coverage_u8 / 255, the anti-aliased
+//     coverage of its outline (tiny-skia fill_path, path.rs:26-75) computed by the
+//     u8 rasterizer and fixed (no gradient to the geometry), or, in the analytic
+//     mode, min(1, |∫ winding|) over the pixel of the path's flattened cubics mapped
+//     by its affine transform, whose gradient flows to the control points and the
+//     transform through the boundary term;
 //   * its paint is a pad-spread linear gradient (path.rs:118-177) along
 //     (x0, y0) -> (x1, y1) with 4 stops (offset, unpremultiplied RGBA), times a
 //     per-path RGB tint, times a per-path opacity (the group opacity of
@@ -40,6 +43,23 @@ struct Dims {
     uint32_t w, h, n;        // width, height, paths
     uint32_t nblk;           // blocks of the per-pixel kernels
 };
+
+// Where the coverage of path i at pixel p comes from: u8 planes (coverage / 255), or
+// the winding-area planes of the analytic mode (coverage min(1, |area|); row stride
+// w + 2, see k_diff_accum).  In the analytic mode the backward also writes
+// dL/d(area) per path and pixel into gW.
+struct Cov {
+    const uint8_t *u8;
+    const float *area;
+    float *gW;
+};
+__device__ __forceinline__ size_t area_index(const Dims &d, uint32_t i, size_t p) {
+    return ((size_t)i * d.h + p / d.w) * (d.w + 2) + p % d.w;
+}
+__device__ __forceinline__ float cov_at(const Cov &cv, const Dims &d, uint32_t i, size_t p) {
+    if (cv.u8) return (float)cv.u8[(size_t)i * d.w * d.h + p] * (1.0f / 255.0f);
+    return fminf(1.0f, fabsf(cv.area[area_index(d, i, p)]));
+}
 
 // ---- paint ------------------------------------------------------------------
 struct PathEval {
@@ -113,7 +133,7 @@ __device__ __forceinline__ float wave_sum32(const float (&v)[32], int lane) {
 }
 
 // ---- forward: paths -----------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_diff_paths_fwd(Dims d, const uint8_t *__restrict__ cov,
+__global__ __launch_bounds__(kThreads) void k_diff_paths_fwd(Dims d, Cov cv,
                                                               const float4 *__restrict__ ends,
                                                               const float *__restrict__ params, float4 *canvas,
                                                               float4 *ck) {
@@ -124,7 +144,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_fwd(Dims d, const uint8
     float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (uint32_t i = 0; i < d.n; i++) {
         if (i % kSeg == 0) ck[(size_t)(i / kSeg) * np + p] = s;
-        const float c = (float)cov[(size_t)i * np + p] * (1.0f / 255.0f);
+        const float c = cov_at(cv, d, i, p);
         if (c == 0.0f) continue;
         const float *pp = params + (size_t)i * kPP;
         s = over(pp, eval_path(pp, ends[i], c, px, py), s);
@@ -219,12 +239,17 @@ struct CmEval {
     float o[4];  // matrix output before clamping
     float q[4];  // clamped
 };
+// The colour is demultiplied by max(a, 1/255): exact for every alpha a u8 pixmap can
+// hold (its non-zero alphas are >= 1/255), and below that a continuous ramp to zero,
+// so that alphas which are rounding residue (the blur's far tails, cancelled winding
+// sums) cannot turn into full-strength colours.
+constexpr float kAlphaFloor = 1.0f / 255.0f;
 __device__ __forceinline__ CmEval cm_eval(const float *m, float4 b) {
     CmEval e;
-    const float a = b.w;
-    e.v[0] = a > 0.0f ? b.x / a : 0.0f;
-    e.v[1] = a > 0.0f ? b.y / a : 0.0f;
-    e.v[2] = a > 0.0f ? b.z / a : 0.0f;
+    const float a = b.w, r = 1.0f / fmaxf(a, kAlphaFloor);
+    e.v[0] = b.x * r;
+    e.v[1] = b.y * r;
+    e.v[2] = b.z * r;
     e.v[3] = a;
     for (int c = 0; c < 4; c++) {
         const float *r = m + 5 * c;
@@ -281,16 +306,13 @@ __global__ __launch_bounds__(kThreads) void k_diff_cm_bwd(Dims d, const float4 *
             }
             gm[5 * c + 4] = go[c];
         }
-        const float a = b.w;
+        const float a = b.w, r = 1.0f / fmaxf(a, kAlphaFloor);
         float4 gb;
-        if (a > 0.0f) {
-            gb.x = gv[0] / a;
-            gb.y = gv[1] / a;
-            gb.z = gv[2] / a;
-            gb.w = gv[3] - (gv[0] * b.x + gv[1] * b.y + gv[2] * b.z) / (a * a);
-        } else {
-            gb = make_float4(0.0f, 0.0f, 0.0f, gv[3]);
-        }
+        gb.x = gv[0] * r;
+        gb.y = gv[1] * r;
+        gb.z = gv[2] * r;
+        // v_c = b_c / a depends on a only above the floor
+        gb.w = a > kAlphaFloor ? gv[3] - (gv[0] * e.v[0] + gv[1] * e.v[1] + gv[2] * e.v[2]) * r : gv[3];
         gB[p] = gb;
     }
     for (int j = 0; j < 20; j++) {
@@ -309,7 +331,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_cm_bwd(Dims d, const float4 *
 // dL/dS arrives as gS (direct term) + gS2 (through the blur).  Segments are walked
 // from the last; each is recomputed from its checkpoint, then reversed.  The block's
 // sums of the per-path gradients go to part[(blockIdx.x * n + i) * kPP ..].
-__global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, const uint8_t *__restrict__ cov,
+__global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, Cov cv,
                                                               const float4 *__restrict__ ends,
                                                               const float *__restrict__ params,
                                                               const float4 *__restrict__ ck,
@@ -336,7 +358,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, const uint8
             canv[j + 1] = canv[j];
             const uint32_t i = i0 + j;
             if (live && i < d.n) {
-                const float c = (float)cov[(size_t)i * np + p] * (1.0f / 255.0f);
+                const float c = cov_at(cv, d, i, p);
                 if (c != 0.0f) {
                     const float *pp = params + (size_t)i * kPP;
                     canv[j + 1] = over(pp, eval_path(pp, ends[i], c, px, py), canv[j]);
@@ -350,7 +372,8 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, const uint8
             float gp[32];  // kPP gradients, padded to 32 for the butterfly
 #pragma unroll
             for (int q = 0; q < 32; q++) gp[q] = 0.0f;
-            const float c = live ? (float)cov[(size_t)i * np + p] * (1.0f / 255.0f) : 0.0f;
+            const float c = live ? cov_at(cv, d, i, p) : 0.0f;
+            float gcov = 0.0f;  // dL/d(coverage)
             if (c != 0.0f) {
                 const float *pp = params + (size_t)i * kPP;
                 const PathEval e = eval_path(pp, ends[i], c, px, py);
@@ -366,6 +389,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, const uint8
                     grgb[ch] = gc[ch] * A * pp[ch];    // stop colour (interpolated)
                 }
                 gp[3] = gA * c * e.a;                  // opacity
+                gcov = gA * pp[3] * e.a;
                 const float ga = gA * c * pp[3];       // stop alpha (interpolated)
                 const float gcol[4] = {grgb[0], grgb[1], grgb[2], ga};
                 const float *col = pp + 4;
@@ -386,6 +410,12 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, const uint8
                 }
                 g = make_float4(g.x * (1.0f - A), g.y * (1.0f - A), g.z * (1.0f - A), g.w * (1.0f - A));
             }
+            if (cv.gW && live) {
+                // coverage = min(1, |area|): d/d(area) = sign(area) inside (0, 1), else 0
+                const float ar = cv.area[area_index(d, i, p)];
+                const float da = fabsf(ar) < 1.0f ? (ar > 0.0f ? 1.0f : (ar < 0.0f ? -1.0f : 0.0f)) : 0.0f;
+                cv.gW[(size_t)i * np + p] = gcov * da;
+            }
             // this wave's sums of the path's gradient
             const bool any = __ballot(c != 0.0f) != 0;
             const float v = any ? wave_sum32(gp, lane) : 0.0f;
@@ -402,6 +432,230 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, const uint8
             }
         }
         __syncthreads();
+    }
+}
+
+// ---- analytic coverage (geometry mode) ---------------------------------------
+// A path is a closed chain of k cubics: user-space points P0 .. P3k (geom floats
+// after the path's 6 affine values sx, ky, kx, sy, tx, ty; device = (sx x + kx y +
+// tx, ky x + sy y + ty)), each cubic flattened into kFlat segments at uniform t,
+// and the polygon closed by a line from P3k back to P0.  Vertex j of path i:
+// vert[i * nv + j], nv = k * kFlat + 1.
+constexpr int kFlat = 16;
+
+__device__ __forceinline__ void bern(float t, float b[4]) {
+    const float u = 1.0f - t;
+    b[0] = u * u * u;
+    b[1] = 3.0f * u * u * t;
+    b[2] = 3.0f * u * t * t;
+    b[3] = t * t * t;
+}
+
+__global__ void k_diff_flatten(uint32_t n, uint32_t k, const float *__restrict__ geom, float2 *vert) {
+    const uint32_t nv = k * kFlat + 1;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n * nv) return;
+    const uint32_t i = gid / nv, j = gid % nv;
+    const float *g = geom + (size_t)i * (6 + 2 * (3 * k + 1));
+    const uint32_t c = j / kFlat < k ? j / kFlat : k - 1;  // the last vertex is t = 1 of the last cubic
+    const float t = (float)(j - c * kFlat) / (float)kFlat;
+    float b[4];
+    bern(t, b);
+    float x = 0.0f, y = 0.0f;
+    for (int m = 0; m < 4; m++) {
+        x += b[m] * g[6 + 2 * (3 * c + m)];
+        y += b[m] * g[6 + 2 * (3 * c + m) + 1];
+    }
+    vert[gid] = make_float2(g[0] * x + g[2] * y + g[4], g[1] * x + g[3] * y + g[5]);
+}
+
+__global__ void k_diff_zero(float *p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+// Signed area accumulation of one polygon edge into its path's plane (row stride
+// w + 2): per pixel row the edge spans, the row's share of the edge's winding is
+// split between the cells its x range touches so that the prefix sum along the row
+// is the winding-weighted area of each pixel (the exact area coverage of the
+// polygon's winding number).  x is clamped to [0, w]: area left of the canvas lands
+// in cell 0 and reaches every pixel of the row, area right of it is dropped.
+__global__ void k_diff_accum(Dims d, uint32_t k, const float2 *__restrict__ vert, float *area) {
+    const uint32_t nv = k * kFlat + 1;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= d.n * nv) return;
+    const uint32_t i = gid / nv, j = gid % nv;
+    float2 a = vert[gid], b = vert[(size_t)i * nv + (j + 1 == nv ? 0 : j + 1)];
+    if (a.y == b.y) return;
+    float dir = 1.0f;
+    if (a.y > b.y) {
+        const float2 t = a;
+        a = b;
+        b = t;
+        dir = -1.0f;
+    }
+    const float W = (float)d.w;
+    const float dxdy = (b.x - a.x) / (b.y - a.y);
+    float *plane = area + (size_t)i * d.h * (d.w + 2);
+    const int y0 = max(0, (int)floorf(a.y)), y1 = min((int)d.h, (int)ceilf(b.y));
+    for (int y = y0; y < y1; y++) {
+        const float ya = fmaxf(a.y, (float)y), yb = fminf(b.y, (float)(y + 1));
+        const float dy = yb - ya;
+        if (!(dy > 0.0f)) continue;
+        const float xa = fminf(fmaxf(a.x + dxdy * (ya - a.y), 0.0f), W);
+        const float xb = fminf(fmaxf(a.x + dxdy * (yb - a.y), 0.0f), W);
+        const float dd = dy * dir;
+        float *row = plane + (size_t)y * (d.w + 2);
+        const float x0 = fminf(xa, xb), x1 = fmaxf(xa, xb);
+        const float x0f = floorf(x0);
+        const int x0i = (int)x0f;
+        const float x1c = ceilf(x1);
+        if (x1c <= x0f + 1.0f) {  // within one column
+            const float xm = 0.5f * (xa + xb) - x0f;
+            atomicAdd(row + x0i, dd * (1.0f - xm));
+            atomicAdd(row + x0i + 1, dd * xm);
+        } else {
+            const float sl = 1.0f / (x1 - x0);
+            const float fx0 = x0 - x0f;
+            const float a0 = 0.5f * sl * (1.0f - fx0) * (1.0f - fx0);
+            const float fx1 = x1 - x1c + 1.0f;
+            const float am = 0.5f * sl * fx1 * fx1;
+            const int x1i = (int)x1c;
+            atomicAdd(row + x0i, dd * a0);
+            if (x1i == x0i + 2) {
+                atomicAdd(row + x0i + 1, dd * (1.0f - a0 - am));
+            } else {
+                const float a1 = sl * (1.5f - fx0);
+                atomicAdd(row + x0i + 1, dd * (a1 - a0));
+                for (int xi = x0i + 2; xi < x1i - 1; xi++) atomicAdd(row + xi, dd * sl);
+                const float a2 = a1 + (float)(x1i - x0i - 3) * sl;
+                atomicAdd(row + x1i - 1, dd * (1.0f - a2 - am));
+            }
+            atomicAdd(row + x1i, dd * am);
+        }
+    }
+}
+
+// Prefix sums along each row of the area planes (one wave per row), in place.
+__global__ void k_diff_prefix(Dims d, float *area) {
+    const uint32_t row = blockIdx.x;  // over n * h
+    const int lane = threadIdx.x;
+    float *r = area + (size_t)row * (d.w + 2);
+    float carry = 0.0f;
+    for (uint32_t x0 = 0; x0 < d.w; x0 += 64) {
+        const uint32_t x = x0 + lane;
+        float v = x < d.w ? r[x] : 0.0f;
+        for (int o = 1; o < 64; o <<= 1) {
+            const float t = __shfl_up(v, o, 64);
+            if (lane >= o) v += t;
+        }
+        if (x < d.w) r[x] = carry + v;
+        carry += __shfl(v, 63, 64);
+    }
+}
+
+// dL/d(area) through one edge: the boundary term.  Moving the edge point x(s) =
+// a + s (b - a) by δx changes the winding area of the pixel it crosses by
+// δx · (-(b-a).y, (b-a).x) ds, so per pixel crossed on s in [s0, s1]:
+// dA/da = n ∫(1 - s) ds, dA/db = n ∫ s ds.  The pixels are walked in order along
+// the edge; pixels outside the canvas are skipped.  Writes (dL/da, dL/db) per edge.
+__global__ void k_diff_boundary(Dims d, uint32_t k, const float2 *__restrict__ vert, const float *__restrict__ gW,
+                                float4 *egrad) {
+    const uint32_t nv = k * kFlat + 1;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= d.n * nv) return;
+    const uint32_t i = gid / nv, j = gid % nv;
+    const float2 a = vert[gid], b = vert[(size_t)i * nv + (j + 1 == nv ? 0 : j + 1)];
+    const float tx = b.x - a.x, ty = b.y - a.y;
+    float wa = 0.0f, wb = 0.0f;
+    // the part of the edge inside the canvas, s in [s0, s1] (Liang-Barsky): the walk
+    // below then crosses at most w + h + 2 pixels whatever the edge's length, and its
+    // step count is capped at w + h + 3 even for non-finite end points (reads stay
+    // behind the bounds check)
+    float s0 = 0.0f, s1 = 1.0f;
+    bool inside = tx != 0.0f || ty != 0.0f;
+    if (tx != 0.0f) {
+        const float u = -a.x / tx, v = ((float)d.w - a.x) / tx;
+        s0 = fmaxf(s0, fminf(u, v));
+        s1 = fminf(s1, fmaxf(u, v));
+    } else {
+        inside = inside && a.x >= 0.0f && a.x < (float)d.w;
+    }
+    if (ty != 0.0f) {
+        const float u = -a.y / ty, v = ((float)d.h - a.y) / ty;
+        s0 = fmaxf(s0, fminf(u, v));
+        s1 = fminf(s1, fmaxf(u, v));
+    } else {
+        inside = inside && a.y >= 0.0f && a.y < (float)d.h;
+    }
+    if (inside && s0 < s1) {
+        const float *g = gW + (size_t)i * d.w * d.h;
+        const float px = fminf(fmaxf(a.x + s0 * tx, 0.0f), (float)d.w - 1.0f);
+        const float py = fminf(fmaxf(a.y + s0 * ty, 0.0f), (float)d.h - 1.0f);
+        int ix = (int)floorf(px), iy = (int)floorf(py);
+        const int sx = tx > 0.0f ? 1 : -1, sy = ty > 0.0f ? 1 : -1;
+        const float inf = 3.0e38f;
+        float tmx = tx != 0.0f ? ((float)(ix + (sx > 0 ? 1 : 0)) - a.x) / tx : inf;
+        float tmy = ty != 0.0f ? ((float)(iy + (sy > 0 ? 1 : 0)) - a.y) / ty : inf;
+        const float tdx = tx != 0.0f ? fabsf(1.0f / tx) : inf, tdy = ty != 0.0f ? fabsf(1.0f / ty) : inf;
+        float s = s0;
+        const int steps = (int)fminf((s1 - s0) * (fabsf(tx) + fabsf(ty)), (float)(d.w + d.h)) + 3;
+        for (int q = 0; q < steps && s < s1; q++) {
+            const float sn = fminf(fminf(tmx, tmy), s1);
+            if (ix >= 0 && ix < (int)d.w && iy >= 0 && iy < (int)d.h) {
+                const float G = g[(size_t)iy * d.w + ix];
+                const float i1 = sn - s, i2 = 0.5f * (sn * sn - s * s);
+                wa += G * (i1 - i2);
+                wb += G * i2;
+            }
+            s = sn;
+            if (tmx < tmy) {
+                ix += sx;
+                tmx += tdx;
+            } else {
+                iy += sy;
+                tmy += tdy;
+            }
+        }
+    }
+    // n = (-t.y, t.x)
+    egrad[gid] = make_float4(-ty * wa, tx * wa, -ty * wb, tx * wb);
+}
+
+// Per path (one thread): vertex gradients from the edge gradients, through the
+// flattening (Bernstein weights) to the device control points, then through the
+// affine to the user-space points and the affine's six values.
+__global__ void k_diff_geom_chain(uint32_t n, uint32_t k, const float *__restrict__ geom,
+                                  const float4 *__restrict__ egrad, float *ggeom) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t nv = k * kFlat + 1, np = 3 * k + 1;
+    const float *g = geom + (size_t)i * (6 + 2 * np);
+    float *o = ggeom + (size_t)i * (6 + 2 * np);
+    for (uint32_t q = 0; q < 6 + 2 * np; q++) o[q] = 0.0f;
+    const float4 *e = egrad + (size_t)i * nv;
+    for (uint32_t j = 0; j < nv; j++) {
+        // vertex j starts edge j and ends edge j - 1 (edge nv - 1 closes to vertex 0)
+        const float4 ea = e[j], eb = e[j == 0 ? nv - 1 : j - 1];
+        const float gx = ea.x + eb.z, gy = ea.y + eb.w;
+        const uint32_t c = j / kFlat < k ? j / kFlat : k - 1;
+        const float t = (float)(j - c * kFlat) / (float)kFlat;
+        float b[4];
+        bern(t, b);
+        for (int m = 0; m < 4; m++) {
+            const uint32_t pi = 3 * c + m;
+            const float ux = g[6 + 2 * pi], uy = g[6 + 2 * pi + 1];
+            const float dx = b[m] * gx, dy = b[m] * gy;  // dL/d(device point)
+            // device = (sx x + kx y + tx, ky x + sy y + ty)
+            o[6 + 2 * pi] += g[0] * dx + g[1] * dy;
+            o[6 + 2 * pi + 1] += g[2] * dx + g[3] * dy;
+            o[0] += dx * ux;
+            o[1] += dy * ux;
+            o[2] += dx * uy;
+            o[3] += dy * uy;
+            o[4] += dx;
+            o[5] += dy;
+        }
     }
 }
 
@@ -426,9 +680,9 @@ __global__ __launch_bounds__(1024) void k_diff_reduce(const float *part, uint32_
 // Workspace layout (bytes, 256-aligned): canvas S | T | B | gB | gS | gS2 | weights |
 // checkpoints | partial rows (paths) | partial rows (matrix) | σ partials (2 rows).
 struct Layout {
-    size_t S, T, B, gB, gS, gS2, wts, ck, part_p, part_m, part_s, total;
+    size_t S, T, B, gB, gS, gS2, wts, ck, part_p, part_m, part_s, area, gW, vert, egrad, total;
 };
-Layout layout(uint32_t w, uint32_t h, uint32_t n) {
+Layout layout(uint32_t w, uint32_t h, uint32_t n, uint32_t k) {
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t img = al((size_t)w * h * sizeof(float4));
     const size_t nblk = ((size_t)w * h + kThreads - 1) / kThreads;
@@ -444,12 +698,20 @@ Layout layout(uint32_t w, uint32_t h, uint32_t n) {
     L.part_p = L.ck + al((size_t)((n + kSeg - 1) / kSeg) * w * h * sizeof(float4));
     L.part_m = L.part_p + al(nblk * n * kPP * sizeof(float));
     L.part_s = L.part_m + al(nblk * 20 * sizeof(float));
-    L.total = L.part_s + al(2 * nblk * sizeof(float));
+    L.area = L.part_s + al(2 * nblk * sizeof(float));
+    const size_t nv = (size_t)k * kFlat + 1;
+    L.gW = L.area + (k ? al((size_t)n * h * (w + 2) * sizeof(float)) : 0);
+    L.vert = L.gW + (k ? al((size_t)n * w * h * sizeof(float)) : 0);
+    L.egrad = L.vert + (k ? al((size_t)n * nv * sizeof(float2)) : 0);
+    L.total = L.egrad + (k ? al((size_t)n * nv * sizeof(float4)) : 0);
     return L;
 }
 
-bool scene_ok(const rvk_diff_scene *s) {
-    return s && s->width > 0 && s->height > 0 && s->path_count > 0 && s->dev_coverage && s->dev_gradient_ends &&
+inline uint8_t *ws_base(void *p) { return (uint8_t *)p; }
+
+bool scene_ok(const rvk_diff_scene *s, const float *geom) {
+    return s && s->width > 0 && s->height > 0 && s->path_count > 0 && s->dev_gradient_ends &&
+           (s->cubics ? geom != nullptr && s->cubics <= 4096 : s->dev_coverage != nullptr) &&
            (uint64_t)s->width * s->height <= (1ull << 30);
 }
 
@@ -457,24 +719,42 @@ bool scene_ok(const rvk_diff_scene *s) {
 
 extern "C" {
 
-rvk_status rvk_diff_workspace_size(uint32_t width, uint32_t height, uint32_t path_count, size_t *bytes) {
-    if (!bytes || width == 0 || height == 0 || path_count == 0) return RVK_ERR_INVALID_ARGUMENT;
-    *bytes = layout(width, height, path_count).total;
+rvk_status rvk_diff_workspace_size(uint32_t width, uint32_t height, uint32_t path_count, uint32_t cubics,
+                                   size_t *bytes) {
+    if (!bytes || width == 0 || height == 0 || path_count == 0 || cubics > 4096) return RVK_ERR_INVALID_ARGUMENT;
+    *bytes = layout(width, height, path_count, cubics).total;
     return RVK_OK;
 }
 
 rvk_status rvk_diff_forward(rvk_context *ctx, const rvk_diff_scene *scene, const float *dev_params,
-                            void *dev_workspace, float *dev_out) {
-    if (!ctx || !scene_ok(scene) || !dev_params || !dev_workspace || !dev_out) return RVK_ERR_INVALID_ARGUMENT;
-    const uint32_t w = scene->width, h = scene->height, n = scene->path_count;
-    const Layout L = layout(w, h, n);
+                            const float *dev_geom, void *dev_workspace, float *dev_out) {
+    if (!ctx || !scene_ok(scene, dev_geom) || !dev_params || !dev_workspace || !dev_out)
+        return RVK_ERR_INVALID_ARGUMENT;
+    const uint32_t w = scene->width, h = scene->height, n = scene->path_count, k = scene->cubics;
+    const Layout L = layout(w, h, n, k);
     uint8_t *ws = (uint8_t *)dev_workspace;
     const Dims d{w, h, n, (uint32_t)(((size_t)w * h + kThreads - 1) / kThreads)};
     float4 *S = (float4 *)(ws + L.S), *T = (float4 *)(ws + L.T), *B = (float4 *)(ws + L.B);
     float *wts = (float *)(ws + L.wts);
     const double px_bytes = (double)w * h;
+    Cov cv{scene->dev_coverage, nullptr, nullptr};
+    if (k) {
+        // analytic coverage: flatten, accumulate winding area, prefix along rows
+        float *area = (float *)(ws + L.area);
+        float2 *vert = (float2 *)(ws + L.vert);
+        const size_t nv = (size_t)k * kFlat + 1, cells = (size_t)n * h * (w + 2);
+        const unsigned ge = (unsigned)((n * nv + 255) / 256);
+        rvk::timed_launch(ctx, "diff_coverage", (double)cells * 8.0, [&] {
+            hipLaunchKernelGGL(k_diff_zero, dim3((unsigned)((cells + 255) / 256 < 65535 ? (cells + 255) / 256 : 65535)),
+                               dim3(256), 0, ctx->stream, area, cells);
+            hipLaunchKernelGGL(k_diff_flatten, dim3(ge), dim3(256), 0, ctx->stream, n, k, dev_geom, vert);
+            hipLaunchKernelGGL(k_diff_accum, dim3(ge), dim3(256), 0, ctx->stream, d, k, (const float2 *)vert, area);
+            hipLaunchKernelGGL(k_diff_prefix, dim3(n * h), dim3(64), 0, ctx->stream, d, area);
+        });
+        cv = Cov{nullptr, area, nullptr};
+    }
     rvk::timed_launch(ctx, "diff_paths_fwd", px_bytes * (16.0 + n), [&] {
-        hipLaunchKernelGGL(k_diff_paths_fwd, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, scene->dev_coverage,
+        hipLaunchKernelGGL(k_diff_paths_fwd, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, cv,
                            (const float4 *)scene->dev_gradient_ends, dev_params, S, (float4 *)(ws + L.ck));
     });
     rvk::timed_launch(ctx, "diff_blur_weights", 0.0, [&] {
@@ -498,11 +778,15 @@ rvk_status rvk_diff_forward(rvk_context *ctx, const rvk_diff_scene *scene, const
 }
 
 rvk_status rvk_diff_backward(rvk_context *ctx, const rvk_diff_scene *scene, const float *dev_params,
-                             void *dev_workspace, const float *dev_grad_out, float *dev_grad_params) {
-    if (!ctx || !scene_ok(scene) || !dev_params || !dev_workspace || !dev_grad_out || !dev_grad_params)
+                             const float *dev_geom, void *dev_workspace, const float *dev_grad_out,
+                             float *dev_grad_params, float *dev_grad_geom) {
+    if (!ctx || !scene_ok(scene, dev_geom) || !dev_params || !dev_workspace || !dev_grad_out || !dev_grad_params ||
+        (scene->cubics && !dev_grad_geom))
         return RVK_ERR_INVALID_ARGUMENT;
-    const uint32_t w = scene->width, h = scene->height, n = scene->path_count;
-    const Layout L = layout(w, h, n);
+    const uint32_t w = scene->width, h = scene->height, n = scene->path_count, k = scene->cubics;
+    const Layout L = layout(w, h, n, k);
+    Cov cv{scene->dev_coverage, nullptr, nullptr};
+    if (k) cv = Cov{nullptr, (const float *)(ws_base(dev_workspace) + L.area), (float *)(ws_base(dev_workspace) + L.gW)};
     uint8_t *ws = (uint8_t *)dev_workspace;
     const Dims d{w, h, n, (uint32_t)(((size_t)w * h + kThreads - 1) / kThreads)};
     const float4 *S = (const float4 *)(ws + L.S), *T = (const float4 *)(ws + L.T), *B = (const float4 *)(ws + L.B);
@@ -533,7 +817,7 @@ rvk_status rvk_diff_backward(rvk_context *ctx, const rvk_diff_scene *scene, cons
                            gB, wts, (const float4 *)nullptr, (float *)nullptr);
     });
     rvk::timed_launch(ctx, "diff_paths_bwd", px_bytes * (48.0 + 2.0 * n), [&] {
-        hipLaunchKernelGGL(k_diff_paths_bwd, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, scene->dev_coverage,
+        hipLaunchKernelGGL(k_diff_paths_bwd, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, cv,
                            (const float4 *)scene->dev_gradient_ends, dev_params,
                            (const float4 *)(ws + L.ck), (const float4 *)gS, (const float4 *)gB, part_p);
     });
@@ -547,6 +831,17 @@ rvk_status rvk_diff_backward(rvk_context *ctx, const rvk_diff_scene *scene, cons
         hipLaunchKernelGGL(k_diff_reduce, dim3(1), dim3(1024), 0, ctx->stream, (const float *)part_m, d.nblk, 20u,
                            dev_grad_params + np_params + 1);
     });
+    if (k) {
+        const size_t nv = (size_t)k * kFlat + 1;
+        const unsigned ge = (unsigned)((n * nv + 255) / 256);
+        float4 *egrad = (float4 *)(ws + L.egrad);
+        rvk::timed_launch(ctx, "diff_boundary", (double)n * w * h * 4.0, [&] {
+            hipLaunchKernelGGL(k_diff_boundary, dim3(ge), dim3(256), 0, ctx->stream, d, k,
+                               (const float2 *)(ws + L.vert), (const float *)(ws + L.gW), egrad);
+            hipLaunchKernelGGL(k_diff_geom_chain, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, n, k, dev_geom,
+                               (const float4 *)egrad, dev_grad_geom);
+        });
+    }
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;
 }

@@ -84,9 +84,9 @@ def test_autograd_function_and_fit_step(ctx):
     w, h, paths, ends, p0, pt = diff_scene(seed=5, size=48, n_paths=8, segments=4, sigma=2.0)
     r = diff.Renderer(ctx, w, h, paths, ends)
     render = diff.render_fn()
-    target = render(torch.as_tensor(pt, device="cuda"), r).detach()
+    target = render(torch.as_tensor(pt, device="cuda"), None, r).detach()
     params = torch.as_tensor(p0, device="cuda").clone().requires_grad_(True)
-    loss = ((render(params, r) - target) ** 2).mean()
+    loss = ((render(params, None, r) - target) ** 2).mean()
     loss.backward()
     out = r.forward(params.detach())
     direct = r.backward(params.detach(), 2.0 * (out - target) / out.numel())
@@ -96,7 +96,93 @@ def test_autograd_function_and_fit_step(ctx):
     first = loss.item()
     for _ in range(20):
         opt.zero_grad()
-        loss = ((render(params, r) - target) ** 2).mean()
+        loss = ((render(params, None, r) - target) ** 2).mean()
         loss.backward()
         opt.step()
     assert loss.item() < first
+
+
+# ---- analytic mode: coverage from the geometry, gradients to control points and
+# transforms -------------------------------------------------------------------------
+def _setup_geom(ctx, seed, size=40, n=5, k=3, sigma=1.3):
+    import torch
+    w, h, paths, ends, p0, _ = diff_scene(seed=seed, size=size, n_paths=n, segments=k, sigma=sigma)
+    r = diff.Renderer(ctx, w, h, paths, ends, cubics=k)
+    params = torch.as_tensor(p0, device="cuda")
+    geom = r.geom0.clone()
+    # a non-trivial affine per path: a small rotation and scale about the canvas centre
+    rng = np.random.default_rng(seed)
+    for i in range(n):
+        a = rng.uniform(-0.1, 0.1)
+        sc = rng.uniform(0.9, 1.0)
+        c, s = sc * np.cos(a), sc * np.sin(a)
+        cx = cy = size / 2
+        geom[i, :6] = torch.tensor([c, s, -s, c, cx - c * cx + s * cy, cy - s * cx - c * cy], device="cuda")
+    return r, params, geom.contiguous(), np.asarray(ends, np.float32), w, h, k
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_analytic_forward_matches_fp64_oracle(ctx, seed):
+    r, params, geom, ends, w, h, k = _setup_geom(ctx, seed)
+    out = r.forward(params, geom).cpu().numpy()
+    ref = D.forward_geom(geom.cpu().numpy(), k, ends, params.cpu().numpy(), w, h)
+    assert np.abs(out - ref).max() < 1e-4
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_analytic_gradient_matches_fp64_finite_differences(ctx, seed):
+    import torch
+    r, params, geom, ends, w, h, k = _setup_geom(ctx, seed)
+    rng = np.random.default_rng(10 + seed)
+    go = rng.normal(0.0, 1.0, (h, w, 4)).astype(np.float32)
+    r.forward(params, geom)
+    gp, gg = r.backward(params, torch.as_tensor(go, device="cuda"), geom)
+    gg = gg.cpu().numpy().astype(np.float64).reshape(-1)
+    gnp = geom.cpu().numpy()
+    idx = sorted(rng.choice(gg.size, size=50, replace=False).tolist() + list(range(6)))  # every path's affine too
+    ref = D.grad_fd_geom(gnp, k, ends, params.cpu().numpy(), w, h, go, idx)
+    e = np.linalg.norm(gg[idx] - ref[idx]) / np.linalg.norm(ref[idx])
+    assert e < 1e-3, f"geometry: relative error {e:.2e}"
+    # the paint gradients with the analytic coverage
+    cov = D.analytic_coverage(gnp, k, w, h) * 255.0
+    refp = D.grad_fd(cov, ends, params.cpu().numpy(), w, h, go)
+    gp = gp.cpu().numpy().astype(np.float64)
+    e = np.linalg.norm(gp - refp) / np.linalg.norm(refp)
+    assert e < 1e-3, f"paint: relative error {e:.2e}"
+
+
+def test_analytic_gradient_matches_f32_central_differences(ctx):
+    import torch
+    r, params, geom, ends, w, h, k = _setup_geom(ctx, 4, size=48, n=6, k=4, sigma=2.2)
+    rng = np.random.default_rng(4)
+    go = torch.as_tensor(rng.normal(0.0, 1.0, (h, w, 4)).astype(np.float32), device="cuda")
+    r.forward(params, geom)
+    _, gg = r.backward(params, go, geom)
+    for t in range(4):
+        v = torch.as_tensor(rng.normal(0.0, 1.0, geom.shape).astype(np.float32), device="cuda")
+        v[:, :4] *= 0.01  # the linear part of the affine moves points by up to the canvas size
+        hstep = 1e-3
+        fp = (r.forward(params, geom + hstep * v) * go).double().sum().item()
+        fm = (r.forward(params, geom - hstep * v) * go).double().sum().item()
+        fd = (fp - fm) / (2 * hstep)
+        an = float((gg * v).double().sum())
+        assert abs(fd - an) <= 2e-2 * max(abs(an), 1.0), f"direction {t}: fd {fd} analytic {an}"
+
+
+def test_analytic_fit_moves_the_geometry(ctx):
+    import torch
+    r, params, geom, ends, w, h, k = _setup_geom(ctx, 6, size=48, n=6, k=4, sigma=1.5)
+    render = diff.render_fn()
+    target_geom = geom.clone()
+    target_geom[:, 4:6] += 1.5  # the target is the same scene shifted by 1.5 px
+    target = render(params, target_geom, r).detach()
+    g = geom.clone().requires_grad_(True)
+    opt = torch.optim.Adam([g], lr=0.05)
+    first = None
+    for _ in range(30):
+        opt.zero_grad()
+        loss = ((render(params, g, r) - target) ** 2).mean()
+        loss.backward()
+        first = loss.item() if first is None else first
+        opt.step()
+    assert loss.item() < 0.5 * first

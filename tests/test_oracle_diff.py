@@ -94,6 +94,19 @@ def test_matrix_identity_and_clamp():
     np.testing.assert_allclose(f[..., 0], b[..., 3])
 
 
+def test_matrix_demultiply_floor():
+    # alpha at or above 1/255 (every non-zero u8 alpha): exact demultiply, so a matrix
+    # that swaps red and blue returns the swapped premultiplied colour
+    swap = np.eye(4, 5)[[2, 1, 0, 3]]
+    b = np.array([[[0.3, 0.2, 0.1, 0.5], [0.6 / 255, 0.2 / 255, 0.4 / 255, 1.0 / 255]]])
+    np.testing.assert_allclose(D._color_matrix(b, swap), b[..., [2, 1, 0, 3]], atol=1e-15)
+    # below: colour / (1/255), continuous at the floor and zero with the colour
+    a = 1e-6
+    b = np.array([[[0.5 * a, 0.25 * a, 0.0, a]]])
+    f = D._color_matrix(b, np.eye(4, 5))
+    np.testing.assert_allclose(f[0, 0], [0.5 * a * 255 * a, 0.25 * a * 255 * a, 0.0, a], rtol=1e-12)
+
+
 def test_blur_of_an_impulse_is_the_kernel_outer_product():
     h, w, s = 31, 29, 2.2
     img = np.zeros((h, w, 4))
@@ -104,3 +117,40 @@ def test_blur_of_an_impulse_is_the_kernel_outer_product():
     exp[15 - r:15 + r + 1, 14 - r:14 + r + 1] = np.outer(k, k)
     np.testing.assert_allclose(out[..., 0], exp, atol=1e-15)
     np.testing.assert_allclose(out[..., 1], 0.5 * exp, atol=1e-15)
+
+
+def _line_cubics(poly):
+    """A closed polygon as a chain of straight cubics (control points at thirds)."""
+    pts = [poly[0]]
+    for i in range(len(poly)):
+        a, b = np.array(poly[i], float), np.array(poly[(i + 1) % len(poly)], float)
+        pts += [tuple(a + (b - a) / 3), tuple(a + 2 * (b - a) / 3), tuple(b)]
+    return np.concatenate([[1, 0, 0, 1, 0, 0], np.array(pts).reshape(-1)]), len(poly)
+
+
+def test_analytic_coverage_of_a_rectangle():
+    g, k = _line_cubics([(2.25, 1.5), (7.75, 1.5), (7.75, 5.5), (2.25, 5.5)])
+    cov = D.analytic_coverage(g, k, 10, 8)[0]
+    col = np.array([0, 0, 0.75, 1, 1, 1, 1, 0.75, 0, 0])
+    row = np.array([0, 0.5, 1, 1, 1, 0.5, 0, 0])
+    np.testing.assert_allclose(cov, np.outer(row, col), atol=1e-12)
+
+
+def test_analytic_coverage_sums_to_the_polygon_area_in_either_orientation():
+    poly = [(3.3, 2.1), (17.6, 5.4), (8.2, 14.9), (2.7, 9.3)]
+    area = 0.5 * abs(sum(poly[i][0] * poly[(i + 1) % 4][1] - poly[(i + 1) % 4][0] * poly[i][1] for i in range(4)))
+    for p in (poly, poly[::-1]):
+        g, k = _line_cubics(p)
+        cov = D.analytic_coverage(g, k, 21, 18)[0]
+        assert abs(cov.sum() - area) < 1e-9
+        assert cov.max() <= 1.0 and cov.min() >= 0.0
+
+
+def test_analytic_coverage_follows_the_affine():
+    # a translation by whole pixels shifts the coverage by the same amount
+    g, k = _line_cubics([(2.4, 2.2), (6.1, 3.0), (4.0, 6.7)])
+    c0 = D.analytic_coverage(g, k, 16, 16)[0]
+    g2 = g.copy()
+    g2[4:6] = (5.0, 3.0)
+    c1 = D.analytic_coverage(g2, k, 16, 16)[0]
+    np.testing.assert_allclose(c1[3:, 5:], c0[:-3, :-5], atol=1e-12)
