@@ -65,10 +65,13 @@ __device__ __forceinline__ float cov_at(const Cov &cv, const Dims &d, uint32_t i
 struct PathEval {
     float rgb[3], a;   // unpremultiplied stop colour at the pixel
     float A;           // c * opacity * a
-    int k;             // stop interval: colour = lerp(stop k, stop k + 1, u)
-    float u, t;
+    float ws[4];       // colour = sum over stops s of ws[s] * stop s
+    int k;             // stop interval: between stop k and stop k + 1, at u
+    float u, t, lo, hi;  // lo, hi: offsets of stops k, k + 1
 };
 
+// The interpolation is written as per-stop weights so that every access to the
+// path's parameters has a compile-time index (the values are uniform over the wave).
 __device__ __forceinline__ PathEval eval_path(const float *pp, const float4 ends, float c, float px, float py) {
     PathEval e;
     const float dx = ends.z - ends.x, dy = ends.w - ends.y;
@@ -77,23 +80,30 @@ __device__ __forceinline__ PathEval eval_path(const float *pp, const float4 ends
     t = fminf(fmaxf(t, 0.0f), 1.0f);
     const float *col = pp + 4;   // 4 stops x rgba
     const float *off = pp + 20;  // 4 offsets
-    int k;
-    float u;
-    if (t <= off[0]) {
-        k = 0; u = 0.0f;
-    } else if (t >= off[3]) {
-        k = 2; u = 1.0f;
-    } else {
-        k = t < off[1] ? 0 : (t < off[2] ? 1 : 2);
-        const float span = off[k + 1] - off[k];
-        u = span > 0.0f ? (t - off[k]) / span : 0.0f;
+    // before the first stop: stop 0 (k = 0, u = 0); after the last: stop 3 (k = 2, u = 1)
+    const bool before = t <= off[0], after = !before && t >= off[3];
+    const int k = before ? 0 : (after ? 2 : (t < off[1] ? 0 : (t < off[2] ? 1 : 2)));
+    const float lo = k == 0 ? off[0] : (k == 1 ? off[1] : off[2]);
+    const float hi = k == 0 ? off[1] : (k == 1 ? off[2] : off[3]);
+    const float span = hi - lo;
+    const float u = before ? 0.0f : (after ? 1.0f : (span > 0.0f ? (t - lo) / span : 0.0f));
+    const float v = 1.0f - u;
+    e.ws[0] = k == 0 ? v : 0.0f;
+    e.ws[1] = k == 1 ? v : (k == 0 ? u : 0.0f);
+    e.ws[2] = k == 2 ? v : (k == 1 ? u : 0.0f);
+    e.ws[3] = k == 2 ? u : 0.0f;
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+        const float q = e.ws[0] * col[ch] + e.ws[1] * col[4 + ch] + e.ws[2] * col[8 + ch] + e.ws[3] * col[12 + ch];
+        if (ch < 3) e.rgb[ch] = q;
+        else e.a = q;
     }
-    for (int ch = 0; ch < 3; ch++) e.rgb[ch] = col[4 * k + ch] + u * (col[4 * (k + 1) + ch] - col[4 * k + ch]);
-    e.a = col[4 * k + 3] + u * (col[4 * (k + 1) + 3] - col[4 * k + 3]);
     e.A = c * pp[3] * e.a;
     e.k = k;
     e.u = u;
     e.t = t;
+    e.lo = lo;
+    e.hi = hi;
     return e;
 }
 
@@ -394,19 +404,28 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, Cov cv,
                 const float gcol[4] = {grgb[0], grgb[1], grgb[2], ga};
                 const float *col = pp + 4;
                 const float *off = pp + 20;
-                float gu = 0.0f;
-                for (int ch = 0; ch < 4; ch++) {
-                    gp[4 + 4 * e.k + ch] += gcol[ch] * (1.0f - e.u);
-                    gp[4 + 4 * (e.k + 1) + ch] += gcol[ch] * e.u;
-                    gu += gcol[ch] * (col[4 * (e.k + 1) + ch] - col[4 * e.k + ch]);
-                }
-                // u = (t - o_k) / (o_k+1 - o_k) inside the stops (constant outside)
-                if (e.t > off[0] && e.t < off[3]) {
-                    const float span = off[e.k + 1] - off[e.k];
-                    if (span > 0.0f) {
-                        gp[20 + e.k] += gu * (e.t - off[e.k + 1]) / (span * span);
-                        gp[20 + e.k + 1] += gu * -(e.t - off[e.k]) / (span * span);
+                // stop colours: d colour / d stop s = ws[s]; du: the colour's slope
+                // along u is stop k+1 - stop k, dotted with gcol
+                float gd[4];
+#pragma unroll
+                for (int st = 0; st < 4; st++) {
+                    gd[st] = 0.0f;
+#pragma unroll
+                    for (int ch = 0; ch < 4; ch++) {
+                        gp[4 + 4 * st + ch] = gcol[ch] * e.ws[st];
+                        gd[st] += gcol[ch] * col[4 * st + ch];
                     }
+                }
+                const float gu = e.k == 0 ? gd[1] - gd[0] : (e.k == 1 ? gd[2] - gd[1] : gd[3] - gd[2]);
+                // u = (t - lo) / (hi - lo) inside the stops (constant outside)
+                const float span = e.hi - e.lo;
+                if (e.t > off[0] && e.t < off[3] && span > 0.0f) {
+                    const float ga_lo = gu * (e.t - e.hi) / (span * span);
+                    const float ga_hi = gu * -(e.t - e.lo) / (span * span);
+                    gp[20] = e.k == 0 ? ga_lo : 0.0f;
+                    gp[21] = e.k == 1 ? ga_lo : (e.k == 0 ? ga_hi : 0.0f);
+                    gp[22] = e.k == 2 ? ga_lo : (e.k == 1 ? ga_hi : 0.0f);
+                    gp[23] = e.k == 2 ? ga_hi : 0.0f;
                 }
                 g = make_float4(g.x * (1.0f - A), g.y * (1.0f - A), g.z * (1.0f - A), g.w * (1.0f - A));
             }
