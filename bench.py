@@ -14,8 +14,10 @@ Prints one JSON line: Mpixel/s of the document, the roofline of the dominant ker
 (algorithmic bytes / its HIP-event time over the timed steps, against 8 TB/s HBM),
 the oracle's CPU throughput on a bounded sample of the same document, and under
 "batched" the documents/s of rvk_render_batch against a loop of rvk_render calls on a
-sample of the batched workload, and under "differentiable" the f32 forward and
-forward+backward times of the differentiable form against the plain u8 forward.
+sample of the batched workload, under "differentiable" the f32 forward and
+forward+backward times of the differentiable form against the plain u8 forward, and
+under "batched_fitting" the steps/s and final PSNR of 64 independent 512x512 fits
+run as one batch.
 """
 from __future__ import annotations
 
@@ -237,6 +239,23 @@ def differentiable(ctx, size, n_paths, seed, reps):
             "kernels_fwd_bwd": kern[:8], "analytic_geometry": analytic}
 
 
+def fitting(ctx, count, size, n_paths, steps, seed):
+    """Batched fitting: `count` independent size² f32 targets (each its own seeded
+    `n_paths`-path scene of 8-cubic closed paths with blur and colour matrix), fitted
+    from re-seeded random parameters (paint, σ, matrix, control points, affines) by
+    Adam (lr 1e-2) for `steps` steps; each step's forward and backward for all scenes
+    is one launch per kernel.  Reports steps/s and the PSNR against the targets of the
+    f32 render and of the plain u8 render of the fitted parameters."""
+    from resvg_amd import fit
+    t0 = time.perf_counter()
+    wl = fit.fit_workload(count=count, size=size, n_paths=n_paths, segments=8, seed=seed)
+    print(f"[bench] fitting: {count} scenes built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    res = fit.fit(ctx, wl, steps=steps, lr=1e-2, log_every=max(1, steps // 5))
+    print(f"[bench] fitting: {res['steps_per_s']} steps/s, PSNR u8 {res['psnr_u8_db']} dB", file=sys.stderr,
+          flush=True)
+    return res
+
+
 def _primitive_rooflines(kernels, steps):
     """Per filter primitive / pipeline stage: its kernels' time per step and the
     achieved fraction of HBM peak on the primitive's algorithmic bytes."""
@@ -272,6 +291,10 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--diff-size", type=int, default=1024, help="canvas side of the differentiable leg (0: skip)")
     ap.add_argument("--diff-paths", type=int, default=256)
+    ap.add_argument("--fit-scenes", type=int, default=64, help="scenes of the batched fitting leg (0: skip)")
+    ap.add_argument("--fit-size", type=int, default=512)
+    ap.add_argument("--fit-paths", type=int, default=128)
+    ap.add_argument("--fit-steps", type=int, default=500)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 1 GiB device buffer once (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -330,6 +353,8 @@ def main() -> int:
     st = ctx.stats()  # of the last render call
     bat = batched(ctx, args.batch_docs, args.seed, args.batch_steps) if args.batch_docs > 0 else None
     dif = differentiable(ctx, args.diff_size, args.diff_paths, args.seed, 5) if args.diff_size > 0 else None
+    fitr = fitting(ctx, args.fit_scenes, args.fit_size, args.fit_paths, args.fit_steps, args.seed) \
+        if args.fit_scenes > 0 else None
     host = {"build_ms": round(st.host_build_ms, 1), "wait_ms": round(st.host_wait_ms, 1), "flushes": st.flushes,
             "edge_piece_slots": st.edge_pieces, "upload_MB": round(st.upload_bytes / 1e6, 1),
             "kernel_launches": st.kernel_launches, "layers": st.layers}
@@ -383,6 +408,7 @@ def main() -> int:
         "host_last_step": host,
         "batched": bat,
         "differentiable": dif,
+        "batched_fitting": fitr,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -412,6 +412,20 @@ rvk_status rvk_diff_forward(rvk_context *ctx, const rvk_diff_scene *scene, const
 rvk_status rvk_diff_backward(rvk_context *ctx, const rvk_diff_scene *scene, const float *dev_params,
                              const float *dev_geom, void *dev_workspace, const float *dev_grad_out,
                              float *dev_grad_params, float *dev_grad_geom);
+/* The same for `batch` (1 .. 65535) scenes of one shape (`scene` gives the shape and
+ * the base pointers), every kernel launched once for the whole batch.  Each
+ * per-scene array is stacked, scene b at b times its per-scene size: parameters
+ * (path_count * 24 + 21 floats), geometry (path_count * RVK_DIFF_GEOM_FLOATS(cubics)
+ * floats), gradient ends (path_count x 4 floats), coverage planes (path_count x
+ * height x width bytes), workspace (rvk_diff_workspace_size bytes, so batch times
+ * that in all), images and their gradients (width x height x 4 floats).  The scenes'
+ * results are those of one call per scene. */
+rvk_status rvk_diff_forward_batch(rvk_context *ctx, const rvk_diff_scene *scene, uint32_t batch,
+                                  const float *dev_params, const float *dev_geom, void *dev_workspace,
+                                  float *dev_out);
+rvk_status rvk_diff_backward_batch(rvk_context *ctx, const rvk_diff_scene *scene, uint32_t batch,
+                                   const float *dev_params, const float *dev_geom, void *dev_workspace,
+                                   const float *dev_grad_out, float *dev_grad_params, float *dev_grad_geom);
 
 /* Per-call statistics of the last rvk_render / rvk_filter_apply on this context. */
 typedef struct rvk_stats {

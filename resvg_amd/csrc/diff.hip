@@ -42,7 +42,23 @@ constexpr int kGlobalParams = 1 + 20;  // σ, matrix
 struct Dims {
     uint32_t w, h, n;        // width, height, paths
     uint32_t nblk;           // blocks of the per-pixel kernels
+    uint32_t k;              // cubics per path (analytic mode), else 0
+    size_t ws;               // workspace bytes per scene
 };
+
+// A batch holds scenes of one shape, stacked: scene blockIdx.y finds its parameters,
+// geometry, gradient ends, coverage planes, workspace and images at blockIdx.y times
+// the per-scene size of each (a null pointer stays null).
+template <class T> __device__ __forceinline__ T *scene_of(T *p, size_t bytes) {
+    return p ? reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.y * bytes) : p;
+}
+__device__ __forceinline__ size_t params_bytes(const Dims &d) {
+    return ((size_t)d.n * kPP + kGlobalParams) * sizeof(float);
+}
+__device__ __forceinline__ size_t geom_bytes(const Dims &d) {
+    return (size_t)d.n * (6 + 2 * (3 * d.k + 1)) * sizeof(float);
+}
+__device__ __forceinline__ size_t image_bytes(const Dims &d) { return (size_t)d.w * d.h * sizeof(float4); }
 
 // Where the coverage of path i at pixel p comes from: u8 planes (coverage / 255), or
 // the winding-area planes of the analytic mode (coverage min(1, |area|); row stride
@@ -55,6 +71,9 @@ struct Cov {
 };
 __device__ __forceinline__ size_t area_index(const Dims &d, uint32_t i, size_t p) {
     return ((size_t)i * d.h + p / d.w) * (d.w + 2) + p % d.w;
+}
+__device__ __forceinline__ Cov scene_cov(const Cov &cv, const Dims &d) {
+    return Cov{scene_of(cv.u8, (size_t)d.n * d.w * d.h), scene_of(cv.area, d.ws), scene_of(cv.gW, d.ws)};
 }
 __device__ __forceinline__ float cov_at(const Cov &cv, const Dims &d, uint32_t i, size_t p) {
     if (cv.u8) return (float)cv.u8[(size_t)i * d.w * d.h + p] * (1.0f / 255.0f);
@@ -147,6 +166,11 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_fwd(Dims d, Cov cv,
                                                               const float4 *__restrict__ ends,
                                                               const float *__restrict__ params, float4 *canvas,
                                                               float4 *ck) {
+    cv = scene_cov(cv, d);
+    ends = scene_of(ends, (size_t)d.n * sizeof(float4));
+    params = scene_of(params, params_bytes(d));
+    canvas = scene_of(canvas, d.ws);
+    ck = scene_of(ck, d.ws);
     const size_t np = (size_t)d.w * d.h;
     const size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x;
     if (p >= np) return;
@@ -166,10 +190,12 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_fwd(Dims d, Cov cv,
 // wts[0] = R (as a float), wts[1 .. 2R+1] = w_k, wts[2R+2 .. 4R+2] = dw_k/dσ for
 // k = -R .. R, with w_k = g_k / Σg, g_k = exp(-k² / (2σ²)), σ clamped to [0.05, 16];
 // dw_k/dσ = w_k (k² - Σ_j w_j j²) / σ³.
-__global__ void k_diff_blur_weights(const float *params, uint32_t n, float *wts) {
+__global__ void k_diff_blur_weights(Dims d, const float *params, float *wts) {
     __shared__ float g[2 * kMaxRadius + 1];
     __shared__ float sums[2];
-    const float raw = params[(size_t)n * kPP];
+    params = scene_of(params, params_bytes(d));
+    wts = scene_of(wts, d.ws);
+    const float raw = params[(size_t)d.n * kPP];
     const float sigma = fminf(fmaxf(raw, 0.05f), 16.0f);
     const float live = raw > 0.05f && raw < 16.0f ? 1.0f : 0.0f;  // no σ gradient where it is clamped
     const int R = min((int)ceilf(3.0f * sigma), kMaxRadius);
@@ -206,6 +232,11 @@ __global__ __launch_bounds__(kThreads) void k_diff_blur(Dims d, int dir, const f
                                                          float4 *__restrict__ out, const float *__restrict__ wts,
                                                          const float4 *__restrict__ gdot, float *part) {
     __shared__ float red[kWaves];
+    in = scene_of(in, d.ws);
+    out = scene_of(out, d.ws);
+    wts = scene_of(wts, d.ws);
+    gdot = scene_of(gdot, d.ws);
+    part = scene_of(part, d.ws);
     const size_t np = (size_t)d.w * d.h;
     const size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x;
     const int R = (int)wts[0];
@@ -273,6 +304,10 @@ __device__ __forceinline__ CmEval cm_eval(const float *m, float4 b) {
 __global__ __launch_bounds__(kThreads) void k_diff_cm_fwd(Dims d, const float4 *__restrict__ B,
                                                           const float4 *__restrict__ S,
                                                           const float *__restrict__ params, float4 *out) {
+    B = scene_of(B, d.ws);
+    S = scene_of(S, d.ws);
+    params = scene_of(params, params_bytes(d));
+    out = scene_of(out, image_bytes(d));
     const size_t np = (size_t)d.w * d.h;
     const size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x;
     if (p >= np) return;
@@ -293,6 +328,13 @@ __global__ __launch_bounds__(kThreads) void k_diff_cm_bwd(Dims d, const float4 *
                                                           const float4 *__restrict__ gout, float4 *gB, float4 *gS,
                                                           float *part) {
     __shared__ float red[kWaves][20];
+    B = scene_of(B, d.ws);
+    S = scene_of(S, d.ws);
+    params = scene_of(params, params_bytes(d));
+    gout = scene_of(gout, image_bytes(d));
+    gB = scene_of(gB, d.ws);
+    gS = scene_of(gS, d.ws);
+    part = scene_of(part, d.ws);
     const size_t np = (size_t)d.w * d.h;
     const size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x;
     const float *m = params + (size_t)d.n * kPP + 1;
@@ -348,6 +390,13 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, Cov cv,
                                                               const float4 *__restrict__ gS,
                                                               const float4 *__restrict__ gS2, float *part) {
     __shared__ float red[kWaves][kSeg][kPP];  // per wave, the segment's path sums
+    cv = scene_cov(cv, d);
+    ends = scene_of(ends, (size_t)d.n * sizeof(float4));
+    params = scene_of(params, params_bytes(d));
+    ck = scene_of(ck, d.ws);
+    gS = scene_of(gS, d.ws);
+    gS2 = scene_of(gS2, d.ws);
+    part = scene_of(part, d.ws);
     const size_t np = (size_t)d.w * d.h;
     const size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x;
     const bool live = p < np;
@@ -470,7 +519,10 @@ __device__ __forceinline__ void bern(float t, float b[4]) {
     b[3] = t * t * t;
 }
 
-__global__ void k_diff_flatten(uint32_t n, uint32_t k, const float *__restrict__ geom, float2 *vert) {
+__global__ void k_diff_flatten(Dims d, const float *__restrict__ geom, float2 *vert) {
+    geom = scene_of(geom, geom_bytes(d));
+    vert = scene_of(vert, d.ws);
+    const uint32_t n = d.n, k = d.k;
     const uint32_t nv = k * kFlat + 1;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n * nv) return;
@@ -488,7 +540,8 @@ __global__ void k_diff_flatten(uint32_t n, uint32_t k, const float *__restrict__
     vert[gid] = make_float2(g[0] * x + g[2] * y + g[4], g[1] * x + g[3] * y + g[5]);
 }
 
-__global__ void k_diff_zero(float *p, size_t n) {
+__global__ void k_diff_zero(float *p, size_t n, size_t stride) {
+    p = scene_of(p, stride);
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0.0f;
 }
@@ -499,8 +552,10 @@ __global__ void k_diff_zero(float *p, size_t n) {
 // is the winding-weighted area of each pixel (the exact area coverage of the
 // polygon's winding number).  x is clamped to [0, w]: area left of the canvas lands
 // in cell 0 and reaches every pixel of the row, area right of it is dropped.
-__global__ void k_diff_accum(Dims d, uint32_t k, const float2 *__restrict__ vert, float *area) {
-    const uint32_t nv = k * kFlat + 1;
+__global__ void k_diff_accum(Dims d, const float2 *__restrict__ vert, float *area) {
+    vert = scene_of(vert, d.ws);
+    area = scene_of(area, d.ws);
+    const uint32_t nv = d.k * kFlat + 1;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= d.n * nv) return;
     const uint32_t i = gid / nv, j = gid % nv;
@@ -557,6 +612,7 @@ __global__ void k_diff_accum(Dims d, uint32_t k, const float2 *__restrict__ vert
 
 // Prefix sums along each row of the area planes (one wave per row), in place.
 __global__ void k_diff_prefix(Dims d, float *area) {
+    area = scene_of(area, d.ws);
     const uint32_t row = blockIdx.x;  // over n * h
     const int lane = threadIdx.x;
     float *r = area + (size_t)row * (d.w + 2);
@@ -578,9 +634,12 @@ __global__ void k_diff_prefix(Dims d, float *area) {
 // δx · (-(b-a).y, (b-a).x) ds, so per pixel crossed on s in [s0, s1]:
 // dA/da = n ∫(1 - s) ds, dA/db = n ∫ s ds.  The pixels are walked in order along
 // the edge; pixels outside the canvas are skipped.  Writes (dL/da, dL/db) per edge.
-__global__ void k_diff_boundary(Dims d, uint32_t k, const float2 *__restrict__ vert, const float *__restrict__ gW,
+__global__ void k_diff_boundary(Dims d, const float2 *__restrict__ vert, const float *__restrict__ gW,
                                 float4 *egrad) {
-    const uint32_t nv = k * kFlat + 1;
+    vert = scene_of(vert, d.ws);
+    gW = scene_of(gW, d.ws);
+    egrad = scene_of(egrad, d.ws);
+    const uint32_t nv = d.k * kFlat + 1;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= d.n * nv) return;
     const uint32_t i = gid / nv, j = gid % nv;
@@ -644,8 +703,12 @@ __global__ void k_diff_boundary(Dims d, uint32_t k, const float2 *__restrict__ v
 // Per path (one thread): vertex gradients from the edge gradients, through the
 // flattening (Bernstein weights) to the device control points, then through the
 // affine to the user-space points and the affine's six values.
-__global__ void k_diff_geom_chain(uint32_t n, uint32_t k, const float *__restrict__ geom,
-                                  const float4 *__restrict__ egrad, float *ggeom) {
+__global__ void k_diff_geom_chain(Dims d, const float *__restrict__ geom, const float4 *__restrict__ egrad,
+                                  float *ggeom) {
+    geom = scene_of(geom, geom_bytes(d));
+    egrad = scene_of(egrad, d.ws);
+    ggeom = scene_of(ggeom, geom_bytes(d));
+    const uint32_t n = d.n, k = d.k;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t nv = k * kFlat + 1, np = 3 * k + 1;
@@ -680,7 +743,10 @@ __global__ void k_diff_geom_chain(uint32_t n, uint32_t k, const float *__restric
 
 // Sums `rows` partial rows of `cols` values into out[col]: a block of 64 columns x
 // 16 row lanes, each lane summing every 16th row, then the lanes in a fixed order.
-__global__ __launch_bounds__(1024) void k_diff_reduce(const float *part, uint32_t rows, uint32_t cols, float *out) {
+__global__ __launch_bounds__(1024) void k_diff_reduce(const float *part, uint32_t rows, uint32_t cols, float *out,
+                                                      size_t part_stride, size_t out_stride) {
+    part = scene_of(part, part_stride);
+    out = scene_of(out, out_stride);
     __shared__ float acc[16][65];
     const uint32_t tx = threadIdx.x % 64, ty = threadIdx.x / 64;
     const uint32_t c = blockIdx.x * 64 + tx;
@@ -726,8 +792,6 @@ Layout layout(uint32_t w, uint32_t h, uint32_t n, uint32_t k) {
     return L;
 }
 
-inline uint8_t *ws_base(void *p) { return (uint8_t *)p; }
-
 bool scene_ok(const rvk_diff_scene *s, const float *geom) {
     return s && s->width > 0 && s->height > 0 && s->path_count > 0 && s->dev_gradient_ends &&
            (s->cubics ? geom != nullptr && s->cubics <= 4096 : s->dev_coverage != nullptr) &&
@@ -745,17 +809,19 @@ rvk_status rvk_diff_workspace_size(uint32_t width, uint32_t height, uint32_t pat
     return RVK_OK;
 }
 
-rvk_status rvk_diff_forward(rvk_context *ctx, const rvk_diff_scene *scene, const float *dev_params,
-                            const float *dev_geom, void *dev_workspace, float *dev_out) {
-    if (!ctx || !scene_ok(scene, dev_geom) || !dev_params || !dev_workspace || !dev_out)
+rvk_status rvk_diff_forward_batch(rvk_context *ctx, const rvk_diff_scene *scene, uint32_t batch,
+                                  const float *dev_params, const float *dev_geom, void *dev_workspace,
+                                  float *dev_out) {
+    if (!ctx || !scene_ok(scene, dev_geom) || !dev_params || !dev_workspace || !dev_out || batch == 0 ||
+        batch > 65535)
         return RVK_ERR_INVALID_ARGUMENT;
     const uint32_t w = scene->width, h = scene->height, n = scene->path_count, k = scene->cubics;
     const Layout L = layout(w, h, n, k);
     uint8_t *ws = (uint8_t *)dev_workspace;
-    const Dims d{w, h, n, (uint32_t)(((size_t)w * h + kThreads - 1) / kThreads)};
+    const Dims d{w, h, n, (uint32_t)(((size_t)w * h + kThreads - 1) / kThreads), k, L.total};
     float4 *S = (float4 *)(ws + L.S), *T = (float4 *)(ws + L.T), *B = (float4 *)(ws + L.B);
     float *wts = (float *)(ws + L.wts);
-    const double px_bytes = (double)w * h;
+    const double px_bytes = (double)w * h * batch;
     Cov cv{scene->dev_coverage, nullptr, nullptr};
     if (k) {
         // analytic coverage: flatten, accumulate winding area, prefix along rows
@@ -763,35 +829,112 @@ rvk_status rvk_diff_forward(rvk_context *ctx, const rvk_diff_scene *scene, const
         float2 *vert = (float2 *)(ws + L.vert);
         const size_t nv = (size_t)k * kFlat + 1, cells = (size_t)n * h * (w + 2);
         const unsigned ge = (unsigned)((n * nv + 255) / 256);
-        rvk::timed_launch(ctx, "diff_coverage", (double)cells * 8.0, [&] {
-            hipLaunchKernelGGL(k_diff_zero, dim3((unsigned)((cells + 255) / 256 < 65535 ? (cells + 255) / 256 : 65535)),
-                               dim3(256), 0, ctx->stream, area, cells);
-            hipLaunchKernelGGL(k_diff_flatten, dim3(ge), dim3(256), 0, ctx->stream, n, k, dev_geom, vert);
-            hipLaunchKernelGGL(k_diff_accum, dim3(ge), dim3(256), 0, ctx->stream, d, k, (const float2 *)vert, area);
-            hipLaunchKernelGGL(k_diff_prefix, dim3(n * h), dim3(64), 0, ctx->stream, d, area);
+        const unsigned gz = (unsigned)((cells + 255) / 256 < 65535 ? (cells + 255) / 256 : 65535);
+        rvk::timed_launch(ctx, "diff_coverage", (double)cells * 8.0 * batch, [&] {
+            hipLaunchKernelGGL(k_diff_zero, dim3(gz, batch), dim3(256), 0, ctx->stream, area, cells, L.total);
+            hipLaunchKernelGGL(k_diff_flatten, dim3(ge, batch), dim3(256), 0, ctx->stream, d, dev_geom, vert);
+            hipLaunchKernelGGL(k_diff_accum, dim3(ge, batch), dim3(256), 0, ctx->stream, d, (const float2 *)vert,
+                               area);
+            hipLaunchKernelGGL(k_diff_prefix, dim3(n * h, batch), dim3(64), 0, ctx->stream, d, area);
         });
         cv = Cov{nullptr, area, nullptr};
     }
     rvk::timed_launch(ctx, "diff_paths_fwd", px_bytes * (16.0 + n), [&] {
-        hipLaunchKernelGGL(k_diff_paths_fwd, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, cv,
+        hipLaunchKernelGGL(k_diff_paths_fwd, dim3(d.nblk, batch), dim3(kThreads), 0, ctx->stream, d, cv,
                            (const float4 *)scene->dev_gradient_ends, dev_params, S, (float4 *)(ws + L.ck));
     });
     rvk::timed_launch(ctx, "diff_blur_weights", 0.0, [&] {
-        hipLaunchKernelGGL(k_diff_blur_weights, dim3(1), dim3(2 * kMaxRadius + 1), 0, ctx->stream, dev_params, n,
-                           wts);
+        hipLaunchKernelGGL(k_diff_blur_weights, dim3(1, batch), dim3(2 * kMaxRadius + 1), 0, ctx->stream, d,
+                           dev_params, wts);
     });
     rvk::timed_launch(ctx, "diff_blur", px_bytes * 32.0, [&] {
-        hipLaunchKernelGGL(k_diff_blur, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, 0, (const float4 *)S, T,
-                           (const float *)wts, (const float4 *)nullptr, (float *)nullptr);
+        hipLaunchKernelGGL(k_diff_blur, dim3(d.nblk, batch), dim3(kThreads), 0, ctx->stream, d, 0, (const float4 *)S,
+                           T, (const float *)wts, (const float4 *)nullptr, (float *)nullptr);
     });
     rvk::timed_launch(ctx, "diff_blur", px_bytes * 32.0, [&] {
-        hipLaunchKernelGGL(k_diff_blur, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, 1, (const float4 *)T, B,
-                           (const float *)wts, (const float4 *)nullptr, (float *)nullptr);
+        hipLaunchKernelGGL(k_diff_blur, dim3(d.nblk, batch), dim3(kThreads), 0, ctx->stream, d, 1, (const float4 *)T,
+                           B, (const float *)wts, (const float4 *)nullptr, (float *)nullptr);
     });
     rvk::timed_launch(ctx, "diff_cm_fwd", px_bytes * 48.0, [&] {
-        hipLaunchKernelGGL(k_diff_cm_fwd, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, (const float4 *)B,
+        hipLaunchKernelGGL(k_diff_cm_fwd, dim3(d.nblk, batch), dim3(kThreads), 0, ctx->stream, d, (const float4 *)B,
                            (const float4 *)S, dev_params, (float4 *)dev_out);
     });
+    RVK_HIP_CHECK(hipGetLastError());
+    return RVK_OK;
+}
+
+rvk_status rvk_diff_forward(rvk_context *ctx, const rvk_diff_scene *scene, const float *dev_params,
+                            const float *dev_geom, void *dev_workspace, float *dev_out) {
+    return rvk_diff_forward_batch(ctx, scene, 1, dev_params, dev_geom, dev_workspace, dev_out);
+}
+
+rvk_status rvk_diff_backward_batch(rvk_context *ctx, const rvk_diff_scene *scene, uint32_t batch,
+                                   const float *dev_params, const float *dev_geom, void *dev_workspace,
+                                   const float *dev_grad_out, float *dev_grad_params, float *dev_grad_geom) {
+    if (!ctx || !scene_ok(scene, dev_geom) || !dev_params || !dev_workspace || !dev_grad_out || !dev_grad_params ||
+        (scene->cubics && !dev_grad_geom) || batch == 0 || batch > 65535)
+        return RVK_ERR_INVALID_ARGUMENT;
+    const uint32_t w = scene->width, h = scene->height, n = scene->path_count, k = scene->cubics;
+    const Layout L = layout(w, h, n, k);
+    uint8_t *ws = (uint8_t *)dev_workspace;
+    Cov cv{scene->dev_coverage, nullptr, nullptr};
+    if (k) cv = Cov{nullptr, (const float *)(ws + L.area), (float *)(ws + L.gW)};
+    const Dims d{w, h, n, (uint32_t)(((size_t)w * h + kThreads - 1) / kThreads), k, L.total};
+    const float4 *S = (const float4 *)(ws + L.S), *T = (const float4 *)(ws + L.T), *B = (const float4 *)(ws + L.B);
+    float4 *gB = (float4 *)(ws + L.gB), *gS = (float4 *)(ws + L.gS), *gS2 = (float4 *)(ws + L.gS2);
+    const float *wts = (const float *)(ws + L.wts);
+    float *part_p = (float *)(ws + L.part_p), *part_m = (float *)(ws + L.part_m), *part_s = (float *)(ws + L.part_s);
+    const double px_bytes = (double)w * h * batch;
+    const dim3 grid(d.nblk, batch);
+    rvk::timed_launch(ctx, "diff_cm_bwd", px_bytes * 80.0, [&] {
+        hipLaunchKernelGGL(k_diff_cm_bwd, grid, dim3(kThreads), 0, ctx->stream, d, B, S, dev_params,
+                           (const float4 *)dev_grad_out, gB, gS, part_m);
+    });
+    // B = V(T): dL/dT = V(gB) (into gS2 for now); σ term Σ gB · V'(T)
+    rvk::timed_launch(ctx, "diff_blur_bwd", px_bytes * 48.0, [&] {
+        hipLaunchKernelGGL(k_diff_blur, grid, dim3(kThreads), 0, ctx->stream, d, 1, (const float4 *)gB, gS2, wts,
+                           (const float4 *)nullptr, (float *)nullptr);
+    });
+    rvk::timed_launch(ctx, "diff_blur_bwd", px_bytes * 48.0, [&] {
+        hipLaunchKernelGGL(k_diff_blur, grid, dim3(kThreads), 0, ctx->stream, d, 1, T, (float4 *)nullptr, wts,
+                           (const float4 *)gB, part_s);
+    });
+    // T = H(S): dL/dS += H(dL/dT) (into gB, free now); σ term Σ dL/dT · H'(S)
+    rvk::timed_launch(ctx, "diff_blur_bwd", px_bytes * 48.0, [&] {
+        hipLaunchKernelGGL(k_diff_blur, grid, dim3(kThreads), 0, ctx->stream, d, 0, S, (float4 *)nullptr, wts,
+                           (const float4 *)gS2, part_s + d.nblk);
+    });
+    rvk::timed_launch(ctx, "diff_blur_bwd", px_bytes * 32.0, [&] {
+        hipLaunchKernelGGL(k_diff_blur, grid, dim3(kThreads), 0, ctx->stream, d, 0, (const float4 *)gS2, gB, wts,
+                           (const float4 *)nullptr, (float *)nullptr);
+    });
+    rvk::timed_launch(ctx, "diff_paths_bwd", px_bytes * (48.0 + 2.0 * n), [&] {
+        hipLaunchKernelGGL(k_diff_paths_bwd, grid, dim3(kThreads), 0, ctx->stream, d, cv,
+                           (const float4 *)scene->dev_gradient_ends, dev_params, (const float4 *)(ws + L.ck),
+                           (const float4 *)gS, (const float4 *)gB, part_p);
+    });
+    const uint32_t np_params = n * kPP;
+    const size_t pstride = ((size_t)np_params + kGlobalParams) * sizeof(float);
+    rvk::timed_launch(ctx, "diff_reduce", 0.0, [&] {
+        hipLaunchKernelGGL(k_diff_reduce, dim3((np_params + 63) / 64, batch), dim3(1024), 0, ctx->stream,
+                           (const float *)part_p, d.nblk, np_params, dev_grad_params, L.total, pstride);
+        // σ: both passes' block sums; matrix: 20 columns of block sums
+        hipLaunchKernelGGL(k_diff_reduce, dim3(1, batch), dim3(1024), 0, ctx->stream, (const float *)part_s,
+                           2u * d.nblk, 1u, dev_grad_params + np_params, L.total, pstride);
+        hipLaunchKernelGGL(k_diff_reduce, dim3(1, batch), dim3(1024), 0, ctx->stream, (const float *)part_m, d.nblk,
+                           20u, dev_grad_params + np_params + 1, L.total, pstride);
+    });
+    if (k) {
+        const size_t nv = (size_t)k * kFlat + 1;
+        const unsigned ge = (unsigned)((n * nv + 255) / 256);
+        float4 *egrad = (float4 *)(ws + L.egrad);
+        rvk::timed_launch(ctx, "diff_boundary", (double)n * w * h * 4.0 * batch, [&] {
+            hipLaunchKernelGGL(k_diff_boundary, dim3(ge, batch), dim3(256), 0, ctx->stream, d,
+                               (const float2 *)(ws + L.vert), (const float *)(ws + L.gW), egrad);
+            hipLaunchKernelGGL(k_diff_geom_chain, dim3((n + 63) / 64, batch), dim3(64), 0, ctx->stream, d, dev_geom,
+                               (const float4 *)egrad, dev_grad_geom);
+        });
+    }
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;
 }
@@ -799,70 +942,8 @@ rvk_status rvk_diff_forward(rvk_context *ctx, const rvk_diff_scene *scene, const
 rvk_status rvk_diff_backward(rvk_context *ctx, const rvk_diff_scene *scene, const float *dev_params,
                              const float *dev_geom, void *dev_workspace, const float *dev_grad_out,
                              float *dev_grad_params, float *dev_grad_geom) {
-    if (!ctx || !scene_ok(scene, dev_geom) || !dev_params || !dev_workspace || !dev_grad_out || !dev_grad_params ||
-        (scene->cubics && !dev_grad_geom))
-        return RVK_ERR_INVALID_ARGUMENT;
-    const uint32_t w = scene->width, h = scene->height, n = scene->path_count, k = scene->cubics;
-    const Layout L = layout(w, h, n, k);
-    Cov cv{scene->dev_coverage, nullptr, nullptr};
-    if (k) cv = Cov{nullptr, (const float *)(ws_base(dev_workspace) + L.area), (float *)(ws_base(dev_workspace) + L.gW)};
-    uint8_t *ws = (uint8_t *)dev_workspace;
-    const Dims d{w, h, n, (uint32_t)(((size_t)w * h + kThreads - 1) / kThreads)};
-    const float4 *S = (const float4 *)(ws + L.S), *T = (const float4 *)(ws + L.T), *B = (const float4 *)(ws + L.B);
-    float4 *gB = (float4 *)(ws + L.gB), *gS = (float4 *)(ws + L.gS), *gS2 = (float4 *)(ws + L.gS2);
-    const float *wts = (const float *)(ws + L.wts);
-    float *part_p = (float *)(ws + L.part_p), *part_m = (float *)(ws + L.part_m), *part_s = (float *)(ws + L.part_s);
-    const double px_bytes = (double)w * h;
-    rvk::timed_launch(ctx, "diff_cm_bwd", px_bytes * 80.0, [&] {
-        hipLaunchKernelGGL(k_diff_cm_bwd, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, B, S, dev_params,
-                           (const float4 *)dev_grad_out, gB, gS, part_m);
-    });
-    // B = V(T): dL/dT = V(gB) (into gS2 for now); σ term Σ gB · V'(T)
-    rvk::timed_launch(ctx, "diff_blur_bwd", px_bytes * 48.0, [&] {
-        hipLaunchKernelGGL(k_diff_blur, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, 1, (const float4 *)gB,
-                           gS2, wts, (const float4 *)nullptr, (float *)nullptr);
-    });
-    rvk::timed_launch(ctx, "diff_blur_bwd", px_bytes * 48.0, [&] {
-        hipLaunchKernelGGL(k_diff_blur, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, 1, T, (float4 *)nullptr,
-                           wts, (const float4 *)gB, part_s);
-    });
-    // T = H(S): dL/dS += H(dL/dT) (into gB, free now); σ term Σ dL/dT · H'(S)
-    rvk::timed_launch(ctx, "diff_blur_bwd", px_bytes * 48.0, [&] {
-        hipLaunchKernelGGL(k_diff_blur, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, 0, S, (float4 *)nullptr,
-                           wts, (const float4 *)gS2, part_s + d.nblk);
-    });
-    rvk::timed_launch(ctx, "diff_blur_bwd", px_bytes * 32.0, [&] {
-        hipLaunchKernelGGL(k_diff_blur, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, 0, (const float4 *)gS2,
-                           gB, wts, (const float4 *)nullptr, (float *)nullptr);
-    });
-    rvk::timed_launch(ctx, "diff_paths_bwd", px_bytes * (48.0 + 2.0 * n), [&] {
-        hipLaunchKernelGGL(k_diff_paths_bwd, dim3(d.nblk), dim3(kThreads), 0, ctx->stream, d, cv,
-                           (const float4 *)scene->dev_gradient_ends, dev_params,
-                           (const float4 *)(ws + L.ck), (const float4 *)gS, (const float4 *)gB, part_p);
-    });
-    const uint32_t np_params = n * kPP;
-    rvk::timed_launch(ctx, "diff_reduce", 0.0, [&] {
-        hipLaunchKernelGGL(k_diff_reduce, dim3((np_params + 63) / 64), dim3(1024), 0, ctx->stream,
-                           (const float *)part_p, d.nblk, np_params, dev_grad_params);
-        // σ: both passes' block sums; matrix: 20 columns of block sums
-        hipLaunchKernelGGL(k_diff_reduce, dim3(1), dim3(1024), 0, ctx->stream, (const float *)part_s, 2u * d.nblk,
-                           1u, dev_grad_params + np_params);
-        hipLaunchKernelGGL(k_diff_reduce, dim3(1), dim3(1024), 0, ctx->stream, (const float *)part_m, d.nblk, 20u,
-                           dev_grad_params + np_params + 1);
-    });
-    if (k) {
-        const size_t nv = (size_t)k * kFlat + 1;
-        const unsigned ge = (unsigned)((n * nv + 255) / 256);
-        float4 *egrad = (float4 *)(ws + L.egrad);
-        rvk::timed_launch(ctx, "diff_boundary", (double)n * w * h * 4.0, [&] {
-            hipLaunchKernelGGL(k_diff_boundary, dim3(ge), dim3(256), 0, ctx->stream, d, k,
-                               (const float2 *)(ws + L.vert), (const float *)(ws + L.gW), egrad);
-            hipLaunchKernelGGL(k_diff_geom_chain, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, n, k, dev_geom,
-                               (const float4 *)egrad, dev_grad_geom);
-        });
-    }
-    RVK_HIP_CHECK(hipGetLastError());
-    return RVK_OK;
+    return rvk_diff_backward_batch(ctx, scene, 1, dev_params, dev_geom, dev_workspace, dev_grad_out,
+                                   dev_grad_params, dev_grad_geom);
 }
 
 }  // extern "C"

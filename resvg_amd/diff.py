@@ -49,6 +49,10 @@ def _bind():
     L.rvk_diff_forward.restype = C.c_int32
     L.rvk_diff_backward.argtypes = [P, C.POINTER(DiffScene), P, P, P, P, P, P]
     L.rvk_diff_backward.restype = C.c_int32
+    L.rvk_diff_forward_batch.argtypes = [P, C.POINTER(DiffScene), u32, P, P, P, P]
+    L.rvk_diff_forward_batch.restype = C.c_int32
+    L.rvk_diff_backward_batch.argtypes = [P, C.POINTER(DiffScene), u32, P, P, P, P, P, P]
+    L.rvk_diff_backward_batch.restype = C.c_int32
     return L
 
 
@@ -78,53 +82,80 @@ def geometry(paths, k: int):
 
 
 class Renderer:
-    """The coverage source, gradient ends and workspace of one scene on one context.
-    `cubics` > 0 selects the analytic mode (paths of that many cubics)."""
+    """The coverage source, gradient ends and workspace of one scene, or of a batch
+    of scenes of one shape (`Renderer.batched`), on one context.  `cubics` > 0
+    selects the analytic mode (paths of that many cubics)."""
 
     def __init__(self, ctx, width: int, height: int, paths, ends, cubics: int = 0):
+        self._setup(ctx, width, height, [(paths, ends)], cubics, single=True)
+
+    @classmethod
+    def batched(cls, ctx, width: int, height: int, scenes, cubics: int = 0):
+        """`scenes`: a list of (paths, ends), every scene with the same number of
+        paths.  Parameters are then (batch, n_params), geometry (batch, paths,
+        GEOM_FLOATS) and images (batch, height, width, 4); every kernel runs once for
+        the whole batch."""
+        r = cls.__new__(cls)
+        r._setup(ctx, width, height, list(scenes), cubics, single=False)
+        return r
+
+    def _setup(self, ctx, width, height, scenes, cubics, single):
         import torch
-        self.ctx, self.w, self.h, self.n, self.k = ctx, int(width), int(height), len(paths), int(cubics)
+        self.ctx, self.w, self.h, self.k = ctx, int(width), int(height), int(cubics)
+        self.batch, self.single = len(scenes), single
+        self.n = len(scenes[0][0])
+        assert self.batch >= 1 and all(len(p) == self.n for p, _ in scenes), "scenes of one path count"
         self._L = _bind()
         dev = torch.device("cuda", ctx.device)
         if self.k == 0:
             # coverage: every path rendered alone, opaque white, through one batch call
-            pxs = torch.zeros((self.n, self.h, self.w, 4), dtype=torch.uint8, device=dev)
-            scenes = [one_path_scene((self.w, self.h), v, p) for v, p in paths]
-            ctx.render_batch(scenes, abi.Transform.identity(), [pxs[i] for i in range(self.n)])
+            pxs = torch.zeros((self.batch * self.n, self.h, self.w, 4), dtype=torch.uint8, device=dev)
+            docs = [one_path_scene((self.w, self.h), v, p) for paths, _ in scenes for v, p in paths]
+            ctx.render_batch(docs, abi.Transform.identity(), [pxs[i] for i in range(len(docs))])
             self.coverage = pxs[..., 3].contiguous()
             del pxs
             self.geom0 = None
         else:
             self.coverage = None
-            self.geom0 = torch.as_tensor(geometry(paths, self.k), device=dev)
-        self.ends = torch.as_tensor(np.asarray(ends, np.float32).reshape(self.n, 4), device=dev).contiguous()
+            g = np.stack([geometry(paths, self.k) for paths, _ in scenes])
+            self.geom0 = torch.as_tensor(g[0] if single else g, device=dev)
+        e = np.stack([np.asarray(ends, np.float32).reshape(self.n, 4) for _, ends in scenes])
+        self.ends = torch.as_tensor(e, device=dev).contiguous()
         nbytes = C.c_size_t(0)
         _check(self._L.rvk_diff_workspace_size(self.w, self.h, self.n, self.k, C.byref(nbytes)),
                "rvk_diff_workspace_size")
-        self.workspace = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
+        self.workspace_bytes = int(nbytes.value)
+        self.workspace = torch.empty(self.batch * self.workspace_bytes, dtype=torch.uint8, device=dev)
         cov = self.coverage.data_ptr() if self.coverage is not None else None
         self.scene = DiffScene(self.w, self.h, self.n, self.k, cov, self.ends.data_ptr())
 
+    def _image_shape(self):
+        return (self.h, self.w, 4) if self.single else (self.batch, self.h, self.w, 4)
+
     def forward(self, params, geom=None):
         import torch
-        assert params.dtype == torch.float32 and params.is_contiguous() and params.numel() == n_params(self.n)
+        assert params.dtype == torch.float32 and params.is_contiguous()
+        assert params.numel() == self.batch * n_params(self.n)
         if self.k:
-            assert geom is not None and geom.is_contiguous() and geom.numel() == self.n * geom_floats(self.k)
-        out = torch.empty((self.h, self.w, 4), dtype=torch.float32, device=params.device)
-        _check(self._L.rvk_diff_forward(self.ctx._h, C.byref(self.scene), _ptr(params),
-                                        _ptr(geom) if self.k else None, _ptr(self.workspace), _ptr(out)),
-               "rvk_diff_forward")
+            assert geom is not None and geom.dtype == torch.float32 and geom.is_contiguous()
+            assert geom.numel() == self.batch * self.n * geom_floats(self.k)
+        out = torch.empty(self._image_shape(), dtype=torch.float32, device=params.device)
+        _check(self._L.rvk_diff_forward_batch(self.ctx._h, C.byref(self.scene), self.batch, _ptr(params),
+                                              _ptr(geom) if self.k else None, _ptr(self.workspace), _ptr(out)),
+               "rvk_diff_forward_batch")
         return out
 
     def backward(self, params, grad_out, geom=None):
         """dL/d(params), and dL/d(geom) in the analytic mode (else None)."""
         import torch
         grad_out = grad_out.contiguous().float()
+        assert grad_out.numel() == self.batch * self.w * self.h * 4
         g = torch.empty_like(params)
         gg = torch.empty_like(geom) if self.k else None
-        _check(self._L.rvk_diff_backward(self.ctx._h, C.byref(self.scene), _ptr(params),
-                                         _ptr(geom) if self.k else None, _ptr(self.workspace), _ptr(grad_out),
-                                         _ptr(g), _ptr(gg) if self.k else None), "rvk_diff_backward")
+        _check(self._L.rvk_diff_backward_batch(self.ctx._h, C.byref(self.scene), self.batch, _ptr(params),
+                                               _ptr(geom) if self.k else None, _ptr(self.workspace),
+                                               _ptr(grad_out), _ptr(g), _ptr(gg) if self.k else None),
+               "rvk_diff_backward_batch")
         return (g, gg) if self.k else g
 
 
@@ -159,24 +190,29 @@ def u8_scene(width, height, paths, ends, params) -> Scene:
     """The same document as an ordinary u8 scene (rvk_render's input): each path in
     an opacity group with its gradient (stops premultiplied by nothing, colours
     quantised to u8), all inside a group filtered by blur -> colour matrix ->
-    composite over the source.  Used to time the plain forward of the same work."""
+    composite over the source, in sRGB (color-interpolation-filters="sRGB": the f32
+    model has no colour-space conversion).  σ is clamped to the model's [0.05, 16]
+    and the offsets made non-decreasing, as usvg does for stops.  Used to time the
+    plain forward of the same work and to score fitted parameters."""
     from .filters import SOURCE_GRAPHIC, FilterGraph
     p = np.asarray(params, np.float64)
     n = len(paths)
     sc = Scene(width, height)
     g = FilterGraph((0.0, 0.0, float(width), float(height)))
-    sigma = float(p[n * PATH_PARAMS])
-    b = g.gaussian_blur(sigma)
-    cm = g.color_matrix(abi.COLOR_MATRIX_MATRIX, p[n * PATH_PARAMS + 1:].astype(np.float32).tolist(), in1=b)
-    g.composite(abi.COMPOSITE_OVER, in1=cm, in2=SOURCE_GRAPHIC)
+    sigma = float(np.clip(p[n * PATH_PARAMS], 0.05, 16.0))
+    cs = abi.CI_SRGB
+    b = g.gaussian_blur(sigma, cs=cs)
+    cm = g.color_matrix(abi.COLOR_MATRIX_MATRIX, p[n * PATH_PARAMS + 1:].astype(np.float32).tolist(), in1=b, cs=cs)
+    g.composite(abi.COMPOSITE_OVER, in1=cm, in2=SOURCE_GRAPHIC, cs=cs)
     root = sc.group(sc.root, filters=[g])
     for i, (verbs, pts) in enumerate(paths):
         q = p[i * PATH_PARAMS:(i + 1) * PATH_PARAMS]
-        tint, opacity, cols, offs = q[0:3], float(q[3]), q[4:20].reshape(4, 4), q[20:24]
+        tint, opacity, cols = q[0:3], float(q[3]), q[4:20].reshape(4, 4)
+        offs = np.maximum.accumulate(np.clip(q[20:24], 0.0, 1.0))
         stops = []
         for k in range(4):
             rgb = tuple(int(np.clip(round(cols[k, c] * tint[c] * 255.0), 0, 255)) for c in range(3))
-            stops.append((float(np.clip(offs[k], 0.0, 1.0)), rgb, float(np.clip(cols[k, 3], 0.0, 1.0))))
+            stops.append((float(offs[k]), rgb, float(np.clip(cols[k, 3], 0.0, 1.0))))
         x0, y0, x1, y1 = ends[i]
         grp = sc.group(root, opacity=float(np.clip(opacity, 0.0, 1.0)))
         sc.path(grp, verbs, pts, fill=PaintSpec.linear(float(x0), float(y0), float(x1), float(y1), stops))

@@ -186,3 +186,47 @@ def test_analytic_fit_moves_the_geometry(ctx):
         first = loss.item() if first is None else first
         opt.step()
     assert loss.item() < 0.5 * first
+
+
+# ---- the batch form: one launch per kernel for a stack of scenes -------------------
+@pytest.mark.parametrize("cubics", [0, 3])
+def test_batch_matches_one_call_per_scene(ctx, cubics):
+    """rvk_diff_forward_batch / rvk_diff_backward_batch give, for every scene of the
+    stack, exactly what rvk_diff_forward / rvk_diff_backward give for it alone (the
+    same kernels, per-scene reductions in the same order)."""
+    import torch
+    size, n, k = 40, 5, (cubics or 4)
+    scenes, ps, gs = [], [], []
+    for b in range(3):
+        w, h, paths, ends, p0, _ = diff_scene(seed=20 + b, size=size, n_paths=n, segments=k, sigma=1.0 + 0.6 * b)
+        scenes.append((paths, ends))
+        ps.append(p0)
+        gs.append(diff.geometry(paths, k))
+    rb = diff.Renderer.batched(ctx, size, size, scenes, cubics=cubics)
+    P = torch.as_tensor(np.stack(ps), device="cuda").contiguous()
+    G = torch.as_tensor(np.stack(gs), device="cuda").contiguous() if cubics else None
+    if cubics:
+        G[:, :, 4:6] += torch.tensor([0.7, -0.4], device="cuda")  # a non-trivial translation
+    rng = np.random.default_rng(7)
+    go = torch.as_tensor(rng.normal(0.0, 1.0, (3, size, size, 4)).astype(np.float32), device="cuda")
+    out_b = rb.forward(P, G).clone()
+    grads_b = rb.backward(P, go, G)
+    for b in range(3):
+        r1 = diff.Renderer(ctx, size, size, scenes[b][0], scenes[b][1], cubics=cubics)
+        g1 = G[b].contiguous() if cubics else None
+        out1 = r1.forward(P[b].contiguous(), g1)
+        assert torch.equal(out1, out_b[b]), f"scene {b}: forward differs"
+        grads1 = r1.backward(P[b].contiguous(), go[b], g1)
+        if cubics:
+            assert torch.equal(grads1[0], grads_b[0][b]) and torch.equal(grads1[1], grads_b[1][b]), f"scene {b}"
+        else:
+            assert torch.equal(grads1, grads_b[b]), f"scene {b}: parameter gradients differ"
+
+
+def test_batch_fit_lowers_every_scene_loss(ctx):
+    from resvg_amd import fit
+    wl = fit.fit_workload(count=4, size=64, n_paths=6, segments=3, seed=3)
+    res = fit.fit(ctx, wl, steps=60, lr=1e-2)
+    assert res["mse_ratio_end_over_start"] < 0.8, res
+    assert res["psnr_f32_db"] > res["psnr_start_db"] + 1.0, res
+    assert np.isfinite(res["psnr_u8_db"]) and res["psnr_u8_db"] > 5.0, res
