@@ -143,7 +143,12 @@ def flatten(g, k):
 
 
 def winding_area(verts, w, h):
-    """Per pixel, the integral of the polygon's winding number (rows x columns)."""
+    """Per pixel, the integral of the polygon's winding number (rows x columns).
+
+    Along each pixel row, an edge's stretch is split where it crosses x = 0 and x = w:
+    a piece left of the canvas is moved onto x = 0 (it changes the winding of every
+    pixel of the row alike), a piece right of it is dropped, so the result is the
+    exact integral over every canvas pixel whatever lies outside."""
     acc = np.zeros((h, w + 2))
     nv = len(verts)
     for j in range(nv):
@@ -159,27 +164,38 @@ def winding_area(verts, w, h):
             dy = yb - ya
             if dy <= 0:
                 continue
-            xa = min(max(ax + dxdy * (ya - ay), 0.0), w)
-            xb = min(max(ax + dxdy * (yb - ay), 0.0), w)
-            dd = dy * d
-            x0, x1 = min(xa, xb), max(xa, xb)
-            x0f = math.floor(x0)
-            x0i = int(x0f)
-            x1c = math.ceil(x1)
-            if x1c <= x0f + 1:
-                xm = 0.5 * (xa + xb) - x0f
-                acc[y, x0i] += dd * (1 - xm)
-                acc[y, x0i + 1] += dd * xm
-            else:
-                # the row's trapezoid between x0 and x1, split by the pixel columns
-                s = 1.0 / (x1 - x0)
-                x1i = int(x1c)
-                for xi in range(x0i, x1i + 1):
-                    # area, within [xi, xi + 1), of the part of the row right of the edge
-                    # = integral over the row of clamp(xi + 1 - x(y), 0, 1) minus the
-                    # same for xi - 1 (the accumulation's difference form)
-                    acc[y, xi] += dd * (_ramp(xi + 1, x0, x1, s) - _ramp(xi, x0, x1, s))
+            xa = ax + dxdy * (ya - ay)
+            xb = ax + dxdy * (yb - ay)
+            ts = sorted(t for t in ((0.0 - xa) / (xb - xa), (w - xa) / (xb - xa)) if 0.0 < t < 1.0) \
+                if xb != xa else []
+            cuts = [0.0] + ts + [1.0]
+            for t0, t1 in zip(cuts[:-1], cuts[1:]):
+                pa = min(max(xa + t0 * (xb - xa), 0.0), w)
+                pb = min(max(xa + t1 * (xb - xa), 0.0), w)
+                _row_piece(acc[y], pa, pb, dy * d * (t1 - t0))
     return np.cumsum(acc[:, :w], axis=1)
+
+
+def _row_piece(row, xa, xb, dd):
+    """One straight piece of an edge within a pixel row, x in [0, w], its share dd of
+    the row's height (signed): adds to `row` the differences whose prefix sum is the
+    piece's contribution to each pixel's winding area."""
+    x0, x1 = min(xa, xb), max(xa, xb)
+    x0f = math.floor(x0)
+    x0i = int(x0f)
+    x1c = math.ceil(x1)
+    if x1c <= x0f + 1:
+        xm = 0.5 * (xa + xb) - x0f
+        row[x0i] += dd * (1 - xm)
+        row[x0i + 1] += dd * xm
+    else:
+        # the trapezoid between x0 and x1, split by the pixel columns
+        s = 1.0 / (x1 - x0)
+        for xi in range(x0i, int(x1c) + 1):
+            # area, within [xi, xi + 1), of the part of the row right of the edge
+            # = integral over the row of clamp(xi + 1 - x(y), 0, 1) minus the same for
+            # xi - 1 (the accumulation's difference form)
+            row[xi] += dd * (_ramp(xi + 1, x0, x1, s) - _ramp(xi, x0, x1, s))
 
 
 def _ramp(c, x0, x1, s):

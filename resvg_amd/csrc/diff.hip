@@ -30,6 +30,8 @@
 // into partial rows and reduced by a second kernel, so they are deterministic.
 #include "rvk_internal.h"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int kSeg = 8;          // paths per checkpoint segment
@@ -44,13 +46,21 @@ struct Dims {
     uint32_t nblk;           // blocks of the per-pixel kernels
     uint32_t k;              // cubics per path (analytic mode), else 0
     size_t ws;               // workspace bytes per scene
+    float qscale;            // analytic mode: fixed-point scale of the area accumulation
 };
 
 // A batch holds scenes of one shape, stacked: scene blockIdx.y finds its parameters,
 // geometry, gradient ends, coverage planes, workspace and images at blockIdx.y times
 // the per-scene size of each (a null pointer stays null).
+// (char arithmetic, not an integer round trip, so that the compiler keeps the
+// pointer in the global address space: scalar loads for uniform values)
+template <class T> __device__ __forceinline__ T *scene_at(T *p, size_t bytes) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T *>(reinterpret_cast<B *>(p) + (size_t)blockIdx.y * bytes);
+}
+// the same for a pointer that may be null (an optional buffer)
 template <class T> __device__ __forceinline__ T *scene_of(T *p, size_t bytes) {
-    return p ? reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.y * bytes) : p;
+    return p ? scene_at(p, bytes) : p;
 }
 __device__ __forceinline__ size_t params_bytes(const Dims &d) {
     return ((size_t)d.n * kPP + kGlobalParams) * sizeof(float);
@@ -167,10 +177,10 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_fwd(Dims d, Cov cv,
                                                               const float *__restrict__ params, float4 *canvas,
                                                               float4 *ck) {
     cv = scene_cov(cv, d);
-    ends = scene_of(ends, (size_t)d.n * sizeof(float4));
-    params = scene_of(params, params_bytes(d));
-    canvas = scene_of(canvas, d.ws);
-    ck = scene_of(ck, d.ws);
+    ends = scene_at(ends, (size_t)d.n * sizeof(float4));
+    params = scene_at(params, params_bytes(d));
+    canvas = scene_at(canvas, d.ws);
+    ck = scene_at(ck, d.ws);
     const size_t np = (size_t)d.w * d.h;
     const size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x;
     if (p >= np) return;
@@ -193,8 +203,8 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_fwd(Dims d, Cov cv,
 __global__ void k_diff_blur_weights(Dims d, const float *params, float *wts) {
     __shared__ float g[2 * kMaxRadius + 1];
     __shared__ float sums[2];
-    params = scene_of(params, params_bytes(d));
-    wts = scene_of(wts, d.ws);
+    params = scene_at(params, params_bytes(d));
+    wts = scene_at(wts, d.ws);
     const float raw = params[(size_t)d.n * kPP];
     const float sigma = fminf(fmaxf(raw, 0.05f), 16.0f);
     const float live = raw > 0.05f && raw < 16.0f ? 1.0f : 0.0f;  // no σ gradient where it is clamped
@@ -232,9 +242,9 @@ __global__ __launch_bounds__(kThreads) void k_diff_blur(Dims d, int dir, const f
                                                          float4 *__restrict__ out, const float *__restrict__ wts,
                                                          const float4 *__restrict__ gdot, float *part) {
     __shared__ float red[kWaves];
-    in = scene_of(in, d.ws);
+    in = scene_at(in, d.ws);
     out = scene_of(out, d.ws);
-    wts = scene_of(wts, d.ws);
+    wts = scene_at(wts, d.ws);
     gdot = scene_of(gdot, d.ws);
     part = scene_of(part, d.ws);
     const size_t np = (size_t)d.w * d.h;
@@ -304,10 +314,10 @@ __device__ __forceinline__ CmEval cm_eval(const float *m, float4 b) {
 __global__ __launch_bounds__(kThreads) void k_diff_cm_fwd(Dims d, const float4 *__restrict__ B,
                                                           const float4 *__restrict__ S,
                                                           const float *__restrict__ params, float4 *out) {
-    B = scene_of(B, d.ws);
-    S = scene_of(S, d.ws);
-    params = scene_of(params, params_bytes(d));
-    out = scene_of(out, image_bytes(d));
+    B = scene_at(B, d.ws);
+    S = scene_at(S, d.ws);
+    params = scene_at(params, params_bytes(d));
+    out = scene_at(out, image_bytes(d));
     const size_t np = (size_t)d.w * d.h;
     const size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x;
     if (p >= np) return;
@@ -328,13 +338,13 @@ __global__ __launch_bounds__(kThreads) void k_diff_cm_bwd(Dims d, const float4 *
                                                           const float4 *__restrict__ gout, float4 *gB, float4 *gS,
                                                           float *part) {
     __shared__ float red[kWaves][20];
-    B = scene_of(B, d.ws);
-    S = scene_of(S, d.ws);
-    params = scene_of(params, params_bytes(d));
-    gout = scene_of(gout, image_bytes(d));
-    gB = scene_of(gB, d.ws);
-    gS = scene_of(gS, d.ws);
-    part = scene_of(part, d.ws);
+    B = scene_at(B, d.ws);
+    S = scene_at(S, d.ws);
+    params = scene_at(params, params_bytes(d));
+    gout = scene_at(gout, image_bytes(d));
+    gB = scene_at(gB, d.ws);
+    gS = scene_at(gS, d.ws);
+    part = scene_at(part, d.ws);
     const size_t np = (size_t)d.w * d.h;
     const size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x;
     const float *m = params + (size_t)d.n * kPP + 1;
@@ -391,12 +401,12 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, Cov cv,
                                                               const float4 *__restrict__ gS2, float *part) {
     __shared__ float red[kWaves][kSeg][kPP];  // per wave, the segment's path sums
     cv = scene_cov(cv, d);
-    ends = scene_of(ends, (size_t)d.n * sizeof(float4));
-    params = scene_of(params, params_bytes(d));
-    ck = scene_of(ck, d.ws);
-    gS = scene_of(gS, d.ws);
-    gS2 = scene_of(gS2, d.ws);
-    part = scene_of(part, d.ws);
+    ends = scene_at(ends, (size_t)d.n * sizeof(float4));
+    params = scene_at(params, params_bytes(d));
+    ck = scene_at(ck, d.ws);
+    gS = scene_at(gS, d.ws);
+    gS2 = scene_at(gS2, d.ws);
+    part = scene_at(part, d.ws);
     const size_t np = (size_t)d.w * d.h;
     const size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x;
     const bool live = p < np;
@@ -520,8 +530,8 @@ __device__ __forceinline__ void bern(float t, float b[4]) {
 }
 
 __global__ void k_diff_flatten(Dims d, const float *__restrict__ geom, float2 *vert) {
-    geom = scene_of(geom, geom_bytes(d));
-    vert = scene_of(vert, d.ws);
+    geom = scene_at(geom, geom_bytes(d));
+    vert = scene_at(vert, d.ws);
     const uint32_t n = d.n, k = d.k;
     const uint32_t nv = k * kFlat + 1;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -541,20 +551,60 @@ __global__ void k_diff_flatten(Dims d, const float *__restrict__ geom, float2 *v
 }
 
 __global__ void k_diff_zero(float *p, size_t n, size_t stride) {
-    p = scene_of(p, stride);
+    p = scene_at(p, stride);
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0.0f;
+}
+
+// One straight piece of an edge within a pixel row (x in [0, w], dd its signed share
+// of the row's height): adds the differences whose prefix sum along the row is the
+// piece's contribution to each pixel's winding area (the trapezoid right of the
+// piece, split by the pixel columns).  The differences are added in fixed point
+// (round(v * q), integer atomics), so the sums do not depend on the order in which
+// edges arrive.
+__device__ __forceinline__ void add_q(int *cell, float v, float q) { atomicAdd(cell, __float2int_rn(v * q)); }
+__device__ __forceinline__ void accum_piece(int *row, float xa, float xb, float dd, float q) {
+    const float x0 = fminf(xa, xb), x1 = fmaxf(xa, xb);
+    const float x0f = floorf(x0);
+    const int x0i = (int)x0f;
+    const float x1c = ceilf(x1);
+    if (x1c <= x0f + 1.0f) {  // within one column
+        const float xm = 0.5f * (xa + xb) - x0f;
+        add_q(row + x0i, dd * (1.0f - xm), q);
+        add_q(row + x0i + 1, dd * xm, q);
+        return;
+    }
+    const float sl = 1.0f / (x1 - x0);
+    const float fx0 = x0 - x0f;
+    const float a0 = 0.5f * sl * (1.0f - fx0) * (1.0f - fx0);
+    const float fx1 = x1 - x1c + 1.0f;
+    const float am = 0.5f * sl * fx1 * fx1;
+    const int x1i = (int)x1c;
+    add_q(row + x0i, dd * a0, q);
+    if (x1i == x0i + 2) {
+        add_q(row + x0i + 1, dd * (1.0f - a0 - am), q);
+    } else {
+        const float a1 = sl * (1.5f - fx0);
+        add_q(row + x0i + 1, dd * (a1 - a0), q);
+        for (int xi = x0i + 2; xi < x1i - 1; xi++) add_q(row + xi, dd * sl, q);
+        const float a2 = a1 + (float)(x1i - x0i - 3) * sl;
+        add_q(row + x1i - 1, dd * (1.0f - a2 - am), q);
+    }
+    add_q(row + x1i, dd * am, q);
 }
 
 // Signed area accumulation of one polygon edge into its path's plane (row stride
 // w + 2): per pixel row the edge spans, the row's share of the edge's winding is
 // split between the cells its x range touches so that the prefix sum along the row
 // is the winding-weighted area of each pixel (the exact area coverage of the
-// polygon's winding number).  x is clamped to [0, w]: area left of the canvas lands
-// in cell 0 and reaches every pixel of the row, area right of it is dropped.
-__global__ void k_diff_accum(Dims d, const float2 *__restrict__ vert, float *area) {
-    vert = scene_of(vert, d.ws);
-    area = scene_of(area, d.ws);
+// polygon's winding number).  The edge's stretch in a row is cut where it crosses
+// x = 0 and x = w: a piece left of the canvas is moved onto x = 0 (cell 0, so it
+// changes the winding of every pixel of the row alike), a piece right of it lands
+// in cell w, beyond the prefix, so the planes are exact whatever lies outside.  The
+// planes hold fixed-point integers (scale d.qscale) until k_diff_prefix.
+__global__ void k_diff_accum(Dims d, const float2 *__restrict__ vert, int *area) {
+    vert = scene_at(vert, d.ws);
+    area = scene_at(area, d.ws);
     const uint32_t nv = d.k * kFlat + 1;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= d.n * nv) return;
@@ -570,61 +620,56 @@ __global__ void k_diff_accum(Dims d, const float2 *__restrict__ vert, float *are
     }
     const float W = (float)d.w;
     const float dxdy = (b.x - a.x) / (b.y - a.y);
-    float *plane = area + (size_t)i * d.h * (d.w + 2);
+    int *plane = area + (size_t)i * d.h * (d.w + 2);
     const int y0 = max(0, (int)floorf(a.y)), y1 = min((int)d.h, (int)ceilf(b.y));
     for (int y = y0; y < y1; y++) {
         const float ya = fmaxf(a.y, (float)y), yb = fminf(b.y, (float)(y + 1));
         const float dy = yb - ya;
         if (!(dy > 0.0f)) continue;
-        const float xa = fminf(fmaxf(a.x + dxdy * (ya - a.y), 0.0f), W);
-        const float xb = fminf(fmaxf(a.x + dxdy * (yb - a.y), 0.0f), W);
+        const float xa = a.x + dxdy * (ya - a.y), xb = a.x + dxdy * (yb - a.y);
+        int *row = plane + (size_t)y * (d.w + 2);
         const float dd = dy * dir;
-        float *row = plane + (size_t)y * (d.w + 2);
-        const float x0 = fminf(xa, xb), x1 = fmaxf(xa, xb);
-        const float x0f = floorf(x0);
-        const int x0i = (int)x0f;
-        const float x1c = ceilf(x1);
-        if (x1c <= x0f + 1.0f) {  // within one column
-            const float xm = 0.5f * (xa + xb) - x0f;
-            atomicAdd(row + x0i, dd * (1.0f - xm));
-            atomicAdd(row + x0i + 1, dd * xm);
-        } else {
-            const float sl = 1.0f / (x1 - x0);
-            const float fx0 = x0 - x0f;
-            const float a0 = 0.5f * sl * (1.0f - fx0) * (1.0f - fx0);
-            const float fx1 = x1 - x1c + 1.0f;
-            const float am = 0.5f * sl * fx1 * fx1;
-            const int x1i = (int)x1c;
-            atomicAdd(row + x0i, dd * a0);
-            if (x1i == x0i + 2) {
-                atomicAdd(row + x0i + 1, dd * (1.0f - a0 - am));
-            } else {
-                const float a1 = sl * (1.5f - fx0);
-                atomicAdd(row + x0i + 1, dd * (a1 - a0));
-                for (int xi = x0i + 2; xi < x1i - 1; xi++) atomicAdd(row + xi, dd * sl);
-                const float a2 = a1 + (float)(x1i - x0i - 3) * sl;
-                atomicAdd(row + x1i - 1, dd * (1.0f - a2 - am));
+        // cut points (fractions of the stretch) at x = 0 and x = w, in order
+        float c0 = 2.0f, c1 = 2.0f;
+        if (xb != xa) {
+            const float u = (0.0f - xa) / (xb - xa), v = (W - xa) / (xb - xa);
+            const float lo = fminf(u, v), hi = fmaxf(u, v);
+            if (lo > 0.0f && lo < 1.0f) c0 = lo;
+            if (hi > 0.0f && hi < 1.0f) {
+                if (c0 > 1.0f) c0 = hi;
+                else c1 = hi;
             }
-            atomicAdd(row + x1i, dd * am);
+        }
+        const float cuts[4] = {0.0f, c0 < 1.0f ? c0 : 1.0f, c1 < 1.0f ? c1 : 1.0f, 1.0f};
+        for (int q = 0; q < 3; q++) {
+            const float t0 = cuts[q], t1 = cuts[q + 1];
+            if (!(t1 > t0)) continue;
+            const float pa = fminf(fmaxf(xa + t0 * (xb - xa), 0.0f), W);
+            const float pb = fminf(fmaxf(xa + t1 * (xb - xa), 0.0f), W);
+            accum_piece(row, pa, pb, dd * (t1 - t0), d.qscale);
         }
     }
 }
 
-// Prefix sums along each row of the area planes (one wave per row), in place.
+// Prefix sums along each row of the area planes (one wave per row), in place: the
+// fixed-point differences are summed in wrapping integer arithmetic (exact, and
+// every prefix, a pixel's winding area, is within range) and written back as floats.
 __global__ void k_diff_prefix(Dims d, float *area) {
-    area = scene_of(area, d.ws);
+    area = scene_at(area, d.ws);
     const uint32_t row = blockIdx.x;  // over n * h
     const int lane = threadIdx.x;
     float *r = area + (size_t)row * (d.w + 2);
-    float carry = 0.0f;
+    const int *ri = (const int *)r;
+    const float inv = 1.0f / d.qscale;
+    uint32_t carry = 0;
     for (uint32_t x0 = 0; x0 < d.w; x0 += 64) {
         const uint32_t x = x0 + lane;
-        float v = x < d.w ? r[x] : 0.0f;
+        uint32_t v = x < d.w ? (uint32_t)ri[x] : 0u;
         for (int o = 1; o < 64; o <<= 1) {
-            const float t = __shfl_up(v, o, 64);
+            const uint32_t t = __shfl_up(v, o, 64);
             if (lane >= o) v += t;
         }
-        if (x < d.w) r[x] = carry + v;
+        if (x < d.w) r[x] = (float)(int)(carry + v) * inv;
         carry += __shfl(v, 63, 64);
     }
 }
@@ -636,9 +681,9 @@ __global__ void k_diff_prefix(Dims d, float *area) {
 // the edge; pixels outside the canvas are skipped.  Writes (dL/da, dL/db) per edge.
 __global__ void k_diff_boundary(Dims d, const float2 *__restrict__ vert, const float *__restrict__ gW,
                                 float4 *egrad) {
-    vert = scene_of(vert, d.ws);
-    gW = scene_of(gW, d.ws);
-    egrad = scene_of(egrad, d.ws);
+    vert = scene_at(vert, d.ws);
+    gW = scene_at(gW, d.ws);
+    egrad = scene_at(egrad, d.ws);
     const uint32_t nv = d.k * kFlat + 1;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= d.n * nv) return;
@@ -705,9 +750,9 @@ __global__ void k_diff_boundary(Dims d, const float2 *__restrict__ vert, const f
 // affine to the user-space points and the affine's six values.
 __global__ void k_diff_geom_chain(Dims d, const float *__restrict__ geom, const float4 *__restrict__ egrad,
                                   float *ggeom) {
-    geom = scene_of(geom, geom_bytes(d));
-    egrad = scene_of(egrad, d.ws);
-    ggeom = scene_of(ggeom, geom_bytes(d));
+    geom = scene_at(geom, geom_bytes(d));
+    egrad = scene_at(egrad, d.ws);
+    ggeom = scene_at(ggeom, geom_bytes(d));
     const uint32_t n = d.n, k = d.k;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -745,8 +790,8 @@ __global__ void k_diff_geom_chain(Dims d, const float *__restrict__ geom, const 
 // 16 row lanes, each lane summing every 16th row, then the lanes in a fixed order.
 __global__ __launch_bounds__(1024) void k_diff_reduce(const float *part, uint32_t rows, uint32_t cols, float *out,
                                                       size_t part_stride, size_t out_stride) {
-    part = scene_of(part, part_stride);
-    out = scene_of(out, out_stride);
+    part = scene_at(part, part_stride);
+    out = scene_at(out, out_stride);
     __shared__ float acc[16][65];
     const uint32_t tx = threadIdx.x % 64, ty = threadIdx.x / 64;
     const uint32_t c = blockIdx.x * 64 + tx;
@@ -792,6 +837,18 @@ Layout layout(uint32_t w, uint32_t h, uint32_t n, uint32_t k) {
     return L;
 }
 
+// Fixed-point scale of the area accumulation: a pixel's winding area is at most half
+// the polygon's edge count (k * kFlat + 1) in magnitude, so with q = 2^(30 - ceil(log2
+// of that + 1)) every prefix fits a 32-bit integer; at most 2^24 (a resolution of
+// 6e-8, f32's near 1).  8-cubic paths: 2^23.
+float area_qscale(uint32_t k) {
+    const uint64_t edges = (uint64_t)k * kFlat + 1;
+    int bits = 0;
+    while ((1ull << bits) < edges / 2 + 1) bits++;
+    const int sh = 30 - bits < 24 ? 30 - bits : 24;
+    return std::ldexp(1.0f, sh);
+}
+
 bool scene_ok(const rvk_diff_scene *s, const float *geom) {
     return s && s->width > 0 && s->height > 0 && s->path_count > 0 && s->dev_gradient_ends &&
            (s->cubics ? geom != nullptr && s->cubics <= 4096 : s->dev_coverage != nullptr) &&
@@ -818,7 +875,7 @@ rvk_status rvk_diff_forward_batch(rvk_context *ctx, const rvk_diff_scene *scene,
     const uint32_t w = scene->width, h = scene->height, n = scene->path_count, k = scene->cubics;
     const Layout L = layout(w, h, n, k);
     uint8_t *ws = (uint8_t *)dev_workspace;
-    const Dims d{w, h, n, (uint32_t)(((size_t)w * h + kThreads - 1) / kThreads), k, L.total};
+    const Dims d{w, h, n, (uint32_t)(((size_t)w * h + kThreads - 1) / kThreads), k, L.total, area_qscale(k)};
     float4 *S = (float4 *)(ws + L.S), *T = (float4 *)(ws + L.T), *B = (float4 *)(ws + L.B);
     float *wts = (float *)(ws + L.wts);
     const double px_bytes = (double)w * h * batch;
@@ -834,7 +891,7 @@ rvk_status rvk_diff_forward_batch(rvk_context *ctx, const rvk_diff_scene *scene,
             hipLaunchKernelGGL(k_diff_zero, dim3(gz, batch), dim3(256), 0, ctx->stream, area, cells, L.total);
             hipLaunchKernelGGL(k_diff_flatten, dim3(ge, batch), dim3(256), 0, ctx->stream, d, dev_geom, vert);
             hipLaunchKernelGGL(k_diff_accum, dim3(ge, batch), dim3(256), 0, ctx->stream, d, (const float2 *)vert,
-                               area);
+                               (int *)area);
             hipLaunchKernelGGL(k_diff_prefix, dim3(n * h, batch), dim3(64), 0, ctx->stream, d, area);
         });
         cv = Cov{nullptr, area, nullptr};
@@ -879,7 +936,7 @@ rvk_status rvk_diff_backward_batch(rvk_context *ctx, const rvk_diff_scene *scene
     uint8_t *ws = (uint8_t *)dev_workspace;
     Cov cv{scene->dev_coverage, nullptr, nullptr};
     if (k) cv = Cov{nullptr, (const float *)(ws + L.area), (float *)(ws + L.gW)};
-    const Dims d{w, h, n, (uint32_t)(((size_t)w * h + kThreads - 1) / kThreads), k, L.total};
+    const Dims d{w, h, n, (uint32_t)(((size_t)w * h + kThreads - 1) / kThreads), k, L.total, area_qscale(k)};
     const float4 *S = (const float4 *)(ws + L.S), *T = (const float4 *)(ws + L.T), *B = (const float4 *)(ws + L.B);
     float4 *gB = (float4 *)(ws + L.gB), *gS = (float4 *)(ws + L.gS), *gS2 = (float4 *)(ws + L.gS2);
     const float *wts = (const float *)(ws + L.wts);

@@ -151,6 +151,28 @@ def test_analytic_gradient_matches_fp64_finite_differences(ctx, seed):
     assert e < 1e-3, f"paint: relative error {e:.2e}"
 
 
+def test_analytic_paths_leaving_the_canvas(ctx):
+    """Paths shifted partly off the canvas (left and bottom for some, right and top
+    for others): the coverage is still the exact winding integral over each canvas
+    pixel, and the geometry gradient is its boundary term inside the canvas."""
+    import torch
+    r, params, geom, ends, w, h, k = _setup_geom(ctx, 2)
+    shift = torch.tensor([[-14.0, 11.0], [15.0, -12.0], [-9.0, -9.0], [12.0, 13.0], [0.0, 0.0]], device="cuda")
+    geom[:, 4:6] += shift
+    out = r.forward(params, geom).cpu().numpy()
+    gnp = geom.cpu().numpy()
+    ref = D.forward_geom(gnp, k, ends, params.cpu().numpy(), w, h)
+    assert np.abs(out - ref).max() < 1e-4
+    rng = np.random.default_rng(12)
+    go = rng.normal(0.0, 1.0, (h, w, 4)).astype(np.float32)
+    _, gg = r.backward(params, torch.as_tensor(go, device="cuda"), geom)
+    gg = gg.cpu().numpy().astype(np.float64).reshape(-1)
+    idx = sorted(rng.choice(gg.size, size=50, replace=False).tolist() + list(range(6)))
+    fd = D.grad_fd_geom(gnp, k, ends, params.cpu().numpy(), w, h, go, idx)
+    e = np.linalg.norm(gg[idx] - fd[idx]) / np.linalg.norm(fd[idx])
+    assert e < 1e-3, f"geometry: relative error {e:.2e}"
+
+
 def test_analytic_gradient_matches_f32_central_differences(ctx):
     import torch
     r, params, geom, ends, w, h, k = _setup_geom(ctx, 4, size=48, n=6, k=4, sigma=2.2)
@@ -230,3 +252,20 @@ def test_batch_fit_lowers_every_scene_loss(ctx):
     assert res["mse_ratio_end_over_start"] < 0.8, res
     assert res["psnr_f32_db"] > res["psnr_start_db"] + 1.0, res
     assert np.isfinite(res["psnr_u8_db"]) and res["psnr_u8_db"] > 5.0, res
+
+
+def test_analytic_mode_is_deterministic(ctx):
+    """The winding-area planes are accumulated in fixed point, so repeated forwards
+    and backwards of a busy scene (many edges per row) are bit-identical."""
+    import torch
+    w, h, paths, ends, p0, _ = diff_scene(seed=8, size=256, n_paths=48, segments=8, sigma=2.0)
+    r = diff.Renderer(ctx, w, h, paths, ends, cubics=8)
+    params = torch.as_tensor(p0, device="cuda")
+    go = torch.as_tensor(np.random.default_rng(8).normal(0.0, 1.0, (h, w, 4)).astype(np.float32), device="cuda")
+    runs = []
+    for _ in range(3):
+        out = r.forward(params, r.geom0).clone()
+        gp, gg = r.backward(params, go, r.geom0)
+        runs.append((out, gp.clone(), gg.clone()))
+    for out, gp, gg in runs[1:]:
+        assert torch.equal(out, runs[0][0]) and torch.equal(gp, runs[0][1]) and torch.equal(gg, runs[0][2])
