@@ -213,13 +213,15 @@ struct Exec {
                 return from_image(out, cs);
             }
             case RVK_FE_MORPHOLOGY: {  // mod.rs:935-956
-                Img out = take(into_cs(input(p.in1), cs));
+                Img in = into_cs(input(p.in1), cs);
+                Img out = new_img(ctx, in->w, in->h, cs, false);  // written whole; the kernel reads `in` only
+                out->region = in->region;
                 float rx, ry;
                 scale_coords(p.u.morphology.radius_x, p.u.morphology.radius_y, ts, rx, ry);
                 if (!(rx > 0.0f && ry > 0.0f))
                     FTRY(rvk::launch_fill_sub(ctx, out->px, out->w, out->h, 0, 0, out->w, out->h, 0u));
                 else
-                    FTRY(rvk::launch_morphology(ctx, out->px, out->w, out->h, p.u.morphology.op, rx, ry));
+                    FTRY(rvk::launch_morphology_to(ctx, in->px, out->px, out->w, out->h, p.u.morphology.op, rx, ry));
                 return from_image(out, cs);
             }
             case RVK_FE_DISPLACEMENT_MAP: {  // mod.rs:958-986

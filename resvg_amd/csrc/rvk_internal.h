@@ -113,6 +113,9 @@ rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
 rvk_status launch_iir_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy);
 rvk_status launch_morphology(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, uint32_t op, float rx,
                              float ry);
+// src -> dst, distinct buffers
+rvk_status launch_morphology_to(rvk_context *ctx, const uint8_t *src, uint8_t *dst, uint32_t w, uint32_t h,
+                                uint32_t op, float rx, float ry);
 rvk_status launch_convolve(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, const rvk_primitive *p,
                            const float *values);
 rvk_status launch_color_matrix(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, const rvk_primitive *p,

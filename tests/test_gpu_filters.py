@@ -99,7 +99,8 @@ def test_iir_blur(ctx, shape, sig):
 
 
 @pytest.mark.parametrize("shape", SHAPES + [(500, 333)])
-@pytest.mark.parametrize("r", [(1.0, 1.0), (2.5, 0.5), (8.0, 8.0), (8.0, 3.0), (20.0, 18.0)])
+@pytest.mark.parametrize("r", [(1.0, 1.0), (2.5, 0.5), (4.0, 4.0), (3.5, 7.5), (8.0, 8.0), (8.0, 3.0),
+                               (12.0, 16.0), (20.0, 18.0), (16.5, 2.0)])
 @pytest.mark.parametrize("op", [abi.MORPH_ERODE, abi.MORPH_DILATE])
 def test_morphology(ctx, shape, r, op):
     px = rand_premul(*shape, seed=4)
