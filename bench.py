@@ -12,7 +12,8 @@ building, uploads and every kernel are inside it.
 
 Prints one JSON line: Mpixel/s of the document, the roofline of the dominant kernel
 (algorithmic bytes / its HIP-event time over the timed steps, against 8 TB/s HBM),
-the oracle's CPU throughput on a bounded sample of the same document, and under
+the oracle's CPU throughput on a bounded sample of the same document (with the
+per-channel mismatch histogram of two seeded documents against the oracle), and under
 "batched" the documents/s of rvk_render_batch against a loop of rvk_render calls on a
 sample of the batched workload, under "differentiable" the f32 forward and
 forward+backward times of the differentiable form against the plain u8 forward, and
@@ -106,6 +107,32 @@ def cpu_baseline(size, paths, seed, workers, plain_stride=64, filter_group=1, n_
                   f"{t_plain:.1f} s over {parts} cores); estimated {t_doc:.0f} s per document per core; "
                   f"wall {wall:.1f} s",
     }
+
+
+def cpu_parity(ctx, seed):
+    """Part of the CPU-baseline leg: the per-channel mismatch histogram (|GPU - oracle|
+    = 0, 1, > 1) of two seeded documents rendered by rvk_render and by the oracle: the
+    512x512 blur -> offset -> composite document, and the full-filter document at
+    1024x1024 with 250 paths (the oracle renders it in about 10 s on one core)."""
+    import numpy as np
+    import torch
+    from oracle import bindings as orc
+    from resvg_amd import abi
+    from resvg_amd.workloads import blur_offset_composite_doc, full_filter_doc
+    out = []
+    for name, (sc, (w, h)) in (("blur_offset_composite 512x512", blur_offset_composite_doc(seed=seed, size=512)),
+                               ("full_filter 1024x1024, 250 paths", full_filter_doc(seed=seed, size=1024,
+                                                                                    n_paths=250))):
+        s = sc.to_abi()
+        ref = np.zeros((h, w, 4), np.uint8)
+        orc.render(s.abi, abi.Transform.identity(), ref)
+        px = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+        ctx.render(s, abi.Transform.identity(), px)
+        d = np.abs(px.cpu().numpy().astype(np.int16) - ref.astype(np.int16))
+        out.append({"document": name, "channels": int(d.size), "equal": int((d == 0).sum()),
+                    "off_by_1": int((d == 1).sum()), "off_by_more": int((d > 1).sum())})
+        print(f"[bench] parity {name}: {out[-1]}", file=sys.stderr, flush=True)
+    return out
 
 
 def batched(ctx, n_docs, seed, steps):
@@ -351,6 +378,8 @@ def main() -> int:
     t_wall = time.perf_counter() - t_wall
     kernels = ctx.profile_end()
     st = ctx.stats()  # of the last render call
+    if cpu is not None:
+        cpu["parity"] = cpu_parity(ctx, args.seed)
     bat = batched(ctx, args.batch_docs, args.seed, args.batch_steps) if args.batch_docs > 0 else None
     dif = differentiable(ctx, args.diff_size, args.diff_paths, args.seed, 5) if args.diff_size > 0 else None
     fitr = fitting(ctx, args.fit_scenes, args.fit_size, args.fit_paths, args.fit_steps, args.seed) \
