@@ -22,65 +22,107 @@ __device__ __forceinline__ uint32_t round_pass(uint32_t s, float iarr) {
     return (uint32_t)rintf((float)s * iarr);
 }
 
+// The same rounding, as the bits of a float: the product rounded to f32 (no fused
+// multiply-add), plus 1.5 * 2^23, whose unit in the last place is 1, so the sum is
+// rounded to the nearest integer (ties to even) and that integer, at most 255, is the
+// low byte of the sum's bits.
+__device__ __forceinline__ uint32_t round_pass_bits(uint32_t s, float iarr) {
+#pragma clang fp contract(off)
+    const float prod = (float)s * iarr;
+    return __float_as_uint(prod + 12582912.0f);
+}
+
 // Sums four u8 channels of packed pixels into four 16-bit lanes of two u32s.
 struct Acc {
     uint32_t rb, ga;  // (r | b << 16), (g | a << 16)
+    // byte selects (v_perm_b32; selector 0x0c gives a zero byte)
+    __device__ __forceinline__ static uint32_t lo(uint32_t p) { return __builtin_amdgcn_perm(0u, p, 0x0c020c00u); }
+    __device__ __forceinline__ static uint32_t hi(uint32_t p) { return __builtin_amdgcn_perm(0u, p, 0x0c030c01u); }
     __device__ __forceinline__ void zero() { rb = 0; ga = 0; }
-    __device__ __forceinline__ void add(uint32_t p) { rb += p & 0x00ff00ffu; ga += (p >> 8) & 0x00ff00ffu; }
-    __device__ __forceinline__ void sub(uint32_t p) { rb -= p & 0x00ff00ffu; ga -= (p >> 8) & 0x00ff00ffu; }
+    __device__ __forceinline__ void add(uint32_t p) { rb += lo(p); ga += hi(p); }
+    __device__ __forceinline__ void sub(uint32_t p) { rb -= lo(p); ga -= hi(p); }
     __device__ __forceinline__ uint32_t out(float iarr) const {
-        return pack(round_pass(rb & 0xffffu, iarr), round_pass(ga & 0xffffu, iarr), round_pass(rb >> 16, iarr),
-                    round_pass(ga >> 16, iarr));
+        const uint32_t r = round_pass_bits(rb & 0xffffu, iarr), g = round_pass_bits(ga & 0xffffu, iarr);
+        const uint32_t b = round_pass_bits(rb >> 16, iarr), a = round_pass_bits(ga >> 16, iarr);
+        return __builtin_amdgcn_perm(g, r, 0x0c0c0400u) | __builtin_amdgcn_perm(a, b, 0x04000c0cu);
     }
 };
 
 // One round: dst = H_rh(V_rv(src)).  Tile covers output [x0, x0+TW) x [y0, y0+TH).
-// LDS: in[(TH + 2 rv) rows][(TW + 2 rh) cols], mid[TH rows][TW + 2 rh cols].
+// LDS: in[(TH + 2 rv) rows][(TW + 2 rh) cols], mid[TH rows][(TW + 2 rh) | 1 cols].
 __global__ __launch_bounds__(kThreads) void k_box_round(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
                                                         uint32_t w, uint32_t h, int rh, int rv) {
     extern __shared__ uint32_t lds[];
     const int cw = TW + 2 * rh;   // staged columns
     const int ch_ = TH + 2 * rv;  // staged rows
+    const int ms = cw | 1;  // odd row stride of mid: the horizontal pass's lanes (rows) hit distinct banks
     uint32_t *in = lds;
     uint32_t *mid = lds + cw * ch_;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    // stage input tile with zero padding outside the image (fv = lv = 0)
-    for (int i = threadIdx.x; i < cw * ch_; i += kThreads) {
-        int ly = i / cw, lx = i - ly * cw;
-        int gx = x0 - rh + lx, gy = y0 - rv + ly;
-        uint32_t v = 0;
-        if (gx >= 0 && gx < (int)w && gy >= 0 && gy < (int)h) v = src[(size_t)gy * w + gx];
-        in[i] = v;
+    // stage the input tile with zero padding outside the image (fv = lv = 0), in chunks
+    // of 4 pixels aligned in memory: chunk c of a tile row holds global x in [a0 + 4c,
+    // a0 + 4c + 3]; 16-byte loads where the image and its rows are 16-byte aligned and
+    // the chunk lies inside the image, single pixels elsewhere
+    {
+        const int gx0 = x0 - rh, gy0 = y0 - rv;
+        const int a0 = gx0 & ~3;
+        const int nc = (gx0 + cw - a0 + 3) >> 2;
+        const float inv_nc = 1.0f / (float)nc;  // (i + 0.5) / nc truncates exactly for these sizes
+        const bool vec = (w & 3u) == 0 && (reinterpret_cast<uintptr_t>(src) & 15u) == 0;
+        for (int i = threadIdx.x; i < ch_ * nc; i += kThreads) {
+            const int ly = (int)(((float)i + 0.5f) * inv_nc), c = i - ly * nc;
+            const int gy = gy0 + ly, gx = a0 + 4 * c;
+            uint32_t v[4];
+            if (gy >= 0 && gy < (int)h && vec && gx >= 0 && gx + 3 < (int)w) {
+                const uint4 q = *reinterpret_cast<const uint4 *>(src + (size_t)gy * w + gx);
+                v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    v[e] = (gy >= 0 && gy < (int)h && gx + e >= 0 && gx + e < (int)w) ? src[(size_t)gy * w + gx + e] : 0u;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int lx = gx + e - gx0;
+                if (lx >= 0 && lx < cw) in[ly * cw + lx] = v[e];
+            }
+        }
     }
     __syncthreads();
     // vertical pass: each thread walks one staged column over TH/segs rows
     if (rv == 0) {
-        for (int i = threadIdx.x; i < cw * TH; i += kThreads) mid[i] = in[i];
+        for (int i = threadIdx.x; i < cw * TH; i += kThreads) {
+            const int ly = i / cw;
+            mid[ly * ms + (i - ly * cw)] = in[i];
+        }
     } else {
         const float iarr = 1.0f / (float)(2 * rv + 1);
-        const int segs = 4, seg_len = TH / segs;
+        // two segments of TH / 2 rows per column: every column's 2 jobs fit one pass of
+        // the block (cw <= 112), and the window sum is started (2 rv + 1 adds) only twice
+        const int segs = 2, seg_len = TH / segs;
         for (int job = threadIdx.x; job < cw * segs; job += kThreads) {
             int col = job % cw, seg = job / cw;
             int t0 = seg * seg_len;
             Acc acc; acc.zero();
             for (int k = 0; k <= 2 * rv; k++) acc.add(in[(t0 + k) * cw + col]);
-            mid[t0 * cw + col] = acc.out(iarr);
+            mid[t0 * ms + col] = acc.out(iarr);
             for (int t = t0 + 1; t < t0 + seg_len; t++) {
                 acc.add(in[(t + 2 * rv) * cw + col]);
                 acc.sub(in[(t - 1) * cw + col]);
-                mid[t * cw + col] = acc.out(iarr);
+                mid[t * ms + col] = acc.out(iarr);
             }
         }
     }
     __syncthreads();
     // horizontal pass: each thread walks a row segment of 8 outputs
-    const int segw = 8, nseg = TW / segw;
+    constexpr int segw = 8, nseg = TW / segw;
+    const bool vec_out = (w & 3u) == 0 && (reinterpret_cast<uintptr_t>(dst) & 15u) == 0;
     const float iarr = 1.0f / (float)(2 * rh + 1);
     for (int job = threadIdx.x; job < TH * nseg; job += kThreads) {
-        int row = job / nseg, seg = job % nseg;
+        int row = job % TH, seg = job / TH;
         int gy = y0 + row;
         if (gy >= (int)h) continue;
-        const uint32_t *m = mid + row * cw;
+        const uint32_t *m = mid + row * ms;
         int c0 = seg * segw;
         uint32_t outv[segw];
         if (rh == 0) {
@@ -97,8 +139,13 @@ __global__ __launch_bounds__(kThreads) void k_box_round(const uint32_t *__restri
         }
         int gx = x0 + c0;
         uint32_t *d = dst + (size_t)gy * w + gx;
-        for (int k = 0; k < segw; k++)
-            if (gx + k < (int)w) d[k] = outv[k];
+        if (vec_out && gx + segw <= (int)w) {  // 32 bytes, 16-byte aligned
+            reinterpret_cast<uint4 *>(d)[0] = make_uint4(outv[0], outv[1], outv[2], outv[3]);
+            reinterpret_cast<uint4 *>(d)[1] = make_uint4(outv[4], outv[5], outv[6], outv[7]);
+        } else {
+            for (int k = 0; k < segw; k++)
+                if (gx + k < (int)w) d[k] = outv[k];
+        }
     }
 }
 
@@ -190,7 +237,7 @@ rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
     for (int i = 0; i < 5 && st == RVK_OK; i++) {
         int rh = (bh[i] - 1) / 2, rv = (bv[i] - 1) / 2;
         if (rh <= kMaxFusedR && rv <= kMaxFusedR) {
-            size_t lds = (size_t)((TW + 2 * rh) * (TH + 2 * rv) + (TW + 2 * rh) * TH) * 4;
+            size_t lds = (size_t)((TW + 2 * rh) * (TH + 2 * rv) + ((TW + 2 * rh) | 1) * TH) * 4;
             dim3 grid(div_up(w, TW), div_up(h, TH));
             timed_launch(ctx, "box_blur_round", round_bytes, [&] {
                 hipLaunchKernelGGL(k_box_round, grid, dim3(kThreads), lds, ctx->stream, a, b, w, h, rh, rv);
