@@ -421,13 +421,22 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, Cov cv,
     for (int32_t sg = (int32_t)nseg - 1; sg >= 0; sg--) {
         const uint32_t i0 = (uint32_t)sg * kSeg;
         float4 canv[kSeg + 1];
+        float cov[kSeg], area[kSeg];  // the segment's coverage (and signed area) at this pixel, read once
         canv[0] = live ? ck[(size_t)sg * np + p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
         for (int j = 0; j < kSeg; j++) {
             canv[j + 1] = canv[j];
             const uint32_t i = i0 + j;
+            cov[j] = 0.0f;
+            area[j] = 0.0f;
             if (live && i < d.n) {
-                const float c = cov_at(cv, d, i, p);
+                if (cv.u8) {
+                    cov[j] = (float)cv.u8[(size_t)i * np + p] * (1.0f / 255.0f);
+                } else {
+                    area[j] = cv.area[area_index(d, i, p)];
+                    cov[j] = fminf(1.0f, fabsf(area[j]));
+                }
+                const float c = cov[j];
                 if (c != 0.0f) {
                     const float *pp = params + (size_t)i * kPP;
                     canv[j + 1] = over(pp, eval_path(pp, ends[i], c, px, py), canv[j]);
@@ -441,7 +450,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, Cov cv,
             float gp[32];  // kPP gradients, padded to 32 for the butterfly
 #pragma unroll
             for (int q = 0; q < 32; q++) gp[q] = 0.0f;
-            const float c = live ? cov_at(cv, d, i, p) : 0.0f;
+            const float c = cov[j];
             float gcov = 0.0f;  // dL/d(coverage)
             if (c != 0.0f) {
                 const float *pp = params + (size_t)i * kPP;
@@ -490,7 +499,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_paths_bwd(Dims d, Cov cv,
             }
             if (cv.gW && live) {
                 // coverage = min(1, |area|): d/d(area) = sign(area) inside (0, 1), else 0
-                const float ar = cv.area[area_index(d, i, p)];
+                const float ar = area[j];
                 const float da = fabsf(ar) < 1.0f ? (ar > 0.0f ? 1.0f : (ar < 0.0f ? -1.0f : 0.0f)) : 0.0f;
                 cv.gW[(size_t)i * np + p] = gcov * da;
             }
@@ -662,15 +671,29 @@ __global__ void k_diff_prefix(Dims d, float *area) {
     const int *ri = (const int *)r;
     const float inv = 1.0f / d.qscale;
     uint32_t carry = 0;
-    for (uint32_t x0 = 0; x0 < d.w; x0 += 64) {
-        const uint32_t x = x0 + lane;
-        uint32_t v = x < d.w ? (uint32_t)ri[x] : 0u;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(v, o, 64);
-            if (lane >= o) v += t;
+    // four 64-column chunks at a time: their loads are issued together and their
+    // scans are independent chains
+    for (uint32_t x0 = 0; x0 < d.w; x0 += 256) {
+        uint32_t v[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint32_t x = x0 + 64 * c + lane;
+            v[c] = x < d.w ? (uint32_t)ri[x] : 0u;
         }
-        if (x < d.w) r[x] = (float)(int)(carry + v) * inv;
-        carry += __shfl(v, 63, 64);
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const uint32_t t = __shfl_up(v[c], o, 64);
+                if (lane >= o) v[c] += t;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint32_t x = x0 + 64 * c + lane;
+            if (x < d.w) r[x] = (float)(int)(carry + v[c]) * inv;
+            carry += __shfl(v[c], 63, 64);
+        }
     }
 }
 
