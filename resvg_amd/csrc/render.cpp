@@ -1022,6 +1022,14 @@ rvk_status render_batch(rvk_context *ctx, uint32_t count, const rvk_scene *const
         all.reserve(ntasks);
         for (uint32_t d = d0; d < d1; d++)
             for (size_t k = 0; k < rs[d]->tasks.size(); k++) all.emplace_back(rs[d].get(), (uint32_t)k);
+        // longest first (the pool hands items out in order), so that the pass does not
+        // end on one long stroke: estimated cost = verbs, times 8 with a stroke outline
+        auto cost = [](const std::pair<Renderer *, uint32_t> &a) {
+            const auto &t = a.first->tasks[a.second];
+            const bool stroked = !t.fill_only && t.path->stroke.kind != RVK_PAINT_NONE;
+            return (uint64_t)t.path->verb_count * (stroked ? 8u : 1u);
+        };
+        std::stable_sort(all.begin(), all.end(), [&](const auto &a, const auto &b) { return cost(a) > cost(b); });
         const auto t0 = std::chrono::steady_clock::now();
         parallel_for(all.size(), [&](size_t i) {
             Renderer &r = *all[i].first;
