@@ -147,28 +147,42 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// Sums 32 values over the 64 lanes of a wave with a halving butterfly: at each
-// step a lane keeps half of its values (the upper half when its lane bit is set)
-// and adds the partner's copy of that half, so 16 + 8 + 4 + 2 + 1 + 1 exchanges
-// replace 32 full reductions.  Returns, in every lane, the wave sum of value
-// (lane & 31).
+// Sums 32 values over the 64 lanes of a wave with a halving butterfly: at each step
+// a lane keeps half of its values and adds a partner lane's copy of that half, so
+// 16 + 8 + 4 + 2 + 1 exchanges and one final one replace 32 full reductions.  The
+// exchanges stay in registers (no LDS): v_permlane16_swap pairs the 16-lane rows
+// (values i and 16 + i), then DPP row rotations by 1, 2, 4 and 8 pair, within each
+// row, the lanes with that lane bit clear with those with it set (a bijection that
+// keeps the lower lane bits, which already select the values a lane holds), and
+// v_permlane32_swap adds the two halves of the wave.  Returns, in every lane, the
+// wave sum of value (lane & 31).
 template <int N>
-__device__ __forceinline__ void halve(const float (&v)[2 * N], float (&o)[N], int lane) {
-    const bool up = (lane & N) != 0;
+__device__ __forceinline__ float row_ror(float x) {
+    return __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(x), 0x120 + N, 0xf, 0xf, false));
+}
+template <int B, int M>  // lane bit B in {1, 2, 4, 8}: M values in, M / 2 out
+__device__ __forceinline__ void halve_row(const float (&v)[M], float (&o)[M / 2], bool up) {
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-        const float keep = up ? v[N + i] : v[i], send = up ? v[i] : v[N + i];
-        o[i] = keep + __shfl_xor(send, N, 64);
+    for (int k = 0; k < M / 2; k++) {
+        const float lo = v[2 * k] + row_ror<B>(v[2 * k]);
+        const float hi = v[2 * k + 1] + row_ror<B>(v[2 * k + 1]);
+        o[k] = up ? hi : lo;
     }
 }
 __device__ __forceinline__ float wave_sum32(const float (&v)[32], int lane) {
     float a[16], b[8], c[4], d[2], e[1];
-    halve<16>(v, a, lane);
-    halve<8>(a, b, lane);
-    halve<4>(b, c, lane);
-    halve<2>(c, d, lane);
-    halve<1>(d, e, lane);
-    return e[0] + __shfl_xor(e[0], 32, 64);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        // rows 0 and 2 end with value i, rows 1 and 3 with value 16 + i
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[i]), __float_as_uint(v[16 + i]), false, false);
+        a[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    halve_row<1, 16>(a, b, (lane & 1) != 0);
+    halve_row<2, 8>(b, c, (lane & 2) != 0);
+    halve_row<4, 4>(c, d, (lane & 4) != 0);
+    halve_row<8, 2>(d, e, (lane & 8) != 0);
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(e[0]), __float_as_uint(e[0]), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 
 // ---- forward: paths -----------------------------------------------------------
