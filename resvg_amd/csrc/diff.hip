@@ -886,10 +886,14 @@ float area_qscale(uint32_t k) {
     return std::ldexp(1.0f, sh);
 }
 
+// The kernels index vertices (path_count x (16 cubics + 1)) and prefix rows
+// (path_count x height) with 32-bit grid and thread indices.
 bool scene_ok(const rvk_diff_scene *s, const float *geom) {
     return s && s->width > 0 && s->height > 0 && s->path_count > 0 && s->dev_gradient_ends &&
            (s->cubics ? geom != nullptr && s->cubics <= 4096 : s->dev_coverage != nullptr) &&
-           (uint64_t)s->width * s->height <= (1ull << 30);
+           (uint64_t)s->width * s->height <= (1ull << 30) &&
+           (uint64_t)s->path_count * ((uint64_t)s->cubics * kFlat + 1) < (1ull << 31) &&
+           (uint64_t)s->path_count * s->height < (1ull << 31);
 }
 
 }  // namespace

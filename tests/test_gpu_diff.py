@@ -269,3 +269,18 @@ def test_analytic_mode_is_deterministic(ctx):
         runs.append((out, gp.clone(), gg.clone()))
     for out, gp, gg in runs[1:]:
         assert torch.equal(out, runs[0][0]) and torch.equal(gp, runs[0][1]) and torch.equal(gg, runs[0][2])
+
+
+def test_scene_limits_are_rejected(ctx):
+    """Scenes whose vertex or prefix-row counts exceed the kernels' 32-bit indexing
+    are refused with RVK_ERR_INVALID_ARGUMENT before anything is launched."""
+    import ctypes as C
+    import torch
+    from resvg_amd import abi
+    L = diff._bind()
+    buf = torch.zeros(64, dtype=torch.float32, device="cuda")
+    p = C.c_void_p(buf.data_ptr())
+    for n, k, h in ((140000, 1000, 16), (3_000_000, 0, 1024)):
+        sc = diff.DiffScene(16, h, n, k, p if k == 0 else None, p)
+        st = L.rvk_diff_forward_batch(ctx._h, C.byref(sc), 1, p, p if k else None, p, p)
+        assert st == abi.ERR_INVALID_ARGUMENT, (n, k, h, st)
