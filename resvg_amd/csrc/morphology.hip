@@ -37,9 +37,9 @@ template <bool ERODE>
 __device__ __forceinline__ uint2 ident() { return ERODE ? make_uint2(0x00ff00ffu, 0x00ff00ffu) : make_uint2(0u, 0u); }
 
 // Fused tile kernel: MTW x MTH outputs per block (64 x 32: at 4096^2 faster than
-// 64 x 64 for every window, and than 64 x 16 from 8 x 8 up), the input tile and its halo in LDS
-// (u32 RGBA8), a horizontal pass into a second LDS buffer, then a vertical pass to
-// memory.  In each pass a thread produces MK consecutive outputs of one row (or
+// 64 x 64 for every window, and than 64 x 16 from 8 x 8 up), the input tile and its
+// halo in LDS (u32 RGBA8), a horizontal pass into a second LDS buffer, then a
+// vertical pass to memory.  In each pass a thread produces MK consecutive outputs of one row (or
 // column) from a sliding window of n values (van Herk / Gil-Werman within the
 // thread's run: out[j] = comb(suffix of v[j .. MK-2], core = v[MK-1 .. n-1], prefix
 // of v[n .. n+j-1]), about (n + 3 MK) / MK combines per output instead of n - 1.
