@@ -1,7 +1,7 @@
 // The differentiable f32 form of the pixel pipeline: a forward renderer and
-// hand-written adjoint kernels for the scene model below (the C4 form of the
-// batched/differentiable workloads).  Everything is f32, premultiplied RGBA, one
-// thread per pixel.
+// hand-written adjoint kernels for the scene model below, for one scene or a stack
+// of equally shaped scenes.  Everything is f32, premultiplied RGBA, one thread per
+// pixel.
 //
 // Model (the reference's operations restated in f32, resvg 0.48.1):
 //   * path i covers pixel p with c_i(p): either coverage_u8 / 255, the anti-aliased
