@@ -305,3 +305,29 @@ def test_random_filter_graphs(ctx, seed):
         assert diff.max() <= 1
     else:
         assert_exact(got, e, f"graph seed {seed}")
+
+
+@pytest.mark.parametrize("case", ["box_blur", "morphology_small", "morphology_large"])
+def test_filters_on_a_pixmap_not_16_byte_aligned(ctx, case):
+    # the fused box blur and morphology kernels take 16-byte loads and stores only
+    # where the pixmap and its rows are 16-byte aligned; a pixmap 4 bytes past such a
+    # boundary (rows 4-pixel aligned) takes the single-pixel paths with the same result
+    h, w = 70, 132
+    px = rand_premul(h, w, seed=9)
+    aligned = dev(px)
+    buf = torch.zeros(h * w * 4 + 16, dtype=torch.uint8, device="cuda")
+    shifted = buf[4:4 + h * w * 4].view(h, w, 4)
+    assert shifted.data_ptr() % 16 == 4 and aligned.data_ptr() % 16 == 0
+    shifted.copy_(aligned)
+    exp = px.copy()
+    if case == "box_blur":
+        for t in (aligned, shifted):
+            ctx.box_blur(t, 3.0, 3.0)
+        orc.box_blur(exp, 3.0, 3.0)
+    else:
+        r = 3.0 if case == "morphology_small" else 8.0
+        for t in (aligned, shifted):
+            ctx.morphology(t, abi.MORPH_ERODE, r, r)
+        orc.morphology(exp, abi.MORPH_ERODE, r, r)
+    assert_exact(host(shifted), host(aligned), f"{case}: shifted vs aligned")
+    assert_exact(host(aligned), exp, f"{case}: vs the oracle")
