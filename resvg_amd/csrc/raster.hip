@@ -568,8 +568,9 @@ constexpr int TX = 2 * kWaveSide, TY = 2 * kWaveSide;
 // mapped to a tile so that each XCD gets runs of kChunk horizontally adjacent
 // row-major tiles (which read the same coverage and pixel cache lines), the runs
 // interleaved over the draw rect so the XCDs share the work evenly; tiles past the
-// last whole round of runs keep their own index.
-constexpr uint32_t kXcds = 8, kChunk = 8;
+// last whole round of runs keep their own index.  Runs of 4 tiles (64 pixels)
+// measured fastest on the full-filter document (1, 2, 8, 16 and 32 all slower).
+constexpr uint32_t kXcds = 8, kChunk = 4;
 
 __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t h, int32_t ox, int32_t oy,
                                               uint32_t tiles_x, DrawArgs a) {
