@@ -560,9 +560,12 @@ struct DrawArgs {
 // One thread per target pixel; each 64-lane wave covers an 8 x 8 pixel square and a
 // block of 4 waves a 16 x 16 tile.  Items are culled per block and per wave; a wave
 // runs an item's shading only when one of its pixels is covered, and compact squares
-// cross path boundaries far less often than 64-pixel row segments do.
-constexpr int kWaveSide = 8;
-constexpr int TX = 2 * kWaveSide, TY = 2 * kWaveSide;
+// cross path boundaries far less often than 64-pixel row segments do (16 x 4 and
+// 4 x 16 waves, with fewer or more cache rows per access, measured no faster).
+constexpr int kWaveW = 8, kWaveH = 8;  // pixels of one 64-lane wave
+constexpr int TX = 16, TY = 16;         // pixels of one block (four waves)
+constexpr int kWavesX = TX / kWaveW;
+static_assert(kWaveW * kWaveH == 64 && (TX / kWaveW) * (TY / kWaveH) == 4, "four 64-pixel waves per tile");
 
 // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.  Block b is
 // mapped to a tile so that each XCD gets runs of kChunk horizontally adjacent
@@ -581,15 +584,15 @@ __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t
     const uint32_t bx = tile % tiles_x, by = tile / tiles_x;
     const int lane = (int)(threadIdx.x % 64), wv = (int)(threadIdx.x / 64);
     const int32_t tx0 = ox + (int32_t)(bx * TX), ty0 = oy + (int32_t)(by * TY);
-    const int32_t wx0 = tx0 + (wv & 1) * kWaveSide, wy0 = ty0 + (wv >> 1) * kWaveSide;
-    const int32_t x = wx0 + lane % kWaveSide, y = wy0 + lane / kWaveSide;
+    const int32_t wx0 = tx0 + (wv % kWavesX) * kWaveW, wy0 = ty0 + (wv / kWavesX) * kWaveH;
+    const int32_t x = wx0 + lane % kWaveW, y = wy0 + lane / kWaveW;
     const bool inside = x < (int32_t)w && y < (int32_t)h;
     uint32_t d = inside ? px[(size_t)y * w + x] : 0u;
     const uint32_t d0 = d;
     for (uint32_t it = 0; it < a.nitems; it++) {
         const DrawItem item = a.items[it];  // with its jobs' rects and coverage offsets
         const bool hit = item.x1 > tx0 && item.x0 < tx0 + TX && item.y1 > ty0 && item.y0 < ty0 + TY &&
-                         item.x1 > wx0 && item.x0 < wx0 + kWaveSide && item.y1 > wy0 && item.y0 < wy0 + kWaveSide;
+                         item.x1 > wx0 && item.x0 < wx0 + kWaveW && item.y1 > wy0 && item.y0 < wy0 + kWaveH;
         if (hit && inside && x >= item.x0 && x < item.x1 && y >= item.y0 && y < item.y1) {
             const int32_t lx = x - item.lx, ly = y - item.ly;
             const uint32_t cf = ref_cov(item.fj, a.arena, lx, ly);
