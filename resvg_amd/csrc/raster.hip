@@ -41,7 +41,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kMaxKeys = (int)rvkr::kCoverageKeys[2];  // crossings per sub-scanline of the widest variant
-constexpr int kWin = 1024;        // pixels accumulated per window
+constexpr int kWin = 512;         // pixels accumulated per window (256, 1024, 2048: slower)
 constexpr int kMaxSub = 4;        // sub-scanlines per pixel row
 
 // ---------------------------------------------------------------------------
