@@ -266,6 +266,19 @@ def test_many_small_flushes(ctx, monkeypatch):
     _assert_same(got, _oracle(sc, ts, w, h), "many flushes")
 
 
+def test_flush_budget_does_not_change_pixels(ctx, monkeypatch):
+    # a large document drawn with the default flush budget (one or two flushes) and
+    # with a budget of 32 MiB (many flushes): the same pixels
+    sc, (w, h) = full_filter_doc(seed=5, size=2048, n_paths=300)
+    ts = abi.Transform.identity()
+    big = _gpu(ctx, sc, ts, w, h)
+    few = ctx.stats().flushes
+    monkeypatch.setenv("RVK_FLUSH_BYTES", str(32 << 20))
+    small = _gpu(ctx, sc, ts, w, h)
+    assert ctx.stats().flushes > few
+    _assert_same(small, big, "flush budget")
+
+
 def test_render_host_matches_device(ctx):
     import resvg_amd
     sc = _random_scene(400, 150, 10)
