@@ -3,6 +3,9 @@
 #include "oracle.h"
 #include "common.h"
 #include "geom.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -280,6 +283,36 @@ ImgP apply_primitive(Ctx &c, const rvk_primitive &p) {
                 orc_lighting(in->px.data(), out->px.data(), out->w, out->h, p.kind == RVK_FE_SPECULAR_LIGHTING,
                              p.u.lighting.surface_scale, p.u.lighting.constant, p.u.lighting.specular_exponent,
                              p.u.lighting.color, &l);
+            return out;
+        }
+        case RVK_FE_TILE: {  // apply_tile (mod.rs:837-859)
+            auto in = get_input(c, p.in1);
+            // subregion = input.region translated by -region.xy; copy_region is
+            // tiny-skia's clone_rect, which cuts the rect to the pixmap
+            const int64_t sx = (int64_t)in->region.x - region.x, sy = (int64_t)in->region.y - region.y;
+            const int64_t l = std::max<int64_t>(sx, 0), t = std::max<int64_t>(sy, 0);
+            const int64_t r = std::min<int64_t>(sx + in->region.w, in->w);
+            const int64_t b = std::min<int64_t>(sy + in->region.h, in->h);
+            if (r <= l || b <= t) throw FilterError{RVK_ERR_INVALID_REGION};
+            const int64_t tw = r - l, th = b - t;
+            std::vector<uint8_t> tile((size_t)tw * th * 4);
+            for (int64_t y = 0; y < th; y++)
+                std::memcpy(&tile[(size_t)(y * tw) * 4], &in->px[((size_t)(t + y) * in->w + l) * 4], (size_t)tw * 4);
+            // fill_rect of the whole region with Pattern(tile, Repeat, translate(sx, sy)):
+            // the pixel centre in tile space, wrapped by the repeat tiling
+            // (v - floor(v / n) n), then sampled nearest (an integer translation
+            // reduces every filter quality to nearest)
+            auto out = std::const_pointer_cast<Img>(make_img(region.w, region.h, RVK_CI_SRGB));
+            auto wrap = [](float v, float n) { return v - std::floor(v / n) * n; };
+            for (uint32_t y = 0; y < out->h; y++) {
+                const float fy = wrap((float)y + 0.5f - (float)sy, (float)th);
+                const int64_t iy = std::min<int64_t>(std::max<int64_t>((int64_t)fy, 0), th - 1);
+                for (uint32_t x = 0; x < out->w; x++) {
+                    const float fx = wrap((float)x + 0.5f - (float)sx, (float)tw);
+                    const int64_t ix = std::min<int64_t>(std::max<int64_t>((int64_t)fx, 0), tw - 1);
+                    std::memcpy(&out->px[((size_t)y * out->w + x) * 4], &tile[(size_t)(iy * tw + ix) * 4], 4);
+                }
+            }
             return out;
         }
         default:

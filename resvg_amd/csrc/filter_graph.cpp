@@ -261,8 +261,25 @@ struct Exec {
                                               p.u.lighting.color, &l));
                 return out;
             }
+            case RVK_FE_TILE: {  // apply_tile (mod.rs:837-859): the input as is, tagged sRGB
+                Img in = input(p.in1);
+                // the input's region relative to the filter region, cut to the input
+                // pixmap (copy_region = clone_rect, which intersects)
+                const int64_t sx = (int64_t)in->region.x - region.x, sy = (int64_t)in->region.y - region.y;
+                const int64_t x0 = std::max<int64_t>(sx, 0), y0 = std::max<int64_t>(sy, 0);
+                const int64_t x1 = std::min<int64_t>(sx + in->region.w, in->w);
+                const int64_t y1 = std::min<int64_t>(sy + in->region.h, in->h);
+                if (x1 <= x0 || y1 <= y0) throw FilterError{RVK_ERR_INVALID_REGION};
+                // Pattern(Repeat) translated by (sx, sy): pixel x reads tile column
+                // (x - sx) mod tw, i.e. (x + phase) mod tw
+                auto phase = [](int64_t s, int64_t t) { int64_t r = (-s) % t; return (uint32_t)(r < 0 ? r + t : r); };
+                Img out = new_img(ctx, region.w, region.h, RVK_CI_SRGB, false);
+                FTRY(rvk::launch_tile(ctx, out->px, out->w, out->h, in->px, in->w, in->h, (uint32_t)x0, (uint32_t)y0,
+                                      (uint32_t)(x1 - x0), (uint32_t)(y1 - y0), phase(sx, x1 - x0),
+                                      phase(sy, y1 - y0)));
+                return out;
+            }
             case RVK_FE_IMAGE:
-            case RVK_FE_TILE:
                 throw FilterError{RVK_ERR_UNSUPPORTED};
             default:
                 throw FilterError{RVK_ERR_INVALID_ARGUMENT};

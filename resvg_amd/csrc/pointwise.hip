@@ -176,6 +176,19 @@ __global__ void k_fill_rect(uint32_t *px, uint32_t w, uint32_t x0, uint32_t y0, 
     }
 }
 
+// apply_tile's fill (filter/mod.rs:837-859): a Repeat pattern under an integer
+// translation samples nearest, so pixel (x, y) takes the tile pixel at
+// ((x + ox) mod tw, (y + oy) mod th); (ox, oy) is the tile phase, (tx0, ty0) the
+// tile's corner in the source.  Columns are fixed per thread, rows strided.
+__global__ void k_tile(uint32_t *dst, uint32_t w, uint32_t h, const uint32_t *src, uint32_t sw, uint32_t tx0,
+                       uint32_t ty0, uint32_t tw, uint32_t th, uint32_t ox, uint32_t oy) {
+    const uint32_t x = blockIdx.x * 64 + threadIdx.x;
+    if (x >= w) return;
+    const uint32_t col = tx0 + (x + ox) % tw;
+    for (uint32_t y = blockIdx.y * 4 + threadIdx.y; y < h; y += gridDim.y * 4)
+        dst[(size_t)y * w + x] = src[(size_t)(ty0 + (y + oy) % th) * sw + col];
+}
+
 __global__ void k_copy(uint4 *dst, const uint4 *src, size_t n16) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -408,6 +421,20 @@ rvk_status launch_fill_sub(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
     timed_launch(ctx, "fill_rect", 4.0 * (double)n, [&] {  // writes only
         hipLaunchKernelGGL(k_fill_rect, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px, w,
                            (uint32_t)x0, (uint32_t)y0, (uint32_t)(x1 - x0), (uint32_t)(y1 - y0), value);
+    });
+    return post_launch(ctx);
+}
+
+rvk_status launch_tile(rvk_context *ctx, uint8_t *dst, uint32_t w, uint32_t h, const uint8_t *src, uint32_t sw,
+                       uint32_t sh, uint32_t tx0, uint32_t ty0, uint32_t tw, uint32_t th, uint32_t ox, uint32_t oy) {
+    if (w == 0 || h == 0) return RVK_OK;
+    if (!dst || !src || tw == 0 || th == 0 || (uint64_t)tx0 + tw > sw || (uint64_t)ty0 + th > sh || ox >= tw ||
+        oy >= th || w > (1u << 30) || h > (1u << 30))
+        return RVK_ERR_INVALID_ARGUMENT;
+    const uint32_t gy = std::min<uint32_t>((h + 3) / 4, 65535u);
+    timed_launch(ctx, "tile", 8.0 * (double)w * h, [&] {
+        hipLaunchKernelGGL(k_tile, dim3((w + 63) / 64, gy), dim3(64, 4), 0, ctx->stream, (uint32_t *)dst, w, h,
+                           (const uint32_t *)src, sw, tx0, ty0, tw, th, ox, oy);
     });
     return post_launch(ctx);
 }

@@ -139,6 +139,10 @@ rvk_status launch_fill(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, co
 rvk_status launch_fill_sub(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, int64_t x0, int64_t y0,
                            int64_t x1, int64_t y1, uint32_t value);
 rvk_status launch_copy(rvk_context *ctx, uint8_t *dst, const uint8_t *src, size_t bytes);
+// Fills a w x h image with the periodic repeat of the tw x th tile at (tx0, ty0)
+// of a sw x sh source, phase (ox, oy): feTile.
+rvk_status launch_tile(rvk_context *ctx, uint8_t *dst, uint32_t w, uint32_t h, const uint8_t *src, uint32_t sw,
+                       uint32_t sh, uint32_t tx0, uint32_t ty0, uint32_t tw, uint32_t th, uint32_t ox, uint32_t oy);
 rvk_status launch_zero_rgb(rvk_context *ctx, uint8_t *px, size_t n);
 // px = div255(px * m) per channel, m from `src` (0 alpha, 1 inverted alpha, 2 luminance)
 rvk_status launch_apply_mask(rvk_context *ctx, uint8_t *px, const uint8_t *src, size_t n, uint32_t mode);

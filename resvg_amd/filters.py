@@ -72,6 +72,11 @@ class FilterGraph:
         p.u.flood.opacity = opacity
         return r
 
+    def tile(self, **kw) -> int:
+        """feTile: repeats ``in1``'s subregion over the filter region."""
+        r, _ = self.add(abi.FE_TILE, **kw)
+        return r
+
     def morphology(self, op: int, rx: float, ry: float | None = None, **kw) -> int:
         r, p = self.add(abi.FE_MORPHOLOGY, **kw)
         p.u.morphology.op = op

@@ -600,5 +600,11 @@ def all_scene_fixtures():
     return out
 
 
+def fixture_guides(name: str) -> tuple:
+    """User-space rects that a fixture SVG strokes over its filtered element (left
+    out of the PNG comparison; only the filter fixtures have them)."""
+    return {fx.name: fx.guides for fx in filter_fixtures()}.get(name, ())
+
+
 def render_ts():
     return abi.Transform(SCALE, 0.0, 0.0, SCALE, 0.0, 0.0)

@@ -231,7 +231,7 @@ FIX = fixtures()
 def test_fixture_graph_gpu_vs_oracle_and_png(ctx, fx):
     got = render_filtered_rect_gpu(ctx, fx)
     assert_exact(got, render_filtered_rect_oracle(fx), fx.name)
-    assert compare_png(got, load_png(fx.name)) == 0
+    assert compare_png(got, load_png(fx.name), guides=fx.guides) == 0
 
 
 def _random_graph(rng, region):
@@ -331,3 +331,39 @@ def test_filters_on_a_pixmap_not_16_byte_aligned(ctx, case):
         orc.morphology(exp, abi.MORPH_ERODE, r, r)
     assert_exact(host(shifted), host(aligned), f"{case}: shifted vs aligned")
     assert_exact(host(aligned), exp, f"{case}: vs the oracle")
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_tile_graphs(ctx, seed):
+    """feTile of a subregioned input (inside, across or outside the layer; from a
+    blur, a flood moved by feOffset, or the whole SourceGraphic), repeated over the
+    filter region or a subregion of its own, then read by one more primitive."""
+    rng = np.random.default_rng(700 + seed)
+    h, w = int(rng.integers(24, 96)), int(rng.integers(24, 96))
+    layer = smooth_premul(h, w, seed)
+    region = (float(rng.uniform(-6, 4)), float(rng.uniform(-6, 4)), float(w * rng.uniform(0.7, 1.3)),
+              float(h * rng.uniform(0.7, 1.3)))
+    x, y, rw, rh = region
+    sub = (x + float(rng.uniform(-0.3, 0.8)) * rw, y + float(rng.uniform(-0.3, 0.8)) * rh,
+           float(rng.uniform(2, 0.5 * rw)), float(rng.uniform(2, 0.5 * rh)))
+    cs = int(rng.integers(2))
+    g = FilterGraph(region)
+    kind = seed % 3
+    if kind == 0:
+        src = g.gaussian_blur(1.5, in1=SOURCE_GRAPHIC, rect=sub, cs=cs)
+    elif kind == 1:
+        f = g.flood((200, 40, 90), 0.8, rect=sub, cs=cs)
+        src = g.offset(float(rng.uniform(-6, 6)), float(rng.uniform(-6, 6)), in1=f, cs=cs)
+    else:
+        src = SOURCE_GRAPHIC
+    tile_rect = None if rng.random() < 0.5 else (x + 0.1 * rw, y + 0.15 * rh, 0.75 * rw, 0.6 * rh)
+    t = g.tile(in1=src, rect=tile_rect, cs=cs)
+    g.composite(abi.COMPOSITE_OVER, in1=t, in2=SOURCE_GRAPHIC, cs=1 - cs)
+    ts = abi.Transform(float(rng.uniform(0.7, 1.6)), 0.0, 0.0, float(rng.uniform(0.7, 1.6)),
+                       float(rng.uniform(-3, 3)), float(rng.uniform(-3, 3)))
+    tt = dev(layer)
+    err_g = ctx.filter_apply(g, ts, tt)
+    e = layer.copy()
+    err_o = orc.filter_apply(g, ts, e)
+    assert err_g == err_o
+    assert_exact(host(tt), e, f"tile seed {seed} kind {kind}")

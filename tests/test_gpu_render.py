@@ -10,7 +10,7 @@ from resvg_amd.scene import PaintSpec, Scene, ellipse_path, rect_path
 from resvg_amd.workloads import blur_offset_composite_doc, document_batch, full_filter_doc
 
 from fixture_scenes import compare_png, load_png
-from scene_fixtures import all_scene_fixtures, render_ts
+from scene_fixtures import all_scene_fixtures, fixture_guides, render_ts
 
 pytestmark = pytest.mark.gpu
 
@@ -45,7 +45,7 @@ def _assert_same(got, exp, what):
 def test_scene_fixture(ctx, name):
     got = _gpu(ctx, SCENES[name], render_ts(), 300, 300)
     _assert_same(got, _oracle(SCENES[name], render_ts(), 300, 300), name)
-    assert compare_png(got, load_png(name)) == 0
+    assert compare_png(got, load_png(name), guides=fixture_guides(name)) == 0
 
 
 # ---------------------------------------------------------------------------

@@ -29,7 +29,7 @@ def rand_premul(h, w, seed):
 @pytest.mark.parametrize("fx", FIXTURES, ids=[f.name for f in FIXTURES])
 def test_reference_fixture_png(fx):
     out = render_filtered_rect_oracle(fx)
-    assert compare_png(out, load_png(fx.name)) == 0
+    assert compare_png(out, load_png(fx.name), guides=fx.guides) == 0
 
 
 def test_colorspace_tables_match_srgb_formula():
