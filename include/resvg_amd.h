@@ -38,7 +38,7 @@ typedef int32_t rvk_status;
 #define RVK_ERR_OUT_OF_MEMORY    2  /* device allocation failed */
 #define RVK_ERR_HIP              3  /* a HIP runtime call failed */
 #define RVK_ERR_INVALID_REGION   4  /* filter::Error::InvalidRegion (filter/mod.rs:86-90) */
-#define RVK_ERR_UNSUPPORTED      5  /* feature outside this library's scope (feImage, feTile, patterns, images) */
+#define RVK_ERR_UNSUPPORTED      5  /* feature outside this library's scope (feImage, patterns, images) */
 #define RVK_ERR_NO_DEVICE        6  /* no HIP device / library built without a device */
 
 /* Returns a static string naming a status code. */
