@@ -590,6 +590,33 @@ def filter_scene(fx) -> Scene:
     return sc
 
 
+def _tile_subregion_fixtures():
+    """filters/feTile/with-subregion-1/-2/-3: two filtered rects (20, 20, 160, 160),
+    each filter a 10x10 flood moved by feOffset (5, 5) and tiled; the first in grey
+    over the whole filter region, the second in seagreen over the subregion (50, 50,
+    100, 100).  In -3 the flood's x = y = 2% of the 200-unit viewport, i.e. 4."""
+    def tiled(color, at, tile_rect=None):
+        g = FilterGraph(tuple(float(np.float32(v)) for v in (4.0, 4.0, 192.0, 192.0)))
+        f = g.flood(color, rect=(at, at, 10.0, 10.0), cs=abi.CI_SRGB)
+        o = g.offset(5.0, 5.0, in1=f, cs=abi.CI_SRGB)
+        g.tile(in1=o, rect=tile_rect, cs=abi.CI_SRGB)
+        return g
+
+    out = {}
+    for name, at in (("with-subregion-1", 4.0), ("with-subregion-2", 20.0), ("with-subregion-3", 4.0)):
+        sc = Scene(200, 200)
+        for graph in (tiled((128, 128, 128), 4.0 if name != "with-subregion-2" else 20.0),
+                      tiled((46, 139, 87), at, (50.0, 50.0, 100.0, 100.0))):
+            g = sc.group(sc.root, filters=[graph])
+            v, p = rect_path(20.0, 20.0, 160.0, 160.0)
+            sc.path(g, v, p, fill=PaintSpec.solid((0, 0, 0)))
+        out[f"feTile/{name}"] = sc
+    return out
+
+
+TILE_SUBREGION_GUIDES = ((50.0, 50.0, 100.0, 100.0),)
+
+
 def all_scene_fixtures():
     out = dict(shape_fixtures())
     out.update(_hairline_fixtures())
@@ -597,12 +624,15 @@ def all_scene_fixtures():
     out.update(_dash_fixtures())
     for fx in filter_fixtures():
         out[fx.name] = filter_scene(fx)
+    out.update(_tile_subregion_fixtures())
     return out
 
 
 def fixture_guides(name: str) -> tuple:
     """User-space rects that a fixture SVG strokes over its filtered element (left
     out of the PNG comparison; only the filter fixtures have them)."""
+    if name.startswith("feTile/with-subregion"):
+        return TILE_SUBREGION_GUIDES
     return {fx.name: fx.guides for fx in filter_fixtures()}.get(name, ())
 
 
