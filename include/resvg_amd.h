@@ -15,9 +15,12 @@
  *   resvg::render_node                crates/resvg/src/lib.rs:55-70
  *   filter::apply                     crates/resvg/src/filter/mod.rs:335-355
  *   per-primitive filter functions    crates/resvg/src/filter/<primitive>.rs (cited per entry)
- *   tiny-skia 0.12 pixel calls        fill_path / stroke_path / draw_pixmap / fill
+ *   tiny-skia 0.12 pixel calls        fill_path / stroke_path / draw_pixmap / fill /
+ *                                     fill_rect / Mask::fill_path / Mask::from_pixmap /
+ *                                     Mask::invert / apply_mask
  *                                     (call sites: crates/resvg/src/path.rs:73,113,
- *                                      render.rs:242-249, filter/mod.rs:624-1111)
+ *                                      render.rs:127-140, clip.rs:11-27, mask.rs:13-45,
+ *                                      filter/mod.rs:457-508, 624-1111)
  */
 #ifndef RESVG_AMD_H
 #define RESVG_AMD_H
@@ -74,7 +77,7 @@ enum {
 };
 
 /* usvg::BlendMode (crates/usvg/src/tree/mod.rs:216-233); mapped to tiny-skia
- * modes by convert_blend_mode (crates/resvg/src/render.rs:254-273). */
+ * modes by convert_blend_mode (crates/resvg/src/render.rs:145-164). */
 enum {
     RVK_MIX_NORMAL = 0, RVK_MIX_MULTIPLY, RVK_MIX_SCREEN, RVK_MIX_OVERLAY, RVK_MIX_DARKEN,
     RVK_MIX_LIGHTEN, RVK_MIX_COLOR_DODGE, RVK_MIX_COLOR_BURN, RVK_MIX_HARD_LIGHT,
@@ -330,7 +333,7 @@ rvk_status rvk_displacement_map(rvk_context *ctx, const uint8_t *dev_src, const 
                                 float scale, uint32_t x_channel, uint32_t y_channel);
 
 /* tiny_skia::PixmapMut::draw_pixmap(x, y, src, PixmapPaint{opacity, blend_mode, Nearest},
- * identity) — the call made by render.rs:242-249 and filter/mod.rs:624-1111. */
+ * identity) — the call made by render.rs:127-140 and filter/mod.rs:624-1111. */
 rvk_status rvk_draw_pixmap(rvk_context *ctx, uint8_t *dev_dst, uint32_t dw, uint32_t dh,
                            int32_t x, int32_t y, const uint8_t *dev_src, uint32_t sw, uint32_t sh,
                            float opacity, uint32_t blend_mode);
@@ -356,14 +359,91 @@ rvk_status rvk_render(rvk_context *ctx, const rvk_scene *scene, rvk_transform tr
  * renders, downloads and synchronizes. */
 rvk_status rvk_render_host(rvk_context *ctx, const rvk_scene *scene, rvk_transform transform,
                            uint32_t width, uint32_t height, uint8_t *host_pixmap);
+/* resvg::render_node (lib.rs:55-70) / resvg_render_node (crates/c-api/lib.rs:907-942)
+ * onto a device pixmap.  `node` is the group or path to draw (the FFI crate resolves
+ * usvg's node id to it); `abs_layer_bbox` is usvg's Node::abs_layer_bounding_box()
+ * (NULL, or a zero, negative or non-finite width or height, when the node has none).
+ * The node is drawn with `transform` pre-translated by (-bbox.x, -bbox.y), so the
+ * node's layer bounding box lands at the pixmap origin; a group node applies its own
+ * transform, opacity, blend mode, filters, clip path and mask (render.rs:21-47).
+ * *rendered (required) receives 1 when the node was drawn and 0 when it has no bounding
+ * box (resvg_render_node's `false`), in which case nothing is drawn and RVK_OK is
+ * returned. */
+rvk_status rvk_render_node(rvk_context *ctx, const rvk_scene *scene, rvk_node_ref node,
+                           const rvk_rect *abs_layer_bbox, rvk_transform transform,
+                           uint32_t width, uint32_t height, uint8_t *dev_pixmap, int32_t *rendered);
+
+/* ------------------------------------------------------------------------- */
+/* Per-operation replacements of the tiny-skia calls the render walk makes     */
+/* (for a caller that keeps resvg's own walk and swaps only the pixel calls)  */
+/* ------------------------------------------------------------------------- */
+/* A tiny_skia_path::Path: verbs (RVK_VERB_*) and their points (x, y pairs, f32). */
+typedef struct rvk_path_geometry {
+    const uint8_t *verbs;   uint32_t verb_count;
+    const float *points;    uint32_t point_count;   /* in points, not floats */
+} rvk_path_geometry;
+
+/* tiny_skia::Stroke as usvg::Stroke::to_tiny_skia builds it (crates/usvg/src/tree/mod.rs:638-662). */
+typedef struct rvk_stroke_style {
+    float width, miter_limit;
+    uint32_t cap, join;                    /* RVK_CAP_*, RVK_JOIN_* */
+    const float *dashes;                   /* resolved dash array (even length, sum > 0), or NULL */
+    uint32_t dash_count;
+    float dash_offset;
+} rvk_stroke_style;
+
+/* resvg path::fill_path (path.rs:26-75): tiny_skia Pixmap::fill_path(path, paint, rule,
+ * transform, None) with the paint's shader built from `paint` (solid colour, or a
+ * linear / radial gradient whose stops are paint->stop_offset .. + stop_count of
+ * `stops`; the paint's opacity folded in as path.rs does), `blend_mode`
+ * RVK_BLEND_SOURCE_OVER (path::render) or RVK_BLEND_CLEAR (clip path children,
+ * clip.rs:30-70), anti-aliasing `anti_alias`.  A path whose bounds have zero width or
+ * height draws nothing (path.rs:36-38).  Other blend modes return RVK_ERR_UNSUPPORTED. */
+rvk_status rvk_fill_path(rvk_context *ctx, uint8_t *dev_px, uint32_t width, uint32_t height,
+                         const rvk_path_geometry *path, uint32_t fill_rule, const rvk_paint *paint,
+                         const rvk_stop *stops, rvk_transform transform, uint32_t blend_mode,
+                         uint32_t anti_alias);
+/* resvg path::stroke_path (path.rs:77-116): tiny_skia Pixmap::stroke_path(path, paint,
+ * stroke, transform, None), SourceOver; a stroke at most one device pixel wide is
+ * drawn as tiny-skia's anti-aliased hairline. */
+rvk_status rvk_stroke_path(rvk_context *ctx, uint8_t *dev_px, uint32_t width, uint32_t height,
+                           const rvk_path_geometry *path, const rvk_stroke_style *stroke,
+                           const rvk_paint *paint, const rvk_stop *stops, rvk_transform transform,
+                           uint32_t anti_alias);
+
+/* Masks: a u8 plane of width * height bytes, row-major (tiny_skia::Mask).  The mask
+ * entry points take 4-byte aligned masks and 16-byte aligned pixmaps (device
+ * allocations and torch tensors are) and return RVK_ERR_INVALID_ARGUMENT otherwise. */
+enum { RVK_MASK_TYPE_ALPHA = 0, RVK_MASK_TYPE_LUMINANCE = 1 };
+/* tiny_skia::Mask::fill_path(path, rule, anti_alias, transform) on a new (zeroed) mask
+ * (mask.rs:20-30): writes the path's coverage into dev_mask. */
+rvk_status rvk_mask_fill_path(rvk_context *ctx, uint8_t *dev_mask, uint32_t width, uint32_t height,
+                              const rvk_path_geometry *path, uint32_t fill_rule, uint32_t anti_alias,
+                              rvk_transform transform);
+/* tiny_skia::Mask::from_pixmap(pixmap, MaskType) (clip.rs:25, mask.rs:44): alpha, or the
+ * luminance of the demultiplied colour (Rec. 709 weights) times alpha, rounded up. */
+rvk_status rvk_mask_from_pixmap(rvk_context *ctx, const uint8_t *dev_px, uint32_t width, uint32_t height,
+                                uint32_t mask_type, uint8_t *dev_mask);
+/* tiny_skia::Mask::invert (clip.rs:26): m = 255 - m. */
+rvk_status rvk_mask_invert(rvk_context *ctx, uint8_t *dev_mask, uint32_t width, uint32_t height);
+/* tiny_skia::Pixmap::apply_mask (clip.rs:27, mask.rs:32,45): every channel c becomes
+ * (c * m + 255) >> 8. */
+rvk_status rvk_apply_mask(rvk_context *ctx, uint8_t *dev_px, uint32_t width, uint32_t height,
+                          const uint8_t *dev_mask);
+/* tiny_skia::Pixmap::fill_rect(rect, Paint{BlendMode::Clear}, identity, None) on a
+ * pixel-aligned rect (filter/mod.rs:457-508, the subregion crop): zeroes
+ * [x, x + rect_w) x [y, y + rect_h) intersected with the pixmap. */
+rvk_status rvk_clear_rect(rvk_context *ctx, uint8_t *dev_px, uint32_t width, uint32_t height,
+                          int32_t x, int32_t y, uint32_t rect_w, uint32_t rect_h);
 /* A loop of resvg::render calls (lib.rs:34-43), one per document, in one call: document i
- * is composited onto dev_pixmaps[i] (widths[i] x heights[i]) with transforms[i].  The
- * documents' kernels run back to back on the context stream with one wait at the end.
+ * is composited onto dev_pixmaps[i] (widths[i] x heights[i]) with transforms[i], with one
+ * host wait at the end.  The documents are spread over up to 16 internal streams
+ * (RVK_BATCH_LANES) that start after the work already queued on the context stream and
+ * are joined back into it, so later work on the context stream sees every document;
+ * documents whose pixmaps overlap are drawn in order on one stream, as the loop would.
  * statuses[i] (if non-null) receives each document's status; the return value is the
  * first non-OK status, RVK_OK when every document rendered.  Arguments are checked for
- * every document before any is drawn.  Documents are spread over up to 16 internal
- * streams (RVK_BATCH_LANES) that start after, and are joined back into, the context
- * stream; documents whose pixmaps overlap are drawn in order, as the loop would. */
+ * every document before any is drawn. */
 rvk_status rvk_render_batch(rvk_context *ctx, uint32_t count, const rvk_scene *const *scenes,
                             const rvk_transform *transforms, const uint32_t *widths, const uint32_t *heights,
                             uint8_t *const *dev_pixmaps, rvk_status *statuses);

@@ -69,6 +69,15 @@ def lib() -> C.CDLL:
             L.orc_render.argtypes = [C.POINTER(abi.Scene), abi.Transform, u32, u32, P]
         if hasattr(L, "orc_path_coverage"):
             L.orc_path_coverage.argtypes = [C.POINTER(abi.Scene), u32, abi.Transform, u32, P, u32, u32]
+        for name in ("orc_fill_path", "orc_stroke_path", "orc_clear_path"):
+            getattr(L, name).argtypes = [C.POINTER(abi.Scene), u32, abi.Transform, P, u32, u32]
+            getattr(L, name).restype = i32
+        L.orc_render_node.argtypes = [C.POINTER(abi.Scene), u32, u32, C.POINTER(abi.Rect), abi.Transform, u32, u32, P]
+        L.orc_render_node.restype = i32
+        L.orc_mask_from_pixmap.argtypes = [P, u32, u32, u32, P]
+        L.orc_mask_from_pixmap.restype = None
+        L.orc_apply_mask.argtypes = [P, u32, u32, P]
+        L.orc_apply_mask.restype = None
         _lib = L
     return _lib
 
@@ -170,3 +179,32 @@ def path_coverage(scene_abi, path_index: int, ts: abi.Transform, stroke: bool, w
     cov = np.zeros((h, w), dtype=np.uint8)
     lib().orc_path_coverage(C.byref(scene_abi), path_index, ts, int(stroke), cov.ctypes.data, w, h)
     return cov
+
+
+def fill_path(scene_abi, path_index: int, ts: abi.Transform, a) -> int:
+    return lib().orc_fill_path(C.byref(scene_abi), path_index, ts, _p(a), *_wh(a))
+
+
+def stroke_path(scene_abi, path_index: int, ts: abi.Transform, a) -> int:
+    return lib().orc_stroke_path(C.byref(scene_abi), path_index, ts, _p(a), *_wh(a))
+
+
+def clear_path(scene_abi, path_index: int, ts: abi.Transform, a) -> int:
+    return lib().orc_clear_path(C.byref(scene_abi), path_index, ts, _p(a), *_wh(a))
+
+
+def render_node(scene_abi, kind: int, index: int, bbox, ts: abi.Transform, a) -> int:
+    bb = C.byref(abi.Rect(*bbox)) if bbox is not None else None
+    return lib().orc_render_node(C.byref(scene_abi), kind, index, bb, ts, *_wh(a), _p(a))
+
+
+def mask_from_pixmap(a, luminance: bool = False) -> np.ndarray:
+    w, h = _wh(a)
+    m = np.zeros((h, w), np.uint8)
+    lib().orc_mask_from_pixmap(_p(a), w, h, int(luminance), m.ctypes.data)
+    return m
+
+
+def apply_mask(a, mask: np.ndarray) -> None:
+    assert mask.dtype == np.uint8 and mask.flags.c_contiguous and mask.shape == a.shape[:2]
+    lib().orc_apply_mask(_p(a), *_wh(a), mask.ctypes.data)

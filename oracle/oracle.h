@@ -54,6 +54,17 @@ int32_t orc_render(const rvk_scene *scene, rvk_transform ts, uint32_t w, uint32_
 // fill a path into a pixmap (tiny-skia fill_path + resvg path.rs:26-75), exposed for tests
 int32_t orc_fill_path(const rvk_scene *scene, uint32_t path_index, rvk_transform ts,
                       uint8_t *px, uint32_t w, uint32_t h);
+// stroke a path (tiny-skia stroke_path + resvg path.rs:77-116); fill it with BlendMode::Clear
+int32_t orc_stroke_path(const rvk_scene *scene, uint32_t path_index, rvk_transform ts,
+                        uint8_t *px, uint32_t w, uint32_t h);
+int32_t orc_clear_path(const rvk_scene *scene, uint32_t path_index, rvk_transform ts,
+                       uint8_t *px, uint32_t w, uint32_t h);
+// resvg::render_node (lib.rs:55-70); returns 1 when drawn, 0 without a bounding box
+int32_t orc_render_node(const rvk_scene *scene, uint32_t kind, uint32_t index, const rvk_rect *bbox,
+                        rvk_transform ts, uint32_t w, uint32_t h, uint8_t *px);
+// tiny_skia::Mask::from_pixmap (alpha, or luminance when `luminance`) and Pixmap::apply_mask
+void orc_mask_from_pixmap(const uint8_t *px, uint32_t w, uint32_t h, uint32_t luminance, uint8_t *mask);
+void orc_apply_mask(uint8_t *px, uint32_t w, uint32_t h, const uint8_t *mask);
 // the u8 coverage mask of a path (for coverage-only tests)
 int32_t orc_path_coverage(const rvk_scene *scene, uint32_t path_index, rvk_transform ts, uint32_t stroke,
                           uint8_t *cov, uint32_t w, uint32_t h);

@@ -1199,3 +1199,13 @@ int32_t orc_fill_path(const rvk_scene *scene, uint32_t path_index, rvk_transform
                       uint32_t h) {
     return orc::fill_path_into(*scene, scene->paths[path_index], orc::ts_from(ts), px, w, h) ? 1 : 0;
 }
+
+int32_t orc_stroke_path(const rvk_scene *scene, uint32_t path_index, rvk_transform ts, uint8_t *px, uint32_t w,
+                        uint32_t h) {
+    return orc::stroke_path_into(*scene, scene->paths[path_index], orc::ts_from(ts), px, w, h) ? 1 : 0;
+}
+
+int32_t orc_clear_path(const rvk_scene *scene, uint32_t path_index, rvk_transform ts, uint8_t *px, uint32_t w,
+                       uint32_t h) {
+    return orc::clear_path_into(*scene, scene->paths[path_index], orc::ts_from(ts), px, w, h) ? 1 : 0;
+}

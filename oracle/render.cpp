@@ -20,7 +20,7 @@ namespace {
 
 struct Ctx { IRect max_bbox; };
 
-// convert_blend_mode (render.rs:254-273)
+// convert_blend_mode (render.rs:145-164)
 uint32_t convert_blend_mode(uint32_t m) {
     static const uint32_t map[RVK_MIX_COUNT] = {
         RVK_BLEND_SOURCE_OVER, RVK_BLEND_MULTIPLY, RVK_BLEND_SCREEN, RVK_BLEND_OVERLAY, RVK_BLEND_DARKEN,
@@ -229,4 +229,30 @@ int32_t orc_render(const rvk_scene *scene, rvk_transform ts, uint32_t w, uint32_
     Ctx ctx{max_filter_bbox(w, h)};
     render_nodes(*scene, scene->groups[0], ctx, ts_from(ts), px, w, h);
     return RVK_OK;
+}
+
+// resvg::render_node (lib.rs:55-70): None (0) without a layer bounding box, else the
+// node (render.rs:21-47) drawn with the transform pre-translated by minus the box's
+// origin (1)
+int32_t orc_render_node(const rvk_scene *scene, uint32_t kind, uint32_t index, const rvk_rect *bbox,
+                        rvk_transform ts, uint32_t w, uint32_t h, uint8_t *px) {
+    if (!bbox || !(bbox->w > 0.0f) || !(bbox->h > 0.0f) || !std::isfinite(bbox->x) || !std::isfinite(bbox->y) ||
+        !std::isfinite(bbox->w) || !std::isfinite(bbox->h))
+        return 0;
+    Ctx ctx{max_filter_bbox(w, h)};
+    const Ts t = ts_concat(ts_from(ts), Ts{1.0f, 0.0f, 0.0f, 1.0f, -bbox->x, -bbox->y});
+    if (kind == RVK_NODE_GROUP) render_group(*scene, scene->groups[index], ctx, t, px, w, h);
+    else render_path(*scene, scene->paths[index], t, px, w, h);
+    return 1;
+}
+
+// tiny_skia::Mask::from_pixmap and Pixmap::apply_mask on their own
+void orc_mask_from_pixmap(const uint8_t *px, uint32_t w, uint32_t h, uint32_t luminance, uint8_t *mask) {
+    std::vector<uint8_t> p(px, px + (size_t)w * h * 4), m;
+    mask_from_pixmap(p, luminance ? RVK_MASK_LUMINANCE : RVK_MASK_ALPHA, m);
+    std::copy(m.begin(), m.end(), mask);
+}
+void orc_apply_mask(uint8_t *px, uint32_t w, uint32_t h, const uint8_t *mask) {
+    std::vector<uint8_t> m(mask, mask + (size_t)w * h);
+    apply_mask(px, m);
 }

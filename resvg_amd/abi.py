@@ -210,6 +210,19 @@ class Scene(C.Structure):
                 ("masks", C.POINTER(Mask)), ("mask_count", C.c_uint32), ("dashes", C.POINTER(C.c_float))]
 
 
+class PathGeometry(C.Structure):
+    _fields_ = [("verbs", C.POINTER(C.c_uint8)), ("verb_count", C.c_uint32), ("points", C.POINTER(C.c_float)),
+                ("point_count", C.c_uint32)]
+
+
+class StrokeStyle(C.Structure):
+    _fields_ = [("width", C.c_float), ("miter_limit", C.c_float), ("cap", C.c_uint32), ("join", C.c_uint32),
+                ("dashes", C.POINTER(C.c_float)), ("dash_count", C.c_uint32), ("dash_offset", C.c_float)]
+
+
+MASK_TYPE_ALPHA, MASK_TYPE_LUMINANCE = 0, 1
+
+
 class Stats(C.Structure):
     _fields_ = [("kernel_launches", C.c_uint64), ("layers", C.c_uint64), ("paths", C.c_uint64),
                 ("filters", C.c_uint64), ("scratch_bytes", C.c_uint64), ("edge_pieces", C.c_uint64),

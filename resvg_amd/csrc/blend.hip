@@ -1,5 +1,5 @@
 // tiny_skia::PixmapMut::draw_pixmap with PixmapPaint{opacity, blend_mode, Nearest} and
-// an identity transform (the only form resvg uses: render.rs:242-249,
+// an identity transform (the only form resvg uses: render.rs:127-140,
 // filter/mod.rs:624-1111): every destination pixel covered by the source rectangle
 // goes through the f32 pipeline gather -> scale by opacity -> load dst -> blend -> store.
 #include "rvk_internal.h"

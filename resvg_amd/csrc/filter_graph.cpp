@@ -467,6 +467,32 @@ rvk_status rvk_fill(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, const
     premultiply_color(rgba, pm);
     return rvk::launch_fill(ctx, px, w, h, pm);
 }
+// tiny_skia::Mask operations (clip.rs:25-27, mask.rs:32-45)
+static bool mask_aligned(const void *mask, const void *px) {
+    return ((uintptr_t)mask % 4) == 0 && ((uintptr_t)px % 16) == 0;
+}
+rvk_status rvk_mask_from_pixmap(rvk_context *ctx, const uint8_t *px, uint32_t w, uint32_t h, uint32_t type,
+                                uint8_t *mask) {
+    CHECK_PX(ctx, px, w, h);
+    if (!mask || type > RVK_MASK_TYPE_LUMINANCE || !mask_aligned(mask, px)) return RVK_ERR_INVALID_ARGUMENT;
+    return rvk::launch_mask_from_pixmap(ctx, px, mask, (size_t)w * h, type);
+}
+rvk_status rvk_mask_invert(rvk_context *ctx, uint8_t *mask, uint32_t w, uint32_t h) {
+    CHECK_PX(ctx, mask, w, h);
+    if (!mask_aligned(mask, nullptr)) return RVK_ERR_INVALID_ARGUMENT;
+    return rvk::launch_mask_invert(ctx, mask, (size_t)w * h);
+}
+rvk_status rvk_apply_mask(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, const uint8_t *mask) {
+    CHECK_PX(ctx, px, w, h);
+    if (!mask || !mask_aligned(mask, px)) return RVK_ERR_INVALID_ARGUMENT;
+    return rvk::launch_apply_mask_u8(ctx, px, mask, (size_t)w * h);
+}
+// Pixmap::fill_rect with BlendMode::Clear on a pixel-aligned rect: zero the rect's pixels
+rvk_status rvk_clear_rect(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, int32_t x, int32_t y, uint32_t rw,
+                          uint32_t rh) {
+    CHECK_PX(ctx, px, w, h);
+    return rvk::launch_fill_sub(ctx, px, w, h, x, y, (int64_t)x + rw, (int64_t)y + rh, 0u);
+}
 rvk_status rvk_filter_apply(rvk_context *ctx, const rvk_filter *filter, const rvk_filter_data *data,
                             rvk_transform ts, uint8_t *px, uint32_t w, uint32_t h, rvk_status *out_error) {
     CHECK_PX(ctx, px, w, h);

@@ -242,6 +242,63 @@ __global__ void k_apply_mask(uint32_t *px, const uint32_t *src, size_t n, uint32
     }
 }
 
+// The tiny_skia::Mask operations on their own (a u8 plane, one byte per pixel).
+// Each thread handles 4 consecutive pixels: one 16-byte pixel load and one 4-byte
+// mask word when the group is whole and aligned, single pixels for the tail.
+__global__ void k_mask_from_pixmap(const uint32_t *px, uint8_t *mask, size_t n, uint32_t mode) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; 4 * q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = 4 * q;
+        if (i + 4 <= n) {
+            const uint4 p = *reinterpret_cast<const uint4 *>(px + i);
+            const uint32_t w = mask_value(p.x, mode) | (mask_value(p.y, mode) << 8) | (mask_value(p.z, mode) << 16) |
+                               (mask_value(p.w, mode) << 24);
+            *reinterpret_cast<uint32_t *>(mask + i) = w;
+        } else {
+            for (size_t k = i; k < n; k++) mask[k] = (uint8_t)mask_value(px[k], mode);
+        }
+    }
+}
+
+__global__ void k_mask_invert(uint8_t *mask, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; 4 * q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = 4 * q;
+        if (i + 4 <= n) *reinterpret_cast<uint32_t *>(mask + i) ^= 0xffffffffu;
+        else for (size_t k = i; k < n; k++) mask[k] = (uint8_t)(255u - mask[k]);
+    }
+}
+
+__device__ __forceinline__ uint32_t scale_px(uint32_t d, uint32_t m) {
+    if (m == 255u) return d;
+    uint32_t o = 0;
+    for (int k = 0; k < 4; k++) o |= ((((d >> (8 * k)) & 0xffu) * m + 255u) >> 8) << (8 * k);
+    return o;
+}
+
+__global__ void k_apply_mask_u8(uint32_t *px, const uint8_t *mask, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; 4 * q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = 4 * q;
+        if (i + 4 <= n) {
+            const uint32_t m = *reinterpret_cast<const uint32_t *>(mask + i);
+            uint4 p = *reinterpret_cast<const uint4 *>(px + i);
+            p.x = scale_px(p.x, m & 0xffu);
+            p.y = scale_px(p.y, (m >> 8) & 0xffu);
+            p.z = scale_px(p.z, (m >> 16) & 0xffu);
+            p.w = scale_px(p.w, m >> 24);
+            *reinterpret_cast<uint4 *>(px + i) = p;
+        } else {
+            for (size_t k = i; k < n; k++) px[k] = scale_px(px[k], mask[k]);
+        }
+    }
+}
+
+__global__ void k_fill_u8(uint8_t *p, size_t n, uint32_t v4) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; 4 * q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = 4 * q;
+        if (i + 4 <= n) *reinterpret_cast<uint32_t *>(p + i) = v4;
+        else for (size_t k = i; k < n; k++) p[k] = (uint8_t)v4;
+    }
+}
+
 __global__ void k_zero_rgb(uint32_t *px, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         px[i] &= 0xff000000u;
@@ -474,6 +531,44 @@ rvk_status launch_apply_mask(rvk_context *ctx, uint8_t *px, const uint8_t *src, 
     timed_launch(ctx, "apply_mask", 12.0 * (double)n, [&] {  // reads both pixmaps, writes one
         hipLaunchKernelGGL(k_apply_mask, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px,
                            (const uint32_t *)src, n, mode);
+    });
+    return post_launch(ctx);
+}
+
+// The mask kernels require 4-byte aligned masks and 16-byte aligned pixmaps (the
+// callers' buffers: device allocations, torch tensors); the C entry points check it.
+rvk_status launch_mask_from_pixmap(rvk_context *ctx, const uint8_t *px, uint8_t *mask, size_t n, uint32_t type) {
+    const uint32_t mode = type == RVK_MASK_TYPE_LUMINANCE ? kMaskLuminance : kMaskAlpha;
+    const size_t groups = (n + 3) / 4;
+    timed_launch(ctx, "mask_from_pixmap", 5.0 * (double)n, [&] {  // 4 B read, 1 B written per pixel
+        hipLaunchKernelGGL(k_mask_from_pixmap, dim3(grid_for(groups)), dim3(kBlock), 0, ctx->stream,
+                           (const uint32_t *)px, mask, n, mode);
+    });
+    return post_launch(ctx);
+}
+
+rvk_status launch_mask_invert(rvk_context *ctx, uint8_t *mask, size_t n) {
+    const size_t groups = (n + 3) / 4;
+    timed_launch(ctx, "mask_invert", 2.0 * (double)n, [&] {
+        hipLaunchKernelGGL(k_mask_invert, dim3(grid_for(groups)), dim3(kBlock), 0, ctx->stream, mask, n);
+    });
+    return post_launch(ctx);
+}
+
+rvk_status launch_apply_mask_u8(rvk_context *ctx, uint8_t *px, const uint8_t *mask, size_t n) {
+    const size_t groups = (n + 3) / 4;
+    timed_launch(ctx, "apply_mask", 9.0 * (double)n, [&] {  // pixel read + write, mask read
+        hipLaunchKernelGGL(k_apply_mask_u8, dim3(grid_for(groups)), dim3(kBlock), 0, ctx->stream, (uint32_t *)px,
+                           mask, n);
+    });
+    return post_launch(ctx);
+}
+
+rvk_status launch_fill_u8(rvk_context *ctx, uint8_t *p, size_t n, uint8_t value) {
+    const size_t groups = (n + 3) / 4;
+    const uint32_t v4 = 0x01010101u * value;
+    timed_launch(ctx, "fill_mask", (double)n, [&] {
+        hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(groups)), dim3(kBlock), 0, ctx->stream, p, n, v4);
     });
     return post_launch(ctx);
 }

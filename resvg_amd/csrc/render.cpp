@@ -73,7 +73,7 @@ struct DrawList {
     int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN;
 };
 
-uint32_t convert_blend_mode(uint32_t m) {  // render.rs:254-273
+uint32_t convert_blend_mode(uint32_t m) {  // render.rs:145-164
     static const uint32_t map[RVK_MIX_COUNT] = {
         RVK_BLEND_SOURCE_OVER, RVK_BLEND_MULTIPLY, RVK_BLEND_SCREEN, RVK_BLEND_OVERLAY, RVK_BLEND_DARKEN,
         RVK_BLEND_LIGHTEN, RVK_BLEND_COLOR_DODGE, RVK_BLEND_COLOR_BURN, RVK_BLEND_HARD_LIGHT, RVK_BLEND_SOFT_LIGHT,
@@ -847,8 +847,13 @@ struct Renderer {
 
 namespace rvk {
 
-rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t w, uint32_t h, uint8_t *px,
-                        uint64_t *overflow_rows) {
+// What a render walks: the root group's children (resvg::render), one node
+// (resvg::render_node), or the fills of the root group's children drawn with
+// BlendMode::Clear (a clip path's children, clip.rs:30-70).
+enum class Walk { kRoot, kNode, kClearFills };
+
+rvk_status render_walk(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t w, uint32_t h, uint8_t *px,
+                       uint64_t *overflow_rows, Walk walk, rvk_node_ref node) {
     Renderer r{ctx, *scene, max_filter_bbox(w, h)};
     r.build_aux();
     void *ovbuf = nullptr;
@@ -861,18 +866,28 @@ rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_
             return s;
         }
     }
+    auto run = [&](DrawList &dl) {
+        switch (walk) {
+            case Walk::kRoot: r.render_nodes(scene->groups[0], ts, dl); break;
+            case Walk::kNode:  // render::render_node (render.rs:21-47)
+                if (node.kind == RVK_NODE_GROUP) r.render_group(scene->groups[node.index], ts, dl);
+                else r.render_path(scene->paths[node.index], ts, dl);
+                break;
+            case Walk::kClearFills: r.clip_children(scene->groups[0], true, ts, dl, 0); break;
+        }
+    };
     DrawList root;
     root.w = w;
     root.h = h;
     r.mode = Renderer::kCollect;
-    r.render_nodes(scene->groups[0], ts, root);
+    run(root);
     r.geoms.resize(r.tasks.size());
     r.mode = Renderer::kExecute;
     root = DrawList{};
     root.px = px;
     root.w = w;
     root.h = h;
-    r.render_nodes(scene->groups[0], ts, root);
+    run(root);
     if (r.err == RVK_OK) r.fail(r.flush(root));
     // the one wait of a render: read the overflow counter
     int32_t ov = 0;
@@ -892,6 +907,11 @@ rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_
     if (overflow_rows) *overflow_rows = r.overflow_rows;
     if (r.err == RVK_OK && r.overflow_rows) return RVK_ERR_UNSUPPORTED;
     return r.err;
+}
+
+rvk_status render_scene(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t w, uint32_t h, uint8_t *px,
+                        uint64_t *overflow_rows) {
+    return render_walk(ctx, scene, ts, w, h, px, overflow_rows, Walk::kRoot, rvk_node_ref{0, 0});
 }
 
 // Streams a batch spreads its documents over (RVK_BATCH_LANES overrides, 1-32).
@@ -1140,6 +1160,164 @@ rvk_status rvk_render_batch(rvk_context *ctx, uint32_t count, const rvk_scene *c
     ctx->stats.host_build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() -
         ctx->stats.host_wait_ms;
+    return s;
+}
+
+// resvg::render_node (lib.rs:55-70): nothing without a layer bounding box, else the
+// node drawn with the transform pre-translated by minus the box's origin.
+rvk_status rvk_render_node(rvk_context *ctx, const rvk_scene *scene, rvk_node_ref node, const rvk_rect *bbox,
+                           rvk_transform transform, uint32_t width, uint32_t height, uint8_t *dev_pixmap,
+                           int32_t *rendered) {
+    if (!ctx || !scene_ok(scene) || !dev_pixmap || !rendered || !rvk::valid_dims(width, height))
+        return RVK_ERR_INVALID_ARGUMENT;
+    if ((node.kind == RVK_NODE_GROUP && node.index >= scene->group_count) ||
+        (node.kind == RVK_NODE_PATH && node.index >= scene->path_count) ||
+        (node.kind != RVK_NODE_GROUP && node.kind != RVK_NODE_PATH))
+        return RVK_ERR_INVALID_ARGUMENT;
+    *rendered = 0;
+    ctx->stats = rvk_stats{};
+    // Node::abs_layer_bounding_box() is a NonZeroRect: None unless positive and finite
+    if (!bbox || !(bbox->w > 0.0f) || !(bbox->h > 0.0f) || !std::isfinite(bbox->x) || !std::isfinite(bbox->y) ||
+        !std::isfinite(bbox->w) || !std::isfinite(bbox->h))
+        return RVK_OK;
+    const Ts ts = ts_concat(ts_from(transform), Ts{1.0f, 0.0f, 0.0f, 1.0f, -bbox->x, -bbox->y});  // pre_translate
+    const auto t0 = std::chrono::steady_clock::now();
+    const rvk_status s = rvk::render_walk(ctx, scene, ts, width, height, dev_pixmap, nullptr, rvk::Walk::kNode, node);
+    ctx->stats.host_build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() -
+        ctx->stats.host_wait_ms;
+    if (s == RVK_OK) *rendered = 1;
+    return s;
+}
+
+// One path under a root group, built from the caller's arrays (they stay in place):
+// what path.rs's fill_path / stroke_path draw when called on their own.
+struct OnePath {
+    rvk_group root{};
+    rvk_path path{};
+    rvk_node_ref child{RVK_NODE_PATH, 0};
+    rvk_scene sc{};
+};
+
+// Checks the verbs against the points they consume; fills `o` (no paint set).
+static rvk_status one_path(const rvk_path_geometry *g, uint32_t anti_alias, OnePath &o) {
+    if (!g || (g->verb_count && !g->verbs) || (g->point_count && !g->points)) return RVK_ERR_INVALID_ARGUMENT;
+    uint64_t need = 0;
+    for (uint32_t i = 0; i < g->verb_count; i++) {
+        switch (g->verbs[i]) {
+            case RVK_VERB_MOVE: case RVK_VERB_LINE: need += 1; break;
+            case RVK_VERB_QUAD: need += 2; break;
+            case RVK_VERB_CUBIC: need += 3; break;
+            case RVK_VERB_CLOSE: break;
+            default: return RVK_ERR_INVALID_ARGUMENT;
+        }
+    }
+    if (need > g->point_count) return RVK_ERR_INVALID_ARGUMENT;
+    o.root.transform = rvk_transform{1, 0, 0, 1, 0, 0};
+    o.root.opacity = 1.0f;
+    o.root.blend_mode = RVK_MIX_NORMAL;
+    o.root.clip_path = o.root.mask = -1;
+    o.root.layer_bbox = rvk_rect{0, 0, 1, 1};
+    o.root.child_offset = 0;
+    o.root.child_count = 1;
+    rvk_path &p = o.path;
+    p.verb_offset = 0;
+    p.verb_count = g->verb_count;
+    p.point_offset = 0;
+    p.visible = 1;
+    p.anti_alias = anti_alias ? 1 : 0;
+    p.fill.kind = p.stroke.kind = RVK_PAINT_NONE;
+    p.fill_rule = RVK_FILL_NONZERO;
+    p.stroke_miter_limit = 4.0f;
+    // tiny_skia_path::Path::bounds(): the bounds of every point
+    float l = INFINITY, t = INFINITY, r = -INFINITY, b = -INFINITY;
+    for (uint64_t i = 0; i < need; i++) {
+        l = std::min(l, g->points[2 * i]); r = std::max(r, g->points[2 * i]);
+        t = std::min(t, g->points[2 * i + 1]); b = std::max(b, g->points[2 * i + 1]);
+    }
+    p.bounds = need ? rvk_rect{l, t, r - l, b - t} : rvk_rect{0, 0, 0, 0};
+    o.sc.groups = &o.root;
+    o.sc.group_count = 1;
+    o.sc.paths = &o.path;
+    o.sc.path_count = 1;
+    o.sc.children = &o.child;
+    o.sc.verbs = g->verbs;
+    o.sc.points = g->points;
+    return need ? RVK_OK : RVK_ERR_UNSUPPORTED;  // no points: nothing to draw (mapped to OK by callers)
+}
+
+static bool paint_ok(const rvk_paint *pt, const rvk_stop *stops) {
+    if (!pt || pt->kind > RVK_PAINT_RADIAL || pt->spread > RVK_SPREAD_REPEAT) return false;
+    return pt->kind < RVK_PAINT_LINEAR || pt->stop_count == 0 || stops;
+}
+
+rvk_status rvk_fill_path(rvk_context *ctx, uint8_t *dev_px, uint32_t width, uint32_t height,
+                         const rvk_path_geometry *path, uint32_t fill_rule, const rvk_paint *paint,
+                         const rvk_stop *stops, rvk_transform transform, uint32_t blend_mode, uint32_t anti_alias) {
+    if (!ctx || !dev_px || !rvk::valid_dims(width, height) || !paint_ok(paint, stops) || fill_rule > RVK_FILL_EVENODD)
+        return RVK_ERR_INVALID_ARGUMENT;
+    if (blend_mode != RVK_BLEND_SOURCE_OVER && blend_mode != RVK_BLEND_CLEAR) return RVK_ERR_UNSUPPORTED;
+    OnePath o;
+    const rvk_status v = one_path(path, anti_alias, o);
+    if (v != RVK_OK) return v == RVK_ERR_UNSUPPORTED ? RVK_OK : v;
+    o.path.fill = *paint;
+    o.path.fill_rule = fill_rule;
+    o.sc.stops = stops;
+    ctx->stats = rvk_stats{};
+    return rvk::render_walk(ctx, &o.sc, ts_from(transform), width, height, dev_px, nullptr,
+                            blend_mode == RVK_BLEND_CLEAR ? rvk::Walk::kClearFills : rvk::Walk::kRoot,
+                            rvk_node_ref{0, 0});
+}
+
+rvk_status rvk_stroke_path(rvk_context *ctx, uint8_t *dev_px, uint32_t width, uint32_t height,
+                           const rvk_path_geometry *path, const rvk_stroke_style *stroke, const rvk_paint *paint,
+                           const rvk_stop *stops, rvk_transform transform, uint32_t anti_alias) {
+    if (!ctx || !dev_px || !rvk::valid_dims(width, height) || !paint_ok(paint, stops) || !stroke ||
+        stroke->cap > RVK_CAP_SQUARE || stroke->join > RVK_JOIN_BEVEL || (stroke->dash_count && !stroke->dashes))
+        return RVK_ERR_INVALID_ARGUMENT;
+    OnePath o;
+    const rvk_status v = one_path(path, anti_alias, o);
+    if (v != RVK_OK) return v == RVK_ERR_UNSUPPORTED ? RVK_OK : v;
+    o.path.stroke = *paint;
+    o.path.stroke_width = stroke->width;
+    o.path.stroke_miter_limit = stroke->miter_limit;
+    o.path.stroke_cap = stroke->cap;
+    o.path.stroke_join = stroke->join;
+    o.path.dash_offset = 0;
+    o.path.dash_count = stroke->dash_count;
+    o.path.dash_phase = stroke->dash_offset;
+    o.sc.stops = stops;
+    o.sc.dashes = stroke->dashes;
+    ctx->stats = rvk_stats{};
+    return rvk::render_walk(ctx, &o.sc, ts_from(transform), width, height, dev_px, nullptr, rvk::Walk::kRoot,
+                            rvk_node_ref{0, 0});
+}
+
+// Mask::fill_path on a new mask: the coverage is the alpha of an opaque white fill
+// (an opaque shader composites with Source, so alpha = coverage) on a cleared pixmap.
+rvk_status rvk_mask_fill_path(rvk_context *ctx, uint8_t *dev_mask, uint32_t width, uint32_t height,
+                              const rvk_path_geometry *path, uint32_t fill_rule, uint32_t anti_alias,
+                              rvk_transform transform) {
+    if (!ctx || !dev_mask || !rvk::valid_dims(width, height) || fill_rule > RVK_FILL_EVENODD)
+        return RVK_ERR_INVALID_ARGUMENT;
+    OnePath o;
+    const rvk_status v = one_path(path, anti_alias, o);
+    if (v != RVK_OK && v != RVK_ERR_UNSUPPORTED) return v;
+    const size_t n = (size_t)width * height;
+    if (v == RVK_ERR_UNSUPPORTED) return rvk::launch_fill_u8(ctx, dev_mask, n, 0);
+    o.path.fill.kind = RVK_PAINT_COLOR;
+    o.path.fill.color[0] = o.path.fill.color[1] = o.path.fill.color[2] = 255;
+    o.path.fill.opacity = 1.0f;
+    o.path.fill_rule = fill_rule;
+    void *buf = nullptr;
+    RVK_TRY(rvk::scratch_alloc(ctx, n * 4, &buf));
+    ctx->stats = rvk_stats{};
+    rvk_status s = rvk::launch_fill_sub(ctx, (uint8_t *)buf, width, height, 0, 0, width, height, 0u);
+    if (s == RVK_OK)
+        s = rvk::render_walk(ctx, &o.sc, ts_from(transform), width, height, (uint8_t *)buf, nullptr, rvk::Walk::kRoot,
+                             rvk_node_ref{0, 0});
+    if (s == RVK_OK) s = rvk::launch_mask_from_pixmap(ctx, (const uint8_t *)buf, dev_mask, n, RVK_MASK_TYPE_ALPHA);
+    rvk::scratch_release(ctx, buf);
     return s;
 }
 

@@ -146,5 +146,10 @@ rvk_status launch_tile(rvk_context *ctx, uint8_t *dst, uint32_t w, uint32_t h, c
 rvk_status launch_zero_rgb(rvk_context *ctx, uint8_t *px, size_t n);
 // px = div255(px * m) per channel, m from `src` (0 alpha, 1 inverted alpha, 2 luminance)
 rvk_status launch_apply_mask(rvk_context *ctx, uint8_t *px, const uint8_t *src, size_t n, uint32_t mode);
+// tiny_skia::Mask operations on u8 planes of n bytes (mask 4-byte, pixmap 16-byte aligned)
+rvk_status launch_mask_from_pixmap(rvk_context *ctx, const uint8_t *px, uint8_t *mask, size_t n, uint32_t type);
+rvk_status launch_mask_invert(rvk_context *ctx, uint8_t *mask, size_t n);
+rvk_status launch_apply_mask_u8(rvk_context *ctx, uint8_t *px, const uint8_t *mask, size_t n);
+rvk_status launch_fill_u8(rvk_context *ctx, uint8_t *p, size_t n, uint8_t value);
 
 }  // namespace rvk

@@ -126,6 +126,8 @@ class Renderer:
                "rvk_diff_workspace_size")
         self.workspace_bytes = int(nbytes.value)
         self.workspace = torch.empty(self.batch * self.workspace_bytes, dtype=torch.uint8, device=dev)
+        # counts forward calls: the workspace holds the intermediates of the latest one
+        self.generation = 0
         cov = self.coverage.data_ptr() if self.coverage is not None else None
         self.scene = DiffScene(self.w, self.h, self.n, self.k, cov, self.ends.data_ptr())
 
@@ -140,19 +142,23 @@ class Renderer:
             assert geom is not None and geom.dtype == torch.float32 and geom.is_contiguous()
             assert geom.numel() == self.batch * self.n * geom_floats(self.k)
         out = torch.empty(self._image_shape(), dtype=torch.float32, device=params.device)
-        _check(self._L.rvk_diff_forward_batch(self.ctx._h, C.byref(self.scene), self.batch, _ptr(params),
+        _check(self._L.rvk_diff_forward_batch(self.ctx.h, C.byref(self.scene), self.batch, _ptr(params),
                                               _ptr(geom) if self.k else None, _ptr(self.workspace), _ptr(out)),
                "rvk_diff_forward_batch")
+        self.generation += 1
         return out
 
     def backward(self, params, grad_out, geom=None):
-        """dL/d(params), and dL/d(geom) in the analytic mode (else None)."""
+        """dL/d(params), and dL/d(geom) in the analytic mode (else None).  Reads the
+        intermediates the latest forward left in the workspace, so it must follow the
+        forward of the same parameters and geometry (the autograd wrapper from
+        render_fn re-runs that forward when another one came in between)."""
         import torch
         grad_out = grad_out.contiguous().float()
         assert grad_out.numel() == self.batch * self.w * self.h * 4
         g = torch.empty_like(params)
         gg = torch.empty_like(geom) if self.k else None
-        _check(self._L.rvk_diff_backward_batch(self.ctx._h, C.byref(self.scene), self.batch, _ptr(params),
+        _check(self._L.rvk_diff_backward_batch(self.ctx.h, C.byref(self.scene), self.batch, _ptr(params),
                                                _ptr(geom) if self.k else None, _ptr(self.workspace),
                                                _ptr(grad_out), _ptr(g), _ptr(gg) if self.k else None),
                "rvk_diff_backward_batch")
@@ -172,16 +178,23 @@ def render_fn():
             out = renderer.forward(params, geom)
             ctx.save_for_backward(params, geom) if geom is not None else ctx.save_for_backward(params)
             ctx.renderer = renderer
+            ctx.generation = renderer.generation
             return out
 
         @staticmethod
         def backward(ctx, grad_out):
             saved = ctx.saved_tensors
-            # the workspace holds the forward intermediates of these parameters
-            if len(saved) == 2:
-                gp, gg = ctx.renderer.backward(saved[0], grad_out, saved[1])
+            params, geom = saved[0], (saved[1] if len(saved) == 2 else None)
+            r = ctx.renderer
+            # the shared workspace holds the intermediates of the renderer's latest
+            # forward; when a later forward replaced them, recompute this one's (the
+            # forward is deterministic, so they are the same bits)
+            if r.generation != ctx.generation:
+                r.forward(params, geom)
+            if geom is not None:
+                gp, gg = r.backward(params, grad_out, geom)
                 return gp, gg, None
-            return ctx.renderer.backward(saved[0], grad_out), None, None
+            return r.backward(params, grad_out), None, None
 
     return DiffRender.apply
 

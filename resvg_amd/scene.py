@@ -61,6 +61,29 @@ class PaintSpec:
                          transform=transform or (1.0, 0.0, 0.0, 1.0, 0.0, 0.0))
 
 
+def paint_to_abi(ps: PaintSpec, stops: list) -> abi.Paint:
+    """The rvk_paint of a PaintSpec; its stops are appended to `stops` (abi.Stop)."""
+    P = abi.Paint()
+    P.kind = ps.kind
+    for k in range(3):
+        P.color[k] = ps.color[k]
+    P.opacity = ps.opacity
+    P.spread = ps.spread
+    P.x1, P.y1, P.x2, P.y2 = ps.x1, ps.y1, ps.x2, ps.y2
+    P.cx, P.cy, P.r, P.fx, P.fy, P.fr = ps.cx, ps.cy, ps.r, ps.fx, ps.fy, ps.fr
+    P.transform = abi.Transform(*ps.transform)
+    P.stop_offset = len(stops)
+    P.stop_count = len(ps.stops)
+    for off, col, op in ps.stops:
+        s = abi.Stop()
+        s.offset = off
+        for k in range(3):
+            s.color[k] = col[k]
+        s.opacity = op
+        stops.append(s)
+    return P
+
+
 class Scene:
     """Builder; node 0 is the root group."""
 
@@ -207,6 +230,47 @@ class Scene:
                                           max(box[3], cb[3]))
         return box
 
+    # ---- absolute bounding boxes (usvg's Node::abs_layer_bounding_box) ----
+    @staticmethod
+    def _concat(a, b):
+        """a.pre_concat(b): maps by b first, then a (6-tuples sx, ky, kx, sy, tx, ty)."""
+        return (a[0] * b[0] + a[2] * b[1], a[1] * b[0] + a[3] * b[1], a[0] * b[2] + a[2] * b[3],
+                a[1] * b[2] + a[3] * b[3], a[0] * b[4] + a[2] * b[5] + a[4], a[1] * b[4] + a[3] * b[5] + a[5])
+
+    def _parent_transforms(self):
+        """Per group, the product of its ancestors' transforms (its parent's abs transform)."""
+        out = {self.root: (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)}
+        stack = [self.root]
+        while stack:
+            g = stack.pop()
+            abs_g = self._concat(out[g], self.groups[g]["transform"])
+            for kind, idx in self.groups[g]["children"]:
+                if kind == abi.NODE_GROUP:
+                    out[idx] = abs_g
+                    stack.append(idx)
+        return out
+
+    def abs_layer_bbox(self, kind: int, index: int):
+        """(x, y, w, h) of the node's layer bounding box in canvas coordinates, or None:
+        a group's layer box mapped by its absolute transform (its ancestors' and its
+        own), a path's fill bounds mapped by its parent's (None for a zero width or
+        height, as NonZeroRect)."""
+        parents = self._parent_transforms()
+        if kind == abi.NODE_GROUP:
+            t = self._concat(parents[index], self.groups[index]["transform"])
+            b = self._group_layer_bounds(index)
+        else:
+            owner = next(g for g, grp in enumerate(self.groups) if (abi.NODE_PATH, index) in grp["children"])
+            t = self._concat(parents.get(owner, (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)), self.groups[owner]["transform"])
+            b = self._path_bounds(self.paths[index])
+        if b is None:
+            return None
+        x0, y0, x1, y1 = self._transform_box(b, t)
+        w, h = f32(x1 - x0), f32(y1 - y0)
+        if not (w > 0 and h > 0):
+            return None
+        return (f32(x0), f32(y0), w, h)
+
     # ---- ABI ----
     def to_abi(self):
         if self._abi is not None:
@@ -234,25 +298,7 @@ class Scene:
         stops = []
 
         def paint(ps: PaintSpec) -> abi.Paint:
-            P = abi.Paint()
-            P.kind = ps.kind
-            for k in range(3):
-                P.color[k] = ps.color[k]
-            P.opacity = ps.opacity
-            P.spread = ps.spread
-            P.x1, P.y1, P.x2, P.y2 = ps.x1, ps.y1, ps.x2, ps.y2
-            P.cx, P.cy, P.r, P.fx, P.fy, P.fr = ps.cx, ps.cy, ps.r, ps.fx, ps.fy, ps.fr
-            P.transform = abi.Transform(*ps.transform)
-            P.stop_offset = len(stops)
-            P.stop_count = len(ps.stops)
-            for off, col, op in ps.stops:
-                s = abi.Stop()
-                s.offset = off
-                for k in range(3):
-                    s.color[k] = col[k]
-                s.opacity = op
-                stops.append(s)
-            return P
+            return paint_to_abi(ps, stops)
 
         for i, p in enumerate(self.paths):
             Q = paths[i]

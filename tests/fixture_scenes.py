@@ -157,7 +157,7 @@ def guide_mask(guides, size: int = SIZE) -> np.ndarray:
 
 
 def layer_geometry(fx: Fixture):
-    """ibbox and layer transform of the filtered group (render.rs:171-215)."""
+    """ibbox and layer transform of the filtered group (render.rs:49-143)."""
     rx, ry, rw, rh = fx.graph.rect
     s = np.float32(SCALE)
     l, t = np.float32(rx) * s, np.float32(ry) * s

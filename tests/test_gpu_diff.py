@@ -102,6 +102,35 @@ def test_autograd_function_and_fit_step(ctx):
     assert loss.item() < first
 
 
+@pytest.mark.parametrize("analytic", [False, True])
+def test_backward_after_a_second_forward(ctx, analytic):
+    # two forwards on one renderer (two parameter sets in one loss), then the first
+    # one's backward: its gradients must equal those of a lone forward + backward,
+    # although the second forward replaced the workspace's intermediates
+    import torch
+    k = 3 if analytic else 0
+    w, h, paths, ends, p0, pt = diff_scene(seed=9, size=40, n_paths=5, segments=3, sigma=1.5)
+    r = diff.Renderer(ctx, w, h, paths, ends, cubics=k)
+    render = diff.render_fn()
+    geom = r.geom0.clone() if analytic else None
+    target = torch.zeros((h, w, 4), device="cuda")
+    pa = torch.as_tensor(p0, device="cuda").clone().requires_grad_(True)
+    pb = torch.as_tensor(pt, device="cuda").clone().requires_grad_(True)
+    ga = geom.clone().requires_grad_(True) if analytic else None
+    gb = (geom * 1.01).contiguous().requires_grad_(True) if analytic else None
+    out_a = render(pa, ga, r)
+    out_b = render(pb, gb, r)
+    ((out_a - target) ** 2).sum().backward()  # only the first forward's graph
+    lone_p = torch.as_tensor(p0, device="cuda").clone().requires_grad_(True)
+    lone_g = geom.clone().requires_grad_(True) if analytic else None
+    ((render(lone_p, lone_g, r) - target) ** 2).sum().backward()
+    assert torch.equal(pa.grad, lone_p.grad)
+    if analytic:
+        assert torch.equal(ga.grad, lone_g.grad)
+    ((out_b - target) ** 2).sum().backward()
+    assert pb.grad is not None and torch.isfinite(pb.grad).all()
+
+
 # ---- analytic mode: coverage from the geometry, gradients to control points and
 # transforms -------------------------------------------------------------------------
 def _setup_geom(ctx, seed, size=40, n=5, k=3, sigma=1.3):
@@ -282,5 +311,5 @@ def test_scene_limits_are_rejected(ctx):
     p = C.c_void_p(buf.data_ptr())
     for n, k, h in ((140000, 1000, 16), (3_000_000, 0, 1024)):
         sc = diff.DiffScene(16, h, n, k, p if k == 0 else None, p)
-        st = L.rvk_diff_forward_batch(ctx._h, C.byref(sc), 1, p, p if k else None, p, p)
+        st = L.rvk_diff_forward_batch(ctx.h, C.byref(sc), 1, p, p if k else None, p, p)
         assert st == abi.ERR_INVALID_ARGUMENT, (n, k, h, st)
