@@ -34,10 +34,16 @@ def _rand_premul(h, w, seed):
 # rvk_render_node
 # ---------------------------------------------------------------------------
 def _node_cases(sc):
-    nodes = [(abi.NODE_GROUP, g) for g in range(1, len(sc.groups)) if g not in
-             {c["root"] for c in sc.clip_paths} | {m["root"] for m in sc.masks}]
-    nodes += [(abi.NODE_PATH, p) for p in range(len(sc.paths))]
-    return nodes
+    """Nodes of the document tree (reachable from the root group, so not the
+    content of clip paths and masks, which render_node does not address), root
+    excluded."""
+    nodes, stack = [], [sc.root]
+    while stack:
+        for kind, idx in sc.groups[stack.pop()]["children"]:
+            nodes.append((kind, idx))
+            if kind == abi.NODE_GROUP:
+                stack.append(idx)
+    return sorted(nodes)
 
 
 @pytest.mark.parametrize("seed", range(3))
