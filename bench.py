@@ -42,11 +42,17 @@ def _workload(size, paths, seed):
 
 
 # ---------------------------------------------------------------------------
-# CPU baseline: the oracle (a scalar C++ restatement of the reference path) on a
-# stratified sample of the same document's top-level items.  resvg sizes a group's
-# layer by the group's bounds (up to 5x the canvas), not by the visible canvas, so a
-# crop of the canvas does not shrink the work; a subset of the items does.
+# CPU baseline: the oracle (a scalar C++ restatement of the reference path), built
+# -O3 -march=native on this machine, on a stratified sample of the same document's
+# top-level items.  resvg sizes a group's layer by the group's bounds (up to 5x the
+# canvas), not by the visible canvas, so a crop of the canvas does not shrink the
+# work; a subset of the items does.
 # ---------------------------------------------------------------------------
+_FE_NAMES = ["blend", "color_matrix", "component_transfer", "composite", "convolve_matrix", "diffuse_lighting",
+             "displacement_map", "drop_shadow", "flood", "gaussian_blur", "image", "merge", "morphology", "offset",
+             "specular_lighting", "tile", "turbulence"]
+
+
 class _Keep:
     """Keeps filter group number `fg` (-1: none) and, of the plain items, every
     `stride`-th one, dealt round-robin over `parts` jobs (this is job `part`)."""
@@ -64,6 +70,10 @@ class _Keep:
         return k % self.stride == 0 and (k // self.stride) % self.parts == self.part
 
 
+def _cpu_init(lib_path):
+    os.environ["RVK_ORACLE_LIB"] = lib_path  # before oracle.bindings is imported
+
+
 def _cpu_job(job):
     size, paths, seed, fg, stride, parts, part = job
     import numpy as np
@@ -73,39 +83,88 @@ def _cpu_job(job):
     sc, _ = full_filter_doc(seed=seed, size=size, n_paths=paths, keep=_Keep(fg, stride, parts, part))
     s = sc.to_abi()
     px = np.zeros((size, size, 4), np.uint8)
+    orc.prof_enable(True)
     t = time.perf_counter()
     orc.render(s.abi, abi.Transform.identity(), px)
-    return time.perf_counter() - t, len(sc.paths)
+    dt = time.perf_counter() - t
+    prof = orc.prof_read()
+    orc.prof_enable(False)
+    return dt, len(sc.paths), prof
 
 
-def cpu_baseline(size, paths, seed, workers, plain_stride=64, filter_group=1, n_filter_groups=8):
-    """One filter group (of 8) on one core and every `plain_stride`-th plain path spread
-    over the other cores; the whole-document time per core is estimated as
-    8 x (filter group time) + plain_stride x (sum of the plain jobs' times)."""
+def _cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(size, paths, seed, workers, plain_stride=128, n_filter_groups=8, runs=5):
+    """The oracle built -O3 -march=native for this host, one document part per core:
+    each of the 8 filter groups alone, and every `plain_stride`-th plain path spread
+    over the remaining cores.  One warm-up run, then `runs` timed runs; per part the
+    median time.  The whole-document time per core is estimated as the sum of the 8
+    filter-group medians + plain_stride x the sum of the plain parts' medians."""
+    import statistics
+    import tempfile
     from oracle import bindings as orc
-    orc.lib()  # builds the oracle if it is missing
-    parts = max(1, workers - 1)
-    jobs = [(size, paths, seed, filter_group, 1 << 30, 1, 1)]  # the filter group alone
+    lib_dir = tempfile.mkdtemp(prefix="rvk_oracle_")
+    t0 = time.perf_counter()
+    lib_path = str(orc.build_native(lib_dir))
+    t_build = time.perf_counter() - t0
+    parts = max(1, workers - n_filter_groups)
+    jobs = [(size, paths, seed, fg, 1 << 30, 1, 1) for fg in range(n_filter_groups)]  # the filter groups alone
     jobs += [(size, paths, seed, -1, plain_stride, parts, p) for p in range(parts)]
     ctx = mp.get_context("spawn")
+    times = [[] for _ in jobs]
+    profs = None
     t0 = time.perf_counter()
-    with ctx.Pool(min(workers, len(jobs))) as pool:
-        res = pool.map(_cpu_job, jobs)
+    with ctx.Pool(min(workers, len(jobs)), initializer=_cpu_init, initargs=(lib_path,)) as pool:
+        for r in range(runs + 1):
+            res = pool.map(_cpu_job, jobs)
+            print(f"[bench] cpu baseline run {r + 1}/{runs + 1} ({'warm-up' if r == 0 else 'timed'}): "
+                  f"{max(x[0] for x in res):.1f} s longest part", file=sys.stderr, flush=True)
+            if r == 0:
+                continue
+            for i, x in enumerate(res):
+                times[i].append(x[0])
+            if profs is None:
+                profs = [x[2] for x in res]
     wall = time.perf_counter() - t0
-    t_filter = res[0][0]
-    t_plain = sum(r[0] for r in res[1:])
-    n_plain = sum(r[1] for r in res[1:])
-    t_doc = n_filter_groups * t_filter + plain_stride * t_plain
+    med = [statistics.median(t) for t in times]
+    n_paths = [len_ for _, len_, _ in res]
+    t_filter = sum(med[:n_filter_groups])
+    t_plain = sum(med[n_filter_groups:])
+    t_doc = t_filter + plain_stride * t_plain
+    # per-operation split of one document per core (ms), from the first timed run:
+    # filter groups count once, the plain sample plain_stride times
+    split = {}
+    for i, pr in enumerate(profs):
+        wgt = 1.0 if i < n_filter_groups else float(plain_stride)
+        for k, name in enumerate(_FE_NAMES):
+            if pr[k] > 0:
+                split[name] = split.get(name, 0.0) + wgt * pr[k]
+        for name, k in orc.PROF_SLOTS.items():
+            split[name] = split.get(name, 0.0) + wgt * pr[k]
+    split = {k: round(v / 1e3, 2) for k, v in sorted(split.items(), key=lambda kv: -kv[1])}
     return {
         "value": round(workers * size * size / 1e6 / t_doc, 4),
         "unit": "Mpixel/s",
         "cores": workers,
         "kind": "port",
-        "sample": f"oracle (scalar C++ restatement, one document per core) on a stratified sample of the same "
-                  f"{size}x{size} {paths}-path document: filter group {filter_group} of {n_filter_groups} "
-                  f"({res[0][1]} paths, {t_filter:.1f} s) and every {plain_stride}th plain path ({n_plain} paths, "
-                  f"{t_plain:.1f} s over {parts} cores); estimated {t_doc:.0f} s per document per core; "
-                  f"wall {wall:.1f} s",
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "sample": f"oracle (C++ restatement of the reference path, not the reference binary; built -O3 "
+                  f"-march=native -ffp-contract=off on this host in {t_build:.0f} s), one document part per core: "
+                  f"the {size}x{size} {paths}-path document's {n_filter_groups} filter groups each alone "
+                  f"({sum(n_paths[:n_filter_groups])} paths, {t_filter:.1f} s) and every {plain_stride}th plain "
+                  f"path ({sum(n_paths[n_filter_groups:])} paths over {parts} parts, {t_plain:.1f} s); median of "
+                  f"{runs} runs after 1 warm-up; estimated {t_doc:.0f} s per document per core; wall {wall:.1f} s",
+        "seconds_per_document_per_core": round(t_doc, 2),
+        "split_s_per_document": split,
     }
 
 
@@ -311,7 +370,8 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--paths", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-stride", type=int, default=64, help="CPU sample: every n-th plain path")
+    ap.add_argument("--cpu-stride", type=int, default=128, help="CPU sample: every n-th plain path")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="CPU sample: timed runs (after 1 warm-up)")
     ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline processes (0: min(16, cores))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-docs", type=int, default=512, help="documents of the batched leg (0: skip it)")
@@ -333,7 +393,7 @@ def main() -> int:
     if not args.no_cpu_baseline:
         workers = args.cpu_workers or max(1, min(16, len(os.sched_getaffinity(0))))
         print(f"[bench] cpu baseline: {workers} workers", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.size, args.paths, args.seed, workers, plain_stride=args.cpu_stride)
+        cpu = cpu_baseline(args.size, args.paths, args.seed, workers, plain_stride=args.cpu_stride, runs=args.cpu_runs)
         print(f"[bench] cpu baseline {cpu['value']} Mpixel/s", file=sys.stderr, flush=True)
 
     import torch

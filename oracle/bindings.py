@@ -8,6 +8,7 @@ in place.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -16,7 +17,9 @@ import numpy as np
 from resvg_amd import abi
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "liboracle.so"
+# RVK_ORACLE_LIB names another build of the same sources (the CPU baseline's
+# -O3 -march=native build, made by build_native on the machine that times it)
+LIB_PATH = Path(os.environ.get("RVK_ORACLE_LIB", HERE / "liboracle.so"))
 _lib = None
 
 
@@ -25,7 +28,18 @@ def build() -> Path:
     r = subprocess.run(["make", "-B", "-C", str(HERE), "liboracle.so"], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
-    return LIB_PATH
+    return HERE / "liboracle.so"
+
+
+def build_native(out_dir) -> Path:
+    """Compiles the same sources with -O3 -march=native (still -ffp-contract=off, no
+    fast-math, so every result is the same) into out_dir, for timing on this machine's
+    own cores; the library is specific to this CPU and is not kept in the tree."""
+    out = Path(out_dir) / "liboracle_native.so"
+    r = subprocess.run(["make", "-B", "-C", str(HERE), f"NATIVE_OUT={out}", "native"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle native build failed:\n{r.stdout}\n{r.stderr}")
+    return out
 
 
 def lib() -> C.CDLL:
@@ -78,6 +92,10 @@ def lib() -> C.CDLL:
         L.orc_mask_from_pixmap.restype = None
         L.orc_apply_mask.argtypes = [P, u32, u32, P]
         L.orc_apply_mask.restype = None
+        L.orc_prof_enable.argtypes = [C.c_int]
+        L.orc_prof_enable.restype = None
+        L.orc_prof_read.argtypes = [C.POINTER(f64), u32]
+        L.orc_prof_read.restype = u32
         _lib = L
     return _lib
 
@@ -208,3 +226,18 @@ def mask_from_pixmap(a, luminance: bool = False) -> np.ndarray:
 def apply_mask(a, mask: np.ndarray) -> None:
     assert mask.dtype == np.uint8 and mask.flags.c_contiguous and mask.shape == a.shape[:2]
     lib().orc_apply_mask(_p(a), *_wh(a), mask.ctypes.data)
+
+
+# per-operation time split of the renders since prof_enable(): filter primitive kinds
+# (indices RVK_FE_*), then fills, strokes, layer composites and whole filters
+PROF_SLOTS = {"fill": 32, "stroke": 33, "layer_draw": 34, "filter_total": 35}
+
+
+def prof_enable(on: bool = True):
+    lib().orc_prof_enable(1 if on else 0)
+
+
+def prof_read() -> list:
+    out = (C.c_double * 36)()
+    n = lib().orc_prof_read(out, 36)
+    return list(out)[:n]

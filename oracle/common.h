@@ -5,8 +5,28 @@
 #include <cstdint>
 #include <cstring>
 #include <algorithm>
+#include <chrono>
 
 namespace orc {
+
+// Time split of a render by operation (filter primitive kinds, then fills, strokes,
+// layer composites and whole filters), for the CPU baseline's per-primitive split.
+// Off unless orc_prof_enable(1) was called; a Scope nested in another counts in both.
+namespace prof {
+enum Slot : int { kFill = 32, kStroke, kLayer, kFilter, kSlots };
+extern bool enabled;
+extern double ms[kSlots];
+struct Scope {
+    int slot;
+    std::chrono::steady_clock::time_point t0;
+    explicit Scope(int s) : slot(enabled ? s : -1) {
+        if (slot >= 0) t0 = std::chrono::steady_clock::now();
+    }
+    ~Scope() {
+        if (slot >= 0) ms[slot] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+}  // namespace prof
 
 // Rust `f32 as u8`: saturating, NaN -> 0, truncation toward zero.
 static inline uint8_t as_u8(float v) {

@@ -130,6 +130,7 @@ rvk_light_source transform_light(rvk_light_source l, IRect region, Ts ts) {
 }
 
 ImgP apply_primitive(Ctx &c, const rvk_primitive &p) {
+    prof::Scope timed((int)p.kind);
     const uint32_t cs = p.color_interpolation;
     const IRect region = c.region;
     switch (p.kind) {
@@ -371,6 +372,7 @@ ImgP apply_inner(const rvk_filter &f, const rvk_filter_data &data, Ts ts, const 
 // filter::apply (mod.rs:335-355) + apply_to_canvas (mod.rs:1100-1114)
 int32_t orc_filter_apply(const rvk_filter *filter, const rvk_filter_data *data, rvk_transform ts, uint8_t *px,
                          uint32_t w, uint32_t h) {
+    orc::prof::Scope timed(orc::prof::kFilter);
     Img source;
     source.px.assign(px, px + (size_t)w * h * 4);
     source.w = w; source.h = h;

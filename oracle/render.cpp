@@ -36,12 +36,20 @@ void render_nodes(const rvk_scene &sc, const rvk_group &g, const Ctx &ctx, Ts ts
 // path::render (path.rs:6-24)
 void render_path(const rvk_scene &sc, const rvk_path &p, Ts ts, uint8_t *px, uint32_t w, uint32_t h) {
     if (!p.visible) return;
+    auto fill = [&] {
+        prof::Scope timed(prof::kFill);
+        fill_path_into(sc, p, ts, px, w, h);
+    };
+    auto stroke = [&] {
+        prof::Scope timed(prof::kStroke);
+        stroke_path_into(sc, p, ts, px, w, h);
+    };
     if (!p.paint_order_stroke_first) {
-        fill_path_into(sc, p, ts, px, w, h);
-        stroke_path_into(sc, p, ts, px, w, h);
+        fill();
+        stroke();
     } else {
-        stroke_path_into(sc, p, ts, px, w, h);
-        fill_path_into(sc, p, ts, px, w, h);
+        stroke();
+        fill();
     }
 }
 
@@ -178,6 +186,7 @@ void render_group(const rvk_scene &sc, const rvk_group &g, const Ctx &ctx, Ts ts
                          rvk_transform{lts.sx, lts.ky, lts.kx, lts.sy, lts.tx, lts.ty}, layer.data(), ib.w, ib.h);
     if (g.clip_path >= 0) clip_apply(sc, g.clip_path, lts, layer.data(), ib.w, ib.h, 0);
     if (g.mask >= 0) mask_apply(sc, ctx, g.mask, lts, layer.data(), ib.w, ib.h, 0);
+    prof::Scope timed(prof::kLayer);
     orc_draw_pixmap(px, w, h, ib.x, ib.y, layer.data(), ib.w, ib.h, g.opacity, convert_blend_mode(g.blend_mode));
 }
 
@@ -255,4 +264,25 @@ void orc_mask_from_pixmap(const uint8_t *px, uint32_t w, uint32_t h, uint32_t lu
 void orc_apply_mask(uint8_t *px, uint32_t w, uint32_t h, const uint8_t *mask) {
     std::vector<uint8_t> m(mask, mask + (size_t)w * h);
     apply_mask(px, m);
+}
+
+// ---------------------------------------------------------------------------
+// per-operation time split (CPU baseline only)
+// ---------------------------------------------------------------------------
+namespace orc {
+namespace prof {
+bool enabled = false;
+double ms[kSlots] = {};
+}  // namespace prof
+}  // namespace orc
+
+extern "C" void orc_prof_enable(int on) {
+    orc::prof::enabled = on != 0;
+    std::fill(orc::prof::ms, orc::prof::ms + orc::prof::kSlots, 0.0);
+}
+
+extern "C" uint32_t orc_prof_read(double *out, uint32_t n) {
+    const uint32_t k = std::min<uint32_t>(n, orc::prof::kSlots);
+    for (uint32_t i = 0; i < k; i++) out[i] = orc::prof::ms[i];
+    return k;
 }

@@ -231,9 +231,9 @@ rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
     RVK_TRY(scratch_alloc(ctx, bytes, &tmp));
     uint32_t *a = (uint32_t *)px, *b = (uint32_t *)tmp;
     rvk_status st = RVK_OK;
-    // algorithmic bytes of a round: the image read once and written once (each of the
-    // reference's passes rounds to u8, so a round is a full pass over the image)
-    const double round_bytes = 8.0 * (double)w * h;
+    // algorithmic bytes: the image read once and written once per blur (8 B/px,
+    // whatever the pass count), split evenly over the five rounds
+    const double round_bytes = 8.0 * (double)w * h / 5.0;
     for (int i = 0; i < 5 && st == RVK_OK; i++) {
         int rh = (bh[i] - 1) / 2, rv = (bv[i] - 1) / 2;
         if (rh <= kMaxFusedR && rv <= kMaxFusedR) {

@@ -40,22 +40,36 @@ struct PieceDev {
     int32_t x, dx, first_y, last_y;
 };
 
-constexpr int kGroupRows = 4;  // pixel rows per entry of a job's edge-window table
+// Coverage is stored per job as a grid of kTile x kTile pixel tiles aligned to the
+// target pixmap (so that one compositor wave, an 8 x 8 pixel square, reads one tile
+// of each job).  A tile whose pixels all have coverage 0 or all 255 is only an entry
+// of the job's tile table; the other (partial) tiles hold their 64 coverage bytes in
+// the job's slot region of the arena.
+constexpr int kTile = 8;
+constexpr int kBandRows = kTile;    // pixel rows of one coverage band (one tile row)
+constexpr int kChunkTiles = 1024;   // tiles of one band a coverage block handles at a time
+// Tile table entries (u16): empty, full, or kTileSlot0 + k for the band's k-th partial
+// tile, whose bytes (8 rows of 8, row-major) sit at slot (band * tiles_x + k).
+constexpr uint32_t kTileEmpty = 0, kTileFull = 1, kTileSlot0 = 2;
+constexpr uint32_t kMaxTilesX = 65279u;  // widest job grid (a u16 table entry and tile state address its slots)
 // crossings per sub-scanline that the coverage kernel's variants hold (more keys,
 // more LDS per wave, fewer waves per CU)
 constexpr uint32_t kCoverageKeys[3] = {128, 256, 512};
 
 // A coverage job computes u8 coverage for the pixel rect [x0, x0+w) x [y0, y0+h)
-// of its pixmap (coordinates of the edge space) into cov[(y - y0) * stride + (x - x0)],
-// stride = w rounded up to 4.  For every group of kGroupRows pixel rows the window
-// table holds the range of the job's edges that can touch the group.
+// of its pixmap (coordinates of the edge space; coverage is 0 outside the rect).
+// Its tile grid starts at (gx0, gy0) (edge space, a multiple of kTile in target
+// pixels) and has tiles_x x tiles_y tiles.  For every band (tile row) the window table
+// holds the range of the job's edges that can touch the band.
 struct CovJob {
     int32_t x0, y0, w, h;
+    int32_t gx0, gy0;
+    uint32_t tiles_x, tiles_y;
     uint32_t edge_offset, edge_count;
-    uint32_t window_offset;  // into the window table (uint2 lo, hi per row group)
+    uint32_t window_offset;  // into the window table (uint2 lo, hi per band)
     uint32_t rule, aa;
-    uint32_t row_start;      // prefix over jobs of h (wave index of the first row)
-    uint64_t cov_offset;     // byte offset into the coverage arena
+    uint32_t table_offset;   // first u16 entry of the job's tile table (tiles_x * tiles_y, row-major)
+    uint64_t cov_offset;     // byte offset of the job's slot region (tiles_x * tiles_y * 64 bytes)
 };
 
 // An anti-aliased hairline segment set up for per-pixel evaluation: the state of
@@ -113,18 +127,20 @@ struct ShaderParams : ShaderHead {
 // (the layer pixel is built in registers, then composited with opacity and blend
 // mode in f32 like draw_pixmap).
 // What the compositor needs of a job, copied into the draw item so that one load of
-// the item brings it: the rect (layer space; w = h = 0 for no job) and where the
-// coverage bytes are, or for a hairline job its tile grid.
+// the item brings it.  Coverage job: its tile grid in target tiles (x0, y0 = origin,
+// w x h tiles; w = h = 0 for no job), its tile table and slot region.  Hairline job:
+// its rect in layer pixels and its bin tile grid.
 struct JobRef {
-    int32_t x0, y0, w, h;
-    uint32_t stride;       // coverage row stride (w rounded up to 4); hairline: tiles_x
-    uint32_t tile_offset;  // hairline: first entry of the job's tile offsets
-    uint64_t cov_offset;   // byte offset of the coverage in the arena
+    int32_t x0, y0;
+    uint32_t w, h;
+    uint32_t table;        // coverage: first tile-table entry; hairline: first entry of its tile offsets
+    uint32_t stride;       // hairline: bin tiles per row
+    uint64_t cov_offset;   // coverage: byte offset of the job's slot region in the arena
 };
 
 struct DrawItem {
     int32_t x0, y0, x1, y1;     // target-pixel rect touched (clipped to the target)
-    int32_t fill_job, stroke_job;  // coverage jobs (-1: none); their rects are in layer space
+    int32_t fill_job, stroke_job;  // coverage jobs (-1: none)
     int32_t lx, ly;             // layer origin in target pixels (0,0 for direct items)
     int32_t fill_shader, stroke_shader;
     uint32_t stroke_first;
@@ -134,6 +150,17 @@ struct DrawItem {
     uint32_t stroke_hair;       // 1: stroke_job is a hairline job
     uint32_t pad_;
     JobRef fj, sj;              // the fill and stroke jobs
+};
+
+// What the compositor's culling pass reads of an item, one lane per item: its target
+// rect in tiles and the two jobs' tile grids and tables.
+struct DrawCull {
+    int32_t tx0, ty0, tx1, ty1;  // target tiles touched: [tx0, tx1) x [ty0, ty1)
+    int32_t fx0, fy0;            // fill job grid origin (target tiles)
+    uint32_t fw, fh, ftable;     // its size (0 x 0 for none) and table
+    int32_t sx0, sy0;
+    uint32_t sw, sh, stable;     // the stroke job (a hairline: sw = sh = 0)
+    uint32_t hair;               // 1: a hairline stroke (drawn wherever the rect meets the tile)
 };
 
 struct Pt { float x, y; };
@@ -169,15 +196,16 @@ struct EdgeSet {
 
 // A coverage job prepared off the walk: the CovJob (offsets relative to the job),
 // its edges sorted by first row with piece slots relative to the job, the coverage
-// kernel's view of them, and per group of kGroupRows rows the edge window and the
-// coverage kernel variant.  add_job places it in a draw list.
+// kernel's view of them, and per band the edge window and the coverage kernel
+// variant.  add_job places it in a draw list.
 struct JobPrep {
     bool ok = false;
+    bool too_wide = false;           // the tile grid is wider than kMaxTilesX
     CovJob j{};
     std::vector<EdgeRec> edges;
     std::vector<EdgeCov> ecov;
     std::vector<uint2> windows;
-    std::vector<uint8_t> tier;       // per row group: index into kCoverageKeys
+    std::vector<uint8_t> tier;       // per band: index into kCoverageKeys
     uint32_t tier_count[3] = {0, 0, 0};
     uint64_t pieces = 0;             // piece slots of all edges
     uint64_t piece_base = 0;         // first piece slot in the draw list (add_job)
@@ -195,11 +223,13 @@ struct HairPrep {
     uint32_t bin_base = 0, line_base = 0;  // positions in the draw list (add_hair_job)
 };
 
-// The coverage job of an edge set drawn into the pixels `clip` of its pixmap: the
+// The coverage job of an edge set drawn into the pixels `clip` of its pixmap, whose
+// origin sits at target pixel (lx, ly): the tile grid aligned to the target, the
 // edges sorted by first row (stable, so equal rows keep tiny-skia's edge order), the
-// edge window of each row group and the coverage kernel variant that holds its
-// crossings.  Leaves P.ok false when nothing is covered.
-void prep_job(const EdgeSet &es, uint32_t rule, bool aa, rvkh::IRect clip, JobPrep &P);
+// edge window of each band and the coverage kernel variant that holds its
+// crossings.  Leaves P.ok false when nothing is covered, and also sets P.too_wide
+// when the job's tile grid is wider than a table entry addresses.
+void prep_job(const EdgeSet &es, uint32_t rule, bool aa, rvkh::IRect clip, int32_t lx, int32_t ly, JobPrep &P);
 // The hairline job of prepared lines, binned into tiles of the part of their pixel
 // bounds inside `clip`.  Leaves P.ok false when nothing is drawn.
 void prep_hair(std::vector<HairLine> &&lines, rvkh::IRect clip, HairPrep &P);

@@ -5,31 +5,33 @@
 // update loop (Skia's forward differencing in 16.16 fixed point) from the state the
 // host left after the first piece, writing every non-empty line piece in order.
 //
-// Coverage (k_coverage): one 64-lane wave per (coverage job, pixel row).  The row's
-// edge window comes from the job's table; each lane takes an edge, binary-searches
-// its pieces for the row and produces the crossings of the row's sub-scanlines.
-// Per sub-scanline the crossings (keys: x, edge id, winding) are sorted with a
-// register-resident bitonic network (up to 8 keys per lane), the windings are
-// scanned into span events (tiny-skia's walk_edges spans), and the SuperBlitter
-// contributions (16 per covered sub-column of a partial pixel, 64 or 63 for a fully
-// covered pixel) are accumulated per 1024-pixel window in LDS, prefix-summed,
-// clamped to 255 and written as u8.
+// Coverage (k_coverage): one 64-lane wave per (coverage job, band of 8 pixel rows).
+// Per row, the band's edge window comes from the job's table; each lane takes an
+// edge, binary-searches its pieces for the row and produces the crossings of the
+// row's sub-scanlines.  Per sub-scanline the crossings (keys: x, edge id, winding)
+// are sorted with a register-resident bitonic network (up to 8 keys per lane), the
+// windings are scanned into span events (tiny-skia's walk_edges spans), and the
+// SuperBlitter contributions (16 per covered sub-column of a partial pixel, 64 or 63
+// for a fully covered pixel) are accumulated per 512-pixel window in LDS,
+// prefix-summed and clamped to 255.  Coverage leaves the kernel as 8 x 8 tiles
+// aligned to the target: a u16 table entry per tile (empty, full, or the slot of a
+// partial tile's 64 bytes), so only tiles that an edge crosses cost memory.
 //
-// Compositing (k_draw): one thread per target pixel of a 64x4 tile, looping over
-// the draw items in order with the pixel in a register: fill/stroke coverage ->
-// gradient or solid shade (tiny-skia lowp stages) -> SourceOver into the target
-// (direct items) or into an elided layer pixel that is then composited in f32 with
-// the group's opacity and blend mode (draw_pixmap semantics).
+// Compositing (k_draw): one thread per target pixel, one wave per 8 x 8 tile,
+// looping over the draw items in order with the pixel in a register: tile-table
+// entry -> coverage -> gradient or solid shade (tiny-skia lowp stages) -> SourceOver
+// into the target (direct items) or into an elided layer pixel that is then
+// composited in f32 with the group's opacity and blend mode (draw_pixmap semantics).
 #include "rvk_internal.h"
 #include "blend_ops.h"
 #include "raster.h"
 
 using namespace rvkd;
 using rvkr::CovJob;
+using rvkr::DrawCull;
 using rvkr::DrawItem;
 using rvkr::EdgeCov;
 using rvkr::EdgeRec;
-using rvkr::HairJob;
 using rvkr::HairLine;
 using rvkr::PieceDev;
 using rvkr::ShaderParams;
@@ -143,13 +145,13 @@ __global__ __launch_bounds__(256) void k_expand(ExpandArgs a) {
 // coverage
 // ---------------------------------------------------------------------------
 struct CovArgs {
-    const uint2 *groups;  // (job, first row in the job) of each row group this launch handles
+    const uint2 *bands;  // (job, band) of each block of this launch
     const CovJob *jobs;
-    uint32_t njobs;
     const EdgeCov *edges;
     const uint2 *windows;
     const PieceDev *pieces;
     const uint32_t *npieces;
+    uint16_t *table;
     uint8_t *arena;
     int32_t *overflow;
 };
@@ -272,163 +274,333 @@ __device__ __forceinline__ void spans_of(uint64_t (*keys)[MK], const int *n, boo
     }
 }
 
+// tile state of the coverage kernel (u16): 0..255 a tile whose rows so far are all
+// that value, kStatePartial + k a partial tile holding slot k
+constexpr uint32_t kStatePartial = 256u;
+static_assert(rvkr::kMaxTilesX + kStatePartial <= 65536u, "a partial tile's state fits 16 bits");
+static_assert(kWin == 8 * kWave, "a lane holds one 8-pixel tile row of a window");
+
 // MK: crossings per sub-scanline the launch can hold (128, 256 or 512; the keys take
 // 4, 8 or 16 KB of LDS, so the smaller variants run two to three times the waves per
-// CU).  The host sends a row group to the smallest variant that holds the number of
-// edges whose rows meet the group, an upper bound on any of its sub-scanlines'
-// crossings.  Block b handles row groups[b / kGroupRows].y + b % kGroupRows of job
-// groups[b / kGroupRows].x.
+// CU).  The host sends a band to the smallest variant that holds the number of edges
+// whose rows meet the band, an upper bound on any of its sub-scanlines' crossings.
+//
+// Block b handles band bands[b].y (8 pixel rows) of job bands[b].x.  At the start of
+// the band the edges of its window that meet its rows are gathered into registers
+// (lane l holds active edges l, l + 64, ...) with the index of the piece each one is
+// on; row by row every active edge steps its piece forward and emits its crossings,
+// the sub-scanlines are sorted together and turned into spans, and the spans are
+// accumulated per 512-pixel window in LDS (each window visits only the spans that
+// reach it), prefix-summed and clamped to 255.  Each lane then holds one 8-pixel row
+// of a tile, which is tracked as uniform (every pixel so far one value) or partial: a
+// tile turns partial at the first row that differs, takes the band's next slot, and
+// from then on its rows are written there (the earlier, uniform rows are filled in at
+// that point); at the end of the band the tile table gets empty, full or the slot.
 template <int MK>
 __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
+    constexpr int J = MK / kWave;  // active edges per lane
     __shared__ uint64_t keys[kMaxSub][MK];  // crossings; reused for span events
-    __shared__ __align__(16) int32_t acc[kWin + 4];  // acc (low 16 bits) + run difference (high 16 bits)
+    __shared__ __align__(16) int32_t acc[kWin + 8];  // acc (low 16 bits) + run difference (high 16 bits)
+    __shared__ uint16_t tstate[rvkr::kChunkTiles];
     const int lane = threadIdx.x;
-    // the group's first row picks the job (a job's last group can be partial)
-    const uint2 gs = a.groups[blockIdx.x / rvkr::kGroupRows];
-    const CovJob job = a.jobs[gs.x];
-    const int32_t r = (int32_t)gs.y + (int32_t)(blockIdx.x % rvkr::kGroupRows);
-    if (r < 0 || r >= job.h) return;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint2 bj = a.bands[blockIdx.x];
+    const CovJob job = a.jobs[bj.x];
+    const uint32_t band = bj.y;
     const int shift = job.aa ? 2 : 0;
-    const int nsub = 1 << shift;
-    const int32_t sy0 = (job.y0 + r) << shift;
-    const uint32_t stride = ((uint32_t)job.w + 3u) & ~3u;
-    uint8_t *out = a.arena + job.cov_offset + (size_t)r * stride;
+    const int nsub_aa = 1 << shift;
     const bool evenodd = job.rule == RVK_FILL_EVENODD;
+    const int32_t row0 = job.gy0 + (int32_t)band * rvkr::kBandRows;
+    const int32_t ry0 = max(row0, job.y0), ry1 = min(row0 + rvkr::kBandRows, job.y0 + job.h);  // covered rows
+    uint16_t *tab = a.table + job.table_offset + (size_t)band * job.tiles_x;
+    uint8_t *slots = a.arena + job.cov_offset + (size_t)band * job.tiles_x * 64u;
+    const int32_t rx0 = job.x0, rx1 = job.x0 + job.w;  // covered columns
+    const EdgeCov *edges = a.edges + job.edge_offset;
+    const uint32_t *npieces = a.npieces + job.edge_offset;
 
-    // ---- gather the crossings of the row's sub-scanlines ----
-    const uint2 win = a.windows[job.window_offset + (uint32_t)r / rvkr::kGroupRows];
-    int cnt[kMaxSub] = {0, 0, 0, 0};
-    for (uint32_t base = win.x; base < win.y; base += kWave) {
-        const uint32_t i = base + lane;
-        uint64_t key[kMaxSub];
-        bool act[kMaxSub] = {false, false, false, false};
-        if (i < win.y) {
-            const EdgeCov e = a.edges[job.edge_offset + i];
-            if (e.first_y <= sy0 + nsub - 1 && e.ymax >= sy0) {
-                const PieceDev *pc = a.pieces + e.piece_offset;
-                const uint32_t n = a.npieces[job.edge_offset + i];
-                uint32_t lo = 0, hi = n;  // first piece with last_y >= sy0
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (pc[mid].last_y < sy0) lo = mid + 1;
-                    else hi = mid;
-                }
-                uint32_t k = lo;
-                for (int s = 0; s < nsub; s++) {
-                    const int32_t sy = sy0 + s;
-                    while (k < n && pc[k].last_y < sy) k++;
-                    if (k < n && pc[k].first_y <= sy) {
-                        const PieceDev p = pc[k];
-                        const int32_t x = (int32_t)((uint32_t)p.x + (uint32_t)(sy - p.first_y) * (uint32_t)p.dx);
-                        key[s] = ((uint64_t)((uint32_t)x ^ 0x80000000u) << 32) | ((uint64_t)(uint32_t)e.id << 1) |
-                                 (e.winding > 0 ? 1u : 0u);
-                        act[s] = true;
+    for (int i = lane; i < kWin + 8; i += kWave) acc[i] = 0;  // each window clears it behind its read
+    // ---- the band's active edges: the window's edges whose rows meet the band ----
+    const int32_t sb0 = ry0 * nsub_aa, sb1 = ry1 * nsub_aa - 1;  // covered sub-scanlines
+    uint32_t eidx[J], ek[J];  // an active edge (index in the job) and the piece it is on
+    int nact = 0;
+    const uint2 win = a.windows[job.window_offset + band];
+    {
+        uint32_t *list = reinterpret_cast<uint32_t *>(&keys[0][0]);
+        for (uint32_t base = win.x; base < win.y; base += kWave) {
+            const uint32_t i = base + (uint32_t)lane;
+            bool meets = false;
+            if (i < win.y) {
+                const EdgeCov e = edges[i];
+                meets = e.first_y <= sb1 && e.ymax >= sb0;
+            }
+            const uint64_t m = __ballot(meets);
+            const int pos = nact + __popcll(m & below);
+            if (meets && pos < MK) list[pos] = i;
+            nact += __popcll(m);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int e = lane + kWave * j;
+            eidx[j] = e < nact && nact <= MK ? list[e] : 0u;
+            ek[j] = 0;
+        }
+        __syncthreads();  // the key area is free again
+        // the first piece that reaches the band's first sub-scanline (an edge that
+        // starts inside the band starts on its first piece)
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            if (lane + kWave * j < nact && nact <= MK) {
+                const EdgeCov e = edges[eidx[j]];
+                if (e.first_y < sb0) {
+                    const PieceDev *pc = a.pieces + e.piece_offset;
+                    uint32_t lo = 0, hi = npieces[eidx[j]];
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (pc[mid].last_y < sb0) lo = mid + 1;
+                        else hi = mid;
                     }
+                    ek[j] = lo;
                 }
             }
         }
-        for (int s = 0; s < nsub; s++) {
-            const uint64_t m = __ballot(act[s]);
-            const int pos = cnt[s] + __popcll(m & ((1ull << lane) - 1ull));
-            if (act[s] && pos < MK) keys[s][pos] = key[s];
-            cnt[s] += __popcll(m);
-        }
     }
-    __syncthreads();
-    // ---- sort the sub-scanlines together and turn them into span events ----
-    int nev[kMaxSub] = {0, 0, 0, 0};
-    int nmax = 0;
-    for (int s = 0; s < nsub; s++) {
-        if (cnt[s] > MK) {
-            if (lane == 0) atomicAdd(a.overflow, 1);
-            cnt[s] = MK;
-        }
-        nmax = max(nmax, cnt[s]);
-    }
-    // keys per lane: the smallest network that holds the fullest sub-scanline (cnt is
-    // at most MK, so the branches past MK are never taken)
-    constexpr int K4 = MK >= 256 ? 4 : 2, K8 = MK >= 512 ? 8 : K4;
-    if (nsub == 1) {
-        if (nmax == 0) {}
-        else if (nmax <= 64) spans_of<1, 1, MK>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 128) spans_of<2, 1, MK>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 256) spans_of<K4, 1, MK>(keys, cnt, evenodd, nev, lane);
-        else spans_of<K8, 1, MK>(keys, cnt, evenodd, nev, lane);
-    } else {
-        if (nmax == 0) {}
-        else if (nmax <= 64) spans_of<1, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 128) spans_of<2, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
-        else if (nmax <= 256) spans_of<K4, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
-        else spans_of<K8, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
-    }
-    __syncthreads();
-    // ---- accumulate per window, prefix-sum, clamp, write u8 ----
-    for (int32_t wx0 = 0; wx0 < job.w; wx0 += kWin) {
-        const int32_t ww = min(kWin, job.w - wx0);
-        for (int i = lane; i <= ww; i += kWave) acc[i] = 0;
-        __syncthreads();
-        const int32_t colbase = (job.x0 + wx0) << shift;
-        const int32_t colmax = ww << shift;
-        for (int s = 0; s < nsub; s++) {
-            const int32_t *ev = reinterpret_cast<const int32_t *>(keys[s]);
-            const int32_t maxv = job.aa ? (64 - (((s & 3) + 1) >> 2)) : 255;
-            for (int k = lane; 2 * k + 1 < nev[s]; k += kWave) {
-                int32_t L = ev[2 * k] - colbase, R = ev[2 * k + 1] - colbase;
-                if (R <= L) continue;
-                L = max(L, 0);
-                R = min(R, colmax);
-                if (R <= L) continue;
-                if (!job.aa) {
-                    atomicAdd(&acc[L], 255 << 16);
-                    atomicAdd(&acc[R], -(255 << 16));
-                    continue;
-                }
-                const int32_t fb = L & 3, fe = R & 3;
-                int32_t c = (R >> 2) - (L >> 2) - 1;
-                if (c < 0) {
-                    atomicAdd(&acc[L >> 2], (fe - fb) * 16);
+
+    uint32_t nslots = 0;  // partial tiles of the band so far
+    for (uint32_t ct0 = 0; ct0 < job.tiles_x; ct0 += rvkr::kChunkTiles) {
+        const uint32_t ctn = min((uint32_t)rvkr::kChunkTiles, job.tiles_x - ct0);
+        const int32_t cx0 = job.gx0 + (int32_t)ct0 * rvkr::kTile;  // first column of the chunk
+        const int32_t cw = (int32_t)ctn * rvkr::kTile;
+        // a chunk after the first walks the band's rows again from its start
+        uint32_t k0[J];
+#pragma unroll
+        for (int j = 0; j < J; j++) k0[j] = ek[j];
+        for (int r = 0; r < rvkr::kBandRows; r++) {
+            const int32_t y = row0 + r;
+            const bool row_in = y >= ry0 && y < ry1;
+            int nev[kMaxSub] = {0, 0, 0, 0};
+            if (row_in) {
+                // ---- the crossings of the row's sub-scanlines ----
+                const int32_t sy0 = y * nsub_aa;
+                int cnt[kMaxSub] = {0, 0, 0, 0};
+                // the crossings of edge i on the row's sub-scanlines, stepping its piece
+                // index k forward; compacted into the key areas in lane order
+                auto gather = [&](bool valid, uint32_t i, uint32_t &k) {
+                    uint64_t key[kMaxSub];
+                    bool act[kMaxSub] = {false, false, false, false};
+                    if (valid) {
+                        const EdgeCov e = edges[i];
+                        const PieceDev *pc = a.pieces + e.piece_offset;
+                        const uint32_t n = npieces[i];
+                        if (e.first_y <= sy0 + nsub_aa - 1 && e.ymax >= sy0 && k < n) {
+                            PieceDev p = pc[k];
+                            for (int s = 0; s < nsub_aa; s++) {
+                                const int32_t sy = sy0 + s;
+                                while (p.last_y < sy && k + 1 < n) p = pc[++k];
+                                if (p.first_y <= sy && p.last_y >= sy) {
+                                    const int32_t x = (int32_t)((uint32_t)p.x + (uint32_t)(sy - p.first_y) * (uint32_t)p.dx);
+                                    key[s] = ((uint64_t)((uint32_t)x ^ 0x80000000u) << 32) |
+                                             ((uint64_t)(uint32_t)e.id << 1) | (e.winding > 0 ? 1u : 0u);
+                                    act[s] = true;
+                                }
+                            }
+                        }
+                    }
+                    for (int s = 0; s < nsub_aa; s++) {
+                        const uint64_t m = __ballot(act[s]);
+                        const int pos = cnt[s] + __popcll(m & below);
+                        if (act[s] && pos < MK) keys[s][pos] = key[s];
+                        cnt[s] += __popcll(m);
+                    }
+                };
+                if (nact <= MK) {
+                    // the band's active edges, each on the piece it reached
+#pragma unroll
+                    for (int j = 0; j < J; j++) {
+                        if (kWave * j >= nact) break;
+                        gather(lane + kWave * j < nact, eidx[j], k0[j]);
+                    }
                 } else {
-                    int32_t st = L >> 2;
-                    if (fb == 0) c += 1;
-                    else { atomicAdd(&acc[st], (4 - fb) * 16); st += 1; }
-                    if (c > 0) {
-                        atomicAdd(&acc[st], maxv << 16);
-                        atomicAdd(&acc[st + c], -(maxv << 16));
+                    // more edges meet the band than the registers hold (many short edges,
+                    // such as the arcs of round joins): every edge of the window, its
+                    // piece found by binary search
+                    for (uint32_t base = win.x; base < win.y; base += kWave) {
+                        const uint32_t i = base + (uint32_t)lane;
+                        uint32_t k = 0;
+                        if (i < win.y) {
+                            const EdgeCov e = edges[i];
+                            if (e.first_y < sy0) {
+                                const PieceDev *pc = a.pieces + e.piece_offset;
+                                uint32_t hi = npieces[i];
+                                while (k < hi) {
+                                    const uint32_t mid = (k + hi) >> 1;
+                                    if (pc[mid].last_y < sy0) k = mid + 1;
+                                    else hi = mid;
+                                }
+                            }
+                        }
+                        gather(i < win.y, i, k);
                     }
-                    if (fe) atomicAdd(&acc[R >> 2], fe * 16);
                 }
+                __syncthreads();
+                // ---- sort the sub-scanlines together and turn them into span events ----
+                int nmax = 0;
+                for (int s = 0; s < nsub_aa; s++) {
+                    if (cnt[s] > MK) {
+                        if (lane == 0) atomicAdd(a.overflow, 1);
+                        cnt[s] = MK;
+                    }
+                    nmax = max(nmax, cnt[s]);
+                }
+                // keys per lane: the smallest network that holds the fullest sub-scanline
+                // (cnt is at most MK, so no branch needs more than MK / 64 keys per lane)
+                constexpr int K4 = MK >= 256 ? 4 : 2, K8 = MK >= 512 ? 8 : K4;
+                static_assert(K8 * kWave <= MK && K4 * kWave <= MK, "a network never reads past the key area");
+                if (nsub_aa == 1) {
+                    if (nmax == 0) {}
+                    else if (nmax <= 64) spans_of<1, 1, MK>(keys, cnt, evenodd, nev, lane);
+                    else if (nmax <= 128) spans_of<2, 1, MK>(keys, cnt, evenodd, nev, lane);
+                    else if (nmax <= 256) spans_of<K4, 1, MK>(keys, cnt, evenodd, nev, lane);
+                    else spans_of<K8, 1, MK>(keys, cnt, evenodd, nev, lane);
+                } else {
+                    if (nmax == 0) {}
+                    else if (nmax <= 64) spans_of<1, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+                    else if (nmax <= 128) spans_of<2, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+                    else if (nmax <= 256) spans_of<K4, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+                    else spans_of<K8, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+                }
+                __syncthreads();
+            }
+            const int nsub = row_in ? nsub_aa : 0;
+            int kst[kMaxSub] = {0, 0, 0, 0};  // per sub-scanline: the first span not yet behind the windows
+            for (int32_t wx0 = 0; wx0 < cw; wx0 += kWin) {
+                const int32_t ww = min(kWin, cw - wx0);  // a multiple of kTile
+                // ---- the spans that reach this window, cut to it and to the job's
+                // columns (coverage is 0 outside them) ----
+                const int32_t wabs = cx0 + wx0;
+                const int32_t colbase = wabs * nsub_aa;
+                const int32_t wend = colbase + ww * nsub_aa;  // first sub-column past the window
+                const int32_t lo = max(0, rx0 - wabs) * nsub_aa, hi = max(0, min(ww, rx1 - wabs)) * nsub_aa;
+                for (int s = 0; s < nsub; s++) {
+                    const int32_t *ev = reinterpret_cast<const int32_t *>(keys[s]);
+                    const int32_t maxv = job.aa ? (64 - (((s & 3) + 1) >> 2)) : 255;
+                    const int nsp = nev[s] >> 1;
+                    for (int k0s = kst[s]; k0s < nsp; k0s += kWave) {
+                        const int k = k0s + lane;
+                        int32_t La = 0, Ra = 0;
+                        if (k < nsp) {
+                            La = ev[2 * k];
+                            Ra = ev[2 * k + 1];
+                        }
+                        // spans are ordered: those ending at or before this window's end
+                        // are behind every later window
+                        kst[s] += __popcll(__ballot(k < nsp && Ra <= wend));
+                        const bool beyond = k >= nsp || La >= wend;
+                        if (!beyond && hi > lo) {
+                            int32_t L = La - colbase, R = Ra - colbase;
+                            L = max(L, lo);
+                            R = min(R, hi);
+                            if (R > L) {
+                                if (!job.aa) {
+                                    atomicAdd(&acc[L], 255 << 16);
+                                    atomicAdd(&acc[R], -(255 << 16));
+                                } else {
+                                    const int32_t fb = L & 3, fe = R & 3;
+                                    int32_t c = (R >> 2) - (L >> 2) - 1;
+                                    if (c < 0) {
+                                        atomicAdd(&acc[L >> 2], (fe - fb) * 16);
+                                    } else {
+                                        int32_t st = L >> 2;
+                                        if (fb == 0) c += 1;
+                                        else { atomicAdd(&acc[st], (4 - fb) * 16); st += 1; }
+                                        if (c > 0) {
+                                            atomicAdd(&acc[st], maxv << 16);
+                                            atomicAdd(&acc[st + c], -(maxv << 16));
+                                        }
+                                        if (fe) atomicAdd(&acc[R >> 2], fe * 16);
+                                    }
+                                }
+                            }
+                        }
+                        if (__ballot(!beyond) == 0 || __ballot(k < nsp && La >= wend) != 0) break;
+                    }
+                }
+                __syncthreads();
+                // ---- prefix sum: lane l holds pixels [8 l, 8 l + 8) of the window, one
+                // tile row; the window's accumulator is cleared behind the read ----
+                const int32_t p0 = 8 * lane;
+                const bool valid = p0 < ww;
+                const int4 qa = *reinterpret_cast<const int4 *>(&acc[p0]);
+                const int4 qb = *reinterpret_cast<const int4 *>(&acc[p0 + 4]);
+                *reinterpret_cast<int4 *>(&acc[p0]) = make_int4(0, 0, 0, 0);
+                *reinterpret_cast<int4 *>(&acc[p0 + 4]) = make_int4(0, 0, 0, 0);
+                if (lane == 0) acc[kWin] = 0;
+                const int32_t packed8[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+                int v[8], d[8], loc = 0;
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int32_t packed = valid ? packed8[q] : 0;
+                    v[q] = packed & 0xffff;
+                    d[q] = packed >> 16;
+                    loc += d[q];
+                }
+                const int incl = wave_incl_scan(loc, lane);
+                int run = incl - loc;
+                uint32_t w0 = 0, w1 = 0;
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    run += d[q];
+                    const uint32_t c = (uint32_t)min(255, v[q] + run);
+                    if (q < 4) w0 |= c << (8 * q);
+                    else w1 |= c << (8 * (q - 4));
+                }
+                // ---- the tile of this row of 8 pixels ----
+                const uint32_t t = (uint32_t)wx0 / rvkr::kTile + (uint32_t)lane;  // tile in the chunk
+                const uint32_t v0 = w0 & 0xffu;
+                const bool tuni = w0 == v0 * 0x01010101u && w1 == w0;
+                const uint32_t st = valid && r > 0 ? (uint32_t)tstate[t] : 0u;
+                const bool was_partial = r > 0 && st >= kStatePartial;
+                const bool fresh = valid && !was_partial && (r == 0 ? !tuni : !(tuni && v0 == st));
+                const uint64_t m = __ballot(fresh);
+                const uint32_t k = was_partial ? st - kStatePartial : nslots + (uint32_t)__popcll(m & below);
+                nslots += (uint32_t)__popcll(m);
+                if (valid && (was_partial || fresh)) {
+                    uint2 *tp = reinterpret_cast<uint2 *>(slots + (size_t)k * 64u);
+                    tp[r] = make_uint2(w0, w1);
+                    if (fresh) {  // the tile's earlier rows were all st
+                        const uint32_t u = st * 0x01010101u;
+                        for (int rr = 0; rr < r; rr++) tp[rr] = make_uint2(u, u);
+                        tstate[t] = (uint16_t)(kStatePartial + k);
+                    }
+                } else if (valid && r == 0) {
+                    tstate[t] = (uint16_t)v0;
+                }
+                __syncthreads();
             }
         }
-        __syncthreads();
-        int carry = 0;
-        for (int32_t base = 0; base < ww; base += 4 * kWave) {
-            const int32_t p0 = base + 4 * lane;
-            // one 16-byte LDS read per lane (acc is 16-byte aligned; p0 + 3 <= kWin + 3
-            // stays inside the padded array, entries past ww are zeroed below)
-            const int4 q4 = *reinterpret_cast<const int4 *>(&acc[p0]);
-            const int32_t packed4[4] = {q4.x, q4.y, q4.z, q4.w};
-            int v[4], d[4], loc = 0;
-            for (int q = 0; q < 4; q++) {
-                const int32_t packed = p0 + q < ww ? packed4[q] : 0;
-                v[q] = packed & 0xffff;
-                d[q] = packed >> 16;
-                loc += d[q];
+        // ---- the chunk's tile table entries ----
+        for (uint32_t t0 = 0; t0 < ctn; t0 += kWave) {
+            const uint32_t t = t0 + (uint32_t)lane;
+            const bool valid = t < ctn;
+            const uint32_t st = valid ? (uint32_t)tstate[t] : 0u;
+            // a uniform tile of another value than 0 or 255 (rows of different sub-row
+            // spans, or a partial pixel value everywhere) gets a slot of its own
+            const bool fill = valid && st < kStatePartial && st != 0u && st != 255u;
+            const uint64_t m = __ballot(fill);
+            const uint32_t k = nslots + (uint32_t)__popcll(m & below);
+            nslots += (uint32_t)__popcll(m);
+            uint32_t e = rvkr::kTileEmpty;
+            if (st >= kStatePartial) {
+                e = rvkr::kTileSlot0 + (st - kStatePartial);
+            } else if (fill) {
+                uint4 *tp = reinterpret_cast<uint4 *>(slots + (size_t)k * 64u);
+                const uint32_t v4 = st * 0x01010101u;
+                for (int q = 0; q < 4; q++) tp[q] = make_uint4(v4, v4, v4, v4);
+                e = rvkr::kTileSlot0 + k;
+            } else if (st == 255u) {
+                e = rvkr::kTileFull;
             }
-            const int incl = wave_incl_scan(loc, lane);
-            int run = carry + incl - loc;
-            uint32_t word = 0;
-            for (int q = 0; q < 4; q++) {
-                run += d[q];
-                word |= (uint32_t)min(255, v[q] + run) << (8 * q);
-            }
-            const int32_t gx = wx0 + p0;
-            if (gx + 3 < job.w) {
-                *(uint32_t *)(out + gx) = word;
-            } else {
-                for (int q = 0; q < 4; q++)
-                    if (gx + q < job.w) out[gx + q] = (uint8_t)(word >> (8 * q));
-            }
-            carry += __shfl(incl, kWave - 1, kWave);
+            if (valid) tab[ct0 + t] = (uint16_t)e;
         }
         __syncthreads();
     }
@@ -520,11 +692,22 @@ __device__ __forceinline__ uint32_t paint_px(const ShaderParams &sp, const Shade
     return over_lowp(src, c, d, sh.as_source != 0);
 }
 
-// coverage of a job at layer pixel (x, y); 0 outside its rect (and for no job, whose
-// rect is empty)
-__device__ __forceinline__ uint32_t ref_cov(const JobRef &r, const uint8_t *arena, int32_t x, int32_t y) {
-    const uint32_t cx = (uint32_t)(x - r.x0), cy = (uint32_t)(y - r.y0);
-    return cx < (uint32_t)r.w && cy < (uint32_t)r.h ? arena[r.cov_offset + (size_t)cy * r.stride + cx] : 0u;
+// tile-table entry of target tile (tx, ty) in a job's grid (origin gx0, gy0, gw x gh
+// tiles); empty outside the grid (and for no job, whose grid is 0 x 0)
+__device__ __forceinline__ uint32_t tile_entry(const uint16_t *table, int32_t gx0, int32_t gy0, uint32_t gw,
+                                               uint32_t gh, uint32_t first, int32_t tx, int32_t ty) {
+    const uint32_t cx = (uint32_t)(tx - gx0), cy = (uint32_t)(ty - gy0);
+    return cx < gw && cy < gh ? (uint32_t)table[first + cy * gw + cx] : rvkr::kTileEmpty;
+}
+
+// coverage of a pixel of target tile row ty from its tile-table entry e (`pix` is the
+// pixel's index within its 8 x 8 tile)
+__device__ __forceinline__ uint32_t tile_cov(const JobRef &r, const uint8_t *arena, uint32_t e, int32_t ty,
+                                             uint32_t pix) {
+    if (e == rvkr::kTileEmpty) return 0u;
+    if (e == rvkr::kTileFull) return 255u;
+    const size_t slot = (size_t)(uint32_t)(ty - r.y0) * r.w + (e - rvkr::kTileSlot0);
+    return arena[r.cov_offset + slot * 64u + pix];
 }
 
 // Coverage of one hairline blit at pixel (x, y) (see HairLine); 0 when the line does
@@ -548,66 +731,81 @@ __device__ __forceinline__ uint32_t hair_alpha(const HairLine &L, int32_t x, int
 
 struct DrawArgs {
     const DrawItem *items;
+    const DrawCull *cull;
     uint32_t nitems;
-    const CovJob *jobs;
+    const uint16_t *table;
     const uint8_t *arena;
     const ShaderParams *shaders;
-    const HairJob *hair_jobs;
     const uint32_t *hair_tiles, *hair_bins;
     const HairLine *hair_lines;
 };
 
-// One thread per target pixel; each 64-lane wave covers an 8 x 8 pixel square and a
-// block of 4 waves a 16 x 16 tile.  Items are culled per block and per wave; a wave
-// runs an item's shading only when one of its pixels is covered, and compact squares
-// cross path boundaries far less often than 64-pixel row segments do (16 x 4 and
-// 4 x 16 waves, with fewer or more cache rows per access, measured no faster).
-constexpr int kWaveW = 8, kWaveH = 8;  // pixels of one 64-lane wave
-constexpr int TX = 16, TY = 16;         // pixels of one block (four waves)
+// One thread per target pixel; each 64-lane wave covers one 8 x 8 coverage tile and a
+// block of 4 waves a 16 x 16 square.  A wave first culls a run of 64 items, one lane
+// per item (its tile rect and its jobs' tile-table entries for the wave's tile, loaded
+// side by side), then composites the items that touch its tile in order: an empty
+// tile costs nothing, a full one no coverage load, a partial one one 64-byte load.
+constexpr int kWaveW = rvkr::kTile, kWaveH = rvkr::kTile;  // pixels of one 64-lane wave
+constexpr int TX = 16, TY = 16;                            // pixels of one block (four waves)
 constexpr int kWavesX = TX / kWaveW;
 static_assert(kWaveW * kWaveH == 64 && (TX / kWaveW) * (TY / kWaveH) == 4, "four 64-pixel waves per tile");
 
 // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.  Block b is
-// mapped to a tile so that each XCD gets runs of kChunk horizontally adjacent
-// row-major tiles (which read the same coverage and pixel cache lines), the runs
-// interleaved over the draw rect so the XCDs share the work evenly; tiles past the
-// last whole round of runs keep their own index.  Runs of 4 tiles (64 pixels)
-// measured fastest on the full-filter document (1, 2, 8, 16 and 32 all slower).
+// mapped to a square so that each XCD gets runs of kChunk horizontally adjacent
+// row-major squares (which read neighbouring coverage and pixel cache lines), the
+// runs interleaved over the draw rect so the XCDs share the work evenly; squares past
+// the last whole round of runs keep their own index.
 constexpr uint32_t kXcds = 8, kChunk = 4;
 
-__global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t h, int32_t ox, int32_t oy,
-                                              uint32_t tiles_x, DrawArgs a) {
+__global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t h, int32_t otx, int32_t oty,
+                                              uint32_t blocks_x, DrawArgs a) {
     const uint32_t b = blockIdx.x;
     const uint32_t round = kXcds * kChunk, full = gridDim.x / round * round;
     const uint32_t xcd = b % kXcds, k = b / kXcds;
-    const uint32_t tile = b < full ? (k / kChunk) * round + xcd * kChunk + k % kChunk : b;
-    const uint32_t bx = tile % tiles_x, by = tile / tiles_x;
-    const int lane = (int)(threadIdx.x % 64), wv = (int)(threadIdx.x / 64);
-    const int32_t tx0 = ox + (int32_t)(bx * TX), ty0 = oy + (int32_t)(by * TY);
-    const int32_t wx0 = tx0 + (wv % kWavesX) * kWaveW, wy0 = ty0 + (wv / kWavesX) * kWaveH;
-    const int32_t x = wx0 + lane % kWaveW, y = wy0 + lane / kWaveW;
+    const uint32_t sq = b < full ? (k / kChunk) * round + xcd * kChunk + k % kChunk : b;
+    const uint32_t bx = sq % blocks_x, by = sq / blocks_x;
+    const int lane = (int)(threadIdx.x % 64);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));  // uniform: tile math stays scalar
+    // this wave's target tile and pixel
+    const int32_t ttx = otx + (int32_t)bx * kWavesX + wv % kWavesX, tty = oty + (int32_t)by * (TY / kWaveH) + wv / kWavesX;
+    const int32_t x = ttx * kWaveW + lane % kWaveW, y = tty * kWaveH + lane / kWaveW;
+    const uint32_t pix = (uint32_t)lane;  // = (y % 8) * 8 + x % 8
     const bool inside = x < (int32_t)w && y < (int32_t)h;
     uint32_t d = inside ? px[(size_t)y * w + x] : 0u;
     const uint32_t d0 = d;
-    for (uint32_t it = 0; it < a.nitems; it++) {
-        const DrawItem item = a.items[it];  // with its jobs' rects and coverage offsets
-        const bool hit = item.x1 > tx0 && item.x0 < tx0 + TX && item.y1 > ty0 && item.y0 < ty0 + TY &&
-                         item.x1 > wx0 && item.x0 < wx0 + kWaveW && item.y1 > wy0 && item.y0 < wy0 + kWaveH;
-        if (hit && inside && x >= item.x0 && x < item.x1 && y >= item.y0 && y < item.y1) {
+    for (uint32_t base = 0; base < a.nitems; base += kWave) {
+        // ---- cull 64 items, one per lane ----
+        const uint32_t i = base + (uint32_t)lane;
+        uint32_t ef = 0, es = 0;
+        bool keep = false;
+        if (i < a.nitems) {
+            const DrawCull c = a.cull[i];
+            if (ttx >= c.tx0 && ttx < c.tx1 && tty >= c.ty0 && tty < c.ty1) {
+                ef = tile_entry(a.table, c.fx0, c.fy0, c.fw, c.fh, c.ftable, ttx, tty);
+                es = tile_entry(a.table, c.sx0, c.sy0, c.sw, c.sh, c.stable, ttx, tty);
+                keep = ef != rvkr::kTileEmpty || es != rvkr::kTileEmpty || c.hair != 0;
+            }
+        }
+        // ---- composite the kept items in order ----
+        for (uint64_t m = __ballot(keep); m != 0; m &= m - 1) {
+            const int kl = __builtin_ctzll(m);
+            const uint32_t efk = (uint32_t)__builtin_amdgcn_readlane((int)ef, kl);
+            const uint32_t esk = (uint32_t)__builtin_amdgcn_readlane((int)es, kl);
+            const DrawItem item = a.items[base + (uint32_t)kl];  // with its jobs' grids and tables
+            if (!inside) continue;
             const int32_t lx = x - item.lx, ly = y - item.ly;
-            const uint32_t cf = ref_cov(item.fj, a.arena, lx, ly);
+            const uint32_t cf = tile_cov(item.fj, a.arena, efk, tty, pix);
             uint32_t cs = 0, hb = 0, he = 0;
             if (item.stroke_hair) {
                 // the hairline lines binned to this pixel's tile, in drawing order
                 const uint32_t cx = (uint32_t)(lx - item.sj.x0), cy = (uint32_t)(ly - item.sj.y0);
-                if (cx < (uint32_t)item.sj.w && cy < (uint32_t)item.sj.h) {
-                    const uint32_t t = item.sj.tile_offset + cy / rvkr::kHairTile * item.sj.stride +
-                                       cx / rvkr::kHairTile;
+                if (cx < item.sj.w && cy < item.sj.h) {
+                    const uint32_t t = item.sj.table + cy / rvkr::kHairTile * item.sj.stride + cx / rvkr::kHairTile;
                     hb = a.hair_tiles[t];
                     he = a.hair_tiles[t + 1];
                 }
             } else {
-                cs = ref_cov(item.sj, a.arena, lx, ly);
+                cs = tile_cov(item.sj, a.arena, esk, tty, pix);
             }
             if ((cf | cs) != 0 || hb != he) {
                 uint32_t target = item.layer ? 0u : d;
@@ -619,8 +817,8 @@ __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t
                     bool shaded = false;
                     if (stroke && item.stroke_hair) {
                         // every blit of the hairline at this pixel, composited in order
-                        for (uint32_t k = hb; k < he; k++) {
-                            const uint32_t c = hair_alpha(a.hair_lines[a.hair_bins[k]], lx, ly);
+                        for (uint32_t kk = hb; kk < he; kk++) {
+                            const uint32_t c = hair_alpha(a.hair_lines[a.hair_bins[kk]], lx, ly);
                             if (c) target = paint_px(sp, sh, lx, ly, c, target, src, shaded);
                         }
                         continue;
@@ -657,39 +855,43 @@ rvk_status launch_expand(rvk_context *ctx, const EdgeRec *edges, uint32_t nedges
     return RVK_OK;
 }
 
-rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint2 *groups128,
-                           uint32_t n128, const uint2 *groups256, uint32_t n256, const uint2 *groups512,
+rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint2 *bands128,
+                           uint32_t n128, const uint2 *bands256, uint32_t n256, const uint2 *bands512,
                            uint32_t n512, const EdgeCov *edges, const uint2 *windows, const PieceDev *pieces,
-                           const uint32_t *npieces, uint8_t *arena, int32_t *overflow, double algo_bytes) {
-    // the algorithmic bytes are split between the launches by their row-group counts
+                           const uint32_t *npieces, uint16_t *table, uint8_t *arena, int32_t *overflow,
+                           double algo_bytes) {
+    (void)njobs;
+    // the algorithmic bytes are split between the launches by their band counts
     const double total = (double)n128 + (double)n256 + (double)n512;
-    auto launch = [&](auto kernel, const uint2 *groups, uint32_t n) -> rvk_status {
+    auto launch = [&](auto kernel, const uint2 *bands, uint32_t n) -> rvk_status {
         if (n == 0) return RVK_OK;
-        CovArgs a{groups, jobs, njobs, edges, windows, pieces, npieces, arena, overflow};
-        timed_launch(ctx, "coverage", algo_bytes * (double)n / total, [&] {
-            hipLaunchKernelGGL(kernel, dim3(n * rvkr::kGroupRows), dim3(kWave), 0, ctx->stream, a);
-        });
+        CovArgs a{bands, jobs, edges, windows, pieces, npieces, table, arena, overflow};
+        timed_launch(ctx, "coverage", algo_bytes * (double)n / total,
+                     [&] { hipLaunchKernelGGL(kernel, dim3(n), dim3(kWave), 0, ctx->stream, a); });
         RVK_HIP_CHECK(hipGetLastError());
         return RVK_OK;
     };
-    RVK_TRY(launch(k_coverage<(int)rvkr::kCoverageKeys[0]>, groups128, n128));
-    RVK_TRY(launch(k_coverage<(int)rvkr::kCoverageKeys[1]>, groups256, n256));
-    RVK_TRY(launch(k_coverage<(int)rvkr::kCoverageKeys[2]>, groups512, n512));
+    RVK_TRY(launch(k_coverage<(int)rvkr::kCoverageKeys[0]>, bands128, n128));
+    RVK_TRY(launch(k_coverage<(int)rvkr::kCoverageKeys[1]>, bands256, n256));
+    RVK_TRY(launch(k_coverage<(int)rvkr::kCoverageKeys[2]>, bands512, n512));
     return RVK_OK;
 }
 
 rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
-                             int32_t x1, int32_t y1, const DrawItem *items, uint32_t nitems, const CovJob *jobs,
-                             const uint8_t *arena, const ShaderParams *shaders, const HairJob *hair_jobs,
+                             int32_t x1, int32_t y1, const DrawItem *items, const DrawCull *cull, uint32_t nitems,
+                             const uint16_t *table, const uint8_t *arena, const ShaderParams *shaders,
                              const uint32_t *hair_tiles, const uint32_t *hair_bins, const HairLine *hair_lines,
                              double algo_bytes) {
-    if (nitems == 0 || x1 <= x0 || y1 <= y0) return RVK_OK;
-    const uint32_t tiles_x = div_up((uint32_t)(x1 - x0), TX), tiles_y = div_up((uint32_t)(y1 - y0), TY);
-    const dim3 grid(tiles_x * tiles_y);
-    const DrawArgs a{items, nitems, jobs, arena, shaders, hair_jobs, hair_tiles, hair_bins, hair_lines};
+    if (nitems == 0 || x1 <= x0 || y1 <= y0 || x0 < 0 || y0 < 0) return RVK_OK;
+    // the draw rect in whole coverage tiles (the tile grids are aligned to the target)
+    const int32_t otx = x0 / rvkr::kTile, oty = y0 / rvkr::kTile;
+    const uint32_t ntx = (uint32_t)(div_up((uint32_t)x1, (uint32_t)rvkr::kTile) - (uint32_t)otx);
+    const uint32_t nty = (uint32_t)(div_up((uint32_t)y1, (uint32_t)rvkr::kTile) - (uint32_t)oty);
+    const uint32_t blocks_x = div_up(ntx, (uint32_t)kWavesX), blocks_y = div_up(nty, (uint32_t)(TY / kWaveH));
+    const DrawArgs a{items, cull, nitems, table, arena, shaders, hair_tiles, hair_bins, hair_lines};
     timed_launch(ctx, "draw_items", algo_bytes, [&] {
-        // four 8 x 8 waves per 16 x 16 tile
-        hipLaunchKernelGGL(k_draw, grid, dim3(256), 0, ctx->stream, (uint32_t *)target, w, h, x0, y0, tiles_x, a);
+        hipLaunchKernelGGL(k_draw, dim3(blocks_x * blocks_y), dim3(256), 0, ctx->stream, (uint32_t *)target, w, h,
+                           otx, oty, blocks_x, a);
     });
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;

@@ -791,7 +791,7 @@ bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s
 // ---------------------------------------------------------------------------
 // Coverage and hairline jobs of one path, prepared off the render walk
 // ---------------------------------------------------------------------------
-void prep_job(const EdgeSet &es, uint32_t rule, bool aa, IRect clip, JobPrep &P) {
+void prep_job(const EdgeSet &es, uint32_t rule, bool aa, IRect clip, int32_t lx, int32_t ly, JobPrep &P) {
     if (!es.ok) return;
     const int shift = aa ? 2 : 0;
     const std::vector<EdgeRec> &er = es.edges;
@@ -817,6 +817,16 @@ void prep_job(const EdgeSet &es, uint32_t rule, bool aa, IRect clip, JobPrep &P)
     j.x0 = (int32_t)x0; j.y0 = (int32_t)y0; j.w = (int32_t)(x1 - x0); j.h = (int32_t)(y1 - y0);
     j.rule = rule;
     j.aa = aa ? 1 : 0;
+    // the tile grid: kTile-aligned in target pixels (edge space + (lx, ly))
+    auto floor_tile = [](int64_t v) { return v >= 0 ? v / kTile * kTile : -((-v + kTile - 1) / kTile) * kTile; };
+    const int64_t gx0 = floor_tile(x0 + lx) - lx, gy0 = floor_tile(y0 + ly) - ly;
+    const int64_t tiles_x = (x1 - gx0 + kTile - 1) / kTile, tiles_y = (y1 - gy0 + kTile - 1) / kTile;
+    if (tiles_x > (int64_t)kMaxTilesX) {
+        P.too_wide = true;
+        return;
+    }
+    j.gx0 = (int32_t)gx0; j.gy0 = (int32_t)gy0;
+    j.tiles_x = (uint32_t)tiles_x; j.tiles_y = (uint32_t)tiles_y;
     // edges sorted by first row, stably (ids keep tiny-skia's edge-list order): a
     // counting sort over the rows when they are few against the edges, else a sort
     // of (row, index) keys
@@ -853,31 +863,31 @@ void prep_job(const EdgeSet &es, uint32_t rule, bool aa, IRect clip, JobPrep &P)
         prefmax[k] = pm;
     }
     P.pieces = slots;
-    // per group of kGroupRows pixel rows: the edges that start at or before the
-    // group's last row, minus the leading ones that all end before its first row
-    const int32_t groups = (j.h + kGroupRows - 1) / kGroupRows;
-    P.windows.resize((size_t)groups);
+    // per band: the edges that start at or before the band's last covered row, minus
+    // the leading ones that all end before its first covered row
+    const int32_t bands = (int32_t)j.tiles_y;
+    P.windows.resize((size_t)bands);
     uint32_t lo = 0, hi = 0;
-    for (int32_t g = 0; g < groups; g++) {
-        const int32_t r0 = j.y0 + g * kGroupRows, r1 = std::min(j.y0 + j.h, r0 + kGroupRows);
+    for (int32_t g = 0; g < bands; g++) {
+        const int32_t r0 = std::max(j.y0, j.gy0 + g * kBandRows), r1 = std::min(j.y0 + j.h, j.gy0 + (g + 1) * kBandRows);
         const int32_t s0 = r0 << shift, s1 = (r1 << shift) - 1;
         while (lo < j.edge_count && prefmax[lo] < s0) lo++;
         while (hi < j.edge_count && P.ecov[hi].first_y <= s1) hi++;
         P.windows[(size_t)g] = uint2{lo, std::max(lo, hi)};
     }
-    // edges whose row range meets each group (difference array): an edge crosses a
+    // edges whose row range meets each band (difference array): an edge crosses a
     // sub-scanline at most once, so this bounds every sub-scanline's crossings and
     // picks the coverage kernel variant that holds them
-    std::vector<int32_t> diff((size_t)groups + 1, 0);
+    std::vector<int32_t> diff((size_t)bands + 1, 0);
     for (size_t i = 0; i < er.size(); i++) {
-        const int64_t fr = (int64_t)(er[i].first_y >> shift) - j.y0, lr = (int64_t)(ylast[i] >> shift) - j.y0;
-        if (lr < 0 || fr >= j.h) continue;
-        const int64_t ga = std::max<int64_t>(fr, 0) / kGroupRows, gb = std::min<int64_t>(lr, j.h - 1) / kGroupRows;
-        diff[ga] += 1;
-        diff[gb + 1] -= 1;
+        const int64_t fr = (int64_t)(er[i].first_y >> shift), lr = (int64_t)(ylast[i] >> shift);
+        const int64_t a = std::max<int64_t>(fr, j.y0), b = std::min<int64_t>(lr, (int64_t)j.y0 + j.h - 1);
+        if (b < a) continue;
+        diff[(size_t)((a - j.gy0) / kBandRows)] += 1;
+        diff[(size_t)((b - j.gy0) / kBandRows) + 1] -= 1;
     }
-    P.tier.resize((size_t)groups);
-    for (int32_t g = 0, run = 0; g < groups; g++) {
+    P.tier.resize((size_t)bands);
+    for (int32_t g = 0, run = 0; g < bands; g++) {
         run += diff[g];
         const uint32_t n = (uint32_t)run;
         const uint8_t t = n <= kCoverageKeys[0] ? 0 : n <= kCoverageKeys[1] ? 1 : 2;

@@ -33,13 +33,14 @@ using namespace rvkr;
 namespace rvk {
 rvk_status launch_expand(rvk_context *ctx, const EdgeRec *edges, uint32_t nedges, const EdgeCov *cov,
                          PieceDev *pieces, uint32_t *npieces, int32_t *overflow, double algo_bytes);
-rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint2 *groups128,
-                           uint32_t n128, const uint2 *groups256, uint32_t n256, const uint2 *groups512,
+rvk_status launch_coverage(rvk_context *ctx, const CovJob *jobs, uint32_t njobs, const uint2 *bands128,
+                           uint32_t n128, const uint2 *bands256, uint32_t n256, const uint2 *bands512,
                            uint32_t n512, const EdgeCov *edges, const uint2 *windows, const PieceDev *pieces,
-                           const uint32_t *npieces, uint8_t *arena, int32_t *overflow, double algo_bytes);
+                           const uint32_t *npieces, uint16_t *table, uint8_t *arena, int32_t *overflow,
+                           double algo_bytes);
 rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint32_t h, int32_t x0, int32_t y0,
-                             int32_t x1, int32_t y1, const DrawItem *items, uint32_t nitems, const CovJob *jobs,
-                             const uint8_t *arena, const ShaderParams *shaders, const HairJob *hair_jobs,
+                             int32_t x1, int32_t y1, const DrawItem *items, const DrawCull *cull, uint32_t nitems,
+                             const uint16_t *table, const uint8_t *arena, const ShaderParams *shaders,
                              const uint32_t *hair_tiles, const uint32_t *hair_bins, const HairLine *hair_lines,
                              double algo_bytes);
 }
@@ -66,10 +67,10 @@ struct DrawList {
     std::vector<HairPrep> hair;       // hairline strokes
     std::vector<ShaderParams> shaders;
     uint64_t n_edges = 0, n_windows = 0, piece_slots = 0;
-    uint64_t n_groups[3] = {0, 0, 0};  // row groups per coverage kernel variant
+    uint64_t n_groups[3] = {0, 0, 0};  // bands per coverage kernel variant
+    uint64_t n_table = 0;              // tile-table entries
     uint64_t n_hair_tiles = 0, n_hair_bins = 0, n_hair_lines = 0;
     uint64_t arena = 0;
-    uint32_t rows = 0;
     int32_t bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = INT32_MIN, by1 = INT32_MIN;
 };
 
@@ -104,11 +105,12 @@ struct PathGeom {
 };
 
 // Device-space geometry of one path drawn into a pw x ph pixmap whose visible
-// pixels are `clip`: tiny-skia's fill_path preconditions and edges, and (unless
-// `fill_only`) the stroke outline's edges or, for a stroke at most one device pixel
-// wide, its hairline segments, each prepared as a job.
+// pixels are `clip` and whose origin sits at target pixel (lx, ly): tiny-skia's
+// fill_path preconditions and edges, and (unless `fill_only`) the stroke outline's
+// edges or, for a stroke at most one device pixel wide, its hairline segments, each
+// prepared as a job.
 void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, uint32_t pw, uint32_t ph, bool fill_only,
-                  IRect clip, PathGeom &g) {
+                  IRect clip, int32_t lx, int32_t ly, PathGeom &g) {
     auto finish = [&](std::vector<Seg> &segs, uint32_t rule, JobPrep &out) {
         if (segs.empty() || bounds_nearly_zero(segs)) return;
         EdgeSet es;
@@ -120,7 +122,7 @@ void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, uint32_t pw, ui
         es.mnx = mnx;
         es.mxx = mxx;
         es.ok = !es.edges.empty();
-        prep_job(es, rule, p.anti_alias != 0, clip, out);
+        prep_job(es, rule, p.anti_alias != 0, clip, lx, ly, out);
     };
     if (!p.visible) return;
     if (p.fill.kind != RVK_PAINT_NONE && p.bounds.w != 0.0f && p.bounds.h != 0.0f) {
@@ -248,6 +250,7 @@ struct Renderer {
         uint32_t pw, ph;  // size of the pixmap drawn into
         bool fill_only;
         IRect clip;       // its visible pixels
+        int32_t lx, ly;   // the pixmap's origin in the target
     };
     std::vector<Task> tasks;
     std::vector<PathGeom> geoms;
@@ -262,7 +265,7 @@ struct Renderer {
             const size_t end = std::min(tasks.size(), k + kChunk);
             parallel_for(end - k, [&](size_t i) {
                 const Task &t = tasks[k + i];
-                compute_geom(*t.scene, *t.path, t.ts, t.pw, t.ph, t.fill_only, t.clip, geoms[k + i]);
+                compute_geom(*t.scene, *t.path, t.ts, t.pw, t.ph, t.fill_only, t.clip, t.lx, t.ly, geoms[k + i]);
             });
             ready_end = end;
             ctx->stats.host_geom_ms +=
@@ -274,6 +277,7 @@ struct Renderer {
     // Places a prepared coverage job in the draw list; returns its index or -1 when
     // it covers nothing.
     int32_t add_job(DrawList &dl, JobPrep &P) {
+        if (P.too_wide) fail(RVK_ERR_UNSUPPORTED);
         if (!P.ok) return -1;
         CovJob &j = P.j;
         j.edge_offset = (uint32_t)dl.n_edges;
@@ -283,11 +287,13 @@ struct Renderer {
         P.piece_base = dl.piece_slots;
         dl.piece_slots += P.pieces;
         for (int t = 0; t < 3; t++) dl.n_groups[t] += P.tier_count[t];
-        const uint64_t stride = ((uint64_t)j.w + 3) & ~3ull;
+        // the slot region holds every tile (at most one slot per tile and band); the
+        // coverage kernel writes only the partial ones
+        const uint64_t tiles = (uint64_t)j.tiles_x * j.tiles_y;
+        j.table_offset = (uint32_t)dl.n_table;
+        dl.n_table += tiles;
         j.cov_offset = dl.arena;
-        dl.arena += (stride * (uint64_t)j.h + 255) & ~255ull;
-        j.row_start = dl.rows;
-        dl.rows += (uint32_t)j.h;
+        dl.arena += (tiles * 64 + 255) & ~255ull;
         dl.jobs.push_back(std::move(P));
         return (int32_t)dl.jobs.size() - 1;
     }
@@ -318,7 +324,7 @@ struct Renderer {
         it.stroke_hair = 0;
         it.stroke_first = p.paint_order_stroke_first ? 1 : 0;
         if (mode == kCollect) {
-            tasks.push_back(Task{&S, &p, ts, pw, ph, fill_only, clip});
+            tasks.push_back(Task{&S, &p, ts, pw, ph, fill_only, clip, it.lx, it.ly});
             return;
         }
         if (mode == kSkip) {
@@ -383,8 +389,11 @@ struct Renderer {
             JobRef r{};
             if (j < 0) return r;
             const CovJob &jb = dl.jobs[j].j;
-            r.x0 = jb.x0; r.y0 = jb.y0; r.w = jb.w; r.h = jb.h;
-            r.stride = ((uint32_t)jb.w + 3u) & ~3u;
+            r.x0 = (jb.gx0 + it.lx) / kTile;  // target-aligned, so exact
+            r.y0 = (jb.gy0 + it.ly) / kTile;
+            r.w = jb.tiles_x;
+            r.h = jb.tiles_y;
+            r.table = jb.table_offset;
             r.cov_offset = jb.cov_offset;
             return r;
         };
@@ -392,9 +401,9 @@ struct Renderer {
         if (it.stroke_hair) {
             const HairJob &hj = dl.hair[it.stroke_job].j;
             it.sj = JobRef{};
-            it.sj.x0 = hj.x0; it.sj.y0 = hj.y0; it.sj.w = hj.w; it.sj.h = hj.h;
+            it.sj.x0 = hj.x0; it.sj.y0 = hj.y0; it.sj.w = (uint32_t)hj.w; it.sj.h = (uint32_t)hj.h;
             it.sj.stride = hj.tiles_x;
-            it.sj.tile_offset = hj.tile_offset;
+            it.sj.table = hj.tile_offset;
         } else {
             it.sj = cov_ref(it.stroke_job);
         }
@@ -409,13 +418,13 @@ struct Renderer {
             dl = DrawList{dl.px, dl.w, dl.h};
             return RVK_OK;
         }
-        // one device block: jobs | edges | ecov | windows | shaders | items | hairline
-        // jobs, tiles, bins, lines | row groups per coverage variant || npieces |
-        // pieces | arena (the part after || is written by the device).  The metadata is
-        // packed straight into a pinned staging buffer (the jobs in parallel, each
-        // moved to its place in the block) and copied without a host wait; device
-        // buffers are reused only by later work on the same stream.
-        const size_t njobs = dl.jobs.size(), nhair = dl.hair.size();
+        // one device block: jobs | edges | ecov | windows | shaders | items | item cull
+        // records | hairline tiles, bins, lines | bands per coverage variant ||
+        // npieces | pieces | tile tables | arena (the part after || is written by the
+        // device).  The metadata is packed straight into a pinned staging buffer (the
+        // jobs in parallel, each moved to its place in the block) and copied without a
+        // host wait; device buffers are reused only by later work on the same stream.
+        const size_t njobs = dl.jobs.size(), nhair = dl.hair.size(), nitems = dl.items.size();
         auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
         const size_t o_jobs = 0;
         const size_t o_edges = al(o_jobs + njobs * sizeof(CovJob));
@@ -423,8 +432,8 @@ struct Renderer {
         const size_t o_win = al(o_ecov + dl.n_edges * sizeof(EdgeCov));
         const size_t o_sh = al(o_win + dl.n_windows * sizeof(uint2));
         const size_t o_it = al(o_sh + dl.shaders.size() * sizeof(ShaderParams));
-        const size_t o_hj = al(o_it + dl.items.size() * sizeof(DrawItem));
-        const size_t o_ht = al(o_hj + nhair * sizeof(HairJob));
+        const size_t o_cull = al(o_it + nitems * sizeof(DrawItem));
+        const size_t o_ht = al(o_cull + nitems * sizeof(DrawCull));
         const size_t o_hb = al(o_ht + dl.n_hair_tiles * 4);
         const size_t o_hl = al(o_hb + dl.n_hair_bins * 4);
         size_t o_grp[3];
@@ -434,14 +443,32 @@ struct Renderer {
         const size_t meta = al(o_grp[2] + dl.n_groups[2] * sizeof(uint2));
         const size_t o_np = meta;
         const size_t o_pc = al(o_np + dl.n_edges * 4);
-        const size_t o_arena = al(o_pc + (size_t)dl.piece_slots * sizeof(PieceDev));
+        const size_t o_tab = al(o_pc + (size_t)dl.piece_slots * sizeof(PieceDev));
+        const size_t o_arena = al(o_tab + (size_t)dl.n_table * 2);
         void *hostv = nullptr;
         uint32_t slot = 0;
         RVK_TRY(rvk::staging_acquire(ctx, meta, &hostv, &slot));
         uint8_t *host = (uint8_t *)hostv;
         if (!dl.shaders.empty()) std::memcpy(host + o_sh, dl.shaders.data(), dl.shaders.size() * sizeof(ShaderParams));
-        std::memcpy(host + o_it, dl.items.data(), dl.items.size() * sizeof(DrawItem));
-        // each job's first entry in the row-group lists
+        std::memcpy(host + o_it, dl.items.data(), nitems * sizeof(DrawItem));
+        {
+            DrawCull *cull = (DrawCull *)(host + o_cull);
+            for (size_t i = 0; i < nitems; i++) {
+                const DrawItem &it = dl.items[i];
+                DrawCull c{};
+                c.tx0 = it.x0 / kTile;
+                c.ty0 = it.y0 / kTile;
+                c.tx1 = (it.x1 + kTile - 1) / kTile;
+                c.ty1 = (it.y1 + kTile - 1) / kTile;
+                c.fx0 = it.fj.x0; c.fy0 = it.fj.y0; c.fw = it.fj.w; c.fh = it.fj.h; c.ftable = it.fj.table;
+                c.hair = it.stroke_hair;
+                if (!it.stroke_hair) {
+                    c.sx0 = it.sj.x0; c.sy0 = it.sj.y0; c.sw = it.sj.w; c.sh = it.sj.h; c.stable = it.sj.table;
+                }
+                cull[i] = c;
+            }
+        }
+        // each job's first entry in the band lists
         std::vector<uint64_t> gbase(njobs * 3);
         {
             uint64_t run[3] = {0, 0, 0};
@@ -472,12 +499,10 @@ struct Renderer {
                                 P.windows.size() * sizeof(uint2));
                 uint2 *gl[3];
                 for (int t = 0; t < 3; t++) gl[t] = (uint2 *)(host + o_grp[t]) + gbase[i * 3 + t];
-                for (size_t g = 0; g < P.tier.size(); g++)
-                    *gl[P.tier[g]]++ = uint2{(uint32_t)i, (uint32_t)(g * kGroupRows)};
+                for (size_t g = 0; g < P.tier.size(); g++) *gl[P.tier[g]]++ = uint2{(uint32_t)i, (uint32_t)g};
             } else {
                 const size_t k = i - njobs;
                 const HairPrep &P = dl.hair[k];
-                std::memcpy(host + o_hj + k * sizeof(HairJob), &P.j, sizeof(HairJob));
                 uint32_t *tl = (uint32_t *)(host + o_ht) + P.j.tile_offset;
                 for (size_t t = 0; t < P.tiles.size(); t++) tl[t] = P.tiles[t] + P.bin_base;
                 uint32_t *bl = (uint32_t *)(host + o_hb) + P.bin_base;
@@ -494,16 +519,17 @@ struct Renderer {
         RVK_TRY(rvk::scratch_alloc(ctx, o_arena + dl.arena, &dev));
         uint8_t *d = (uint8_t *)dev;
         // algorithmic bytes: expansion reads each edge once and writes its pieces;
-        // coverage writes one byte per job pixel and reads edges, windows and pieces
-        // once; the compositor reads and writes each target pixel of its rect once
-        // and reads each coverage byte once
-        double cov_px = 0.0;
-        for (const JobPrep &P : dl.jobs) cov_px += (double)P.j.w * (double)P.j.h;
+        // coverage reads edges, windows and pieces once and writes the tile tables; the
+        // compositor reads and writes each target pixel of its rect once and reads the
+        // tables and the hairline lines (the partial tiles' 64 bytes each, written by
+        // coverage and read by the compositor, are data-dependent and not counted)
+        double job_tiles = 0.0;
+        for (const JobPrep &P : dl.jobs) job_tiles += (double)P.j.tiles_x * (double)P.j.tiles_y;
         const double piece_bytes = (double)dl.piece_slots * sizeof(PieceDev);
         const double exp_bytes = (double)dl.n_edges * (sizeof(EdgeRec) + 4) + piece_bytes;
-        const double cov_bytes = cov_px + (double)dl.n_edges * sizeof(EdgeCov) +
+        const double cov_bytes = 2.0 * job_tiles + (double)dl.n_edges * sizeof(EdgeCov) +
                                  (double)dl.n_windows * sizeof(uint2) + piece_bytes;
-        const double draw_bytes = 8.0 * (double)(dl.bx1 - dl.bx0) * (double)(dl.by1 - dl.by0) + cov_px +
+        const double draw_bytes = 8.0 * (double)(dl.bx1 - dl.bx0) * (double)(dl.by1 - dl.by0) + 2.0 * job_tiles +
                                   (double)(dl.n_hair_lines * sizeof(HairLine) + dl.n_hair_bins * 4);
         ctx->stats.edge_pieces += dl.piece_slots;
         ctx->stats.band_entries += dl.n_windows;
@@ -519,13 +545,13 @@ struct Renderer {
                                       (const uint2 *)(d + o_grp[1]), (uint32_t)dl.n_groups[1],
                                       (const uint2 *)(d + o_grp[2]), (uint32_t)dl.n_groups[2],
                                       (const EdgeCov *)(d + o_ecov), (const uint2 *)(d + o_win),
-                                      (const PieceDev *)(d + o_pc), (const uint32_t *)(d + o_np), d + o_arena,
-                                      dev_overflow, cov_bytes);
+                                      (const PieceDev *)(d + o_pc), (const uint32_t *)(d + o_np),
+                                      (uint16_t *)(d + o_tab), d + o_arena, dev_overflow, cov_bytes);
         if (st == RVK_OK)
             st = rvk::launch_draw_items(ctx, dl.px, dl.w, dl.h, dl.bx0, dl.by0, dl.bx1, dl.by1,
-                                        (const DrawItem *)(d + o_it), (uint32_t)dl.items.size(),
-                                        (const CovJob *)(d + o_jobs), d + o_arena, (const ShaderParams *)(d + o_sh),
-                                        (const HairJob *)(d + o_hj), (const uint32_t *)(d + o_ht),
+                                        (const DrawItem *)(d + o_it), (const DrawCull *)(d + o_cull),
+                                        (uint32_t)nitems, (const uint16_t *)(d + o_tab), d + o_arena,
+                                        (const ShaderParams *)(d + o_sh), (const uint32_t *)(d + o_ht),
                                         (const uint32_t *)(d + o_hb), (const HairLine *)(d + o_hl), draw_bytes);
         rvk::scratch_release(ctx, dev);
         ctx->stats.flushes++;
@@ -1056,7 +1082,8 @@ rvk_status render_batch(rvk_context *ctx, uint32_t count, const rvk_scene *const
         parallel_for(all.size(), [&](size_t i) {
             Renderer &r = *all[i].first;
             const auto &t = r.tasks[all[i].second];
-            compute_geom(*t.scene, *t.path, t.ts, t.pw, t.ph, t.fill_only, t.clip, r.geoms[all[i].second]);
+            compute_geom(*t.scene, *t.path, t.ts, t.pw, t.ph, t.fill_only, t.clip, t.lx, t.ly,
+                         r.geoms[all[i].second]);
         });
         ctx->stats.host_geom_ms +=
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
