@@ -43,6 +43,7 @@ typedef int32_t rvk_status;
 #define RVK_ERR_INVALID_REGION   4  /* filter::Error::InvalidRegion (filter/mod.rs:86-90) */
 #define RVK_ERR_UNSUPPORTED      5  /* feature outside this library's scope (feImage, patterns, images) */
 #define RVK_ERR_NO_DEVICE        6  /* no HIP device / library built without a device */
+#define RVK_ERR_TOO_LARGE        7  /* input beyond a fixed limit of the entry point (the no-allocation form) */
 
 /* Returns a static string naming a status code. */
 const char *rvk_status_string(rvk_status status);
@@ -297,6 +298,15 @@ rvk_status rvk_multiply_alpha(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uin
 /* box_blur::apply (filter/box_blur.rs:23-34); in place. */
 rvk_status rvk_box_blur(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h,
                         double sigma_x, double sigma_y);
+/* No counterpart in the reference: rvk_box_blur for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The back buffer is a 64 MiB device array of the library (one per GPU, held until the
+ * process ends), so w * h * 4 must not exceed 64 MiB (4096 x 4096); larger images return
+ * RVK_ERR_TOO_LARGE.  No hipMalloc, hipFree or other allocation is made.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (several streams or threads)
+ * race and must be ordered by the caller. */
+rvk_status rvk_box_blur_noalloc(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h,
+                                double sigma_x, double sigma_y);
 /* iir_blur::apply (filter/iir_blur.rs:47-65); in place. */
 rvk_status rvk_iir_blur(rvk_context *ctx, uint8_t *dev_px, uint32_t w, uint32_t h,
                         double sigma_x, double sigma_y);

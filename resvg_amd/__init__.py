@@ -55,6 +55,7 @@ def _declare(L: C.CDLL) -> None:
         "rvk_demultiply_alpha": ([P, P, u32, u32], i32),
         "rvk_multiply_alpha": ([P, P, u32, u32], i32),
         "rvk_box_blur": ([P, P, u32, u32, f64, f64], i32),
+        "rvk_box_blur_noalloc": ([P, P, u32, u32, f64, f64], i32),
         "rvk_iir_blur": ([P, P, u32, u32, f64, f64], i32),
         "rvk_morphology": ([P, P, u32, u32, u32, f32, f32], i32),
         "rvk_convolve_matrix": ([P, P, u32, u32, C.POINTER(abi.Primitive), C.POINTER(f32)], i32),
@@ -215,6 +216,12 @@ class Context:
     def box_blur(self, px, sigma_x: float, sigma_y: float):
         w, h = _dims(px)
         _check(lib().rvk_box_blur(self.h, _ptr(px), w, h, sigma_x, sigma_y), "rvk_box_blur")
+
+    def box_blur_noalloc(self, px, sigma_x: float, sigma_y: float) -> int:
+        """rvk_box_blur_noalloc: the box blur without any device allocation (its back
+        buffer is the library's 64 MiB device array); returns the status."""
+        w, h = _dims(px)
+        return lib().rvk_box_blur_noalloc(self.h, _ptr(px), w, h, sigma_x, sigma_y)
 
     def iir_blur(self, px, sigma_x: float, sigma_y: float):
         w, h = _dims(px)

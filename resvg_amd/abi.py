@@ -8,7 +8,8 @@ from __future__ import annotations
 import ctypes as C
 
 # status
-OK, ERR_INVALID_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_HIP, ERR_INVALID_REGION, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(7)
+OK, ERR_INVALID_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_HIP, ERR_INVALID_REGION, ERR_UNSUPPORTED, ERR_NO_DEVICE, \
+    ERR_TOO_LARGE = range(8)
 
 # tiny_skia::BlendMode (declaration order)
 BLEND_MODES = [

@@ -7,6 +7,7 @@
 // outside the image), times f32 1/(2r+1), round-to-nearest-even, cast to u8.
 #include "rvk_internal.h"
 #include "device_math.h"
+#include <mutex>
 
 using namespace rvkd;
 
@@ -16,6 +17,10 @@ constexpr int TW = 64;        // output tile width
 constexpr int TH = 32;        // output tile height
 constexpr int kThreads = 256;
 constexpr int kMaxFusedR = 24;  // halo limit of the fused kernel (LDS budget)
+
+// Back buffer of rvk_box_blur_noalloc: 64 MiB, a 4096 x 4096 RGBA8 image.
+constexpr size_t kNoAllocScratchBytes = rvk::kNoAllocBoxBlurBytes;
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
 
 __device__ __forceinline__ uint32_t round_pass(uint32_t s, float iarr) {
     // round() of box_blur.rs:327-331 is round-to-nearest-even on [0, 2^22].
@@ -222,13 +227,14 @@ static void box_gauss_sizes(float sigma, int32_t out[5]) {
     }
 }
 
-rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy) {
+// The five rounds with `tmp` (w * h * 4 bytes of device memory) as the back buffer;
+// allocates nothing.
+static rvk_status box_blur_with(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy,
+                                void *tmp) {
     int32_t bh[5], bv[5];
     box_gauss_sizes((float)sx, bh);
     box_gauss_sizes((float)sy, bv);
-    void *tmp = nullptr;
-    size_t bytes = (size_t)w * h * 4;
-    RVK_TRY(scratch_alloc(ctx, bytes, &tmp));
+    const size_t bytes = (size_t)w * h * 4;
     uint32_t *a = (uint32_t *)px, *b = (uint32_t *)tmp;
     rvk_status st = RVK_OK;
     // algorithmic bytes: the image read once and written once per blur (8 B/px,
@@ -269,8 +275,50 @@ rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h
     if (st == RVK_OK && a != (uint32_t *)px) {
         if (hipMemcpyAsync(px, a, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) st = RVK_ERR_HIP;
     }
+    return st;
+}
+
+rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy) {
+    void *tmp = nullptr;
+    RVK_TRY(scratch_alloc(ctx, (size_t)w * h * 4, &tmp));
+    const rvk_status st = box_blur_with(ctx, px, w, h, sx, sy, tmp);
     scratch_release(ctx, tmp);
     return st;
+}
+
+// ---------------------------------------------------------------------------
+// The no-allocation form: the back buffer is a file-scope device array, one per GPU
+// for the life of the process.  Its address is looked up once per device, with that
+// device current, when a context is created on it; host code uses only that address
+// (the array's own name is a host-side stand-in, not a device pointer).
+// ---------------------------------------------------------------------------
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[kMaxDevices] = {};
+
+rvk_status noalloc_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return RVK_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return RVK_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        set_last_error("hipGetSymbolAddress(scratch)", __FILE__);
+        return RVK_ERR_HIP;
+    }
+    g_noalloc_ptr[device] = p;
+    return RVK_OK;
+}
+
+rvk_status launch_box_blur_noalloc(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy) {
+    if ((size_t)w * h * 4 > kNoAllocScratchBytes) return RVK_ERR_TOO_LARGE;
+    void *tmp = nullptr;
+    if (ctx->device >= 0 && ctx->device < kMaxDevices) {
+        std::lock_guard<std::mutex> g(g_noalloc_mu);
+        tmp = g_noalloc_ptr[ctx->device];
+    }
+    if (!tmp) return RVK_ERR_HIP;  // the context's set-up could not look the array up
+    return box_blur_with(ctx, px, w, h, sx, sy, tmp);
 }
 
 }  // namespace rvk

@@ -202,6 +202,7 @@ const char *rvk_status_string(rvk_status s) {
         case RVK_ERR_INVALID_REGION: return "invalid filter region";
         case RVK_ERR_UNSUPPORTED: return "unsupported feature";
         case RVK_ERR_NO_DEVICE: return "no HIP device";
+        case RVK_ERR_TOO_LARGE: return "input larger than the entry point's fixed limit";
         default: return "unknown status";
     }
 }
@@ -216,6 +217,8 @@ rvk_status rvk_context_create(int device, rvk_context **out) {
     }
     if (device < 0 || device >= count) return RVK_ERR_INVALID_ARGUMENT;
     RVK_HIP_CHECK(hipSetDevice(device));
+    // the no-allocation entry point's device array: its address on this device
+    RVK_TRY(rvk::noalloc_init(device));
     rvk_context *ctx = new rvk_context();
     ctx->device = device;
     if (hipStreamCreate(&ctx->stream) != hipSuccess) {

@@ -404,6 +404,10 @@ rvk_status rvk_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, d
     CHECK_PX(ctx, px, w, h);
     return rvk::launch_box_blur(ctx, px, w, h, sx, sy);
 }
+rvk_status rvk_box_blur_noalloc(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy) {
+    CHECK_PX(ctx, px, w, h);
+    return rvk::launch_box_blur_noalloc(ctx, px, w, h, sx, sy);
+}
 rvk_status rvk_iir_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy) {
     CHECK_PX(ctx, px, w, h);
     return rvk::launch_iir_blur(ctx, px, w, h, sx, sy);

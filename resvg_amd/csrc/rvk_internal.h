@@ -110,6 +110,10 @@ rvk_status launch_into_cs(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h,
 rvk_status launch_demultiply(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h);
 rvk_status launch_multiply(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h);
 rvk_status launch_box_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy);
+// rvk_box_blur_noalloc: the back buffer is a file-scope device array of this size
+constexpr size_t kNoAllocBoxBlurBytes = (size_t)64 << 20;
+rvk_status noalloc_init(int device);  // looks the array up on `device` (current)
+rvk_status launch_box_blur_noalloc(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy);
 rvk_status launch_iir_blur(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, double sx, double sy);
 rvk_status launch_morphology(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_t h, uint32_t op, float rx,
                              float ry);
