@@ -481,48 +481,85 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
                 const int32_t colbase = wabs * nsub_aa;
                 const int32_t wend = colbase + ww * nsub_aa;  // first sub-column past the window
                 const int32_t lo = max(0, rx0 - wabs) * nsub_aa, hi = max(0, min(ww, rx1 - wabs)) * nsub_aa;
-                for (int s = 0; s < nsub; s++) {
-                    const int32_t *ev = reinterpret_cast<const int32_t *>(keys[s]);
-                    const int32_t maxv = job.aa ? (64 - (((s & 3) + 1) >> 2)) : 255;
-                    const int nsp = nev[s] >> 1;
-                    for (int k0s = kst[s]; k0s < nsp; k0s += kWave) {
-                        const int k = k0s + lane;
-                        int32_t La = 0, Ra = 0;
-                        if (k < nsp) {
-                            La = ev[2 * k];
-                            Ra = ev[2 * k + 1];
+                // lanes 16 s .. 16 s + 15 look at the next 16 spans of sub-scanline s: the
+                // spans that start before the window's end are a prefix of them (spans
+                // are ordered), as are those that end at or before it (behind every
+                // later window); all the window's spans are then accumulated in one
+                // pass, one per lane
+                int nin[kMaxSub] = {0, 0, 0, 0}, off[kMaxSub + 1] = {0, 0, 0, 0, 0};
+                bool more = false;
+                if (nsub > 0 && hi > lo) {
+                    const int sl = lane >> 4;
+                    const int kk = kst[sl] + (lane & 15);
+                    const int nspl = sl < nsub ? nev[sl] >> 1 : 0;
+                    int32_t La = 0, Ra = 0;
+                    if (kk < nspl) {
+                        const int32_t *ev = reinterpret_cast<const int32_t *>(keys[sl]);
+                        La = ev[2 * kk];
+                        Ra = ev[2 * kk + 1];
+                    }
+                    const uint64_t min_ = __ballot(kk < nspl && La < wend);
+                    const uint64_t mdone = __ballot(kk < nspl && Ra <= wend);
+                    for (int q = 0; q < kMaxSub; q++) {
+                        nin[q] = __popcll((min_ >> (16 * q)) & 0xffffull);
+                        more = more || nin[q] == 16;
+                        off[q + 1] = off[q] + nin[q];
+                    }
+                    for (int q = 0; q < kMaxSub; q++) kst[q] += __popcll((mdone >> (16 * q)) & 0xffffull);
+                    // the accumulation of one span (window-relative sub-columns)
+                    auto add_span = [&](int q, int32_t La2, int32_t Ra2) {
+                        const int32_t maxv = job.aa ? (64 - (((q & 3) + 1) >> 2)) : 255;
+                        int32_t L = max(La2 - colbase, lo), R = min(Ra2 - colbase, hi);
+                        if (R <= L) return;
+                        if (!job.aa) {
+                            atomicAdd(&acc[L], 255 << 16);
+                            atomicAdd(&acc[R], -(255 << 16));
+                            return;
                         }
-                        // spans are ordered: those ending at or before this window's end
-                        // are behind every later window
-                        kst[s] += __popcll(__ballot(k < nsp && Ra <= wend));
-                        const bool beyond = k >= nsp || La >= wend;
-                        if (!beyond && hi > lo) {
-                            int32_t L = La - colbase, R = Ra - colbase;
-                            L = max(L, lo);
-                            R = min(R, hi);
-                            if (R > L) {
-                                if (!job.aa) {
-                                    atomicAdd(&acc[L], 255 << 16);
-                                    atomicAdd(&acc[R], -(255 << 16));
-                                } else {
-                                    const int32_t fb = L & 3, fe = R & 3;
-                                    int32_t c = (R >> 2) - (L >> 2) - 1;
-                                    if (c < 0) {
-                                        atomicAdd(&acc[L >> 2], (fe - fb) * 16);
-                                    } else {
-                                        int32_t st = L >> 2;
-                                        if (fb == 0) c += 1;
-                                        else { atomicAdd(&acc[st], (4 - fb) * 16); st += 1; }
-                                        if (c > 0) {
-                                            atomicAdd(&acc[st], maxv << 16);
-                                            atomicAdd(&acc[st + c], -(maxv << 16));
-                                        }
-                                        if (fe) atomicAdd(&acc[R >> 2], fe * 16);
-                                    }
-                                }
+                        const int32_t fb = L & 3, fe = R & 3;
+                        int32_t c = (R >> 2) - (L >> 2) - 1;
+                        if (c < 0) {
+                            atomicAdd(&acc[L >> 2], (fe - fb) * 16);
+                        } else {
+                            int32_t st = L >> 2;
+                            if (fb == 0) c += 1;
+                            else { atomicAdd(&acc[st], (4 - fb) * 16); st += 1; }
+                            if (c > 0) {
+                                atomicAdd(&acc[st], maxv << 16);
+                                atomicAdd(&acc[st + c], -(maxv << 16));
                             }
+                            if (fe) atomicAdd(&acc[R >> 2], fe * 16);
                         }
-                        if (__ballot(!beyond) == 0 || __ballot(k < nsp && La >= wend) != 0) break;
+                    };
+                    if (lane < off[kMaxSub]) {
+                        const int q = (lane >= off[1]) + (lane >= off[2]) + (lane >= off[3]);
+                        const int k = kst[q] - __popcll((mdone >> (16 * q)) & 0xffffull) + (lane - off[q]);
+                        const int32_t *ev = reinterpret_cast<const int32_t *>(keys[q]);
+                        add_span(q, ev[2 * k], ev[2 * k + 1]);
+                    }
+                    if (more) {
+                        // a sub-scanline with 16 or more spans in this window: the rest of
+                        // them, 64 at a time
+                        for (int q = 0; q < nsub; q++) {
+                            if (nin[q] < 16) continue;
+                            const int32_t *ev = reinterpret_cast<const int32_t *>(keys[q]);
+                            const int nsp = nev[q] >> 1;
+                            const int k0q = kst[q] - __popcll((mdone >> (16 * q)) & 0xffffull) + 16;
+                            int extra_done = 0;
+                            for (int kb = k0q; kb < nsp; kb += kWave) {
+                                const int k = kb + lane;
+                                int32_t La2 = 0, Ra2 = 0;
+                                if (k < nsp) {
+                                    La2 = ev[2 * k];
+                                    Ra2 = ev[2 * k + 1];
+                                }
+                                const bool in = k < nsp && La2 < wend;
+                                extra_done += __popcll(__ballot(k < nsp && Ra2 <= wend));
+                                if (in) add_span(q, La2, Ra2);
+                                if (__ballot(in) != ~0ull) break;
+                            }
+                            kst[q] += extra_done;
+                        }
                     }
                 }
                 __syncthreads();
