@@ -13,6 +13,9 @@
 //
 // Strokes use resvg_amd/csrc/stroker.cpp (tiny-skia's PathStroker restated), the one
 // piece of geometry shared with the product.  Test infrastructure only.
+//
+// Restates tiny-skia (BSD-3-Clause, Copyright 2020 Yevhenii Reizner) and Skia
+// (BSD-3-Clause, Copyright Google Inc.); see THIRD_PARTY_NOTICES.md.
 #include "oracle.h"
 #include "common.h"
 #include "geom.h"

@@ -3,6 +3,9 @@
 // SkContourMeasure), used by Pixmap::stroke_path before stroking when the stroke
 // has a dash array.  Float expressions keep the reference's operation order; build
 // without FMA contraction.
+//
+// Restates tiny-skia (BSD-3-Clause, Copyright 2020 Yevhenii Reizner) and Skia
+// (BSD-3-Clause, Copyright Google Inc.); see THIRD_PARTY_NOTICES.md.
 #include "stroker.h"
 
 #include <cmath>

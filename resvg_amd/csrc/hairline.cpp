@@ -2,6 +2,9 @@
 // line_clipper.rs, path_geometry.rs; Skia's SkScan_Hairline.cpp, SkScan_Antihair.cpp,
 // SkLineClipper.cpp).  Float expressions keep the reference's operation order; build
 // without FMA contraction.
+//
+// Restates tiny-skia (BSD-3-Clause, Copyright 2020 Yevhenii Reizner) and Skia
+// (BSD-3-Clause, Copyright Google Inc.); see THIRD_PARTY_NOTICES.md.
 #include "hairline.h"
 
 #include <algorithm>

@@ -9,6 +9,9 @@
 // and the clip each runs under).  Plain host C++ compiled into librvk.so and into the
 // CPU oracle; the pixel stepping is written separately on each side (the oracle
 // blits sequentially, the device evaluates each pixel's blits in closed form).
+//
+// Restates tiny-skia (BSD-3-Clause, Copyright 2020 Yevhenii Reizner) and Skia
+// (BSD-3-Clause, Copyright Google Inc.); see THIRD_PARTY_NOTICES.md.
 #pragma once
 #include <cstddef>
 #include <cstdint>

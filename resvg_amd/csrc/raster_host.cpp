@@ -6,6 +6,9 @@
 // MAX_COEFF_SHIFT 6, curves chopped at Y extrema first).  Every update step of a
 // curve edge is run here once and each non-empty line piece is kept, which is the
 // sequence tiny-skia's scanline walk visits lazily.
+//
+// Restates tiny-skia (BSD-3-Clause, Copyright 2020 Yevhenii Reizner) and Skia
+// (BSD-3-Clause, Copyright Google Inc.); see THIRD_PARTY_NOTICES.md.
 #include "raster.h"
 #include "hairline.h"
 #include "stroker.h"

@@ -2,6 +2,9 @@
 // themselves ports of Skia's SkStroke.cpp, SkStrokerPriv.cpp, SkGeometry.cpp).
 // Every float expression keeps the reference's operation order; build without FMA
 // contraction.
+//
+// Restates tiny-skia (BSD-3-Clause, Copyright 2020 Yevhenii Reizner) and Skia
+// (BSD-3-Clause, Copyright Google Inc.); see THIRD_PARTY_NOTICES.md.
 #include "stroker.h"
 
 #include <algorithm>

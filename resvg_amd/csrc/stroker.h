@@ -8,6 +8,9 @@
 // Plain host C++ with no device code; the same file is compiled into librvk.so and
 // into the CPU oracle, so stroke geometry is pinned by the reference's stroke
 // fixtures rather than by the GPU parity tests.
+//
+// Restates tiny-skia (BSD-3-Clause, Copyright 2020 Yevhenii Reizner) and Skia
+// (BSD-3-Clause, Copyright Google Inc.); see THIRD_PARTY_NOTICES.md.
 #pragma once
 #include <cstddef>
 #include <cstdint>

@@ -1,7 +1,8 @@
 """Copies the reference's own known-answer PNGs used by the filter fixture tests.
 
-The PNGs are resvg's test references (crates/resvg/tests/tests/filters/..., MIT or
-Apache-2.0, Copyright the Resvg Authors).  Only PNG data files are copied — no
+The PNGs are resvg's test references (crates/resvg/tests/tests/{filters, masking,
+paint-servers, painting, shapes}/..., MIT or Apache-2.0, Copyright the Resvg
+Authors).  Only PNG data files are copied — no
 reference source.  Run in a checkout that has the reference tree mounted; the
 copies are committed so the tests never read the reference at run time.
 """
