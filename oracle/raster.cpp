@@ -600,6 +600,12 @@ struct Shader {
     bool radial = false;
     float r_scale = 1.0f, r_bias = 0.0f;
     bool r_twist = false;
+    // two-point conical gradient with its focal point off the centre (tiny-skia's
+    // FocalData): 1 focal on the end circle, 2 well behaved (focal inside), 3 greater
+    // (focal outside; degenerate pixels masked); such a shader runs in tiny-skia's f32
+    // pipeline
+    int focal = 0;
+    float p0 = 1.0f;
     uint32_t spread = RVK_SPREAD_PAD;
     bool pad_clamp = false;
     bool two_stop = false;
@@ -714,9 +720,32 @@ static Shader make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, float 
         Ts tr{1, 0, 0, 1, -p.x1, -p.y1};
         Ts scl{inv, 0, 0, inv, 0, 0};
         unit = ts_concat(scl, ts_concat(tr, rot));
+    } else if (float fdx = p.fx - p.cx, fdy = p.fy - p.cy, dc = std::sqrt(fdx * fdx + fdy * fdy);
+               !(dc <= 1.0f / 4096.0f)) {
+        // RadialGradient::new with the focal point (fx, fy) off the centre (cx, cy):
+        // SkTwoPointConicalGradient::Create's kFocal with start radius 0.  The gradient
+        // space puts the focal point at the origin and the centre at (1, 0)
+        // (ts_from_poly_to_poly), r1 = r / |centre - focal|; FocalData then scales it
+        // by (1/2, 1/2) when the focal point is on the circle (|1 - r1| nearly zero),
+        // else by (r1 / (r1^2 - 1), 1 / sqrt(|r1^2 - 1|)).  A focal radius together
+        // with a moved focal point is not restated.
+        if (!std::isfinite(dc) || p.fr != 0.0f || !(p.r > 0.0f)) return s;
+        const Ts src{p.cy - p.fy, p.fx - p.cx, p.cx - p.fx, p.cy - p.fy, p.fx, p.fy};  // from_poly2
+        const Ts dst{0.0f, -1.0f, 1.0f, 0.0f, 0.0f, 0.0f};                            // from_poly2((0,0), (1,0))
+        Ts src_inv;
+        if (!ts_invert(src, src_inv)) return s;
+        const Ts unit0 = ts_concat(dst, src_inv);
+        const float r1 = p.r / dc;
+        if (std::fabs(1.0f - r1) <= 1.0f / 4096.0f) {
+            s.focal = 1;
+            unit = ts_concat(Ts{0.5f, 0, 0, 0.5f, 0, 0}, unit0);
+        } else {
+            s.focal = r1 > 1.0f ? 2 : 3;
+            unit = ts_concat(Ts{r1 / (r1 * r1 - 1.0f), 0, 0, 1.0f / std::sqrt(std::fabs(r1 * r1 - 1.0f)), 0, 0}, unit0);
+        }
+        s.p0 = 1.0f / r1;
+        s.radial = true;
     } else {
-        float fdx = p.fx - p.cx, fdy = p.fy - p.cy;
-        if (!(std::sqrt(fdx * fdx + fdy * fdy) <= 1.0f / 4096.0f)) return s;  // focal gradients are outside this build
         float r0 = p.fr, r1 = p.r;
         // equal radii are not rejected: the twist below then divides by zero and every
         // t becomes NaN, which the clamps map to 0 (the first stop), as tiny-skia draws
@@ -806,15 +835,28 @@ static inline float clamp01(float v) {
     return v < 1.0f ? v : 1.0f;
 }
 
-static void shade(const Shader &s, uint32_t px, uint32_t py, uint32_t out[4]) {
-    if (s.solid) { for (int k = 0; k < 4; k++) out[k] = s.rgba[k]; return; }
+// The gradient coordinate of pixel (px, py): seed_shader, transform, the radial or
+// two-point conical stage (with mask_2pt_conical_degenerates: t <= 0 or NaN is
+// degenerate and becomes 0) and the spread mode.
+static float grad_t(const Shader &s, uint32_t px, uint32_t py, bool &degenerate) {
     float x = (float)px + 0.5f, y = (float)py + 0.5f;
     float tx = x * s.ts.sx + (y * s.ts.kx + s.ts.tx);
     float ty = x * s.ts.ky + (y * s.ts.sy + s.ts.ty);
     float t = tx;
-    if (s.radial) {
+    degenerate = false;
+    if (s.focal == 1) {
+        t = tx + ty * ty / tx;  // xy_to_2pt_conical_focal_on_circle
+    } else if (s.focal == 2) {
+        t = std::sqrt(tx * tx + ty * ty) - tx * s.p0;  // xy_to_2pt_conical_well_behaved
+    } else if (s.focal == 3) {
+        t = std::sqrt(tx * tx - ty * ty) - tx * s.p0;  // xy_to_2pt_conical_greater
+    } else if (s.radial) {
         t = std::sqrt(tx * tx + ty * ty);
         if (s.r_twist) t = (t - s.r_scale) / s.r_bias;
+    }
+    if (s.focal == 1 || s.focal == 3) {
+        degenerate = t <= 0.0f || t != t;
+        if (degenerate) t = 0.0f;
     }
     if (s.spread == RVK_SPREAD_REFLECT) {
         float u = t - 1.0f;
@@ -824,7 +866,11 @@ static void shade(const Shader &s, uint32_t px, uint32_t py, uint32_t out[4]) {
     } else if (s.pad_clamp) {
         t = clamp01(t);
     }
-    float c[4];
+    return t;
+}
+
+// gradient / evenly_spaced_2_stop_gradient: the colour at t, f32, unclamped
+static void grad_color(const Shader &s, float t, float c[4]) {
     if (s.two_stop) {
         for (int k = 0; k < 4; k++) c[k] = t * s.factor[k] + s.bias[k];
     } else {
@@ -832,6 +878,13 @@ static void shade(const Shader &s, uint32_t px, uint32_t py, uint32_t out[4]) {
         for (size_t i = 1; i < s.t.size(); i++) idx += t >= s.t[i] ? 1 : 0;
         for (int k = 0; k < 4; k++) c[k] = t * s.f[idx][k] + s.b[idx][k];
     }
+}
+
+static void shade(const Shader &s, uint32_t px, uint32_t py, uint32_t out[4]) {
+    if (s.solid) { for (int k = 0; k < 4; k++) out[k] = s.rgba[k]; return; }
+    bool degenerate;
+    float c[4];
+    grad_color(s, grad_t(s, px, py, degenerate), c);
     for (int k = 0; k < 4; k++) {
         float v = c[k];
         v = (v != v) ? 0.0f : clamp01(v);
@@ -844,6 +897,29 @@ static void shade(const Shader &s, uint32_t px, uint32_t py, uint32_t out[4]) {
 // (scale_1_float, source_over), or for an opaque shader Source (full coverage stores
 // the colour, partial coverage is lerp_1_float: div255(d * (255 - c) + s * c)).
 static void composite_px(const Shader &s, uint32_t c, uint32_t x, uint32_t y, uint8_t *d) {
+    if (s.focal) {
+        // tiny-skia's f32 pipeline: gradient, premultiply, apply_vector_mask,
+        // scale_1_float (coverage * (1/255)), load_dst (v * (1/255)), source_over
+        // (d * (1 - sa) + s), store (round-to-nearest-even of clamp(v) * 255)
+        bool degenerate;
+        float sc[4];
+        grad_color(s, grad_t(s, x, y, degenerate), sc);
+        if (!s.opaque) for (int k = 0; k < 3; k++) sc[k] *= sc[3];
+        if (degenerate) for (int k = 0; k < 4; k++) sc[k] = 0.0f;
+        if (c != 255) {
+            const float cf = (float)c * (1.0f / 255.0f);
+            for (int k = 0; k < 4; k++) sc[k] *= cf;
+        }
+        const float sa = sc[3];
+        for (int k = 0; k < 4; k++) {
+            const float dv = (float)d[k] * (1.0f / 255.0f);
+            float o = dv * (1.0f - sa) + sc[k];
+            o = o > 0.0f ? o : 0.0f;
+            o = o < 1.0f ? o : 1.0f;
+            d[k] = (uint8_t)std::nearbyint(o * 255.0f);
+        }
+        return;
+    }
     uint32_t src[4];
     shade(s, x, y, src);
     if (s.as_source) {

@@ -114,8 +114,15 @@ struct ShaderHead {
     uint32_t clear;
     float sx, ky, kx, sy, tx, ty;  // device -> unit transform
     float r_scale, r_bias;  // focal radius: t = (t - r_scale) / r_bias
+    // two-point conical gradient with the focal point off the centre (tiny-skia's
+    // FocalData, Skia's kFocal): kFocal* below, 0 for none; p0 = 1 / r1.  Such a shader
+    // runs in tiny-skia's f32 pipeline (its stages have no u8 form), so the colour,
+    // the coverage scale and SourceOver stay in f32 until the store
+    uint32_t focal;
+    float p0;
     uint32_t pad_;
 };
+constexpr uint32_t kFocalOnCircle = 1, kFocalWellBehaved = 2, kFocalGreater = 3;
 struct ShaderParams : ShaderHead {
     float t[kMaxStops + 2];
     float f[kMaxStops + 2][4];

@@ -649,20 +649,28 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
 __device__ __forceinline__ uint32_t div255(uint32_t v) { return (v + 255u) >> 8; }
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
-// premultiplied u8 source colour of a shader at pixel (px, py)
-__device__ __forceinline__ void shade(const ShaderParams &sp, const ShaderHead &s, int32_t px, int32_t py,
-                                      uint32_t o[4]) {
-    if (s.kind == 1) {
-        o[0] = s.rgba & 0xff; o[1] = (s.rgba >> 8) & 0xff; o[2] = (s.rgba >> 16) & 0xff; o[3] = s.rgba >> 24;
-        return;
-    }
+// gradient coordinate t of a gradient shader at pixel (px, py), after the spread
+// mode; `degenerate` is set where a two-point conical gradient is undefined (tiny-skia
+// mask_2pt_conical_degenerates: t <= 0 or NaN, with t forced to 0)
+__device__ __forceinline__ float grad_t(const ShaderHead &s, int32_t px, int32_t py, bool &degenerate) {
     const float x = (float)px + 0.5f, y = (float)py + 0.5f;
     const float tx = x * s.sx + (y * s.kx + s.tx);
     const float ty = x * s.ky + (y * s.sy + s.ty);
     float t = tx;
-    if (s.radial) {
+    degenerate = false;
+    if (s.focal == rvkr::kFocalOnCircle) {
+        t = tx + ty * ty / tx;
+    } else if (s.focal == rvkr::kFocalWellBehaved) {
+        t = sqrtf(tx * tx + ty * ty) - tx * s.p0;
+    } else if (s.focal == rvkr::kFocalGreater) {
+        t = sqrtf(tx * tx - ty * ty) - tx * s.p0;
+    } else if (s.radial) {
         t = sqrtf(tx * tx + ty * ty);
         if (s.r_twist) t = (t - s.r_scale) / s.r_bias;
+    }
+    if (s.focal != 0 && s.focal != rvkr::kFocalWellBehaved) {
+        degenerate = t <= 0.0f || t != t;
+        if (degenerate) t = 0.0f;
     }
     if (s.spread == RVK_SPREAD_REFLECT) {
         const float u = t - 1.0f;
@@ -672,7 +680,12 @@ __device__ __forceinline__ void shade(const ShaderParams &sp, const ShaderHead &
     } else if (s.pad_clamp) {
         t = clamp01(t);
     }
-    float c[4];
+    return t;
+}
+
+// the gradient's colour at t (tiny-skia's gradient / evenly_spaced_2_stop_gradient
+// stages, f32, before any clamp)
+__device__ __forceinline__ void grad_color(const ShaderParams &sp, const ShaderHead &s, float t, float c[4]) {
     if (s.two_stop) {
         for (int k = 0; k < 4; k++) c[k] = t * sp.f[0][k] + sp.b[0][k];
     } else {
@@ -687,11 +700,36 @@ __device__ __forceinline__ void shade(const ShaderParams &sp, const ShaderHead &
         }
         for (int k = 0; k < 4; k++) c[k] = t * sp.f[idx][k] + sp.b[idx][k];
     }
+}
+
+// premultiplied u8 source colour of a shader at pixel (px, py) (tiny-skia lowp)
+__device__ __forceinline__ void shade(const ShaderParams &sp, const ShaderHead &s, int32_t px, int32_t py,
+                                      uint32_t o[4]) {
+    if (s.kind == 1) {
+        o[0] = s.rgba & 0xff; o[1] = (s.rgba >> 8) & 0xff; o[2] = (s.rgba >> 16) & 0xff; o[3] = s.rgba >> 24;
+        return;
+    }
+    bool degenerate;
+    const float t = grad_t(s, px, py, degenerate);
+    float c[4];
+    grad_color(sp, s, t, c);
     for (int k = 0; k < 4; k++) {
         const float v = (c[k] != c[k]) ? 0.0f : clamp01(c[k]);
         o[k] = (uint32_t)(v * 255.0f + 0.5f);
     }
     if (!s.opaque) for (int k = 0; k < 3; k++) o[k] = div255(o[k] * o[3]);
+}
+
+// f32 premultiplied source colour of a two-point conical shader (tiny-skia highp:
+// gradient, premultiply, apply_vector_mask)
+__device__ __forceinline__ F4 shade_hp(const ShaderParams &sp, const ShaderHead &s, int32_t px, int32_t py) {
+    bool degenerate;
+    const float t = grad_t(s, px, py, degenerate);
+    float c[4];
+    grad_color(sp, s, t, c);
+    if (!s.opaque) { c[0] *= c[3]; c[1] *= c[3]; c[2] *= c[3]; }
+    if (degenerate) return F4{0.0f, 0.0f, 0.0f, 0.0f};
+    return F4{c[0], c[1], c[2], c[3]};
 }
 
 // lowp: scale by coverage (unless full), then SourceOver onto `d`; an opaque shader
@@ -713,7 +751,8 @@ __device__ __forceinline__ uint32_t over_lowp(const uint32_t src_in[4], uint32_t
 }
 
 // one covered pixel of a draw item: Clear, or the shaded colour composited with
-// over_lowp (the colour is shaded only when needed)
+// over_lowp, or for a two-point conical shader in f32 (scale_1_float by c / 255,
+// SourceOver, store); the colour is shaded only when needed
 __device__ __forceinline__ uint32_t paint_px(const ShaderParams &sp, const ShaderHead &sh, int32_t lx, int32_t ly,
                                              uint32_t c, uint32_t d, uint32_t src[4], bool &shaded) {
     if (sh.clear) {
@@ -721,6 +760,14 @@ __device__ __forceinline__ uint32_t paint_px(const ShaderParams &sp, const Shade
         uint32_t o = 0;
         for (int k = 0; k < 4; k++) o |= div255(((d >> (8 * k)) & 0xffu) * ic) << (8 * k);
         return o;
+    }
+    if (sh.focal) {
+        F4 s = shade_hp(sp, sh, lx, ly);
+        if (c != 255u) {
+            const float cf = (float)c * (1.0f / 255.0f);
+            s.r *= cf; s.g *= cf; s.b *= cf; s.a *= cf;
+        }
+        return store_px(blend(RVK_BLEND_SOURCE_OVER, s, load_px(d)));
     }
     if (!shaded) {
         shade(sp, sh, lx, ly, src);

@@ -656,6 +656,18 @@ void set_solid(ShaderParams &s, const float c[4]) {
 
 }  // namespace
 
+// ts_from_poly_to_poly(c0, c1, (0, 0), (1, 0)) (tiny-skia radial_gradient.rs): the
+// transform that maps c0 to the origin and c1 to (1, 0); false when c0 and c1 coincide.
+static bool poly_to_unit_x(float x0, float y0, float x1, float y1, Ts &out) {
+    // from_poly2(p0, p1) = [p1.y - p0.y, p0.x - p1.x, p1.x - p0.x, p1.y - p0.y, p0.x, p0.y]
+    const Ts src{y1 - y0, x0 - x1, x1 - x0, y1 - y0, x0, y0};
+    const Ts dst{0.0f, -1.0f, 1.0f, 0.0f, 0.0f, 0.0f};  // from_poly2((0, 0), (1, 0))
+    Ts inv;
+    if (!invert(src, inv)) return false;
+    out = ts_concat(dst, inv);
+    return true;
+}
+
 bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s, float extra_opacity) {
     s = ShaderParams{};
     // Shader::apply_opacity (Color::apply_opacity on the colour or on every stop)
@@ -708,9 +720,26 @@ bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s
         float sn = -vy, cs = vx, cinv = 1.0f - cs;
         Ts rot{cs, sn, -sn, cs, sn * p.y1 + cinv * p.x1, -sn * p.x1 + cinv * p.y1};
         unit = ts_concat(Ts{inv, 0, 0, inv, 0, 0}, ts_concat(Ts{1, 0, 0, 1, -p.x1, -p.y1}, rot));
+    } else if (float fdx = p.fx - p.cx, fdy = p.fy - p.cy, dc = std::sqrt(fdx * fdx + fdy * fdy);
+               !(dc <= 1.0f / 4096.0f)) {
+        // the focal point (fx, fy) off the centre: a two-point conical gradient from
+        // (fx, fy), radius fr, to (cx, cy), radius r (tiny-skia RadialGradient::new,
+        // SkTwoPointConicalGradient kFocal).  A focal radius with a moved focal point is
+        // not restated here.
+        if (!std::isfinite(dc) || p.fr != 0.0f || !(p.r > 0.0f)) return false;
+        Ts unit0;
+        if (!poly_to_unit_x(p.fx, p.fy, p.cx, p.cy, unit0)) return false;
+        const float r1 = p.r / dc;
+        if (std::fabs(1.0f - r1) <= 1.0f / 4096.0f) {
+            s.focal = kFocalOnCircle;
+            unit = ts_concat(Ts{0.5f, 0, 0, 0.5f, 0, 0}, unit0);
+        } else {
+            s.focal = r1 > 1.0f ? kFocalWellBehaved : kFocalGreater;
+            unit = ts_concat(Ts{r1 / (r1 * r1 - 1.0f), 0, 0, 1.0f / std::sqrt(std::fabs(r1 * r1 - 1.0f)), 0, 0}, unit0);
+        }
+        s.p0 = 1.0f / r1;
+        s.radial = 1;
     } else {
-        float fdx = p.fx - p.cx, fdy = p.fy - p.cy;
-        if (!(std::sqrt(fdx * fdx + fdy * fdy) <= 1.0f / 4096.0f)) return false;  // focal: not supported
         float r0 = p.fr, r1 = p.r;
         // equal radii make the twist divide by zero; the NaN t clamps to the first stop
         if (!(r1 >= 0.0f) || !(std::max(r0, r1) > 0.0f)) return false;

@@ -38,6 +38,12 @@ OTHER = [
     "paint-servers/radialGradient/default-attributes", "paint-servers/radialGradient/gradientUnits=userSpaceOnUse",
     "paint-servers/radialGradient/spreadMethod=reflect", "paint-servers/radialGradient/fr=0.5",
     "paint-servers/radialGradient/fr=0.2", "paint-servers/radialGradient/fr=0.7",
+    "paint-servers/radialGradient/fx-resolving-2", "paint-servers/radialGradient/fy-resolving-2",
+    "paint-servers/radialGradient/attributes-via-xlink-href",
+    "paint-servers/radialGradient/attributes-via-xlink-href-only-required",
+    "paint-servers/radialGradient/attributes-via-xlink-href-complex-order",
+    "paint-servers/radialGradient/gradientUnits=userSpaceOnUse-with-percent",
+    "paint-servers/radialGradient/focal-point-correction",
     "painting/opacity/50percent",
     "painting/mix-blend-mode/multiply", "painting/mix-blend-mode/screen", "painting/mix-blend-mode/hue",
     "painting/mix-blend-mode/darken", "painting/mix-blend-mode/color-burn", "painting/mix-blend-mode/normal",

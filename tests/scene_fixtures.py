@@ -94,6 +94,25 @@ def shape_fixtures():
     for fr in ("0.2", "0.5", "0.7"):
         out[f"paint-servers/radialGradient/fr={fr}"] = _rect_scene(
             PaintSpec.radial(0.5, 0.5, 0.5, [(0.0, BLACK, 1.0), (1.0, WHITE, 1.0)], fr=float(fr), transform=bb))
+    # focal points off the centre (two-point conical gradients): inside the circle,
+    # with pad and reflect spreads, in bbox and user units, and outside the circle
+    # (focal-point-correction: usvg keeps the point, so part of the plane is the
+    # undefined region tiny-skia masks out; its 100-unit viewBox is restated in
+    # 200-unit coordinates, which the 1.5 render scale maps to the same device space)
+    rg = "paint-servers/radialGradient/"
+    out[rg + "fx-resolving-2"] = _rect_scene(PaintSpec.radial(0.7, 0.5, 0.5, _wb(), fx=0.6, fy=0.5, transform=bb))
+    out[rg + "fy-resolving-2"] = _rect_scene(PaintSpec.radial(0.5, 0.7, 0.5, _wb(), fx=0.5, fy=0.6, transform=bb))
+    for n in ("attributes-via-xlink-href", "attributes-via-xlink-href-only-required"):
+        out[rg + n] = _rect_scene(PaintSpec.radial(0.5, 0.55, 0.5, _wb(), fx=0.6, fy=0.65, transform=bb))
+    out[rg + "attributes-via-xlink-href-complex-order"] = _rect_scene(
+        PaintSpec.radial(0.5, 0.55, 0.5, _wb(), fx=0.6, fy=0.65, spread=abi.SPREAD_REFLECT, transform=bb))
+    out[rg + "gradientUnits=userSpaceOnUse-with-percent"] = _rect_scene(
+        PaintSpec.radial(120.0, 100.0, 60.0, _wb(), fx=80.0, fy=90.0))
+    fpc = _rect_scene(PaintSpec.radial(20.0, 20.0, 150.0, _wb(), fx=float(np.float32(83.33)) * 2.0, fy=150.0),
+                      rect=(0.0, 0.0, 200.0, 200.0))
+    v, p = rect_path(2.0, 2.0, 196.0, 196.0)  # its frame, 1 unit wide at scale 3, reaches past the masked band
+    fpc.path(fpc.root, v, p, stroke=PaintSpec.solid(BLACK), stroke_width=2.0)
+    out[rg + "focal-point-correction"] = fpc
     sc = Scene(200, 200)
     g = sc.group(sc.root, opacity=0.5)
     v, p = rect_path(*R)

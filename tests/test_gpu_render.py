@@ -122,6 +122,38 @@ def test_random_strokes_and_groups(ctx, seed):
     _assert_same(_gpu(ctx, sc, ts, 220, 190), _oracle(sc, ts, 220, 190), f"strokes/groups seed {seed}")
 
 
+def _focal_paint(rng, size):
+    """A radial gradient whose focal point is off the centre: inside the circle (well
+    behaved), outside it (degenerate regions masked) or on it."""
+    c = rng.uniform(0.2 * size, 0.8 * size, 2)
+    r = float(np.float32(rng.uniform(0.1, 0.5) * size))
+    where = rng.random()
+    d = r * (rng.uniform(0.1, 0.9) if where < 0.45 else rng.uniform(1.1, 2.0) if where < 0.9 else 1.0)
+    a = rng.uniform(0, 2 * np.pi)
+    f = c + d * np.array([np.cos(a), np.sin(a)])
+    stops = _stops(rng, int(rng.integers(2, 6)), rng.random() < 0.5)
+    op = 1.0 if rng.random() < 0.6 else float(np.float32(rng.uniform(0.2, 1.0)))
+    return PaintSpec.radial(float(np.float32(c[0])), float(np.float32(c[1])), r, stops, opacity=op,
+                            spread=int(rng.integers(0, 3)), fx=float(np.float32(f[0])), fy=float(np.float32(f[1])))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_focal_radial_gradients(ctx, seed):
+    # two-point conical gradients (tiny-skia's f32 pipeline) on fills, strokes and
+    # hairlines, direct and in opacity groups, under a transform
+    rng = np.random.default_rng(700 + seed)
+    size = 160
+    sc = Scene(size, size)
+    for _ in range(10):
+        parent = sc.group(sc.root, opacity=float(np.float32(rng.uniform(0.3, 1.0)))) if rng.random() < 0.4 else sc.root
+        verbs, pts = _random_path(rng, size, int(rng.integers(1, 8)))
+        sc.path(parent, verbs, pts, fill=_focal_paint(rng, size),
+                stroke=_focal_paint(rng, size) if rng.random() < 0.5 else None,
+                stroke_width=float(np.float32(rng.uniform(0.5, 6.0))))
+    ts = abi.Transform(1.1, 0.05, -0.1, 0.95, 4.0, 2.0) if seed % 2 else abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, size, size), _oracle(sc, ts, size, size), f"focal radial seed {seed}")
+
+
 @pytest.mark.parametrize("seed", range(3))
 def test_paths_beyond_the_pixmap(ctx, seed):
     sc = _random_scene(200 + seed, 128, 10, margin=0.5)

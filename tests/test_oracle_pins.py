@@ -364,3 +364,59 @@ def test_turbulence_tables_equal_spec_init(seed):
     elat, egrad = npr.turbulence_init(seed)
     assert np.array_equal(lat, elat)
     assert np.array_equal(grad.view(np.uint64), egrad.view(np.uint64))
+
+
+# ---------------------------------------------------------------------------
+# Two-point conical gradients with the focal point off the centre (tiny-skia's
+# focal radial gradient): the gradient parameter at a pixel centre p is the largest
+# t >= 0 with |p - f - t (c - f)| = t r (circles grow from the focal point f to the
+# circle (c, r)), a quadratic in t; where it has no such root the shader is
+# transparent.  A black-to-white gradient shows t directly as the colour.
+# ---------------------------------------------------------------------------
+from resvg_amd.scene import PaintSpec, Scene, rect_path  # noqa: E402
+
+
+def _focal_t(px, py, f, c, r):
+    qx, qy = px - f[0], py - f[1]
+    dx, dy = c[0] - f[0], c[1] - f[1]
+    a = dx * dx + dy * dy - r * r
+    b = qx * dx + qy * dy
+    q2 = qx * qx + qy * qy
+    with np.errstate(invalid="ignore", divide="ignore"):
+        if abs(a) < 1e-9:
+            t = q2 / (2.0 * b)
+        else:
+            disc = b * b - a * q2
+            t = (b - np.sqrt(disc)) / a if a < 0 else (b + np.sqrt(disc)) / a
+    return t
+
+
+@pytest.mark.parametrize("where", ["inside", "on", "outside"])
+def test_focal_gradient_parameter_is_the_conical_root(where):
+    size = 96
+    c, r = (50.0, 46.0), 30.0
+    d = {"inside": 0.55, "on": 1.0, "outside": 1.6}[where] * r
+    f = (c[0] + d * 0.8, c[1] - d * 0.6)
+    f = (float(np.float32(f[0])), float(np.float32(f[1])))
+    sc = Scene(size, size)
+    v, p = rect_path(0.0, 0.0, float(size), float(size))
+    sc.path(sc.root, v, p, fill=PaintSpec.radial(c[0], c[1], r, [(0.0, (0, 0, 0), 1.0), (1.0, (255, 255, 255), 1.0)],
+                                                 fx=f[0], fy=f[1]))
+    px = np.zeros((size, size, 4), np.uint8)
+    orc.render(sc.to_abi().abi, abi.Transform.identity(), px)
+    ys, xs = np.mgrid[0:size, 0:size].astype(np.float64) + 0.5
+    t = np.vectorize(lambda x, y: _focal_t(x, y, f, c, r))(xs, ys)
+    defined = np.isfinite(t) & (t > 0)
+    exp = np.rint(np.clip(np.where(defined, t, 0.0), 0.0, 1.0) * 255.0)
+    got_a = px[..., 3].astype(np.int32)
+    # away from the boundary of the undefined region (where f32 and f64 may disagree),
+    # defined pixels are opaque with colour = t and undefined ones are transparent
+    interior = defined.copy()
+    if not defined.all():
+        from scipy.ndimage import binary_erosion
+        interior = binary_erosion(defined, iterations=2) | binary_erosion(~defined, iterations=2)
+    assert np.array_equal(got_a[interior], np.where(defined, 255, 0)[interior])
+    diff = np.abs(px[..., 0].astype(np.int32) - exp)[interior & defined]
+    assert diff.max() <= 1, (where, diff.max())
+    if where == "outside":
+        assert (~defined).sum() > 100  # the undefined cone is on the canvas
