@@ -227,8 +227,14 @@ __device__ __forceinline__ void bitonic_from(uint64_t (&v)[S][K], int lane) {
 // Sorts the S sub-scanlines' keys (n[s] each, in keys[s]) together and writes each
 // one's span events (pixel-rounded supersampled columns; starts and ends alternate)
 // over its own key area.  Returns the event counts in nev.
+// A column chunk of the row (crossings rounded to [c0, c1) only) starts from the
+// backdrop winding base[s] of the crossings left of it: inside a span there, the
+// events get a leading start at c0, and a span still open after the last crossing
+// gets an end at c1.  The whole row is the chunk with base 0 (closed paths end
+// outside).
 template <int K, int S, int MK>
-__device__ __forceinline__ void spans_of(uint64_t (*keys)[MK], const int *n, bool evenodd, int *nev, int lane) {
+__device__ __forceinline__ void spans_of(uint64_t (*keys)[MK], const int *n, bool evenodd, int *nev, int lane,
+                                         const int *base, int32_t c0, int32_t c1) {
     uint64_t v[S][K];
 #pragma unroll
     for (int s = 0; s < S; s++)
@@ -249,7 +255,10 @@ __device__ __forceinline__ void spans_of(uint64_t (*keys)[MK], const int *n, boo
             local += w[j];
         }
         const int incl = wave_incl_scan(local, lane);
-        int before = incl - local;
+        int before = base[s] + incl - local;
+        const bool in0 = evenodd ? (base[s] & 1) != 0 : base[s] != 0;
+        const int total = base[s] + __shfl(incl, kWave - 1, kWave);
+        const bool in1 = evenodd ? (total & 1) != 0 : total != 0;
         bool evt[K];
         int nevt = 0;
 #pragma unroll
@@ -262,7 +271,7 @@ __device__ __forceinline__ void spans_of(uint64_t (*keys)[MK], const int *n, boo
             before = after;
         }
         const int eincl = wave_incl_scan(nevt, lane);
-        int pos = eincl - nevt;
+        int pos = eincl - nevt + (in0 ? 1 : 0);
         int32_t *ev = reinterpret_cast<int32_t *>(keys[s]);
 #pragma unroll
         for (int j = 0; j < K; j++)
@@ -270,7 +279,10 @@ __device__ __forceinline__ void spans_of(uint64_t (*keys)[MK], const int *n, boo
                 const int32_t x = (int32_t)((uint32_t)(v[s][j] >> 32) ^ 0x80000000u);
                 ev[pos++] = (x + 0x8000) >> 16;
             }
-        nev[s] = __shfl(eincl, kWave - 1, kWave);
+        const int ne = (in0 ? 1 : 0) + __shfl(eincl, kWave - 1, kWave);
+        if (lane == 0 && in0) ev[0] = c0;
+        if (lane == 0 && in1) ev[ne] = c1;
+        nev[s] = ne + (in1 ? 1 : 0);
     }
 }
 
@@ -367,15 +379,25 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
     }
 
     uint32_t nslots = 0;  // partial tiles of the band so far
-    for (uint32_t ct0 = 0; ct0 < job.tiles_x; ct0 += rvkr::kChunkTiles) {
-        const uint32_t ctn = min((uint32_t)rvkr::kChunkTiles, job.tiles_x - ct0);
+    // The band is processed in column chunks of up to kChunkTiles tiles.  Normally a
+    // row's every crossing is sorted; when a sub-scanline holds more crossings than the
+    // variant's key area, the chunk is redone at half the width, and from then on each
+    // chunk keeps only the crossings that round into its columns and starts from the
+    // winding of those left of it (down to single tiles).
+    uint32_t chunk_tiles = rvkr::kChunkTiles;
+    bool cut = false;  // chunks keep only their own crossings
+    for (uint32_t ct0 = 0; ct0 < job.tiles_x;) {
+        const uint32_t ctn = min(chunk_tiles, job.tiles_x - ct0);
         const int32_t cx0 = job.gx0 + (int32_t)ct0 * rvkr::kTile;  // first column of the chunk
         const int32_t cw = (int32_t)ctn * rvkr::kTile;
+        const int32_t c0s = cut ? cx0 * nsub_aa : INT32_MIN, c1s = cut ? (cx0 + cw) * nsub_aa : INT32_MAX;
+        const uint32_t nslots0 = nslots;
+        bool redo = false;
         // a chunk after the first walks the band's rows again from its start
         uint32_t k0[J];
 #pragma unroll
         for (int j = 0; j < J; j++) k0[j] = ek[j];
-        for (int r = 0; r < rvkr::kBandRows; r++) {
+        for (int r = 0; r < rvkr::kBandRows && !redo; r++) {
             const int32_t y = row0 + r;
             const bool row_in = y >= ry0 && y < ry1;
             int nev[kMaxSub] = {0, 0, 0, 0};
@@ -383,11 +405,14 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
                 // ---- the crossings of the row's sub-scanlines ----
                 const int32_t sy0 = y * nsub_aa;
                 int cnt[kMaxSub] = {0, 0, 0, 0};
+                int bd[kMaxSub] = {0, 0, 0, 0};  // winding of the crossings left of the chunk
                 // the crossings of edge i on the row's sub-scanlines, stepping its piece
-                // index k forward; compacted into the key areas in lane order
+                // index k forward; those in the chunk are compacted into the key areas in
+                // lane order, those left of it are summed into bd
                 auto gather = [&](bool valid, uint32_t i, uint32_t &k) {
                     uint64_t key[kMaxSub];
                     bool act[kMaxSub] = {false, false, false, false};
+                    int left[kMaxSub] = {0, 0, 0, 0};
                     if (valid) {
                         const EdgeCov e = edges[i];
                         const PieceDev *pc = a.pieces + e.piece_offset;
@@ -399,9 +424,14 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
                                 while (p.last_y < sy && k + 1 < n) p = pc[++k];
                                 if (p.first_y <= sy && p.last_y >= sy) {
                                     const int32_t x = (int32_t)((uint32_t)p.x + (uint32_t)(sy - p.first_y) * (uint32_t)p.dx);
-                                    key[s] = ((uint64_t)((uint32_t)x ^ 0x80000000u) << 32) |
-                                             ((uint64_t)(uint32_t)e.id << 1) | (e.winding > 0 ? 1u : 0u);
-                                    act[s] = true;
+                                    const int32_t xr = (x + 0x8000) >> 16;  // the sub-column of its event
+                                    if (xr < c0s) {
+                                        left[s] = e.winding > 0 ? 1 : -1;
+                                    } else if (xr < c1s) {
+                                        key[s] = ((uint64_t)((uint32_t)x ^ 0x80000000u) << 32) |
+                                                 ((uint64_t)(uint32_t)e.id << 1) | (e.winding > 0 ? 1u : 0u);
+                                        act[s] = true;
+                                    }
                                 }
                             }
                         }
@@ -411,6 +441,7 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
                         const int pos = cnt[s] + __popcll(m & below);
                         if (act[s] && pos < MK) keys[s][pos] = key[s];
                         cnt[s] += __popcll(m);
+                        if (cut) bd[s] += __popcll(__ballot(left[s] > 0)) - __popcll(__ballot(left[s] < 0));
                     }
                 };
                 if (nact <= MK) {
@@ -445,29 +476,34 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
                 __syncthreads();
                 // ---- sort the sub-scanlines together and turn them into span events ----
                 int nmax = 0;
-                for (int s = 0; s < nsub_aa; s++) {
-                    if (cnt[s] > MK) {
-                        if (lane == 0) atomicAdd(a.overflow, 1);
-                        cnt[s] = MK;
+                for (int s = 0; s < nsub_aa; s++) nmax = max(nmax, cnt[s]);
+                if (nmax > MK) {
+                    if (ctn > 1) {  // too many crossings: this chunk again, narrower
+                        redo = true;
+                        break;
                     }
-                    nmax = max(nmax, cnt[s]);
+                    // more than MK crossings round into one tile's columns: kept
+                    // truncated, and the render reports it
+                    if (lane == 0) atomicAdd(a.overflow, 1);
+                    for (int s = 0; s < nsub_aa; s++) cnt[s] = min(cnt[s], MK);
+                    nmax = MK;
                 }
                 // keys per lane: the smallest network that holds the fullest sub-scanline
                 // (cnt is at most MK, so no branch needs more than MK / 64 keys per lane)
                 constexpr int K4 = MK >= 256 ? 4 : 2, K8 = MK >= 512 ? 8 : K4;
                 static_assert(K8 * kWave <= MK && K4 * kWave <= MK, "a network never reads past the key area");
-                if (nsub_aa == 1) {
-                    if (nmax == 0) {}
-                    else if (nmax <= 64) spans_of<1, 1, MK>(keys, cnt, evenodd, nev, lane);
-                    else if (nmax <= 128) spans_of<2, 1, MK>(keys, cnt, evenodd, nev, lane);
-                    else if (nmax <= 256) spans_of<K4, 1, MK>(keys, cnt, evenodd, nev, lane);
-                    else spans_of<K8, 1, MK>(keys, cnt, evenodd, nev, lane);
+                // (a cut chunk with no crossings inside can still lie inside a span)
+                if (nmax == 0 && !cut) {
+                } else if (nsub_aa == 1) {
+                    if (nmax <= 64) spans_of<1, 1, MK>(keys, cnt, evenodd, nev, lane, bd, c0s, c1s);
+                    else if (nmax <= 128) spans_of<2, 1, MK>(keys, cnt, evenodd, nev, lane, bd, c0s, c1s);
+                    else if (nmax <= 256) spans_of<K4, 1, MK>(keys, cnt, evenodd, nev, lane, bd, c0s, c1s);
+                    else spans_of<K8, 1, MK>(keys, cnt, evenodd, nev, lane, bd, c0s, c1s);
                 } else {
-                    if (nmax == 0) {}
-                    else if (nmax <= 64) spans_of<1, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
-                    else if (nmax <= 128) spans_of<2, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
-                    else if (nmax <= 256) spans_of<K4, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
-                    else spans_of<K8, kMaxSub, MK>(keys, cnt, evenodd, nev, lane);
+                    if (nmax <= 64) spans_of<1, kMaxSub, MK>(keys, cnt, evenodd, nev, lane, bd, c0s, c1s);
+                    else if (nmax <= 128) spans_of<2, kMaxSub, MK>(keys, cnt, evenodd, nev, lane, bd, c0s, c1s);
+                    else if (nmax <= 256) spans_of<K4, kMaxSub, MK>(keys, cnt, evenodd, nev, lane, bd, c0s, c1s);
+                    else spans_of<K8, kMaxSub, MK>(keys, cnt, evenodd, nev, lane, bd, c0s, c1s);
                 }
                 __syncthreads();
             }
@@ -615,6 +651,13 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
                 __syncthreads();
             }
         }
+        if (redo) {
+            chunk_tiles = max(ctn / 2, 1u);
+            cut = true;
+            nslots = nslots0;
+            __syncthreads();
+            continue;
+        }
         // ---- the chunk's tile table entries ----
         for (uint32_t t0 = 0; t0 < ctn; t0 += kWave) {
             const uint32_t t = t0 + (uint32_t)lane;
@@ -640,6 +683,7 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
             if (valid) tab[ct0 + t] = (uint16_t)e;
         }
         __syncthreads();
+        ct0 += ctn;
     }
 }
 
@@ -649,29 +693,8 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
 __device__ __forceinline__ uint32_t div255(uint32_t v) { return (v + 255u) >> 8; }
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
-// gradient coordinate t of a gradient shader at pixel (px, py), after the spread
-// mode; `degenerate` is set where a two-point conical gradient is undefined (tiny-skia
-// mask_2pt_conical_degenerates: t <= 0 or NaN, with t forced to 0)
-__device__ __forceinline__ float grad_t(const ShaderHead &s, int32_t px, int32_t py, bool &degenerate) {
-    const float x = (float)px + 0.5f, y = (float)py + 0.5f;
-    const float tx = x * s.sx + (y * s.kx + s.tx);
-    const float ty = x * s.ky + (y * s.sy + s.ty);
-    float t = tx;
-    degenerate = false;
-    if (s.focal == rvkr::kFocalOnCircle) {
-        t = tx + ty * ty / tx;
-    } else if (s.focal == rvkr::kFocalWellBehaved) {
-        t = sqrtf(tx * tx + ty * ty) - tx * s.p0;
-    } else if (s.focal == rvkr::kFocalGreater) {
-        t = sqrtf(tx * tx - ty * ty) - tx * s.p0;
-    } else if (s.radial) {
-        t = sqrtf(tx * tx + ty * ty);
-        if (s.r_twist) t = (t - s.r_scale) / s.r_bias;
-    }
-    if (s.focal != 0 && s.focal != rvkr::kFocalWellBehaved) {
-        degenerate = t <= 0.0f || t != t;
-        if (degenerate) t = 0.0f;
-    }
+// the spread mode applied to a gradient coordinate
+__device__ __forceinline__ float spread_t(const ShaderHead &s, float t) {
     if (s.spread == RVK_SPREAD_REFLECT) {
         const float u = t - 1.0f;
         t = clamp01(fabsf(u - 2.0f * floorf(u * 0.5f) - 1.0f));
@@ -681,6 +704,38 @@ __device__ __forceinline__ float grad_t(const ShaderHead &s, int32_t px, int32_t
         t = clamp01(t);
     }
     return t;
+}
+
+// gradient coordinate t of a linear or concentric radial shader at pixel (px, py)
+__device__ __forceinline__ float grad_t(const ShaderHead &s, int32_t px, int32_t py) {
+    const float x = (float)px + 0.5f, y = (float)py + 0.5f;
+    const float tx = x * s.sx + (y * s.kx + s.tx);
+    float t = tx;
+    if (s.radial) {
+        const float ty = x * s.ky + (y * s.sy + s.ty);
+        t = sqrtf(tx * tx + ty * ty);
+        if (s.r_twist) t = (t - s.r_scale) / s.r_bias;
+    }
+    return spread_t(s, t);
+}
+
+// gradient coordinate t of a two-point conical shader; `degenerate` is set where the
+// gradient is undefined (tiny-skia mask_2pt_conical_degenerates: t <= 0 or NaN, with t
+// forced to 0)
+__device__ __forceinline__ float grad_t_focal(const ShaderHead &s, int32_t px, int32_t py, bool &degenerate) {
+    const float x = (float)px + 0.5f, y = (float)py + 0.5f;
+    const float tx = x * s.sx + (y * s.kx + s.tx);
+    const float ty = x * s.ky + (y * s.sy + s.ty);
+    float t;
+    if (s.focal == rvkr::kFocalOnCircle) t = tx + ty * ty / tx;
+    else if (s.focal == rvkr::kFocalWellBehaved) t = sqrtf(tx * tx + ty * ty) - tx * s.p0;
+    else t = sqrtf(tx * tx - ty * ty) - tx * s.p0;
+    degenerate = false;
+    if (s.focal != rvkr::kFocalWellBehaved) {
+        degenerate = t <= 0.0f || t != t;
+        if (degenerate) t = 0.0f;
+    }
+    return spread_t(s, t);
 }
 
 // the gradient's colour at t (tiny-skia's gradient / evenly_spaced_2_stop_gradient
@@ -709,10 +764,8 @@ __device__ __forceinline__ void shade(const ShaderParams &sp, const ShaderHead &
         o[0] = s.rgba & 0xff; o[1] = (s.rgba >> 8) & 0xff; o[2] = (s.rgba >> 16) & 0xff; o[3] = s.rgba >> 24;
         return;
     }
-    bool degenerate;
-    const float t = grad_t(s, px, py, degenerate);
     float c[4];
-    grad_color(sp, s, t, c);
+    grad_color(sp, s, grad_t(s, px, py), c);
     for (int k = 0; k < 4; k++) {
         const float v = (c[k] != c[k]) ? 0.0f : clamp01(c[k]);
         o[k] = (uint32_t)(v * 255.0f + 0.5f);
@@ -720,16 +773,22 @@ __device__ __forceinline__ void shade(const ShaderParams &sp, const ShaderHead &
     if (!s.opaque) for (int k = 0; k < 3; k++) o[k] = div255(o[k] * o[3]);
 }
 
-// f32 premultiplied source colour of a two-point conical shader (tiny-skia highp:
-// gradient, premultiply, apply_vector_mask)
-__device__ __forceinline__ F4 shade_hp(const ShaderParams &sp, const ShaderHead &s, int32_t px, int32_t py) {
+// A covered pixel under a two-point conical shader, in tiny-skia's f32 pipeline:
+// gradient, premultiply, apply_vector_mask, scale_1_float by c / 255, SourceOver onto
+// d, store (kept out of line: the compositor's common paths do not carry it)
+__device__ __noinline__ uint32_t paint_px_hp(const ShaderParams &sp, const ShaderHead &s, int32_t px, int32_t py,
+                                             uint32_t c, uint32_t d) {
     bool degenerate;
-    const float t = grad_t(s, px, py, degenerate);
-    float c[4];
-    grad_color(sp, s, t, c);
-    if (!s.opaque) { c[0] *= c[3]; c[1] *= c[3]; c[2] *= c[3]; }
-    if (degenerate) return F4{0.0f, 0.0f, 0.0f, 0.0f};
-    return F4{c[0], c[1], c[2], c[3]};
+    const float t = grad_t_focal(s, px, py, degenerate);
+    float col[4];
+    grad_color(sp, s, t, col);
+    if (!s.opaque) { col[0] *= col[3]; col[1] *= col[3]; col[2] *= col[3]; }
+    F4 src = degenerate ? F4{0.0f, 0.0f, 0.0f, 0.0f} : F4{col[0], col[1], col[2], col[3]};
+    if (c != 255u) {
+        const float cf = (float)c * (1.0f / 255.0f);
+        src.r *= cf; src.g *= cf; src.b *= cf; src.a *= cf;
+    }
+    return store_px(blend(RVK_BLEND_SOURCE_OVER, src, load_px(d)));
 }
 
 // lowp: scale by coverage (unless full), then SourceOver onto `d`; an opaque shader
@@ -761,14 +820,7 @@ __device__ __forceinline__ uint32_t paint_px(const ShaderParams &sp, const Shade
         for (int k = 0; k < 4; k++) o |= div255(((d >> (8 * k)) & 0xffu) * ic) << (8 * k);
         return o;
     }
-    if (sh.focal) {
-        F4 s = shade_hp(sp, sh, lx, ly);
-        if (c != 255u) {
-            const float cf = (float)c * (1.0f / 255.0f);
-            s.r *= cf; s.g *= cf; s.b *= cf; s.a *= cf;
-        }
-        return store_px(blend(RVK_BLEND_SOURCE_OVER, s, load_px(d)));
-    }
+    if (sh.focal) return paint_px_hp(sp, sh, lx, ly, c, d);
     if (!shaded) {
         shade(sp, sh, lx, ly, src);
         shaded = true;

@@ -387,3 +387,41 @@ def test_render_batch_arguments(ctx):
     assert e.value.status == abi.ERR_INVALID_ARGUMENT
     ctx.synchronize()
     assert not pxs[0].any()  # arguments are checked before any document is drawn
+
+
+def _comb(n_teeth, x0, x1, y0, y1, jitter_seed=None):
+    """A closed zigzag: n_teeth narrow teeth between x0 and x1, so that every row in
+    (y0, y1) crosses 2 * n_teeth edges."""
+    rng = np.random.default_rng(jitter_seed) if jitter_seed is not None else None
+    xs = np.linspace(x0, x1, 2 * n_teeth + 1)
+    pts = [(float(x0), float(y1))]
+    for i, x in enumerate(xs[1:]):
+        y = y0 if i % 2 == 0 else y1
+        if rng is not None:
+            y += float(rng.uniform(-0.3, 0.3))
+        pts.append((float(x), float(y)))
+    pts.append((float(x1), float(y1) + 4.0))
+    pts.append((float(x0), float(y1) + 4.0))
+    verbs = [abi.VERB_MOVE] + [abi.VERB_LINE] * (len(pts) - 1) + [abi.VERB_CLOSE]
+    return verbs, [float(np.float32(v)) for p in pts for v in p]
+
+
+@pytest.mark.parametrize("case", ["nonzero", "evenodd", "stroked", "layer"])
+def test_more_crossings_per_row_than_the_key_area(ctx, case):
+    # 1100 teeth across 2304 pixels: 2200 crossings on every sub-scanline of the comb's
+    # rows, four times the widest coverage variant's 512 keys; the band is cut into
+    # column chunks that each hold their own crossings plus the winding left of them
+    w, h = 2304, 112
+    sc = Scene(w, h)
+    verbs, pts = _comb(1100, 2.0, w - 2.0, 8.0, 96.0, jitter_seed=3)
+    parent = sc.group(sc.root, opacity=0.7) if case == "layer" else sc.root
+    sc.path(parent, verbs, pts, fill=PaintSpec.linear(0.0, 0.0, float(w), 0.0,
+                                                      [(0.0, (250, 20, 40), 1.0), (1.0, (20, 40, 250), 0.8)]),
+            fill_rule=abi.FILL_EVENODD if case == "evenodd" else abi.FILL_NONZERO,
+            stroke=PaintSpec.solid((0, 0, 0), opacity=0.6) if case == "stroked" else None, stroke_width=1.5)
+    # a second comb, shifted by a fraction of a tooth, wound the other way round
+    v2, p2 = _comb(1100, 3.3, w - 0.7, 20.0, 80.0)
+    p2 = [p2[i] for i in range(len(p2))]
+    sc.path(sc.root, v2, p2, fill=PaintSpec.solid((30, 160, 60), opacity=0.5))
+    ts = abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, w, h), _oracle(sc, ts, w, h), f"comb {case}")
