@@ -194,6 +194,11 @@ def cpu_parity(ctx, seed):
     return out
 
 
+def _build_doc(spec):
+    from resvg_amd.workloads import batch_doc
+    return batch_doc(*spec)
+
+
 def batched(ctx, n_docs, seed, steps):
     """The batched form: `n_docs` documents of the batched distribution (sides
     log-uniform 64-2048, aspect 0.5-2, 5-200 paths, a 1-6 primitive filter chain each)
@@ -202,9 +207,13 @@ def batched(ctx, n_docs, seed, steps):
     the timed region; both forms composite onto the same resident device pixmaps."""
     import torch
     from resvg_amd import abi
-    from resvg_amd.workloads import document_batch
+    from resvg_amd.workloads import document_specs
     t0 = time.perf_counter()
-    docs = document_batch(n_docs, seed=seed)
+    specs = document_specs(n_docs, seed=seed)
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    with mp.get_context("spawn").Pool(workers) as pool:
+        built = pool.map(_build_doc, specs, chunksize=max(1, n_docs // (4 * workers)))
+    docs = [(sc, (sp[1], sp[2])) for sc, sp in zip(built, specs)]
     scenes = [sc.to_abi() for sc, _ in docs]
     pxs = [torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda") for _, (w, h) in docs]
     print(f"[bench] batched: {n_docs} documents built in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
@@ -235,7 +244,7 @@ def batched(ctx, n_docs, seed, steps):
             "speedup_vs_loop": round(out["loop"] / out["batch"], 3),
             "mpixel_per_s": round(mpix / out["batch"], 2), "ms_per_pass": round(out["batch"] * 1e3, 2),
             "loop_ms_per_pass": round(out["loop"] * 1e3, 2), "passes": steps,
-            "sample": f"{n_docs} documents of the 4096-document batched distribution (seed {seed})"}
+            "sample": f"{n_docs} documents of the batched distribution (seed {seed})"}
 
 
 def differentiable(ctx, size, n_paths, seed, reps):
@@ -374,8 +383,8 @@ def main() -> int:
     ap.add_argument("--cpu-runs", type=int, default=5, help="CPU sample: timed runs (after 1 warm-up)")
     ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline processes (0: min(16, cores))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch-docs", type=int, default=512, help="documents of the batched leg (0: skip it)")
-    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-docs", type=int, default=4096, help="documents of the batched leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=1)
     ap.add_argument("--diff-size", type=int, default=1024, help="canvas side of the differentiable leg (0: skip)")
     ap.add_argument("--diff-paths", type=int, default=256)
     ap.add_argument("--fit-scenes", type=int, default=64, help="scenes of the batched fitting leg (0: skip)")

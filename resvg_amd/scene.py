@@ -92,7 +92,8 @@ class Scene:
         self.groups: list[dict] = []
         self.paths: list[dict] = []
         self.verbs: list[int] = []
-        self.points: list[float] = []
+        self._point_chunks: list[np.ndarray] = []  # float32 x, y pairs of every path, in order
+        self._n_point_values = 0
         self.stops: list[tuple] = []
         self.filters: list[tuple] = []  # (rect, prim_offset, prim_count)
         self.primitives: list = []
@@ -158,18 +159,19 @@ class Scene:
              stroke_cap: int = abi.CAP_BUTT, miter_limit: float = 4.0, anti_alias: bool = True,
              visible: bool = True, stroke_first: bool = False, dasharray=None, dashoffset: float = 0.0) -> int:
         """`dasharray` is usvg's resolved list (non-negative, even length, non-zero sum)."""
-        pts = [f32(v) for v in np.asarray(points, dtype=np.float64).reshape(-1)]
+        arr = np.asarray(points, dtype=np.float64).reshape(-1).astype(np.float32)  # f32(v) of every value
         pid = len(self.paths)
-        self.paths.append(dict(verb_offset=len(self.verbs), verb_count=len(verbs), point_offset=len(self.points) // 2,
+        self.paths.append(dict(verb_offset=len(self.verbs), verb_count=len(verbs), point_offset=self._n_point_values // 2,
                                fill=fill or PaintSpec(), fill_rule=fill_rule, stroke=stroke or PaintSpec(),
                                stroke_width=stroke_width, join=stroke_join, cap=stroke_cap, miter=miter_limit,
                                aa=anti_alias, visible=visible, stroke_first=stroke_first,
-                               npts=len(pts) // 2, pts=np.array(pts, np.float32).reshape(-1, 2),
+                               npts=len(arr) // 2, pts=arr.reshape(-1, 2),
                                dash_offset=len(self.dashes), dash_count=len(dasharray or ()),
                                dash_phase=f32(dashoffset)))
         self.dashes.extend(f32(v) for v in (dasharray or ()))
         self.verbs.extend(int(v) for v in verbs)
-        self.points.extend(pts)
+        self._point_chunks.append(arr)
+        self._n_point_values += len(arr)
         self.groups[parent]["children"].append((abi.NODE_PATH, pid))
         self._abi = None
         return pid
@@ -314,7 +316,8 @@ class Scene:
             Q.dash_offset, Q.dash_count, Q.dash_phase = p["dash_offset"], p["dash_count"], p["dash_phase"]
         ch = abi.array(abi.NodeRef, children)
         verbs = abi.array(C.c_uint8, self.verbs)
-        pts = np.ascontiguousarray(np.array(self.points if self.points else [0.0], dtype=np.float32))
+        pts = np.ascontiguousarray(np.concatenate(self._point_chunks) if self._point_chunks
+                                   else np.zeros(1, np.float32))
         st = abi.array(abi.Stop, stops)
         filt = abi.array(abi.Filter, [abi.Filter(abi.Rect(*r), o, n) for r, o, n in self.filters])
         prims = abi.array(abi.Primitive, self.primitives)

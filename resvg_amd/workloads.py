@@ -280,13 +280,13 @@ def batch_doc(seed: int, w: int, h: int, n_paths: int, n_prims: int):
     return sc
 
 
-def document_batch(n_docs: int, seed: int = 0, min_side: int = 64, max_side: int = 2048, min_paths: int = 5,
+def document_specs(n_docs: int, seed: int = 0, min_side: int = 64, max_side: int = 2048, min_paths: int = 5,
                    max_paths: int = 200):
-    """The batched workload: `n_docs` documents with sides log-uniform in
-    [min_side, max_side] and aspect 0.5-2, 5-200 paths and a 1-6 primitive filter
-    chain each.  Returns [(Scene, (w, h))]."""
+    """The batched workload's documents as batch_doc arguments (seed, w, h, paths,
+    filter primitives): sides log-uniform in [min_side, max_side], aspect 0.5-2,
+    5-200 paths and a 1-6 primitive filter chain each."""
     rng = np.random.default_rng(seed)
-    docs = []
+    specs = []
     for _ in range(n_docs):
         side = float(np.exp(rng.uniform(np.log(min_side), np.log(max_side))))
         aspect = float(np.exp(rng.uniform(np.log(0.5), np.log(2.0))))
@@ -294,8 +294,15 @@ def document_batch(n_docs: int, seed: int = 0, min_side: int = 64, max_side: int
         h = int(max(min_side, min(max_side, round(side / np.sqrt(aspect)))))
         n = int(rng.integers(min_paths, max_paths + 1))
         k = int(rng.integers(1, 7))
-        docs.append((batch_doc(int(rng.integers(0, 2**31)), w, h, n, k), (w, h)))
-    return docs
+        specs.append((int(rng.integers(0, 2**31)), w, h, n, k))
+    return specs
+
+
+def document_batch(n_docs: int, seed: int = 0, min_side: int = 64, max_side: int = 2048, min_paths: int = 5,
+                   max_paths: int = 200):
+    """The batched workload (document_specs, built).  Returns [(Scene, (w, h))]."""
+    return [(batch_doc(*sp), (sp[1], sp[2]))
+            for sp in document_specs(n_docs, seed, min_side, max_side, min_paths, max_paths)]
 
 
 def diff_scene(seed: int = 0, size: int = 1024, n_paths: int = 256, segments: int = 8, sigma: float = 4.0):
