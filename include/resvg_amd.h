@@ -41,7 +41,7 @@ typedef int32_t rvk_status;
 #define RVK_ERR_OUT_OF_MEMORY    2  /* device allocation failed */
 #define RVK_ERR_HIP              3  /* a HIP runtime call failed */
 #define RVK_ERR_INVALID_REGION   4  /* filter::Error::InvalidRegion (filter/mod.rs:86-90) */
-#define RVK_ERR_UNSUPPORTED      5  /* feature outside this library's scope (feImage, patterns, images) */
+#define RVK_ERR_UNSUPPORTED      5  /* feature outside this library's scope (raster images, patterns) */
 #define RVK_ERR_NO_DEVICE        6  /* no HIP device / library built without a device */
 #define RVK_ERR_TOO_LARGE        7  /* input beyond a fixed limit of the entry point (the no-allocation form) */
 
@@ -166,6 +166,9 @@ typedef struct rvk_primitive {
         struct { float dx, dy; } offset;
         struct { float base_frequency_x, base_frequency_y; uint32_t num_octaves;
                  int32_t seed; uint32_t stitch_tiles, fractal_noise; } turbulence;
+        struct { uint32_t root; } image;  /* feImage (usvg filter::Image::root): a scene group whose
+                                             children are drawn; only inside rvk_render and its
+                                             relatives (rvk_filter_apply has no scene: unsupported) */
         float pad_[48];
     } u;
 } rvk_primitive;

@@ -199,6 +199,20 @@ def path_coverage(scene_abi, path_index: int, ts: abi.Transform, stroke: bool, w
     return cov
 
 
+def edge_clip(segs: np.ndarray, w: int, h: int) -> np.ndarray:
+    """tiny-skia's EdgeClipper on rows (point count, x0, y0, .. x3, y3) of float32."""
+    segs = np.ascontiguousarray(segs, dtype=np.float32).reshape(-1, 9)
+    cap = 16 * len(segs) + 16
+    out = np.zeros((cap, 9), np.float32)
+    L = lib()
+    L.orc_edge_clip.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]
+    L.orc_edge_clip.restype = C.c_int32
+    n = L.orc_edge_clip(segs.ctypes.data, len(segs), w, h, out.ctypes.data, cap)
+    if n < 0:
+        raise ValueError("bad segment rows")
+    return out[:n]
+
+
 def fill_path(scene_abi, path_index: int, ts: abi.Transform, a) -> int:
     return lib().orc_fill_path(C.byref(scene_abi), path_index, ts, _p(a), *_wh(a))
 

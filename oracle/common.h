@@ -7,7 +7,15 @@
 #include <algorithm>
 #include <chrono>
 
+#include <functional>
+
 namespace orc {
+
+// feImage content drawing (filter/mod.rs:861-889): set by the render walk while it
+// applies a group's filters; draws the children of scene group `root` with the given
+// transform (sx, ky, kx, sy, tx, ty) into a w x h pixmap; false on a bad group.
+using FeImageHook = std::function<bool(uint32_t root, const float ts[6], uint8_t *px, uint32_t w, uint32_t h)>;
+extern thread_local FeImageHook fe_image_hook;
 
 // Time split of a render by operation (filter primitive kinds, then fills, strokes,
 // layer composites and whole filters), for the CPU baseline's per-primitive split.

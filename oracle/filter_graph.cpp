@@ -129,7 +129,7 @@ rvk_light_source transform_light(rvk_light_source l, IRect region, Ts ts) {
     return l;
 }
 
-ImgP apply_primitive(Ctx &c, const rvk_primitive &p) {
+ImgP apply_primitive(Ctx &c, const rvk_primitive &p, IRect subregion) {
     prof::Scope timed((int)p.kind);
     const uint32_t cs = p.color_interpolation;
     const IRect region = c.region;
@@ -286,6 +286,18 @@ ImgP apply_primitive(Ctx &c, const rvk_primitive &p) {
                              p.u.lighting.color, &l);
             return out;
         }
+        case RVK_FE_IMAGE: {  // apply_image (mod.rs:861-889)
+            // render_nodes of the content into a region-sized pixmap with transform
+            // [sx 0 0 sy subregion.x subregion.y] and max_bbox the pixmap
+            if (!orc::fe_image_hook) throw FilterError{RVK_ERR_UNSUPPORTED};
+            auto out = std::const_pointer_cast<Img>(make_img(region.w, region.h, RVK_CI_SRGB));
+            float sx, sy;
+            ts_get_scale(c.ts, sx, sy);
+            const float t6[6] = {sx, 0.0f, 0.0f, sy, (float)subregion.x, (float)subregion.y};
+            if (!orc::fe_image_hook(p.u.image.root, t6, out->px.data(), out->w, out->h))
+                throw FilterError{RVK_ERR_INVALID_ARGUMENT};
+            return from_pixels(out, RVK_CI_SRGB);
+        }
         case RVK_FE_TILE: {  // apply_tile (mod.rs:837-859)
             auto in = get_input(c, p.in1);
             // subregion = input.region translated by -region.xy; copy_region is
@@ -352,7 +364,7 @@ ImgP apply_inner(const rvk_filter &f, const rvk_filter_data &data, Ts ts, const 
             for (auto it = c.results.rbegin(); it != c.results.rend(); ++it)
                 if (it->name == p.in1) { sub = it->img->region; break; }
         }
-        ImgP res = apply_primitive(c, p);
+        ImgP res = apply_primitive(c, p, sub);
         if (!ir_eq(region, sub)) {
             IRect sub2 = p.kind == RVK_FE_OFFSET ? IRect{0, 0, region.w, region.h}
                                                  : IRect{sub.x - region.x, sub.y - region.y, sub.w, sub.h};

@@ -65,6 +65,10 @@ int32_t orc_render_node(const rvk_scene *scene, uint32_t kind, uint32_t index, c
 // tiny_skia::Mask::from_pixmap (alpha, or luminance when `luminance`) and Pixmap::apply_mask
 void orc_mask_from_pixmap(const uint8_t *px, uint32_t w, uint32_t h, uint32_t luminance, uint8_t *mask);
 void orc_apply_mask(uint8_t *px, uint32_t w, uint32_t h, const uint8_t *mask);
+// tiny-skia's EdgeClipper against [0, w] x [0, h] on device-space segments, each 9
+// floats (point count 2..4, then x0 y0 .. x3 y3); writes the clipped pieces in the
+// same layout and returns their number, -1 on a bad count or too small an output
+int32_t orc_edge_clip(const float *in, uint32_t n, uint32_t w, uint32_t h, float *out, uint32_t cap);
 // the u8 coverage mask of a path (for coverage-only tests)
 int32_t orc_path_coverage(const rvk_scene *scene, uint32_t path_index, rvk_transform ts, uint32_t stroke,
                           uint8_t *cov, uint32_t w, uint32_t h);

@@ -23,6 +23,8 @@
 #include "../resvg_amd/csrc/stroker.h"
 #include <algorithm>
 #include <array>
+#include <cmath>
+#include <cstring>
 #include <vector>
 
 using namespace orc;
@@ -407,7 +409,7 @@ static Combine combine_vertical(const Edge &e, Edge &last) {
     return kNo;
 }
 
-static void build_edges(const std::vector<Seg> &segs, int shift, std::vector<Edge> &edges) {
+static void build_edges(const std::vector<Seg> &segs, int shift, std::vector<Edge> &edges, bool monotonic) {
     auto push_line = [&](Pt a, Pt b) {
         Edge e;
         if (!make_line(a, b, shift, e)) return;
@@ -420,7 +422,10 @@ static void build_edges(const std::vector<Seg> &segs, int shift, std::vector<Edg
     };
     for (const Seg &s : segs) {
         if (s.n == 2) push_line(s.p[0], s.p[1]);
-        else if (s.n == 3) {
+        else if (monotonic) {  // clipped pieces are already monotonic (BasicEdgeBuilder::build)
+            Edge e;
+            if (s.n == 3 ? make_quad(s.p, shift, e) : make_cubic(s.p, shift, e)) edges.push_back(e);
+        } else if (s.n == 3) {
             Pt mono[5];
             int n = chop_quad_at_y_extrema(s.p, mono);
             for (int i = 0; i <= n; i++) {
@@ -439,6 +444,607 @@ static void build_edges(const std::vector<Seg> &segs, int shift, std::vector<Edg
     for (size_t i = 0; i < edges.size(); i++) edges[i].id = (int)i;
 }
 
+// ---- EdgeClipper (tiny-skia 0.11 scan/edge_clipper.rs, scan/line_clipper.rs;
+// Skia SkEdgeClipper / SkLineClipper) ----
+// A fill whose rounded-out bounds leave the pixmap has every segment clipped to
+// [0, w] x [0, h]: parts above or below dropped, parts to the left or right turned
+// into vertical lines on the clip edge; curves chopped at Y then X extrema first.
+// Cubic chops at a clip line use Skia's path-ops intercept in double precision
+// (tiny-skia-path path64/, from SkPathOpsCubic.cpp, SkPathOpsQuad.cpp,
+// SkPathOpsTypes.h, SkLineCubicIntersections).
+
+static Pt transpose(Pt p) { return Pt{p.y, p.x}; }
+
+// chop_quad_at_x_extrema / chop_cubic_at_x_extrema: the Y versions on swapped axes
+static int chop_quad_at_x_extrema(const Pt *src, Pt *dst) {
+    Pt s[3], d[5];
+    for (int i = 0; i < 3; i++) s[i] = transpose(src[i]);
+    int n = chop_quad_at_y_extrema(s, d);
+    for (int i = 0; i < 2 * n + 3; i++) dst[i] = transpose(d[i]);
+    return n;
+}
+static int chop_cubic_at_x_extrema(const Pt *src, Pt *dst) {
+    Pt s[4], d[10];
+    for (int i = 0; i < 4; i++) s[i] = transpose(src[i]);
+    int n = chop_cubic_at_y_extrema(s, d);
+    for (int i = 0; i < 3 * n + 4; i++) dst[i] = transpose(d[i]);
+    return n;
+}
+
+namespace pathops {
+const double FLT_EPS = 1.1920928955078125e-07;  // FLT_EPSILON
+static bool approximately_zero(double x) { return std::fabs(x) < FLT_EPS; }
+static bool approximately_zero_cubed(double x) { return std::fabs(x) < FLT_EPS * FLT_EPS * FLT_EPS; }
+static bool approximately_zero_half(double x) { return x < FLT_EPS / 2; }  // Skia takes no fabs here
+static bool approximately_zero_inverse(double x) { return std::fabs(x) > 1 / FLT_EPS; }
+static bool approximately_zero_when_compared_to(double x, double y) {
+    return x == 0 || std::fabs(x) < std::fabs(y * FLT_EPS);
+}
+static bool approximately_equal(double x, double y) { return approximately_zero(x - y); }
+static bool approximately_equal_half(double x, double y) { return approximately_zero_half(x - y); }
+static bool approximately_one_or_less(double x) { return x < 1 + FLT_EPS; }
+static bool approximately_zero_or_more(double x) { return x > -FLT_EPS; }
+static bool between(double a, double b, double c) { return (a - b) * (c - b) <= 0; }
+static double pin(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// AlmostDequalUlps: within 16 ulps as floats (SkFloatAs2sCompliment ordering)
+static int32_t ordered_bits(float f) {
+    int32_t b;
+    std::memcpy(&b, &f, sizeof b);
+    return b < 0 ? -(b & 0x7FFFFFFF) : b;
+}
+static bool almost_dequal_ulps(double a, double b) {
+    if (std::fabs(a) < 3.40282346638528859812e+38 && std::fabs(b) < 3.40282346638528859812e+38) {
+        int32_t x = ordered_bits((float)a), y = ordered_bits((float)b);
+        return x < y + 16 && y < x + 16;
+    }
+    return std::fabs(a - b) / std::max(std::fabs(a), std::fabs(b)) < FLT_EPS * 16;
+}
+
+// SkDQuad::RootsReal (x^2 + p x + q normal form) and AddValidTs
+static int quad_roots_real(double A, double B, double C, double s[3]) {
+    if (A == 0 || (approximately_zero(A) && (approximately_zero_inverse(B / (2 * A)) ||
+                                             approximately_zero_inverse(C / A)))) {
+        if (approximately_zero(B)) {
+            s[0] = 0;
+            return C == 0;
+        }
+        s[0] = -C / B;
+        return 1;
+    }
+    const double p = B / (2 * A), q = C / A;
+    const double p2 = p * p;
+    if (!almost_dequal_ulps(p2, q) && p2 < q) return 0;
+    const double sqrt_d = p2 > q ? std::sqrt(p2 - q) : 0;
+    s[0] = sqrt_d - p;
+    s[1] = -sqrt_d - p;
+    return almost_dequal_ulps(s[0], s[1]) ? 1 : 2;
+}
+static int add_valid_ts(const double *s, int n, double *t) {
+    int found = 0;
+    for (int i = 0; i < n; i++) {
+        double v = s[i];
+        if (!approximately_zero_or_more(v) || !approximately_one_or_less(v)) continue;
+        v = pin(v, 0, 1);
+        int k = 0;
+        while (k < found && !approximately_equal(t[k], v)) k++;
+        if (k == found) t[found++] = v;
+    }
+    return found;
+}
+static int quad_roots_valid_t(double A, double B, double C, double t[2]) {
+    double s[3];
+    return add_valid_ts(s, quad_roots_real(A, B, C, s), t);
+}
+
+// SkDCubeRoot (halley_cbrt3d of cbrt_5d)
+static double cube_root(double x) {
+    if (approximately_zero_cubed(x)) return 0;
+    const double ax = std::fabs(x);
+    uint32_t w[2];
+    std::memcpy(w, &ax, sizeof ax);  // little-endian: w[1] holds the sign/exponent word
+    uint32_t g[2] = {0, w[1] / 3 + 715094163};
+    double a;
+    std::memcpy(&a, g, sizeof a);
+    for (int it = 0; it < 3; it++) {
+        double a3 = a * a * a;
+        a = a * (a3 + ax + ax) / (a3 + a3 + ax);
+    }
+    return x < 0 ? -a : a;
+}
+
+// SkDCubic::RootsReal and RootsValidT
+static int cubic_roots_real(double A, double B, double C, double D, double s[3]) {
+    if (approximately_zero(A) && approximately_zero_when_compared_to(A, B) &&
+        approximately_zero_when_compared_to(A, C) && approximately_zero_when_compared_to(A, D))
+        return quad_roots_real(B, C, D, s);  // a quadratic
+    if (approximately_zero_when_compared_to(D, A) && approximately_zero_when_compared_to(D, B) &&
+        approximately_zero_when_compared_to(D, C)) {  // 0 is a root
+        int n = quad_roots_real(A, B, C, s);
+        for (int i = 0; i < n; i++)
+            if (approximately_zero(s[i])) return n;
+        s[n] = 0;
+        return n + 1;
+    }
+    if (approximately_zero(A + B + C + D)) {  // 1 is a root
+        int n = quad_roots_real(A, A + B, -D, s);
+        for (int i = 0; i < n; i++)
+            if (almost_dequal_ulps(s[i], 1)) return n;
+        s[n] = 1;
+        return n + 1;
+    }
+    double invA = 1 / A;
+    double a = B * invA, b = C * invA, c = D * invA;
+    double a2 = a * a;
+    double Q = (a2 - b * 3) / 9;
+    double R = (2 * a2 * a - 9 * a * b + 27 * c) / 54;
+    double R2 = R * R, Q3 = Q * Q * Q;
+    double R2MinusQ3 = R2 - Q3;
+    double adiv3 = a / 3;
+    double *roots = s;
+    if (R2MinusQ3 < 0) {
+        const double PI = 3.141592653589793;
+        double theta = std::acos(pin(R / std::sqrt(Q3), -1., 1.));
+        double neg2RootQ = -2 * std::sqrt(Q);
+        double r = neg2RootQ * std::cos(theta / 3) - adiv3;
+        *roots++ = r;
+        r = neg2RootQ * std::cos((theta + 2 * PI) / 3) - adiv3;
+        if (!almost_dequal_ulps(s[0], r)) *roots++ = r;
+        r = neg2RootQ * std::cos((theta - 2 * PI) / 3) - adiv3;
+        if (!almost_dequal_ulps(s[0], r) && (roots - s == 1 || !almost_dequal_ulps(s[1], r))) *roots++ = r;
+    } else {
+        double sqrtR2MinusQ3 = std::sqrt(R2MinusQ3);
+        double Av = std::fabs(R) + sqrtR2MinusQ3;
+        Av = cube_root(Av);
+        if (R > 0) Av = -Av;
+        if (Av != 0) Av += Q / Av;
+        double r = Av - adiv3;
+        *roots++ = r;
+        if (almost_dequal_ulps(R2, Q3)) {
+            r = -Av / 2 - adiv3;
+            if (!almost_dequal_ulps(s[0], r)) *roots++ = r;
+        }
+    }
+    return (int)(roots - s);
+}
+static int cubic_roots_valid_t(double A, double B, double C, double D, double t[3]) {
+    double s[3];
+    int real = cubic_roots_real(A, B, C, D, s);
+    int found = add_valid_ts(s, real, t);
+    for (int i = 0; i < real; i++) {
+        double v = s[i];
+        double snap;
+        if (!approximately_one_or_less(v) && between(1, v, 1.00005)) snap = 1;
+        else if (!approximately_zero_or_more(v) && between(-0.00005, v, 0)) snap = 0;
+        else continue;
+        int k = 0;
+        while (k < found && !approximately_equal(t[k], snap)) k++;
+        if (k == found) t[found++] = snap;
+    }
+    return found;
+}
+
+struct DCubic {
+    double x[4], y[4];
+    const double *axis(int a) const { return a ? y : x; }
+    void at(double t, double &ox, double &oy) const {  // SkDCubic::ptAtT
+        if (t == 0) { ox = x[0]; oy = y[0]; return; }
+        if (t == 1) { ox = x[3]; oy = y[3]; return; }
+        double one_t = 1 - t, one_t2 = one_t * one_t;
+        double a = one_t2 * one_t, b = 3 * one_t2 * t, t2 = t * t, c = 3 * one_t * t2, d = t2 * t;
+        ox = a * x[0] + b * x[1] + c * x[2] + d * x[3];
+        oy = a * y[0] + b * y[1] + c * y[2] + d * y[3];
+    }
+    double at_axis(double t, int a) const {
+        double ox, oy;
+        at(t, ox, oy);
+        return a ? oy : ox;
+    }
+};
+
+// SkDCubic::binarySearch: t in [min, max] where the axis coordinate meets the
+// intercept, -1 when the search stalls
+static double binary_search(const DCubic &c, double mn, double mx, double intercept, int axis) {
+    double t = (mn + mx) / 2;
+    double step = (t - mn) / 2;
+    double cx, cy;
+    c.at(t, cx, cy);
+    double calc_pos = axis ? cy : cx;
+    double calc_dist = calc_pos - intercept;
+    do {
+        double prior_t = std::max(mn, t - step);
+        double lx, ly;
+        c.at(prior_t, lx, ly);
+        if (approximately_equal_half(lx, cx) && approximately_equal_half(ly, cy)) return -1;
+        double less_dist = (axis ? ly : lx) - intercept;
+        double last_step = step;
+        step /= 2;
+        if (calc_dist > 0 ? calc_dist > less_dist : calc_dist < less_dist) {
+            t = prior_t;
+        } else {
+            double next_t = t + last_step;
+            if (next_t > mx) return -1;
+            double mxp, myp;
+            c.at(next_t, mxp, myp);
+            if (approximately_equal_half(mxp, cx) && approximately_equal_half(myp, cy)) return -1;
+            double more_dist = (axis ? myp : mxp) - intercept;
+            if (calc_dist > 0 ? calc_dist <= more_dist : calc_dist >= more_dist) continue;
+            t = next_t;
+        }
+        c.at(t, cx, cy);
+        calc_pos = axis ? cy : cx;
+        calc_dist = calc_pos - intercept;
+    } while (!approximately_equal(calc_pos, intercept));
+    return t;
+}
+
+// LineCubicIntersections::{Horizontal,Vertical}Intersect
+static int axis_intersect(const DCubic &c, double intercept, int axis, double roots[3]) {
+    const double *s = c.axis(axis);
+    // SkDCubic::Coefficients
+    double A = s[3], B = s[2] * 3, C = s[1] * 3, D = s[0];
+    A -= D - C + B;
+    B += 3 * D - 2 * C;
+    C -= 3 * D;
+    D -= intercept;
+    int count = cubic_roots_valid_t(A, B, C, D, roots);
+    for (int i = 0; i < count; i++) {
+        if (approximately_equal(c.at_axis(roots[i], axis), intercept)) continue;
+        // SkDCubic::FindExtrema + searchRoots (with findInflections)
+        double ext[6];
+        int n = quad_roots_valid_t(s[3] - s[0] + 3 * (s[1] - s[2]), 2 * (s[0] - s[1] - s[1] + s[2]), s[1] - s[0], ext);
+        double Ax = c.x[1] - c.x[0], Ay = c.y[1] - c.y[0];
+        double Bx = c.x[2] - 2 * c.x[1] + c.x[0], By = c.y[2] - 2 * c.y[1] + c.y[0];
+        double Cx = c.x[3] + 3 * (c.x[1] - c.x[2]) - c.x[0], Cy = c.y[3] + 3 * (c.y[1] - c.y[2]) - c.y[0];
+        n += quad_roots_valid_t(Bx * Cy - By * Cx, Ax * Cy - Ay * Cx, Ax * By - Ay * Bx, ext + n);
+        ext[n++] = 0;
+        ext[n] = 1;
+        std::sort(ext, ext + n + 1);
+        int valid = 0;
+        for (int k = 0; k < n;) {
+            double lo = ext[k], hi = ext[++k];
+            if (lo == hi) continue;
+            double nt = binary_search(c, lo, hi, intercept, axis);
+            if (nt >= 0) {
+                if (valid >= 3) return 0;
+                roots[valid++] = nt;
+            }
+        }
+        return valid;
+    }
+    return count;
+}
+
+// SkDCubic::chopAt on one axis (interp_cubic_coords; exact halves at t = 0.5)
+static void chop_axis(const double *s, double t, double *d) {
+    if (t == 0.5) {
+        d[0] = s[0];
+        d[1] = (s[0] + s[1]) / 2;
+        d[2] = (s[0] + 2 * s[1] + s[2]) / 4;
+        d[3] = (s[0] + 3 * (s[1] + s[2]) + s[3]) / 8;
+        d[4] = (s[1] + 2 * s[2] + s[3]) / 4;
+        d[5] = (s[2] + s[3]) / 2;
+        d[6] = s[3];
+        return;
+    }
+    auto lerp = [](double a, double b, double u) { return a + (b - a) * u; };
+    double ab = lerp(s[0], s[1], t), bc = lerp(s[1], s[2], t), cd = lerp(s[2], s[3], t);
+    double abc = lerp(ab, bc, t), bcd = lerp(bc, cd, t);
+    d[0] = s[0]; d[1] = ab; d[2] = abc; d[3] = lerp(abc, bcd, t); d[4] = bcd; d[5] = cd; d[6] = s[3];
+}
+}  // namespace pathops
+
+// chop_mono_cubic_at_{x,y} (edge_clipper.rs): path_geometry's double intercept
+// (cubic_dchop_at_intercept), else mono_cubic_closest_t's bisection in f32
+static void chop_mono_cubic_at(const Pt *src, float v, int axis, Pt *dst) {
+    pathops::DCubic c;
+    for (int i = 0; i < 4; i++) { c.x[i] = src[i].x; c.y[i] = src[i].y; }
+    double roots[3];
+    if (pathops::axis_intersect(c, v, axis, roots) > 0) {
+        double dx[7], dy[7];
+        pathops::chop_axis(c.x, roots[0], dx);
+        pathops::chop_axis(c.y, roots[0], dy);
+        for (int i = 0; i < 7; i++) dst[i] = Pt{(float)dx[i], (float)dy[i]};
+        return;
+    }
+    float s0 = axis ? src[0].y : src[0].x, s1 = axis ? src[1].y : src[1].x;
+    float s2 = axis ? src[2].y : src[2].x, s3 = axis ? src[3].y : src[3].x;
+    float t = 0.5f, last_t, best_t = t, step = 0.25f;
+    float d = s0, a = s3 + 3.0f * (s1 - s2) - d, b = 3.0f * (s2 - s1 - s1 + d), c3 = 3.0f * (s1 - d);
+    float x = v - d;
+    float closest = 3.40282347e+38f;
+    do {
+        float loc = ((a * t + b) * t + c3) * t;
+        float dist = std::fabs(loc - x);
+        if (closest > dist) { closest = dist; best_t = t; }
+        last_t = t;
+        t += loc < x ? step : -step;
+        step *= 0.5f;
+    } while (closest > 0.25f && last_t != t);
+    chop_cubic_at2(src, best_t, dst);
+}
+
+// chop_mono_quad_at: first root of the quadratic through c0, c1, c2 at `target`
+static bool chop_mono_quad_at(float c0, float c1, float c2, float target, float &t) {
+    float roots[3];
+    if (find_unit_quad_roots(c0 - c1 - c1 + c2, 2.0f * (c1 - c0), c0 - target, roots) == 0) return false;
+    t = roots[0];
+    return true;
+}
+static void chop_quad_at(const Pt *s, float t, Pt *d) {
+    Pt p01 = interp(s[0], s[1], t), p12 = interp(s[1], s[2], t);
+    d[0] = s[0]; d[1] = p01; d[2] = interp(p01, p12, t); d[3] = p12; d[4] = s[2];
+}
+
+class EdgeClipper {
+  public:
+    EdgeClipper(float w, float h, std::vector<Seg> &out) : L(0.0f), T(0.0f), R(w), B(h), out_(out) {}
+
+    void segment(const Seg &s) {
+        if (s.n == 2) clip_line(s.p[0], s.p[1]);
+        else if (s.n == 3) clip_quad(s.p);
+        else clip_cubic(s.p);
+    }
+
+  private:
+    float L, T, R, B;
+    std::vector<Seg> &out_;
+
+    void emit_line(Pt a, Pt b) { out_.push_back(Seg{2, {a, b}}); }
+    void emit_vline(float x, float y0, float y1, bool reverse) {
+        if (reverse) emit_line(Pt{x, y1}, Pt{x, y0});
+        else emit_line(Pt{x, y0}, Pt{x, y1});
+    }
+    void emit_quad(const Pt *p, bool reverse) {
+        if (reverse) out_.push_back(Seg{3, {p[2], p[1], p[0]}});
+        else out_.push_back(Seg{3, {p[0], p[1], p[2]}});
+    }
+    void emit_cubic(const Pt *p, bool reverse) {
+        if (reverse) out_.push_back(Seg{4, {p[3], p[2], p[1], p[0]}});
+        else out_.push_back(Seg{4, {p[0], p[1], p[2], p[3]}});
+    }
+
+    // line_clipper.rs sect_with_horizontal / sect_with_vertical (pinned variants)
+    static float sect_with_horizontal(const Pt *s, float y) {
+        float dy = s[1].y - s[0].y;
+        if (std::fabs(dy) <= 1.0f / 4096.0f) return (s[0].x + s[1].x) * 0.5f;
+        double x0 = s[0].x, y0 = s[0].y, x1 = s[1].x, y1 = s[1].y;
+        double r = x0 + ((double)y - y0) * (x1 - x0) / (y1 - y0);
+        return (float)(x1 < x0 ? pathops::pin(r, x1, x0) : pathops::pin(r, x0, x1));
+    }
+    static float sect_clamp_with_vertical(const Pt *s, float x) {
+        float dx = s[1].x - s[0].x;
+        float y;
+        if (std::fabs(dx) <= 1.0f / 4096.0f) {
+            y = (s[0].y + s[1].y) * 0.5f;
+        } else {
+            double x0 = s[0].x, y0 = s[0].y, x1 = s[1].x, y1 = s[1].y;
+            y = (float)(y0 + ((double)x - x0) * (y1 - y0) / (x1 - x0));
+        }
+        float lo = s[0].y, hi = s[1].y;
+        if (hi < lo) std::swap(lo, hi);
+        return y < lo ? lo : (y > hi ? hi : y);
+    }
+
+    // line_clipper::clip: up to three segments (left vline, middle, right vline)
+    void clip_line(Pt a, Pt b) {
+        const Pt src[2] = {a, b};
+        int lo = src[0].y < src[1].y ? 0 : 1;
+        int hi = 1 - lo;
+        if (src[hi].y <= T || src[lo].y >= B) return;
+        Pt tmp[2] = {a, b};
+        if (src[lo].y < T) tmp[lo] = Pt{sect_with_horizontal(src, T), T};
+        if (tmp[hi].y > B) tmp[hi] = Pt{sect_with_horizontal(src, B), B};
+        bool reverse = !(src[0].x < src[1].x);
+        int left = reverse ? 1 : 0, right = 1 - left;
+        Pt res[4];
+        int count;
+        if (tmp[right].x <= L) {
+            res[0] = Pt{L, tmp[0].y}; res[1] = Pt{L, tmp[1].y};
+            count = 1;
+            reverse = false;
+        } else if (tmp[left].x >= R) {
+            res[0] = Pt{R, tmp[0].y}; res[1] = Pt{R, tmp[1].y};
+            count = 1;
+            reverse = false;
+        } else {
+            int k = 0;
+            if (tmp[left].x < L) {
+                res[k++] = Pt{L, tmp[left].y};
+                res[k++] = Pt{L, sect_clamp_with_vertical(tmp, L)};
+            } else {
+                res[k++] = tmp[left];
+            }
+            if (tmp[right].x > R) {
+                res[k++] = Pt{R, sect_clamp_with_vertical(tmp, R)};
+                res[k++] = Pt{R, tmp[right].y};
+            } else {
+                res[k++] = tmp[right];
+            }
+            count = k - 1;
+        }
+        for (int i = 0; i < count; i++) {
+            if (reverse) emit_line(res[count - i], res[count - i - 1]);
+            else emit_line(res[i], res[i + 1]);
+        }
+    }
+
+    static bool sort_increasing_y(const Pt *src, Pt *dst, int n) {
+        bool rev = src[0].y > src[n - 1].y;
+        for (int i = 0; i < n; i++) dst[i] = rev ? src[n - 1 - i] : src[i];
+        return rev;
+    }
+
+    void clip_quad(const Pt *p) {
+        float top = std::min(p[0].y, std::min(p[1].y, p[2].y));
+        float bot = std::max(p[0].y, std::max(p[1].y, p[2].y));
+        if (top >= B || bot <= T) return;
+        Pt my[5];
+        int ny = chop_quad_at_y_extrema(p, my);
+        for (int i = 0; i <= ny; i++) {
+            Pt mx[5];
+            int nx = chop_quad_at_x_extrema(my + 2 * i, mx);
+            for (int k = 0; k <= nx; k++) clip_mono_quad(mx + 2 * k);
+        }
+    }
+
+    void clip_mono_quad(const Pt *src) {
+        Pt pts[3];
+        bool reverse = sort_increasing_y(src, pts, 3);
+        if (pts[2].y <= T || pts[0].y >= B) return;
+        float t;
+        Pt tmp[5];
+        // chop_quad_in_y
+        if (pts[0].y < T) {
+            if (chop_mono_quad_at(pts[0].y, pts[1].y, pts[2].y, T, t)) {
+                chop_quad_at(pts, t, tmp);
+                tmp[2].y = T;
+                tmp[3].y = std::max(tmp[3].y, T);
+                pts[0] = tmp[2];
+                pts[1] = tmp[3];
+            } else {
+                for (Pt &q : pts) q.y = std::max(q.y, T);
+            }
+        }
+        if (pts[2].y > B) {
+            if (chop_mono_quad_at(pts[0].y, pts[1].y, pts[2].y, B, t)) {
+                chop_quad_at(pts, t, tmp);
+                tmp[1].y = std::min(tmp[1].y, B);
+                tmp[2].y = B;
+                pts[1] = tmp[1];
+                pts[2] = tmp[2];
+            } else {
+                for (Pt &q : pts) q.y = std::min(q.y, B);
+            }
+        }
+        if (pts[0].x > pts[2].x) {
+            std::swap(pts[0], pts[2]);
+            reverse = !reverse;
+        }
+        if (pts[2].x <= L) return emit_vline(L, pts[0].y, pts[2].y, reverse);
+        if (pts[0].x >= R) return emit_vline(R, pts[0].y, pts[2].y, reverse);
+        if (pts[0].x < L) {
+            if (!chop_mono_quad_at(pts[0].x, pts[1].x, pts[2].x, L, t))
+                return emit_vline(L, pts[0].y, pts[2].y, reverse);
+            chop_quad_at(pts, t, tmp);
+            emit_vline(L, tmp[0].y, tmp[2].y, reverse);
+            tmp[2].x = L;
+            tmp[3].x = std::max(tmp[3].x, L);
+            pts[0] = tmp[2];
+            pts[1] = tmp[3];
+        }
+        if (pts[2].x > R) {
+            if (chop_mono_quad_at(pts[0].x, pts[1].x, pts[2].x, R, t)) {
+                chop_quad_at(pts, t, tmp);
+                tmp[1].x = std::min(tmp[1].x, R);
+                tmp[2].x = R;
+                emit_quad(tmp, reverse);
+                emit_vline(R, tmp[2].y, tmp[4].y, reverse);
+            } else {
+                pts[1].x = std::min(pts[1].x, R);
+                pts[2].x = std::min(pts[2].x, R);
+                emit_quad(pts, reverse);
+            }
+        } else {
+            emit_quad(pts, reverse);
+        }
+    }
+
+    void clip_cubic(const Pt *p) {
+        float l = p[0].x, r = p[0].x, top = p[0].y, bot = p[0].y;
+        for (int i = 1; i < 4; i++) {
+            l = std::min(l, p[i].x); r = std::max(r, p[i].x);
+            top = std::min(top, p[i].y); bot = std::max(bot, p[i].y);
+        }
+        if (!(bot > T && top < B)) return;
+        const float lim = 4194304.0f;  // too_big_for_reliable_float_math: 1 << 22
+        if (l < -lim || top < -lim || r > lim || bot > lim) return clip_line(p[0], p[3]);
+        Pt my[10];
+        int ny = chop_cubic_at_y_extrema(p, my);
+        for (int i = 0; i <= ny; i++) {
+            Pt mx[10];
+            int nx = chop_cubic_at_x_extrema(my + 3 * i, mx);
+            for (int k = 0; k <= nx; k++) clip_mono_cubic(mx + 3 * k);
+        }
+    }
+
+    void clip_mono_cubic(const Pt *src) {
+        Pt pts[4];
+        bool reverse = sort_increasing_y(src, pts, 4);
+        if (pts[3].y <= T || pts[0].y >= B) return;
+        // chop_cubic_in_y
+        if (pts[0].y < T) {
+            Pt tmp[7];
+            chop_mono_cubic_at(pts, T, 1, tmp);
+            if (tmp[3].y < T && tmp[4].y < T && tmp[5].y < T) {  // a poor first chop: chop again
+                Pt again[4] = {tmp[3], tmp[4], tmp[5], tmp[6]};
+                chop_mono_cubic_at(again, T, 1, tmp);
+            }
+            tmp[3].y = T;
+            tmp[4].y = std::max(tmp[4].y, T);
+            pts[0] = tmp[3];
+            pts[1] = tmp[4];
+            pts[2] = tmp[5];
+        }
+        if (pts[3].y > B) {
+            Pt tmp[7];
+            chop_mono_cubic_at(pts, B, 1, tmp);
+            tmp[3].y = B;
+            tmp[2].y = std::min(tmp[2].y, B);
+            pts[1] = tmp[1];
+            pts[2] = tmp[2];
+            pts[3] = tmp[3];
+        }
+        if (pts[0].x > pts[3].x) {
+            std::swap(pts[0], pts[3]);
+            std::swap(pts[1], pts[2]);
+            reverse = !reverse;
+        }
+        if (pts[3].x <= L) return emit_vline(L, pts[0].y, pts[3].y, reverse);
+        if (pts[0].x >= R) return emit_vline(R, pts[0].y, pts[3].y, reverse);
+        if (pts[0].x < L) {
+            Pt tmp[7];
+            chop_mono_cubic_at(pts, L, 0, tmp);
+            emit_vline(L, tmp[0].y, tmp[3].y, reverse);
+            tmp[3].x = L;
+            tmp[4].x = std::max(tmp[4].x, L);
+            pts[0] = tmp[3];
+            pts[1] = tmp[4];
+            pts[2] = tmp[5];
+        }
+        if (pts[3].x > R) {
+            Pt tmp[7];
+            chop_mono_cubic_at(pts, R, 0, tmp);
+            tmp[3].x = R;
+            tmp[2].x = std::min(tmp[2].x, R);
+            emit_cubic(tmp, reverse);
+            emit_vline(R, tmp[3].y, tmp[6].y, reverse);
+        } else {
+            emit_cubic(pts, reverse);
+        }
+    }
+};
+
+// path_contained_in_clip of scan/path_aa.rs (floor/ceil bounds) or scan/path.rs
+// (conservative_round_to_int, bias 0.5 + 1.5/64): the bounds as a screen rect
+// (non-negative origin) inside the pixmap.
+static bool contained_in_pixmap(const std::vector<Seg> &segs, bool aa, uint32_t w, uint32_t h) {
+    float l = INFINITY, t = INFINITY, r = -INFINITY, b = -INFINITY;
+    for (const Seg &s : segs)
+        for (int i = 0; i < s.n; i++) {
+            l = std::min(l, s.p[i].x); t = std::min(t, s.p[i].y);
+            r = std::max(r, s.p[i].x); b = std::max(b, s.p[i].y);
+        }
+    double bl, bt, br, bb;
+    if (aa) {
+        bl = std::floor(l); bt = std::floor(t); br = std::ceil(r); bb = std::ceil(b);
+    } else {
+        const double bias = 0.5 + 1.5 / 64;
+        bl = std::ceil(l - bias); bt = std::ceil(t - bias); br = std::floor(r + bias); bb = std::floor(b + bias);
+    }
+    return bl >= 0 && bt >= 0 && br <= w && bb <= h;
+}
+
 // Per-pixel coverage of `segs` (already in device space) inside a w x h pixmap:
 // the SuperBlitter accumulation of tiny-skia's scan/path_aa.rs for anti-aliased
 // fills, or full-pixel spans for non-AA fills.
@@ -447,7 +1053,14 @@ static void coverage(const std::vector<Seg> &segs, uint32_t rule, bool aa, uint3
     cov.assign((size_t)w * h, 0);
     const int shift = aa ? 2 : 0;
     std::vector<Edge> edges;
-    build_edges(segs, shift, edges);
+    if (contained_in_pixmap(segs, aa, w, h)) {
+        build_edges(segs, shift, edges, false);
+    } else {
+        std::vector<Seg> clipped;
+        EdgeClipper clipper((float)w, (float)h, clipped);
+        for (const Seg &s : segs) clipper.segment(s);
+        build_edges(clipped, shift, edges, true);
+    }
     if (edges.empty()) return;
     // Curve edges expose only their current piece, so the walk runs until every
     // edge has been used up (or the pixmap ends), not to the first pieces' last row.
@@ -1272,6 +1885,27 @@ int32_t orc_path_coverage(const rvk_scene *scene, uint32_t path_index, rvk_trans
                           uint8_t *cov, uint32_t w, uint32_t h) {
     orc::path_coverage_of(*scene, scene->paths[path_index], orc::ts_from(ts), stroke != 0, cov, w, h);
     return 0;
+}
+
+int32_t orc_edge_clip(const float *in, uint32_t n, uint32_t w, uint32_t h, float *out, uint32_t cap) {
+    std::vector<Seg> segs, clipped;
+    for (uint32_t i = 0; i < n; i++) {
+        const float *r = in + 9 * i;
+        Seg s{(int)r[0], {}};
+        if (s.n < 2 || s.n > 4) return -1;
+        for (int k = 0; k < s.n; k++) s.p[k] = Pt{r[1 + 2 * k], r[2 + 2 * k]};
+        segs.push_back(s);
+    }
+    EdgeClipper clipper((float)w, (float)h, clipped);
+    for (const Seg &s : segs) clipper.segment(s);
+    if (clipped.size() > cap) return -1;
+    for (size_t i = 0; i < clipped.size(); i++) {
+        float *r = out + 9 * i;
+        std::fill(r, r + 9, 0.0f);
+        r[0] = (float)clipped[i].n;
+        for (int k = 0; k < clipped[i].n; k++) { r[1 + 2 * k] = clipped[i].p[k].x; r[2 + 2 * k] = clipped[i].p[k].y; }
+    }
+    return (int32_t)clipped.size();
 }
 
 int32_t orc_fill_path(const rvk_scene *scene, uint32_t path_index, rvk_transform ts, uint8_t *px, uint32_t w,

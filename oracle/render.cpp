@@ -181,9 +181,21 @@ void render_group(const rvk_scene &sc, const rvk_group &g, const Ctx &ctx, Ts ts
     Ts lts = ts_concat(Ts{1, 0, 0, 1, -dx, -dy}, ts);
     std::vector<uint8_t> layer((size_t)ib.w * ib.h * 4, 0);
     render_nodes(sc, g, ctx, lts, layer.data(), ib.w, ib.h);
-    for (uint32_t i = 0; i < g.filter_count; i++)
-        orc_filter_apply(&sc.filters[g.filter_offset + i], &sc.filter_data,
-                         rvk_transform{lts.sx, lts.ky, lts.kx, lts.sy, lts.tx, lts.ty}, layer.data(), ib.w, ib.h);
+    if (g.filter_count > 0) {
+        // feImage content: render_nodes of a scene group with max_bbox the primitive's
+        // pixmap (filter/mod.rs:878-882)
+        const FeImageHook saved = fe_image_hook;
+        fe_image_hook = [&sc](uint32_t root, const float *t, uint8_t *img, uint32_t iw, uint32_t ih) {
+            if (root >= sc.group_count) return false;
+            render_nodes(sc, sc.groups[root], Ctx{IRect{0, 0, (int32_t)iw, (int32_t)ih}},
+                         Ts{t[0], t[1], t[2], t[3], t[4], t[5]}, img, iw, ih);
+            return true;
+        };
+        for (uint32_t i = 0; i < g.filter_count; i++)
+            orc_filter_apply(&sc.filters[g.filter_offset + i], &sc.filter_data,
+                             rvk_transform{lts.sx, lts.ky, lts.kx, lts.sy, lts.tx, lts.ty}, layer.data(), ib.w, ib.h);
+        fe_image_hook = saved;
+    }
     if (g.clip_path >= 0) clip_apply(sc, g.clip_path, lts, layer.data(), ib.w, ib.h, 0);
     if (g.mask >= 0) mask_apply(sc, ctx, g.mask, lts, layer.data(), ib.w, ib.h, 0);
     prof::Scope timed(prof::kLayer);
@@ -270,6 +282,7 @@ void orc_apply_mask(uint8_t *px, uint32_t w, uint32_t h, const uint8_t *mask) {
 // per-operation time split (CPU baseline only)
 // ---------------------------------------------------------------------------
 namespace orc {
+thread_local FeImageHook fe_image_hook;
 namespace prof {
 bool enabled = false;
 double ms[kSlots] = {};

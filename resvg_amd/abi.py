@@ -139,12 +139,16 @@ class _Turbulence(C.Structure):
                 ("seed", C.c_int32), ("stitch_tiles", C.c_uint32), ("fractal_noise", C.c_uint32)]
 
 
+class _Image(C.Structure):
+    _fields_ = [("root", C.c_uint32)]
+
+
 class _PrimU(C.Union):
     _fields_ = [("blend", _Blend), ("color_matrix", _ColorMatrix), ("component_transfer", _ComponentTransfer),
                 ("composite", _Composite), ("convolve", _Convolve), ("lighting", _Lighting),
                 ("displacement", _Displacement), ("drop_shadow", _DropShadow), ("flood", _Flood), ("blur", _Blur),
                 ("merge", _Merge), ("morphology", _Morphology), ("offset", _Offset), ("turbulence", _Turbulence),
-                ("pad_", C.c_float * 48)]
+                ("image", _Image), ("pad_", C.c_float * 48)]
 
 
 class Primitive(C.Structure):

@@ -91,6 +91,7 @@ struct Exec {
     IRect region;
     Img source;  // the layer, borrowed
     std::vector<std::pair<int32_t, Img>> results;
+    const rvk::FeImageHook *fe_image = nullptr;  // draws feImage content (render walks only)
 
     // get_input (mod.rs:523-564)
     Img input(int32_t in) {
@@ -111,7 +112,7 @@ struct Exec {
         else FTRY(rvk::launch_iir_blur(ctx, im->px, im->w, im->h, sx, sy));
     }
 
-    Img primitive(const rvk_primitive &p) {
+    Img primitive(const rvk_primitive &p, IRect sub) {
         const uint32_t cs = p.color_interpolation;
         switch (p.kind) {
             case RVK_FE_BLEND: {  // mod.rs:695-729
@@ -279,8 +280,16 @@ struct Exec {
                                       phase(sy, y1 - y0)));
                 return out;
             }
-            case RVK_FE_IMAGE:
-                throw FilterError{RVK_ERR_UNSUPPORTED};
+            case RVK_FE_IMAGE: {  // apply_image (mod.rs:861-889): the content drawn into a
+                // region-sized pixmap with the layer scale, moved by the subregion's origin
+                if (!fe_image || !*fe_image) throw FilterError{RVK_ERR_UNSUPPORTED};
+                Img out = new_img(ctx, region.w, region.h, RVK_CI_SRGB);
+                float sx, sy;
+                ts_get_scale(ts, sx, sy);
+                FTRY((*fe_image)(p.u.image.root, Ts{sx, 0.0f, 0.0f, sy, (float)sub.x, (float)sub.y}, out->px, out->w,
+                                 out->h));
+                return from_image(out, RVK_CI_SRGB);
+            }
             default:
                 throw FilterError{RVK_ERR_INVALID_ARGUMENT};
         }
@@ -310,7 +319,7 @@ struct Exec {
                 for (auto it = results.rbegin(); it != results.rend(); ++it)
                     if (it->first == p.in1) { sub = it->second->region; break; }
             }
-            Img res = primitive(p);
+            Img res = primitive(p, sub);
             if (!ir_eq(region, sub)) {
                 IRect sub2 = p.kind == RVK_FE_OFFSET ? IRect{0, 0, region.w, region.h}
                                                      : IRect{sub.x - region.x, sub.y - region.y, sub.w, sub.h};
@@ -336,7 +345,7 @@ namespace rvk {
 
 // filter::apply + apply_to_canvas (mod.rs:335-355, 1100-1114) on a device layer.
 rvk_status filter_apply(rvk_context *ctx, const rvk_filter *filter, const rvk_filter_data *data, Ts ts, uint8_t *px,
-                        uint32_t w, uint32_t h, rvk_status *out_error) {
+                        uint32_t w, uint32_t h, rvk_status *out_error, const FeImageHook *fe_image) {
     rvk_status ferr = RVK_OK;
     try {
         auto src = std::make_shared<DImg>();
@@ -344,7 +353,7 @@ rvk_status filter_apply(rvk_context *ctx, const rvk_filter *filter, const rvk_fi
         src->px = px;
         src->w = w; src->h = h;
         src->owned = false;
-        Exec ex{ctx, data, ts, IRect{}, src, {}};
+        Exec ex{ctx, data, ts, IRect{}, src, {}, fe_image};
         Img res = ex.run(*filter);
         ex.source.reset();
         res = into_cs(std::move(res), RVK_CI_SRGB);

@@ -2,6 +2,7 @@
 // Transform / Rect / IntRect arithmetic in f32 as resvg uses it
 // (crates/resvg/src/render.rs:49-143, filter/mod.rs:357-521, 1063-1147).
 #pragma once
+#include <functional>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -189,6 +190,10 @@ rvk_status launch_shadow_flood(rvk_context *ctx, uint8_t *px, size_t n, const ui
 }  // namespace rvkh
 
 namespace rvk {
+// Draws the children of scene group `root` with transform `ts` into the w x h device
+// pixmap `px` (feImage content, filter/mod.rs:861-889); supplied by the render walk.
+using FeImageHook = std::function<rvk_status(uint32_t root, rvkh::Ts ts, uint8_t *px, uint32_t w, uint32_t h)>;
 rvk_status filter_apply(rvk_context *ctx, const rvk_filter *filter, const rvk_filter_data *data, rvkh::Ts ts,
-                        uint8_t *px, uint32_t w, uint32_t h, rvk_status *out_error);
+                        uint8_t *px, uint32_t w, uint32_t h, rvk_status *out_error,
+                        const FeImageHook *fe_image = nullptr);
 }

@@ -178,10 +178,20 @@ void fill_segments(const rvk_scene &sc, const rvk_path &path, rvkh::Ts ts, std::
 // Stroke outline (tiny-skia's stroker, stroker.cpp) in device space, contours closed.
 void stroke_segments(const rvk_scene &sc, const rvk_path &path, rvkh::Ts ts, std::vector<Seg> &out);
 // Skia edges of `segs` on the supersampled grid (lines combined like tiny-skia's
-// edge builder, curves chopped at Y extrema), each set up to its first piece, in
+// edge builder, curves chopped at Y extrema unless `monotonic` says they already
+// are, as clip_segments leaves them), each set up to its first piece, in
 // tiny-skia's edge-list order (ids 0..n-1).  Returns the maximum number of pieces
 // each edge can produce in `caps`.
-void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &out, std::vector<uint32_t> &caps);
+void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &out, std::vector<uint32_t> &caps,
+                 bool monotonic = false);
+// Whether the rounded-out control-point bounds of `segs` lie inside a w x h pixmap
+// (tiny-skia's path_contained_in_clip: floor/ceil for anti-aliased fills, the
+// conservative rounding of scan/path.rs otherwise); when not, tiny-skia builds the
+// edges from the clipped segments.
+bool segs_inside_pixmap(const std::vector<Seg> &segs, bool aa, uint32_t w, uint32_t h);
+// tiny-skia's EdgeClipper against [0, w] x [0, h] (edge_clip.cpp): lines, and
+// quadratics and cubics monotonic in x and y.
+void clip_segments(const std::vector<Seg> &segs, uint32_t w, uint32_t h, std::vector<Seg> &out);
 // tiny-skia's hairline stroke of a path drawn with `ts` into a w x h pixmap
 // (Pixmap::stroke_path's hairline branch): the prepared lines in drawing order.
 void hairline_lines(const rvk_scene &sc, const rvk_path &path, rvkh::Ts ts, uint32_t w, uint32_t h,

@@ -395,7 +395,8 @@ int cubic_y_monotonic(const Pt *src, Pt *dst) {
 
 }  // namespace
 
-void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &out, std::vector<uint32_t> &caps) {
+void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &out, std::vector<uint32_t> &caps,
+                 bool monotonic) {
     Builder b{shift, out, caps};
     // a line gives at most one edge, a quadratic two, a cubic three
     size_t bound = 0;
@@ -405,6 +406,9 @@ void build_edges(const std::vector<Seg> &segs, int shift, std::vector<EdgeRec> &
     for (const Seg &s : segs) {
         if (s.n == 2) {
             b.line(s.p[0], s.p[1]);
+        } else if (monotonic) {  // clipped pieces are pushed as they are (edge_builder.rs build)
+            if (s.n == 3) b.quad(s.p);
+            else b.cubic(s.p);
         } else if (s.n == 3) {
             Pt mono[5];
             int n = quad_y_monotonic(s.p, mono);

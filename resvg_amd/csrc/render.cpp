@@ -45,6 +45,16 @@ rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint
                              double algo_bytes);
 }
 
+namespace rvk {
+// What a render walks: the root group's children (resvg::render), one node
+// (resvg::render_node), the fills of the root group's children drawn with
+// BlendMode::Clear (a clip path's children, clip.rs:30-70), or the children of one
+// group (feImage content, filter/mod.rs:878).
+enum class Walk { kRoot, kNode, kClearFills, kChildren };
+rvk_status render_walk(rvk_context *ctx, const rvk_scene *scene, rvkh::Ts ts, uint32_t w, uint32_t h, uint8_t *px,
+                       uint64_t *overflow_rows, Walk walk, rvk_node_ref node, const rvkh::IRect *max_bbox = nullptr);
+}  // namespace rvk
+
 namespace {
 
 // Coverage bytes per flush (4 GiB: fewer, fuller coverage and compositing launches;
@@ -118,7 +128,13 @@ void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, uint32_t pw, ui
         for (const Seg &s : segs)
             for (int i = 0; i < s.n; i++) { mnx = std::min(mnx, s.p[i].x); mxx = std::max(mxx, s.p[i].x); }
         if (!std::isfinite(mnx) || !std::isfinite(mxx)) return;
-        build_edges(segs, p.anti_alias ? 2 : 0, es.edges, es.caps);
+        if (segs_inside_pixmap(segs, p.anti_alias != 0, pw, ph)) {
+            build_edges(segs, p.anti_alias ? 2 : 0, es.edges, es.caps);
+        } else {
+            std::vector<Seg> clipped;
+            clip_segments(segs, pw, ph, clipped);
+            build_edges(clipped, p.anti_alias ? 2 : 0, es.edges, es.caps, true);
+        }
         es.mnx = mnx;
         es.mxx = mxx;
         es.ok = !es.edges.empty();
@@ -590,7 +606,8 @@ struct Renderer {
     }
 
     // ---- clip paths (crates/resvg/src/clip.rs) ----
-    static constexpr int kMaxNesting = 64;  // deeper clip / mask chains are malformed input
+    static constexpr int kMaxNesting = 64;  // deeper clip / mask / feImage chains are malformed input
+    static thread_local inline int fe_image_depth = 0;  // feImage renders in progress on this thread
 
     // clip::draw_children (clip.rs:30-70) into `dl`: fills of the visible paths (Clear
     // when `clear`), groups with only their transforms; a group with its own clip path
@@ -855,9 +872,20 @@ struct Renderer {
         } else {
             skip_nodes(g, lts, child);
         }
+        // feImage content: the children of a scene group drawn by a render of their own
+        // into the primitive's pixmap, with max_bbox the pixmap (filter/mod.rs:878-882)
+        const rvk::FeImageHook fe_image = [this](uint32_t root, Ts t, uint8_t *img, uint32_t iw, uint32_t ih) {
+            if (root >= sc.group_count || fe_image_depth >= kMaxNesting) return (rvk_status)RVK_ERR_INVALID_ARGUMENT;
+            const IRect mb{0, 0, (int32_t)iw, (int32_t)ih};
+            fe_image_depth++;
+            const rvk_status st = rvk::render_walk(ctx, &sc, t, iw, ih, img, nullptr, rvk::Walk::kChildren,
+                                                   rvk_node_ref{RVK_NODE_GROUP, root}, &mb);
+            fe_image_depth--;
+            return st;
+        };
         for (uint32_t i = 0; s == RVK_OK && err == RVK_OK && i < g.filter_count; i++)
             s = rvk::filter_apply(ctx, &sc.filters[g.filter_offset + i], &sc.filter_data, lts, layer, ib.w, ib.h,
-                                  nullptr);
+                                  nullptr, &fe_image);
         if (s == RVK_OK && err == RVK_OK && g.clip_path >= 0) s = apply_clip(g.clip_path, lts, layer, ib.w, ib.h, 0);
         if (s == RVK_OK && err == RVK_OK && g.mask >= 0) s = apply_mask(g.mask, lts, layer, ib.w, ib.h, 0);
         if (s == RVK_OK) s = flush(dl);
@@ -873,14 +901,9 @@ struct Renderer {
 
 namespace rvk {
 
-// What a render walks: the root group's children (resvg::render), one node
-// (resvg::render_node), or the fills of the root group's children drawn with
-// BlendMode::Clear (a clip path's children, clip.rs:30-70).
-enum class Walk { kRoot, kNode, kClearFills };
-
 rvk_status render_walk(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t w, uint32_t h, uint8_t *px,
-                       uint64_t *overflow_rows, Walk walk, rvk_node_ref node) {
-    Renderer r{ctx, *scene, max_filter_bbox(w, h)};
+                       uint64_t *overflow_rows, Walk walk, rvk_node_ref node, const IRect *max_bbox) {
+    Renderer r{ctx, *scene, max_bbox ? *max_bbox : max_filter_bbox(w, h)};
     r.build_aux();
     void *ovbuf = nullptr;
     RVK_TRY(rvk::scratch_alloc(ctx, 256, &ovbuf));
@@ -900,6 +923,7 @@ rvk_status render_walk(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t
                 else r.render_path(scene->paths[node.index], ts, dl);
                 break;
             case Walk::kClearFills: r.clip_children(scene->groups[0], true, ts, dl, 0); break;
+            case Walk::kChildren: r.render_nodes(scene->groups[node.index], ts, dl); break;
         }
     };
     DrawList root;

@@ -141,6 +141,14 @@ class FilterGraph:
         t.stitch_tiles, t.fractal_noise = int(stitch_tiles), int(fractal_noise)
         return r
 
+    def image(self, root: int, **kw) -> int:
+        """feImage referencing an element (usvg filter::Image): the children of scene
+        group ``root`` (a content group, see Scene.content_group) drawn into the
+        primitive's pixmap with the layer's scale, moved to the subregion's origin."""
+        r, p = self.add(abi.FE_IMAGE, **kw)
+        p.u.image.root = root
+        return r
+
     def lighting(self, specular: bool, light: abi.LightSource, surface_scale: float = 1.0, constant: float = 1.0,
                  specular_exponent: float = 1.0, color=(255, 255, 255), **kw) -> int:
         r, p = self.add(abi.FE_SPECULAR_LIGHTING if specular else abi.FE_DIFFUSE_LIGHTING, **kw)

@@ -130,6 +130,11 @@ class Scene:
         self._abi = None
         return len(self.masks) - 1, root
 
+    def content_group(self) -> int:
+        """A group outside the tree (usvg filter::Image::root, the element an feImage
+        links to): only an feImage primitive draws its children."""
+        return self._new_group(None, (1.0, 0.0, 0.0, 1.0, 0.0, 0.0), 1.0, abi.MIX["normal"], False, [])
+
     def group(self, parent: int, transform=(1.0, 0.0, 0.0, 1.0, 0.0, 0.0), opacity: float = 1.0,
               blend: str | int = "normal", isolate: bool = False, filters: list[FilterGraph] = (),
               clip_path: int = -1, mask: int = -1) -> int:

@@ -28,6 +28,11 @@ NAMES = [
     "feTurbulence/type=fractalNoise", "feTurbulence/no-attributes", "feTurbulence/seed=20",
     "feTurbulence/stitchTiles=stitch", "feTurbulence/numOctaves=5", "feTurbulence/baseFrequency=0.05-0.01",
     "filter/subregion-and-primitiveUnits=objectBoundingBox-1", "filter/subregion-and-primitiveUnits=objectBoundingBox-2",
+    "feImage/link-to-an-element", "feImage/link-to-g", "feImage/link-to-use", "feImage/link-to-an-element-with-opacity",
+    "feImage/link-to-an-element-with-transform", "feImage/link-on-an-element-with-transform",
+    "feImage/link-on-an-element-with-complex-transform", "feImage/link-to-an-element-outside-defs-1",
+    "feImage/link-to-an-element-outside-defs-2", "feImage/with-x-y", "feImage/with-x-y-and-protruding-subregion-1",
+    "feImage/with-x-y-and-protruding-subregion-2", "feImage/chained-feImage",
 ]
 # fixtures outside filters/ (paths relative to crates/resvg/tests/tests)
 OTHER = [

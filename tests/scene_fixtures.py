@@ -13,7 +13,7 @@ import numpy as np
 
 from resvg_amd import abi
 from resvg_amd.filters import FilterGraph
-from resvg_amd.scene import PaintSpec, Scene, rect_path
+from resvg_amd.scene import PaintSpec, Scene, f32, rect_path
 
 from fixture_scenes import fixtures as filter_fixtures
 
@@ -636,6 +636,84 @@ def _tile_subregion_fixtures():
 TILE_SUBREGION_GUIDES = ((50.0, 50.0, 100.0, 100.0),)
 
 
+def _fe_image_fixtures():
+    """filters/feImage/* that link to an element: usvg converts the element into a
+    group of its own (filter::Image::root, filter.rs:831-855) and resvg draws that
+    group's children into the primitive's pixmap (filter/mod.rs:861-889).  Filter
+    regions follow usvg (default -10%/120% of the element bbox; x=0 y=0 w=1 h=1 is the
+    bbox), element transforms become group transforms (parsed in f64, stored as f32),
+    and the SVG's 1-unit guide strokes are drawn as strokes."""
+    out = {}
+    ident = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
+    t50 = f32(math.tan(math.radians(50.0)))
+    black, green = PaintSpec.solid(BLACK), PaintSpec.solid(GREEN)
+
+    def build(rect, fill, content, region=None, prim=None, transform=ident, before_strokes=(), strokes=(),
+              stroke_color=BLACK):
+        sc = Scene(200, 200)
+        root = sc.content_group()
+        content(sc, root)
+        g = FilterGraph(region or tuple(f32(v) for v in (rect[0] - 0.1 * rect[2], rect[1] - 0.1 * rect[3],
+                                                          1.2 * rect[2], 1.2 * rect[3])))
+        g.image(root, **({} if prim is None else {"rect": prim}))
+        grp = sc.group(sc.root, transform=transform, filters=[g])
+        sc.path(grp, *rect_path(*rect), fill=PaintSpec.solid(fill))
+        for r, paint in before_strokes:
+            sc.path(sc.root, *rect_path(*r), fill=paint)
+        for r in strokes:
+            sc.path(sc.root, *rect_path(*r), stroke=PaintSpec.solid(stroke_color))
+        return sc
+
+    def rect3(x, y, w, h, paint=green):
+        return lambda sc, root: sc.path(root, *rect_path(x, y, w, h), fill=paint)
+
+    R1, R2 = (20.0, 20.0, 160.0, 160.0), (40.0, 40.0, 120.0, 120.0)
+    out["feImage/link-to-an-element"] = build(R1, RED, rect3(36.0, 36.0, 120.0, 120.0), strokes=(R2,))
+
+    def g1(sc, root):
+        sc.path(sc.group(root), *rect_path(36.0, 36.0, 120.0, 120.0), fill=green)
+    out["feImage/link-to-g"] = build(R1, RED, g1, strokes=(R2,))
+
+    def use1(sc, root):  # <use> becomes a group (its translate(0 0)) holding g1's group
+        sc.path(sc.group(sc.group(root)), *rect_path(36.0, 36.0, 120.0, 120.0), fill=green)
+    out["feImage/link-to-use"] = build(R1, RED, use1, strokes=(R2,))
+
+    def half(sc, root):  # opacity moves to a group of the element's own
+        sc.path(sc.group(root, opacity=0.5), *rect_path(36.0, 36.0, 120.0, 120.0), fill=green)
+    out["feImage/link-to-an-element-with-opacity"] = build(R1, RED, half, strokes=(R2,))
+
+    def skewed(sc, root):
+        sc.path(sc.group(root, transform=(1.0, 0.0, t50, 1.0, 0.0, 0.0)), *rect_path(-60.0, 0.0, 120.0, 120.0),
+                fill=green)
+    out["feImage/link-to-an-element-with-transform"] = build(R2, RED, skewed, region=R2, strokes=(R2,))
+    out["feImage/link-on-an-element-with-transform"] = build(
+        R2, RED, rect3(0.0, 0.0, 120.0, 120.0), region=R2, transform=(0.5, 0.0, 0.0, 0.5, 30.0, 30.0),
+        strokes=(R2,))
+    out["feImage/link-on-an-element-with-complex-transform"] = build(
+        R2, RED, rect3(-60.0, 0.0, 120.0, 120.0), region=R2, transform=(1.0, 0.0, t50, 1.0, -90.0, 0.0),
+        strokes=(R2,))
+    for n, at in (("1", (0.0, 0.0)), ("2", (100.0, 0.0))):
+        out[f"feImage/link-to-an-element-outside-defs-{n}"] = build(
+            R1, RED, rect3(at[0], at[1], 160.0, 160.0), region=(20.0, 20.0, 192.0, 192.0),
+            before_strokes=(((at[0], at[1], 160.0, 160.0), green),), strokes=(R1,))
+
+    R5 = (50.0, 50.0, 100.0, 100.0)
+    for name, xy in (("with-x-y", (50.0, 50.0)), ("with-x-y-and-protruding-subregion-1", (60.0, 90.0)),
+                     ("with-x-y-and-protruding-subregion-2", (20.0, 0.0))):
+        # feImage x/y with width/height from the filter region (filter.rs:430-435)
+        out[f"feImage/{name}"] = build(R5, (0x1E, 0x17, 0x05), rect3(0.0, 0.0, 100.0, 100.0),
+                                       prim=(xy[0], xy[1], 120.0, 120.0), strokes=(R5,), stroke_color=(0, 0, 255))
+
+    def chained(sc, root):  # rect3 carries filter2, whose feImage draws rect4
+        inner = sc.content_group()
+        sc.path(inner, *rect_path(40.0, 40.0, 120.0, 120.0), fill=green)
+        g2 = FilterGraph((0.0, 0.0, 120.0, 120.0))
+        g2.image(inner)
+        sc.path(sc.group(root, filters=[g2]), *rect_path(0.0, 0.0, 120.0, 120.0), fill=PaintSpec.solid(RED))
+    out["feImage/chained-feImage"] = build(R1, RED, chained, region=R1, strokes=((60.0, 60.0, 80.0, 80.0),))
+    return out
+
+
 def all_scene_fixtures():
     out = dict(shape_fixtures())
     out.update(_hairline_fixtures())
@@ -644,6 +722,7 @@ def all_scene_fixtures():
     for fx in filter_fixtures():
         out[fx.name] = filter_scene(fx)
     out.update(_tile_subregion_fixtures())
+    out.update(_fe_image_fixtures())
     return out
 
 

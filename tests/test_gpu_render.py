@@ -249,6 +249,58 @@ def test_random_clip_paths_and_masks(ctx, seed):
     _assert_same(_gpu(ctx, sc, ts, 176, 176), _oracle(sc, ts, 176, 176), f"clip/mask seed {seed}")
 
 
+def _fe_image_scene(seed, size=160):
+    """Filtered groups whose graphs start from feImage primitives (content groups of
+    random paths, some under transforms, opacity or a nested feImage filter of their
+    own), with x/y subregions and a blur or composite after the image."""
+    from resvg_amd.filters import SOURCE_GRAPHIC, FilterGraph
+    rng = np.random.default_rng(seed)
+    sc = Scene(size, size)
+    contents = []
+    for i in range(3):
+        root = sc.content_group()
+        parent = root
+        if rng.random() < 0.5:
+            k = float(np.float32(rng.uniform(-0.4, 0.4)))
+            parent = sc.group(root, transform=(1.0, 0.0, k, 1.0, float(np.float32(rng.uniform(-10, 10))), 0.0),
+                              opacity=float(np.float32(rng.uniform(0.4, 1.0))))
+        if contents and rng.random() < 0.6:  # nested: a filtered group drawing an earlier content group
+            g = FilterGraph((0.0, 0.0, float(size), float(size)))
+            g.image(contents[-1])
+            parent = sc.group(parent, filters=[g])
+        for _ in range(int(rng.integers(1, 4))):
+            v, p = _random_path(rng, size, int(rng.integers(2, 8)))
+            sc.path(parent, v, p, fill=_paint(rng, size))
+        contents.append(root)
+    for i in range(4):
+        x, y = (float(np.float32(v)) for v in rng.uniform(-10, size * 0.4, 2))
+        w, h = (float(np.float32(v)) for v in rng.uniform(size * 0.4, size * 0.9, 2))
+        g = FilterGraph((x, y, w, h))
+        prim = None
+        if rng.random() < 0.5:
+            prim = (x + float(np.float32(rng.uniform(-8, 20))), y + float(np.float32(rng.uniform(-8, 20))), w, h)
+        r = g.image(contents[int(rng.integers(len(contents)))], **({} if prim is None else {"rect": prim}))
+        if rng.random() < 0.5:
+            g.gaussian_blur(float(np.float32(rng.uniform(0.5, 3.0))), in1=r)
+        else:
+            g.composite(abi.COMPOSITE_OVER, in1=r, in2=SOURCE_GRAPHIC)
+        ts = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
+        if rng.random() < 0.4:
+            a = float(np.float32(rng.uniform(0.7, 1.2)))
+            ts = (a, 0.0, 0.0, a, float(np.float32(rng.uniform(-10, 10))), float(np.float32(rng.uniform(-10, 10))))
+        grp = sc.group(sc.root, transform=ts, filters=[g])
+        v, p = _random_path(rng, size, int(rng.integers(2, 6)))
+        sc.path(grp, v, p, fill=_paint(rng, size))
+    return sc
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_fe_image(ctx, seed):
+    sc = _fe_image_scene(900 + seed)
+    ts = abi.Transform(1.2, 0.0, 0.0, 1.2, 3.0, -2.0) if seed % 2 else abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, 192, 192), _oracle(sc, ts, 192, 192), f"feImage seed {seed}")
+
+
 def test_composites_onto_existing_pixels(ctx):
     rng = np.random.default_rng(7)
     base = rng.integers(0, 256, (96, 128, 4)).astype(np.uint8)
