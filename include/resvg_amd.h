@@ -41,7 +41,7 @@ typedef int32_t rvk_status;
 #define RVK_ERR_OUT_OF_MEMORY    2  /* device allocation failed */
 #define RVK_ERR_HIP              3  /* a HIP runtime call failed */
 #define RVK_ERR_INVALID_REGION   4  /* filter::Error::InvalidRegion (filter/mod.rs:86-90) */
-#define RVK_ERR_UNSUPPORTED      5  /* feature outside this library's scope (raster images, patterns) */
+#define RVK_ERR_UNSUPPORTED      5  /* feature outside this library's scope (nested SVG images) */
 #define RVK_ERR_NO_DEVICE        6  /* no HIP device / library built without a device */
 #define RVK_ERR_TOO_LARGE        7  /* input beyond a fixed limit of the entry point (the no-allocation form) */
 
@@ -188,12 +188,15 @@ typedef struct rvk_filter_data {
 /* Scene: a flattened usvg::Tree (crates/usvg/src/tree/mod.rs)                */
 /* ------------------------------------------------------------------------- */
 enum { RVK_VERB_MOVE = 0, RVK_VERB_LINE, RVK_VERB_QUAD, RVK_VERB_CUBIC, RVK_VERB_CLOSE };
-enum { RVK_PAINT_NONE = 0, RVK_PAINT_COLOR, RVK_PAINT_LINEAR, RVK_PAINT_RADIAL };
+enum { RVK_PAINT_NONE = 0, RVK_PAINT_COLOR, RVK_PAINT_LINEAR, RVK_PAINT_RADIAL, RVK_PAINT_PATTERN };
 enum { RVK_SPREAD_PAD = 0, RVK_SPREAD_REFLECT, RVK_SPREAD_REPEAT };
 enum { RVK_FILL_NONZERO = 0, RVK_FILL_EVENODD };
 enum { RVK_CAP_BUTT = 0, RVK_CAP_ROUND, RVK_CAP_SQUARE };
 enum { RVK_JOIN_MITER = 0, RVK_JOIN_MITER_CLIP, RVK_JOIN_ROUND, RVK_JOIN_BEVEL };
-enum { RVK_NODE_GROUP = 0, RVK_NODE_PATH };
+enum { RVK_NODE_GROUP = 0, RVK_NODE_PATH, RVK_NODE_IMAGE };
+/* tiny_skia::FilterQuality of a raster image (image.rs:176-183: OptimizeSpeed, CrispEdges
+ * and Pixelated sample nearest, Smooth bilinear, OptimizeQuality and HighQuality bicubic) */
+enum { RVK_QUALITY_NEAREST = 0, RVK_QUALITY_BILINEAR, RVK_QUALITY_BICUBIC };
 
 typedef struct rvk_stop {                  /* usvg::Stop */
     float offset;
@@ -208,8 +211,10 @@ typedef struct rvk_paint {                 /* usvg::Paint + fill/stroke opacity 
     uint32_t spread;                       /* RVK_SPREAD_* */
     float x1, y1, x2, y2;                  /* linear */
     float cx, cy, r, fx, fy, fr;           /* radial */
-    rvk_transform transform;               /* gradient transform */
+    rvk_transform transform;               /* gradient transform; a pattern's Pattern::transform */
     uint32_t stop_offset, stop_count;
+    rvk_rect rect;                         /* pattern: Pattern::rect (user space, units resolved) */
+    uint32_t root;                         /* pattern: group holding Pattern::root's children */
 } rvk_paint;
 
 typedef struct rvk_path {                  /* usvg::Path (mod.rs:1282-1440) */
@@ -238,6 +243,16 @@ typedef struct rvk_group {                 /* usvg::Group (mod.rs:1035-1225) */
 } rvk_group;
 
 typedef struct rvk_node_ref { uint32_t kind, index; } rvk_node_ref;
+
+typedef struct rvk_image {                 /* usvg::Image of a raster kind (image.rs:4-35, 159-206) */
+    const uint8_t *pixels;                 /* the decoded image as tiny-skia holds it: premultiplied
+                                              RGBA8, row-major, width * 4 bytes a row, host memory */
+    uint32_t width, height;                /* drawn over the rect (0, 0, width, height) with the
+                                              node's transform (usvg puts the view box transform on
+                                              the parent group) */
+    uint32_t quality;                      /* RVK_QUALITY_* */
+    uint32_t visible;                      /* Image::is_visible */
+} rvk_image;
 
 enum { RVK_MASK_LUMINANCE = 0, RVK_MASK_ALPHA };
 
@@ -268,6 +283,7 @@ typedef struct rvk_scene {
     const rvk_clip_path *clip_paths;  uint32_t clip_path_count;
     const rvk_mask *masks;            uint32_t mask_count;
     const float *dashes;              /* dash arrays of the paths' strokes */
+    const rvk_image *images;          uint32_t image_count;  /* raster images (RVK_NODE_IMAGE) */
 } rvk_scene;
 
 /* ------------------------------------------------------------------------- */
@@ -534,6 +550,7 @@ typedef struct rvk_stats {
     double host_build_ms;   /* host time building edges, bands and shaders */
     double host_wait_ms;    /* host time waiting for the device */
     double host_geom_ms;    /* part of host_build_ms: path geometry (edges, strokes), on all host threads */
+    uint64_t validated_flushes; /* flush blocks index-checked on the host (RVK_VALIDATE set) */
 } rvk_stats;
 rvk_status rvk_get_stats(rvk_context *ctx, rvk_stats *out);
 

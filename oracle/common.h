@@ -17,6 +17,12 @@ namespace orc {
 using FeImageHook = std::function<bool(uint32_t root, const float ts[6], uint8_t *px, uint32_t w, uint32_t h)>;
 extern thread_local FeImageHook fe_image_hook;
 
+// Pattern tiles (render_pattern_pixmap, path.rs:179-205): set by the render walk
+// around each path it draws; draws the children of scene group `root` with the given
+// transform into a w x h pixmap with the walk's Context; false on a bad group.
+using PatternHook = std::function<bool(uint32_t root, const float ts[6], uint8_t *px, uint32_t w, uint32_t h)>;
+extern thread_local PatternHook pattern_hook;
+
 // Time split of a render by operation (filter primitive kinds, then fills, strokes,
 // layer composites and whole filters), for the CPU baseline's per-primitive split.
 // Off unless orc_prof_enable(1) was called; a Scope nested in another counts in both.

@@ -1230,6 +1230,17 @@ struct Shader {
     // never a radial gradient): RasterPipelineBlitter::new then reduces SourceOver to
     // Source, whose partial coverage is lerp_1_float instead of scale_1_float + over.
     bool as_source = false;
+    // Pattern shader (tiny-skia shaders/pattern.rs): a pixmap (premultiplied RGBA8)
+    // under the device -> pixmap transform `ts`, with the Repeat spread (a pattern
+    // paint, path.rs:58-68) or Pad (a raster image, image.rs:185-191), sampled nearest
+    // (0), bilinear (1) or bicubic (2) -- nearest whenever `ts` is a translation -- and
+    // the opacity applied by scale_1_float; runs in the f32 pipeline
+    bool pattern = false;
+    std::vector<uint8_t> tile;
+    uint32_t tile_w = 0, tile_h = 0;
+    bool tile_repeat = true;
+    int quality = 0;
+    float pattern_opacity = 1.0f;
 };
 
 // Transform::invert (tiny-skia-path transform.rs)
@@ -1270,6 +1281,44 @@ static void set_solid(Shader &s, const float c[4]) {
     out[3] = (uint32_t)(a * 255.0f + 0.5f);
 }
 
+// render_pattern_pixmap (path.rs:179-205): the tile of round(rect * scale) pixels
+// drawn by the render walk (orc::pattern_hook) with Transform::from_scale(sx, sy), and
+// the shader transform pattern.transform * translate(rect.xy) * scale(1 / sx, 1 / sy),
+// post-concatenated with the path transform (Shader::transform) and inverted
+// (Pattern::push_stages); None (an invalid shader) on an empty tile or a singular
+// transform.  `op` is Shader::apply_opacity's factor.
+static Shader make_pattern_shader(const rvk_paint &p, Ts ts, float op) {
+    Shader s;
+    if (!pattern_hook) return s;  // outside a render walk
+    float sx, sy;
+    ts_get_scale(ts_concat(ts, ts_from(p.transform)), sx, sy);
+    auto to_u32 = [](float v) -> uint32_t {  // `as u32`: saturating, NaN to 0
+        if (!(v > 0.0f)) return 0u;
+        return v >= 4294967296.0f ? UINT32_MAX : (uint32_t)v;
+    };
+    const uint32_t w = to_u32(std::round(p.rect.w * sx)), h = to_u32(std::round(p.rect.h * sy));
+    if (w == 0 || h == 0) return s;                                   // IntSize::from_wh
+    if ((uint64_t)w * 4 > (uint64_t)INT32_MAX || (uint64_t)w * h * 4 > (1ull << 31)) return s;  // Pixmap::new
+    s.tile.assign((size_t)w * h * 4, 0);
+    const float t6[6] = {sx, 0.0f, 0.0f, sy, 0.0f, 0.0f};
+    if (!pattern_hook(p.root, t6, s.tile.data(), w, h)) return Shader{};
+    Ts pts = ts_concat(ts_concat(ts_from(p.transform), Ts{1.0f, 0.0f, 0.0f, 1.0f, p.rect.x, p.rect.y}),
+                       Ts{1.0f / sx, 0.0f, 0.0f, 1.0f / sy, 0.0f, 0.0f});
+    if (!ts_invert(ts_concat(ts, pts), s.ts)) return Shader{};
+    // FilterQuality::Bicubic, lowered to Nearest for an identity or translate transform
+    s.quality = (s.ts.sx == 1.0f && s.ts.sy == 1.0f && s.ts.kx == 0.0f && s.ts.ky == 0.0f) ? 0 : 2;
+    s.tile_repeat = true;
+    const float o = std::min(std::max(p.opacity, 0.0f), 1.0f);
+    s.pattern_opacity = std::min(std::max(o * op, 0.0f), 1.0f);
+    s.tile_w = w;
+    s.tile_h = h;
+    s.pattern = true;
+    s.solid = false;
+    s.opaque = false;
+    s.valid = true;
+    return s;
+}
+
 // `opacity` is Shader::apply_opacity's factor (a hairline's coverage), applied to the
 // colour's alpha or to every stop's.
 static Shader make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, float opacity = 1.0f) {
@@ -1284,6 +1333,7 @@ static Shader make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, float 
         s.valid = true;
         return s;
     }
+    if (p.kind == RVK_PAINT_PATTERN) return make_pattern_shader(p, ts, op);
     if (p.kind != RVK_PAINT_LINEAR && p.kind != RVK_PAINT_RADIAL) return s;
     // convert_base_gradient (path.rs:154-177)
     struct GS { float pos; float c[4]; };
@@ -1506,19 +1556,105 @@ static void shade(const Shader &s, uint32_t px, uint32_t py, uint32_t out[4]) {
     if (!s.opaque) for (int k = 0; k < 3; k++) out[k] = div255(out[k] * out[3]);
 }
 
+// The premultiplied colour of a pattern shader at pixel (px, py), f32 (tiny-skia
+// highp seed_shader, transform, then repeat + gather, or bicubic + clamp_0 + clamp_a,
+// then scale_1_float by the opacity).
+static void pattern_color(const Shader &s, uint32_t px, uint32_t py, float out[4]) {
+    const float x = (float)px + 0.5f, y = (float)py + 0.5f;
+    const float tx = x * s.ts.sx + (y * s.ts.kx + s.ts.tx);
+    const float ty = x * s.ts.ky + (y * s.ts.sy + s.ts.ty);
+    const float fw = (float)s.tile_w, fh = (float)s.tile_h;
+    const float iw = 1.0f / fw, ih = 1.0f / fh;
+    // the largest float below the limit (ulp_sub): gather clamps to [0, limit) this way
+    auto below = [](float v) {
+        uint32_t b;
+        std::memcpy(&b, &v, 4);
+        b -= 1;
+        std::memcpy(&v, &b, 4);
+        return v;
+    };
+    const float wmax = below(fw), hmax = below(fh);
+    // Repeat tiling (none for Pad) then gather_ix + load_8888 (v * (1/255))
+    auto fetch = [&](float sx, float sy, float c[4]) {
+        if (s.tile_repeat) {
+            sx = sx - std::floor(sx * iw) * fw;
+            sy = sy - std::floor(sy * ih) * fh;
+        }
+        sx = std::fmin(std::fmax(sx, 0.0f), wmax);  // NaN goes to 0, like SIMD max(v, 0)
+        sy = std::fmin(std::fmax(sy, 0.0f), hmax);
+        const size_t i = ((size_t)(int32_t)sy * s.tile_w + (size_t)(int32_t)sx) * 4;
+        for (int k = 0; k < 4; k++) c[k] = (float)s.tile[i + k] * (1.0f / 255.0f);
+    };
+    if (s.quality == 0) {
+        fetch(tx, ty, out);
+    } else if (s.quality == 1) {
+        // bilinear: weights 1 - f, f of the fractions of x + 0.5 and y + 0.5 over the
+        // 2 x 2 texels from x - 0.5, y - 0.5 (no clamp stages follow)
+        const float fx = (tx + 0.5f) - std::floor(tx + 0.5f), fy = (ty + 0.5f) - std::floor(ty + 0.5f);
+        const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy};
+        for (int k = 0; k < 4; k++) out[k] = 0.0f;
+        float yy = ty - 0.5f;
+        for (int j = 0; j < 2; j++) {
+            float xx = tx - 0.5f;
+            for (int i = 0; i < 2; i++) {
+                float c[4];
+                fetch(xx, yy, c);
+                const float wgt = wx[i] * wy[j];
+                for (int k = 0; k < 4; k++) out[k] = wgt * c[k] + out[k];
+                xx = xx + 1.0f;
+            }
+            yy = yy + 1.0f;
+        }
+    } else {
+        // bicubic_near(t) = ((-21/18 t + 27/18) t + 9/18) t + 1/18, bicubic_far(t) =
+        // t^2 (7/18 t - 6/18); weights far(1-f), near(1-f), near(f), far(f) of the
+        // fractions of x + 0.5 and y + 0.5, over the 4 x 4 texels from x - 1.5, y - 1.5
+        auto near_w = [](float t) {
+            return t * (t * ((-21.0f / 18.0f) * t + 27.0f / 18.0f) + 9.0f / 18.0f) + 1.0f / 18.0f;
+        };
+        auto far_w = [](float t) { return (t * t) * ((7.0f / 18.0f) * t + -6.0f / 18.0f); };
+        const float fx = (tx + 0.5f) - std::floor(tx + 0.5f), fy = (ty + 0.5f) - std::floor(ty + 0.5f);
+        const float wx[4] = {far_w(1.0f - fx), near_w(1.0f - fx), near_w(fx), far_w(fx)};
+        const float wy[4] = {far_w(1.0f - fy), near_w(1.0f - fy), near_w(fy), far_w(fy)};
+        for (int k = 0; k < 4; k++) out[k] = 0.0f;
+        float yy = ty - 1.5f;
+        for (int j = 0; j < 4; j++) {
+            float xx = tx - 1.5f;
+            for (int i = 0; i < 4; i++) {
+                float c[4];
+                fetch(xx, yy, c);
+                const float wgt = wx[i] * wy[j];
+                for (int k = 0; k < 4; k++) out[k] = wgt * c[k] + out[k];
+                xx = xx + 1.0f;
+            }
+            yy = yy + 1.0f;
+        }
+        for (int k = 0; k < 4; k++) out[k] = std::fmax(out[k], 0.0f);  // clamp_0
+        out[3] = std::fmin(out[3], 1.0f);                               // clamp_a
+        for (int k = 0; k < 3; k++) out[k] = std::fmin(out[k], out[3]);
+    }
+    if (s.pattern_opacity != 1.0f)
+        for (int k = 0; k < 4; k++) out[k] = out[k] * s.pattern_opacity;
+}
+
 // Composite coverage `c` (1..255) x shader onto one pixel: lowp SourceOver
 // (scale_1_float, source_over), or for an opaque shader Source (full coverage stores
 // the colour, partial coverage is lerp_1_float: div255(d * (255 - c) + s * c)).
 static void composite_px(const Shader &s, uint32_t c, uint32_t x, uint32_t y, uint8_t *d) {
-    if (s.focal) {
-        // tiny-skia's f32 pipeline: gradient, premultiply, apply_vector_mask,
-        // scale_1_float (coverage * (1/255)), load_dst (v * (1/255)), source_over
-        // (d * (1 - sa) + s), store (round-to-nearest-even of clamp(v) * 255)
-        bool degenerate;
+    if (s.focal || s.pattern) {
+        // tiny-skia's f32 pipeline: the shader's colour (gradient, premultiply,
+        // apply_vector_mask; or the pattern stages), scale_1_float (coverage * (1/255)),
+        // load_dst (v * (1/255)), source_over (d * (1 - sa) + s), store
+        // (round-to-nearest-even of clamp(v) * 255)
         float sc[4];
-        grad_color(s, grad_t(s, x, y, degenerate), sc);
-        if (!s.opaque) for (int k = 0; k < 3; k++) sc[k] *= sc[3];
-        if (degenerate) for (int k = 0; k < 4; k++) sc[k] = 0.0f;
+        if (s.pattern) {
+            pattern_color(s, x, y, sc);
+        } else {
+            bool degenerate;
+            grad_color(s, grad_t(s, x, y, degenerate), sc);
+            if (!s.opaque) for (int k = 0; k < 3; k++) sc[k] *= sc[3];
+            if (degenerate) for (int k = 0; k < 4; k++) sc[k] = 0.0f;
+        }
         if (c != 255) {
             const float cf = (float)c * (1.0f / 255.0f);
             for (int k = 0; k < 4; k++) sc[k] *= cf;
@@ -1777,7 +1913,12 @@ static void stroke_hairline(const rvk_scene &sc, const rvk_path &path, Ts ts, co
     }
 }
 
+// Also true for segments tiny-skia cannot fill: a non-finite point (Path::transform's
+// Rect::from_points fails) or bounds beyond f32::MAX / 4 (is_too_big_for_math).
 static bool path_bounds_nearly_zero(const std::vector<Seg> &segs) {
+    for (auto &s : segs)
+        for (int i = 0; i < s.n; i++)
+            if (!(std::fabs(s.p[i].x) <= 8.50705917e+37f && std::fabs(s.p[i].y) <= 8.50705917e+37f)) return true;
     float mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
     for (auto &s : segs)
         for (int i = 0; i < s.n; i++) {
@@ -1827,6 +1968,46 @@ bool stroke_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t 
     if (!stroke_segments(sc, path, ts, segs) || path_bounds_nearly_zero(segs)) return false;
     std::vector<uint8_t> cov;
     coverage(segs, RVK_FILL_NONZERO, path.anti_alias != 0, w, h, cov);
+    blit(s, cov, px, w, h);
+    return true;
+}
+
+// image::render_raster (image.rs:159-206): Pixmap::fill_rect of (0, 0, w, h) with a
+// Pattern shader over the decoded pixmap (Pad, the image's filter quality, opacity 1,
+// identity pattern transform), anti-aliased SourceOver; with a transform fill_rect is
+// fill_path of the rect, and without one fill_rect_aa covers an integer rect exactly
+// as fill_path does.
+bool draw_image_into(const rvk_image &img, Ts ts, uint8_t *px, uint32_t w, uint32_t h) {
+    if (!img.visible || img.width == 0 || img.height == 0 || !img.pixels) return false;
+    const float iw = (float)img.width, ih = (float)img.height;
+    const uint8_t verbs[5] = {RVK_VERB_MOVE, RVK_VERB_LINE, RVK_VERB_LINE, RVK_VERB_LINE, RVK_VERB_CLOSE};
+    const float pts[8] = {0.0f, 0.0f, iw, 0.0f, iw, ih, 0.0f, ih};
+    rvk_scene sc{};
+    sc.verbs = verbs;
+    sc.points = pts;
+    rvk_path path{};
+    path.verb_count = 5;
+    path.visible = 1;
+    path.anti_alias = 1;
+    path.fill_rule = RVK_FILL_NONZERO;
+    path.bounds = rvk_rect{0.0f, 0.0f, iw, ih};
+    Shader s;
+    if (!ts_invert(ts, s.ts)) return false;
+    s.tile.assign(img.pixels, img.pixels + (size_t)img.width * img.height * 4);
+    s.tile_w = img.width;
+    s.tile_h = img.height;
+    s.tile_repeat = false;
+    s.quality = (s.ts.sx == 1.0f && s.ts.sy == 1.0f && s.ts.kx == 0.0f && s.ts.ky == 0.0f) ? 0 : (int)img.quality;
+    s.pattern_opacity = 1.0f;
+    s.pattern = true;
+    s.solid = false;
+    s.opaque = false;
+    s.valid = true;
+    std::vector<Seg> segs;
+    fill_segments(sc, path, ts, segs);
+    if (segs.empty() || path_bounds_nearly_zero(segs)) return false;
+    std::vector<uint8_t> cov;
+    coverage(segs, RVK_FILL_NONZERO, true, w, h, cov);
     blit(s, cov, px, w, h);
     return true;
 }

@@ -13,6 +13,7 @@ namespace orc {
 bool fill_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h);
 bool stroke_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h);
 bool clear_path_into(const rvk_scene &sc, const rvk_path &path, Ts ts, uint8_t *px, uint32_t w, uint32_t h);
+bool draw_image_into(const rvk_image &img, Ts ts, uint8_t *px, uint32_t w, uint32_t h);
 void rect_mask(const rvk_rect &r, Ts ts, uint32_t w, uint32_t h, std::vector<uint8_t> &mask);
 }
 
@@ -33,9 +34,20 @@ uint32_t convert_blend_mode(uint32_t m) {
 void render_nodes(const rvk_scene &sc, const rvk_group &g, const Ctx &ctx, Ts ts, uint8_t *px, uint32_t w,
                   uint32_t h);
 
-// path::render (path.rs:6-24)
-void render_path(const rvk_scene &sc, const rvk_path &p, Ts ts, uint8_t *px, uint32_t w, uint32_t h) {
+// path::render (path.rs:6-24); a pattern paint renders its tile with this walk's
+// Context (render_pattern_pixmap passes `ctx` on)
+void render_path(const rvk_scene &sc, const rvk_path &p, const Ctx &ctx, Ts ts, uint8_t *px, uint32_t w,
+                 uint32_t h) {
     if (!p.visible) return;
+    const PatternHook saved = pattern_hook;
+    const bool has_pattern = p.fill.kind == RVK_PAINT_PATTERN || p.stroke.kind == RVK_PAINT_PATTERN;
+    if (has_pattern) {
+        pattern_hook = [&sc, &ctx](uint32_t root, const float *t, uint8_t *img, uint32_t iw, uint32_t ih) {
+            if (root >= sc.group_count) return false;
+            render_nodes(sc, sc.groups[root], ctx, Ts{t[0], t[1], t[2], t[3], t[4], t[5]}, img, iw, ih);
+            return true;
+        };
+    }
     auto fill = [&] {
         prof::Scope timed(prof::kFill);
         fill_path_into(sc, p, ts, px, w, h);
@@ -51,6 +63,7 @@ void render_path(const rvk_scene &sc, const rvk_path &p, Ts ts, uint8_t *px, uin
         stroke();
         fill();
     }
+    if (has_pattern) pattern_hook = saved;
 }
 
 // ---------------------------------------------------------------------------
@@ -100,6 +113,7 @@ void clip_draw_children(const rvk_scene &sc, const rvk_group &parent, bool clear
                         uint32_t h, int depth) {
     for (uint32_t i = 0; i < parent.child_count; i++) {
         const rvk_node_ref &n = sc.children[parent.child_offset + i];
+        if (n.kind == RVK_NODE_IMAGE) continue;  // not clip path content
         if (n.kind == RVK_NODE_PATH) {
             const rvk_path &p = sc.paths[n.index];
             if (!p.visible) continue;
@@ -228,7 +242,8 @@ void render_nodes(const rvk_scene &sc, const rvk_group &g, const Ctx &ctx, Ts ts
     for (uint32_t i = 0; i < g.child_count; i++) {
         const rvk_node_ref &n = sc.children[g.child_offset + i];
         if (n.kind == RVK_NODE_GROUP) render_group(sc, sc.groups[n.index], ctx, ts, px, w, h);
-        else render_path(sc, sc.paths[n.index], ts, px, w, h);
+        else if (n.kind == RVK_NODE_IMAGE) draw_image_into(sc.images[n.index], ts, px, w, h);  // image::render
+        else render_path(sc, sc.paths[n.index], ctx, ts, px, w, h);
     }
 }
 
@@ -263,7 +278,8 @@ int32_t orc_render_node(const rvk_scene *scene, uint32_t kind, uint32_t index, c
     Ctx ctx{max_filter_bbox(w, h)};
     const Ts t = ts_concat(ts_from(ts), Ts{1.0f, 0.0f, 0.0f, 1.0f, -bbox->x, -bbox->y});
     if (kind == RVK_NODE_GROUP) render_group(*scene, scene->groups[index], ctx, t, px, w, h);
-    else render_path(*scene, scene->paths[index], t, px, w, h);
+    else if (kind == RVK_NODE_IMAGE) draw_image_into(scene->images[index], t, px, w, h);
+    else render_path(*scene, scene->paths[index], ctx, t, px, w, h);
     return 1;
 }
 
@@ -283,6 +299,7 @@ void orc_apply_mask(uint8_t *px, uint32_t w, uint32_t h, const uint8_t *mask) {
 // ---------------------------------------------------------------------------
 namespace orc {
 thread_local FeImageHook fe_image_hook;
+thread_local PatternHook pattern_hook;
 namespace prof {
 bool enabled = false;
 double ms[kSlots] = {};

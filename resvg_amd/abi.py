@@ -42,12 +42,13 @@ CHANNEL_R, CHANNEL_G, CHANNEL_B, CHANNEL_A = range(4)
 LIGHT_DISTANT, LIGHT_POINT, LIGHT_SPOT = range(3)
 
 VERB_MOVE, VERB_LINE, VERB_QUAD, VERB_CUBIC, VERB_CLOSE = range(5)
-PAINT_NONE, PAINT_COLOR, PAINT_LINEAR, PAINT_RADIAL = range(4)
+PAINT_NONE, PAINT_COLOR, PAINT_LINEAR, PAINT_RADIAL, PAINT_PATTERN = range(5)
 SPREAD_PAD, SPREAD_REFLECT, SPREAD_REPEAT = range(3)
 FILL_NONZERO, FILL_EVENODD = range(2)
 CAP_BUTT, CAP_ROUND, CAP_SQUARE = range(3)
 JOIN_MITER, JOIN_MITER_CLIP, JOIN_ROUND, JOIN_BEVEL = range(4)
-NODE_GROUP, NODE_PATH = range(2)
+NODE_GROUP, NODE_PATH, NODE_IMAGE = range(3)
+QUALITY_NEAREST, QUALITY_BILINEAR, QUALITY_BICUBIC = range(3)
 
 
 class Transform(C.Structure):
@@ -174,7 +175,7 @@ class Paint(C.Structure):
                 ("x1", C.c_float), ("y1", C.c_float), ("x2", C.c_float), ("y2", C.c_float),
                 ("cx", C.c_float), ("cy", C.c_float), ("r", C.c_float), ("fx", C.c_float), ("fy", C.c_float),
                 ("fr", C.c_float), ("transform", Transform), ("stop_offset", C.c_uint32),
-                ("stop_count", C.c_uint32)]
+                ("stop_count", C.c_uint32), ("rect", Rect), ("root", C.c_uint32)]
 
 
 class Path(C.Structure):
@@ -207,12 +208,18 @@ class Mask(C.Structure):
     _fields_ = [("rect", Rect), ("kind", C.c_uint32), ("mask", C.c_int32), ("root", C.c_uint32)]
 
 
+class Image(C.Structure):
+    _fields_ = [("pixels", C.POINTER(C.c_uint8)), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("quality", C.c_uint32), ("visible", C.c_uint32)]
+
+
 class Scene(C.Structure):
     _fields_ = [("groups", C.POINTER(Group)), ("group_count", C.c_uint32), ("paths", C.POINTER(Path)),
                 ("path_count", C.c_uint32), ("children", C.POINTER(NodeRef)), ("verbs", C.POINTER(C.c_uint8)),
                 ("points", C.POINTER(C.c_float)), ("stops", C.POINTER(Stop)), ("filters", C.POINTER(Filter)),
                 ("filter_data", FilterData), ("clip_paths", C.POINTER(ClipPath)), ("clip_path_count", C.c_uint32),
-                ("masks", C.POINTER(Mask)), ("mask_count", C.c_uint32), ("dashes", C.POINTER(C.c_float))]
+                ("masks", C.POINTER(Mask)), ("mask_count", C.c_uint32), ("dashes", C.POINTER(C.c_float)),
+                ("images", C.POINTER(Image)), ("image_count", C.c_uint32)]
 
 
 class PathGeometry(C.Structure):
@@ -233,7 +240,7 @@ class Stats(C.Structure):
                 ("filters", C.c_uint64), ("scratch_bytes", C.c_uint64), ("edge_pieces", C.c_uint64),
                 ("band_entries", C.c_uint64), ("upload_bytes", C.c_uint64), ("flushes", C.c_uint64),
                 ("host_build_ms", C.c_double), ("host_wait_ms", C.c_double),
-                ("host_geom_ms", C.c_double)]
+                ("host_geom_ms", C.c_double), ("validated_flushes", C.c_uint64)]
 
 
 class KernelStat(C.Structure):

@@ -103,7 +103,7 @@ struct HairJob {
 constexpr int kMaxStops = 16;
 // The scalar part of a shader, read by the compositor in one piece.
 struct ShaderHead {
-    uint32_t kind;       // 0 none, 1 solid, 2 gradient
+    uint32_t kind;       // 0 none, 1 solid, 2 gradient, 3 pattern
     uint32_t radial, spread, pad_clamp, two_stop, opaque, r_twist, nstops;
     uint32_t rgba;       // solid premultiplied colour
     // Opaque shader (solid alpha 1, linear gradient with opaque stops): tiny-skia
@@ -120,7 +120,18 @@ struct ShaderHead {
     // the coverage scale and SourceOver stay in f32 until the store
     uint32_t focal;
     float p0;
-    uint32_t pad_;
+    // 1: the shader runs in tiny-skia's f32 pipeline (a focal gradient or a pattern)
+    uint32_t f32_pipeline;
+    // pattern (tiny-skia's Pattern shader: a pattern paint as path.rs:58-68 sets it
+    // up, spread Repeat; a raster image as image.rs:185-191 does, spread Pad): the
+    // pixmap's device address and size, 1 / size, nearest (0), bilinear (1) or
+    // bicubic (2) sampling, and the opacity applied by scale_1_float; sx..ty map
+    // device pixels to pixmap pixels; `spread` holds the spread mode
+    uint64_t tile;
+    uint32_t tile_w, tile_h;
+    float tile_iw, tile_ih;
+    uint32_t quality;
+    float pattern_opacity;
 };
 constexpr uint32_t kFocalOnCircle = 1, kFocalWellBehaved = 2, kFocalGreater = 3;
 struct ShaderParams : ShaderHead {
@@ -200,6 +211,22 @@ void hairline_lines(const rvk_scene &sc, const rvk_path &path, rvkh::Ts ts, uint
 // scales the colour's or every stop's alpha (Shader::apply_opacity).
 bool make_shader(const rvk_scene &sc, const rvk_paint &paint, rvkh::Ts ts, ShaderParams &out,
                  float extra_opacity = 1.0f);
+// render_pattern_pixmap's tile for a pattern paint drawn with `ts` (path.rs:179-205):
+// the scale (sx, sy) of ts * Pattern::transform and the size round(rect * scale);
+// false when the size is empty or too large for a pixmap.
+bool pattern_tile_size(const rvk_paint &paint, rvkh::Ts ts, float &sx, float &sy, uint32_t &w, uint32_t &h);
+// tiny-skia's Pattern shader over a tile rendered at scale (sx, sy) and stored at
+// device address `tile` (Pattern::new, Shader::transform, Pattern::push_stages);
+// false when the transform cannot be inverted.
+bool make_pattern_shader(const rvk_paint &paint, rvkh::Ts ts, float sx, float sy, const void *tile, uint32_t w,
+                         uint32_t h, ShaderParams &out, float extra_opacity = 1.0f);
+// Device-space segments tiny-skia can fill: every point finite (Path::transform's
+// Rect::from_points) and within f32::MAX / 4 (is_too_big_for_math).
+bool segs_drawable(const std::vector<Seg> &segs);
+// image::render_raster's Pattern shader (Pad, the image's filter quality, opacity 1,
+// identity pattern transform) for the image drawn with `ts`, its pixmap at device
+// address `tile`; false when `ts` cannot be inverted.
+bool make_image_shader(const rvk_image &image, rvkh::Ts ts, const void *tile, ShaderParams &out);
 // Control-point bounds of device-space segments nearly zero (tiny-skia skips such fills).
 bool bounds_nearly_zero(const std::vector<Seg> &segs);
 

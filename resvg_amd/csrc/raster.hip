@@ -235,6 +235,7 @@ __device__ __forceinline__ void bitonic_from(uint64_t (&v)[S][K], int lane) {
 template <int K, int S, int MK>
 __device__ __forceinline__ void spans_of(uint64_t (*keys)[MK], const int *n, bool evenodd, int *nev, int lane,
                                          const int *base, int32_t c0, int32_t c1) {
+    static_assert(K * 64 <= MK, "each lane's K keys stay inside its sub-scanline's key row");
     uint64_t v[S][K];
 #pragma unroll
     for (int s = 0; s < S; s++)
@@ -773,17 +774,111 @@ __device__ __forceinline__ void shade(const ShaderParams &sp, const ShaderHead &
     if (!s.opaque) for (int k = 0; k < 3; k++) o[k] = div255(o[k] * o[3]);
 }
 
-// A covered pixel under a two-point conical shader, in tiny-skia's f32 pipeline:
-// gradient, premultiply, apply_vector_mask, scale_1_float by c / 255, SourceOver onto
-// d, store (kept out of line: the compositor's common paths do not carry it)
+// One texel of a pattern pixmap at pixmap coordinates (u, v): Repeat tiling (none for
+// Pad), then gather_ix (clamped to the last texel) and load_8888 (v * (1/255)).
+__device__ __forceinline__ F4 tile_texel(const ShaderHead &s, const uint32_t *tile, float u, float v) {
+    const float fw = (float)s.tile_w, fh = (float)s.tile_h;
+    if (s.spread == RVK_SPREAD_REPEAT) {
+        u = u - floorf(u * s.tile_iw) * fw;
+        v = v - floorf(v * s.tile_ih) * fh;
+    }
+    // the largest float below the size (ulp_sub)
+    u = fminf(fmaxf(u, 0.0f), __uint_as_float(__float_as_uint(fw) - 1u));
+    v = fminf(fmaxf(v, 0.0f), __uint_as_float(__float_as_uint(fh) - 1u));
+    const uint32_t p = tile[(uint32_t)(int32_t)v * s.tile_w + (uint32_t)(int32_t)u];
+    const float k = 1.0f / 255.0f;
+    return F4{(float)(p & 0xffu) * k, (float)((p >> 8) & 0xffu) * k, (float)((p >> 16) & 0xffu) * k,
+              (float)(p >> 24) * k};
+}
+
+// The premultiplied colour of a pattern shader at pixel (px, py) (tiny-skia's f32
+// seed_shader and transform, then repeat (Repeat spread) + gather for nearest
+// sampling, bilinear, or bicubic + clamp_0 + clamp_a, then scale_1_float by the
+// opacity).
+__device__ F4 pattern_color(const ShaderHead &s, int32_t px, int32_t py) {
+    const float x = (float)px + 0.5f, y = (float)py + 0.5f;
+    const float tx = x * s.sx + (y * s.kx + s.tx);
+    const float ty = x * s.ky + (y * s.sy + s.ty);
+    const uint32_t *tile = (const uint32_t *)s.tile;
+    F4 o;
+    if (s.quality == 0) {
+        o = tile_texel(s, tile, tx, ty);
+    } else if (s.quality == 1) {
+        // bilinear: weights 1 - f, f of the fractions of x + 0.5 and y + 0.5 over the
+        // 2 x 2 texels from (x - 0.5, y - 0.5); no clamp stages follow
+        const float fx = (tx + 0.5f) - floorf(tx + 0.5f), fy = (ty + 0.5f) - floorf(ty + 0.5f);
+        o = F4{0.0f, 0.0f, 0.0f, 0.0f};
+        float v = ty - 0.5f;
+#pragma unroll 1
+        for (int j = 0; j < 2; j++) {
+            float u = tx - 0.5f;
+            const float wyj = j == 0 ? 1.0f - fy : fy;
+#pragma unroll 1
+            for (int i = 0; i < 2; i++) {
+                const F4 c = tile_texel(s, tile, u, v);
+                const float wgt = (i == 0 ? 1.0f - fx : fx) * wyj;
+                o.r = wgt * c.r + o.r; o.g = wgt * c.g + o.g; o.b = wgt * c.b + o.b; o.a = wgt * c.a + o.a;
+                u = u + 1.0f;
+            }
+            v = v + 1.0f;
+        }
+    } else {
+        // weights far(1 - f), near(1 - f), near(f), far(f) of the fractions of x + 0.5
+        // and y + 0.5: near(t) = ((-21/18 t + 27/18) t + 9/18) t + 1/18, far(t) =
+        // t^2 (7/18 t - 6/18); texels from (x - 1.5, y - 1.5) in steps of 1
+        auto near_w = [](float t) {
+            return t * (t * ((-21.0f / 18.0f) * t + 27.0f / 18.0f) + 9.0f / 18.0f) + 1.0f / 18.0f;
+        };
+        auto far_w = [](float t) { return (t * t) * ((7.0f / 18.0f) * t + -6.0f / 18.0f); };
+        const float fx = (tx + 0.5f) - floorf(tx + 0.5f), fy = (ty + 0.5f) - floorf(ty + 0.5f);
+        auto weight = [&](int k, float f) {
+            return k == 0 ? far_w(1.0f - f) : k == 1 ? near_w(1.0f - f) : k == 2 ? near_w(f) : far_w(f);
+        };
+        o = F4{0.0f, 0.0f, 0.0f, 0.0f};
+        float v = ty - 1.5f;
+        // rolled loops with the weights formed in place keep the compositor's register
+        // budget (the kernel is sized for the deepest call it makes)
+#pragma unroll 1
+        for (int j = 0; j < 4; j++) {
+            float u = tx - 1.5f;
+            const float wyj = weight(j, fy);
+#pragma unroll 1
+            for (int i = 0; i < 4; i++) {
+                const F4 c = tile_texel(s, tile, u, v);
+                const float wgt = weight(i, fx) * wyj;
+                o.r = wgt * c.r + o.r; o.g = wgt * c.g + o.g; o.b = wgt * c.b + o.b; o.a = wgt * c.a + o.a;
+                u = u + 1.0f;
+            }
+            v = v + 1.0f;
+        }
+        o.r = fmaxf(o.r, 0.0f); o.g = fmaxf(o.g, 0.0f); o.b = fmaxf(o.b, 0.0f); o.a = fmaxf(o.a, 0.0f);
+        o.a = fminf(o.a, 1.0f);
+        o.r = fminf(o.r, o.a); o.g = fminf(o.g, o.a); o.b = fminf(o.b, o.a);
+    }
+    if (s.pattern_opacity != 1.0f) {
+        o.r *= s.pattern_opacity; o.g *= s.pattern_opacity; o.b *= s.pattern_opacity; o.a *= s.pattern_opacity;
+    }
+    return o;
+}
+
+// A covered pixel under a shader of tiny-skia's f32 pipeline (a two-point conical
+// gradient: gradient, premultiply, apply_vector_mask; or a pattern), then
+// scale_1_float by c / 255, SourceOver onto d, store (kept out of line: the
+// compositor's common paths do not carry it)
+template <bool kPatterns>
 __device__ __noinline__ uint32_t paint_px_hp(const ShaderParams &sp, const ShaderHead &s, int32_t px, int32_t py,
                                              uint32_t c, uint32_t d) {
-    bool degenerate;
-    const float t = grad_t_focal(s, px, py, degenerate);
-    float col[4];
-    grad_color(sp, s, t, col);
-    if (!s.opaque) { col[0] *= col[3]; col[1] *= col[3]; col[2] *= col[3]; }
-    F4 src = degenerate ? F4{0.0f, 0.0f, 0.0f, 0.0f} : F4{col[0], col[1], col[2], col[3]};
+    F4 src;
+    if (kPatterns && s.kind == 3) {
+        src = pattern_color(s, px, py);
+    } else {
+        bool degenerate;
+        const float t = grad_t_focal(s, px, py, degenerate);
+        float col[4];
+        grad_color(sp, s, t, col);
+        if (!s.opaque) { col[0] *= col[3]; col[1] *= col[3]; col[2] *= col[3]; }
+        src = degenerate ? F4{0.0f, 0.0f, 0.0f, 0.0f} : F4{col[0], col[1], col[2], col[3]};
+    }
     if (c != 255u) {
         const float cf = (float)c * (1.0f / 255.0f);
         src.r *= cf; src.g *= cf; src.b *= cf; src.a *= cf;
@@ -810,8 +905,10 @@ __device__ __forceinline__ uint32_t over_lowp(const uint32_t src_in[4], uint32_t
 }
 
 // one covered pixel of a draw item: Clear, or the shaded colour composited with
-// over_lowp, or for a two-point conical shader in f32 (scale_1_float by c / 255,
-// SourceOver, store); the colour is shaded only when needed
+// over_lowp, or for a shader of the f32 pipeline (two-point conical gradients,
+// patterns) in f32 (scale_1_float by c / 255, SourceOver, store); the colour is
+// shaded only when needed
+template <bool kPatterns>
 __device__ __forceinline__ uint32_t paint_px(const ShaderParams &sp, const ShaderHead &sh, int32_t lx, int32_t ly,
                                              uint32_t c, uint32_t d, uint32_t src[4], bool &shaded) {
     if (sh.clear) {
@@ -820,7 +917,7 @@ __device__ __forceinline__ uint32_t paint_px(const ShaderParams &sp, const Shade
         for (int k = 0; k < 4; k++) o |= div255(((d >> (8 * k)) & 0xffu) * ic) << (8 * k);
         return o;
     }
-    if (sh.focal) return paint_px_hp(sp, sh, lx, ly, c, d);
+    if (sh.f32_pipeline) return paint_px_hp<kPatterns>(sp, sh, lx, ly, c, d);
     if (!shaded) {
         shade(sp, sh, lx, ly, src);
         shaded = true;
@@ -893,6 +990,10 @@ static_assert(kWaveW * kWaveH == 64 && (TX / kWaveW) * (TY / kWaveH) == 4, "four
 // the last whole round of runs keep their own index.
 constexpr uint32_t kXcds = 8, kChunk = 4;
 
+// kPatterns: the draw list holds pattern shaders.  The pattern sampler makes the
+// out-of-line f32 path (and with it the whole kernel, which is sized for its deepest
+// call) need more registers, so lists without patterns use the variant without it.
+template <bool kPatterns>
 __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t h, int32_t otx, int32_t oty,
                                               uint32_t blocks_x, DrawArgs a) {
     const uint32_t b = blockIdx.x;
@@ -955,12 +1056,12 @@ __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t
                         // every blit of the hairline at this pixel, composited in order
                         for (uint32_t kk = hb; kk < he; kk++) {
                             const uint32_t c = hair_alpha(a.hair_lines[a.hair_bins[kk]], lx, ly);
-                            if (c) target = paint_px(sp, sh, lx, ly, c, target, src, shaded);
+                            if (c) target = paint_px<kPatterns>(sp, sh, lx, ly, c, target, src, shaded);
                         }
                         continue;
                     }
                     const uint32_t c = stroke ? cs : cf;
-                    if (c) target = paint_px(sp, sh, lx, ly, c, target, src, shaded);
+                    if (c) target = paint_px<kPatterns>(sp, sh, lx, ly, c, target, src, shaded);
                 }
                 if (!item.layer) {
                     d = target;
@@ -1017,7 +1118,7 @@ rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint
                              int32_t x1, int32_t y1, const DrawItem *items, const DrawCull *cull, uint32_t nitems,
                              const uint16_t *table, const uint8_t *arena, const ShaderParams *shaders,
                              const uint32_t *hair_tiles, const uint32_t *hair_bins, const HairLine *hair_lines,
-                             double algo_bytes) {
+                             bool patterns, double algo_bytes) {
     if (nitems == 0 || x1 <= x0 || y1 <= y0 || x0 < 0 || y0 < 0) return RVK_OK;
     // the draw rect in whole coverage tiles (the tile grids are aligned to the target)
     const int32_t otx = x0 / rvkr::kTile, oty = y0 / rvkr::kTile;
@@ -1026,8 +1127,12 @@ rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint
     const uint32_t blocks_x = div_up(ntx, (uint32_t)kWavesX), blocks_y = div_up(nty, (uint32_t)(TY / kWaveH));
     const DrawArgs a{items, cull, nitems, table, arena, shaders, hair_tiles, hair_bins, hair_lines};
     timed_launch(ctx, "draw_items", algo_bytes, [&] {
-        hipLaunchKernelGGL(k_draw, dim3(blocks_x * blocks_y), dim3(256), 0, ctx->stream, (uint32_t *)target, w, h,
-                           otx, oty, blocks_x, a);
+        if (patterns)
+            hipLaunchKernelGGL(k_draw<true>, dim3(blocks_x * blocks_y), dim3(256), 0, ctx->stream, (uint32_t *)target,
+                               w, h, otx, oty, blocks_x, a);
+        else
+            hipLaunchKernelGGL(k_draw<false>, dim3(blocks_x * blocks_y), dim3(256), 0, ctx->stream, (uint32_t *)target,
+                               w, h, otx, oty, blocks_x, a);
     });
     RVK_HIP_CHECK(hipGetLastError());
     return RVK_OK;

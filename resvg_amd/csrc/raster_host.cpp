@@ -71,6 +71,16 @@ void fill_segments(const rvk_scene &sc, const rvk_path &path, Ts ts, std::vector
     close();
 }
 
+bool segs_drawable(const std::vector<Seg> &segs) {
+    // Rect::from_points (non-finite -> None, so Path::transform fails) and
+    // is_too_big_for_math (bounds beyond f32::MAX / 4)
+    const float lim = 3.40282347e+38f * 0.25f;
+    for (const Seg &s : segs)
+        for (int i = 0; i < s.n; i++)
+            if (!(s.p[i].x >= -lim && s.p[i].x <= lim && s.p[i].y >= -lim && s.p[i].y <= lim)) return false;
+    return !segs.empty();
+}
+
 bool bounds_nearly_zero(const std::vector<Seg> &segs) {
     float mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
     for (const Seg &s : segs)
@@ -92,10 +102,12 @@ inline int32_t fdot6_to_fixed(int32_t x) { return (int32_t)((uint32_t)x << 10); 
 inline int32_t fixed_mul(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)b) >> 16); }
 inline int32_t fdot6_div(int32_t a, int32_t b) {
     if ((int32_t)(int16_t)a == a) return (int32_t)((uint32_t)a << 16) / b;
-    int64_t v = ((int64_t)a << 16) / (int64_t)b;
+    int64_t v = (int64_t)a * 65536 / (int64_t)b;
     return (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, v));
 }
-inline int32_t compute_dy(int32_t top, int32_t y0) { return ((top << 6) + 32) - y0; }
+inline int32_t compute_dy(int32_t top, int32_t y0) {  // ((top << 6) + 32) - y0, wrapping
+    return (int32_t)(((uint32_t)top << 6) + 32u - (uint32_t)y0);
+}
 inline int32_t diff_to_shift(int32_t dx, int32_t dy, int32_t shift_aa) {
     dx = std::abs(dx);
     dy = std::abs(dy);
@@ -660,6 +672,64 @@ void set_solid(ShaderParams &s, const float c[4]) {
 
 }  // namespace
 
+bool pattern_tile_size(const rvk_paint &p, Ts ts, float &sx, float &sy, uint32_t &w, uint32_t &h) {
+    ts_get_scale(ts_concat(ts, ts_from(p.transform)), sx, sy);
+    auto to_u32 = [](float v) -> uint32_t {  // `as u32`: saturating, NaN to 0
+        if (!(v > 0.0f)) return 0u;
+        return v >= 4294967296.0f ? UINT32_MAX : (uint32_t)v;
+    };
+    w = to_u32(std::round(p.rect.w * sx));
+    h = to_u32(std::round(p.rect.h * sy));
+    if (w == 0 || h == 0) return false;  // IntSize::from_wh
+    // Pixmap::new: the row bytes and the whole buffer stay within i32
+    return (uint64_t)w * 4 <= (uint64_t)INT32_MAX && (uint64_t)w * h * 4 <= (1ull << 31);
+}
+
+bool make_pattern_shader(const rvk_paint &p, Ts ts, float sx, float sy, const void *tile, uint32_t w, uint32_t h,
+                         ShaderParams &s, float extra_opacity) {
+    s = ShaderParams{};
+    // pattern.transform * translate(rect.xy) * scale(1 / sx, 1 / sy), post-concatenated
+    // with the path transform (Shader::transform), inverted by push_stages
+    const Ts pts = ts_concat(ts_concat(ts_from(p.transform), Ts{1.0f, 0.0f, 0.0f, 1.0f, p.rect.x, p.rect.y}),
+                             Ts{1.0f / sx, 0.0f, 0.0f, 1.0f / sy, 0.0f, 0.0f});
+    Ts inv;
+    if (!invert(ts_concat(ts, pts), inv)) return false;
+    s.kind = 3;
+    s.f32_pipeline = 1;
+    s.sx = inv.sx; s.ky = inv.ky; s.kx = inv.kx; s.sy = inv.sy; s.tx = inv.tx; s.ty = inv.ty;
+    // FilterQuality::Bicubic, lowered to Nearest when the transform is a translation
+    s.quality = (inv.sx == 1.0f && inv.sy == 1.0f && inv.kx == 0.0f && inv.ky == 0.0f) ? 0u : 2u;
+    s.spread = RVK_SPREAD_REPEAT;
+    s.tile = (uint64_t)(uintptr_t)tile;
+    s.tile_w = w;
+    s.tile_h = h;
+    s.tile_iw = 1.0f / (float)w;
+    s.tile_ih = 1.0f / (float)h;
+    // NormalizedF32::new_clamped(opacity), then Shader::apply_opacity
+    const float o = std::min(std::max(p.opacity, 0.0f), 1.0f);
+    s.pattern_opacity = std::min(std::max(o * std::min(std::max(extra_opacity, 0.0f), 1.0f), 0.0f), 1.0f);
+    return true;
+}
+
+bool make_image_shader(const rvk_image &im, Ts ts, const void *tile, ShaderParams &s) {
+    s = ShaderParams{};
+    Ts inv;  // Pattern transform identity, post-concatenated with `ts`, inverted
+    if (!invert(ts, inv)) return false;
+    s.kind = 3;
+    s.f32_pipeline = 1;
+    s.sx = inv.sx; s.ky = inv.ky; s.kx = inv.kx; s.sy = inv.sy; s.tx = inv.tx; s.ty = inv.ty;
+    const bool translate = inv.sx == 1.0f && inv.sy == 1.0f && inv.kx == 0.0f && inv.ky == 0.0f;
+    s.quality = translate ? 0u : std::min(im.quality, 2u);
+    s.spread = RVK_SPREAD_PAD;
+    s.tile = (uint64_t)(uintptr_t)tile;
+    s.tile_w = im.width;
+    s.tile_h = im.height;
+    s.tile_iw = 1.0f / (float)im.width;
+    s.tile_ih = 1.0f / (float)im.height;
+    s.pattern_opacity = 1.0f;
+    return true;
+}
+
 // ts_from_poly_to_poly(c0, c1, (0, 0), (1, 0)) (tiny-skia radial_gradient.rs): the
 // transform that maps c0 to the origin and c1 to (1, 0); false when c0 and c1 coincide.
 static bool poly_to_unit_x(float x0, float y0, float x1, float y1, Ts &out) {
@@ -736,9 +806,11 @@ bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s
         const float r1 = p.r / dc;
         if (std::fabs(1.0f - r1) <= 1.0f / 4096.0f) {
             s.focal = kFocalOnCircle;
+            s.f32_pipeline = 1;
             unit = ts_concat(Ts{0.5f, 0, 0, 0.5f, 0, 0}, unit0);
         } else {
             s.focal = r1 > 1.0f ? kFocalWellBehaved : kFocalGreater;
+            s.f32_pipeline = 1;
             unit = ts_concat(Ts{r1 / (r1 * r1 - 1.0f), 0, 0, 1.0f / std::sqrt(std::fabs(r1 * r1 - 1.0f)), 0, 0}, unit0);
         }
         s.p0 = 1.0f / r1;
@@ -843,8 +915,9 @@ void prep_job(const EdgeSet &es, uint32_t rule, bool aa, IRect clip, int32_t lx,
         ymin = std::min(ymin, e.first_y);
         ymax = std::max(ymax, ylast[i]);
     }
-    int64_t x0 = std::max<int64_t>((int64_t)std::floor(es.mnx) - 1, clip.x);
-    int64_t x1 = std::min<int64_t>((int64_t)std::ceil(es.mxx) + 2, (int64_t)clip.x + clip.w);
+    // (clamped in double: the control points may lie far outside the pixmap)
+    int64_t x0 = (int64_t)std::max<double>(std::floor((double)es.mnx) - 1.0, (double)clip.x);
+    int64_t x1 = (int64_t)std::min<double>(std::ceil((double)es.mxx) + 2.0, (double)clip.x + clip.w);
     int64_t y0 = std::max<int64_t>(ymin >> shift, clip.y);
     int64_t y1 = std::min<int64_t>((ymax >> shift) + 1, (int64_t)clip.y + clip.h);
     if (x1 <= x0 || y1 <= y0) return;

@@ -42,7 +42,7 @@ rvk_status launch_draw_items(rvk_context *ctx, uint8_t *target, uint32_t w, uint
                              int32_t x1, int32_t y1, const DrawItem *items, const DrawCull *cull, uint32_t nitems,
                              const uint16_t *table, const uint8_t *arena, const ShaderParams *shaders,
                              const uint32_t *hair_tiles, const uint32_t *hair_bins, const HairLine *hair_lines,
-                             double algo_bytes);
+                             bool patterns, double algo_bytes);
 }
 
 namespace rvk {
@@ -122,12 +122,11 @@ struct PathGeom {
 void compute_geom(const rvk_scene &sc, const rvk_path &p, Ts ts, uint32_t pw, uint32_t ph, bool fill_only,
                   IRect clip, int32_t lx, int32_t ly, PathGeom &g) {
     auto finish = [&](std::vector<Seg> &segs, uint32_t rule, JobPrep &out) {
-        if (segs.empty() || bounds_nearly_zero(segs)) return;
+        if (!segs_drawable(segs) || bounds_nearly_zero(segs)) return;
         EdgeSet es;
         float mnx = INFINITY, mxx = -INFINITY;
         for (const Seg &s : segs)
             for (int i = 0; i < s.n; i++) { mnx = std::min(mnx, s.p[i].x); mxx = std::max(mxx, s.p[i].x); }
-        if (!std::isfinite(mnx) || !std::isfinite(mxx)) return;
         if (segs_inside_pixmap(segs, p.anti_alias != 0, pw, ph)) {
             build_edges(segs, p.anti_alias ? 2 : 0, es.edges, es.caps);
         } else {
@@ -351,7 +350,7 @@ struct Renderer {
         if (!p.visible) return;
         ShaderParams sp;
         // path::fill_path (path.rs:26-75)
-        if (g.fill.ok && make_shader(S, p.fill, ts, sp)) {
+        if (g.fill.ok && paint_shader(S, p.fill, ts, sp)) {
             it.fill_job = add_job(dl, g.fill);
             if (it.fill_job >= 0) {
                 sp.clear = clear ? 1 : 0;
@@ -363,7 +362,7 @@ struct Renderer {
         // path::stroke_path (path.rs:77-116): a hairline, or tiny-skia's stroker outline
         // filled nonzero
         if (g.hair) {
-            if (make_shader(S, p.stroke, ts, sp, g.hair_opacity)) {
+            if (paint_shader(S, p.stroke, ts, sp, g.hair_opacity)) {
                 it.stroke_job = add_hair_job(dl, g.hair_job);
                 if (it.stroke_job >= 0) {
                     it.stroke_hair = 1;
@@ -371,13 +370,67 @@ struct Renderer {
                     dl.shaders.push_back(sp);
                 }
             }
-        } else if (g.stroke.ok && make_shader(S, p.stroke, ts, sp)) {
+        } else if (g.stroke.ok && paint_shader(S, p.stroke, ts, sp)) {
             it.stroke_job = add_job(dl, g.stroke);
             if (it.stroke_job >= 0) {
                 it.stroke_shader = (int32_t)dl.shaders.size();
                 dl.shaders.push_back(sp);
             }
         }
+    }
+
+    // The shader of a paint; a pattern first has its tile rendered
+    // (render_pattern_pixmap, path.rs:179-205) by a walk of its own over the pattern's
+    // group with Transform::from_scale(sx, sy) and this walk's max_bbox, into a
+    // transparent device pixmap that lives until the end of this render.
+    std::vector<void *> pattern_tiles;
+    static thread_local inline int pattern_depth = 0;  // pattern renders in progress on this thread
+
+    // A raster image's pixmap on the device, uploaded once per render (the fill of its
+    // aux rect path carries kPaintImage and the image index).
+    static constexpr uint32_t kPaintImage = 0x100;
+    std::vector<void *> image_tiles;
+
+    bool image_shader(uint32_t index, Ts ts, ShaderParams &sp) {
+        if (index >= sc.image_count) return false;
+        const rvk_image &im = sc.images[index];
+        if (!image_tiles[index]) {
+            const size_t bytes = (size_t)im.width * im.height * 4;
+            void *tile = nullptr;
+            if (rvk::scratch_alloc(ctx, bytes, &tile) != RVK_OK) return false;
+            pattern_tiles.push_back(tile);
+            const rvk_status st = rvk::upload_async(ctx, tile, im.pixels, bytes);
+            if (st != RVK_OK) {
+                fail(st);
+                return false;
+            }
+            image_tiles[index] = tile;
+        }
+        return make_image_shader(im, ts, image_tiles[index], sp);
+    }
+
+    bool paint_shader(const rvk_scene &S, const rvk_paint &paint, Ts ts, ShaderParams &sp, float extra = 1.0f) {
+        if (paint.kind == kPaintImage) return image_shader(paint.root, ts, sp);
+        if (paint.kind != RVK_PAINT_PATTERN) return make_shader(S, paint, ts, sp, extra);
+        float sx, sy;
+        uint32_t w, h;
+        if (paint.root >= S.group_count || pattern_depth >= kMaxNesting || !pattern_tile_size(paint, ts, sx, sy, w, h))
+            return false;
+        void *tile = nullptr;
+        if (rvk::scratch_alloc(ctx, (size_t)w * h * 4, &tile) != RVK_OK) return false;  // Pixmap::new failure
+        pattern_tiles.push_back(tile);
+        rvk_status st = rvk::launch_fill_sub(ctx, (uint8_t *)tile, w, h, 0, 0, w, h, 0u);
+        if (st == RVK_OK) {
+            pattern_depth++;
+            st = rvk::render_walk(ctx, &S, Ts{sx, 0.0f, 0.0f, sy, 0.0f, 0.0f}, w, h, (uint8_t *)tile, nullptr,
+                                  rvk::Walk::kChildren, rvk_node_ref{RVK_NODE_GROUP, paint.root}, &max_bbox);
+            pattern_depth--;
+        }
+        if (st != RVK_OK) {
+            if (err == RVK_OK) err = st;
+            return false;
+        }
+        return make_pattern_shader(paint, ts, sx, sy, tile, w, h, sp, extra);
     }
 
     void push_item(DrawList &dl, DrawItem &it) {
@@ -427,6 +480,109 @@ struct Renderer {
         dl.bx1 = std::max(dl.bx1, it.x1); dl.by1 = std::max(dl.by1, it.y1);
         dl.items.push_back(it);
         ctx->stats.paths++;
+    }
+
+    // Checks every index the device will read from a packed flush block: job edge,
+    // window, table and arena ranges; each edge's piece slots; the band lists; the
+    // items' jobs, shaders and job references; the hairline tiles and bins; pattern
+    // tiles.  Host-side and off unless RVK_VALIDATE is set; returns the first
+    // violation, or an empty string.
+    static bool validating() {
+        const char *e = std::getenv("RVK_VALIDATE");  // read per flush: tests switch it on and off
+        return e && *e && *e != '0';
+    }
+    std::string validate_block(const DrawList &dl, const uint8_t *host, size_t o_jobs, size_t o_edges, size_t o_ecov,
+                               size_t o_win, const size_t o_grp[3], size_t o_ht, size_t o_hb) const {
+        char buf[256];
+        auto bad = [&](const char *what, uint64_t i, uint64_t v, uint64_t lim) {
+            std::snprintf(buf, sizeof buf, "%s %llu: %llu not below %llu", what, (unsigned long long)i,
+                          (unsigned long long)v, (unsigned long long)lim);
+            return std::string(buf);
+        };
+        const size_t njobs = dl.jobs.size(), nhair = dl.hair.size();
+        const CovJob *jobs = (const CovJob *)(host + o_jobs);
+        const EdgeRec *er = (const EdgeRec *)(host + o_edges);
+        const EdgeCov *ec = (const EdgeCov *)(host + o_ecov);
+        const uint2 *win = (const uint2 *)(host + o_win);
+        for (size_t i = 0; i < njobs; i++) {
+            const CovJob &j = jobs[i];
+            const uint64_t tiles = (uint64_t)j.tiles_x * j.tiles_y;
+            if ((uint64_t)j.edge_offset + j.edge_count > dl.n_edges)
+                return bad("job edge end", i, (uint64_t)j.edge_offset + j.edge_count, dl.n_edges + 1);
+            if ((uint64_t)j.window_offset + j.tiles_y > dl.n_windows)
+                return bad("job window end", i, (uint64_t)j.window_offset + j.tiles_y, dl.n_windows + 1);
+            if ((uint64_t)j.table_offset + tiles > dl.n_table)
+                return bad("job table end", i, (uint64_t)j.table_offset + tiles, dl.n_table + 1);
+            if (j.cov_offset + tiles * 64 > dl.arena) return bad("job arena end", i, j.cov_offset + tiles * 64, dl.arena + 1);
+            if (j.tiles_x > kMaxTilesX) return bad("job tiles_x", i, j.tiles_x, kMaxTilesX + 1);
+            for (uint32_t g = 0; g < j.tiles_y; g++) {
+                const uint2 w = win[j.window_offset + g];
+                if (w.x > w.y || w.y > j.edge_count) return bad("job window", i, w.y, j.edge_count + 1);
+            }
+            // piece slots: increasing offsets inside the job's range
+            const uint64_t base = dl.jobs[i].piece_base, end = base + dl.jobs[i].pieces;
+            if (end > dl.piece_slots) return bad("job piece end", i, end, dl.piece_slots + 1);
+            uint64_t prev = base;
+            for (uint32_t k = 0; k < j.edge_count; k++) {
+                const EdgeRec &e = er[j.edge_offset + k];
+                const EdgeCov &c = ec[j.edge_offset + k];
+                if (e.piece_offset < prev || e.piece_offset > end) return bad("edge piece offset", k, e.piece_offset, end + 1);
+                if (c.piece_offset != e.piece_offset) return bad("edge piece mirror", k, c.piece_offset, e.piece_offset + 1);
+                if ((uint32_t)c.id >= j.edge_count) return bad("edge id", k, (uint32_t)c.id, j.edge_count);
+                prev = e.piece_offset;
+            }
+        }
+        for (int t = 0; t < 3; t++) {
+            const uint2 *b = (const uint2 *)(host + o_grp[t]);
+            for (uint64_t k = 0; k < dl.n_groups[t]; k++) {
+                if (b[k].x >= njobs) return bad("band job", k, b[k].x, njobs);
+                if (b[k].y >= jobs[b[k].x].tiles_y) return bad("band index", k, b[k].y, jobs[b[k].x].tiles_y);
+                if (dl.jobs[b[k].x].tier[b[k].y] != t) return bad("band tier", k, dl.jobs[b[k].x].tier[b[k].y], t);
+            }
+        }
+        const uint32_t *ht = (const uint32_t *)(host + o_ht);
+        const uint32_t *hb = (const uint32_t *)(host + o_hb);
+        for (uint64_t k = 0; k < dl.n_hair_tiles; k++)
+            if (ht[k] > dl.n_hair_bins) return bad("hair tile", k, ht[k], dl.n_hair_bins + 1);
+        for (uint64_t k = 0; k < dl.n_hair_bins; k++)
+            if (hb[k] >= dl.n_hair_lines) return bad("hair bin", k, hb[k], dl.n_hair_lines);
+        for (size_t i = 0; i < dl.items.size(); i++) {
+            const DrawItem &it = dl.items[i];
+            const size_t ns = dl.shaders.size();
+            if (it.fill_job >= 0) {
+                if ((size_t)it.fill_job >= njobs) return bad("item fill job", i, (uint64_t)it.fill_job, njobs);
+                if (it.fill_shader < 0 || (size_t)it.fill_shader >= ns) return bad("item fill shader", i, (uint64_t)it.fill_shader, ns);
+                const CovJob &j = jobs[it.fill_job];
+                if (it.fj.table != j.table_offset || it.fj.cov_offset != j.cov_offset || it.fj.w != j.tiles_x ||
+                    it.fj.h != j.tiles_y)
+                    return bad("item fill ref", i, it.fj.table, j.table_offset + 1);
+            } else if (it.fj.w != 0 || it.fj.h != 0) {
+                return bad("item empty fill grid", i, it.fj.w, 1);
+            }
+            if (it.stroke_job >= 0) {
+                if (it.stroke_shader < 0 || (size_t)it.stroke_shader >= ns)
+                    return bad("item stroke shader", i, (uint64_t)it.stroke_shader, ns);
+                if (it.stroke_hair) {
+                    if ((size_t)it.stroke_job >= nhair) return bad("item hair job", i, (uint64_t)it.stroke_job, nhair);
+                    const HairJob &h = dl.hair[it.stroke_job].j;
+                    const uint64_t tend = (uint64_t)h.tile_offset + (uint64_t)h.tiles_x * h.tiles_y + 1;
+                    if (tend > dl.n_hair_tiles) return bad("item hair tiles", i, tend, dl.n_hair_tiles + 1);
+                    if (it.sj.table != h.tile_offset) return bad("item hair ref", i, it.sj.table, h.tile_offset + 1);
+                } else {
+                    if ((size_t)it.stroke_job >= njobs) return bad("item stroke job", i, (uint64_t)it.stroke_job, njobs);
+                    const CovJob &j = jobs[it.stroke_job];
+                    if (it.sj.table != j.table_offset || it.sj.cov_offset != j.cov_offset || it.sj.w != j.tiles_x ||
+                        it.sj.h != j.tiles_y)
+                        return bad("item stroke ref", i, it.sj.table, j.table_offset + 1);
+                }
+            }
+        }
+        for (size_t k = 0; k < dl.shaders.size(); k++) {
+            const ShaderHead &s = dl.shaders[k];
+            if (s.kind == 3 && (s.tile == 0 || s.tile_w == 0 || s.tile_h == 0)) return bad("pattern tile", k, 0, 1);
+            if (s.kind == 2 && !s.two_stop && s.nstops > (uint32_t)kMaxStops + 2) return bad("stops", k, s.nstops, kMaxStops + 3);
+        }
+        return std::string();
     }
 
     rvk_status flush(DrawList &dl) {
@@ -531,6 +687,14 @@ struct Renderer {
         } else {
             for (size_t i = 0; i < njobs + nhair; i++) pack(i);
         }
+        if (validating()) {
+            const std::string why = validate_block(dl, host, o_jobs, o_edges, o_ecov, o_win, o_grp, o_ht, o_hb);
+            if (!why.empty()) {
+                std::fprintf(stderr, "rvk: flush block invalid: %s\n", why.c_str());
+                return RVK_ERR_INVALID_ARGUMENT;
+            }
+            ctx->stats.validated_flushes++;
+        }
         void *dev = nullptr;
         RVK_TRY(rvk::scratch_alloc(ctx, o_arena + dl.arena, &dev));
         uint8_t *d = (uint8_t *)dev;
@@ -568,7 +732,10 @@ struct Renderer {
                                         (const DrawItem *)(d + o_it), (const DrawCull *)(d + o_cull),
                                         (uint32_t)nitems, (const uint16_t *)(d + o_tab), d + o_arena,
                                         (const ShaderParams *)(d + o_sh), (const uint32_t *)(d + o_ht),
-                                        (const uint32_t *)(d + o_hb), (const HairLine *)(d + o_hl), draw_bytes);
+                                        (const uint32_t *)(d + o_hb), (const HairLine *)(d + o_hl),
+                                        std::any_of(dl.shaders.begin(), dl.shaders.end(),
+                                                    [](const ShaderParams &q) { return q.kind == 3; }),
+                                        draw_bytes);
         rvk::scratch_release(ctx, dev);
         ctx->stats.flushes++;
         dl = DrawList{dl.px, dl.w, dl.h};
@@ -619,6 +786,7 @@ struct Renderer {
         }
         for (uint32_t i = 0; i < parent.child_count && err == RVK_OK; i++) {
             const rvk_node_ref &n = sc.children[parent.child_offset + i];
+            if (n.kind == RVK_NODE_IMAGE) continue;  // not clip path content
             if (n.kind == RVK_NODE_PATH) {
                 const rvk_path &p = sc.paths[n.index];
                 if (!p.visible) continue;
@@ -785,16 +953,55 @@ struct Renderer {
             for (float v : {l, t, rr, t, rr, b, l, b}) aux.points.push_back(v);
             aux.paths.push_back(p);
         }
+        // every raster image's rect (0, 0, w, h) as a path filled with the image
+        // (render_raster's fill_rect), after the mask regions
+        for (uint32_t i = 0; i < sc.image_count; i++) {
+            const rvk_image &im = sc.images[i];
+            const float w = (float)im.width, h = (float)im.height;
+            rvk_path p{};
+            p.verb_offset = (uint32_t)aux.verbs.size();
+            p.verb_count = 5;
+            p.point_offset = (uint32_t)(aux.points.size() / 2);
+            p.visible = 1;
+            p.anti_alias = 1;
+            p.fill.kind = kPaintImage;
+            p.fill.root = i;
+            p.fill.opacity = 1.0f;
+            p.fill_rule = RVK_FILL_NONZERO;
+            p.stroke.kind = RVK_PAINT_NONE;
+            p.bounds = rvk_rect{0.0f, 0.0f, w, h};
+            for (uint8_t v : {RVK_VERB_MOVE, RVK_VERB_LINE, RVK_VERB_LINE, RVK_VERB_LINE, RVK_VERB_CLOSE})
+                aux.verbs.push_back(v);
+            for (float v : {0.0f, 0.0f, w, 0.0f, w, h, 0.0f, h}) aux.points.push_back(v);
+            aux.paths.push_back(p);
+        }
+        image_tiles.assign(sc.image_count, nullptr);
         aux.scene.verbs = aux.verbs.data();
         aux.scene.points = aux.points.data();
         aux.scene.paths = aux.paths.data();
         aux.scene.path_count = (uint32_t)aux.paths.size();
     }
 
+    // image::render (image.rs:4-35): a visible raster image is its rect filled with the
+    // image's pixmap
+    void render_image(uint32_t index, Ts ts, DrawList &dl) {
+        const rvk_image &im = sc.images[index];
+        if (!im.visible || im.width == 0 || im.height == 0 || !im.pixels) return;
+        if (dl.arena > budget) fail(flush(dl));
+        DrawItem it{};
+        it.layer = 0;
+        it.opacity = 1.0f;
+        it.blend_mode = RVK_BLEND_SOURCE_OVER;
+        path_jobs(dl, aux.paths[sc.mask_count + index], ts, dl.w, dl.h, IRect{0, 0, (int32_t)dl.w, (int32_t)dl.h}, it,
+                  true, false, &aux.scene);
+        push_item(dl, it);
+    }
+
     void render_nodes(const rvk_group &g, Ts ts, DrawList &dl) {
         for (uint32_t i = 0; i < g.child_count && err == RVK_OK; i++) {
             const rvk_node_ref &n = sc.children[g.child_offset + i];
             if (n.kind == RVK_NODE_GROUP) render_group(sc.groups[n.index], ts, dl);
+            else if (n.kind == RVK_NODE_IMAGE) render_image(n.index, ts, dl);
             else render_path(sc.paths[n.index], ts, dl);
         }
     }
@@ -920,6 +1127,7 @@ rvk_status render_walk(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t
             case Walk::kRoot: r.render_nodes(scene->groups[0], ts, dl); break;
             case Walk::kNode:  // render::render_node (render.rs:21-47)
                 if (node.kind == RVK_NODE_GROUP) r.render_group(scene->groups[node.index], ts, dl);
+                else if (node.kind == RVK_NODE_IMAGE) r.render_image(node.index, ts, dl);
                 else r.render_path(scene->paths[node.index], ts, dl);
                 break;
             case Walk::kClearFills: r.clip_children(scene->groups[0], true, ts, dl, 0); break;
@@ -953,6 +1161,7 @@ rvk_status render_walk(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     rvk::scratch_release(ctx, ovbuf);
+    for (void *t : r.pattern_tiles) rvk::scratch_release(ctx, t);
     r.overflow_rows = (uint64_t)(ov > 0 ? ov : 0);
     if (overflow_rows) *overflow_rows = r.overflow_rows;
     if (r.err == RVK_OK && r.overflow_rows) return RVK_ERR_UNSUPPORTED;
@@ -1180,7 +1389,8 @@ extern "C" {
 
 static bool scene_ok(const rvk_scene *s) {
     return s && s->group_count > 0 && s->groups && (s->path_count == 0 || (s->paths && s->verbs && s->points)) &&
-           (s->clip_path_count == 0 || s->clip_paths) && (s->mask_count == 0 || s->masks);
+           (s->clip_path_count == 0 || s->clip_paths) && (s->mask_count == 0 || s->masks) &&
+           (s->image_count == 0 || s->images);
 }
 
 rvk_status rvk_render(rvk_context *ctx, const rvk_scene *scene, rvk_transform transform, uint32_t width,
@@ -1223,7 +1433,8 @@ rvk_status rvk_render_node(rvk_context *ctx, const rvk_scene *scene, rvk_node_re
         return RVK_ERR_INVALID_ARGUMENT;
     if ((node.kind == RVK_NODE_GROUP && node.index >= scene->group_count) ||
         (node.kind == RVK_NODE_PATH && node.index >= scene->path_count) ||
-        (node.kind != RVK_NODE_GROUP && node.kind != RVK_NODE_PATH))
+        (node.kind == RVK_NODE_IMAGE && node.index >= scene->image_count) ||
+        (node.kind != RVK_NODE_GROUP && node.kind != RVK_NODE_PATH && node.kind != RVK_NODE_IMAGE))
         return RVK_ERR_INVALID_ARGUMENT;
     *rendered = 0;
     ctx->stats = rvk_stats{};

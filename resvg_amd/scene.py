@@ -43,6 +43,8 @@ class PaintSpec:
     fr: float = 0.0
     transform: tuple = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
     stops: list = field(default_factory=list)  # (offset, (r, g, b), opacity)
+    rect: tuple = (0.0, 0.0, 0.0, 0.0)  # pattern tile (x, y, w, h)
+    root: int = 0                        # pattern content group
 
     @staticmethod
     def solid(color, opacity=1.0) -> "PaintSpec":
@@ -60,6 +62,13 @@ class PaintSpec:
                          fy=cy if fy is None else fy, fr=fr, stops=list(stops), opacity=opacity, spread=spread,
                          transform=transform or (1.0, 0.0, 0.0, 1.0, 0.0, 0.0))
 
+    @staticmethod
+    def pattern(rect, root, opacity=1.0, transform=None) -> "PaintSpec":
+        """A usvg::Pattern with its units resolved: the tile rect in user space, the
+        content group (Scene.content_group, children in tile space) and patternTransform."""
+        return PaintSpec(kind=abi.PAINT_PATTERN, rect=tuple(f32(v) for v in rect), root=root, opacity=opacity,
+                         transform=transform or (1.0, 0.0, 0.0, 1.0, 0.0, 0.0))
+
 
 def paint_to_abi(ps: PaintSpec, stops: list) -> abi.Paint:
     """The rvk_paint of a PaintSpec; its stops are appended to `stops` (abi.Stop)."""
@@ -72,6 +81,8 @@ def paint_to_abi(ps: PaintSpec, stops: list) -> abi.Paint:
     P.x1, P.y1, P.x2, P.y2 = ps.x1, ps.y1, ps.x2, ps.y2
     P.cx, P.cy, P.r, P.fx, P.fy, P.fr = ps.cx, ps.cy, ps.r, ps.fx, ps.fy, ps.fr
     P.transform = abi.Transform(*ps.transform)
+    P.rect = abi.Rect(*ps.rect)
+    P.root = ps.root
     P.stop_offset = len(stops)
     P.stop_count = len(ps.stops)
     for off, col, op in ps.stops:
@@ -102,6 +113,7 @@ class Scene:
         self.dashes: list[float] = []
         self.clip_paths: list[dict] = []
         self.masks: list[dict] = []
+        self.images: list[dict] = []
         self.root = self._new_group(None, (1.0, 0.0, 0.0, 1.0, 0.0, 0.0), 1.0, abi.MIX["normal"], False, [])
         self._abi = None
 
@@ -181,6 +193,19 @@ class Scene:
         self._abi = None
         return pid
 
+    def image(self, parent: int, pixels, quality: int = abi.QUALITY_BICUBIC, visible: bool = True) -> int:
+        """A raster image node: `pixels` is the decoded image as tiny-skia holds it
+        (H x W x 4 premultiplied RGBA8), drawn over (0, 0, W, H) in the parent's space
+        (usvg puts the image's view box transform on a group of its own)."""
+        px = np.ascontiguousarray(pixels, dtype=np.uint8)
+        if px.ndim != 3 or px.shape[2] != 4:
+            raise ValueError("pixels must be H x W x 4 uint8")
+        iid = len(self.images)
+        self.images.append(dict(pixels=px, quality=int(quality), visible=bool(visible)))
+        self.groups[parent]["children"].append((abi.NODE_IMAGE, iid))
+        self._abi = None
+        return iid
+
     # ---- bounding boxes (usvg's role) ----
     def _path_bounds(self, p):
         pts = p["pts"]
@@ -227,6 +252,9 @@ class Scene:
         for kind, idx in g["children"]:
             if kind == abi.NODE_PATH:
                 cb = self._path_layer_bounds(self.paths[idx])
+            elif kind == abi.NODE_IMAGE:  # Image::bounding_box: the size at the origin
+                h, w = self.images[idx]["pixels"].shape[:2]
+                cb = (0.0, 0.0, float(w), float(h))
             else:
                 cb = self._group_layer_bounds(idx)
                 if cb is not None:
@@ -260,16 +288,20 @@ class Scene:
     def abs_layer_bbox(self, kind: int, index: int):
         """(x, y, w, h) of the node's layer bounding box in canvas coordinates, or None:
         a group's layer box mapped by its absolute transform (its ancestors' and its
-        own), a path's fill bounds mapped by its parent's (None for a zero width or
-        height, as NonZeroRect)."""
+        own), a path's fill bounds or an image's rect mapped by its parent's (None for a
+        zero width or height, as NonZeroRect)."""
         parents = self._parent_transforms()
         if kind == abi.NODE_GROUP:
             t = self._concat(parents[index], self.groups[index]["transform"])
             b = self._group_layer_bounds(index)
         else:
-            owner = next(g for g, grp in enumerate(self.groups) if (abi.NODE_PATH, index) in grp["children"])
+            owner = next(g for g, grp in enumerate(self.groups) if (kind, index) in grp["children"])
             t = self._concat(parents.get(owner, (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)), self.groups[owner]["transform"])
-            b = self._path_bounds(self.paths[index])
+            if kind == abi.NODE_IMAGE:
+                h, w = self.images[index]["pixels"].shape[:2]
+                b = (0.0, 0.0, float(w), float(h))
+            else:
+                b = self._path_bounds(self.paths[index])
         if b is None:
             return None
         x0, y0, x1, y1 = self._transform_box(b, t)
@@ -349,13 +381,21 @@ class Scene:
         sc.mask_count = len(self.masks)
         dsh = abi.array(C.c_float, self.dashes)
         sc.dashes = C.cast(dsh, C.POINTER(C.c_float))
+        imgs = (abi.Image * max(1, len(self.images)))()
+        for i, im in enumerate(self.images):
+            h, w = im["pixels"].shape[:2]
+            imgs[i] = abi.Image(im["pixels"].ctypes.data_as(C.POINTER(C.c_uint8)), w, h, im["quality"],
+                                int(im["visible"]))
+        sc.images = C.cast(imgs, C.POINTER(abi.Image))
+        sc.image_count = len(self.images)
 
         class _SceneABI:
             pass
 
         out = _SceneABI()
         out.abi = sc
-        out.keep = (groups, paths, ch, verbs, pts, st, filt, prims, vals, ins, cps, msk, dsh)
+        out.keep = (groups, paths, ch, verbs, pts, st, filt, prims, vals, ins, cps, msk, dsh, imgs,
+                    [im["pixels"] for im in self.images])
         self._abi = out
         return out
 

@@ -1,11 +1,14 @@
-"""Copies the reference's own known-answer PNGs used by the filter fixture tests.
+"""Copies the reference's own known-answer PNGs used by the fixture tests, and the
+raster images the image fixtures draw.
 
 The PNGs are resvg's test references (crates/resvg/tests/tests/{filters, masking,
-paint-servers, painting, shapes}/..., MIT or Apache-2.0, Copyright the Resvg
-Authors).  Only PNG data files are copied — no
+paint-servers, painting, shapes, structure}/..., MIT or Apache-2.0, Copyright the
+Resvg Authors).  Only PNG data files are copied — no
 reference source.  Run in a checkout that has the reference tree mounted; the
 copies are committed so the tests never read the reference at run time.
 """
+import base64
+import re
 import shutil
 import sys
 from pathlib import Path
@@ -83,7 +86,27 @@ OTHER = [
     "painting/stroke-dashoffset/default", "painting/stroke-dashoffset/em-units",
     "painting/stroke-dashoffset/mm-units", "painting/stroke-dashoffset/negative-value",
     "painting/stroke-dashoffset/percent-units", "painting/stroke-dashoffset/px-units",
+    "paint-servers/pattern/simple-case", "paint-servers/pattern/overflow=visible",
+    "paint-servers/pattern/with-patternTransform", "paint-servers/pattern/with-x-and-y",
+    "paint-servers/pattern/patternUnits=userSpaceOnUse-with-percent",
+    "paint-servers/pattern/transform-and-patternTransform", "paint-servers/pattern/with-viewBox",
+    "paint-servers/pattern/preserveAspectRatio", "paint-servers/pattern/tiny-pattern-upscaled",
+    "paint-servers/pattern/patternUnits=objectBoundingBox", "paint-servers/pattern/patternContentUnits=objectBoundingBox",
+    "paint-servers/pattern/patternContentUnits-with-viewBox", "paint-servers/pattern/nested-objectBoundingBox",
+    "paint-servers/pattern/pattern-on-child",
+    "structure/image/external-png", "structure/image/embedded-png", "structure/image/with-transform",
+    "structure/image/raster-image-and-size-with-odd-numbers", "structure/image/image-with-float-size-scaling",
+    "structure/image/preserveAspectRatio=xMidYMid-meet", "structure/image/preserveAspectRatio=xMinYMin-meet",
+    "structure/image/preserveAspectRatio=xMaxYMax-meet", "structure/image/preserveAspectRatio=xMidYMid-slice",
+    "structure/image/preserveAspectRatio=xMinYMin-slice", "structure/image/preserveAspectRatio=xMaxYMax-slice",
+    "structure/image/preserveAspectRatio=none",
 ]
+# raster images the image fixtures draw: the reference's resource PNGs, and the PNGs
+# embedded as data URIs in fixture SVGs (the base64 payload decoded; data only)
+IMAGES_DST = Path(__file__).resolve().parent / "images"
+RESOURCE_PNGS = ["image.png", "image-63x61.png"]
+EMBEDDED_PNGS = ["structure/image/embedded-png", "structure/image/with-transform",
+                 "structure/image/image-with-float-size-scaling", "structure/image/preserveAspectRatio=none"]
 
 
 def main() -> int:
@@ -100,9 +123,26 @@ def main() -> int:
             print("missing", src, file=sys.stderr)
             return 1
         shutil.copyfile(src, DST / (n.replace("/", "__") + ".png"))
-    (DST / "LICENSE-NOTE.txt").write_text(
-        "PNG test references copied from linebender/resvg 0.48.1 (crates/resvg/tests/tests/filters),\n"
-        "Copyright 2018-2025 the Resvg Authors, dual-licensed MIT OR Apache-2.0.\n")
+    IMAGES_DST.mkdir(exist_ok=True)
+    for n in RESOURCE_PNGS:
+        shutil.copyfile(SRC.parents[1] / "resources" / n, IMAGES_DST / n)
+    for n in EMBEDDED_PNGS:
+        text = (SRC.parent / f"{n}.svg").read_text()
+        payloads = re.findall(r"data:image/png;base64,([A-Za-z0-9+/=\s]+)", text)
+        if not payloads:
+            print("no embedded PNG in", n, file=sys.stderr)
+            return 1
+        data = base64.b64decode("".join(payloads[0].split()))
+        (IMAGES_DST / (n.split("/")[-1] + ".png")).write_bytes(data)
+    note = ("PNG test references copied from linebender/resvg 0.48.1, from\n"
+            "crates/resvg/tests/tests/{filters,masking,paint-servers,painting,shapes,structure}\n"
+            "(file names: the fixture path with \"/\" replaced by \"__\"),\n"
+            "Copyright 2018-2025 the Resvg Authors, dual-licensed MIT OR Apache-2.0.\n")
+    (DST / "LICENSE-NOTE.txt").write_text(note)
+    (IMAGES_DST / "LICENSE-NOTE.txt").write_text(
+        "Raster images from linebender/resvg 0.48.1: crates/resvg/tests/resources/{image.png,image-63x61.png},\n"
+        "and the PNGs embedded as data URIs in crates/resvg/tests/tests/structure/image/*.svg (named after\n"
+        "their fixture), Copyright 2018-2025 the Resvg Authors, dual-licensed MIT OR Apache-2.0.\n")
     return 0
 
 

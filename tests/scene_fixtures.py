@@ -8,6 +8,7 @@ become groups) and renders it at 300 px (scale 1.5).
 from __future__ import annotations
 
 import math
+from pathlib import Path
 
 import numpy as np
 
@@ -714,6 +715,232 @@ def _fe_image_fixtures():
     return out
 
 
+def _pattern_fixtures():
+    """paint-servers/pattern/*: usvg resolves a pattern per element into a user-space
+    tile rect (objectBoundingBox units through NonZeroRect::bbox_transform in f32), a
+    patternTransform and a content group whose children carry the content transform
+    (Transform::from_scale(bbox w, h) for objectBoundingBox content, the viewBox
+    transform of utils::view_box_to_transform otherwise; paint_server.rs:150-223,
+    1007-1072).  The shapes are rounded rects (rx = ry = 20) or plain rects with a
+    1-unit darkblue stroke."""
+    out = {}
+    grey, green, darkblue = (128, 128, 128), GREEN, (0, 0, 139)
+    F = np.float32
+
+    def content(sc, cells, transform=None, paints=None):
+        root = sc.content_group()
+        parent = root if transform is None else sc.group(root, transform=transform)
+        for k, (x, y, w, h) in enumerate(cells):
+            paint = (paints or [PaintSpec.solid(grey), PaintSpec.solid(green)])[k]
+            sc.path(parent, *rect_path(x, y, w, h), fill=paint)
+        return root
+
+    def shape(sc, paint, rect, rounded=True, transform=None):
+        parent = sc.root if transform is None else sc.group(sc.root, transform=transform)
+        geo = rounded_rect_usvg(*rect, 20.0, 20.0) if rounded else rect_path(*rect)
+        sc.path(parent, *geo, fill=paint, stroke=PaintSpec.solid(darkblue))
+
+    def bbox_rect(r, bbox):  # NonZeroRect::bbox_transform in f32
+        x, y, w, h = (F(v) for v in r)
+        bx, by, bw, bh = (F(v) for v in bbox)
+        return (float(x * bw + bx), float(y * bh + by), float(w * bw), float(h * bh))
+
+    def view_box_ts(vb, size, none=False):  # utils::view_box_to_transform, xMidYMid meet
+        vx, vy, vw, vh = (F(v) for v in vb)
+        w, h = (F(v) for v in size)
+        sx, sy = w / vw, h / vh
+        if not none:
+            sx = sy = sy if sx > sy else sx
+        x, y = -vx * sx, -vy * sy
+        lw, lh = w - vw * sx, h - vh * sy
+        tx, ty = (x, y) if none else (x + lw / F(2.0), y + lh / F(2.0))
+        return tuple(float(v) for v in (sx, F(0), F(0), sy, tx, ty))
+
+    user_cells = [(0.0, 0.0, 10.0, 10.0), (10.0, 10.0, 10.0, 10.0)]
+    RR = (20.0, 20.0, 160.0, 160.0)
+
+    def simple(rect=(0.0, 0.0, 20.0, 20.0), transform=None, shape_rect=RR, shape_ts=None, cells_ts=None):
+        sc = Scene(200, 200)
+        root = content(sc, user_cells, cells_ts)
+        shape(sc, PaintSpec.pattern(rect, root, transform=transform), shape_rect, transform=shape_ts)
+        return sc
+
+    c30, s30 = math.cos(math.radians(30.0)), math.sin(math.radians(30.0))
+    rot30 = tuple(f32(v) for v in (c30, s30, -s30, c30, 0.0, 0.0))
+    pp = "paint-servers/pattern/"
+    out[pp + "simple-case"] = simple()
+    out[pp + "overflow=visible"] = simple()
+    out[pp + "with-patternTransform"] = simple(transform=rot30)
+    out[pp + "with-x-and-y"] = simple(rect=(5.0, 10.0, 20.0, 20.0), shape_rect=(20.0, 20.0, 140.0, 140.0))
+    out[pp + "patternUnits=userSpaceOnUse-with-percent"] = simple(rect=(15.0, 10.0, 20.0, 20.0))
+    # rotate(-30, 110, 70) = translate(110 70) rotate(-30) translate(-110 -70), in f64
+    cm, sm = math.cos(math.radians(-30.0)), math.sin(math.radians(-30.0))
+    rot_about = tuple(f32(v) for v in (cm, sm, -sm, cm, 110.0 - cm * 110.0 + sm * 70.0, 70.0 - sm * 110.0 - cm * 70.0))
+    out[pp + "transform-and-patternTransform"] = simple(transform=rot30, shape_rect=(20.0, 20.0, 140.0, 140.0),
+                                                        shape_ts=rot_about)
+    out[pp + "with-viewBox"] = simple(rect=(0.0, 0.0, 20.0, 40.0),
+                                      cells_ts=view_box_ts((0, 0, 40, 40), (20, 40)))
+    out[pp + "preserveAspectRatio"] = simple(rect=(0.0, 0.0, 20.0, 40.0),
+                                             cells_ts=view_box_ts((0, 0, 40, 40), (20, 40), none=True))
+
+    sc = Scene(200, 200)
+    root = sc.content_group()
+    sc.path(root, *ellipse_path_usvg(1.0, 1.0, 1.0, 1.0), fill=PaintSpec.solid(green))
+    shape(sc, PaintSpec.pattern((0.0, 0.0, 2.0, 2.0), root, transform=(10.0, 0.0, 0.0, 10.0, 0.0, 0.0)), RR)
+    out[pp + "tiny-pattern-upscaled"] = sc
+
+    obb_cells = [(0.0, 0.0, 0.1, 0.1), (0.1, 0.1, 0.1, 0.1)]
+    two = ((20.0, 20.0, 160.0, 70.0), (20.0, 110.0, 70.0, 70.0))
+    P = (0.075, 0.05, 0.2, 0.2)
+    for name in ("patternUnits=objectBoundingBox", "patternContentUnits=objectBoundingBox",
+                 "patternContentUnits-with-viewBox"):
+        sc = Scene(200, 200)
+        for bbox in two:
+            rect = bbox_rect(P, bbox)
+            if name == "patternUnits=objectBoundingBox":
+                root = content(sc, user_cells)
+            elif name == "patternContentUnits=objectBoundingBox":
+                root = content(sc, obb_cells, (bbox[2], 0.0, 0.0, bbox[3], 0.0, 0.0))
+            else:  # the viewBox wins over objectBoundingBox content units
+                root = content(sc, user_cells, view_box_ts((0, 0, 10, 20), rect[2:]))
+            shape(sc, PaintSpec.pattern(rect, root), bbox, rounded=False)
+        out[pp + name] = sc
+
+    # nested-objectBoundingBox: the grey cell is a white->black gradient in its own bbox units
+    sc = Scene(200, 200)
+    rect = bbox_rect(P, RR)
+    lg = PaintSpec.linear(0.0, 0.0, 1.0, 0.0, [(0.0, WHITE, 0.0), (1.0, BLACK, 1.0)],
+                          transform=_bbox_ts(*(f32(v) for v in (0.0, 0.0, 0.1, 0.1))))
+    root = content(sc, obb_cells, (160.0, 0.0, 0.0, 160.0, 0.0, 0.0), paints=[lg, PaintSpec.solid(green)])
+    shape(sc, PaintSpec.pattern(rect, root), RR)
+    out[pp + "nested-objectBoundingBox"] = sc
+
+    # pattern-on-child: patt1 (objectBoundingBox tile 0.3 x 0.4 and content) on rect5 and
+    # on rect4, a cell of patt2 (user space 20 x 20)
+    sc = Scene(200, 200)
+    cells1 = [(0.0, 0.0, 0.2, 0.2), (0.2, 0.2, 0.2, 0.2)]
+
+    def patt1(bbox):
+        r = content(sc, cells1, (bbox[2], 0.0, 0.0, bbox[3], 0.0, 0.0))
+        return PaintSpec.pattern(bbox_rect((0.0, 0.0, 0.3, 0.4), bbox), r)
+
+    shape(sc, patt1((20.0, 20.0, 160.0, 70.0)), (20.0, 20.0, 160.0, 70.0), rounded=False)
+    root2 = content(sc, [(0.0, 0.0, 10.0, 10.0), (10.0, 10.0, 10.0, 10.0)],
+                    paints=[PaintSpec.solid(grey), patt1((10.0, 10.0, 10.0, 10.0))])
+    shape(sc, PaintSpec.pattern((0.0, 0.0, 20.0, 20.0), root2), (20.0, 110.0, 160.0, 70.0), rounded=False)
+    out[pp + "pattern-on-child"] = sc
+    return out
+
+
+IMAGES_DIR = Path(__file__).resolve().parent / "golden" / "images"
+# render scale of fixtures whose viewBox is not 200 units wide (300 px / viewBox width)
+RENDER_SCALE = {}
+
+
+def load_image(name: str) -> np.ndarray:
+    """A fixture PNG as tiny-skia's Pixmap::decode_png holds it: 8-bit RGBA (palettes
+    and transparency expanded), premultiplied with premultiply_u8,
+    p = c * a + 128, (p + (p >> 8)) >> 8."""
+    from PIL import Image
+    rgba = np.asarray(Image.open(IMAGES_DIR / name).convert("RGBA")).astype(np.uint32)
+    prod = rgba[..., :3] * rgba[..., 3:4] + 128
+    out = rgba.copy()
+    out[..., :3] = (prod + (prod >> 8)) >> 8
+    return out.astype(np.uint8)
+
+
+def _image_fixtures():
+    """structure/image/*: usvg's convert_inner (parser/image.rs:206-290) puts a raster
+    image in a group whose transform maps the image's pixel rect onto the view box that
+    fit_view_box / aligned_pos give for its preserveAspectRatio (slice adds a clip path
+    of the image rect around it); resvg fills the pixel rect with the image's pixmap
+    (image.rs:159-206, bicubic for the default image-rendering).  Sizes in f32."""
+    out = {}
+    F = np.float32
+    black = PaintSpec.solid(BLACK)
+
+    def size_scale(s1, s2, expand):  # usvg Size::scale_to / expand_to
+        rw = s2[1] * s1[0] / s1[1]
+        with_h = rw <= s2[0] if expand else rw >= s2[0]
+        return (rw, s2[1]) if not with_h else (s2[0], s2[0] * s1[1] / s1[0])
+
+    def image_ts(size, rect, align=("mid", "mid"), slice_=False):
+        size = (F(size[0]), F(size[1]))
+        x, y, w, h = (F(v) for v in rect)
+        if align is None:
+            vw, vh = w, h
+        else:
+            vw, vh = size_scale(size, (w, h), slice_)
+        lw, lh = w - vw, h - vh
+        ax = x if align is None else (x + (lw / F(2) if align[0] == "mid" else lw if align[0] == "max" else F(0)))
+        ay = y if align is None else (y + (lh / F(2) if align[1] == "mid" else lh if align[1] == "max" else F(0)))
+        return tuple(float(v) for v in (vw / size[0], F(0), F(0), vh / size[1], ax, ay))
+
+    def frame(sc, vb):
+        if vb == 80:
+            sc.path(sc.root, *rect_path(1.0, 1.0, 78.0, 78.0), stroke=black)
+        elif vb == 10:
+            sc.path(sc.root, *rect_path(0.1, 0.1, 9.8, 9.8), stroke=black, stroke_width=0.1)
+        else:
+            sc.path(sc.root, *rect_path(1.0, 1.0, 198.0, 198.0), stroke=black)
+
+    def place(sc, parent, pixels, rect, align=("mid", "mid"), slice_=False, transform=None):
+        if transform is not None:
+            parent = sc.group(parent, transform=transform)
+        if slice_:  # a clip path of the image rect around the image group
+            c, croot = sc.clip_path()
+            sc.path(croot, *rect_path(*rect), fill=black)
+            parent = sc.group(parent, clip_path=c)
+        g = sc.group(parent, transform=image_ts(pixels.shape[1::-1], rect, align, slice_))
+        sc.image(g, pixels)
+
+    si = "structure/image/"
+    for name, png, rect in (("external-png", "image.png", (8.0, 8.0, 64.0, 64.0)),
+                            ("embedded-png", "embedded-png.png", (8.0, 8.0, 64.0, 64.0)),
+                            ("raster-image-and-size-with-odd-numbers", "image-63x61.png", (8.0, 8.0, 63.0, 61.0))):
+        sc = Scene(80, 80)
+        place(sc, sc.root, load_image(png), rect)
+        frame(sc, 80)
+        out[si + name] = sc
+        RENDER_SCALE[si + name] = 300.0 / 80.0
+    # transform="scale(0.6) skewX(30)", composed in f64 and stored as f32
+    t30 = math.tan(math.radians(30.0))
+    sc = Scene(80, 80)
+    place(sc, sc.root, load_image("with-transform.png"), (8.0, 8.0, 64.0, 64.0),
+          transform=tuple(f32(v) for v in (0.6, 0.0, 0.6 * t30, 0.6, 0.0, 0.0)))
+    frame(sc, 80)
+    out[si + "with-transform"] = sc
+    RENDER_SCALE[si + "with-transform"] = 300.0 / 80.0
+    sc = Scene(10, 10)
+    place(sc, sc.root, load_image("image-with-float-size-scaling.png"), (4.0, 4.0, 1.0, 1.5))
+    sc.path(sc.root, *rect_path(4.0, 4.0, 1.0, 1.5), stroke=black, stroke_width=0.05)
+    frame(sc, 10)
+    out[si + "image-with-float-size-scaling"] = sc
+    RENDER_SCALE[si + "image-with-float-size-scaling"] = 30.0
+    tile = load_image("preserveAspectRatio=none.png")
+    names = {"min": "Min", "mid": "Mid", "max": "Max"}
+    for a in ("min", "mid", "max"):
+        for slice_ in (False, True):
+            sc = Scene(200, 200)
+            for rect in ((36.0, 3.0, 128.0, 64.0), (68.0, 70.0, 64.0, 128.0)):
+                place(sc, sc.root, tile, rect, (a, a), slice_)
+                sc.path(sc.root, *rect_path(*rect), stroke=black)
+            frame(sc, 200)
+            out[si + f"preserveAspectRatio=x{names[a]}Y{names[a]}-{'slice' if slice_ else 'meet'}"] = sc
+    sc = Scene(200, 200)
+    place(sc, sc.root, tile, (36.0, 68.0, 128.0, 64.0), None)
+    sc.path(sc.root, *rect_path(36.0, 68.0, 128.0, 64.0), stroke=black)
+    frame(sc, 200)
+    out[si + "preserveAspectRatio=none"] = sc
+    return out
+
+
+def fixture_ts(name: str):
+    """The render transform of a fixture: 300 px over its viewBox width."""
+    s = RENDER_SCALE.get(name, SCALE)
+    return abi.Transform(s, 0.0, 0.0, s, 0.0, 0.0)
+
+
 def all_scene_fixtures():
     out = dict(shape_fixtures())
     out.update(_hairline_fixtures())
@@ -723,6 +950,8 @@ def all_scene_fixtures():
         out[fx.name] = filter_scene(fx)
     out.update(_tile_subregion_fixtures())
     out.update(_fe_image_fixtures())
+    out.update(_pattern_fixtures())
+    out.update(_image_fixtures())
     return out
 
 

@@ -1,6 +1,8 @@
 """rvk_render (the product's device pipeline) against the oracle's render walk,
 bit-exact on every channel, on the reference's fixture scenes and on seeded
 random scenes that exercise the rasterizer, shaders, groups and filters."""
+import math
+
 import numpy as np
 import pytest
 
@@ -10,7 +12,7 @@ from resvg_amd.scene import PaintSpec, Scene, ellipse_path, rect_path
 from resvg_amd.workloads import blur_offset_composite_doc, document_batch, full_filter_doc
 
 from fixture_scenes import compare_png, load_png
-from scene_fixtures import all_scene_fixtures, fixture_guides, render_ts
+from scene_fixtures import all_scene_fixtures, fixture_guides, fixture_ts, render_ts
 
 pytestmark = pytest.mark.gpu
 
@@ -43,8 +45,8 @@ def _assert_same(got, exp, what):
 
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_scene_fixture(ctx, name):
-    got = _gpu(ctx, SCENES[name], render_ts(), 300, 300)
-    _assert_same(got, _oracle(SCENES[name], render_ts(), 300, 300), name)
+    got = _gpu(ctx, SCENES[name], fixture_ts(name), 300, 300)
+    _assert_same(got, _oracle(SCENES[name], fixture_ts(name), 300, 300), name)
     assert compare_png(got, load_png(name), guides=fixture_guides(name)) == 0
 
 
@@ -301,6 +303,100 @@ def test_random_fe_image(ctx, seed):
     _assert_same(_gpu(ctx, sc, ts, 192, 192), _oracle(sc, ts, 192, 192), f"feImage seed {seed}")
 
 
+def _pattern_scene(seed, size=160):
+    """Paths filled and stroked with pattern paints: random tile contents (one nested
+    pattern), tile rects, pattern transforms (translations sample nearest, rotations
+    and scales bicubic), opacities, hairline strokes and transformed groups."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(size, size)
+    roots = []
+    for i in range(3):
+        root = sc.content_group()
+        for _ in range(int(rng.integers(1, 4))):
+            v, p = _random_path(rng, 24, int(rng.integers(2, 6)))
+            paint = _paint(rng, 24)
+            if roots and rng.random() < 0.3:
+                paint = PaintSpec.pattern((0.0, 0.0, 7.0, 5.0), roots[-1])
+            sc.path(root, v, p, fill=paint)
+        roots.append(root)
+
+    def pattern():
+        kind = rng.integers(3)
+        ts = None
+        if kind == 1:
+            a = math.radians(float(rng.uniform(-60, 60)))
+            s = float(rng.uniform(0.5, 2.0))
+            ts = tuple(float(np.float32(v)) for v in (s * math.cos(a), s * math.sin(a), -s * math.sin(a),
+                                                      s * math.cos(a), rng.uniform(-5, 5), rng.uniform(-5, 5)))
+        elif kind == 2:
+            ts = (1.0, 0.0, 0.0, 1.0, float(np.float32(rng.uniform(-9, 9))), float(np.float32(rng.uniform(-9, 9))))
+        rect = tuple(float(np.float32(v)) for v in (rng.uniform(-5, 5), rng.uniform(-5, 5), rng.uniform(6, 30),
+                                                     rng.uniform(6, 30)))
+        return PaintSpec.pattern(rect, roots[int(rng.integers(len(roots)))],
+                                 opacity=float(np.float32(rng.uniform(0.3, 1.0))) if rng.random() < 0.4 else 1.0,
+                                 transform=ts)
+
+    for i in range(8):
+        parent = sc.root
+        if rng.random() < 0.3:
+            a = float(np.float32(rng.uniform(0.7, 1.3)))
+            parent = sc.group(sc.root, transform=(a, 0.0, float(np.float32(rng.uniform(-0.3, 0.3))), a, 3.0, -2.0),
+                              opacity=float(np.float32(rng.uniform(0.5, 1.0))))
+        v, p = _random_path(rng, size, int(rng.integers(2, 8)))
+        stroke = pattern() if rng.random() < 0.4 else None
+        sc.path(parent, v, p, fill=pattern() if rng.random() < 0.9 else _paint(rng, size), stroke=stroke,
+                stroke_width=float(np.float32(rng.choice([0.5, 1.0, 3.0, 7.0]))))
+    return sc
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_patterns(ctx, seed):
+    sc = _pattern_scene(1100 + seed)
+    ts = abi.Transform(1.25, 0.0, 0.0, 1.25, -6.0, 4.0) if seed % 2 else abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, 192, 192), _oracle(sc, ts, 192, 192), f"patterns seed {seed}")
+
+
+def _image_scene(seed, size=160):
+    """Raster images (random premultiplied pixels, a few sharp-edged) under integer and
+    fractional translations (nearest), scales, rotations and skews with each filter
+    quality, in opacity groups and partly outside the pixmap, between filled paths."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(size, size)
+    for i in range(7):
+        h, w = (int(v) for v in rng.integers(3, 40, 2))
+        px = rng.integers(0, 256, (h, w, 4)).astype(np.uint8)
+        if rng.random() < 0.4:
+            px[..., 3] = 255
+        if rng.random() < 0.3:
+            px[: h // 2] = 0
+        px[..., :3] = np.minimum(px[..., :3], px[..., 3:4])
+        kind = rng.integers(4)
+        if kind == 0:
+            ts = (1.0, 0.0, 0.0, 1.0, float(rng.integers(-10, size)), float(rng.integers(-10, size)))
+        elif kind == 1:
+            ts = (1.0, 0.0, 0.0, 1.0) + tuple(float(np.float32(v)) for v in rng.uniform(-10, size, 2))
+        else:
+            a = math.radians(float(rng.uniform(-50, 50))) if kind == 3 else 0.0
+            s = float(rng.uniform(0.4, 4.0))
+            k = float(rng.uniform(-0.5, 0.5)) if kind == 3 else 0.0
+            ts = tuple(float(np.float32(v)) for v in (s * math.cos(a), s * math.sin(a), -s * math.sin(a) + k,
+                                                      s * math.cos(a), rng.uniform(-10, size), rng.uniform(-10, size)))
+        parent = sc.group(sc.root, opacity=float(np.float32(rng.uniform(0.4, 1.0)))) if rng.random() < 0.3 else sc.root
+        g = sc.group(parent, transform=ts)
+        sc.image(g, px, quality=int(rng.integers(3)), visible=rng.random() < 0.95)
+        if rng.random() < 0.5:
+            v, p = _random_path(rng, size, int(rng.integers(2, 6)))
+            sc.path(sc.root, v, p, fill=_paint(rng, size))
+    return sc
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_images(ctx, seed):
+    sc = _image_scene(1300 + seed)
+    ts = abi.Transform(1.25, 0.0, 0.0, 1.25, -3.0, 5.0) if seed % 3 == 1 else abi.Transform.identity()
+    _assert_same(_gpu(ctx, sc, ts, 192, 176), _oracle(sc, ts, 192, 176), f"images seed {seed}")
+
+
 def test_composites_onto_existing_pixels(ctx):
     rng = np.random.default_rng(7)
     base = rng.integers(0, 256, (96, 128, 4)).astype(np.uint8)
@@ -477,3 +573,44 @@ def test_more_crossings_per_row_than_the_key_area(ctx, case):
     sc.path(sc.root, v2, p2, fill=PaintSpec.solid((30, 160, 60), opacity=0.5))
     ts = abi.Transform.identity()
     _assert_same(_gpu(ctx, sc, ts, w, h), _oracle(sc, ts, w, h), f"comb {case}")
+
+
+def test_flush_blocks_pass_the_index_validator(ctx, monkeypatch):
+    # RVK_VALIDATE checks, on the host, every index the device will read from each
+    # flush block (job edge / window / table / arena ranges, piece slots, band lists,
+    # item job and shader references, hairline tiles and bins, pattern pixmaps) before
+    # it is uploaded; a violation fails the render.  Fixture scenes, seeded scenes
+    # (patterns, images, feImage, clips and masks, paths beyond the pixmap, tiny
+    # flushes) and the 4096 x 4096 document.  The context's stats count one render.
+    monkeypatch.setenv("RVK_VALIDATE", "1")
+
+    total = [0]
+
+    def check(sc, ts, w, h, what, exp=None):
+        got = _gpu(ctx, sc, ts, w, h)
+        st = ctx.stats()
+        assert st.validated_flushes == st.flushes, (what, st.validated_flushes, st.flushes)
+        total[0] += st.validated_flushes
+        _assert_same(got, _oracle(sc, ts, w, h) if exp is None else exp, what)
+
+    for name in sorted(SCENES):
+        check(SCENES[name], fixture_ts(name), 300, 300, name)
+    ident = abi.Transform.identity()
+    for sc, (w, h) in ((_random_scene(200, 128, 10, margin=0.5), (128, 128)), (_pattern_scene(1100), (160, 160)),
+                       (_image_scene(1300), (160, 160)), (_fe_image_scene(900), (160, 160)),
+                       (_clip_or_mask_scene(700), (160, 160)),
+                       (full_filter_doc(seed=5, size=384, n_paths=40, paths_per_filter=3, max_segments=24)[0],
+                        (384, 384))):
+        check(sc, ident, w, h, "validated scene")
+    monkeypatch.setenv("RVK_FLUSH_BYTES", "4096")
+    sc, (w, h) = full_filter_doc(seed=3, size=256, n_paths=24, paths_per_filter=2, max_segments=16)
+    check(sc, ident, w, h, "validated tiny flushes")
+    # the 4096 x 4096 full-filter document: validated render == unvalidated render
+    monkeypatch.delenv("RVK_FLUSH_BYTES")
+    monkeypatch.delenv("RVK_VALIDATE")
+    sc, (w, h) = full_filter_doc()
+    plain = _gpu(ctx, sc, ident, w, h)
+    assert ctx.stats().validated_flushes == 0
+    monkeypatch.setenv("RVK_VALIDATE", "1")
+    check(sc, ident, w, h, "validated 4096 document", plain)
+    assert total[0] > len(SCENES)

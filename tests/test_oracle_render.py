@@ -8,7 +8,7 @@ from resvg_amd import abi
 from resvg_amd.scene import PaintSpec, Scene, ellipse_path, rect_path
 
 from fixture_scenes import compare_png, load_png
-from scene_fixtures import all_scene_fixtures, fixture_guides, render_ts
+from scene_fixtures import all_scene_fixtures, fixture_guides, fixture_ts, render_ts
 
 SCENES = all_scene_fixtures()
 
@@ -16,7 +16,7 @@ SCENES = all_scene_fixtures()
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_scene_fixture_png(name):
     px = np.zeros((300, 300, 4), np.uint8)
-    orc.render(SCENES[name].to_abi().abi, render_ts(), px)
+    orc.render(SCENES[name].to_abi().abi, fixture_ts(name), px)
     assert compare_png(px, load_png(name), guides=fixture_guides(name)) == 0
 
 
