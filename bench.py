@@ -482,6 +482,13 @@ def main() -> int:
             "avg_launch_ms": round(dom["total_ms"] / dom["launches"], 4),
             "kernel_share_of_step": round(dom["total_ms"] / (ms * args.steps), 4),
         }
+        if traffic:
+            # the bytes the kernel actually moves (the rocprofv3 counters above) at its
+            # live launch time: far below `achieved`, since the fused compositor reads
+            # and writes each target pixel once per flush rather than once per path
+            moved = traffic / (dom["total_ms"] / dom["launches"] / 1e3) / 1e9
+            roofline["moved_GBps"] = round(moved, 2)
+            roofline["moved_frac"] = round(moved / HBM_PEAK_GBS, 4)
 
     line = {
         "metric": "Mpixel/s at 4096x4096 full-filter scene (resvg::render pixel pipeline)",

@@ -101,6 +101,9 @@ __device__ __forceinline__ float pd_channel(uint32_t mode, float x, float y, flo
 
 __device__ __forceinline__ F4 blend(uint32_t mode, F4 s, F4 d) {
     const float sa = s.a, da = d.a;
+    // SourceOver (nearly every layer): decided once rather than in each channel's switch
+    if (mode == RVK_BLEND_SOURCE_OVER)
+        return F4{d.r * inv(sa) + s.r, d.g * inv(sa) + s.g, d.b * inv(sa) + s.b, d.a * inv(sa) + s.a};
     if (mode <= RVK_BLEND_SCREEN || mode == RVK_BLEND_MULTIPLY) {
         return F4{pd_channel(mode, s.r, d.r, sa, da), pd_channel(mode, s.g, d.g, sa, da),
                   pd_channel(mode, s.b, d.b, sa, da), pd_channel(mode, s.a, d.a, sa, da)};

@@ -691,7 +691,26 @@ __global__ __launch_bounds__(kWave) void k_coverage(CovArgs a) {
 // ---------------------------------------------------------------------------
 // shading (tiny-skia lowp) and compositing
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t div255(uint32_t v) { return (v + 255u) >> 8; }
+// lowp arithmetic on packed pixels, two channels per 24-bit multiply (one full-rate
+// VALU operation): channels 0 and 2 sit in bits 0-7 and 16-23 of one word, 1 and 3 in
+// bits 8-15 and 24-31 of the other, and each 16-bit field's value (at most
+// 255 * 255 + 255) never carries into the next.  div255(v) = (v + 255) >> 8.
+constexpr uint32_t kEven = 0x00ff00ffu;
+
+// div255(c_k * m) for every channel k of c (m <= 255)
+__device__ __forceinline__ uint32_t scale_px(uint32_t c, uint32_t m) {
+    const uint32_t lo = ((__umul24(c & kEven, m) + kEven) >> 8) & kEven;
+    const uint32_t hi = (__umul24((c >> 8) & kEven, m) + kEven) & ~kEven;
+    return lo | hi;
+}
+
+// div255(d_k * (255 - m) + s_k * m) for every channel k (lowp lerp, m <= 255)
+__device__ __forceinline__ uint32_t lerp_px(uint32_t d, uint32_t s, uint32_t m) {
+    const uint32_t im = 255u - m;
+    const uint32_t lo = ((__umul24(d & kEven, im) + __umul24(s & kEven, m) + kEven) >> 8) & kEven;
+    const uint32_t hi = (__umul24((d >> 8) & kEven, im) + __umul24((s >> 8) & kEven, m) + kEven) & ~kEven;
+    return lo | hi;
+}
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
 // the spread mode applied to a gradient coordinate
@@ -758,20 +777,19 @@ __device__ __forceinline__ void grad_color(const ShaderParams &sp, const ShaderH
     }
 }
 
-// premultiplied u8 source colour of a shader at pixel (px, py) (tiny-skia lowp)
-__device__ __forceinline__ void shade(const ShaderParams &sp, const ShaderHead &s, int32_t px, int32_t py,
-                                      uint32_t o[4]) {
-    if (s.kind == 1) {
-        o[0] = s.rgba & 0xff; o[1] = (s.rgba >> 8) & 0xff; o[2] = (s.rgba >> 16) & 0xff; o[3] = s.rgba >> 24;
-        return;
-    }
+// premultiplied u8 source colour of a shader at pixel (px, py), packed (tiny-skia lowp)
+__device__ __forceinline__ uint32_t shade(const ShaderParams &sp, const ShaderHead &s, int32_t px, int32_t py) {
+    if (s.kind == 1) return s.rgba;
     float c[4];
     grad_color(sp, s, grad_t(s, px, py), c);
+    uint32_t o = 0;
     for (int k = 0; k < 4; k++) {
         const float v = (c[k] != c[k]) ? 0.0f : clamp01(c[k]);
-        o[k] = (uint32_t)(v * 255.0f + 0.5f);
+        o |= (uint32_t)(v * 255.0f + 0.5f) << (8 * k);
     }
-    if (!s.opaque) for (int k = 0; k < 3; k++) o[k] = div255(o[k] * o[3]);
+    // premultiply: every colour channel times alpha / 255 (alpha's own field is empty)
+    if (!s.opaque) o = scale_px(o & 0x00ffffffu, o >> 24) | (o & 0xff000000u);
+    return o;
 }
 
 // One texel of a pattern pixmap at pixmap coordinates (u, v): Repeat tiling (none for
@@ -887,21 +905,13 @@ __device__ __noinline__ uint32_t paint_px_hp(const ShaderParams &sp, const Shade
 }
 
 // lowp: scale by coverage (unless full), then SourceOver onto `d`; an opaque shader
-// composites with Source instead (full coverage stores, partial coverage lerps).
-__device__ __forceinline__ uint32_t over_lowp(const uint32_t src_in[4], uint32_t cov, uint32_t d, bool as_source) {
-    if (as_source) {
-        if (cov == 255) return src_in[0] | (src_in[1] << 8) | (src_in[2] << 16) | (src_in[3] << 24);
-        const uint32_t ic = 255u - cov;
-        uint32_t o = 0;
-        for (int k = 0; k < 4; k++) o |= div255(((d >> (8 * k)) & 0xffu) * ic + src_in[k] * cov) << (8 * k);
-        return o;
-    }
-    uint32_t src[4] = {src_in[0], src_in[1], src_in[2], src_in[3]};
-    if (cov != 255) for (int k = 0; k < 4; k++) src[k] = div255(src[k] * cov);
-    const uint32_t ia = 255u - src[3];
-    uint32_t o = 0;
-    for (int k = 0; k < 4; k++) o |= (src[k] + div255(((d >> (8 * k)) & 0xffu) * ia)) << (8 * k);
-    return o;
+// composites with Source instead (full coverage stores, partial coverage lerps).  A
+// premultiplied colour never exceeds its alpha, so SourceOver's per-channel sums stay
+// within 255.
+__device__ __forceinline__ uint32_t over_lowp(uint32_t src, uint32_t cov, uint32_t d, bool as_source) {
+    if (as_source) return cov == 255 ? src : lerp_px(d, src, cov);
+    if (cov != 255) src = scale_px(src, cov);
+    return src + scale_px(d, 255u - (src >> 24));
 }
 
 // one covered pixel of a draw item: Clear, or the shaded colour composited with
@@ -910,16 +920,11 @@ __device__ __forceinline__ uint32_t over_lowp(const uint32_t src_in[4], uint32_t
 // shaded only when needed
 template <bool kPatterns>
 __device__ __forceinline__ uint32_t paint_px(const ShaderParams &sp, const ShaderHead &sh, int32_t lx, int32_t ly,
-                                             uint32_t c, uint32_t d, uint32_t src[4], bool &shaded) {
-    if (sh.clear) {
-        const uint32_t ic = 255u - c;
-        uint32_t o = 0;
-        for (int k = 0; k < 4; k++) o |= div255(((d >> (8 * k)) & 0xffu) * ic) << (8 * k);
-        return o;
-    }
+                                             uint32_t c, uint32_t d, uint32_t &src, bool &shaded) {
+    if (sh.clear) return scale_px(d, 255u - c);
     if (sh.f32_pipeline) return paint_px_hp<kPatterns>(sp, sh, lx, ly, c, d);
     if (!shaded) {
-        shade(sp, sh, lx, ly, src);
+        src = shade(sp, sh, lx, ly);
         shaded = true;
     }
     return over_lowp(src, c, d, sh.as_source != 0);
@@ -1050,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t
                     const bool stroke = (pass == 0) == (item.stroke_first != 0);
                     const ShaderParams &sp = a.shaders[stroke ? item.stroke_shader : item.fill_shader];
                     const ShaderHead &sh = sp;
-                    uint32_t src[4];
+                    uint32_t src = 0;
                     bool shaded = false;
                     if (stroke && item.stroke_hair) {
                         // every blit of the hairline at this pixel, composited in order

@@ -699,18 +699,25 @@ struct Renderer {
         RVK_TRY(rvk::scratch_alloc(ctx, o_arena + dl.arena, &dev));
         uint8_t *d = (uint8_t *)dev;
         // algorithmic bytes: expansion reads each edge once and writes its pieces;
-        // coverage reads edges, windows and pieces once and writes the tile tables; the
-        // compositor reads and writes each target pixel of its rect once and reads the
-        // tables and the hairline lines (the partial tiles' 64 bytes each, written by
-        // coverage and read by the compositor, are data-dependent and not counted)
-        double job_tiles = 0.0;
-        for (const JobPrep &P : dl.jobs) job_tiles += (double)P.j.tiles_x * (double)P.j.tiles_y;
+        // coverage reads edges, windows and pieces once and writes the tile tables (the
+        // partial tiles' 64 bytes each, written by coverage and read by the compositor,
+        // are data-dependent and not counted); the compositor is credited what the
+        // reference's per-path fills move: 8 B (destination read + write) per pixel of
+        // each fill's, stroke's and hairline's rect, and 12 B per pixel of each
+        // single-path layer it composites in registers (draw_pixmap)
+        double job_tiles = 0.0, job_px = 0.0, layer_px = 0.0;
+        for (const JobPrep &P : dl.jobs) {
+            job_tiles += (double)P.j.tiles_x * (double)P.j.tiles_y;
+            job_px += (double)P.j.w * (double)P.j.h;
+        }
+        for (const HairPrep &P : dl.hair) job_px += (double)P.j.w * (double)P.j.h;
+        for (const DrawItem &it : dl.items)
+            if (it.layer) layer_px += (double)(it.x1 - it.x0) * (double)(it.y1 - it.y0);
         const double piece_bytes = (double)dl.piece_slots * sizeof(PieceDev);
         const double exp_bytes = (double)dl.n_edges * (sizeof(EdgeRec) + 4) + piece_bytes;
         const double cov_bytes = 2.0 * job_tiles + (double)dl.n_edges * sizeof(EdgeCov) +
                                  (double)dl.n_windows * sizeof(uint2) + piece_bytes;
-        const double draw_bytes = 8.0 * (double)(dl.bx1 - dl.bx0) * (double)(dl.by1 - dl.by0) + 2.0 * job_tiles +
-                                  (double)(dl.n_hair_lines * sizeof(HairLine) + dl.n_hair_bins * 4);
+        const double draw_bytes = 8.0 * job_px + 12.0 * layer_px;
         ctx->stats.edge_pieces += dl.piece_slots;
         ctx->stats.band_entries += dl.n_windows;
         ctx->stats.upload_bytes += meta;
