@@ -103,17 +103,22 @@ struct HairJob {
 constexpr int kMaxStops = 16;
 // The scalar part of a shader, read by the compositor in one piece.
 struct ShaderHead {
+    // the fields below that the compositor branches on, packed into one word (kMode*
+    // bits, set by seal_shader once the shader is complete), so that one scalar load
+    // decides every branch; then the solid colour and the gradient transform, adjacent
+    // so that one load brings them
+    uint32_t mode;
+    uint32_t rgba;       // solid premultiplied colour
+    float sx, ky, kx, sy, tx, ty;  // device -> unit transform
+    float r_scale, r_bias;  // focal radius: t = (t - r_scale) / r_bias
     uint32_t kind;       // 0 none, 1 solid, 2 gradient, 3 pattern
     uint32_t radial, spread, pad_clamp, two_stop, opaque, r_twist, nstops;
-    uint32_t rgba;       // solid premultiplied colour
     // Opaque shader (solid alpha 1, linear gradient with opaque stops): tiny-skia
     // reduces SourceOver to Source, so partial coverage lerps towards the colour.
     uint32_t as_source;
     // BlendMode::Clear (clip path children): partial coverage lerps every channel
     // toward zero, div255(d * (255 - c)); the colour is not used
     uint32_t clear;
-    float sx, ky, kx, sy, tx, ty;  // device -> unit transform
-    float r_scale, r_bias;  // focal radius: t = (t - r_scale) / r_bias
     // two-point conical gradient with the focal point off the centre (tiny-skia's
     // FocalData, Skia's kFocal): kFocal* below, 0 for none; p0 = 1 / r1.  Such a shader
     // runs in tiny-skia's f32 pipeline (its stages have no u8 form), so the colour,
@@ -134,6 +139,10 @@ struct ShaderHead {
     float pattern_opacity;
 };
 constexpr uint32_t kFocalOnCircle = 1, kFocalWellBehaved = 2, kFocalGreater = 3;
+// ShaderHead::mode: kind in bits 0-1, spread in bits 3-4, nstops from bit 16
+constexpr uint32_t kModeKind = 3u, kModeRadial = 1u << 2, kModeSpreadShift = 3, kModePadClamp = 1u << 5,
+                   kModeTwoStop = 1u << 6, kModeOpaque = 1u << 7, kModeRTwist = 1u << 8, kModeAsSource = 1u << 9,
+                   kModeClear = 1u << 10, kModeF32 = 1u << 11, kModeStopsShift = 16;
 struct ShaderParams : ShaderHead {
     float t[kMaxStops + 2];
     float f[kMaxStops + 2][4];
@@ -209,6 +218,8 @@ void hairline_lines(const rvk_scene &sc, const rvk_path &path, rvkh::Ts ts, uint
                     std::vector<HairLine> &out);
 // tiny-skia shader setup; false when the paint draws nothing.  `extra_opacity`
 // scales the colour's or every stop's alpha (Shader::apply_opacity).
+// Packs the branch fields of a complete shader into its mode word.
+void seal_shader(ShaderHead &s);
 bool make_shader(const rvk_scene &sc, const rvk_paint &paint, rvkh::Ts ts, ShaderParams &out,
                  float extra_opacity = 1.0f);
 // render_pattern_pixmap's tile for a pattern paint drawn with `ts` (path.rs:179-205):

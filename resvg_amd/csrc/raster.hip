@@ -713,30 +713,31 @@ __device__ __forceinline__ uint32_t lerp_px(uint32_t d, uint32_t s, uint32_t m) 
 }
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
-// the spread mode applied to a gradient coordinate
-__device__ __forceinline__ float spread_t(const ShaderHead &s, float t) {
-    if (s.spread == RVK_SPREAD_REFLECT) {
+// the spread mode applied to a gradient coordinate (m: the shader's mode word)
+__device__ __forceinline__ float spread_t(uint32_t m, float t) {
+    const uint32_t spread = (m >> rvkr::kModeSpreadShift) & 3u;
+    if (spread == RVK_SPREAD_REFLECT) {
         const float u = t - 1.0f;
         t = clamp01(fabsf(u - 2.0f * floorf(u * 0.5f) - 1.0f));
-    } else if (s.spread == RVK_SPREAD_REPEAT) {
+    } else if (spread == RVK_SPREAD_REPEAT) {
         t = clamp01(t - floorf(t));
-    } else if (s.pad_clamp) {
+    } else if (m & rvkr::kModePadClamp) {
         t = clamp01(t);
     }
     return t;
 }
 
 // gradient coordinate t of a linear or concentric radial shader at pixel (px, py)
-__device__ __forceinline__ float grad_t(const ShaderHead &s, int32_t px, int32_t py) {
+__device__ __forceinline__ float grad_t(const ShaderHead &s, uint32_t m, int32_t px, int32_t py) {
     const float x = (float)px + 0.5f, y = (float)py + 0.5f;
     const float tx = x * s.sx + (y * s.kx + s.tx);
     float t = tx;
-    if (s.radial) {
+    if (m & rvkr::kModeRadial) {
         const float ty = x * s.ky + (y * s.sy + s.ty);
         t = sqrtf(tx * tx + ty * ty);
-        if (s.r_twist) t = (t - s.r_scale) / s.r_bias;
+        if (m & rvkr::kModeRTwist) t = (t - s.r_scale) / s.r_bias;
     }
-    return spread_t(s, t);
+    return spread_t(m, t);
 }
 
 // gradient coordinate t of a two-point conical shader; `degenerate` is set where the
@@ -755,40 +756,41 @@ __device__ __forceinline__ float grad_t_focal(const ShaderHead &s, int32_t px, i
         degenerate = t <= 0.0f || t != t;
         if (degenerate) t = 0.0f;
     }
-    return spread_t(s, t);
+    return spread_t(s.mode, t);
 }
 
 // the gradient's colour at t (tiny-skia's gradient / evenly_spaced_2_stop_gradient
 // stages, f32, before any clamp)
-__device__ __forceinline__ void grad_color(const ShaderParams &sp, const ShaderHead &s, float t, float c[4]) {
-    if (s.two_stop) {
+__device__ __forceinline__ void grad_color(const ShaderParams &sp, uint32_t m, float t, float c[4]) {
+    if (m & rvkr::kModeTwoStop) {
         for (int k = 0; k < 4; k++) c[k] = t * sp.f[0][k] + sp.b[0][k];
     } else {
         // the thresholds are read four at a time (t has kMaxStops + 2 entries)
+        const uint32_t nstops = m >> rvkr::kModeStopsShift;
         uint32_t idx = 0;
-        for (uint32_t i = 1; i < s.nstops; i += 4) {
+        for (uint32_t i = 1; i < nstops; i += 4) {
             float q[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) q[k] = sp.t[min(i + (uint32_t)k, (uint32_t)kMaxStops + 1u)];
 #pragma unroll
-            for (int k = 0; k < 4; k++) idx += i + (uint32_t)k < s.nstops && t >= q[k] ? 1u : 0u;
+            for (int k = 0; k < 4; k++) idx += i + (uint32_t)k < nstops && t >= q[k] ? 1u : 0u;
         }
         for (int k = 0; k < 4; k++) c[k] = t * sp.f[idx][k] + sp.b[idx][k];
     }
 }
 
 // premultiplied u8 source colour of a shader at pixel (px, py), packed (tiny-skia lowp)
-__device__ __forceinline__ uint32_t shade(const ShaderParams &sp, const ShaderHead &s, int32_t px, int32_t py) {
-    if (s.kind == 1) return s.rgba;
+__device__ __forceinline__ uint32_t shade(const ShaderParams &sp, uint32_t m, int32_t px, int32_t py) {
+    if ((m & rvkr::kModeKind) == 1u) return sp.rgba;
     float c[4];
-    grad_color(sp, s, grad_t(s, px, py), c);
+    grad_color(sp, m, grad_t(sp, m, px, py), c);
     uint32_t o = 0;
     for (int k = 0; k < 4; k++) {
         const float v = (c[k] != c[k]) ? 0.0f : clamp01(c[k]);
         o |= (uint32_t)(v * 255.0f + 0.5f) << (8 * k);
     }
     // premultiply: every colour channel times alpha / 255 (alpha's own field is empty)
-    if (!s.opaque) o = scale_px(o & 0x00ffffffu, o >> 24) | (o & 0xff000000u);
+    if (!(m & rvkr::kModeOpaque)) o = scale_px(o & 0x00ffffffu, o >> 24) | (o & 0xff000000u);
     return o;
 }
 
@@ -893,7 +895,7 @@ __device__ __noinline__ uint32_t paint_px_hp(const ShaderParams &sp, const Shade
         bool degenerate;
         const float t = grad_t_focal(s, px, py, degenerate);
         float col[4];
-        grad_color(sp, s, t, col);
+        grad_color(sp, s.mode, t, col);
         if (!s.opaque) { col[0] *= col[3]; col[1] *= col[3]; col[2] *= col[3]; }
         src = degenerate ? F4{0.0f, 0.0f, 0.0f, 0.0f} : F4{col[0], col[1], col[2], col[3]};
     }
@@ -921,13 +923,14 @@ __device__ __forceinline__ uint32_t over_lowp(uint32_t src, uint32_t cov, uint32
 template <bool kPatterns>
 __device__ __forceinline__ uint32_t paint_px(const ShaderParams &sp, const ShaderHead &sh, int32_t lx, int32_t ly,
                                              uint32_t c, uint32_t d, uint32_t &src, bool &shaded) {
-    if (sh.clear) return scale_px(d, 255u - c);
-    if (sh.f32_pipeline) return paint_px_hp<kPatterns>(sp, sh, lx, ly, c, d);
+    const uint32_t m = sh.mode;
+    if (m & rvkr::kModeClear) return scale_px(d, 255u - c);
+    if (m & rvkr::kModeF32) return paint_px_hp<kPatterns>(sp, sh, lx, ly, c, d);
     if (!shaded) {
-        src = shade(sp, sh, lx, ly);
+        src = shade(sp, m, lx, ly);
         shaded = true;
     }
-    return over_lowp(src, c, d, sh.as_source != 0);
+    return over_lowp(src, c, d, (m & rvkr::kModeAsSource) != 0);
 }
 
 // tile-table entry of target tile (tx, ty) in a job's grid (origin gx0, gy0, gw x gh

@@ -742,6 +742,13 @@ static bool poly_to_unit_x(float x0, float y0, float x1, float y1, Ts &out) {
     return true;
 }
 
+void seal_shader(ShaderHead &s) {
+    s.mode = (s.kind & kModeKind) | (s.radial ? kModeRadial : 0u) | ((s.spread & 3u) << kModeSpreadShift) |
+             (s.pad_clamp ? kModePadClamp : 0u) | (s.two_stop ? kModeTwoStop : 0u) | (s.opaque ? kModeOpaque : 0u) |
+             (s.r_twist ? kModeRTwist : 0u) | (s.as_source ? kModeAsSource : 0u) | (s.clear ? kModeClear : 0u) |
+             (s.f32_pipeline ? kModeF32 : 0u) | (s.nstops << kModeStopsShift);
+}
+
 bool make_shader(const rvk_scene &sc, const rvk_paint &p, Ts ts, ShaderParams &s, float extra_opacity) {
     s = ShaderParams{};
     // Shader::apply_opacity (Color::apply_opacity on the colour or on every stop)

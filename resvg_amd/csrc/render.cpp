@@ -621,6 +621,7 @@ struct Renderer {
         uint32_t slot = 0;
         RVK_TRY(rvk::staging_acquire(ctx, meta, &hostv, &slot));
         uint8_t *host = (uint8_t *)hostv;
+        for (ShaderParams &sp : dl.shaders) seal_shader(sp);
         if (!dl.shaders.empty()) std::memcpy(host + o_sh, dl.shaders.data(), dl.shaders.size() * sizeof(ShaderParams));
         std::memcpy(host + o_it, dl.items.data(), nitems * sizeof(DrawItem));
         {
