@@ -20,7 +20,7 @@ NAMES = [
     "feGaussianBlur/simple-case", "feGaussianBlur/small-stdDeviation", "feGaussianBlur/stdDeviation=5-0",
     "feOffset/simple-case", "feOffset/negative-offset",
     "feTile/simple-case", "feTile/empty-region", "feTile/with-region", "feTile/with-subregion-1",
-    "feTile/with-subregion-2", "feTile/with-subregion-3",
+    "feTile/with-subregion-2", "feTile/with-subregion-3", "feTile/complex-transform",
     "feComposite/operator=arithmetic", "feComposite/operator=over", "feComposite/operator=in",
     "feComposite/operator=xor",
     "feBlend/mode=multiply", "feBlend/mode=screen", "feBlend/mode=hue", "feBlend/mode=color-burn",

@@ -614,7 +614,14 @@ def _tile_subregion_fixtures():
     """filters/feTile/with-subregion-1/-2/-3: two filtered rects (20, 20, 160, 160),
     each filter a 10x10 flood moved by feOffset (5, 5) and tiled; the first in grey
     over the whole filter region, the second in seagreen over the subregion (50, 50,
-    100, 100).  In -3 the flood's x = y = 2% of the 200-unit viewport, i.e. 4."""
+    100, 100).  In -3 the flood's x = y = 2% of the 200-unit viewport, i.e. 4.  The
+    SVG's guide rect over the subregion is stroked as the SVG strokes it, so the
+    subregion's edge pixels are compared too.
+
+    filters/feTile/complex-transform: the same flood (at 28, 28), offset and tile on a
+    rect (40, 30, 90, 75) under skewX(30) rotate(30) (usvg parses the transform in f64
+    and stores it in f32); the default filter region, -10% / 120% of the rect's
+    bounding box in its user space; the guide rect and the frame stroked."""
     def tiled(color, at, tile_rect=None):
         g = FilterGraph(tuple(float(np.float32(v)) for v in (4.0, 4.0, 192.0, 192.0)))
         f = g.flood(color, rect=(at, at, 10.0, 10.0), cs=abi.CI_SRGB)
@@ -630,11 +637,22 @@ def _tile_subregion_fixtures():
             g = sc.group(sc.root, filters=[graph])
             v, p = rect_path(20.0, 20.0, 160.0, 160.0)
             sc.path(g, v, p, fill=PaintSpec.solid((0, 0, 0)))
+        sc.path(sc.root, *rect_path(50.0, 50.0, 100.0, 100.0), stroke=PaintSpec.solid(BLACK))
         out[f"feTile/{name}"] = sc
+
+    g = FilterGraph((31.0, 22.5, 108.0, 90.0))
+    f = g.flood((46, 139, 87), rect=(28.0, 28.0, 10.0, 10.0), cs=abi.CI_SRGB)
+    o = g.offset(5.0, 5.0, in1=f, cs=abi.CI_SRGB)
+    g.tile(in1=o, cs=abi.CI_SRGB)
+    t, c, s = math.tan(math.radians(30.0)), math.cos(math.radians(30.0)), math.sin(math.radians(30.0))
+    ts = tuple(f32(v) for v in (c + t * s, s, -s + t * c, c, 0.0, 0.0))  # skewX(30) * rotate(30)
+    sc = Scene(200, 200)
+    grp = sc.group(sc.root, transform=ts, filters=[g])
+    sc.path(grp, *rect_path(40.0, 30.0, 90.0, 75.0), fill=PaintSpec.solid((0, 0, 0)))
+    for r in ((28.0, 28.0, 144.0, 144.0), (1.0, 1.0, 198.0, 198.0)):
+        sc.path(sc.root, *rect_path(*r), stroke=PaintSpec.solid(BLACK))
+    out["feTile/complex-transform"] = sc
     return out
-
-
-TILE_SUBREGION_GUIDES = ((50.0, 50.0, 100.0, 100.0),)
 
 
 def _fe_image_fixtures():
@@ -957,9 +975,7 @@ def all_scene_fixtures():
 
 def fixture_guides(name: str) -> tuple:
     """User-space rects that a fixture SVG strokes over its filtered element (left
-    out of the PNG comparison; only the filter fixtures have them)."""
-    if name.startswith("feTile/with-subregion"):
-        return TILE_SUBREGION_GUIDES
+    out of the PNG comparison; only the direct filter fixtures have them)."""
     return {fx.name: fx.guides for fx in filter_fixtures()}.get(name, ())
 
 
