@@ -100,10 +100,24 @@ struct Exec {
                 if (it->first == in) return it->second;
             in = RVK_INPUT_SOURCE_GRAPHIC;
         }
-        Img c = clone_img(source);
+        if (in == RVK_INPUT_SOURCE_ALPHA) {
+            Img c = clone_img(source);
+            c->region = region;
+            c->cs = RVK_CI_SRGB;
+            FTRY(rvk::launch_zero_rgb(ctx, c->px, (size_t)c->w * c->h));
+            return c;
+        }
+        // SourceGraphic: the reference clones the layer for every use; here each use
+        // gets an unowned view of it, which take() copies the first time a primitive
+        // writes to it (the layer itself is not written until the graph has run)
+        auto c = std::make_shared<DImg>();
+        c->ctx = ctx;
+        c->px = source->px;
+        c->w = source->w;
+        c->h = source->h;
+        c->owned = false;
         c->region = region;
         c->cs = RVK_CI_SRGB;
-        if (in == RVK_INPUT_SOURCE_ALPHA) FTRY(rvk::launch_zero_rgb(ctx, c->px, (size_t)c->w * c->h));
         return c;
     }
 
@@ -357,7 +371,9 @@ rvk_status filter_apply(rvk_context *ctx, const rvk_filter *filter, const rvk_fi
         Img res = ex.run(*filter);
         ex.source.reset();
         res = into_cs(std::move(res), RVK_CI_SRGB);
-        if (res->w == w && res->h == h) {
+        if (res->px == px) {
+            // the result is the layer itself (SourceGraphic passed through unchanged)
+        } else if (res->w == w && res->h == h) {
             // SourceOver onto a cleared pixmap of the same size is a copy
             RVK_TRY(launch_copy(ctx, px, res->px, res->bytes()));
         } else {
