@@ -727,9 +727,9 @@ __device__ __forceinline__ float spread_t(uint32_t m, float t) {
     return t;
 }
 
-// gradient coordinate t of a linear or concentric radial shader at pixel (px, py)
-__device__ __forceinline__ float grad_t(const ShaderHead &s, uint32_t m, int32_t px, int32_t py) {
-    const float x = (float)px + 0.5f, y = (float)py + 0.5f;
+// gradient coordinate t of a linear or concentric radial shader at the pixel centre
+// (x, y)
+__device__ __forceinline__ float grad_t(const ShaderHead &s, uint32_t m, float x, float y) {
     const float tx = x * s.sx + (y * s.kx + s.tx);
     float t = tx;
     if (m & rvkr::kModeRadial) {
@@ -779,14 +779,15 @@ __device__ __forceinline__ void grad_color(const ShaderParams &sp, uint32_t m, f
     }
 }
 
-// premultiplied u8 source colour of a shader at pixel (px, py), packed (tiny-skia lowp)
-__device__ __forceinline__ uint32_t shade(const ShaderParams &sp, uint32_t m, int32_t px, int32_t py) {
+// premultiplied u8 source colour of a shader at the pixel centre (x, y), packed
+// (tiny-skia lowp)
+__device__ __forceinline__ uint32_t shade(const ShaderParams &sp, uint32_t m, float x, float y) {
     if ((m & rvkr::kModeKind) == 1u) return sp.rgba;
     float c[4];
-    grad_color(sp, m, grad_t(sp, m, px, py), c);
+    grad_color(sp, m, grad_t(sp, m, x, y), c);
     uint32_t o = 0;
     for (int k = 0; k < 4; k++) {
-        const float v = (c[k] != c[k]) ? 0.0f : clamp01(c[k]);
+        const float v = clamp01(c[k]);  // fmaxf(NaN, 0) = 0
         o |= (uint32_t)(v * 255.0f + 0.5f) << (8 * k);
     }
     // premultiply: every colour channel times alpha / 255 (alpha's own field is empty)
@@ -916,18 +917,19 @@ __device__ __forceinline__ uint32_t over_lowp(uint32_t src, uint32_t cov, uint32
     return src + scale_px(d, 255u - (src >> 24));
 }
 
-// one covered pixel of a draw item: Clear, or the shaded colour composited with
-// over_lowp, or for a shader of the f32 pipeline (two-point conical gradients,
-// patterns) in f32 (scale_1_float by c / 255, SourceOver, store); the colour is
-// shaded only when needed
+// one covered pixel (lx, ly), centre (cx, cy), of a draw item: Clear, or the shaded
+// colour composited with over_lowp, or for a shader of the f32 pipeline (two-point
+// conical gradients, patterns) in f32 (scale_1_float by c / 255, SourceOver, store);
+// the colour is shaded only when needed
 template <bool kPatterns>
 __device__ __forceinline__ uint32_t paint_px(const ShaderParams &sp, const ShaderHead &sh, int32_t lx, int32_t ly,
-                                             uint32_t c, uint32_t d, uint32_t &src, bool &shaded) {
+                                             float cx, float cy, uint32_t c, uint32_t d, uint32_t &src,
+                                             bool &shaded) {
     const uint32_t m = sh.mode;
     if (m & rvkr::kModeClear) return scale_px(d, 255u - c);
     if (m & rvkr::kModeF32) return paint_px_hp<kPatterns>(sp, sh, lx, ly, c, d);
     if (!shaded) {
-        src = shade(sp, m, lx, ly);
+        src = shade(sp, m, cx, cy);
         shaded = true;
     }
     return over_lowp(src, c, d, (m & rvkr::kModeAsSource) != 0);
@@ -1015,6 +1017,7 @@ __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t
     const int32_t ttx = otx + (int32_t)bx * kWavesX + wv % kWavesX, tty = oty + (int32_t)by * (TY / kWaveH) + wv / kWavesX;
     const int32_t x = ttx * kWaveW + lane % kWaveW, y = tty * kWaveH + lane / kWaveW;
     const uint32_t pix = (uint32_t)lane;  // = (y % 8) * 8 + x % 8
+    const float fx = (float)x + 0.5f, fy = (float)y + 0.5f;
     const bool inside = x < (int32_t)w && y < (int32_t)h;
     uint32_t d = inside ? px[(size_t)y * w + x] : 0u;
     const uint32_t d0 = d;
@@ -1039,6 +1042,8 @@ __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t
             const DrawItem item = a.items[base + (uint32_t)kl];  // with its jobs' grids and tables
             if (!inside) continue;
             const int32_t lx = x - item.lx, ly = y - item.ly;
+            // the pixel centre in layer coordinates, exact in f32 (|x|, |lx| < 2^23)
+            const float cx = fx - (float)item.lx, cy = fy - (float)item.ly;
             const uint32_t cf = tile_cov(item.fj, a.arena, efk, tty, pix);
             uint32_t cs = 0, hb = 0, he = 0;
             if (item.stroke_hair) {
@@ -1064,12 +1069,12 @@ __global__ __launch_bounds__(256) void k_draw(uint32_t *px, uint32_t w, uint32_t
                         // every blit of the hairline at this pixel, composited in order
                         for (uint32_t kk = hb; kk < he; kk++) {
                             const uint32_t c = hair_alpha(a.hair_lines[a.hair_bins[kk]], lx, ly);
-                            if (c) target = paint_px<kPatterns>(sp, sh, lx, ly, c, target, src, shaded);
+                            if (c) target = paint_px<kPatterns>(sp, sh, lx, ly, cx, cy, c, target, src, shaded);
                         }
                         continue;
                     }
                     const uint32_t c = stroke ? cs : cf;
-                    if (c) target = paint_px<kPatterns>(sp, sh, lx, ly, c, target, src, shaded);
+                    if (c) target = paint_px<kPatterns>(sp, sh, lx, ly, cx, cy, c, target, src, shaded);
                 }
                 if (!item.layer) {
                     d = target;
