@@ -334,3 +334,42 @@ def diff_scene(seed: int = 0, size: int = 1024, n_paths: int = 256, segments: in
         o = target[i * PATH_PARAMS + 20:i * PATH_PARAMS + 24]
         target[i * PATH_PARAMS + 20:i * PATH_PARAMS + 24] = np.sort(o)
     return size, size, paths, ends, flat, target
+
+
+def main(argv=None) -> int:
+    """Writes a seeded workload document as SVG (so that the reference can render it):
+    ``python -m resvg_amd.workloads full-filter --size 4096 --paths 2000 --seed 0 -o c2.svg``;
+    ``batch`` writes documents ``--first`` .. ``--first + --count - 1`` of the batched
+    workload into the directory given by ``-o``."""
+    import argparse
+    from pathlib import Path
+
+    from .svg_export import scene_to_svg
+
+    ap = argparse.ArgumentParser(prog="python -m resvg_amd.workloads", description=main.__doc__)
+    ap.add_argument("doc", choices=["blur-offset-composite", "full-filter", "batch"])
+    ap.add_argument("-o", "--out", required=True)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--size", type=int, default=None)
+    ap.add_argument("--paths", type=int, default=2000)
+    ap.add_argument("--docs", type=int, default=4096, help="batch: documents in the workload")
+    ap.add_argument("--first", type=int, default=0)
+    ap.add_argument("--count", type=int, default=1)
+    a = ap.parse_args(argv)
+    if a.doc == "batch":
+        out = Path(a.out)
+        out.mkdir(parents=True, exist_ok=True)
+        specs = document_specs(a.docs, a.seed)
+        for i in range(a.first, min(a.first + a.count, len(specs))):
+            (out / f"doc_{i:04d}.svg").write_text(scene_to_svg(batch_doc(*specs[i])))
+        return 0
+    if a.doc == "blur-offset-composite":
+        sc, _ = blur_offset_composite_doc(a.seed, a.size or 512)
+    else:
+        sc, _ = full_filter_doc(a.seed, a.size or 4096, a.paths)
+    Path(a.out).write_text(scene_to_svg(sc))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
