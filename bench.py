@@ -16,9 +16,10 @@ the oracle's CPU throughput on a bounded sample of the same document (with the
 per-channel mismatch histogram of two seeded documents against the oracle), and under
 "batched" the documents/s of rvk_render_batch against a loop of rvk_render calls on a
 sample of the batched workload, under "differentiable" the f32 forward and
-forward+backward times of the differentiable form against the plain u8 forward, and
+forward+backward times of the differentiable form against the plain u8 forward,
 under "batched_fitting" the steps/s and final PSNR of 64 independent 512x512 fits
-run as one batch.
+run as one batch, and under "png_output" the PNG encoding of the rendered document
+(the device's demultiply + row-filter kernel against HBM peak, the host's deflate).
 """
 from __future__ import annotations
 
@@ -351,6 +352,37 @@ def fitting(ctx, count, size, n_paths, steps, seed):
     return res
 
 
+def png_output(ctx, px, reps):
+    """PNG output of the rendered document (tiny-skia Pixmap::encode_png): the device's
+    demultiply + per-row filter kernel, timed with HIP events over `reps` launches
+    (8 algorithmic bytes per pixel), then the host's deflate (zlib level 6, in parallel
+    1 MiB pieces) of its rows and the whole Context.encode_png call."""
+    import torch
+
+    h, w = px.shape[:2]
+    rows = torch.empty(h * (4 * w + 1), dtype=torch.uint8, device=px.device)
+    ctx.png_filter_rows(px, rows)  # warm-up
+    ctx.synchronize()
+    ctx.profile_begin()
+    for _ in range(reps):
+        ctx.png_filter_rows(px, rows)
+    ks = {k["name"]: k for k in ctx.profile_end()}
+    k = ks["png_rows"]
+    ms = k["total_ms"] / k["launches"]
+    gbs = k["bytes"] / k["launches"] / (ms / 1e3) / 1e9
+    host_rows = rows.cpu().numpy().tobytes()
+    from resvg_amd import png
+    t0 = time.perf_counter()
+    deflated = png.deflate(host_rows, 6)
+    deflate_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    data = ctx.encode_png(px)
+    encode_ms = (time.perf_counter() - t0) * 1e3
+    return {"rows_kernel_ms": round(ms, 4), "rows_GBps": round(gbs, 1), "rows_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "deflate_ms": round(deflate_ms, 1), "encode_png_ms": round(encode_ms, 1),
+            "file_MB": round(len(data) / 1e6, 2), "idat_MB": round(len(deflated) / 1e6, 2)}
+
+
 def _primitive_rooflines(kernels, steps):
     """Per filter primitive / pipeline stage: its kernels' time per step and the
     achieved fraction of HBM peak on the primitive's algorithmic bytes."""
@@ -391,6 +423,7 @@ def main() -> int:
     ap.add_argument("--fit-size", type=int, default=512)
     ap.add_argument("--fit-paths", type=int, default=128)
     ap.add_argument("--fit-steps", type=int, default=500)
+    ap.add_argument("--png-reps", type=int, default=3, help="timed launches of the PNG-output leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 1 GiB device buffer once (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -449,6 +482,7 @@ def main() -> int:
     st = ctx.stats()  # of the last render call
     if cpu is not None:
         cpu["parity"] = cpu_parity(ctx, args.seed)
+    pngo = png_output(ctx, px, args.png_reps) if args.png_reps > 0 else None
     bat = batched(ctx, args.batch_docs, args.seed, args.batch_steps) if args.batch_docs > 0 else None
     dif = differentiable(ctx, args.diff_size, args.diff_paths, args.seed, 5) if args.diff_size > 0 else None
     fitr = fitting(ctx, args.fit_scenes, args.fit_size, args.fit_paths, args.fit_steps, args.seed) \
@@ -511,6 +545,7 @@ def main() -> int:
         "primitives": _primitive_rooflines(kernels, args.steps),
         "wall_s_per_step": round(t_wall / args.steps, 3),
         "host_last_step": host,
+        "png_output": pngo,
         "batched": bat,
         "differentiable": dif,
         "batched_fitting": fitr,

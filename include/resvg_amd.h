@@ -388,6 +388,17 @@ rvk_status rvk_render(rvk_context *ctx, const rvk_scene *scene, rvk_transform tr
  * renders, downloads and synchronizes. */
 rvk_status rvk_render_host(rvk_context *ctx, const rvk_scene *scene, rvk_transform transform,
                            uint32_t width, uint32_t height, uint8_t *host_pixmap);
+/* PNG output, the device part of tiny-skia's Pixmap::encode_png / save_png that
+ * resvg's CLI writes its result with (crates/resvg/src/main.rs:104-117): every pixel
+ * of the premultiplied RGBA8 pixmap `dev_px` demultiplied like
+ * PremultipliedColorU8::demultiply (c / (a / 255) + 0.5 in f64, cast to u8; restated
+ * by the reference's test harness, crates/resvg/tests/integration/main.rs:204-211),
+ * each row PNG-filtered with the filter type (0-4) of least signed-magnitude sum, and
+ * written to `dev_out` as the IDAT stream's uncompressed data: per row one filter-type
+ * byte, then 4 * width bytes (height * (4 * width + 1) bytes in all, device memory).
+ * The caller deflates it and adds the chunks (resvg_amd.Context.encode_png). */
+rvk_status rvk_png_filter_rows(rvk_context *ctx, const uint8_t *dev_px, uint32_t width, uint32_t height,
+                               uint8_t *dev_out);
 /* resvg::render_node (lib.rs:55-70) / resvg_render_node (crates/c-api/lib.rs:907-942)
  * onto a device pixmap.  `node` is the group or path to draw (the FFI crate resolves
  * usvg's node id to it); `abs_layer_bbox` is usvg's Node::abs_layer_bounding_box()

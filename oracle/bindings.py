@@ -213,6 +213,19 @@ def edge_clip(segs: np.ndarray, w: int, h: int) -> np.ndarray:
     return out[:n]
 
 
+def png_filter_rows(px: np.ndarray) -> np.ndarray:
+    """The IDAT stream's uncompressed rows of Pixmap::encode_png for an (H, W, 4)
+    premultiplied RGBA8 array: H rows of a filter-type byte and 4W filtered bytes."""
+    px = np.ascontiguousarray(px, dtype=np.uint8)
+    h, w = px.shape[:2]
+    out = np.zeros(h * (4 * w + 1), np.uint8)
+    L = lib()
+    L.orc_png_filter_rows.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+    L.orc_png_filter_rows.restype = None
+    L.orc_png_filter_rows(px.ctypes.data, w, h, out.ctypes.data)
+    return out
+
+
 def fill_path(scene_abi, path_index: int, ts: abi.Transform, a) -> int:
     return lib().orc_fill_path(C.byref(scene_abi), path_index, ts, _p(a), *_wh(a))
 

@@ -69,6 +69,8 @@ void orc_apply_mask(uint8_t *px, uint32_t w, uint32_t h, const uint8_t *mask);
 // floats (point count 2..4, then x0 y0 .. x3 y3); writes the clipped pieces in the
 // same layout and returns their number, -1 on a bad count or too small an output
 int32_t orc_edge_clip(const float *in, uint32_t n, uint32_t w, uint32_t h, float *out, uint32_t cap);
+// the IDAT stream's rows of tiny-skia's Pixmap::encode_png (png.cpp): h * (4w + 1) bytes
+void orc_png_filter_rows(const uint8_t *px, uint32_t w, uint32_t h, uint8_t *out);
 // the u8 coverage mask of a path (for coverage-only tests)
 int32_t orc_path_coverage(const rvk_scene *scene, uint32_t path_index, rvk_transform ts, uint32_t stroke,
                           uint8_t *cov, uint32_t w, uint32_t h);

@@ -70,6 +70,7 @@ def _declare(L: C.CDLL) -> None:
         "rvk_fill": ([P, P, u32, u32, C.POINTER(C.c_uint8)], i32),
         "rvk_filter_apply": ([P, C.POINTER(abi.Filter), C.POINTER(abi.FilterData), abi.Transform, P, u32, u32,
                               C.POINTER(i32)], i32),
+        "rvk_png_filter_rows": ([P, P, u32, u32, P], i32),
         "rvk_get_stats": ([P, C.POINTER(abi.Stats)], i32),
         "rvk_profile_begin": ([P], i32),
         "rvk_profile_end": ([P, C.POINTER(abi.KernelStat), C.c_uint32, C.POINTER(C.c_uint32)], i32),
@@ -287,6 +288,29 @@ class Context:
                "rvk_filter_apply")
         del keep
         return err.value
+
+    # --- PNG output (tiny-skia Pixmap::encode_png) ---
+    def png_filter_rows(self, px, out):
+        """rvk_png_filter_rows: the IDAT stream's uncompressed rows of a (H, W, 4) uint8
+        device tensor written to `out`, a contiguous uint8 device tensor of at least
+        H * (4W + 1) bytes."""
+        w, h = _dims(px)
+        if out.dtype.itemsize != 1 or not out.is_contiguous() or out.numel() < h * (4 * w + 1):
+            raise ValueError("out must be a contiguous uint8 tensor of H * (4W + 1) bytes")
+        _check(lib().rvk_png_filter_rows(self.h, _ptr(px), w, h, _ptr(out)), "rvk_png_filter_rows")
+
+    def encode_png(self, px, level: int = 6) -> bytes:
+        """Pixmap::encode_png of a premultiplied (H, W, 4) uint8 device tensor: the
+        device demultiplies and filters the rows, the host deflates them (zlib
+        `level`) and returns the PNG file's bytes."""
+        import torch
+
+        from . import png
+        w, h = _dims(px)
+        rows = torch.empty(h * (4 * w + 1), dtype=torch.uint8, device=px.device)
+        self.png_filter_rows(px, rows)
+        self.synchronize()
+        return png.from_rows(w, h, rows.cpu().numpy().tobytes(), level)
 
     # --- rendering ---
     def render(self, scene, ts: abi.Transform, pixmap):

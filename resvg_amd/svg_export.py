@@ -15,12 +15,10 @@ data after demultiplying their premultiplied pixels.
 from __future__ import annotations
 
 import base64
-import struct
-import zlib
 
 import numpy as np
 
-from . import abi
+from . import abi, png
 
 _JOIN = {abi.JOIN_MITER: "miter", abi.JOIN_MITER_CLIP: "miter-clip", abi.JOIN_ROUND: "round",
          abi.JOIN_BEVEL: "bevel"}
@@ -77,13 +75,7 @@ def png_bytes(rgba: np.ndarray) -> bytes:
     """A PNG file (8-bit RGBA, filter type None, zlib level 6) of straight-alpha pixels."""
     rgba = np.ascontiguousarray(rgba, np.uint8)
     h, w, _ = rgba.shape
-    raw = b"".join(b"\x00" + rgba[y].tobytes() for y in range(h))
-
-    def chunk(tag: bytes, data: bytes) -> bytes:
-        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
-
-    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)) +
-            chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
+    return png.from_rows(w, h, b"".join(b"\x00" + rgba[y].tobytes() for y in range(h)))
 
 
 def demultiply(px: np.ndarray) -> np.ndarray:
