@@ -174,10 +174,20 @@ rvk_status staging_acquire(rvk_context *ctx, size_t bytes, void **host, uint32_t
 }
 
 rvk_status upload_staged(rvk_context *ctx, uint32_t slot, void *dev_dst, size_t bytes) {
+    return upload_staged_on(ctx, slot, dev_dst, bytes, ctx->stream);
+}
+
+rvk_status upload_staged_on(rvk_context *ctx, uint32_t slot, void *dev_dst, size_t bytes, hipStream_t stream) {
     rvk_staging &sb = ctx->staging[slot];
-    RVK_HIP_CHECK(hipMemcpyAsync(dev_dst, sb.host, bytes, hipMemcpyHostToDevice, ctx->stream));
-    RVK_HIP_CHECK(hipEventRecord(sb.done, ctx->stream));
+    RVK_HIP_CHECK(hipMemcpyAsync(dev_dst, sb.host, bytes, hipMemcpyHostToDevice, stream));
+    RVK_HIP_CHECK(hipEventRecord(sb.done, stream));
     sb.pending = true;
+    return RVK_OK;
+}
+
+rvk_status side_stream(rvk_context *ctx, hipStream_t *out) {
+    if (!ctx->side) RVK_HIP_CHECK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    *out = ctx->side;
     return RVK_OK;
 }
 
@@ -241,6 +251,10 @@ rvk_status rvk_context_destroy(rvk_context *ctx) {
     for (auto &sb : ctx->staging) {
         if (sb.host) (void)hipHostFree(sb.host);
         if (sb.done) (void)hipEventDestroy(sb.done);
+    }
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamDestroy(ctx->side);
     }
     for (auto &f : ctx->free_list) (void)hipFree(f.second);
     for (auto &f : ctx->live) (void)hipFree(f.second);

@@ -254,6 +254,55 @@ struct Renderer {
     const uint64_t budget = arena_budget();
     bool parallel_pack = true;  // flush packs large draw lists on the host thread pool
 
+    // Flushes alternate between two device blocks.  A flush's upload, curve expansion
+    // and coverage run on the context's side stream and its compositing on the
+    // context stream, which waits for them; the side stream waits until the draw of
+    // the flush before last, the previous reader of the block, is done (or, for a
+    // block just taken from the scratch cache, until the context stream's work so far
+    // is), so the coverage of a flush overlaps the compositing of the one before.
+    void *blk[2] = {nullptr, nullptr};
+    size_t blk_cap[2] = {0, 0};
+    hipEvent_t blk_free[2] = {nullptr, nullptr};  // recorded on the context stream after a block's draw
+    bool blk_used[2] = {false, false};
+    hipEvent_t ev_cov = nullptr, ev_main = nullptr, ev_side = nullptr;
+    uint64_t nflush = 0;
+    bool side_used = false;
+
+    ~Renderer() { (void)finish_flushes(); }
+
+    // Joins the side stream into the context stream and returns the blocks to the
+    // scratch cache (the context stream's later work is ordered after every use).
+    rvk_status finish_flushes() {
+        rvk_status st = RVK_OK;
+        if (side_used) {
+            if (hipEventRecord(ev_side, ctx->side) != hipSuccess ||
+                hipStreamWaitEvent(ctx->stream, ev_side, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                (void)hipStreamSynchronize(ctx->side);  // the join failed: wait for the side stream
+                st = RVK_ERR_HIP;
+            }
+            side_used = false;
+        }
+        for (int b = 0; b < 2; b++) {
+            if (blk[b]) rvk::scratch_release(ctx, blk[b]);
+            blk[b] = nullptr;
+            blk_cap[b] = 0;
+            blk_used[b] = false;
+        }
+        for (hipEvent_t *e : {&blk_free[0], &blk_free[1], &ev_cov, &ev_main, &ev_side})
+            if (*e) {
+                (void)hipEventDestroy(*e);
+                *e = nullptr;
+            }
+        return st;
+    }
+
+    rvk_status flush_events() {
+        for (hipEvent_t *e : {&blk_free[0], &blk_free[1], &ev_cov, &ev_main, &ev_side})
+            if (!*e) RVK_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        return RVK_OK;
+    }
+
     // The walk runs twice: kCollect records every path drawn (with its device
     // transform), the geometry is computed in parallel, and kExecute replays the
     // walk consuming it in the same order (kSkip steps over a skipped subtree).
@@ -696,9 +745,22 @@ struct Renderer {
             }
             ctx->stats.validated_flushes++;
         }
-        void *dev = nullptr;
-        RVK_TRY(rvk::scratch_alloc(ctx, o_arena + dl.arena, &dev));
-        uint8_t *d = (uint8_t *)dev;
+        // the flush's block (see blk), replaced by a larger one from the scratch cache
+        // when it is too small (the old one's last reader is on the context stream)
+        const int b = (int)(nflush++ & 1);
+        const size_t need = o_arena + dl.arena;
+        RVK_TRY(flush_events());
+        bool fresh = false;
+        if (blk_cap[b] < need) {
+            if (blk[b]) rvk::scratch_release(ctx, blk[b]);
+            blk[b] = nullptr;
+            blk_cap[b] = 0;
+            blk_used[b] = false;
+            RVK_TRY(rvk::scratch_alloc(ctx, need, &blk[b]));
+            blk_cap[b] = need;
+            fresh = true;
+        }
+        uint8_t *d = (uint8_t *)blk[b];
         // algorithmic bytes: expansion reads each edge once and writes its pieces;
         // coverage reads edges, windows and pieces once and writes the tile tables (the
         // partial tiles' 64 bytes each, written by coverage and read by the compositor,
@@ -722,19 +784,39 @@ struct Renderer {
         ctx->stats.edge_pieces += dl.piece_slots;
         ctx->stats.band_entries += dl.n_windows;
         ctx->stats.upload_bytes += meta;
-        rvk_status st = rvk::upload_staged(ctx, slot, d, meta);
-        if (st == RVK_OK)
-            st = rvk::launch_expand(ctx, (const EdgeRec *)(d + o_edges), (uint32_t)dl.n_edges,
-                                    (const EdgeCov *)(d + o_ecov), (PieceDev *)(d + o_pc), (uint32_t *)(d + o_np),
-                                    dev_overflow, exp_bytes);
-        if (st == RVK_OK)
-            st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)njobs,
-                                      (const uint2 *)(d + o_grp[0]), (uint32_t)dl.n_groups[0],
-                                      (const uint2 *)(d + o_grp[1]), (uint32_t)dl.n_groups[1],
-                                      (const uint2 *)(d + o_grp[2]), (uint32_t)dl.n_groups[2],
-                                      (const EdgeCov *)(d + o_ecov), (const uint2 *)(d + o_win),
-                                      (const PieceDev *)(d + o_pc), (const uint32_t *)(d + o_np),
-                                      (uint16_t *)(d + o_tab), d + o_arena, dev_overflow, cov_bytes);
+        hipStream_t side = nullptr;
+        RVK_TRY(rvk::side_stream(ctx, &side));
+        side_used = true;
+        if (fresh) {
+            RVK_HIP_CHECK(hipEventRecord(ev_main, ctx->stream));
+            RVK_HIP_CHECK(hipStreamWaitEvent(side, ev_main, 0));
+        } else if (blk_used[b]) {
+            RVK_HIP_CHECK(hipStreamWaitEvent(side, blk_free[b], 0));
+        }
+        rvk_status st = rvk::upload_staged_on(ctx, slot, d, meta, side);
+        {
+            // expansion and coverage are launched on the side stream
+            const hipStream_t main_stream = ctx->stream;
+            ctx->stream = side;
+            if (st == RVK_OK)
+                st = rvk::launch_expand(ctx, (const EdgeRec *)(d + o_edges), (uint32_t)dl.n_edges,
+                                        (const EdgeCov *)(d + o_ecov), (PieceDev *)(d + o_pc),
+                                        (uint32_t *)(d + o_np), dev_overflow, exp_bytes);
+            if (st == RVK_OK)
+                st = rvk::launch_coverage(ctx, (const CovJob *)(d + o_jobs), (uint32_t)njobs,
+                                          (const uint2 *)(d + o_grp[0]), (uint32_t)dl.n_groups[0],
+                                          (const uint2 *)(d + o_grp[1]), (uint32_t)dl.n_groups[1],
+                                          (const uint2 *)(d + o_grp[2]), (uint32_t)dl.n_groups[2],
+                                          (const EdgeCov *)(d + o_ecov), (const uint2 *)(d + o_win),
+                                          (const PieceDev *)(d + o_pc), (const uint32_t *)(d + o_np),
+                                          (uint16_t *)(d + o_tab), d + o_arena, dev_overflow, cov_bytes);
+            ctx->stream = main_stream;
+        }
+        if (st == RVK_OK &&
+            (hipEventRecord(ev_cov, side) != hipSuccess || hipStreamWaitEvent(ctx->stream, ev_cov, 0) != hipSuccess)) {
+            (void)hipGetLastError();
+            st = RVK_ERR_HIP;
+        }
         if (st == RVK_OK)
             st = rvk::launch_draw_items(ctx, dl.px, dl.w, dl.h, dl.bx0, dl.by0, dl.bx1, dl.by1,
                                         (const DrawItem *)(d + o_it), (const DrawCull *)(d + o_cull),
@@ -744,7 +826,14 @@ struct Renderer {
                                         std::any_of(dl.shaders.begin(), dl.shaders.end(),
                                                     [](const ShaderParams &q) { return q.kind == 3; }),
                                         draw_bytes);
-        rvk::scratch_release(ctx, dev);
+        if (st == RVK_OK) {
+            if (hipEventRecord(blk_free[b], ctx->stream) != hipSuccess) {
+                (void)hipGetLastError();
+                st = RVK_ERR_HIP;
+            } else {
+                blk_used[b] = true;
+            }
+        }
         ctx->stats.flushes++;
         dl = DrawList{dl.px, dl.w, dl.h};
         return st;
@@ -1155,6 +1244,7 @@ rvk_status render_walk(rvk_context *ctx, const rvk_scene *scene, Ts ts, uint32_t
     root.h = h;
     run(root);
     if (r.err == RVK_OK) r.fail(r.flush(root));
+    r.fail(r.finish_flushes());  // the side stream joins the context stream
     // the one wait of a render: read the overflow counter
     int32_t ov = 0;
     {
@@ -1255,6 +1345,7 @@ rvk_status render_batch(rvk_context *ctx, uint32_t count, const rvk_scene *const
             root.h = hs[d];
             r.render_nodes(scenes[d]->groups[0], ts_from(tss[d]), root);
             if (r.err == RVK_OK) r.fail(r.flush(root));
+            r.fail(r.finish_flushes());
             st[d] = r.err;
             if (r.err == RVK_ERR_HIP || r.err == RVK_ERR_OUT_OF_MEMORY) {  // the stream is unusable
                 int32_t expect = RVK_OK;

@@ -55,6 +55,10 @@ struct rvk_context {
     // Sub-contexts with their own streams, scratch caches and staging rings on the
     // same device; rvk_render_batch spreads documents over them.
     std::vector<rvk_context *> lanes;
+    // A second stream (default priority, created on first use) that a render's
+    // coverage work runs on, so that it overlaps the previous flush's compositing on
+    // `stream`; every render joins it back into `stream` before it returns.
+    hipStream_t side = nullptr;
 };
 
 namespace rvk {
@@ -72,6 +76,10 @@ rvk_status upload_sync(rvk_context *ctx, void *dev_dst, const void *host_src, si
 rvk_status staging_acquire(rvk_context *ctx, size_t bytes, void **host, uint32_t *slot);
 // Enqueues the copy of `bytes` of staging slot `slot` to `dev_dst` (no host wait).
 rvk_status upload_staged(rvk_context *ctx, uint32_t slot, void *dev_dst, size_t bytes);
+// upload_staged on `stream` instead of the context stream.
+rvk_status upload_staged_on(rvk_context *ctx, uint32_t slot, void *dev_dst, size_t bytes, hipStream_t stream);
+// The context's side stream (see rvk_context::side), created on first use.
+rvk_status side_stream(rvk_context *ctx, hipStream_t *out);
 // Copies host memory to the device through the staging ring without waiting.
 rvk_status upload_async(rvk_context *ctx, void *dev_dst, const void *host_src, size_t bytes);
 
