@@ -57,16 +57,17 @@ rvk_status render_walk(rvk_context *ctx, const rvk_scene *scene, rvkh::Ts ts, ui
 
 namespace {
 
-// Coverage bytes per flush (4 GiB: fewer, fuller coverage and compositing launches;
-// on the 4096x4096 full-filter document 24 flushes instead of 48 at 1 GiB, 2.6%
-// faster, the same pixels).  RVK_FLUSH_BYTES overrides it, which tests use to drive
-// many flushes through small documents.
+// Coverage bytes per flush (1 GiB: with a flush's coverage overlapping the previous
+// flush's compositing, the 4096x4096 full-filter document renders in 48 flushes at
+// 152.7 ms, against 162-156 ms at 0.5, 2 and 4 GiB and 221 ms at 256 MiB; the same
+// pixels).  RVK_FLUSH_BYTES overrides it, which tests use to drive many flushes
+// through small documents.
 uint64_t arena_budget() {
     if (const char *e = std::getenv("RVK_FLUSH_BYTES")) {
         const long long v = std::atoll(e);
         if (v > 0) return (uint64_t)v;
     }
-    return 4ull << 30;
+    return 1ull << 30;
 }
 
 struct DrawList {
