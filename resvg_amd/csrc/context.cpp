@@ -107,6 +107,7 @@ rvk_status lanes_prepare(rvk_context *ctx, uint32_t n) {
             return RVK_ERR_HIP;
         }
         l->own_stream = true;
+        l->is_lane = true;
         ctx->lanes.push_back(l);
     }
     for (uint32_t i = 0; i < n; i++) {

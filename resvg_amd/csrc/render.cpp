@@ -268,6 +268,9 @@ struct Renderer {
     hipEvent_t ev_cov = nullptr, ev_main = nullptr, ev_side = nullptr;
     uint64_t nflush = 0;
     bool side_used = false;
+    // false: a flush runs on the context stream alone (on a batch's lanes, which
+    // already run documents side by side)
+    const bool overlap = !ctx->is_lane;
 
     ~Renderer() { (void)finish_flushes(); }
 
@@ -785,14 +788,16 @@ struct Renderer {
         ctx->stats.edge_pieces += dl.piece_slots;
         ctx->stats.band_entries += dl.n_windows;
         ctx->stats.upload_bytes += meta;
-        hipStream_t side = nullptr;
-        RVK_TRY(rvk::side_stream(ctx, &side));
-        side_used = true;
-        if (fresh) {
-            RVK_HIP_CHECK(hipEventRecord(ev_main, ctx->stream));
-            RVK_HIP_CHECK(hipStreamWaitEvent(side, ev_main, 0));
-        } else if (blk_used[b]) {
-            RVK_HIP_CHECK(hipStreamWaitEvent(side, blk_free[b], 0));
+        hipStream_t side = ctx->stream;
+        if (overlap) {
+            RVK_TRY(rvk::side_stream(ctx, &side));
+            side_used = true;
+            if (fresh) {
+                RVK_HIP_CHECK(hipEventRecord(ev_main, ctx->stream));
+                RVK_HIP_CHECK(hipStreamWaitEvent(side, ev_main, 0));
+            } else if (blk_used[b]) {
+                RVK_HIP_CHECK(hipStreamWaitEvent(side, blk_free[b], 0));
+            }
         }
         rvk_status st = rvk::upload_staged_on(ctx, slot, d, meta, side);
         {
@@ -813,7 +818,7 @@ struct Renderer {
                                           (uint16_t *)(d + o_tab), d + o_arena, dev_overflow, cov_bytes);
             ctx->stream = main_stream;
         }
-        if (st == RVK_OK &&
+        if (st == RVK_OK && side != ctx->stream &&
             (hipEventRecord(ev_cov, side) != hipSuccess || hipStreamWaitEvent(ctx->stream, ev_cov, 0) != hipSuccess)) {
             (void)hipGetLastError();
             st = RVK_ERR_HIP;

@@ -59,6 +59,7 @@ struct rvk_context {
     // coverage work runs on, so that it overlaps the previous flush's compositing on
     // `stream`; every render joins it back into `stream` before it returns.
     hipStream_t side = nullptr;
+    bool is_lane = false;  // a batch lane (renders on its own stream only)
 };
 
 namespace rvk {
