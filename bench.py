@@ -383,6 +383,58 @@ def png_output(ctx, px, reps):
             "file_MB": round(len(data) / 1e6, 2), "idat_MB": round(len(deflated) / 1e6, 2)}
 
 
+def _counter_per_dispatch(csv_dir: str, counter: str) -> dict:
+    """The values (KiB) of `counter`, one per dispatch, per kernel name, from the
+    *_counter_collection.csv files rocprofv3 wrote under `csv_dir`."""
+    import csv
+    import glob
+    vals: dict = {}
+    for f in glob.glob(os.path.join(csv_dir, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            if r.get("Counter_Name") == counter:
+                vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return vals
+
+
+def live_traffic(args):
+    """HBM bytes per launch of the compositor kernel, measured in this run: two child
+    processes render one step of the same document under rocprofv3 --pmc (FETCH_SIZE,
+    then WRITE_SIZE; one counter per pass), after a 1 GiB device copy whose counter
+    values give the scale from counter units to bytes.  None when rocprofv3 is not
+    available or a pass fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None
+    child = [sys.executable, str(Path(__file__).resolve()), "--steps", "1", "--warmup", "0", "--no-cpu-baseline",
+             "--batch-docs", "0", "--diff-size", "0", "--fit-scenes", "0", "--png-reps", "0", "--copy-calibration",
+             "--traffic", "stored", "--size", str(args.size), "--paths", str(args.paths), "--seed", str(args.seed)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    per = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="rvk_pmc_", dir="/tmp")
+        print(f"[bench] traffic: rocprofv3 --pmc {counter}", file=sys.stderr, flush=True)
+        try:
+            r = subprocess.run(["rocprofv3", "--pmc", counter, "-d", d, "-o", "pmc", "--output-format", "csv", "--",
+                                *child], cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                               timeout=240)
+        except subprocess.TimeoutExpired:
+            return None
+        vals = _counter_per_dispatch(d, counter)
+        shutil.rmtree(d, ignore_errors=True)
+        # the 1 GiB copy is the largest copy dispatch; the profiler may end with a
+        # crash after writing complete results, so the results decide
+        cal = max(vals.get("__amd_rocclr_copyBuffer", [0.0]))
+        draw = [x for k, v in vals.items() if "k_draw<false>" in k or "k_draw<true>" in k for x in v]
+        if cal <= 0 or not draw:
+            print(f"[bench] traffic: {counter} pass gave no values (exit {r.returncode})", file=sys.stderr, flush=True)
+            return None
+        per[counter] = sum(draw) / len(draw) * 1024.0 * ((1 << 20) / cal)  # KiB -> bytes, times GiB / copy value
+    return {"bytes_per_launch": round(per["FETCH_SIZE"] + per["WRITE_SIZE"]),
+            "fetch_bytes": round(per["FETCH_SIZE"]), "write_bytes": round(per["WRITE_SIZE"])}
+
+
 def _primitive_rooflines(kernels, steps):
     """Per filter primitive / pipeline stage: its kernels' time per step and the
     achieved fraction of HBM peak on the primitive's algorithmic bytes."""
@@ -424,6 +476,9 @@ def main() -> int:
     ap.add_argument("--fit-paths", type=int, default=128)
     ap.add_argument("--fit-steps", type=int, default=500)
     ap.add_argument("--png-reps", type=int, default=3, help="timed launches of the PNG-output leg (0: skip)")
+    ap.add_argument("--traffic", choices=["live", "stored"], default="live",
+                    help="roofline.traffic: measured by two rocprofv3 --pmc child runs (live) or read from "
+                         "profiles/traffic.json (stored)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 1 GiB device buffer once (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -498,12 +553,16 @@ def main() -> int:
     roofline = None
     if dom:
         achieved = dom["bytes"] / (dom["total_ms"] / 1e3) / 1e9
-        traffic = None
+        traffic, source = None, None
         tf = ROOT / "profiles" / "traffic.json"
         if tf.exists():
             rec = json.loads(tf.read_text()).get(dom["name"])
             if rec and rec.get("workload") == f"full-filter {args.size} {args.paths} {args.seed}":
-                traffic = rec.get("bytes_per_launch")
+                traffic, source = rec.get("bytes_per_launch"), "profiles/traffic.json"
+        if args.traffic == "live":
+            live = live_traffic(args)
+            if live:
+                traffic, source = live["bytes_per_launch"], "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE in this run"
         roofline = {
             "bound": "hbm",
             "kernel": dom["name"],
@@ -512,6 +571,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": source,
             "algorithmic_bytes_per_launch": round(dom["bytes"] / dom["launches"], 1),
             "avg_launch_ms": round(dom["total_ms"] / dom["launches"], 4),
             "kernel_share_of_step": round(dom["total_ms"] / (ms * args.steps), 4),
