@@ -60,11 +60,38 @@ def n_params(n_paths: int) -> int:
     return n_paths * PATH_PARAMS + 1 + 20
 
 
-def one_path_scene(size, verbs, points) -> Scene:
-    """A document holding one path with an opaque white fill (its alpha = coverage)."""
+# A path entry of a scene is (verbs, points) or (verbs, points, style): style
+# {"fill_rule": ...} fills with that rule, style {"stroke_width": ..., "join": ...,
+# "cap": ..., "miter": ..., "dasharray": ..., "dashoffset": ...} makes the entry the
+# path's stroke (its coverage is the stroke outline's, tiny-skia's stroker and
+# dashing).  The coverage-plane mode takes every kind; the analytic mode takes
+# nonzero fills of closed cubic chains.
+_STROKE_KEYS = {"stroke_width": "stroke_width", "join": "stroke_join", "cap": "stroke_cap", "miter": "miter_limit",
+                "dasharray": "dasharray", "dashoffset": "dashoffset"}
+
+
+def _entry(path):
+    verbs, points = path[0], path[1]
+    style = dict(path[2]) if len(path) > 2 and path[2] else {}
+    return verbs, points, style
+
+
+def _draw(sc: Scene, parent: int, path, paint: PaintSpec) -> None:
+    """Adds one path entry to `sc` under `parent`, painted with `paint` (as its fill,
+    or as its stroke for a stroke entry)."""
+    verbs, points, style = _entry(path)
+    if "stroke_width" in style:
+        kw = {_STROKE_KEYS[k]: v for k, v in style.items() if k in _STROKE_KEYS}
+        sc.path(parent, verbs, points, stroke=paint, **kw)
+    else:
+        sc.path(parent, verbs, points, fill=paint, fill_rule=style.get("fill_rule", abi.FILL_NONZERO))
+
+
+def one_path_scene(size, verbs, points, style=None) -> Scene:
+    """A document holding one path entry drawn opaque white (its alpha = coverage)."""
     w, h = size
     sc = Scene(w, h)
-    sc.path(sc.root, verbs, points, fill=PaintSpec.solid((255, 255, 255)), fill_rule=abi.FILL_NONZERO)
+    _draw(sc, sc.root, (verbs, points, style), PaintSpec.solid((255, 255, 255)))
     return sc
 
 
@@ -72,7 +99,10 @@ def geometry(paths, k: int):
     """The analytic mode's geometry parameters of closed k-cubic paths (identity
     transforms), float32 (n, GEOM_FLOATS(k))."""
     out = np.zeros((len(paths), geom_floats(k)), np.float32)
-    for i, (verbs, pts) in enumerate(paths):
+    for i, path in enumerate(paths):
+        verbs, pts, style = _entry(path)
+        assert "stroke_width" not in style and style.get("fill_rule", abi.FILL_NONZERO) == abi.FILL_NONZERO, \
+            "analytic mode takes nonzero fills"
         v = list(verbs)
         assert v[0] == abi.VERB_MOVE and v[1:1 + k] == [abi.VERB_CUBIC] * k and len(v) in (k + 1, k + 2), \
             "analytic mode takes closed chains of k cubics"
@@ -110,7 +140,7 @@ class Renderer:
         if self.k == 0:
             # coverage: every path rendered alone, opaque white, through one batch call
             pxs = torch.zeros((self.batch * self.n, self.h, self.w, 4), dtype=torch.uint8, device=dev)
-            docs = [one_path_scene((self.w, self.h), v, p) for paths, _ in scenes for v, p in paths]
+            docs = [one_path_scene((self.w, self.h), *_entry(path)) for paths, _ in scenes for path in paths]
             ctx.render_batch(docs, abi.Transform.identity(), [pxs[i] for i in range(len(docs))])
             self.coverage = pxs[..., 3].contiguous()
             del pxs
@@ -218,7 +248,7 @@ def u8_scene(width, height, paths, ends, params) -> Scene:
     cm = g.color_matrix(abi.COLOR_MATRIX_MATRIX, p[n * PATH_PARAMS + 1:].astype(np.float32).tolist(), in1=b, cs=cs)
     g.composite(abi.COMPOSITE_OVER, in1=cm, in2=SOURCE_GRAPHIC, cs=cs)
     root = sc.group(sc.root, filters=[g])
-    for i, (verbs, pts) in enumerate(paths):
+    for i, path in enumerate(paths):
         q = p[i * PATH_PARAMS:(i + 1) * PATH_PARAMS]
         tint, opacity, cols = q[0:3], float(q[3]), q[4:20].reshape(4, 4)
         offs = np.maximum.accumulate(np.clip(q[20:24], 0.0, 1.0))
@@ -228,5 +258,5 @@ def u8_scene(width, height, paths, ends, params) -> Scene:
             stops.append((float(offs[k]), rgb, float(np.clip(cols[k, 3], 0.0, 1.0))))
         x0, y0, x1, y1 = ends[i]
         grp = sc.group(root, opacity=float(np.clip(opacity, 0.0, 1.0)))
-        sc.path(grp, verbs, pts, fill=PaintSpec.linear(float(x0), float(y0), float(x1), float(y1), stops))
+        _draw(sc, grp, path, PaintSpec.linear(float(x0), float(y0), float(x1), float(y1), stops))
     return sc

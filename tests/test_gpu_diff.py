@@ -313,3 +313,59 @@ def test_scene_limits_are_rejected(ctx):
         sc = diff.DiffScene(16, h, n, k, p if k == 0 else None, p)
         st = L.rvk_diff_forward_batch(ctx.h, C.byref(sc), 1, p, p if k else None, p, p)
         assert st == abi.ERR_INVALID_ARGUMENT, (n, k, h, st)
+
+
+def _styled_scene(size=48, seed=3):
+    """Path entries of every coverage kind the plane mode takes: an even-odd star
+    (its centre left empty), the same star nonzero (centre filled), a round-joined
+    stroke of a random cubic chain and a dashed square stroke."""
+    from resvg_amd import abi
+    rng = np.random.default_rng(seed)
+    c = size / 2.0
+    star = [(c + 0.45 * size * np.cos(np.pi / 2 + k * 4 * np.pi / 5), c - 0.45 * size * np.sin(np.pi / 2 + k * 4 * np.pi / 5))
+            for k in range(5)]
+    sv = [abi.VERB_MOVE] + [abi.VERB_LINE] * 4 + [abi.VERB_CLOSE]
+    cub = rng.uniform(4, size - 4, (1 + 3 * 3, 2)).astype(np.float32)
+    cv = [abi.VERB_MOVE] + [abi.VERB_CUBIC] * 3
+    sq = [(8.0, 8.0), (size - 8.0, 8.0), (size - 8.0, size - 8.0), (8.0, size - 8.0)]
+    qv = [abi.VERB_MOVE] + [abi.VERB_LINE] * 3 + [abi.VERB_CLOSE]
+    paths = [(sv, star, {"fill_rule": abi.FILL_EVENODD}), (sv, star, None),
+             (cv, cub, {"stroke_width": 3.0, "join": abi.JOIN_ROUND, "cap": abi.CAP_ROUND}),
+             (qv, sq, {"stroke_width": 2.0, "dasharray": [5.0, 3.0], "dashoffset": 1.0})]
+    n = len(paths)
+    ends = rng.uniform(0, size, (n, 4)).astype(np.float32)
+    q = []
+    for _ in range(n):
+        v = np.empty(diff.PATH_PARAMS)
+        v[0:3], v[3] = rng.uniform(0.5, 1.0, 3), rng.uniform(0.3, 1.0)
+        v[4:20], v[20:24] = rng.uniform(0.1, 0.9, 16), np.sort(rng.uniform(0.05, 0.95, 4))
+        q.append(v)
+    m = np.eye(4, 5) + rng.normal(0.0, 0.1, (4, 5))
+    params = np.concatenate(q + [np.array([1.1]), m.reshape(-1)]).astype(np.float32)
+    return size, paths, ends, params
+
+
+def test_planes_of_evenodd_fills_and_strokes(ctx):
+    """The coverage-plane mode with even-odd fills, strokes and dashes: each plane is
+    the u8 rasterizer's coverage of the entry (bit-exact against the oracle), the
+    forward matches the FP64 oracle and the gradient its finite differences."""
+    import torch
+    from oracle import bindings as orc
+    from resvg_amd import abi
+    size, paths, ends, params = _styled_scene()
+    r = diff.Renderer(ctx, size, size, paths, ends)
+    cov = r.coverage.cpu().numpy()
+    for i, path in enumerate(paths):
+        px = np.zeros((size, size, 4), np.uint8)
+        orc.render(diff.one_path_scene((size, size), *diff._entry(path)).to_abi().abi, abi.Transform.identity(), px)
+        assert np.array_equal(cov[i], px[..., 3]), f"plane {i}"
+    c = size // 2
+    assert cov[0][c, c] == 0 and cov[1][c, c] == 255  # even-odd leaves the star's centre empty
+    assert 0 < (cov[2] > 0).mean() < 0.5 and 0 < (cov[3] > 0).mean() < 0.3  # strokes: thin outlines
+    p = torch.as_tensor(params, device="cuda")
+    out = r.forward(p).cpu().numpy()
+    assert np.abs(out - D.forward(cov, ends, params, size, size)).max() < 2e-5
+    go = np.random.default_rng(5).normal(0.0, 1.0, (size, size, 4)).astype(np.float32)
+    g = r.backward(p, torch.as_tensor(go, device="cuda")).cpu().numpy().astype(np.float64)
+    ref = D.grad_fd(cov, ends, params, size, size, go)
+    assert np.linalg.norm(g - ref) / np.linalg.norm(ref) < 1e-3
