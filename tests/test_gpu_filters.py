@@ -98,6 +98,35 @@ def test_iir_blur(ctx, shape, sig):
     assert_exact(host(t), e, f"iir {sig}")
 
 
+def sparse_premul(h, w, seed, density=0.003):
+    """Isolated pixels on a transparent background: every chain has long runs of
+    zeros next to content, where values decay towards f64 underflow."""
+    rng = np.random.default_rng(seed)
+    px = np.zeros((h, w, 4), np.uint8)
+    a = np.where(rng.random((h, w)) < density, rng.integers(1, 256, size=(h, w)), 0)
+    px[..., 3] = a
+    px[..., :3] = (rng.integers(0, 256, size=(h, w, 3)) * a[..., None] // 255).astype(np.uint8)
+    return px
+
+
+@pytest.mark.parametrize("shape,sig", [((1, 4096), s) for s in [(1.99, 1.99), (0.5, 1.5), (0.05, 0.0), (0.0, 1.2),
+                                                                  (0.0, 0.0)]] +
+                         [(sh, s) for sh in [(4096, 2), (3, 4097), (4097, 5), (700, 1500), (1024, 1024)]
+                          for s in [(1.99, 1.99), (0.5, 1.5), (0.05, 0.0), (0.0, 1.2), (0.0, 0.0)]] +
+                         [((1800, 2600), (1.5, 0.5)), ((2600, 1800), (0.5, 1.5))])
+@pytest.mark.parametrize("kind", ["sparse", "blobs"])
+def test_iir_blur_long_chains_and_transparent_runs(ctx, shape, sig, kind):
+    """Chains of every on-chip chunk length (8 to 64 values a lane, up to 4096 values)
+    and past the on-chip limit (the streaming form), on images whose chains mix
+    content with long transparent runs."""
+    px = sparse_premul(*shape, seed=5) if kind == "sparse" else smooth_premul(*shape, seed=5)
+    t = dev(px)
+    ctx.iir_blur(t, *sig)
+    e = px.copy()
+    orc.iir_blur(e, *sig)
+    assert_exact(host(t), e, f"iir {sig} {kind}")
+
+
 @pytest.mark.parametrize("shape", SHAPES + [(500, 333)])
 @pytest.mark.parametrize("r", [(1.0, 1.0), (2.5, 0.5), (4.0, 4.0), (3.5, 7.5), (8.0, 8.0), (8.0, 3.0),
                                (12.0, 16.0), (20.0, 18.0), (16.5, 2.0)])
