@@ -18,8 +18,10 @@ per-channel mismatch histogram of two seeded documents against the oracle), and 
 sample of the batched workload, under "differentiable" the f32 forward and
 forward+backward times of the differentiable form against the plain u8 forward,
 under "batched_fitting" the steps/s and final PSNR of 64 independent 512x512 fits
-run as one batch, and under "png_output" the PNG encoding of the rendered document
-(the device's demultiply + row-filter kernel against HBM peak, the host's deflate).
+run as one batch, under "png_output" the PNG encoding of the rendered document
+(the device's demultiply + row-filter kernel against HBM peak, the host's deflate),
+and under "clip_mask" a 4096x4096 document of 300 clipped and masked groups (clip
+paths and masks at scale; its 1024x1024 quarter is timed on the oracle beside it).
 """
 from __future__ import annotations
 
@@ -93,6 +95,20 @@ def _cpu_job(job):
     return dt, len(sc.paths), prof
 
 
+def _cpu_clip_job(job):
+    size, groups, seed = job
+    import numpy as np
+    from oracle import bindings as orc
+    from resvg_amd import abi
+    from resvg_amd.workloads import clip_mask_doc
+    sc, _ = clip_mask_doc(seed=seed, size=size, n_groups=groups)
+    s = sc.to_abi()
+    px = np.zeros((size, size, 4), np.uint8)
+    t = time.perf_counter()
+    orc.render(s.abi, abi.Transform.identity(), px)
+    return time.perf_counter() - t
+
+
 def _cpu_model():
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -103,7 +119,7 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(size, paths, seed, workers, plain_stride=128, n_filter_groups=8, runs=5):
+def cpu_baseline(size, paths, seed, workers, plain_stride=128, n_filter_groups=8, runs=5, clip_groups=0):
     """The oracle built -O3 -march=native for this host, one document part per core:
     each of the 8 filter groups alone, and every `plain_stride`-th plain path spread
     over the remaining cores.  One warm-up run, then `runs` timed runs; per part the
@@ -134,6 +150,13 @@ def cpu_baseline(size, paths, seed, workers, plain_stride=128, n_filter_groups=8
                 times[i].append(x[0])
             if profs is None:
                 profs = [x[2] for x in res]
+        # the clip/mask document at 1024x1024 (a quarter of the groups), one core,
+        # after a warm-up render
+        clip_s = None
+        if clip_groups > 0:
+            cjob = (1024, max(1, clip_groups // 4), seed)
+            pool.apply(_cpu_clip_job, (cjob,))
+            clip_s = pool.apply(_cpu_clip_job, (cjob,))
     wall = time.perf_counter() - t0
     med = [statistics.median(t) for t in times]
     n_paths = [len_ for _, len_, _ in res]
@@ -166,6 +189,9 @@ def cpu_baseline(size, paths, seed, workers, plain_stride=128, n_filter_groups=8
                   f"{runs} runs after 1 warm-up; estimated {t_doc:.0f} s per document per core; wall {wall:.1f} s",
         "seconds_per_document_per_core": round(t_doc, 2),
         "split_s_per_document": split,
+        "clip_mask": None if clip_s is None else {
+            "workload": f"clip/mask 1024x1024, {max(1, clip_groups // 4)} groups, seed {seed}",
+            "seconds": round(clip_s, 2), "mpixel_per_s_per_core": round(1024 * 1024 / 1e6 / clip_s, 4)},
     }
 
 
@@ -383,6 +409,55 @@ def png_output(ctx, px, reps):
             "file_MB": round(len(data) / 1e6, 2), "idat_MB": round(len(deflated) / 1e6, 2)}
 
 
+def clip_mask(ctx, size, groups, seed, steps):
+    """The clip/mask document (clip paths and masks at scale): `steps` renders timed
+    with HIP events on the context stream after one warm-up render, with the per-kernel
+    split; the same document at 1024x1024 with a quarter of the groups is what the
+    CPU baseline times (`cpu_baseline.clip_mask`)."""
+    import torch
+    from resvg_amd import abi
+    from resvg_amd.workloads import clip_mask_doc
+    sc, (w, h) = clip_mask_doc(seed=seed, size=size, n_groups=groups)
+    scene = sc.to_abi()
+    ts = abi.Transform.identity()
+    px = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda:0")
+    small = torch.zeros((1024, 1024, 4), dtype=torch.uint8, device="cuda:0")
+    sc_small, _ = clip_mask_doc(seed=seed, size=1024, n_groups=max(1, groups // 4))
+    scene_small = sc_small.to_abi()
+    ctx.render(scene, ts, px)
+    ctx.render(scene_small, ts, small)
+    ctx.synchronize()
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ctx.profile_begin()
+    start.record(stream)
+    for _ in range(steps):
+        px.zero_()
+        ctx.render(scene, ts, px)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ks = ctx.profile_end()
+    st = ctx.stats()
+    ms = start.elapsed_time(stop) / steps
+    start.record(stream)
+    for _ in range(steps):
+        small.zero_()
+        ctx.render(scene_small, ts, small)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms_small = start.elapsed_time(stop) / steps
+    per = lambda k: round(k["total_ms"] / steps, 3)
+    masks = [k for k in ks if k["name"] in ("apply_mask", "mask_from_pixmap", "mask_invert", "fill_mask")]
+    return {"workload": f"clip/mask {w}x{h}, {groups} groups, 16 clip paths, 16 masks, seed {seed}",
+            "mpixel_per_s": round(w * h / 1e6 / (ms / 1e3), 2), "ms_per_render": round(ms, 2), "steps": steps,
+            "layers": st.layers, "flushes": st.flushes, "kernel_launches": st.kernel_launches,
+            "mask_kernels_ms_per_render": round(sum(k["total_ms"] for k in masks) / steps, 3),
+            "small_ms_per_render": round(ms_small, 3),
+            "kernels": [{"name": k["name"], "launches": k["launches"] // steps, "ms_per_render": per(k),
+                         "GB/s": round(k["bytes"] / (k["total_ms"] / 1e3) / 1e9, 1) if k["total_ms"] > 0 else None}
+                        for k in ks[:8]]}
+
+
 def _counter_per_dispatch(csv_dir: str, counter: str) -> dict:
     """The values (KiB) of `counter`, one per dispatch, per kernel name, from the
     *_counter_collection.csv files rocprofv3 wrote under `csv_dir`."""
@@ -476,6 +551,8 @@ def main() -> int:
     ap.add_argument("--fit-paths", type=int, default=128)
     ap.add_argument("--fit-steps", type=int, default=500)
     ap.add_argument("--png-reps", type=int, default=3, help="timed launches of the PNG-output leg (0: skip)")
+    ap.add_argument("--clip-groups", type=int, default=300, help="groups of the clip/mask leg's document (0: skip)")
+    ap.add_argument("--clip-steps", type=int, default=3)
     ap.add_argument("--traffic", choices=["live", "stored"], default="live",
                     help="roofline.traffic: measured by two rocprofv3 --pmc child runs (live) or read from "
                          "profiles/traffic.json (stored)")
@@ -490,7 +567,8 @@ def main() -> int:
     if not args.no_cpu_baseline:
         workers = args.cpu_workers or max(1, min(16, len(os.sched_getaffinity(0))))
         print(f"[bench] cpu baseline: {workers} workers", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.size, args.paths, args.seed, workers, plain_stride=args.cpu_stride, runs=args.cpu_runs)
+        cpu = cpu_baseline(args.size, args.paths, args.seed, workers, plain_stride=args.cpu_stride, runs=args.cpu_runs,
+                           clip_groups=args.clip_groups)
         print(f"[bench] cpu baseline {cpu['value']} Mpixel/s", file=sys.stderr, flush=True)
 
     import torch
@@ -538,6 +616,7 @@ def main() -> int:
     if cpu is not None:
         cpu["parity"] = cpu_parity(ctx, args.seed)
     pngo = png_output(ctx, px, args.png_reps) if args.png_reps > 0 else None
+    clipm = clip_mask(ctx, args.size, args.clip_groups, args.seed, args.clip_steps) if args.clip_groups > 0 else None
     bat = batched(ctx, args.batch_docs, args.seed, args.batch_steps) if args.batch_docs > 0 else None
     dif = differentiable(ctx, args.diff_size, args.diff_paths, args.seed, 5) if args.diff_size > 0 else None
     fitr = fitting(ctx, args.fit_scenes, args.fit_size, args.fit_paths, args.fit_steps, args.seed) \
@@ -606,6 +685,7 @@ def main() -> int:
         "wall_s_per_step": round(t_wall / args.steps, 3),
         "host_last_step": host,
         "png_output": pngo,
+        "clip_mask": clipm,
         "batched": bat,
         "differentiable": dif,
         "batched_fitting": fitr,

@@ -12,6 +12,9 @@
   component transfer, turbulence, distant/point/spot lighting and multiply/screen.
 * ``document_batch`` — documents with log-uniform sides 64-2048 (aspect 0.5-2), 5-200
   paths of the full-filter distributions and a 1-6 primitive filter chain each.
+* ``clip_mask_doc`` — 4096x4096 by default: 300 groups of 1-3 such paths, clipped
+  (60%) and masked (40%) by 16 clip paths and 16 masks that nest and clip their own
+  children.
 
 Random numbers come from numpy's PCG64 seeded per document; every float is rounded
 to f32 before use, so the same document is produced on every machine.
@@ -200,6 +203,60 @@ def full_filter_doc(seed: int = 0, size: int = 4096, n_paths: int = 2000, paths_
                 sc.path(grp, verbs, pts, fill=fill, fill_rule=rule, stroke=stroke, stroke_width=width,
                         stroke_join=join)
             i += 1
+    return sc, (size, size)
+
+
+def clip_mask_doc(seed: int = 0, size: int = 4096, n_groups: int = 300, n_clips: int = 16, n_masks: int = 16,
+                  max_segments: int = 40):
+    """The clip/mask document: `n_groups` top-level groups with opacity U[0.5, 1], 60%
+    clipped by one of `n_clips` clip paths and 40% masked by one of `n_masks` masks
+    (independently), each holding 1-3 paths of the full-filter distributions (3 to
+    `max_segments` segments, a 2-8 stop gradient, 30% also stroked).  A clip path holds
+    1-3 black paths (even-odd or nonzero), half of them under a transform of its own,
+    a third nesting the clip path before it, and 30% of its children sit in a group
+    clipped again by an earlier clip path; a mask covers a random rect, luminance (60%)
+    or alpha, a third nesting the mask before it, with 1-3 gradient-filled paths."""
+    rng = np.random.default_rng(seed)
+    sc = Scene(size, size)
+
+    def rand_ts():
+        a, k = _f(rng.uniform(0.6, 1.3)), _f(rng.uniform(-0.3, 0.3))
+        return (a, k, -k, a, _f(rng.uniform(-0.1, 0.1) * size), _f(rng.uniform(-0.1, 0.1) * size))
+
+    def paths_into(parent, count, paint):
+        for _ in range(count):
+            verbs, pts = _random_path(rng, size, int(rng.integers(3, max_segments + 1)))
+            stroke = _random_paint(rng, size, int(rng.integers(2, 9))) if paint and rng.random() < 0.3 else None
+            sc.path(parent, verbs, pts,
+                    fill=_random_paint(rng, size, int(rng.integers(2, 9))) if paint else PaintSpec.solid((0, 0, 0)),
+                    fill_rule=abi.FILL_EVENODD if rng.random() < 0.5 else abi.FILL_NONZERO,
+                    stroke=stroke, stroke_width=_f(rng.uniform(0.5, 8.0)))
+
+    clips = []
+    for _ in range(n_clips):
+        nested = clips[-1] if clips and rng.random() < 1 / 3 else -1
+        c, root = sc.clip_path(transform=rand_ts() if rng.random() < 0.5 else (1.0, 0.0, 0.0, 1.0, 0.0, 0.0),
+                               clip_path=nested)
+        for _ in range(int(rng.integers(1, 4))):
+            parent = root
+            if clips and rng.random() < 0.3:
+                parent = sc.group(root, clip_path=clips[int(rng.integers(len(clips)))])
+            paths_into(parent, 1, False)
+        clips.append(c)
+    masks = []
+    for _ in range(n_masks):
+        nested = masks[-1] if masks and rng.random() < 1 / 3 else -1
+        x, y = rng.uniform(-0.1 * size, 0.5 * size, 2)
+        wm, hm = rng.uniform(0.3 * size, size, 2)
+        m, root = sc.mask((_f(x), _f(y), _f(wm), _f(hm)),
+                          kind=abi.MASK_LUMINANCE if rng.random() < 0.6 else abi.MASK_ALPHA, mask=nested)
+        paths_into(root, int(rng.integers(1, 4)), True)
+        masks.append(m)
+    for _ in range(n_groups):
+        clip = clips[int(rng.integers(len(clips)))] if rng.random() < 0.6 else -1
+        mask = masks[int(rng.integers(len(masks)))] if rng.random() < 0.4 else -1
+        g = sc.group(sc.root, opacity=_f(rng.uniform(0.5, 1.0)), clip_path=clip, mask=mask)
+        paths_into(g, int(rng.integers(1, 4)), True)
     return sc, (size, size)
 
 

@@ -8,6 +8,8 @@
   same pixels as the default one, rendering onto a canvas twice composites twice).
 * The batched workload at its real sizes: documents of 1024-2048 px sides with
   100-200 paths, in one rvk_render_batch call, each compared with the oracle.
+* The clip/mask document: its generator at 512x512 against the oracle, and the
+  4096x4096 document by the repeat and flush-budget properties.
 """
 import numpy as np
 import pytest
@@ -99,3 +101,32 @@ def test_batched_documents_at_full_size(ctx):
     assert st == [abi.OK] * len(docs)
     for i, (px, ref) in enumerate(zip(pxs, refs)):
         _assert_same(px.cpu().numpy(), ref, f"batched document {i} {docs[i][1]}")
+
+
+
+def test_clip_mask_document_matches_oracle(ctx):
+    """The clip/mask workload's generator at 512x512 (40 groups, 8 clip paths, 8
+    masks): nested clip paths, clip-path'd clip children, nested luminance and alpha
+    masks, bit-exact against the oracle."""
+    from oracle import bindings as orc
+    from resvg_amd.workloads import clip_mask_doc
+    sc, (w, h) = clip_mask_doc(seed=3, size=512, n_groups=40, n_clips=8, n_masks=8)
+    s = sc.to_abi()
+    got = _render(ctx, s, w, h).cpu().numpy()
+    exp = np.zeros((h, w, 4), np.uint8)
+    orc.render(s.abi, abi.Transform.identity(), exp)
+    assert (exp[..., 3] > 0).mean() > 0.3
+    _assert_same(got, exp, "clip/mask 512")
+
+
+def test_clip_mask_document_full_size_is_deterministic(ctx, monkeypatch):
+    """The 4096x4096 clip/mask document (300 groups) renders the same pixels twice and
+    with a 64 MiB flush budget."""
+    from resvg_amd.workloads import clip_mask_doc
+    sc, (w, h) = clip_mask_doc(seed=0)
+    s = sc.to_abi()
+    a = _render(ctx, s, w, h).cpu().numpy()
+    assert (a[..., 3] > 0).mean() > 0.3
+    assert np.array_equal(a, _render(ctx, s, w, h).cpu().numpy())
+    monkeypatch.setenv("RVK_FLUSH_BYTES", str(64 << 20))
+    assert np.array_equal(a, _render(ctx, s, w, h).cpu().numpy())
