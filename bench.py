@@ -483,7 +483,8 @@ def live_traffic(args):
     if shutil.which("rocprofv3") is None:
         return None
     child = [sys.executable, str(Path(__file__).resolve()), "--steps", "1", "--warmup", "0", "--no-cpu-baseline",
-             "--batch-docs", "0", "--diff-size", "0", "--fit-scenes", "0", "--png-reps", "0", "--copy-calibration",
+             "--batch-docs", "0", "--diff-size", "0", "--fit-scenes", "0", "--png-reps", "0", "--clip-groups", "0",
+             "--copy-calibration",
              "--traffic", "stored", "--size", str(args.size), "--paths", str(args.paths), "--seed", str(args.seed)]
     env = dict(os.environ, TMPDIR="/tmp")
     per = {}
