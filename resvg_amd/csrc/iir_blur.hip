@@ -459,15 +459,36 @@ static rvk_status iir_on_chip(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_
         const rvk_status st = scratch_alloc(ctx, n * 4 * sizeof(double), &pa);
         if (st != RVK_OK) { scratch_release(ctx, pu); return st; }
     }
+    // Columns longer than 2048 read the row pass's planes transposed (column-major,
+    // contiguous chains): reading them across rows touches a cache line per value
+    // and, with many such columns in flight, fetched 5.7 times the planes at 4096^2;
+    // below that the transpose costs about what it saves.
+    const bool transposed = rows && h > 2048u;
+    void *pt = nullptr;
+    if (transposed) {
+        const rvk_status st = scratch_alloc(ctx, n * 4 * sizeof(double), &pt);
+        if (st != RVK_OK) { scratch_release(ctx, pa); scratch_release(ctx, pu); return st; }
+    }
     uint8_t *cplanes = (uint8_t *)pu;
-    double *planes = (double *)pa;
+    double *planes = (double *)pa, *tplanes = (double *)pt;
     if (rows) {
         timed_launch(ctx, "iir_rows", pxb, [&] {
             launch_chains(ctx->stream, px, nullptr, nullptr, planes, w, h, 1u, w, n32, dx, 4, 1.0, 0);
         });
-        timed_launch(ctx, "iir_cols", 0.0, [&] {
-            launch_chains(ctx->stream, nullptr, planes, cplanes, nullptr, h, w, w, 1u, n32, dy, 4, post, 1);
-        });
+        if (transposed) {
+            // row-major planes (h rows of w) read as column-major ones of h x w
+            timed_launch(ctx, "iir_transpose", 0.0, [&] {
+                hipLaunchKernelGGL(k_transpose_planes, dim3(div_up(h, T), div_up(w, T), 4), dim3(T, 8), 0,
+                                   ctx->stream, (const double *)planes, tplanes, h, w);
+            });
+            timed_launch(ctx, "iir_cols", 0.0, [&] {
+                launch_chains(ctx->stream, nullptr, tplanes, cplanes, nullptr, h, w, 1u, h, n32, dy, 4, post, 1);
+            });
+        } else {
+            timed_launch(ctx, "iir_cols", 0.0, [&] {
+                launch_chains(ctx->stream, nullptr, planes, cplanes, nullptr, h, w, w, 1u, n32, dy, 4, post, 1);
+            });
+        }
     } else {
         timed_launch(ctx, "iir_cols", pxb, [&] {
             launch_chains(ctx->stream, px, nullptr, cplanes, nullptr, h, w, w, 1u, n32, dy, 4, post, 1);
@@ -478,6 +499,7 @@ static rvk_status iir_on_chip(rvk_context *ctx, uint8_t *px, uint32_t w, uint32_
                            (const uint8_t *)cplanes, (uint32_t *)px, w, h);
     });
     const hipError_t e = hipGetLastError();
+    if (pt) scratch_release(ctx, pt);
     if (pa) scratch_release(ctx, pa);
     scratch_release(ctx, pu);
     RVK_HIP_CHECK(e);
