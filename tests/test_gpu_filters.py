@@ -88,8 +88,10 @@ def test_box_blur(ctx, shape, sig):
 
 
 @pytest.mark.parametrize("shape", SHAPES + [(300, 410)])
-@pytest.mark.parametrize("sig", [(0.5, 0.5), (1.5, 1.0), (1.9, 0.0), (0.0, 0.7)])
+@pytest.mark.parametrize("sig", [(0.5, 0.5), (1.5, 1.0), (1.9, 0.0), (0.0, 0.7), (40.0, 3.0), (0.0, 9.0)])
 def test_iir_blur(ctx, shape, sig):
+    """Sigmas below 2 (the filter graph's IIR range) and, through the C-ABI entry,
+    larger ones (dnu up to 0.95: the longest warm-up and the most fixpoint rounds)."""
     px = rand_premul(*shape, seed=3)
     t = dev(px)
     ctx.iir_blur(t, *sig)
