@@ -396,8 +396,8 @@ static void coefficients(double sigma, double *lambda, double *dnu) {
 
 // Warm-up length of a speculated chunk: enough steps for dnu^K to fall below 2^-56
 // (an error in the speculated start is then below an ulp of any value that is not
-// itself vanishing), rounded up to 16, 24, 32 or 48: 16 for sigma up to about 0.8,
-// 24 up to about 1.6, 32 up to 2 (the largest sigma the IIR form gets from the filter
+// itself vanishing), rounded up to 16, 24, 32 or 48: 16 for sigma up to about 0.9,
+// 24 up to about 1.5, 32 below 2 (the largest sigma the IIR form gets from the filter
 // graph), 48 beyond.  Only a cost: the fixpoint makes every chunk exact for any K.
 static int warmup_steps(double dnu) {
     const double k = std::ceil(56.0 * std::log(2.0) / std::log(1.0 / dnu));
